@@ -1,0 +1,199 @@
+#!/usr/bin/env python
+"""Batched ARAP throughput on one MI355X.
+
+Workload: 512 disk-topology islands with log-uniform triangle counts in
+[200, 40000] (Delaunay of Poisson-disk points on the unit disk lifted by six
+Gaussian bumps, seed 2), LSCM initialisation, then 10 ARAP iterations per
+island with every global step solved to relative residual 1e-6.
+
+One step = one replay of the captured graph (prologue + 10 ARAP iterations
+over all islands) from the LSCM layout.  value = sum(t_i) * 10 / step time in
+Mtri*iter/s.  The roofline figure uses the algorithmic byte model
+    B_i = 124 t_i + 72 n_i + 8 nnz_i + k_i (100 n_i + 8 nnz_i)
+per island and ARAP iteration (k_i = CG iterations executed, nnz_i with the
+diagonal).  The CPU baseline is the float64 CPU restatement of the reference
+path (SuperLU) on a bounded sample of the same islands, one core.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--iters", type=int, default=10)
+    ap.add_argument("--islands", type=int, default=512)
+    ap.add_argument("--cg-tol", type=float, default=1e-6)
+    ap.add_argument("--precision", default="mixed", choices=["mixed", "f32", "f64"])
+    ap.add_argument("--cpu-sample", type=int, default=16, help="islands in the CPU baseline sample (0 = skip)")
+    ap.add_argument("--no-graph", action="store_true")
+    return ap.parse_args()
+
+
+def unique_edge_count(T):
+    a = np.concatenate([T[:, 0], T[:, 1], T[:, 2]])
+    b = np.concatenate([T[:, 1], T[:, 2], T[:, 0]])
+    n = int(T.max()) + 1
+    return len(np.unique(np.minimum(a, b) * n + np.maximum(a, b)))
+
+
+def orient_majority(U, plan, vo, to, T_local):
+    """charts.py:407-410: mirror islands with more negative than positive triangles."""
+    import torch
+
+    sa = plan.signed_areas(U).cpu().numpy()
+    flip = np.zeros(plan.n, bool)
+    for i in range(len(vo) - 1):
+        s = sa[to[i]:to[i + 1]]
+        if (s < 0).sum() > (s > 0).sum():
+            flip[vo[i]:vo[i + 1]] = True
+    f = torch.as_tensor(flip, device=U.device)
+    U[:, 1] = torch.where(f, -U[:, 1], U[:, 1])
+    return U
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (islands would shard with no "
+              "exchange, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    from ultimate_uv_artist_amd import _lib, workloads
+    from ultimate_uv_artist_amd.plan import IslandPlan
+
+    prec = {"mixed": _lib.UVA_PREC_MIXED, "f32": _lib.UVA_PREC_F32, "f64": _lib.UVA_PREC_F64}[args.precision]
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    t0 = time.perf_counter()
+    charts = workloads.disk_islands(seed=2, n_islands=args.islands)
+    t_gen = time.perf_counter() - t0
+    tri = np.array([len(T) for _, T in charts])
+    nv = np.array([len(P) for P, _ in charts])
+    nnz = np.array([len(P) + 2 * unique_edge_count(T) for P, T in charts])
+    print(f"bench: {len(charts)} islands, {tri.sum()} triangles, {nv.sum()} vertices (generated in {t_gen:.1f}s)",
+          flush=True)
+
+    t0 = time.perf_counter()
+    plan = IslandPlan(charts, dev=dev)
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t0
+    U0 = torch.zeros(plan.n, 2, dtype=torch.float64, device=dev)
+    t0 = time.perf_counter()
+    plan.lscm(U0)
+    torch.cuda.synchronize()
+    t_lscm = time.perf_counter() - t0
+    st_l = plan.stats()
+    U0 = orient_majority(U0, plan, plan.vert_off, plan.tri_off, plan.T_local)
+    print(f"bench: setup {t_setup:.2f}s, LSCM {t_lscm * 1e3:.1f} ms (CG iters mean {st_l['cg_iters'].mean():.0f}, "
+          f"max {st_l['cg_iters'].max()}, converged {int(st_l['converged'].sum())}/{len(charts)})", flush=True)
+
+    U = torch.empty_like(U0)
+    stream = torch.cuda.current_stream(dev)
+
+    def step():
+        U.copy_(U0)
+        plan.arap(U, args.iters, line_search=True, mode=_lib.UVA_MODE_REFINE, tol=0.0, cg_tol=args.cg_tol,
+                  precision=prec, use_graph=not args.no_graph)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        step()
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1) / max(args.steps, 1)
+    st = plan.stats()
+    k = st["cg_iters"].astype(np.float64)                    # CG iterations over the step, per island
+    iters_run = st["arap_iters"]
+    bytes_step = float(np.sum(iters_run * (124.0 * tri + 72.0 * nv + 8.0 * nnz) + k * (100.0 * nv + 8.0 * nnz)))
+    gbs = bytes_step / (ms * 1e-3) / 1e9
+    value = float(tri.sum()) * args.iters / (ms * 1e-3) / 1e6
+    kpi = k / np.maximum(iters_run, 1)
+    print(f"bench: {ms:.2f} ms/step, {value:.1f} Mtri*iter/s, {gbs:.0f} GB/s algorithmic; CG iters per global "
+          f"step mean {kpi.mean():.1f} p95 {np.percentile(kpi, 95):.1f}; converged {int(st['converged'].sum())}/"
+          f"{len(charts)}; max rel resid {st['rel_resid'].max():.2e}", flush=True)
+
+    cpu = None
+    parity = None
+    if args.cpu_sample > 0:
+        from oracle import param_cpu as oracle
+
+        order = np.argsort(tri)
+        pick = order[np.linspace(0, len(order) - 1, min(args.cpu_sample, len(order))).astype(int)]
+        Uh0 = U0.cpu().numpy()
+        Uh = U.cpu().numpy()
+        vo = plan.vert_off
+        t_cpu = 0.0
+        err = []
+        for i in pick:
+            P, T = charts[i]
+            u0 = Uh0[vo[i]:vo[i + 1]]
+            c0 = time.perf_counter()
+            ur, _ = oracle.refine(P, T, u0, iters=args.iters, tol=0.0)
+            t_cpu += time.perf_counter() - c0
+            g = Uh[vo[i]:vo[i + 1]]
+            lo = ur.min(0)
+            s = float((ur.max(0) - lo).max())
+            err.append(float(np.abs((g - lo) / s - (ur - lo) / s).max()))
+        cpu_tri_iter = float(tri[pick].sum()) * args.iters
+        cpu = {"value": cpu_tri_iter / t_cpu / 1e6, "unit": "Mtri*iter/s", "cores": 1, "kind": "port",
+               "sample": f"{len(pick)} islands spread over the size range ({int(tri[pick].sum())} triangles), "
+                         f"{args.iters} ARAP iterations each incl. SuperLU factorisation, {t_cpu:.1f}s"}
+        parity = {"islands": len(pick), "max_uv_err_unit_box": max(err)}
+        print(f"bench: CPU oracle sample {cpu['value']:.3f} Mtri*iter/s; parity max err {max(err):.2e}", flush=True)
+
+    rec = {
+        "metric": "batched ARAP Mtri*iter/s at relative residual 1e-6",
+        "value": round(value, 3),
+        "unit": "Mtri*iter/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 SpMV/vectors, f64 dots+residual" if prec == _lib.UVA_PREC_MIXED else args.precision,
+        "data": "synthetic",
+        "config": {"workload": f"{len(charts)} disk islands, t log-uniform [200,40000], LSCM + {args.iters} ARAP, "
+                               f"CG rel resid {args.cg_tol:g}",
+                   "islands": len(charts), "triangles": int(tri.sum()), "vertices": int(nv.sum()),
+                   "arap_iters": args.iters, "mesh_sha256": workloads.mesh_digest(charts)[:16],
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None},
+        "cpu_baseline": cpu,
+        "cg_iters_per_step": {"mean": round(float(kpi.mean()), 2), "p95": round(float(np.percentile(kpi, 95)), 2),
+                              "max": int(st["cg_iters_max"].max())},
+        "converged_islands": int(st["converged"].sum()),
+        "setup_s": round(t_setup, 3),
+        "lscm_ms": round(t_lscm * 1e3, 2),
+        "parity": parity,
+    }
+    print(json.dumps(rec), flush=True)
+
+
+if __name__ == "__main__":
+    main()

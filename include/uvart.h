@@ -1,0 +1,164 @@
+/*
+ * uvart -- MI355X (gfx950) UV parameterization hot path: LSCM initialisation
+ * and ARAP local/global iterations whose global step is a batched
+ * Jacobi-preconditioned conjugate-gradient solve on per-island cotangent
+ * Laplacians.
+ *
+ * Plain C ABI over raw pointers.  Host arrays are read during plan creation
+ * only; the UV arrays passed to the solve calls are DEVICE pointers owned by
+ * the caller (n_vertices x 2 float64, row-major).  Streams are hipStream_t
+ * passed as void* (NULL = the default stream).
+ *
+ * Every call returns UVA_OK (0) or a negative status; uva_last_error() gives a
+ * message for the calling thread.  A plan is not safe to use from two threads
+ * at once; distinct plans are independent.
+ *
+ * Reference interfaces these entry points replace (Python, engine/):
+ *   uva_plan_create     gpu.py:69-142 Batch.__init__ / _build_laplacian,
+ *                       param.py:19-56 local_frames, cot_weights, cot_laplacian
+ *   uva_lscm            param.py:136-179 lscm (+ _far_pair)
+ *   uva_arap            param.py:379-414 refine(method="arap") and
+ *                       gpu.py:230-249 Batch.arap
+ *   uva_local_rhs       param.py:206-233 _arap_rotations (mode 0) +
+ *                       param.py:457-474 _arap_rhs; gpu.py:165-182 Batch._rhs
+ *   uva_cg_solve        param.py:103-132 SPDSolver.solve; gpu.py:184-202 Batch._cg
+ *   uva_max_step        param.py:259-291 _max_step; gpu.py:204-227 Batch._max_step
+ *   uva_energy          param.py:294-338 _energy
+ *   uva_signed_areas    param.py:97-100 signed_areas; gpu.py:153-156
+ *   uva_flips_per_chart gpu.py:158-163 Batch.flips_per_chart
+ *   uva_repair_jacobi   gpu.py:251-273 Batch.repair_flips
+ *   uva_orient_positive gpu.py:275-281 Batch.orient_positive
+ *   uva_repair_seq      param.py:478-516 repair_flips (host, exact order)
+ */
+#ifndef UVART_H
+#define UVART_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define UVA_OK 0
+#define UVA_EINVAL (-1)     /* bad argument or shape */
+#define UVA_EHIP (-2)       /* HIP runtime error */
+#define UVA_ENOMEM (-3)     /* device or host allocation failed */
+#define UVA_ENOBOUNDARY (-5) /* LSCM on an island without boundary (closed) */
+
+/* plan flags */
+#define UVA_PLAN_BATCH 1u   /* gpu.Batch conventions: thin-triangle mask,
+                               1e-30 clamps, batch-global pin weight */
+
+/* ARAP semantics */
+#define UVA_MODE_REFINE 0   /* param.refine: per-island pin weight, energy early
+                               exit, line search only if the start has no
+                               flips, exact-root step rule */
+#define UVA_MODE_BATCH 1    /* Batch.arap: no early exit, line search when
+                               asked, non-finite islands keep their layout */
+
+/* precision of the CG global step */
+#define UVA_PREC_MIXED 0    /* FP32 SpMV/vectors, FP64 dots, FP64 true residual */
+#define UVA_PREC_F32 1      /* FP32 everywhere (precision study) */
+#define UVA_PREC_F64 2      /* FP64 everywhere */
+
+typedef struct uva_plan uva_plan;
+
+typedef struct {
+    int64_t n_islands;
+    int64_t n_vertices;
+    int64_t n_triangles;
+    int64_t nnz;             /* stored off-diagonal entries incl. padding */
+    int64_t nnz_true;        /* off-diagonal nonzeros without padding */
+    int64_t device_bytes;    /* device memory held by the plan */
+    int32_t n_group_tasks;   /* islands solved by a whole workgroup */
+    int32_t n_wave_tasks;    /* islands solved by a single wavefront */
+} uva_plan_info;
+
+typedef struct {
+    int32_t cg_iters;        /* inner CG iterations, summed over the call */
+    int32_t cg_iters_max;    /* largest single global-step iteration count */
+    int32_t arap_iters;      /* ARAP iterations run (early exit stops early) */
+    int32_t converged;       /* last global step met the tolerance */
+    int32_t flips0;          /* triangles with signed area <= 0 at the start */
+    int32_t use_ls;          /* line search was active */
+    int32_t status;          /* 0, or UVA_ENOBOUNDARY for LSCM */
+    int32_t pad;
+    double rel_resid;        /* last true relative residual (max over u, v) */
+    double energy;           /* ARAP energy of the returned layout (REFINE) */
+    double step;             /* last line-search step */
+} uva_island_stats;
+
+typedef struct {
+    int32_t iters;           /* ARAP iterations */
+    int32_t line_search;     /* 0/1 */
+    int32_t mode;            /* UVA_MODE_* */
+    int32_t precision;       /* UVA_PREC_* */
+    double tol;              /* energy early-exit tolerance (REFINE) */
+    double cg_tol;           /* relative residual per global step */
+    int32_t cg_max_iters;    /* cap per global step (Batch's cg_iters) */
+    int32_t use_graph;       /* capture the ARAP iterations in a hipGraph */
+} uva_arap_opts;
+
+int uva_version(void);
+const char *uva_last_error(void);
+
+/* Build the device plan of a batch of islands.
+ *   vert_off, tri_off  n_islands+1 prefix offsets (host, int64)
+ *   P                  n_vertices x 3 float64 (host)
+ *   T                  n_triangles x 3 int64, island-local vertex ids (host)
+ *   pins               per-island soft-pin vertex (local id) or NULL (= 0)
+ *   X, area, cots      optional precomputed frames (t x 3 x 2), areas (t) and
+ *                      cotangents (t x 3) in float64 (host), all NULL or all set
+ *   flags              UVA_PLAN_* */
+int uva_plan_create(uva_plan **out, int64_t n_islands, const int64_t *vert_off, const int64_t *tri_off,
+                    const double *P, const int64_t *T, const int64_t *pins, const double *X,
+                    const double *area, const double *cots, uint32_t flags, void *stream);
+/* Plan of one symmetric positive definite matrix (CSR, host, int64 indices,
+ * float64 values, both triangles stored) for uva_cg_solve: the GPU form of
+ * param.py:103-132 SPDSolver(A). */
+int uva_plan_create_matrix(uva_plan **out, int64_t n, const int64_t *indptr, const int64_t *indices,
+                           const double *data, void *stream);
+int uva_plan_destroy(uva_plan *plan);
+int uva_plan_get_info(const uva_plan *plan, uva_plan_info *info);
+/* Per-island statistics of the last solve call (host array of n_islands). */
+int uva_plan_get_stats(uva_plan *plan, uva_island_stats *stats, void *stream);
+/* Set the per-triangle target scale (gpu.py:123 Batch.scale); device array. */
+int uva_plan_set_scale(uva_plan *plan, const double *scale_dev, void *stream);
+
+/* LSCM of every island into U (device, n x 2 float64).  pins: 2 local ids per
+ * island (host) or NULL for the reference's farthest boundary pair.  FP64 CG
+ * to relative residual tol. */
+int uva_lscm(uva_plan *plan, double *U, const int64_t *pins, double tol, int32_t max_iters, void *stream);
+
+/* ARAP iterations on U in place (device, n x 2 float64). */
+int uva_arap(uva_plan *plan, double *U, const uva_arap_opts *opts, void *stream);
+
+/* Building blocks (one launch each). */
+/* local step + rhs: rotations of U and b (device, n x 2 float64); mode picks
+ * the rotation rule (REFINE: identity when degenerate; BATCH: times scale) */
+int uva_local_rhs(uva_plan *plan, const double *U, double *b, int32_t mode, void *stream);
+/* batched CG: x (device, n x 2) warm start in, solution out, for rhs b. */
+int uva_cg_solve(uva_plan *plan, const double *b, double *x, double tol, int32_t max_iters, int32_t precision,
+                 void *stream);
+/* flip-preventing step of U along D per island -> tmax (device, n_islands) */
+int uva_max_step(uva_plan *plan, const double *U, const double *D, double *tmax, int32_t mode, void *stream);
+/* per-island energy (kind 0 ARAP, 1 symmetric Dirichlet) -> e (device) */
+int uva_energy(uva_plan *plan, const double *U, int32_t kind, double *e, void *stream);
+/* per-triangle signed UV area -> sa (device, n_triangles) */
+int uva_signed_areas(uva_plan *plan, const double *U, double *sa, void *stream);
+/* Batch.flips_per_chart -> counts (device, n_islands float64) */
+int uva_flips_per_chart(uva_plan *plan, const double *U, double *counts, void *stream);
+/* Batch.repair_flips: Jacobi 1-ring relaxation, at most `rounds` rounds */
+int uva_repair_jacobi(uva_plan *plan, double *U, int32_t rounds, void *stream);
+/* Batch.orient_positive: mirror islands with negative area-weighted sign sum */
+int uva_orient_positive(uva_plan *plan, double *U, void *stream);
+
+/* param.repair_flips: sequential Gauss-Seidel flip repair of one chart on the
+ * host (the reference's exact update order).  U (n x 2 float64) is updated
+ * in place; T is t x 3 int64. */
+int uva_repair_seq(double *U, int64_t n, const int64_t *T, int64_t t, int32_t rounds);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

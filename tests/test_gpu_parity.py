@@ -1,0 +1,329 @@
+"""GPU parity: every libuvart entry point against the CPU oracle on seeded inputs.
+
+Tolerances (written per test): UVs are compared after normalising each
+island to the unit box (max |dU| <= 1e-4 for ARAP),
+LSCM (FP64 CG to 1e-8) much tighter; per-island scalars relative 1e-9 where
+both sides are FP64 arithmetic on identical inputs.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+def unit_box_err(U, Uref):
+    lo = Uref.min(0)
+    s = float((Uref.max(0) - lo).max())
+    return float(np.abs((U - lo) / s - (Uref - lo) / s).max())
+
+
+@pytest.fixture(scope="module")
+def grid():
+    from ultimate_uv_artist_amd import workloads
+
+    return workloads.grid_island(seed=1)
+
+
+@pytest.fixture(scope="module")
+def small_disks():
+    from ultimate_uv_artist_amd import workloads
+
+    rng = np.random.default_rng(11)
+    sizes = [200, 450, 900, 1600, 2600, 3400, 5000, 260]
+    return [workloads.disk_island(rng, t) for t in sizes]
+
+
+def test_lscm_matches_oracle(grid, small_disks):
+    from oracle import param_cpu as oracle
+    from ultimate_uv_artist_amd import param
+
+    for P, T in [grid] + small_disks[:4]:
+        print(f"lscm: {len(T)} triangles", flush=True)
+        U = param.lscm(P, T)
+        Ur = oracle.lscm(P, T)
+        assert unit_box_err(U, Ur) <= 1e-6
+
+
+def test_lscm_given_pins_and_frames(small_disks):
+    from oracle import param_cpu as oracle
+    from ultimate_uv_artist_amd import param
+
+    P, T = small_disks[2]
+    X, area = oracle.local_frames(P, T)
+    cots = oracle.cot_weights(X)
+    U = param.lscm(P, T, X, cots, pins=(3, 17))
+    Ur = oracle.lscm(P, T, X, cots, pins=(3, 17))
+    assert unit_box_err(U, Ur) <= 1e-6
+
+
+def test_lscm_closed_chart_raises():
+    from ultimate_uv_artist_amd import param
+
+    P = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1.0]])
+    T = np.array([[0, 2, 1], [0, 1, 3], [0, 3, 2], [1, 2, 3]])
+    with pytest.raises(ValueError):
+        param.lscm(P, T)
+
+
+def test_refine_grid_20_iters(grid):
+    """The small workload: 7,938-triangle grid island, LSCM then 20 ARAP iterations."""
+    from oracle import param_cpu as oracle
+    from ultimate_uv_artist_amd import param
+
+    P, T = grid
+    U0 = oracle.lscm(P, T)
+    U, solver = param.refine(P, T, U0, iters=20, tol=0.0)
+    Ur, _ = oracle.refine(P, T, U0, iters=20, tol=0.0)
+    err = unit_box_err(U, Ur)
+    print(f"grid 20 ARAP: unit-box err {err:.2e}", flush=True)
+    assert err <= 1e-4
+    q = param.chart_quality(P, T, U)
+    qr = oracle.chart_quality(P, T, Ur)
+    assert abs(q["aniso_p95"] / qr["aniso_p95"] - 1) <= 0.01
+    assert abs(q["area_p95"] / qr["area_p95"] - 1) <= 0.01
+    assert q["flips"] == qr["flips"] == 0
+    # the returned solver is reusable
+    U2, _ = param.refine(P, T, U0, iters=20, tol=0.0, solver=solver)
+    assert np.array_equal(U, U2)
+
+
+def test_refine_early_exit_and_pin(small_disks):
+    from oracle import param_cpu as oracle
+    from ultimate_uv_artist_amd import _lib
+    from ultimate_uv_artist_amd.plan import IslandPlan
+
+    for P, T in small_disks[3:6]:
+        U0 = oracle.flatten_lscm_arap(P, T, iters=0)
+        trace = []
+        Ur, _ = oracle.refine(P, T, U0, iters=12, tol=1e-3, pin=5, trace=trace)
+        plan = IslandPlan([(P, T)], pins=[5])
+        Ud = plan.device_uv(U0)
+        plan.arap(Ud, 12, line_search=True, mode=_lib.UVA_MODE_REFINE, tol=1e-3)
+        st = plan.stats()
+        U = Ud.cpu().numpy()
+        assert st["arap_iters"][0] == len(trace)
+        assert unit_box_err(U, Ur) <= 1e-4
+        assert np.allclose(U[5], U0[5], atol=1e-9 * np.ptp(U0))   # the soft pin holds exactly
+        assert abs(st["energy"][0] / trace[-1][0] - 1) <= 1e-6
+
+
+def test_refine_folded_start_no_line_search(small_disks):
+    from oracle import param_cpu as oracle
+    from ultimate_uv_artist_amd import param
+
+    P, T = small_disks[1]
+    U0 = oracle.lscm(P, T)
+    rng = np.random.default_rng(5)
+    U0 = U0 + rng.normal(0, 0.05 * np.ptp(U0), U0.shape)      # folds the start
+    assert (oracle.signed_areas(U0, T) <= 0).any()
+    U, _ = param.refine(P, T, U0, iters=6, tol=0.0)
+    Ur, _ = oracle.refine(P, T, U0, iters=6, tol=0.0)
+    assert unit_box_err(U, Ur) <= 1e-4
+
+
+def test_building_blocks(small_disks):
+    """local step + rhs, max step, energy, signed areas against the oracle (FP64 both)."""
+    import scipy.sparse as sp
+
+    from oracle import param_cpu as oracle
+    from ultimate_uv_artist_amd.plan import IslandPlan
+
+    torch = _torch()
+    charts = small_disks[:6]
+    plan = IslandPlan(charts)
+    U0s = [oracle.lscm(P, T) for P, T in charts]
+    U = plan.device_uv(np.concatenate(U0s))
+    b = plan.local_rhs(U).cpu().numpy()
+    rng = np.random.default_rng(2)
+    D = torch.as_tensor(rng.normal(0, 0.02, (plan.n, 2)), device=plan.dev)
+    tm = plan.max_step(U, D).cpu().numpy()
+    e0 = plan.energy(U, 0).cpu().numpy()
+    e1 = plan.energy(U, 1).cpu().numpy()
+    sa = plan.signed_areas(U).cpu().numpy()
+    vo, to = plan.vert_off, plan.tri_off
+    Dh = D.cpu().numpy()
+    for i, (P, T) in enumerate(charts):
+        X, area = oracle.local_frames(P, T)
+        cots = oracle.cot_weights(X)
+        u = U0s[i]
+        R, _ = oracle.arap_rotations(u, T, X, cots, 0)
+        br = oracle.arap_rhs(len(P), T, X, cots, R)
+        I, J, W = oracle.lap_triplets(T, cots)
+        br[0] += np.abs(W).max() * u[0]
+        assert np.abs(b[vo[i]:vo[i + 1]] - br).max() <= 1e-10 * np.abs(br).max()
+        assert tm[i] == pytest.approx(oracle.max_step(u, Dh[vo[i]:vo[i + 1]], T), rel=1e-9)
+        assert e0[i] == pytest.approx(oracle.energy(u, T, X, area, 0), rel=1e-9)
+        assert e1[i] == pytest.approx(oracle.energy(u, T, X, area, 1), rel=1e-9)
+        assert np.abs(sa[to[i]:to[i + 1]] - oracle.signed_areas(u, T)).max() <= 1e-12 * np.ptp(u) ** 2
+        del sp
+
+
+@pytest.mark.parametrize("precision", [0, 2])
+def test_cg_solve_vs_direct(small_disks, precision):
+    """Batched CG (mixed and FP64) against SuperLU on the pinned Laplacians."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+
+    from oracle import param_cpu as oracle
+    from ultimate_uv_artist_amd.plan import IslandPlan
+
+    torch = _torch()
+    charts = small_disks
+    plan = IslandPlan(charts)
+    rng = np.random.default_rng(3)
+    b = rng.normal(size=(plan.n, 2))
+    for i in range(len(charts)):                      # rhs of an ARAP step sums to zero per island
+        s = slice(plan.vert_off[i], plan.vert_off[i + 1])
+        b[s] -= b[s].mean(0)
+    x = torch.zeros(plan.n, 2, dtype=torch.float64, device=plan.dev)
+    plan.cg_solve(torch.as_tensor(b, device=plan.dev), x, tol=1e-10 if precision == 2 else 1e-6,
+                  precision=precision)
+    st = plan.stats()
+    xh = x.cpu().numpy()
+    for i, (P, T) in enumerate(charts):
+        X, _ = oracle.local_frames(P, T)
+        cots = oracle.cot_weights(X)
+        I, J, W = oracle.lap_triplets(T, cots)
+        n = len(P)
+        L = sp.csc_matrix((np.r_[W, np.abs(W).max()], (np.r_[I, 0], np.r_[J, 0])), shape=(n, n))
+        s = slice(plan.vert_off[i], plan.vert_off[i + 1])
+        xr = spla.splu(L).solve(b[s])
+        r = np.linalg.norm(b[s] - L @ xh[s], axis=0) / np.linalg.norm(b[s], axis=0)
+        assert st["converged"][i] == 1
+        assert r.max() <= (1e-10 if precision == 2 else 1e-6) * 1.0001
+        assert np.abs(xh[s] - xr).max() <= (1e-6 if precision == 2 else 1e-2) * np.abs(xr).max()
+
+
+def test_batch_matches_batch_oracle(small_disks):
+    """gpu.Batch semantics: orient, repair, free ARAP, flip-safe ARAP."""
+    from oracle import param_cpu as oracle
+    from oracle.batch_cpu import BatchOracle
+    from ultimate_uv_artist_amd import gpu
+
+    items = [(P, T, oracle.lscm(P, T)) for P, T in small_disks]
+    b = gpu.Batch(items)
+    r = BatchOracle(items)
+    b.orient_positive()
+    r.orient_positive()
+    assert np.abs(b._U.cpu().numpy() - r.U).max() == 0.0
+    b.arap(3, line_search=False, cg_iters=100000, cg_tol=1e-7)
+    r.arap(3, line_search=False)
+    b.arap(3, line_search=True, cg_iters=100000, cg_tol=1e-7)
+    r.arap(3, line_search=True)
+    for u, ur in zip(b.chart_uvs(), r.chart_uvs()):
+        assert unit_box_err(u, ur) <= 1e-4
+    assert b.U.dtype == _torch().float32
+
+
+def test_batch_repair_matches_oracle(small_disks):
+    from oracle import param_cpu as oracle
+    from oracle.batch_cpu import BatchOracle
+    from ultimate_uv_artist_amd import gpu
+
+    rng = np.random.default_rng(9)
+    items = []
+    for P, T in small_disks[:5]:
+        U = oracle.lscm(P, T)
+        U = U + rng.normal(0, 0.02 * np.ptp(U), U.shape)
+        items.append((P, T, U))
+    b = gpu.Batch(items)
+    r = BatchOracle(items)
+    b.repair_flips(30)
+    r.repair_flips(30)
+    for u, ur in zip(b.chart_uvs(), r.chart_uvs()):
+        assert np.abs(u - ur).max() <= 1e-12 * max(1.0, np.abs(ur).max())
+    f = b.flips_per_chart().cpu().numpy()
+    assert np.array_equal(f, r.flips_per_chart())
+
+
+def test_repair_flips_sequential_host():
+    from oracle import param_cpu as oracle
+    from ultimate_uv_artist_amd import param, workloads
+
+    P, T = workloads.disk_island(np.random.default_rng(4), 300)
+    U = oracle.lscm(P, T)
+    U = U + np.random.default_rng(1).normal(0, 0.03 * np.ptp(U), U.shape)
+    assert np.array_equal(param.repair_flips(U, T, 20), oracle.repair_flips(U, T, 20))
+
+
+def test_spdsolver_and_tutte(small_disks):
+    from oracle import param_cpu as oracle
+    from ultimate_uv_artist_amd import param
+
+    P, T = small_disks[2]
+    U = param.tutte(P, T)
+    Ur = oracle.tutte(P, T)
+    assert np.abs(U - Ur).max() <= 1e-8 * np.abs(Ur).max()
+
+
+def test_batched_equals_single_island_plans(small_disks):
+    """Islands are independent: one batched launch gives the same layout as
+    solving each island alone (bitwise), on both the wavefront and the
+    workgroup paths."""
+    from oracle import param_cpu as oracle
+    from ultimate_uv_artist_amd import _lib, workloads
+    from ultimate_uv_artist_amd.plan import IslandPlan
+
+    charts = small_disks[:3] + [workloads.disk_island(np.random.default_rng(21), 9000)]
+    U0 = np.concatenate([oracle.lscm(P, T) for P, T in charts])
+    plan = IslandPlan(charts)
+    Ub = plan.device_uv(U0)
+    plan.arap(Ub, 5, line_search=True, mode=_lib.UVA_MODE_REFINE, tol=0.0)
+    Ub = Ub.cpu().numpy()
+    for i, (P, T) in enumerate(charts):
+        s = slice(plan.vert_off[i], plan.vert_off[i + 1])
+        p1 = IslandPlan([(P, T)])
+        u = p1.device_uv(U0[s])
+        p1.arap(u, 5, line_search=True, mode=_lib.UVA_MODE_REFINE, tol=0.0)
+        assert np.array_equal(u.cpu().numpy(), Ub[s])
+
+
+def test_graph_replays_match_eager(small_disks):
+    """hipGraph replays of the captured ARAP sequence, with eager work and
+    synchronisations between them, reproduce the eager result every time."""
+    from oracle import param_cpu as oracle
+    from ultimate_uv_artist_amd import _lib
+    from ultimate_uv_artist_amd.plan import IslandPlan
+
+    torch = _torch()
+    charts = small_disks
+    U0 = np.concatenate([oracle.lscm(P, T) for P, T in charts])
+    plan = IslandPlan(charts)
+    ref = plan.device_uv(U0)
+    plan.arap(ref, 4, mode=_lib.UVA_MODE_REFINE, tol=0.0)
+    ref = ref.cpu().numpy()
+    U = plan.device_uv(U0)
+    for rep in range(4):
+        U.copy_(torch.as_tensor(U0, device=plan.dev))
+        plan.arap(U, 4, mode=_lib.UVA_MODE_REFINE, tol=0.0, use_graph=True)
+        torch.cuda.synchronize()
+        assert np.array_equal(U.cpu().numpy(), ref), f"replay {rep}"
+        _ = plan.energy(U)                              # eager work between replays
+        torch.cuda.synchronize()
+
+
+def test_nonfinite_island_reverts_in_batch(small_disks):
+    from oracle import param_cpu as oracle
+    from ultimate_uv_artist_amd import gpu
+
+    items = [(P, T, oracle.lscm(P, T)) for P, T in small_disks[:4]]
+    good = gpu.Batch(items)
+    good.arap(2, line_search=True, cg_iters=100000)
+    bad_items = list(items)
+    Ubad = items[1][2].copy()
+    Ubad[3] = np.nan
+    bad_items[1] = (items[1][0], items[1][1], Ubad)
+    b = gpu.Batch(bad_items)
+    b.arap(2, line_search=True, cg_iters=100000)
+    a, c = good.chart_uvs(), b.chart_uvs()
+    for i in (0, 2, 3):
+        assert np.array_equal(a[i], c[i])
+    assert np.isnan(c[1][3]).all()

@@ -1,0 +1,493 @@
+// C ABI of libuvart: plan lifetime, device memory, launch sequencing and
+// hipGraph capture of the ARAP iterations.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/uvart.h"
+#include "kernels.hpp"
+#include "setup.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                             \
+    do {                                                                                          \
+        hipError_t _e = (expr);                                                                   \
+        if (_e != hipSuccess) return fail(UVA_EHIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+constexpr int kMaxSlots = 2048;  // queue counter pairs per call sequence
+constexpr int kDefaultMaxIters = 20000;  // CG iterations per global step when the caller gives none
+
+}  // namespace
+
+struct uva_plan {
+    uva::DevPlan dp{};
+    std::vector<void*> allocs;
+    int64_t device_bytes = 0;
+    int64_t nnz_pad = 0, nnz_true = 0;
+    int grid = 1;
+    std::vector<double> P;                 // host positions (LSCM pins given later)
+    std::vector<int> lpin0, lpin1, v0;     // host copies
+    int* d_lpin0 = nullptr;
+    int* d_lpin1 = nullptr;
+    double* d_ldist = nullptr;
+    // graph cache of uva_arap
+    hipStream_t cap = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+    std::vector<char> gkey;
+
+    ~uva_plan() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (cap) (void)hipStreamDestroy(cap);
+        for (void* p : allocs) (void)hipFree(p);
+    }
+
+    template <class T>
+    hipError_t alloc(T** out, size_t n) {
+        void* p = nullptr;
+        size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e != hipSuccess) return e;
+        allocs.push_back(p);
+        device_bytes += (int64_t)bytes;
+        *out = (T*)p;
+        return hipSuccess;
+    }
+    template <class T>
+    hipError_t upload(const T** out, const std::vector<T>& v, hipStream_t s) {
+        T* p = nullptr;
+        hipError_t e = alloc(&p, v.size());
+        if (e != hipSuccess) return e;
+        if (!v.empty()) e = hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s);
+        *out = p;
+        return e;
+    }
+};
+
+namespace {
+
+int launch(uva_plan* pl, int kind, int prec, const uva::TaskArgs& a, int slot, hipStream_t s) {
+    HIP_TRY(uva::launch_island_tasks(kind, prec, pl->dp, a, slot, pl->grid, s));
+    return UVA_OK;
+}
+
+int reset_queue(uva_plan* pl, int slots, hipStream_t s) {
+    HIP_TRY(uva::launch_fill_int(pl->dp.queue, 2 * slots, 0, s));
+    return UVA_OK;
+}
+
+uva::TaskArgs base_args(double* U) {
+    uva::TaskArgs a{};
+    a.U = (double2*)U;
+    a.cg_tol = 1e-6;
+    a.cg_max = kDefaultMaxIters;
+    return a;
+}
+
+// One sequence of launches (queue reset, prologue, iterations) with slots
+// reused in chunks of kMaxSlots.
+int arap_sequence(uva_plan* pl, double* U, const uva_arap_opts* o, hipStream_t s) {
+    uva::TaskArgs a = base_args(U);
+    a.cg_tol = o->cg_tol;
+    a.cg_max = o->cg_max_iters > 0 ? o->cg_max_iters : kDefaultMaxIters;
+    a.tol = o->tol;
+    a.line_search = o->line_search;
+    a.batch = o->mode == UVA_MODE_BATCH;
+    int total = 1 + o->iters;
+    int done = 0;
+    while (done < total) {
+        int chunk = std::min(total - done, kMaxSlots);
+        int rc = reset_queue(pl, chunk, s);
+        if (rc) return rc;
+        for (int k = 0; k < chunk; ++k) {
+            int step = done + k;
+            rc = launch(pl, step == 0 ? uva::TASK_PROLOGUE : uva::TASK_ARAP_ITER, o->precision, a, k, s);
+            if (rc) return rc;
+        }
+        done += chunk;
+    }
+    return UVA_OK;
+}
+
+bool valid_plan(const uva_plan* p) { return p != nullptr; }
+
+}  // namespace
+
+namespace {
+
+// Upload a host plan and allocate the scratch of a new device plan.
+int upload_plan(uva::HostPlan& hp, const double* P, hipStream_t s, uva_plan** out) {
+    std::unique_ptr<uva_plan> hold(new (std::nothrow) uva_plan());
+    uva_plan* pl = hold.get();
+    if (!pl) return fail(UVA_ENOMEM, "plan allocation failed");
+    auto bail = [&](int code, const std::string& m) { return fail(code, m); };
+#define UP(field, vec)                                                                           \
+    do {                                                                                         \
+        hipError_t _e = pl->upload(&pl->dp.field, hp.vec, s);                                    \
+        if (_e != hipSuccess) return bail(_e == hipErrorOutOfMemory ? UVA_ENOMEM : UVA_EHIP,     \
+                                          std::string("upload " #vec ": ") + hipGetErrorString(_e)); \
+    } while (0)
+    uva::DevPlan& d = pl->dp;
+    d.n_isl = hp.n_isl;
+    d.n_vert = hp.n_vert;
+    d.n_tri = hp.n_tri;
+    d.mode_batch = hp.batch;
+    UP(isl_v0, isl_v0); UP(isl_nv, isl_nv); UP(isl_t0, isl_t0); UP(isl_nt, isl_nt);
+    UP(isl_s0, isl_s0); UP(isl_ns, isl_ns); UP(isl_pin, isl_pin); UP(isl_wpin, isl_wpin);
+    UP(tv0, tv0); UP(tv1, tv1); UP(tv2, tv2);
+    UP(fx1, fx1); UP(fx2, fx2); UP(fy2, fy2); UP(farea, farea); UP(fok, fok);
+    UP(inc_ptr, inc_ptr); UP(inc_tri, inc_tri);
+    UP(sl_ptr, sl_ptr); UP(m_col, m_col); UP(m_hi, m_hi); UP(m_lo, m_lo); UP(m_b, m_b);
+    UP(diag, diag); UP(dinv32, dinv32);
+    UP(adj_ptr, adj_ptr); UP(adj_idx, adj_idx); UP(vbnd, vbnd);
+    UP(order_group, order_group); UP(order_wave, order_wave);
+#undef UP
+    {
+        const double* g = nullptr;
+        hipError_t e = pl->upload(&g, hp.inc_g, s);
+        if (e != hipSuccess) return bail(UVA_EHIP, "upload inc_g");
+        d.inc_g = (const double2*)g;
+    }
+    // LSCM pins (mutable through uva_lscm)
+    pl->lpin0 = hp.isl_lpin0;
+    pl->lpin1 = hp.isl_lpin1;
+    pl->v0 = hp.isl_v0;
+    if (P) pl->P.assign(P, P + 3 * (size_t)hp.n_vert);
+    {
+        hipError_t e;
+        if ((e = pl->alloc(&pl->d_lpin0, hp.n_isl)) != hipSuccess ||
+            (e = pl->alloc(&pl->d_lpin1, hp.n_isl)) != hipSuccess ||
+            (e = pl->alloc(&pl->d_ldist, hp.n_isl)) != hipSuccess)
+            return bail(UVA_ENOMEM, "pin arrays");
+        HIP_TRY(hipMemcpyAsync(pl->d_lpin0, hp.isl_lpin0.data(), hp.n_isl * sizeof(int), hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(pl->d_lpin1, hp.isl_lpin1.data(), hp.n_isl * sizeof(int), hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(pl->d_ldist, hp.isl_ldist.data(), hp.n_isl * sizeof(double), hipMemcpyHostToDevice, s));
+        d.isl_lpin0 = pl->d_lpin0;
+        d.isl_lpin1 = pl->d_lpin1;
+        d.isl_ldist = pl->d_ldist;
+    }
+    d.n_group = (int)hp.order_group.size();
+    d.n_wave = (int)hp.order_wave.size();
+    // scratch
+    {
+        hipError_t e = hipSuccess;
+        size_t nv = hp.n_vert, nt = hp.n_tri;
+        double* sc = nullptr;
+        if ((e = pl->alloc(&sc, nt)) != hipSuccess) return bail(UVA_ENOMEM, "scale");
+        d.fscale = sc;
+        std::vector<double> ones(nt, 1.0);
+        if (nt) HIP_TRY(hipMemcpyAsync(sc, ones.data(), nt * sizeof(double), hipMemcpyHostToDevice, s));
+        double2* p2 = nullptr;
+        if ((e = pl->alloc(&p2, nt)) != hipSuccess) return bail(UVA_ENOMEM, "R");
+        d.R = p2;
+        if ((e = pl->alloc(&p2, nv)) != hipSuccess) return bail(UVA_ENOMEM, "b64");
+        d.b64 = p2;
+        if ((e = pl->alloc(&p2, nv)) != hipSuccess) return bail(UVA_ENOMEM, "X64");
+        d.X64 = p2;
+        void** vecs[4] = {&d.vr, &d.vp, &d.vAp, &d.ve};
+        for (void** v : vecs) {
+            double2* q = nullptr;
+            if ((e = pl->alloc(&q, nv)) != hipSuccess) return bail(UVA_ENOMEM, "CG vectors");
+            *v = q;
+        }
+        double* red = nullptr;
+        if ((e = pl->alloc(&red, 4 * (size_t)hp.n_isl)) != hipSuccess) return bail(UVA_ENOMEM, "red");
+        d.red = red;
+        uva::IslandState* st = nullptr;
+        if ((e = pl->alloc(&st, hp.n_isl)) != hipSuccess) return bail(UVA_ENOMEM, "state");
+        HIP_TRY(hipMemsetAsync(st, 0, hp.n_isl * sizeof(uva::IslandState), s));
+        d.st = st;
+        int* q = nullptr;
+        if ((e = pl->alloc(&q, 2 * (size_t)kMaxSlots)) != hipSuccess) return bail(UVA_ENOMEM, "queue");
+        d.queue = q;
+    }
+    pl->nnz_pad = hp.sl_ptr.empty() ? 0 : hp.sl_ptr.back();
+    pl->nnz_true = hp.nnz_true;
+    // grid: enough workgroups to hold every group task and the wave tasks,
+    // capped at what the device keeps resident
+    int dev = 0, cus = 1;
+    HIP_TRY(hipGetDevice(&dev));
+    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    int per_cu = uva::island_kernel_occupancy();
+    int want = d.n_group + (d.n_wave + uva::kBlock / 64 - 1) / (uva::kBlock / 64);
+    pl->grid = std::max(1, std::min(want, cus * per_cu));
+    hipError_t se = hipStreamSynchronize(s);
+    if (se != hipSuccess) return bail(UVA_EHIP, std::string("plan upload: ") + hipGetErrorString(se));
+    *out = hold.release();
+    return UVA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int uva_version(void) { return 10000; }
+
+const char* uva_last_error(void) { return g_err.c_str(); }
+
+int uva_plan_create(uva_plan** out, int64_t n_islands, const int64_t* vert_off, const int64_t* tri_off,
+                    const double* P, const int64_t* T, const int64_t* pins, const double* X, const double* area,
+                    const double* cots, uint32_t flags, void* stream) {
+    if (!out || !vert_off || !tri_off || !P || !T) return fail(UVA_EINVAL, "null argument");
+    *out = nullptr;
+    if ((X || area || cots) && !(X && area && cots)) return fail(UVA_EINVAL, "X, area and cots go together");
+    hipStream_t s = (hipStream_t)stream;
+    uva::HostPlan hp;
+    std::string err;
+    try {
+        err = uva::build_host_plan(hp, n_islands, vert_off, tri_off, P, T, pins, X, area, cots,
+                                   (flags & UVA_PLAN_BATCH) != 0, nullptr);
+    } catch (const std::bad_alloc&) {
+        return fail(UVA_ENOMEM, "host allocation failed while building the plan");
+    }
+    if (!err.empty()) return fail(UVA_EINVAL, err);
+    return upload_plan(hp, P, s, out);
+}
+
+int uva_plan_create_matrix(uva_plan** out, int64_t n, const int64_t* indptr, const int64_t* indices,
+                           const double* data, void* stream) {
+    if (!out || !indptr || (!indices && n > 0) || (!data && n > 0)) return fail(UVA_EINVAL, "null argument");
+    *out = nullptr;
+    uva::HostPlan hp;
+    std::string err;
+    try {
+        err = uva::build_matrix_plan(hp, n, indptr, indices, data);
+    } catch (const std::bad_alloc&) {
+        return fail(UVA_ENOMEM, "host allocation failed while building the plan");
+    }
+    if (!err.empty()) return fail(UVA_EINVAL, err);
+    return upload_plan(hp, nullptr, (hipStream_t)stream, out);
+}
+
+int uva_plan_destroy(uva_plan* plan) {
+    if (!plan) return UVA_OK;
+    (void)hipDeviceSynchronize();
+    delete plan;
+    return UVA_OK;
+}
+
+int uva_plan_get_info(const uva_plan* plan, uva_plan_info* info) {
+    if (!valid_plan(plan) || !info) return fail(UVA_EINVAL, "null argument");
+    info->n_islands = plan->dp.n_isl;
+    info->n_vertices = plan->dp.n_vert;
+    info->n_triangles = plan->dp.n_tri;
+    info->nnz = plan->nnz_pad;
+    info->nnz_true = plan->nnz_true;
+    info->device_bytes = plan->device_bytes;
+    info->n_group_tasks = plan->dp.n_group;
+    info->n_wave_tasks = plan->dp.n_wave;
+    return UVA_OK;
+}
+
+int uva_plan_get_stats(uva_plan* plan, uva_island_stats* stats, void* stream) {
+    if (!valid_plan(plan) || !stats) return fail(UVA_EINVAL, "null argument");
+    hipStream_t s = (hipStream_t)stream;
+    std::vector<uva::IslandState> st(plan->dp.n_isl);
+    HIP_TRY(hipMemcpyAsync(st.data(), plan->dp.st, st.size() * sizeof(uva::IslandState), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    for (size_t i = 0; i < st.size(); ++i) {
+        uva_island_stats& o = stats[i];
+        o.cg_iters = st[i].cg_iters;
+        o.cg_iters_max = st[i].cg_iters_max;
+        o.arap_iters = st[i].arap_iters;
+        o.converged = st[i].converged;
+        o.flips0 = st[i].flips0;
+        o.use_ls = st[i].use_ls;
+        o.status = st[i].status;
+        o.pad = 0;
+        o.rel_resid = st[i].rel_resid;
+        o.energy = st[i].e_prev;
+        o.step = st[i].step;
+    }
+    return UVA_OK;
+}
+
+int uva_plan_set_scale(uva_plan* plan, const double* scale_dev, void* stream) {
+    if (!valid_plan(plan) || !scale_dev) return fail(UVA_EINVAL, "null argument");
+    if (plan->dp.n_tri)
+        HIP_TRY(hipMemcpyAsync(plan->dp.fscale, scale_dev, plan->dp.n_tri * sizeof(double), hipMemcpyDeviceToDevice,
+                               (hipStream_t)stream));
+    return UVA_OK;
+}
+
+int uva_lscm(uva_plan* plan, double* U, const int64_t* pins, double tol, int32_t max_iters, void* stream) {
+    if (!valid_plan(plan) || !U) return fail(UVA_EINVAL, "null argument");
+    if (!(tol > 0.0)) return fail(UVA_EINVAL, "tol must be positive");
+    hipStream_t s = (hipStream_t)stream;
+    uva::DevPlan& d = plan->dp;
+    if (pins) {
+        std::vector<int> a(d.n_isl), b(d.n_isl);
+        std::vector<double> dist(d.n_isl);
+        for (int i = 0; i < d.n_isl; ++i) {
+            int nv = (i + 1 < d.n_isl ? plan->v0[i + 1] : d.n_vert) - plan->v0[i];
+            if (pins[2 * i] < 0 || pins[2 * i] >= nv || pins[2 * i + 1] < 0 || pins[2 * i + 1] >= nv)
+                return fail(UVA_EINVAL, "LSCM pin outside its island");
+            a[i] = plan->v0[i] + (int)pins[2 * i];
+            b[i] = plan->v0[i] + (int)pins[2 * i + 1];
+            const double* p0 = &plan->P[3 * (size_t)a[i]];
+            const double* p1 = &plan->P[3 * (size_t)b[i]];
+            dist[i] = std::sqrt((p1[0] - p0[0]) * (p1[0] - p0[0]) + (p1[1] - p0[1]) * (p1[1] - p0[1]) +
+                                (p1[2] - p0[2]) * (p1[2] - p0[2]));
+        }
+        HIP_TRY(hipMemcpyAsync(plan->d_lpin0, a.data(), a.size() * sizeof(int), hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(plan->d_lpin1, b.data(), b.size() * sizeof(int), hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(plan->d_ldist, dist.data(), dist.size() * sizeof(double), hipMemcpyHostToDevice, s));
+        HIP_TRY(hipStreamSynchronize(s));
+    } else {
+        for (int i = 0; i < d.n_isl; ++i)
+            if (plan->lpin0[i] < 0)
+                return fail(UVA_ENOBOUNDARY, "island " + std::to_string(i) +
+                                                 " has no boundary (closed surface): needs a cut");
+    }
+    uva::TaskArgs a = base_args(U);
+    a.cg_tol = tol;
+    a.cg_max = max_iters > 0 ? max_iters : kDefaultMaxIters;
+    int rc = reset_queue(plan, 1, s);
+    if (rc) return rc;
+    return launch(plan, uva::TASK_LSCM, UVA_PREC_F64, a, 0, s);
+}
+
+int uva_arap(uva_plan* plan, double* U, const uva_arap_opts* o, void* stream) {
+    if (!valid_plan(plan) || !U || !o) return fail(UVA_EINVAL, "null argument");
+    if (o->iters < 0) return fail(UVA_EINVAL, "iters must be >= 0");
+    if (!(o->cg_tol > 0.0)) return fail(UVA_EINVAL, "cg_tol must be positive");
+    if (o->precision < UVA_PREC_MIXED || o->precision > UVA_PREC_F64) return fail(UVA_EINVAL, "bad precision");
+    if (o->mode != UVA_MODE_REFINE && o->mode != UVA_MODE_BATCH) return fail(UVA_EINVAL, "bad mode");
+    hipStream_t s = (hipStream_t)stream;
+    if (!o->use_graph) return arap_sequence(plan, U, o, s);
+    // graph of the whole sequence, cached per (U, options)
+    std::vector<char> key(sizeof(uva_arap_opts) + sizeof(double*));
+    std::memcpy(key.data(), o, sizeof(uva_arap_opts));
+    std::memcpy(key.data() + sizeof(uva_arap_opts), &U, sizeof(double*));
+    if (!plan->gexec || key != plan->gkey) {
+        if (plan->gexec) { (void)hipGraphExecDestroy(plan->gexec); plan->gexec = nullptr; }
+        if (plan->graph) { (void)hipGraphDestroy(plan->graph); plan->graph = nullptr; }
+        if (!plan->cap) HIP_TRY(hipStreamCreateWithFlags(&plan->cap, hipStreamNonBlocking));
+        HIP_TRY(hipStreamBeginCapture(plan->cap, hipStreamCaptureModeThreadLocal));
+        int rc = arap_sequence(plan, U, o, plan->cap);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(plan->cap, &g);
+        if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+        if (e != hipSuccess) return fail(UVA_EHIP, std::string("graph capture: ") + hipGetErrorString(e));
+        plan->graph = g;
+        HIP_TRY(hipGraphInstantiate(&plan->gexec, g, nullptr, nullptr, 0));
+        plan->gkey = key;
+    }
+    HIP_TRY(hipGraphLaunch(plan->gexec, s));
+    return UVA_OK;
+}
+
+int uva_local_rhs(uva_plan* plan, const double* U, double* b, int32_t mode, void* stream) {
+    if (!valid_plan(plan) || !U || !b) return fail(UVA_EINVAL, "null argument");
+    hipStream_t s = (hipStream_t)stream;
+    uva::TaskArgs a = base_args(const_cast<double*>(U));
+    a.out = (double2*)b;
+    a.batch = mode == UVA_MODE_BATCH;
+    int rc = reset_queue(plan, 1, s);
+    if (rc) return rc;
+    return launch(plan, uva::TASK_LOCAL_RHS, UVA_PREC_MIXED, a, 0, s);
+}
+
+int uva_cg_solve(uva_plan* plan, const double* b, double* x, double tol, int32_t max_iters, int32_t precision,
+                 void* stream) {
+    if (!valid_plan(plan) || !b || !x) return fail(UVA_EINVAL, "null argument");
+    if (!(tol > 0.0)) return fail(UVA_EINVAL, "tol must be positive");
+    if (precision < UVA_PREC_MIXED || precision > UVA_PREC_F64) return fail(UVA_EINVAL, "bad precision");
+    hipStream_t s = (hipStream_t)stream;
+    uva::TaskArgs a = base_args(x);
+    a.in = (const double2*)b;
+    a.cg_tol = tol;
+    a.cg_max = max_iters > 0 ? max_iters : kDefaultMaxIters;
+    int rc = reset_queue(plan, 1, s);
+    if (rc) return rc;
+    return launch(plan, uva::TASK_CG, precision, a, 0, s);
+}
+
+int uva_max_step(uva_plan* plan, const double* U, const double* D, double* tmax, int32_t mode, void* stream) {
+    if (!valid_plan(plan) || !U || !D || !tmax) return fail(UVA_EINVAL, "null argument");
+    hipStream_t s = (hipStream_t)stream;
+    uva::TaskArgs a = base_args(const_cast<double*>(U));
+    a.in = (const double2*)D;
+    a.out_isl = tmax;
+    a.batch = mode == UVA_MODE_BATCH;
+    int rc = reset_queue(plan, 1, s);
+    if (rc) return rc;
+    return launch(plan, uva::TASK_MAX_STEP, UVA_PREC_MIXED, a, 0, s);
+}
+
+int uva_energy(uva_plan* plan, const double* U, int32_t kind, double* e, void* stream) {
+    if (!valid_plan(plan) || !U || !e) return fail(UVA_EINVAL, "null argument");
+    if (kind != 0 && kind != 1) return fail(UVA_EINVAL, "kind must be 0 (ARAP) or 1 (symmetric Dirichlet)");
+    hipStream_t s = (hipStream_t)stream;
+    uva::TaskArgs a = base_args(const_cast<double*>(U));
+    a.out_isl = e;
+    a.kind = kind;
+    int rc = reset_queue(plan, 1, s);
+    if (rc) return rc;
+    return launch(plan, uva::TASK_ENERGY, UVA_PREC_MIXED, a, 0, s);
+}
+
+int uva_signed_areas(uva_plan* plan, const double* U, double* sa, void* stream) {
+    if (!valid_plan(plan) || !U || !sa) return fail(UVA_EINVAL, "null argument");
+    HIP_TRY(uva::launch_signed_areas(plan->dp, (const double2*)U, sa, (hipStream_t)stream));
+    return UVA_OK;
+}
+
+int uva_flips_per_chart(uva_plan* plan, const double* U, double* counts, void* stream) {
+    if (!valid_plan(plan) || !U || !counts) return fail(UVA_EINVAL, "null argument");
+    hipStream_t s = (hipStream_t)stream;
+    uva::TaskArgs a = base_args(const_cast<double*>(U));
+    a.out_isl = counts;
+    int rc = reset_queue(plan, 1, s);
+    if (rc) return rc;
+    return launch(plan, uva::TASK_FLIPS, UVA_PREC_MIXED, a, 0, s);
+}
+
+int uva_repair_jacobi(uva_plan* plan, double* U, int32_t rounds, void* stream) {
+    if (!valid_plan(plan) || !U) return fail(UVA_EINVAL, "null argument");
+    if (rounds < 0) return fail(UVA_EINVAL, "rounds must be >= 0");
+    hipStream_t s = (hipStream_t)stream;
+    uva::TaskArgs a = base_args(U);
+    a.rounds = rounds;
+    int rc = reset_queue(plan, 1, s);
+    if (rc) return rc;
+    return launch(plan, uva::TASK_REPAIR, UVA_PREC_MIXED, a, 0, s);
+}
+
+int uva_orient_positive(uva_plan* plan, double* U, void* stream) {
+    if (!valid_plan(plan) || !U) return fail(UVA_EINVAL, "null argument");
+    hipStream_t s = (hipStream_t)stream;
+    uva::TaskArgs a = base_args(U);
+    int rc = reset_queue(plan, 1, s);
+    if (rc) return rc;
+    return launch(plan, uva::TASK_ORIENT, UVA_PREC_MIXED, a, 0, s);
+}
+
+int uva_repair_seq(double* U, int64_t n, const int64_t* T, int64_t t, int32_t rounds) {
+    if (!U || (!T && t > 0) || n < 0 || t < 0) return fail(UVA_EINVAL, "bad argument");
+    for (int64_t i = 0; i < 3 * t; ++i)
+        if (T[i] < 0 || T[i] >= n) return fail(UVA_EINVAL, "triangle vertex out of range");
+    try {
+        uva::repair_sequential(U, n, T, t, rounds);
+    } catch (const std::bad_alloc&) {
+        return fail(UVA_ENOMEM, "host allocation failed");
+    }
+    return UVA_OK;
+}
+
+}  // extern "C"

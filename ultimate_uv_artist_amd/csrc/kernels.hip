@@ -1,0 +1,846 @@
+// gfx950 kernels of the UV parameterization hot path.
+//
+// One persistent kernel serves every per-island phase.  Its workgroups pull
+// islands from a device-side queue ordered most-expensive-first: large
+// islands are solved by the whole workgroup (block barriers only), small ones
+// by a single wavefront (no barriers at all), so no island ever needs a
+// grid-wide synchronisation and every island carries its own convergence
+// state.  Rows are mapped slice-by-slice (64 rows = one wavefront, lane =
+// row) so matrix loads are contiguous and every phase touches the rows owned
+// by the same lane.
+//
+// All reductions are in FP64 with a fixed order (butterfly inside a
+// wavefront, then wavefront slots in order), so results do not depend on
+// which workgroup picked an island.
+#include "kernels.hpp"
+
+namespace uva {
+
+// ---------------------------------------------------------------- reductions
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;  // identical in every lane: each butterfly stage adds a commuted pair
+}
+
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v = fmin(v, __shfl_xor(v, m, 64));
+    return v;
+}
+
+struct WaveTeam {
+    __device__ int lane() const { return threadIdx.x & 63; }
+    __device__ int wave() const { return 0; }
+    __device__ int nwaves() const { return 1; }
+    __device__ int rank() const { return threadIdx.x & 63; }
+    __device__ int size() const { return 64; }
+    // Global-memory writes of this wavefront become visible to its other lanes.
+    __device__ void sync() const {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    }
+    __device__ double4 sum4(double4 v) const {
+        v.x = wave_sum(v.x); v.y = wave_sum(v.y); v.z = wave_sum(v.z); v.w = wave_sum(v.w);
+        sync();
+        return v;
+    }
+    __device__ double min1(double v) const {
+        v = wave_min(v);
+        sync();
+        return v;
+    }
+};
+
+template <int BS>
+struct BlockTeam {
+    static constexpr int kWaves = BS / 64;
+    double* lds;  // 2 * kWaves * 4 doubles
+    int par;
+    __device__ explicit BlockTeam(double* l) : lds(l), par(0) {}
+    __device__ int lane() const { return threadIdx.x & 63; }
+    __device__ int wave() const { return threadIdx.x >> 6; }
+    __device__ int nwaves() const { return kWaves; }
+    __device__ int rank() const { return threadIdx.x; }
+    __device__ int size() const { return BS; }
+    __device__ void sync() const { __syncthreads(); }
+    // The two slot buffers alternate; a buffer is rewritten only after the
+    // barrier of the following reduction, when every thread has read it.
+    __device__ double4 sum4(double4 v) {
+        v.x = wave_sum(v.x); v.y = wave_sum(v.y); v.z = wave_sum(v.z); v.w = wave_sum(v.w);
+        double* buf = lds + par * kWaves * 4;
+        par ^= 1;
+        if (lane() == 0) {
+            buf[4 * wave()] = v.x; buf[4 * wave() + 1] = v.y; buf[4 * wave() + 2] = v.z; buf[4 * wave() + 3] = v.w;
+        }
+        __syncthreads();
+        double4 s = make_double4(0.0, 0.0, 0.0, 0.0);
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) { s.x += buf[4 * w]; s.y += buf[4 * w + 1]; s.z += buf[4 * w + 2]; s.w += buf[4 * w + 3]; }
+        return s;
+    }
+    __device__ double min1(double v) {
+        v = wave_min(v);
+        double* buf = lds + par * kWaves * 4;
+        par ^= 1;
+        if (lane() == 0) buf[4 * wave()] = v;
+        __syncthreads();
+        double m = buf[0];
+#pragma unroll
+        for (int w = 1; w < kWaves; ++w) m = fmin(m, buf[4 * w]);
+        return m;
+    }
+};
+
+// ---------------------------------------------------------------- island view
+struct Isl {
+    int v0, nv, t0, nt, s0, ns, pin;
+    double wpin;
+};
+
+__device__ __forceinline__ Isl load_isl(const DevPlan& P, int i) {
+    Isl d;
+    d.v0 = P.isl_v0[i]; d.nv = P.isl_nv[i]; d.t0 = P.isl_t0[i]; d.nt = P.isl_nt[i];
+    d.s0 = P.isl_s0[i]; d.ns = P.isl_ns[i]; d.pin = P.isl_pin[i]; d.wpin = P.isl_wpin[i];
+    return d;
+}
+
+// Iterate the rows owned by this lane: wavefront w of the team takes slices
+// w, w + W, ...; lane l takes row l of the slice.
+#define FOR_ROWS(tm, d, row, sg)                                                           \
+    for (int _s = (tm).wave(); _s < (d).ns; _s += (tm).nwaves())                            \
+        for (int row = (d).v0 + _s * kSlice + (tm).lane(), sg = (d).s0 + _s, _once = 1;     \
+             _once && (_s * kSlice + (tm).lane() < (d).nv); _once = 0)
+
+template <class T>
+struct Vec2;
+template <>
+struct Vec2<float> { using type = float2; };
+template <>
+struct Vec2<double> { using type = double2; };
+
+// ---------------------------------------------------------------- sparse rows
+enum { OP_ARAP = 0, OP_LSCM = 1 };
+
+// y = A x for one row: FP32 values (hi) for float vectors, hi + lo for double.
+template <typename V>
+__device__ __forceinline__ V mval(const DevPlan& P, int o);
+template <>
+__device__ __forceinline__ float mval<float>(const DevPlan& P, int o) { return P.m_hi[o]; }
+template <>
+__device__ __forceinline__ double mval<double>(const DevPlan& P, int o) {
+    return (double)P.m_hi[o] + (double)P.m_lo[o];
+}
+
+template <typename V, int OP>
+__device__ __forceinline__ typename Vec2<V>::type spmv_row(const DevPlan& P, int row, int sg, int lane,
+                                                           const typename Vec2<V>::type* __restrict__ x, V dg) {
+    const int base = P.sl_ptr[sg];
+    const int width = (P.sl_ptr[sg + 1] - base) >> 6;
+    typename Vec2<V>::type xr = x[row];
+    V yx = dg * xr.x, yy = dg * xr.y;
+    int o = base + lane;
+    for (int k = 0; k < width; ++k, o += kSlice) {
+        const int c = P.m_col[o];
+        const V v = mval<V>(P, o);
+        const typename Vec2<V>::type xc = x[c];
+        if (OP == OP_ARAP) {
+            yx = fma(v, xc.x, yx);
+            yy = fma(v, xc.y, yy);
+        } else {
+            const V bv = (V)P.m_b[o];
+            yx = fma(v, xc.x, fma(bv, xc.y, yx));
+            yy = fma(v, xc.y, -fma(bv, xc.x, -yy));
+        }
+    }
+    typename Vec2<V>::type y;
+    y.x = yx;
+    y.y = yy;
+    return y;
+}
+
+template <int OP>
+__device__ __forceinline__ double diag_of(const DevPlan& P, const Isl& d, int row) {
+    double g = P.diag[row];
+    if (OP == OP_ARAP && row == d.pin) g += d.wpin;
+    return g;
+}
+
+template <int OP>
+__device__ __forceinline__ double dinv_of(const DevPlan& P, const Isl& d, int row) {
+    if (OP == OP_ARAP) return (double)P.dinv32[row];
+    double g = P.diag[row];
+    return g > 1e-30 ? 1.0 / g : 1.0;
+}
+
+// ---------------------------------------------------------------- precision
+template <int PREC>
+struct Prec;
+template <>
+struct Prec<UVA_PREC_MIXED> {
+    using V = float;
+    static constexpr int kMaxOuter = 8;
+    static constexpr double kInnerFrac2 = 0.25;  // inner target (0.5 tol)^2 leaves room for the true residual
+    static constexpr bool kResid64 = true;
+    static constexpr bool kRoundX = false;
+};
+template <>
+struct Prec<UVA_PREC_F32> {
+    using V = float;
+    static constexpr int kMaxOuter = 1;
+    static constexpr double kInnerFrac2 = 1.0;
+    static constexpr bool kResid64 = false;
+    static constexpr bool kRoundX = true;
+};
+template <>
+struct Prec<UVA_PREC_F64> {
+    using V = double;
+    static constexpr int kMaxOuter = 8;
+    static constexpr double kInnerFrac2 = 1.0;
+    static constexpr bool kResid64 = true;
+    static constexpr bool kRoundX = false;
+};
+
+// FP32-only accumulation for the all-FP32 variant, FP64 otherwise.
+template <int PREC>
+__device__ __forceinline__ double acc_mul(float a, float b) {
+    if (PREC == UVA_PREC_F32) return (double)(a * b);
+    return (double)a * (double)b;
+}
+template <int PREC>
+__device__ __forceinline__ double acc_mul(double a, double b) { return a * b; }
+
+struct CGOut {
+    int iters;
+    int conv;
+    double rel;
+};
+
+// Jacobi-preconditioned CG on the rows of one island, warm-started from X64,
+// right-hand side b64.  Two independent columns (u, v) for the ARAP operator,
+// one coupled 2n system for LSCM.  Mixed precision: FP32 inner iterations on
+// the correction e with FP64 dots; every outer pass recomputes the true
+// residual b - A X in FP64 and folds e into X64 (iterative refinement).
+template <int PREC, int OP, class Team>
+__device__ CGOut cg_island(Team& tm, const DevPlan& P, const Isl& d, double2 b2, double tol, int maxit, int lp0,
+                           int lp1) {
+    using PT = Prec<PREC>;
+    using V = typename PT::V;
+    using V2 = typename Vec2<V>::type;
+    V2* __restrict__ r = (V2*)P.vr;
+    V2* __restrict__ p = (V2*)P.vp;
+    V2* __restrict__ Ap = (V2*)P.vAp;
+    V2* __restrict__ e = (V2*)P.ve;
+    const bool coupled = (OP == OP_LSCM);
+    const double bx2 = coupled ? fmax(b2.x + b2.y, 1e-300) : fmax(b2.x, 1e-300);
+    const double by2 = coupled ? bx2 : fmax(b2.y, 1e-300);
+    const double t2 = tol * tol;
+    const int lane = tm.lane();
+    int it = 0;
+    CGOut out{0, 0, 0.0};
+    double relx = 0.0, rely = 0.0;
+    for (int outer = 0;; ++outer) {
+        // true residual and restart
+        double4 acc = make_double4(0.0, 0.0, 0.0, 0.0);
+        FOR_ROWS(tm, d, row, sg) {
+            double rx, ry;
+            const bool pinned = (OP == OP_LSCM) && (row == lp0 || row == lp1);
+            if (PT::kResid64) {
+                double2 y = spmv_row<double, OP>(P, row, sg, lane, P.X64, diag_of<OP>(P, d, row));
+                double2 bb = P.b64[row];
+                rx = bb.x - y.x;
+                ry = bb.y - y.y;
+            } else {
+                // all-FP32: residual from the FP32 matrix and the FP32-rounded iterate
+                const int base = P.sl_ptr[sg];
+                const int width = (P.sl_ptr[sg + 1] - base) >> 6;
+                double2 xr = P.X64[row];
+                float dg = (float)diag_of<OP>(P, d, row);
+                float yx = dg * (float)xr.x, yy = dg * (float)xr.y;
+                int o = base + lane;
+                for (int k = 0; k < width; ++k, o += kSlice) {
+                    double2 xc = P.X64[P.m_col[o]];
+                    yx = fmaf(P.m_hi[o], (float)xc.x, yx);
+                    yy = fmaf(P.m_hi[o], (float)xc.y, yy);
+                }
+                double2 bb = P.b64[row];
+                rx = (double)((float)bb.x - yx);
+                ry = (double)((float)bb.y - yy);
+            }
+            if (pinned) { rx = 0.0; ry = 0.0; }
+            V2 rv;
+            rv.x = (V)rx;
+            rv.y = (V)ry;
+            const V di = (V)dinv_of<OP>(P, d, row);
+            V2 z;
+            z.x = di * rv.x;
+            z.y = di * rv.y;
+            r[row] = rv;
+            p[row] = z;
+            V2 zero;
+            zero.x = 0;
+            zero.y = 0;
+            e[row] = zero;
+            acc.x += PT::kResid64 ? rx * rx : (double)((float)rx * (float)rx);
+            acc.y += PT::kResid64 ? ry * ry : (double)((float)ry * (float)ry);
+            acc.z += acc_mul<PREC>(rv.x, z.x);
+            acc.w += acc_mul<PREC>(rv.y, z.y);
+        }
+        double4 s = tm.sum4(acc);
+        if (coupled) {
+            relx = rely = (s.x + s.y) / bx2;
+        } else {
+            relx = s.x / bx2;
+            rely = s.y / by2;
+        }
+        if (relx <= t2 && rely <= t2) { out.conv = 1; break; }
+        if (outer >= PT::kMaxOuter || it >= maxit || !isfinite(relx) || !isfinite(rely)) break;
+        double rzx = s.z, rzy = s.w;
+        if (coupled) { rzx = rzy = s.z + s.w; }
+        for (;;) {
+            // Ap = A p and p.Ap
+            double4 pa = make_double4(0.0, 0.0, 0.0, 0.0);
+            FOR_ROWS(tm, d, row, sg) {
+                V2 ap = spmv_row<V, OP>(P, row, sg, lane, p, (V)diag_of<OP>(P, d, row));
+                if (OP == OP_LSCM && (row == lp0 || row == lp1)) { ap.x = 0; ap.y = 0; }
+                Ap[row] = ap;
+                V2 pr = p[row];
+                pa.x += acc_mul<PREC>(pr.x, ap.x);
+                pa.y += acc_mul<PREC>(pr.y, ap.y);
+            }
+            double4 ps = tm.sum4(pa);
+            double pax = ps.x, pay = ps.y;
+            if (coupled) pax = pay = ps.x + ps.y;
+            const V ax = (V)(pax > 0.0 ? rzx / pax : 0.0);
+            const V ay = (V)(pay > 0.0 ? rzy / pay : 0.0);
+            // e += alpha p, r -= alpha Ap, z = D^-1 r
+            double4 acc2 = make_double4(0.0, 0.0, 0.0, 0.0);
+            FOR_ROWS(tm, d, row, sg) {
+                V2 pr = p[row], apr = Ap[row], rr = r[row], ee = e[row];
+                ee.x = fma(ax, pr.x, ee.x);
+                ee.y = fma(ay, pr.y, ee.y);
+                rr.x = fma(-ax, apr.x, rr.x);
+                rr.y = fma(-ay, apr.y, rr.y);
+                e[row] = ee;
+                r[row] = rr;
+                const V di = (V)dinv_of<OP>(P, d, row);
+                acc2.x += acc_mul<PREC>(rr.x, rr.x);
+                acc2.y += acc_mul<PREC>(rr.y, rr.y);
+                acc2.z += acc_mul<PREC>(rr.x, di * rr.x);
+                acc2.w += acc_mul<PREC>(rr.y, di * rr.y);
+            }
+            double4 s2 = tm.sum4(acc2);
+            ++it;
+            double rrx = s2.x, rry = s2.y;
+            if (coupled) rrx = rry = s2.x + s2.y;
+            if ((rrx <= PT::kInnerFrac2 * t2 * bx2 && rry <= PT::kInnerFrac2 * t2 * by2) || it >= maxit) break;
+            if (!isfinite(rrx) || !isfinite(rry)) break;   // a non-finite island stops (it is not converged)
+            double nzx = s2.z, nzy = s2.w;
+            if (coupled) nzx = nzy = s2.z + s2.w;
+            const V bex = (V)(rzx > 0.0 ? nzx / rzx : 0.0);
+            const V bey = (V)(rzy > 0.0 ? nzy / rzy : 0.0);
+            rzx = nzx;
+            rzy = nzy;
+            FOR_ROWS(tm, d, row, sg) {
+                V2 pr = p[row], rr = r[row];
+                const V di = (V)dinv_of<OP>(P, d, row);
+                pr.x = fma(bex, pr.x, di * rr.x);
+                pr.y = fma(bey, pr.y, di * rr.y);
+                p[row] = pr;
+            }
+            tm.sync();
+        }
+        // fold the correction into the FP64 iterate
+        FOR_ROWS(tm, d, row, sg) {
+            double2 x = P.X64[row];
+            V2 ee = e[row];
+            x.x += (double)ee.x;
+            x.y += (double)ee.y;
+            if (PT::kRoundX) { x.x = (double)(float)x.x; x.y = (double)(float)x.y; }
+            P.X64[row] = x;
+        }
+        tm.sync();
+    }
+    out.iters = it;
+    out.rel = sqrt(fmax(relx, rely));
+    return out;
+}
+
+// ---------------------------------------------------------------- per-triangle pieces
+struct Jac {
+    double j00, j01, j10, j11;
+};
+
+__device__ __forceinline__ Jac jacobian(const DevPlan& P, int f, const double2* __restrict__ U) {
+    const double2 ua = U[P.tv0[f]], ub = U[P.tv1[f]], uc = U[P.tv2[f]];
+    const double x1 = P.fx1[f], x2 = P.fx2[f], y2 = P.fy2[f];
+    Jac J;
+    J.j00 = (ub.x - ua.x) / x1;
+    J.j10 = (ub.y - ua.y) / x1;
+    J.j01 = ((uc.x - ua.x) - J.j00 * x2) / y2;
+    J.j11 = ((uc.y - ua.y) - J.j10 * x2) / y2;
+    return J;
+}
+
+// ARAP energy of one triangle (param.py:306-311): signed singular values q +- r.
+__device__ __forceinline__ double arap_energy_tri(const Jac& J, double area) {
+    const double qe = 0.5 * (J.j00 + J.j11), qh = 0.5 * (J.j10 - J.j01);
+    const double rf = 0.5 * (J.j00 - J.j11), rg = 0.5 * (J.j10 + J.j01);
+    const double q = sqrt(qe * qe + qh * qh), r = sqrt(rf * rf + rg * rg);
+    const double a = q + r - 1.0, b = q - r - 1.0;
+    return area * (a * a + b * b);
+}
+
+// Symmetric Dirichlet energy of one triangle (param.py:313-318).
+__device__ __forceinline__ double sd_energy_tri(const Jac& J, double area) {
+    const double dj = J.j00 * J.j11 - J.j01 * J.j10;
+    if (!(dj > 0.0)) return __longlong_as_double(0x7ff0000000000000LL);  // +inf (also for NaN)
+    const double fro = J.j00 * J.j00 + J.j01 * J.j01 + J.j10 * J.j10 + J.j11 * J.j11;
+    return area * (fro + fro / (dj * dj) - 4.0);
+}
+
+// Local step: per-triangle target rotation (cos, sin) into P.R, and the ARAP
+// energy of U.  refine: normalised conformal part of J, identity when the
+// frame is degenerate or |.| < 1e-300 (param.py:214-233).  batch: the same
+// rotation (= angle of atan2) for thick finite triangles, identity otherwise,
+// times the triangle's target scale (gpu.py:166-170).
+template <class Team>
+__device__ double local_step(Team& tm, const DevPlan& P, const Isl& d, const double2* __restrict__ U, bool batch,
+                             bool want_energy) {
+    double en = 0.0;
+    for (int f = d.t0 + tm.rank(); f < d.t0 + d.nt; f += tm.size()) {
+        const bool ok = P.fok[f];
+        double ca = 1.0, cb = 0.0;
+        if (ok || batch) {
+            const Jac J = jacobian(P, f, U);
+            const double A = J.j00 + J.j11, B = J.j10 - J.j01;
+            const double h = sqrt(A * A + B * B);
+            if (batch) {
+                if (ok && isfinite(h) && h > 0.0) { ca = A / h; cb = B / h; }
+                const double sc = P.fscale[f];
+                ca *= sc;
+                cb *= sc;
+            } else if (!(h < 1e-300)) {
+                ca = A / h;
+                cb = B / h;
+            }
+            if (want_energy && ok) en += arap_energy_tri(J, P.farea[f]);
+        }
+        P.R[f] = make_double2(ca, cb);
+    }
+    if (!want_energy) { tm.sync(); return 0.0; }
+    return tm.sum4(make_double4(en, 0.0, 0.0, 0.0)).x;
+}
+
+// rhs: b[v] = sum over incident corners of R(tri) g (param.py:457-468, gpu.py:171-181)
+// plus the soft-pin term; X64 = U (warm start).  Returns |b|^2 per column.
+template <class Team>
+__device__ double2 rhs_step(Team& tm, const DevPlan& P, const Isl& d, const double2* __restrict__ U) {
+    double4 acc = make_double4(0.0, 0.0, 0.0, 0.0);
+    FOR_ROWS(tm, d, row, sg) {
+        (void)sg;
+        double bx = 0.0, by = 0.0;
+        for (int q = P.inc_ptr[row]; q < P.inc_ptr[row + 1]; ++q) {
+            const double2 R = P.R[P.inc_tri[q]];
+            const double2 g = P.inc_g[q];
+            bx += R.x * g.x - R.y * g.y;
+            by += R.y * g.x + R.x * g.y;
+        }
+        const double2 u = U[row];
+        if (row == d.pin) {
+            bx += d.wpin * u.x;
+            by += d.wpin * u.y;
+        }
+        P.b64[row] = make_double2(bx, by);
+        P.X64[row] = u;
+        acc.x += bx * bx;
+        acc.y += by * by;
+    }
+    double4 s = tm.sum4(acc);
+    return make_double2(s.x, s.y);
+}
+
+// Flip-preventing step along D = X64 - U (param.py:259-291 for refine;
+// gpu.py:204-227 for batch), minimum over the island's triangles.
+template <class Team>
+__device__ double max_step_island(Team& tm, const DevPlan& P, const Isl& d, const double2* __restrict__ U,
+                                  const double2* __restrict__ Xn, bool batch) {
+    double best = 1e30;
+    for (int f = d.t0 + tm.rank(); f < d.t0 + d.nt; f += tm.size()) {
+        const int a = P.tv0[f], b = P.tv1[f], c = P.tv2[f];
+        const double2 ua = U[a], ub = U[b], uc = U[c];
+        const double2 xa = Xn[a], xb = Xn[b], xc = Xn[c];
+        const double ux1 = ub.x - ua.x, uy1 = ub.y - ua.y, ux2 = uc.x - ua.x, uy2 = uc.y - ua.y;
+        const double dx1 = (xb.x - ub.x) - (xa.x - ua.x), dy1 = (xb.y - ub.y) - (xa.y - ua.y);
+        const double dx2 = (xc.x - uc.x) - (xa.x - ua.x), dy2 = (xc.y - uc.y) - (xa.y - ua.y);
+        const double C = ux1 * uy2 - ux2 * uy1;
+        const double B = ux1 * dy2 + dx1 * uy2 - ux2 * dy1 - dx2 * uy1;
+        const double A = dx1 * dy2 - dx2 * dy1;
+        if (!batch) {
+            if (C <= 0.0) continue;
+            double bm = fabs(B);
+            if (!(bm >= 1e-300)) bm = (bm != bm) ? bm : 1e-300;  // Python max(|B|, 1e-300), NaN kept
+            if (A == 0.0 || fabs(A) < 1e-30 * bm) {
+                if (B < 0.0) {
+                    const double rt = -C / B;
+                    if (rt < best) best = rt;
+                }
+                continue;
+            }
+            const double disc = B * B - 4.0 * A * C;
+            if (disc < 0.0) continue;
+            const double sq = sqrt(disc);
+            const double r1 = (-B - sq) / (2.0 * A), r2 = (-B + sq) / (2.0 * A);
+            if (r1 > 0.0 && r1 < best) best = r1;
+            if (r2 > 0.0 && r2 < best) best = r2;
+        } else {
+            const double big = 1e30;
+            const double lin = (fabs(A) < 1e-30 && B < 0.0) ? -C / B : big;
+            const double disc = B * B - 4.0 * A * C;
+            const double sq = sqrt(fmax(disc, 0.0));
+            const double A2 = fabs(A) < 1e-30 ? 1.0 : 2.0 * A;
+            const double r1 = (-B - sq) / A2, r2 = (-B + sq) / A2;
+            double quad = fmin(r1 > 0.0 ? r1 : big, r2 > 0.0 ? r2 : big);
+            quad = (disc >= 0.0 && fabs(A) >= 1e-30) ? quad : big;
+            const double t = C > 0.0 ? fmin(lin, quad) : big;
+            best = fmin(best, t);
+        }
+    }
+    return tm.min1(best);
+}
+
+__device__ __forceinline__ double signed_area(const DevPlan& P, int f, const double2* __restrict__ U) {
+    const double2 a = U[P.tv0[f]], b = U[P.tv1[f]], c = U[P.tv2[f]];
+    return 0.5 * ((b.x - a.x) * (c.y - a.y) - (c.x - a.x) * (b.y - a.y));
+}
+
+// ---------------------------------------------------------------- island tasks
+template <int KIND, int PREC, class Team>
+__device__ void run_task(Team& tm, const DevPlan& P, int isl, const TaskArgs& a) {
+    IslandState* st = P.st + isl;
+    const Isl d = load_isl(P, isl);
+    double2* U = a.U;
+    if (KIND == TASK_PROLOGUE) {
+        // flips of the start, line-search decision, rotations and energy of U
+        double nflip = 0.0;
+        for (int f = d.t0 + tm.rank(); f < d.t0 + d.nt; f += tm.size()) nflip += signed_area(P, f, U) <= 0.0 ? 1.0 : 0.0;
+        const double fl = tm.sum4(make_double4(nflip, 0.0, 0.0, 0.0)).x;
+        const double en = local_step(tm, P, d, U, a.batch, !a.batch);
+        if (tm.rank() == 0) {
+            st->flips0 = (int)fl;
+            st->use_ls = a.line_search && (a.batch || fl == 0.0);
+            st->e_prev = en;
+            st->done = 0;
+            st->cg_iters = 0;
+            st->cg_iters_max = 0;
+            st->arap_iters = 0;
+            st->converged = 0;
+            st->rel_resid = 0.0;
+            st->step = 0.0;
+            st->tmax = 1e30;
+            st->status = 0;
+        }
+    } else if (KIND == TASK_ARAP_ITER) {
+        if (st->done) return;
+        const double2 b2 = rhs_step(tm, P, d, U);
+        const CGOut cg = cg_island<PREC, OP_ARAP>(tm, P, d, b2, a.cg_tol, a.cg_max, -1, -1);
+        if (a.batch) {
+            // islands whose solve went non-finite keep their layout (gpu.py:235-240)
+            double bad = 0.0;
+            FOR_ROWS(tm, d, row, sg) {
+                (void)sg;
+                const double2 x = P.X64[row];
+                bad += (isfinite(x.x) && isfinite(x.y)) ? 0.0 : 1.0;
+            }
+            if (tm.sum4(make_double4(bad, 0.0, 0.0, 0.0)).x > 0.0) {
+                FOR_ROWS(tm, d, row, sg) {
+                    (void)sg;
+                    P.X64[row] = U[row];
+                }
+                tm.sync();
+            }
+        }
+        const int use_ls = st->use_ls;
+        double step = 1.0, tmax = 1e30;
+        if (use_ls) {
+            tmax = max_step_island(tm, P, d, U, P.X64, a.batch);
+            step = fmin(1.0, 0.9 * tmax);
+        }
+        FOR_ROWS(tm, d, row, sg) {
+            (void)sg;
+            const double2 x = P.X64[row];
+            if (use_ls) {
+                const double2 u = U[row];
+                U[row] = make_double2(u.x + step * (x.x - u.x), u.y + step * (x.y - u.y));
+            } else {
+                U[row] = x;
+            }
+        }
+        tm.sync();
+        const double en = local_step(tm, P, d, U, a.batch, !a.batch);
+        if (tm.rank() == 0) {
+            st->cg_iters += cg.iters;
+            st->cg_iters_max = max(st->cg_iters_max, cg.iters);
+            st->arap_iters += 1;
+            st->converged = cg.conv;
+            st->rel_resid = cg.rel;
+            st->step = step;
+            st->tmax = tmax;
+            if (!a.batch) {
+                const double prev = st->e_prev;
+                if (fabs(prev - en) <= a.tol * fmax(prev, 1e-300)) st->done = 1;
+                st->e_prev = en;
+            }
+        }
+    } else if (KIND == TASK_LSCM) {
+        const int p0 = P.isl_lpin0[isl], p1 = P.isl_lpin1[isl];
+        if (p0 < 0) {
+            if (tm.rank() == 0) st->status = -5;
+            return;
+        }
+        const double dist = P.isl_ldist[isl];
+        // rhs = -H x_pin on the free rows (param.py:153-165)
+        FOR_ROWS(tm, d, row, sg) {
+            (void)sg;
+            P.X64[row] = make_double2(row == p1 ? dist : 0.0, 0.0);
+        }
+        tm.sync();
+        double4 acc = make_double4(0.0, 0.0, 0.0, 0.0);
+        FOR_ROWS(tm, d, row, sg) {
+            double2 y = spmv_row<double, OP_LSCM>(P, row, sg, tm.lane(), P.X64, P.diag[row]);
+            double2 bb = (row == p0 || row == p1) ? make_double2(0.0, 0.0) : make_double2(-y.x, -y.y);
+            P.b64[row] = bb;
+            acc.x += bb.x * bb.x;
+            acc.y += bb.y * bb.y;
+        }
+        const double4 s = tm.sum4(acc);
+        FOR_ROWS(tm, d, row, sg) {
+            (void)sg;
+            P.X64[row] = make_double2(0.0, 0.0);
+        }
+        tm.sync();
+        const CGOut cg = cg_island<UVA_PREC_F64, OP_LSCM>(tm, P, d, make_double2(s.x, s.y), a.cg_tol, a.cg_max, p0, p1);
+        FOR_ROWS(tm, d, row, sg) {
+            (void)sg;
+            double2 x = P.X64[row];
+            if (row == p0) x = make_double2(0.0, 0.0);
+            if (row == p1) x = make_double2(dist, 0.0);
+            U[row] = x;
+        }
+        if (tm.rank() == 0) {
+            st->cg_iters = cg.iters;
+            st->cg_iters_max = cg.iters;
+            st->converged = cg.conv;
+            st->rel_resid = cg.rel;
+            st->status = 0;
+        }
+        tm.sync();
+    } else if (KIND == TASK_LOCAL_RHS) {
+        local_step(tm, P, d, U, a.batch, false);
+        const double2 b2 = rhs_step(tm, P, d, U);
+        (void)b2;
+        FOR_ROWS(tm, d, row, sg) {
+            (void)sg;
+            a.out[row] = P.b64[row];
+        }
+    } else if (KIND == TASK_CG) {
+        // b in a.in, warm start and result in U
+        double4 acc = make_double4(0.0, 0.0, 0.0, 0.0);
+        FOR_ROWS(tm, d, row, sg) {
+            (void)sg;
+            const double2 bb = a.in[row];
+            P.b64[row] = bb;
+            P.X64[row] = U[row];
+            acc.x += bb.x * bb.x;
+            acc.y += bb.y * bb.y;
+        }
+        const double4 s = tm.sum4(acc);
+        const CGOut cg = cg_island<PREC, OP_ARAP>(tm, P, d, make_double2(s.x, s.y), a.cg_tol, a.cg_max, -1, -1);
+        FOR_ROWS(tm, d, row, sg) {
+            (void)sg;
+            U[row] = P.X64[row];
+        }
+        if (tm.rank() == 0) {
+            st->cg_iters = cg.iters;
+            st->cg_iters_max = cg.iters;
+            st->converged = cg.conv;
+            st->rel_resid = cg.rel;
+        }
+        tm.sync();
+    } else if (KIND == TASK_MAX_STEP) {
+        // D in a.in: step limit along D from U; X64 = U + D
+        FOR_ROWS(tm, d, row, sg) {
+            (void)sg;
+            const double2 u = U[row], dd = a.in[row];
+            P.X64[row] = make_double2(u.x + dd.x, u.y + dd.y);
+        }
+        tm.sync();
+        const double t = max_step_island(tm, P, d, U, P.X64, a.batch);
+        if (tm.rank() == 0) a.out_isl[isl] = t;
+    } else if (KIND == TASK_ENERGY) {
+        double en = 0.0;
+        for (int f = d.t0 + tm.rank(); f < d.t0 + d.nt; f += tm.size()) {
+            if (!P.fok[f]) continue;
+            const Jac J = jacobian(P, f, U);
+            en += a.kind == 0 ? arap_energy_tri(J, P.farea[f]) : sd_energy_tri(J, P.farea[f]);
+        }
+        const double s = tm.sum4(make_double4(en, 0.0, 0.0, 0.0)).x;
+        if (tm.rank() == 0) a.out_isl[isl] = s;
+    } else if (KIND == TASK_REPAIR) {
+        // Jacobi 1-ring relaxation of the free vertices of folded thick triangles
+        // (gpu.py:251-273); the batch-wide "no fold left" stop is per island here,
+        // which gives the same layout.
+        int* mark = (int*)P.vr;
+        FOR_ROWS(tm, d, row, sg) {
+            (void)sg;
+            mark[row] = 0;
+        }
+        tm.sync();
+        for (int round = 0; round < a.rounds; ++round) {
+            double nbad = 0.0;
+            for (int f = d.t0 + tm.rank(); f < d.t0 + d.nt; f += tm.size()) {
+                if (signed_area(P, f, U) < 0.0 && P.fok[f]) {
+                    nbad += 1.0;
+                    mark[P.tv0[f]] = 1;
+                    mark[P.tv1[f]] = 1;
+                    mark[P.tv2[f]] = 1;
+                }
+            }
+            if (tm.sum4(make_double4(nbad, 0.0, 0.0, 0.0)).x == 0.0) break;
+            FOR_ROWS(tm, d, row, sg) {
+                (void)sg;
+                double2 u = U[row];
+                if (mark[row] && !P.vbnd[row]) {
+                    double sx = 0.0, sy = 0.0;
+                    const int q0 = P.adj_ptr[row], q1 = P.adj_ptr[row + 1];
+                    for (int q = q0; q < q1; ++q) {
+                        const double2 w = U[P.adj_idx[q]];
+                        sx += w.x;
+                        sy += w.y;
+                    }
+                    const double dg = fmax((double)(q1 - q0), 1.0);
+                    u = make_double2(sx / dg, sy / dg);
+                }
+                P.X64[row] = u;
+                mark[row] = 0;
+            }
+            tm.sync();
+            FOR_ROWS(tm, d, row, sg) {
+                (void)sg;
+                U[row] = P.X64[row];
+            }
+            tm.sync();
+        }
+    } else if (KIND == TASK_ORIENT) {
+        double s = 0.0;
+        for (int f = d.t0 + tm.rank(); f < d.t0 + d.nt; f += tm.size()) {
+            const double sa = signed_area(P, f, U);
+            const double sg = sa > 0.0 ? 1.0 : (sa < 0.0 ? -1.0 : 0.0);
+            s += sg * P.farea[f];
+        }
+        const double tot = tm.sum4(make_double4(s, 0.0, 0.0, 0.0)).x;
+        if (tot < 0.0) {
+            FOR_ROWS(tm, d, row, sg) {
+                (void)sg;
+                double2 u = U[row];
+                U[row] = make_double2(u.x, -u.y);
+            }
+        }
+        tm.sync();
+    } else if (KIND == TASK_FLIPS) {
+        double c = 0.0;
+        for (int f = d.t0 + tm.rank(); f < d.t0 + d.nt; f += tm.size())
+            c += (signed_area(P, f, U) < 0.0 && P.fok[f]) ? 1.0 : 0.0;
+        const double s = tm.sum4(make_double4(c, 0.0, 0.0, 0.0)).x;
+        if (tm.rank() == 0) a.out_isl[isl] = s;
+    }
+}
+
+template <int BS, int KIND, int PREC>
+__global__ __launch_bounds__(BS) void island_kernel(DevPlan P, TaskArgs a, int slot) {
+    __shared__ double red[2 * (BS / 64) * 4];
+    __shared__ int s_task;
+    BlockTeam<BS> bt(red);
+    for (;;) {
+        if (threadIdx.x == 0) s_task = atomicAdd(P.queue + 2 * slot, 1);
+        __syncthreads();
+        const int task = s_task;
+        __syncthreads();
+        if (task >= P.n_group) break;
+        run_task<KIND, PREC>(bt, P, P.order_group[task], a);
+        __syncthreads();
+    }
+    WaveTeam wt;
+    for (;;) {
+        int task = 0;
+        if ((threadIdx.x & 63) == 0) task = atomicAdd(P.queue + 2 * slot + 1, 1);
+        task = __shfl(task, 0, 64);
+        if (task >= P.n_wave) break;
+        run_task<KIND, PREC>(wt, P, P.order_wave[task], a);
+    }
+}
+
+__global__ void fill_int_kernel(int* p, int n, int v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void signed_areas_kernel(DevPlan P, const double2* __restrict__ U, double* __restrict__ sa) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f < P.n_tri) sa[f] = signed_area(P, f, U);
+}
+
+// ---------------------------------------------------------------- launchers
+template <int KIND, int PREC>
+static hipError_t launch_kind(const DevPlan& P, const TaskArgs& a, int slot, int grid, hipStream_t s) {
+    hipLaunchKernelGGL((island_kernel<kBlock, KIND, PREC>), dim3(grid), dim3(kBlock), 0, s, P, a, slot);
+    return hipGetLastError();
+}
+
+hipError_t launch_island_tasks(int kind, int prec, const DevPlan& P, const TaskArgs& a, int slot, int grid,
+                               hipStream_t s) {
+#define UVA_K(K)                                                                 \
+    case K:                                                                      \
+        switch (prec) {                                                          \
+            case UVA_PREC_F32: return launch_kind<K, UVA_PREC_F32>(P, a, slot, grid, s); \
+            case UVA_PREC_F64: return launch_kind<K, UVA_PREC_F64>(P, a, slot, grid, s); \
+            default: return launch_kind<K, UVA_PREC_MIXED>(P, a, slot, grid, s);   \
+        }
+    switch (kind) {
+        UVA_K(TASK_ARAP_ITER)
+        UVA_K(TASK_CG)
+        case TASK_PROLOGUE: return launch_kind<TASK_PROLOGUE, UVA_PREC_MIXED>(P, a, slot, grid, s);
+        case TASK_LSCM: return launch_kind<TASK_LSCM, UVA_PREC_F64>(P, a, slot, grid, s);
+        case TASK_LOCAL_RHS: return launch_kind<TASK_LOCAL_RHS, UVA_PREC_MIXED>(P, a, slot, grid, s);
+        case TASK_MAX_STEP: return launch_kind<TASK_MAX_STEP, UVA_PREC_MIXED>(P, a, slot, grid, s);
+        case TASK_ENERGY: return launch_kind<TASK_ENERGY, UVA_PREC_MIXED>(P, a, slot, grid, s);
+        case TASK_REPAIR: return launch_kind<TASK_REPAIR, UVA_PREC_MIXED>(P, a, slot, grid, s);
+        case TASK_ORIENT: return launch_kind<TASK_ORIENT, UVA_PREC_MIXED>(P, a, slot, grid, s);
+        case TASK_FLIPS: return launch_kind<TASK_FLIPS, UVA_PREC_MIXED>(P, a, slot, grid, s);
+        default: return hipErrorInvalidValue;
+    }
+#undef UVA_K
+}
+
+hipError_t launch_fill_int(int* p, int n, int v, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(fill_int_kernel, dim3((n + 255) / 256), dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_signed_areas(const DevPlan& P, const double2* U, double* sa, hipStream_t s) {
+    if (P.n_tri <= 0) return hipSuccess;
+    hipLaunchKernelGGL(signed_areas_kernel, dim3((P.n_tri + 255) / 256), dim3(256), 0, s, P, U, sa);
+    return hipGetLastError();
+}
+
+int island_kernel_occupancy() {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, island_kernel<kBlock, TASK_ARAP_ITER, UVA_PREC_MIXED>,
+                                                     kBlock, 0) != hipSuccess)
+        return 1;
+    return nb > 0 ? nb : 1;
+}
+
+}  // namespace uva

@@ -1,0 +1,45 @@
+// Kernel launch interface (host side of kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "../../include/uvart.h"
+#include "plan.hpp"
+
+namespace uva {
+
+constexpr int kBlock = 512;   // workgroup size of the island kernel (8 wavefronts)
+
+enum TaskKind {
+    TASK_PROLOGUE = 0,   // flips of the start, line-search decision, rotations + energy
+    TASK_ARAP_ITER = 1,  // rhs -> CG -> (revert) -> line search -> update -> rotations + energy
+    TASK_LSCM = 2,
+    TASK_LOCAL_RHS = 3,
+    TASK_CG = 4,
+    TASK_MAX_STEP = 5,
+    TASK_ENERGY = 6,
+    TASK_REPAIR = 7,
+    TASK_ORIENT = 8,
+    TASK_FLIPS = 9,
+};
+
+struct TaskArgs {
+    double2* U;              // layout (in/out)
+    const double2* in;       // rhs (CG) or direction (max step)
+    double2* out;            // rhs output (local_rhs)
+    double* out_isl;         // per-island scalar output
+    double cg_tol;
+    double tol;              // energy early exit
+    int cg_max;
+    int line_search;
+    int batch;
+    int kind;                // energy kind
+    int rounds;              // repair rounds
+};
+
+hipError_t launch_island_tasks(int kind, int prec, const DevPlan& P, const TaskArgs& a, int slot, int grid,
+                               hipStream_t s);
+hipError_t launch_fill_int(int* p, int n, int v, hipStream_t s);
+hipError_t launch_signed_areas(const DevPlan& P, const double2* U, double* sa, hipStream_t s);
+int island_kernel_occupancy();
+
+}  // namespace uva

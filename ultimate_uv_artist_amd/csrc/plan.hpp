@@ -1,0 +1,95 @@
+// Device-side layout of a batch of islands (shared by host setup and kernels).
+//
+// Vertices, triangles and matrix rows of all islands are concatenated; each
+// island owns contiguous ranges.  Vectors are indexed by global vertex id.
+//
+// Matrix: the off-diagonal part of each island's cotangent Laplacian in
+// sliced ELL with 64-row slices (one slice = one wavefront, lane = row), so
+// every matrix load of a wavefront is one contiguous 256-byte segment.  Slice
+// s covers rows [slice_row[s], slice_row[s] + 64) of one island; its entries
+// live at sl_ptr[s] + k*64 + lane for k < (sl_ptr[s+1] - sl_ptr[s]) / 64.
+// Values are stored as an FP32 pair hi + lo (hi = fp32(v), lo = fp32(v - hi))
+// so the FP32 inner iterations read 4 bytes per entry and the FP64 residual
+// reads 8.  Padding entries point at the row itself with value 0.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace uva {
+
+constexpr int kSlice = 64;          // rows per slice (= wavefront width)
+constexpr int kWaveMaxRows = 1536;  // islands up to this many vertices run on one wavefront
+
+struct IslandState {
+    double e_prev;       // ARAP energy of the current layout
+    double rel_resid;    // last true relative residual
+    double step;         // last line-search step
+    double tmax;         // last max step (before the 0.9 factor)
+    int done;            // energy early exit reached
+    int use_ls;
+    int cg_iters;        // summed over the call
+    int cg_iters_max;
+    int arap_iters;
+    int converged;
+    int flips0;
+    int status;
+};
+
+struct DevPlan {
+    int n_isl, n_vert, n_tri;
+    int mode_batch;              // UVA_PLAN_BATCH conventions
+    // islands
+    const int* isl_v0;
+    const int* isl_nv;
+    const int* isl_t0;
+    const int* isl_nt;
+    const int* isl_s0;           // first slice
+    const int* isl_ns;           // number of slices
+    const int* isl_pin;          // global id of the soft pin
+    const double* isl_wpin;
+    const int* isl_lpin0;        // LSCM pins (global ids), -1 when the island has no boundary
+    const int* isl_lpin1;
+    const double* isl_ldist;     // |P[p1] - P[p0]|
+    // triangles (global vertex ids)
+    const int* tv0;
+    const int* tv1;
+    const int* tv2;
+    const double* fx1;           // Jacobian frame: x1, x2, y2 (clamped in batch mode)
+    const double* fx2;
+    const double* fy2;
+    const double* farea;
+    const unsigned char* fok;    // frame usable (non-degenerate / thick)
+    double* fscale;              // per-triangle target scale (batch mode)
+    // rhs gather: corners incident to each vertex
+    const int* inc_ptr;          // n_vert + 1
+    const int* inc_tri;
+    const double2* inc_g;        // b[v] += R(tri) g
+    // matrix
+    const int* sl_ptr;           // n_slices + 1
+    const int* m_col;
+    const float* m_hi;
+    const float* m_lo;
+    const float* m_b;            // LSCM boundary coupling B_ij (exact in fp32)
+    const double* diag;          // Laplacian diagonal without pin
+    const float* dinv32;         // 1 / (diag + pin) for the ARAP operator
+    // adjacency (unique neighbours) for Jacobi flip repair; boundary flags
+    const int* adj_ptr;
+    const int* adj_idx;
+    const unsigned char* vbnd;
+    // scratch (16 bytes per vertex each so FP32 and FP64 vectors fit)
+    double2* R;                  // per-triangle rotation (cos, sin) x scale
+    double2* b64;
+    double2* X64;                // CG iterate (warm start = U)
+    void* vr;
+    void* vp;
+    void* vAp;
+    void* ve;
+    double* red;                 // per-island scalars for multi-launch paths
+    IslandState* st;
+    int* queue;                  // task counters
+    const int* order_group;      // island ids, most expensive first
+    const int* order_wave;
+    int n_group, n_wave;
+};
+
+}  // namespace uva

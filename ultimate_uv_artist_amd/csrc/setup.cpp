@@ -1,0 +1,570 @@
+// Host construction of an island batch.  Follows the reference's frame and
+// Laplacian conventions:
+//   param mode: param.py:19-56 (local_frames, cot_weights, _lap_triplets),
+//               soft pin weight max|W| per island (param.py:393-396),
+//               LSCM boundary terms and pins (param.py:142-179)
+//   batch mode: gpu.py:88-142 (1e-30 clamps, thin-triangle mask, batch-global
+//               pin weight at every island's first vertex)
+#include "setup.hpp"
+
+#include "plan.hpp"
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <thread>
+
+namespace uva {
+namespace {
+
+struct Edge {
+    int64_t key;     // a * nv + b with a < b
+    double w;        // half-cotangent weight of one triangle edge
+    int dir;         // +1 for a->b as the triangle's half-edge, -1 for b->a
+};
+
+struct Island {
+    int nv = 0, nt = 0;
+    std::vector<double> x1, x2, y2, area, g;   // g: 6 per triangle (corner c: g[6f+2c], g[6f+2c+1])
+    std::vector<unsigned char> ok;
+    std::vector<double> wabs_max;              // one value: max |w|
+    std::vector<int> row_ptr, row_col;         // unique neighbours (local ids), sorted
+    std::vector<double> row_val, row_b;        // -sum w, LSCM coupling B_ij
+    std::vector<double> diag;
+    std::vector<unsigned char> bnd;            // endpoint of an edge used by one triangle
+    std::vector<int> inc_ptr, inc_tri, inc_corner;
+    int lp0 = -1, lp1 = -1;
+    double ldist = 0.0;
+    double wmax = 0.0;
+    int64_t nnz = 0;
+};
+
+inline void cross3(const double* a, const double* b, double* o) {
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+void build_island(Island& I, int nv, int nt, const double* P, const int64_t* T, const double* X,
+                  const double* area, const double* cots, bool batch, const int64_t* lpins) {
+    I.nv = nv;
+    I.nt = nt;
+    I.x1.resize(nt); I.x2.resize(nt); I.y2.resize(nt); I.area.resize(nt); I.ok.resize(nt);
+    I.g.assign(6 * (size_t)nt, 0.0);
+    std::vector<double> w(3 * (size_t)nt);
+    double wmax = 0.0;
+    for (int f = 0; f < nt; ++f) {
+        double Xf[3][2];
+        double ar;
+        double cf[3];
+        bool okf;
+        if (X && !batch) {
+            for (int c = 0; c < 3; ++c) { Xf[c][0] = X[6 * (size_t)f + 2 * c]; Xf[c][1] = X[6 * (size_t)f + 2 * c + 1]; }
+            ar = area[f];
+        } else {
+            const double* p0 = P + 3 * T[3 * (size_t)f];
+            const double* p1 = P + 3 * T[3 * (size_t)f + 1];
+            const double* p2 = P + 3 * T[3 * (size_t)f + 2];
+            double e1[3] = {p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2]};
+            double e2[3] = {p2[0] - p0[0], p2[1] - p0[1], p2[2] - p0[2]};
+            double l1 = std::sqrt(e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2]);
+            double nrm[3];
+            cross3(e1, e2, nrm);
+            double nn = std::sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
+            double tiny = batch ? 1e-30 : 1e-300;
+            if (batch) l1 = std::max(l1, 1e-30);
+            double lden = batch ? l1 : std::max(l1, 1e-300);
+            double ex[3] = {e1[0] / lden, e1[1] / lden, e1[2] / lden};
+            double nd = std::max(nn, tiny);
+            double nh[3] = {nrm[0] / nd, nrm[1] / nd, nrm[2] / nd};
+            double ey[3];
+            cross3(nh, ex, ey);
+            Xf[0][0] = 0.0; Xf[0][1] = 0.0;
+            Xf[1][0] = l1; Xf[1][1] = 0.0;
+            Xf[2][0] = e2[0] * ex[0] + e2[1] * ex[1] + e2[2] * ex[2];
+            Xf[2][1] = e2[0] * ey[0] + e2[1] * ey[1] + e2[2] * ey[2];
+            ar = 0.5 * nn;
+            if (batch) {
+                okf = (nn > 1e-14 * std::max(l1 * l1, 1e-300)) && std::isfinite(nn);
+                if (!okf) for (int c = 0; c < 3; ++c) Xf[c][0] = Xf[c][1] = 0.0;
+            }
+        }
+        // cotangent of the angle at corner k (opposite edge k+1 -> k+2)
+        for (int k = 0; k < 3; ++k) {
+            int i = (k + 1) % 3, j = (k + 2) % 3;
+            double ax = Xf[i][0] - Xf[k][0], ay = Xf[i][1] - Xf[k][1];
+            double bx = Xf[j][0] - Xf[k][0], by = Xf[j][1] - Xf[k][1];
+            double cr = std::fabs(ax * by - ay * bx);
+            double dt = ax * bx + ay * by;
+            double c = dt / std::max(cr, batch ? 1e-30 : 1e-300);
+            cf[k] = std::min(std::max(c, -1e4), 1e4);
+        }
+        if (cots && !batch) for (int k = 0; k < 3; ++k) cf[k] = cots[3 * (size_t)f + k];
+        if (batch) {
+            I.x1[f] = std::max(Xf[1][0], 1e-30);
+            I.x2[f] = Xf[2][0];
+            I.y2[f] = std::fabs(Xf[2][1]) > 1e-30 ? Xf[2][1] : 1e-30;
+            I.ok[f] = okf;
+            for (int k = 0; k < 3; ++k) w[3 * (size_t)f + k] = okf ? 0.5 * cf[k] : 0.0;
+        } else {
+            I.x1[f] = Xf[1][0];
+            I.x2[f] = Xf[2][0];
+            I.y2[f] = Xf[2][1];
+            I.ok[f] = std::fabs(Xf[1][0] * Xf[2][1]) >= 1e-300;
+            for (int k = 0; k < 3; ++k) w[3 * (size_t)f + k] = 0.5 * cf[k];
+        }
+        I.area[f] = ar;
+        for (int k = 0; k < 3; ++k) wmax = std::max(wmax, std::fabs(w[3 * (size_t)f + k]));
+        // rhs corner vectors: corner c gathers w_{c-1} d_{c-1} - w_{c+1} d_{c+1},
+        // d_k = X_{k+1} - X_{k+2}   (param.py:457-468 scatter, regrouped per corner)
+        double d[3][2];
+        for (int k = 0; k < 3; ++k) {
+            d[k][0] = Xf[(k + 1) % 3][0] - Xf[(k + 2) % 3][0];
+            d[k][1] = Xf[(k + 1) % 3][1] - Xf[(k + 2) % 3][1];
+        }
+        for (int c = 0; c < 3; ++c) {
+            int km = (c + 2) % 3, kp = (c + 1) % 3;
+            double wm = w[3 * (size_t)f + km], wp = w[3 * (size_t)f + kp];
+            I.g[6 * (size_t)f + 2 * c] = wm * d[km][0] - wp * d[kp][0];
+            I.g[6 * (size_t)f + 2 * c + 1] = wm * d[km][1] - wp * d[kp][1];
+        }
+    }
+    I.wmax = wmax;
+
+    // unique undirected edges with summed weights, use counts and boundary coupling
+    std::vector<Edge> E(3 * (size_t)nt);
+    std::vector<int64_t> dkeys(3 * (size_t)nt);
+    for (int f = 0; f < nt; ++f) {
+        for (int k = 0; k < 3; ++k) {
+            int64_t a = T[3 * (size_t)f + (k + 1) % 3], b = T[3 * (size_t)f + (k + 2) % 3];
+            Edge& e = E[3 * (size_t)f + k];
+            e.key = std::min(a, b) * nv + std::max(a, b);
+            e.w = w[3 * (size_t)f + k];
+            e.dir = a < b ? 1 : -1;
+            // directed half-edges of the triangle in corner order 0->1, 1->2, 2->0
+            dkeys[3 * (size_t)f + k] = T[3 * (size_t)f + k] * nv + T[3 * (size_t)f + (k + 1) % 3];
+        }
+    }
+    std::sort(dkeys.begin(), dkeys.end());
+    std::sort(E.begin(), E.end(), [](const Edge& x, const Edge& y) { return x.key < y.key; });
+    std::vector<int64_t> uk;
+    std::vector<double> uw, ub;
+    std::vector<int> ucnt;
+    std::vector<int> bsrc;       // sources of boundary half-edges
+    auto n_half = [&](int64_t a, int64_t b) {
+        auto rg = std::equal_range(dkeys.begin(), dkeys.end(), a * nv + b);
+        return (int)(rg.second - rg.first);
+    };
+    for (size_t q = 0; q < E.size();) {
+        size_t r = q;
+        double ws = 0.0;
+        while (r < E.size() && E[r].key == E[q].key) { ws += E[r].w; ++r; }
+        int64_t a = E[q].key / nv, b = E[q].key % nv;
+        // every boundary half-edge (its reverse absent) a->b adds B_ab -= 1/2,
+        // B_ba += 1/2 (param.py:142-150)
+        double beta = 0.0;
+        int nab = n_half(a, b), nba = n_half(b, a);
+        if (nab && !nba) { beta -= 0.5 * nab; bsrc.push_back((int)a); }
+        if (nba && !nab) { beta += 0.5 * nba; bsrc.push_back((int)b); }
+        uk.push_back(E[q].key);
+        uw.push_back(ws);
+        ub.push_back(beta);
+        ucnt.push_back((int)(r - q));
+        q = r;
+    }
+    // rows
+    std::vector<int> deg(nv, 0);
+    for (size_t u = 0; u < uk.size(); ++u) { deg[uk[u] / nv]++; deg[uk[u] % nv]++; }
+    I.row_ptr.assign(nv + 1, 0);
+    for (int v = 0; v < nv; ++v) I.row_ptr[v + 1] = I.row_ptr[v] + deg[v];
+    I.row_col.resize(I.row_ptr[nv]);
+    I.row_val.resize(I.row_ptr[nv]);
+    I.row_b.resize(I.row_ptr[nv]);
+    I.diag.assign(nv, 0.0);
+    I.bnd.assign(nv, 0);
+    std::vector<int> fill(I.row_ptr.begin(), I.row_ptr.end() - 1);
+    for (size_t u = 0; u < uk.size(); ++u) {
+        int a = (int)(uk[u] / nv), b = (int)(uk[u] % nv);
+        int pa = fill[a]++, pb = fill[b]++;
+        I.row_col[pa] = b; I.row_val[pa] = -uw[u]; I.row_b[pa] = ub[u];
+        I.row_col[pb] = a; I.row_val[pb] = -uw[u]; I.row_b[pb] = -ub[u];
+        I.diag[a] += uw[u];
+        I.diag[b] += uw[u];
+        if (ucnt[u] == 1) { I.bnd[a] = 1; I.bnd[b] = 1; }
+    }
+    I.nnz = I.row_ptr[nv];
+    // rows are sorted: keys (b, a) with b < a precede keys (a, c) with c > a
+    // corner incidence (vertex -> (triangle, corner)) in triangle order
+    I.inc_ptr.assign(nv + 1, 0);
+    for (int f = 0; f < nt; ++f)
+        for (int c = 0; c < 3; ++c) I.inc_ptr[T[3 * (size_t)f + c] + 1]++;
+    for (int v = 0; v < nv; ++v) I.inc_ptr[v + 1] += I.inc_ptr[v];
+    I.inc_tri.resize(3 * (size_t)nt);
+    I.inc_corner.resize(3 * (size_t)nt);
+    std::vector<int> ip(I.inc_ptr.begin(), I.inc_ptr.end() - 1);
+    for (int f = 0; f < nt; ++f)
+        for (int c = 0; c < 3; ++c) {
+            int v = (int)T[3 * (size_t)f + c];
+            I.inc_tri[ip[v]] = f;
+            I.inc_corner[ip[v]] = c;
+            ip[v]++;
+        }
+    // LSCM pins: farthest boundary pair (param.py:172-179) unless given
+    std::sort(bsrc.begin(), bsrc.end());
+    bsrc.erase(std::unique(bsrc.begin(), bsrc.end()), bsrc.end());
+    if (lpins) {
+        I.lp0 = (int)lpins[0];
+        I.lp1 = (int)lpins[1];
+    } else if (!bsrc.empty()) {
+        double c[3] = {0, 0, 0};
+        for (int v : bsrc) for (int k = 0; k < 3; ++k) c[k] += P[3 * (size_t)v + k];
+        for (int k = 0; k < 3; ++k) c[k] /= (double)bsrc.size();
+        auto far_from = [&](const double* o) {
+            int best = bsrc[0];
+            double bd = -1.0;
+            for (int v : bsrc) {
+                double dx = P[3 * (size_t)v] - o[0], dy = P[3 * (size_t)v + 1] - o[1], dz = P[3 * (size_t)v + 2] - o[2];
+                double d2 = dx * dx + dy * dy + dz * dz;
+                if (d2 > bd) { bd = d2; best = v; }
+            }
+            return best;
+        };
+        int i = far_from(c);
+        int j = far_from(P + 3 * (size_t)i);
+        if (i == j) j = bsrc[0] != i ? bsrc[0] : bsrc.back();
+        I.lp0 = i;
+        I.lp1 = j;
+    }
+    if (I.lp0 >= 0) {
+        const double* a = P + 3 * (size_t)I.lp0;
+        const double* b = P + 3 * (size_t)I.lp1;
+        I.ldist = std::sqrt((b[0] - a[0]) * (b[0] - a[0]) + (b[1] - a[1]) * (b[1] - a[1]) + (b[2] - a[2]) * (b[2] - a[2]));
+    }
+}
+
+template <class F>
+void parallel_for(int n, F fn) {
+    int nth = (int)std::min<unsigned>(std::max(1u, std::thread::hardware_concurrency()), 16u);
+    nth = std::min(nth, std::max(1, n));
+    std::atomic<int> next(0);
+    auto work = [&]() {
+        for (int i = next++; i < n; i = next++) fn(i);
+    };
+    if (nth <= 1) { work(); return; }
+    std::vector<std::thread> th;
+    for (int k = 0; k < nth; ++k) th.emplace_back(work);
+    for (auto& t : th) t.join();
+}
+
+}  // namespace
+
+std::string build_host_plan(HostPlan& hp, int64_t n_islands, const int64_t* vert_off, const int64_t* tri_off,
+                            const double* P, const int64_t* T, const int64_t* pins, const double* X,
+                            const double* area, const double* cots, bool batch, const int64_t* lscm_pins) {
+    if (n_islands <= 0) return "n_islands must be positive";
+    if (vert_off[0] != 0 || tri_off[0] != 0) return "offsets must start at 0";
+    for (int64_t i = 0; i < n_islands; ++i) {
+        if (vert_off[i + 1] < vert_off[i] || tri_off[i + 1] < tri_off[i]) return "offsets must be non-decreasing";
+        if (vert_off[i + 1] == vert_off[i]) return "island " + std::to_string(i) + " has no vertices";
+    }
+    int64_t nv_all = vert_off[n_islands], nt_all = tri_off[n_islands];
+    if (nv_all >= (int64_t(1) << 31) || 3 * nt_all >= (int64_t(1) << 31)) return "batch too large for 32-bit indices";
+    for (int64_t i = 0; i < n_islands; ++i) {
+        int64_t nv = vert_off[i + 1] - vert_off[i];
+        for (int64_t f = tri_off[i]; f < tri_off[i + 1]; ++f)
+            for (int c = 0; c < 3; ++c) {
+                int64_t v = T[3 * f + c];
+                if (v < 0 || v >= nv) return "triangle " + std::to_string(f) + " has a vertex outside its island";
+            }
+        if (pins && (pins[i] < 0 || pins[i] >= nv)) return "pin outside its island";
+        if (lscm_pins && (lscm_pins[2 * i] < 0 || lscm_pins[2 * i] >= nv || lscm_pins[2 * i + 1] < 0 ||
+                          lscm_pins[2 * i + 1] >= nv))
+            return "LSCM pin outside its island";
+    }
+    int n = (int)n_islands;
+    std::vector<Island> isl(n);
+    parallel_for(n, [&](int i) {
+        int64_t v0 = vert_off[i], t0 = tri_off[i];
+        build_island(isl[i], (int)(vert_off[i + 1] - v0), (int)(tri_off[i + 1] - t0), P + 3 * v0, T + 3 * t0,
+                     X ? X + 6 * t0 : nullptr, area ? area + t0 : nullptr, cots ? cots + 3 * t0 : nullptr, batch,
+                     lscm_pins ? lscm_pins + 2 * i : nullptr);
+    });
+    hp.n_isl = n;
+    hp.n_vert = (int)nv_all;
+    hp.n_tri = (int)nt_all;
+    hp.batch = batch;
+    double wglob = 0.0;
+    for (auto& I : isl) wglob = std::max(wglob, I.wmax);
+    wglob = std::max(wglob, 1e-12);
+    // island table and slice layout
+    hp.isl_v0.resize(n); hp.isl_nv.resize(n); hp.isl_t0.resize(n); hp.isl_nt.resize(n);
+    hp.isl_s0.resize(n); hp.isl_ns.resize(n); hp.isl_pin.resize(n); hp.isl_wpin.resize(n);
+    hp.isl_lpin0.resize(n); hp.isl_lpin1.resize(n); hp.isl_ldist.resize(n);
+    int64_t nslices = 0;
+    std::vector<int64_t> slice_start(n);
+    std::vector<std::vector<int>> widths(n);
+    int64_t nnz_pad = 0;
+    for (int i = 0; i < n; ++i) {
+        const Island& I = isl[i];
+        int ns = (I.nv + kSlice - 1) / kSlice;
+        widths[i].resize(ns);
+        for (int s = 0; s < ns; ++s) {
+            int wmx = 0;
+            for (int r = s * kSlice; r < std::min(I.nv, (s + 1) * kSlice); ++r)
+                wmx = std::max(wmx, I.row_ptr[r + 1] - I.row_ptr[r]);
+            widths[i][s] = wmx;
+            nnz_pad += (int64_t)wmx * kSlice;
+        }
+        slice_start[i] = nslices;
+        nslices += ns;
+        hp.nnz_true += I.nnz;
+    }
+    if (nnz_pad >= (int64_t(1) << 31)) return "batch too large for 32-bit matrix offsets";
+    hp.sl_ptr.assign(nslices + 1, 0);
+    hp.m_col.resize(nnz_pad); hp.m_hi.resize(nnz_pad); hp.m_lo.resize(nnz_pad); hp.m_b.resize(nnz_pad);
+    hp.diag.resize(nv_all); hp.dinv32.resize(nv_all); hp.vbnd.resize(nv_all);
+    hp.tv0.resize(nt_all); hp.tv1.resize(nt_all); hp.tv2.resize(nt_all);
+    hp.fx1.resize(nt_all); hp.fx2.resize(nt_all); hp.fy2.resize(nt_all); hp.farea.resize(nt_all); hp.fok.resize(nt_all);
+    hp.inc_ptr.assign(nv_all + 1, 0);
+    hp.inc_tri.resize(3 * nt_all);
+    hp.inc_g.resize(6 * nt_all);
+    hp.adj_ptr.assign(nv_all + 1, 0);
+    int64_t adj_total = 0;
+    for (auto& I : isl) adj_total += I.nnz;
+    hp.adj_idx.resize(adj_total);
+    {
+        int64_t s = 0, off = 0;
+        for (int i = 0; i < n; ++i)
+            for (int w : widths[i]) { hp.sl_ptr[s] = (int)off; off += (int64_t)w * kSlice; ++s; }
+        hp.sl_ptr[nslices] = (int)off;
+    }
+    // prefix of adjacency
+    {
+        int64_t a = 0;
+        for (int i = 0; i < n; ++i) {
+            const Island& I = isl[i];
+            for (int v = 0; v < I.nv; ++v) {
+                hp.adj_ptr[vert_off[i] + v] = (int)a;
+                a += I.row_ptr[v + 1] - I.row_ptr[v];
+            }
+        }
+        hp.adj_ptr[nv_all] = (int)a;
+    }
+    parallel_for(n, [&](int i) {
+        const Island& I = isl[i];
+        int v0 = (int)vert_off[i], t0 = (int)tri_off[i];
+        hp.isl_v0[i] = v0; hp.isl_nv[i] = I.nv; hp.isl_t0[i] = t0; hp.isl_nt[i] = I.nt;
+        hp.isl_s0[i] = (int)slice_start[i]; hp.isl_ns[i] = (int)widths[i].size();
+        int lp = pins ? (int)pins[i] : 0;
+        hp.isl_pin[i] = v0 + lp;
+        double wp = batch ? wglob : (I.nt ? I.wmax : 1.0);
+        hp.isl_wpin[i] = wp;
+        hp.isl_lpin0[i] = I.lp0 >= 0 ? v0 + I.lp0 : -1;
+        hp.isl_lpin1[i] = I.lp1 >= 0 ? v0 + I.lp1 : -1;
+        hp.isl_ldist[i] = I.ldist;
+        for (int f = 0; f < I.nt; ++f) {
+            hp.tv0[t0 + f] = v0 + (int)T[3 * ((size_t)t0 + f)];
+            hp.tv1[t0 + f] = v0 + (int)T[3 * ((size_t)t0 + f) + 1];
+            hp.tv2[t0 + f] = v0 + (int)T[3 * ((size_t)t0 + f) + 2];
+            hp.fx1[t0 + f] = I.x1[f]; hp.fx2[t0 + f] = I.x2[f]; hp.fy2[t0 + f] = I.y2[f];
+            hp.farea[t0 + f] = I.area[f]; hp.fok[t0 + f] = I.ok[f];
+        }
+        // incidences: global offsets are 3 * t0 + local offsets
+        for (int v = 0; v <= I.nv; ++v) hp.inc_ptr[v0 + v] = 3 * t0 + I.inc_ptr[v];
+        for (int q = 0; q < 3 * I.nt; ++q) {
+            int f = I.inc_tri[q], c = I.inc_corner[q];
+            hp.inc_tri[3 * (size_t)t0 + q] = t0 + f;
+            hp.inc_g[2 * (3 * (size_t)t0 + q)] = I.g[6 * (size_t)f + 2 * c];
+            hp.inc_g[2 * (3 * (size_t)t0 + q) + 1] = I.g[6 * (size_t)f + 2 * c + 1];
+        }
+        for (int v = 0; v < I.nv; ++v) {
+            double dg = I.diag[v];
+            hp.diag[v0 + v] = dg;
+            double dp = dg + (v == lp ? wp : 0.0);
+            hp.dinv32[v0 + v] = (float)(dp > 1e-30 ? 1.0 / dp : 1.0);
+            hp.vbnd[v0 + v] = I.bnd[v];
+            int a = hp.adj_ptr[v0 + v];
+            for (int q = I.row_ptr[v]; q < I.row_ptr[v + 1]; ++q) hp.adj_idx[a++] = v0 + I.row_col[q];
+        }
+        // sliced ELL
+        for (int s = 0; s < (int)widths[i].size(); ++s) {
+            int base = hp.sl_ptr[slice_start[i] + s];
+            int wdt = widths[i][s];
+            for (int l = 0; l < kSlice; ++l) {
+                int r = s * kSlice + l;
+                int cnt = r < I.nv ? I.row_ptr[r + 1] - I.row_ptr[r] : 0;
+                for (int k = 0; k < wdt; ++k) {
+                    size_t o = (size_t)base + (size_t)k * kSlice + l;
+                    if (k < cnt) {
+                        int q = I.row_ptr[r] + k;
+                        double v = I.row_val[q];
+                        float hi = (float)v;
+                        hp.m_col[o] = v0 + I.row_col[q];
+                        hp.m_hi[o] = hi;
+                        hp.m_lo[o] = (float)(v - (double)hi);
+                        hp.m_b[o] = (float)I.row_b[q];
+                    } else {
+                        hp.m_col[o] = v0 + std::min(r, I.nv - 1);
+                        hp.m_hi[o] = 0.0f; hp.m_lo[o] = 0.0f; hp.m_b[o] = 0.0f;
+                    }
+                }
+            }
+        }
+    });
+    // task order: most expensive first (cost ~ rows^1.5 for Jacobi-PCG on a 2-D Laplacian)
+    std::vector<std::pair<double, int>> g, wv;
+    for (int i = 0; i < n; ++i) {
+        double cost = std::pow((double)isl[i].nv, 1.5);
+        (isl[i].nv > kWaveMaxRows ? g : wv).push_back({-cost, i});
+    }
+    std::sort(g.begin(), g.end());
+    std::sort(wv.begin(), wv.end());
+    for (auto& p : g) hp.order_group.push_back(p.second);
+    for (auto& p : wv) hp.order_wave.push_back(p.second);
+    return "";
+}
+
+std::string build_matrix_plan(HostPlan& hp, int64_t n, const int64_t* indptr, const int64_t* indices,
+                              const double* data) {
+    if (n <= 0) return "matrix must have at least one row";
+    if (n >= (int64_t(1) << 31)) return "matrix too large for 32-bit indices";
+    if (indptr[0] != 0) return "indptr must start at 0";
+    for (int64_t i = 0; i < n; ++i)
+        if (indptr[i + 1] < indptr[i]) return "indptr must be non-decreasing";
+    for (int64_t q = 0; q < indptr[n]; ++q)
+        if (indices[q] < 0 || indices[q] >= n) return "column index out of range";
+    int nv = (int)n;
+    hp = HostPlan();
+    hp.n_isl = 1;
+    hp.n_vert = nv;
+    hp.n_tri = 0;
+    hp.isl_v0 = {0}; hp.isl_nv = {nv}; hp.isl_t0 = {0}; hp.isl_nt = {0}; hp.isl_s0 = {0};
+    int ns = (nv + kSlice - 1) / kSlice;
+    hp.isl_ns = {ns};
+    hp.isl_pin = {-1}; hp.isl_wpin = {0.0}; hp.isl_lpin0 = {-1}; hp.isl_lpin1 = {-1}; hp.isl_ldist = {0.0};
+    // rows: off-diagonal entries with duplicates summed, columns ascending
+    std::vector<std::vector<std::pair<int, double>>> rows(nv);
+    hp.diag.assign(nv, 0.0);
+    for (int i = 0; i < nv; ++i) {
+        auto& r = rows[i];
+        for (int64_t q = indptr[i]; q < indptr[i + 1]; ++q) {
+            int j = (int)indices[q];
+            if (j == i) hp.diag[i] += data[q];
+            else r.push_back({j, data[q]});
+        }
+        std::sort(r.begin(), r.end(), [](const std::pair<int, double>& a, const std::pair<int, double>& b) {
+            return a.first < b.first;
+        });
+        size_t w = 0;
+        for (size_t k = 0; k < r.size(); ++k) {
+            if (w && r[w - 1].first == r[k].first) r[w - 1].second += r[k].second;
+            else r[w++] = r[k];
+        }
+        r.resize(w);
+        hp.nnz_true += (int64_t)w;
+    }
+    hp.sl_ptr.assign(ns + 1, 0);
+    int64_t off = 0;
+    std::vector<int> widths(ns);
+    for (int s = 0; s < ns; ++s) {
+        int wmx = 0;
+        for (int r = s * kSlice; r < std::min(nv, (s + 1) * kSlice); ++r) wmx = std::max(wmx, (int)rows[r].size());
+        widths[s] = wmx;
+        hp.sl_ptr[s] = (int)off;
+        off += (int64_t)wmx * kSlice;
+    }
+    if (off >= (int64_t(1) << 31)) return "matrix too large for 32-bit offsets";
+    hp.sl_ptr[ns] = (int)off;
+    hp.m_col.resize(off); hp.m_hi.resize(off); hp.m_lo.resize(off); hp.m_b.assign(off, 0.0f);
+    for (int s = 0; s < ns; ++s)
+        for (int l = 0; l < kSlice; ++l) {
+            int r = s * kSlice + l;
+            for (int k = 0; k < widths[s]; ++k) {
+                size_t o = (size_t)hp.sl_ptr[s] + (size_t)k * kSlice + l;
+                if (r < nv && k < (int)rows[r].size()) {
+                    double v = rows[r][k].second;
+                    float hi = (float)v;
+                    hp.m_col[o] = rows[r][k].first;
+                    hp.m_hi[o] = hi;
+                    hp.m_lo[o] = (float)(v - (double)hi);
+                } else {
+                    hp.m_col[o] = std::min(r, nv - 1);
+                    hp.m_hi[o] = 0.0f;
+                    hp.m_lo[o] = 0.0f;
+                }
+            }
+        }
+    hp.dinv32.resize(nv);
+    hp.vbnd.assign(nv, 0);
+    hp.adj_ptr.assign(nv + 1, 0);
+    for (int i = 0; i < nv; ++i) {
+        double g = hp.diag[i];
+        hp.dinv32[i] = (float)(std::fabs(g) > 1e-300 ? 1.0 / g : 1.0);
+        hp.adj_ptr[i + 1] = hp.adj_ptr[i] + (int)rows[i].size();
+    }
+    hp.adj_idx.reserve(hp.adj_ptr[nv]);
+    for (int i = 0; i < nv; ++i)
+        for (auto& e : rows[i]) hp.adj_idx.push_back(e.first);
+    hp.inc_ptr.assign(nv + 1, 0);
+    (nv > kWaveMaxRows ? hp.order_group : hp.order_wave).push_back(0);
+    return "";
+}
+
+// param.py:478-516: move each non-boundary corner of a triangle with signed
+// area <= 0 to the mean of its unique 1-ring neighbours, in triangle order,
+// in place; stop after a round without such triangles.
+void repair_sequential(double* U, int64_t n, const int64_t* T, int64_t t, int rounds) {
+    std::vector<int64_t> keys;
+    keys.reserve(6 * (size_t)t);
+    for (int64_t f = 0; f < t; ++f)
+        for (int k = 0; k < 3; ++k) {
+            int64_t a = T[3 * f + k], b = T[3 * f + (k + 1) % 3];
+            keys.push_back(a * n + b);
+            keys.push_back(b * n + a);
+        }
+    std::sort(keys.begin(), keys.end());
+    keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+    std::vector<int64_t> ptr(n + 1, 0);
+    for (int64_t k : keys) ptr[k / n + 1]++;
+    for (int64_t v = 0; v < n; ++v) ptr[v + 1] += ptr[v];
+    // boundary: endpoints of directed half-edges whose reverse is absent
+    std::vector<int64_t> dk;
+    dk.reserve(3 * (size_t)t);
+    for (int64_t f = 0; f < t; ++f)
+        for (int k = 0; k < 3; ++k) dk.push_back(T[3 * f + k] * n + T[3 * f + (k + 1) % 3]);
+    std::vector<int64_t> sdk(dk);
+    std::sort(sdk.begin(), sdk.end());
+    std::vector<unsigned char> bnd(n, 0);
+    for (int64_t f = 0; f < t; ++f)
+        for (int k = 0; k < 3; ++k) {
+            int64_t a = T[3 * f + k], b = T[3 * f + (k + 1) % 3];
+            if (!std::binary_search(sdk.begin(), sdk.end(), b * n + a)) { bnd[a] = 1; bnd[b] = 1; }
+        }
+    for (int r = 0; r < rounds; ++r) {
+        int64_t bad = 0;
+        for (int64_t f = 0; f < t; ++f) {
+            int64_t a = T[3 * f], b = T[3 * f + 1], c = T[3 * f + 2];
+            double ar = (U[2 * b] - U[2 * a]) * (U[2 * c + 1] - U[2 * a + 1]) -
+                        (U[2 * c] - U[2 * a]) * (U[2 * b + 1] - U[2 * a + 1]);
+            if (ar > 0) continue;
+            ++bad;
+            for (int k = 0; k < 3; ++k) {
+                int64_t v = T[3 * f + k];
+                if (bnd[v]) continue;
+                double sx = 0.0, sy = 0.0;
+                int64_t cnt = 0;
+                for (int64_t q = ptr[v]; q < ptr[v + 1]; ++q) {
+                    int64_t w = keys[q] % n;
+                    sx += U[2 * w];
+                    sy += U[2 * w + 1];
+                    ++cnt;
+                }
+                if (cnt > 0) { U[2 * v] = sx / cnt; U[2 * v + 1] = sy / cnt; }
+            }
+        }
+        if (bad == 0) break;
+    }
+}
+
+}  // namespace uva

@@ -1,0 +1,43 @@
+// Host-side construction of an island batch (frames, Laplacian in sliced ELL,
+// rhs incidence, LSCM pins, task order).  Runs once per plan.
+#pragma once
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+namespace uva {
+
+struct HostPlan {
+    int n_isl = 0, n_vert = 0, n_tri = 0;
+    bool batch = false;
+    std::vector<int> isl_v0, isl_nv, isl_t0, isl_nt, isl_s0, isl_ns, isl_pin, isl_lpin0, isl_lpin1;
+    std::vector<double> isl_wpin, isl_ldist;
+    std::vector<int> tv0, tv1, tv2;
+    std::vector<double> fx1, fx2, fy2, farea;
+    std::vector<unsigned char> fok;
+    std::vector<int> inc_ptr, inc_tri;
+    std::vector<double> inc_g;          // 2 per incidence
+    std::vector<int> sl_ptr, m_col;
+    std::vector<float> m_hi, m_lo, m_b;
+    std::vector<double> diag;
+    std::vector<float> dinv32;
+    std::vector<int> adj_ptr, adj_idx;
+    std::vector<unsigned char> vbnd;
+    std::vector<int> order_group, order_wave;
+    int64_t nnz_true = 0;
+};
+
+// Returns an empty string on success, else an error message.
+std::string build_host_plan(HostPlan& hp, int64_t n_islands, const int64_t* vert_off, const int64_t* tri_off,
+                            const double* P, const int64_t* T, const int64_t* pins, const double* X,
+                            const double* area, const double* cots, bool batch, const int64_t* lscm_pins);
+
+// One island holding a general symmetric matrix (no triangles, no pin).
+std::string build_matrix_plan(HostPlan& hp, int64_t n, const int64_t* indptr, const int64_t* indices,
+                              const double* data);
+
+// param.repair_flips (sequential Gauss-Seidel order) on one chart.
+void repair_sequential(double* U, int64_t n, const int64_t* T, int64_t t, int rounds);
+
+}  // namespace uva

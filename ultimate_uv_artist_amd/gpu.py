@@ -1,0 +1,167 @@
+"""Drop-in replacement of ``engine/gpu.py``: batched parameterization of many
+charts on one MI355X.
+
+``Batch`` keeps the reference's constructor and methods (``orient_positive``,
+``repair_flips``, ``arap``, ``chart_uvs`` ... , gpu.py:66-299) but every
+method is one persistent HIP kernel over all charts: islands are pulled from
+a device work queue, each solved by a workgroup or a single wavefront with its
+own convergence state, instead of ~40 torch launches and a host sync per CG
+iteration.  The layout is kept in FP64 on the device; ``Batch.U`` returns it
+in the requested dtype.
+
+The small torch helpers (``csr32``, ``spmm``, ``device``, ``unique_edges``,
+``sym_csr``) keep their reference behaviour because ``hier.py`` imports them.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import _lib
+from .plan import CG_TOL, IslandPlan, require_gpu
+
+
+def csr32(A):
+    """CSR with int32 indices (gpu.py:16-20)."""
+    A = A.to_sparse_csr() if A.layout != torch.sparse_csr else A
+    return torch.sparse_csr_tensor(A.crow_indices().to(torch.int32), A.col_indices().to(torch.int32), A.values(),
+                                   A.shape)
+
+
+def spmm(L, X):
+    """L @ X one column at a time (gpu.py:23-28)."""
+    if X.dim() == 1:
+        return L @ X
+    return torch.stack([L @ X[:, k].contiguous() for k in range(X.shape[1])], 1)
+
+
+def device():
+    """The GPU this package runs on (gpu.py:30-36); the product path needs one."""
+    try:
+        import comfy.model_management as mm  # noqa: F401  (host application, when present)
+        d = mm.get_torch_device()
+        if d.type == "cuda":
+            return d
+    except Exception:
+        pass
+    return require_gpu()
+
+
+def unique_edges(T, n):
+    """Undirected edges (a < b), inverse map of the 3t triangle edges and use
+    counts (gpu.py:39-49)."""
+    a = torch.cat([T[:, 1], T[:, 2], T[:, 0]])
+    b = torch.cat([T[:, 2], T[:, 0], T[:, 1]])
+    key = torch.minimum(a, b) * n + torch.maximum(a, b)
+    uk, inv = torch.unique(key, return_inverse=True)
+    cnt = torch.bincount(inv, minlength=len(uk))
+    return uk // n, uk % n, inv, cnt
+
+
+def sym_csr(n, ea, eb, off, diag):
+    """Symmetric CSR (int32 indices) from unique edges and a diagonal (gpu.py:52-63)."""
+    dev = ea.device
+    ar = torch.arange(n, device=dev)
+    rows = torch.cat([ea, eb, ar])
+    cols = torch.cat([eb, ea, ar])
+    vals = torch.cat([off, off, diag])
+    order = torch.argsort(rows * n + cols)
+    rows, cols, vals = rows[order], cols[order], vals[order]
+    crow = torch.zeros(n + 1, dtype=torch.int64, device=dev)
+    crow[1:] = torch.cumsum(torch.bincount(rows, minlength=n), 0)
+    return torch.sparse_csr_tensor(crow.to(torch.int32), cols.to(torch.int32), vals, (n, n))
+
+
+class Batch:
+    """charts: list of (P (k,3), T (t,3), U (k,2)) (gpu.py:66-124)."""
+
+    def __init__(self, charts, dev=None, dtype=torch.float32):
+        self.dev = torch.device(dev) if dev is not None else device()
+        self.dt = dtype
+        charts = list(charts)
+        self.nc = len(charts)
+        self.offsets = np.cumsum([0] + [len(p) for p, _, _ in charts])
+        self.n = int(self.offsets[-1])
+        self._plan = IslandPlan([(p, t) for p, t, _ in charts], batch=True, dev=self.dev)
+        U = np.concatenate([np.asarray(u, np.float64).reshape(-1, 2) for _, _, u in charts])
+        self._U = torch.as_tensor(U, device=self.dev).contiguous()
+        toff = np.cumsum([0] + [len(t) for _, t, _ in charts])
+        self.T = torch.as_tensor(self._plan.T_local + np.repeat(self.offsets[:-1], np.diff(toff))[:, None],
+                                 device=self.dev)
+        self.cv = torch.as_tensor(np.repeat(np.arange(self.nc), np.diff(self.offsets)), device=self.dev)
+        self.ct = torch.as_tensor(np.repeat(np.arange(self.nc), np.diff(toff)), device=self.dev)
+        self.pins = torch.as_tensor(self.offsets[:-1], device=self.dev)
+        self._scale = torch.ones(len(self.T), device=self.dev, dtype=torch.float64)
+        self._frames = None
+
+    # ---------------------------------------------------------------- layout
+    @property
+    def U(self):
+        return self._U.to(self.dt)
+
+    @U.setter
+    def U(self, value):
+        self._U = value.to(self.dev, torch.float64).reshape(self.n, 2).contiguous()
+
+    @property
+    def scale(self):
+        return self._scale.to(self.dt)
+
+    @scale.setter
+    def scale(self, value):
+        self._scale = value.to(self.dev, torch.float64).contiguous()
+        self._plan.set_scale(self._scale)
+
+    # ---------------------------------------------------------------- pieces
+    def signed_area(self, U):
+        """Signed UV area per triangle (gpu.py:153-156)."""
+        T = self.T
+        a, b, c = U[T[:, 0]], U[T[:, 1]], U[T[:, 2]]
+        return 0.5 * ((b[:, 0] - a[:, 0]) * (c[:, 1] - a[:, 1]) - (c[:, 0] - a[:, 0]) * (b[:, 1] - a[:, 1]))
+
+    def flips_per_chart(self, U=None):
+        """Folded thick triangles per chart (gpu.py:158-163)."""
+        U = self._U if U is None else U.to(self.dev, torch.float64).contiguous()
+        return self._plan.flips_per_chart(U).to(self.dt)
+
+    # ---------------------------------------------------------------- solvers
+    def arap(self, iters=8, line_search=True, cg_iters=40, cg_tol=CG_TOL):
+        """ARAP iterations of every chart (gpu.py:230-249): rotations x scale,
+        rhs, CG to relative residual ``cg_tol`` per chart (at most ``cg_iters``
+        iterations per global step), revert of non-finite charts, per-chart
+        flip-safe step when ``line_search``."""
+        self._plan.arap(self._U, iters, line_search=line_search, mode=_lib.UVA_MODE_BATCH, cg_tol=cg_tol,
+                        cg_max_iters=cg_iters)
+        return self.U
+
+    def repair_flips(self, rounds=60):
+        """Jacobi 1-ring relaxation of the vertices of folded triangles (gpu.py:251-273)."""
+        self._plan.repair_jacobi(self._U, rounds)
+        return self.U
+
+    def orient_positive(self):
+        """Mirror charts whose area-weighted orientation is negative (gpu.py:275-281)."""
+        self._plan.orient_positive(self._U)
+
+    def shrink(self, tri_idx, factor=0.6, floor=0.05, rings=3):
+        """Lower the target size of the given triangles and their ``rings``
+        neighbourhood (gpu.py:283-295)."""
+        if len(tri_idx) == 0:
+            return
+        T = self.T
+        mark = torch.zeros(self.n, dtype=torch.bool, device=self.dev)
+        sel = torch.zeros(T.shape[0], dtype=torch.bool, device=self.dev)
+        sel[torch.as_tensor(tri_idx, device=self.dev)] = True
+        for _ in range(rings):
+            mark[T[sel].reshape(-1)] = True
+            sel = mark[T].any(1)
+        self.scale = torch.where(sel, (self._scale * factor).clamp_min(floor), self._scale)
+
+    def chart_uvs(self):
+        """Per-chart float64 numpy views of one host copy (gpu.py:297-299)."""
+        U = self._U.cpu().numpy()
+        return [U[a:b] for a, b in zip(self.offsets[:-1], self.offsets[1:])]
+
+    def stats(self):
+        """Per-chart solver statistics of the last call."""
+        return self._plan.stats()

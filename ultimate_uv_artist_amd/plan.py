@@ -1,0 +1,189 @@
+"""Host handle of a device plan: a batch of islands prepared for the kernels.
+
+``IslandPlan`` owns one ``uva_plan`` (frames, sliced-ELL Laplacian, rhs
+incidence, LSCM pins, task order, scratch) and runs LSCM / ARAP on device UV
+tensors (n x 2 float64, torch, on the plan's GPU).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import check, f64p, i64p, ptr
+
+CG_TOL = 1e-6        # relative residual of every ARAP global step
+LSCM_TOL = 1e-8      # relative residual of the LSCM solve (FP64)
+
+
+def require_gpu(dev=None):
+    """The product path runs on a ROCm GPU only; fail loudly without one."""
+    if not torch.cuda.is_available():
+        raise RuntimeError("ultimate_uv_artist_amd needs a ROCm GPU (torch.cuda.is_available() is False)")
+    d = torch.device(dev) if dev is not None else torch.device("cuda", torch.cuda.current_device())
+    if d.type != "cuda":
+        raise RuntimeError(f"ultimate_uv_artist_amd runs on a GPU device, got {d}")
+    return d
+
+
+def stream_of(dev):
+    return C.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+
+
+def concat_charts(charts):
+    """Concatenate [(P, T), ...] into offsets, P (n,3) f64 and island-local T (t,3) i64."""
+    vo = np.zeros(len(charts) + 1, np.int64)
+    to = np.zeros(len(charts) + 1, np.int64)
+    for i, (P, T) in enumerate(charts):
+        vo[i + 1] = vo[i] + len(P)
+        to[i + 1] = to[i] + len(T)
+    P = np.ascontiguousarray(np.concatenate([np.asarray(p, np.float64).reshape(-1, 3) for p, _ in charts]))
+    T = np.ascontiguousarray(np.concatenate([np.asarray(t, np.int64).reshape(-1, 3) for _, t in charts]))
+    return vo, to, P, T
+
+
+class IslandPlan:
+    """A batch of islands on one GPU.
+
+    charts  list of (P (k,3), T (t,3)) with island-local vertex ids
+    pins    soft-pin vertex (local id) per island, default 0
+    frames  optional (X (t,3,2), area (t,), cots (t,3)) concatenated over the
+            batch, used instead of recomputing them from P
+    batch   gpu.Batch conventions instead of param.refine conventions
+    """
+
+    def __init__(self, charts, pins=None, frames=None, batch=False, dev=None):
+        self.dev = require_gpu(dev)
+        L = _lib.lib()
+        self.vert_off, self.tri_off, P, T = concat_charts(charts)
+        self.n_islands = len(charts)
+        self.n = int(self.vert_off[-1])
+        self.t = int(self.tri_off[-1])
+        pins_a = None if pins is None else np.ascontiguousarray(np.asarray(pins, np.int64).reshape(-1))
+        X = area = cots = None
+        if frames is not None:
+            X = np.ascontiguousarray(np.asarray(frames[0], np.float64).reshape(-1, 3, 2))
+            area = np.ascontiguousarray(np.asarray(frames[1], np.float64).reshape(-1))
+            cots = np.ascontiguousarray(np.asarray(frames[2], np.float64).reshape(-1, 3))
+        self._h = C.c_void_p()
+        with torch.cuda.device(self.dev):
+            rc = L.uva_plan_create(C.byref(self._h), self.n_islands, i64p(self.vert_off), i64p(self.tri_off),
+                                   f64p(P), i64p(T), i64p(pins_a), f64p(X), f64p(area), f64p(cots),
+                                   _lib.UVA_PLAN_BATCH if batch else 0, stream_of(self.dev))
+        check(rc, "uva_plan_create")
+        self.T_local = T
+
+    # ------------------------------------------------------------- lifetime
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            _lib.lib().uva_plan_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        out = _lib.PlanInfo()
+        check(_lib.lib().uva_plan_get_info(self._h, C.byref(out)), "uva_plan_get_info")
+        return {k: getattr(out, k) for k, _ in _lib.PlanInfo._fields_}
+
+    def stats(self):
+        """Per-island statistics of the last call (synchronises the stream)."""
+        arr = (_lib.IslandStats * self.n_islands)()
+        check(_lib.lib().uva_plan_get_stats(self._h, arr, stream_of(self.dev)), "uva_plan_get_stats")
+        names = [k for k, _ in _lib.IslandStats._fields_ if k != "pad"]
+        return {k: np.array([getattr(s, k) for s in arr]) for k in names}
+
+    # ------------------------------------------------------------- helpers
+    def device_uv(self, U):
+        """(n,2) float64 contiguous tensor on the plan's GPU (a copy of numpy input)."""
+        if isinstance(U, torch.Tensor):
+            U = U.to(self.dev, torch.float64)
+        else:
+            U = torch.as_tensor(np.asarray(U, np.float64), device=self.dev)
+        U = U.reshape(self.n, 2).contiguous()
+        return U
+
+    def _check_uv(self, U):
+        if not (isinstance(U, torch.Tensor) and U.dtype == torch.float64 and U.is_contiguous()
+                and U.device == self.dev and tuple(U.shape) == (self.n, 2)):
+            raise ValueError(f"expected a contiguous ({self.n}, 2) float64 tensor on {self.dev}")
+
+    # ------------------------------------------------------------- solves
+    def lscm(self, U, pins=None, tol=LSCM_TOL, max_iters=0):
+        """LSCM of every island into U (device tensor, written in place)."""
+        self._check_uv(U)
+        p = None if pins is None else np.ascontiguousarray(np.asarray(pins, np.int64).reshape(-1))
+        check(_lib.lib().uva_lscm(self._h, ptr(U), i64p(p), float(tol), int(max_iters), stream_of(self.dev)),
+              "uva_lscm")
+        return U
+
+    def arap(self, U, iters, line_search=True, mode=_lib.UVA_MODE_REFINE, tol=1e-5, cg_tol=CG_TOL,
+             cg_max_iters=0, precision=_lib.UVA_PREC_MIXED, use_graph=False):
+        """ARAP iterations on U (device tensor) in place."""
+        self._check_uv(U)
+        o = _lib.ArapOpts(int(iters), int(bool(line_search)), int(mode), int(precision), float(tol), float(cg_tol),
+                          int(cg_max_iters), int(bool(use_graph)))
+        check(_lib.lib().uva_arap(self._h, ptr(U), C.byref(o), stream_of(self.dev)), "uva_arap")
+        return U
+
+    def local_rhs(self, U, mode=_lib.UVA_MODE_REFINE):
+        self._check_uv(U)
+        b = torch.empty_like(U)
+        check(_lib.lib().uva_local_rhs(self._h, ptr(U), ptr(b), int(mode), stream_of(self.dev)), "uva_local_rhs")
+        return b
+
+    def cg_solve(self, b, x, tol=CG_TOL, max_iters=0, precision=_lib.UVA_PREC_MIXED):
+        self._check_uv(b)
+        self._check_uv(x)
+        check(_lib.lib().uva_cg_solve(self._h, ptr(b), ptr(x), float(tol), int(max_iters), int(precision),
+                                      stream_of(self.dev)), "uva_cg_solve")
+        return x
+
+    def max_step(self, U, D, mode=_lib.UVA_MODE_REFINE):
+        self._check_uv(U)
+        self._check_uv(D)
+        out = torch.empty(self.n_islands, dtype=torch.float64, device=self.dev)
+        check(_lib.lib().uva_max_step(self._h, ptr(U), ptr(D), ptr(out), int(mode), stream_of(self.dev)),
+              "uva_max_step")
+        return out
+
+    def energy(self, U, kind=0):
+        self._check_uv(U)
+        out = torch.empty(self.n_islands, dtype=torch.float64, device=self.dev)
+        check(_lib.lib().uva_energy(self._h, ptr(U), int(kind), ptr(out), stream_of(self.dev)), "uva_energy")
+        return out
+
+    def signed_areas(self, U):
+        self._check_uv(U)
+        out = torch.empty(self.t, dtype=torch.float64, device=self.dev)
+        check(_lib.lib().uva_signed_areas(self._h, ptr(U), ptr(out), stream_of(self.dev)), "uva_signed_areas")
+        return out
+
+    def flips_per_chart(self, U):
+        self._check_uv(U)
+        out = torch.empty(self.n_islands, dtype=torch.float64, device=self.dev)
+        check(_lib.lib().uva_flips_per_chart(self._h, ptr(U), ptr(out), stream_of(self.dev)), "uva_flips_per_chart")
+        return out
+
+    def repair_jacobi(self, U, rounds):
+        self._check_uv(U)
+        check(_lib.lib().uva_repair_jacobi(self._h, ptr(U), int(rounds), stream_of(self.dev)), "uva_repair_jacobi")
+        return U
+
+    def orient_positive(self, U):
+        self._check_uv(U)
+        check(_lib.lib().uva_orient_positive(self._h, ptr(U), stream_of(self.dev)), "uva_orient_positive")
+        return U
+
+    def set_scale(self, scale):
+        s = scale.to(self.dev, torch.float64).contiguous()
+        if s.numel() != self.t:
+            raise ValueError("scale needs one value per triangle")
+        check(_lib.lib().uva_plan_set_scale(self._h, ptr(s), stream_of(self.dev)), "uva_plan_set_scale")
+        torch.cuda.current_stream(self.dev).synchronize()
