@@ -1,0 +1,124 @@
+"""Write the golden fixtures of the parameterization path from the reference itself.
+
+Run in the build container only (the reference tree is not available where
+the tests run on a GPU):   python tests/golden/make_golden.py /root/reference
+
+The reference's engine/param.py imports numba, which is not installed here;
+a stand-in module whose ``njit`` returns the function unchanged and whose
+``prange`` is ``range`` turns the JIT kernels into plain Python, which gives
+the same float64 arithmetic.  engine/gpu.py runs on CPU torch.
+
+Only behaviour that the oracle's own checks cannot pin is recorded:
+  * refine trajectories (per-iteration energies, the iteration at which the
+    energy early exit fires) on the 7,938-triangle grid island, from the
+    reference's LSCM (whose output is recorded too),
+  * the order-dependent sequential repair_flips on a folded chart,
+  * chart_quality scalars,
+  * gpu.Batch on three islands (orient, repair, free and flip-safe ARAP) with
+    a CG cap high enough to converge.
+Output: tests/golden/param_golden.npz and tests/golden/batch_golden.npz.
+"""
+from __future__ import annotations
+
+import importlib.util
+import sys
+import types
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+sys.path.insert(0, str(HERE.parent.parent))
+
+
+def load_reference(root):
+    nb = types.ModuleType("numba")
+
+    def njit(*args, **kw):
+        if args and callable(args[0]) and not kw:
+            return args[0]
+        return lambda f: f
+
+    nb.njit = njit
+    nb.prange = range
+    sys.modules["numba"] = nb
+    mods = {}
+    for name in ("param", "gpu"):
+        spec = importlib.util.spec_from_file_location(f"ref_engine_{name}", Path(root) / "engine" / f"{name}.py")
+        m = importlib.util.module_from_spec(spec)
+        spec.loader.exec_module(m)
+        mods[name] = m
+    return mods["param"], mods["gpu"]
+
+
+def main(root):
+    from ultimate_uv_artist_amd import workloads
+
+    rp, rg = load_reference(root)
+    out = {}
+    # grid island (7,938 triangles): LSCM, 20 ARAP iterations, early-exit trajectory
+    P, T = workloads.grid_island(seed=1)
+    X, area = rp.local_frames(P, T)
+    cots = rp.cot_weights(X)
+    U0 = rp.lscm(P, T, X, cots)
+    out["grid_P"], out["grid_T"], out["grid_lscm"] = P, T.astype(np.int32), U0
+    energies = []
+    real_energy = rp._energy
+
+    def rec(U, T_, X_, area_, kind):
+        e = real_energy(U, T_, X_, area_, kind)
+        energies.append(e)
+        return e
+
+    rp._energy = rec
+    U20, _ = rp.refine(P, T, U0, iters=20, X=X, area=area, cots=cots, tol=0.0)
+    out["grid_refine20"] = U20
+    out["grid_refine20_energy"] = np.array(energies)
+    energies.clear()
+    Ue, _ = rp.refine(P, T, U0, iters=50, X=X, area=area, cots=cots, tol=1e-3)
+    out["grid_refine_tol1e-3"] = Ue
+    out["grid_refine_tol1e-3_energy"] = np.array(energies)
+    rp._energy = real_energy
+    q = rp.chart_quality(P, T, U20, X, area)
+    for k in ("flips", "aniso_p95", "aniso_mean", "area_p95", "area_logdev", "stretch", "scale"):
+        out[f"grid_quality_{k}"] = np.array(q[k])
+    # folded chart through the sequential flip repair
+    rng = np.random.default_rng(200)
+    Pf, Tf = workloads.disk_island(rng, 200)
+    Uf = rp.lscm(Pf, Tf)
+    Uf = Uf + rng.normal(0, 0.04 * np.ptp(Uf), Uf.shape)
+    out["fold_P"], out["fold_T"], out["fold_U"] = Pf, Tf.astype(np.int32), Uf
+    out["fold_repaired"] = rp.repair_flips(Uf, Tf, rounds=60)
+    np.savez_compressed(HERE / "param_golden.npz", **out)
+
+    # gpu.Batch on CPU torch: three islands
+    import torch
+
+    rng = np.random.default_rng(300)
+    items = []
+    for t in (300, 700, 1200):
+        Pb, Tb = workloads.disk_island(rng, t)
+        Ub = rp.lscm(Pb, Tb)
+        Ub[:, 1] *= -1.0                                   # mirrored: orient_positive flips it back
+        Ub = Ub + rng.normal(0, 0.01 * np.ptp(Ub), Ub.shape)
+        items.append((Pb, Tb, Ub))
+    b = rg.Batch(items, dev=torch.device("cpu"), dtype=torch.float64)
+    b.orient_positive()
+    b.repair_flips(80)
+    u_rep = b.U.clone()
+    b.arap(3, line_search=False, cg_iters=4000)
+    u_free = b.U.clone()
+    b.arap(3, line_search=True, cg_iters=4000)
+    gold = {"n": np.array(len(items))}
+    for i, (Pb, Tb, Ub) in enumerate(items):
+        gold[f"P{i}"], gold[f"T{i}"], gold[f"U{i}"] = Pb, Tb.astype(np.int32), Ub
+    gold["after_repair"] = u_rep.numpy()
+    gold["after_free"] = u_free.numpy()
+    gold["after_ls"] = b.U.numpy()
+    np.savez_compressed(HERE / "batch_golden.npz", **gold)
+    for f in ("param_golden.npz", "batch_golden.npz"):
+        print(f, (HERE / f).stat().st_size, "bytes")
+
+
+if __name__ == "__main__":
+    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")

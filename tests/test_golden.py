@@ -1,0 +1,70 @@
+"""The oracle against golden vectors written by the reference itself
+(tests/golden/make_golden.py): LSCM, refine trajectories with and without the
+energy early exit, chart_quality, the sequential flip repair, and gpu.Batch."""
+from __future__ import annotations
+
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from oracle import param_cpu as pc
+from oracle.batch_cpu import BatchOracle
+
+GOLD = Path(__file__).resolve().parent / "golden"
+
+
+@pytest.fixture(scope="module")
+def g():
+    return dict(np.load(GOLD / "param_golden.npz"))
+
+
+def test_grid_lscm(g):
+    P, T = g["grid_P"], g["grid_T"].astype(np.int64)
+    assert np.abs(pc.lscm(P, T) - g["grid_lscm"]).max() < 1e-10
+
+
+def test_grid_refine_trajectory(g):
+    P, T = g["grid_P"], g["grid_T"].astype(np.int64)
+    trace = []
+    U, _ = pc.refine(P, T, g["grid_lscm"], iters=20, tol=0.0, trace=trace)
+    assert np.abs(U - g["grid_refine20"]).max() < 1e-10
+    e_ref = g["grid_refine20_energy"]            # energy of the start, then one per iteration
+    assert len(e_ref) == 21
+    assert np.allclose([t[0] for t in trace], e_ref[1:], rtol=1e-11)
+    assert pc.energy(g["grid_lscm"], T, *pc.local_frames(P, T), 0) == pytest.approx(e_ref[0], rel=1e-12)
+
+
+def test_grid_refine_early_exit(g):
+    P, T = g["grid_P"], g["grid_T"].astype(np.int64)
+    trace = []
+    U, _ = pc.refine(P, T, g["grid_lscm"], iters=50, tol=1e-3, trace=trace)
+    assert len(trace) == len(g["grid_refine_tol1e-3_energy"]) - 1
+    assert np.abs(U - g["grid_refine_tol1e-3"]).max() < 1e-10
+
+
+def test_grid_chart_quality(g):
+    P, T = g["grid_P"], g["grid_T"].astype(np.int64)
+    q = pc.chart_quality(P, T, g["grid_refine20"])
+    for k in ("flips", "aniso_p95", "aniso_mean", "area_p95", "area_logdev", "stretch", "scale"):
+        assert q[k] == pytest.approx(float(g[f"grid_quality_{k}"]), rel=1e-12), k
+
+
+def test_sequential_repair(g):
+    T = g["fold_T"].astype(np.int64)
+    assert (pc.signed_areas(g["fold_U"], T) <= 0).any()
+    assert np.array_equal(pc.repair_flips(g["fold_U"], T, 60), g["fold_repaired"])
+
+
+def test_batch_semantics():
+    b = dict(np.load(GOLD / "batch_golden.npz"))
+    items = [(b[f"P{i}"], b[f"T{i}"].astype(np.int64), b[f"U{i}"]) for i in range(int(b["n"]))]
+    r = BatchOracle(items)
+    r.orient_positive()
+    r.repair_flips(80)
+    assert np.abs(r.U - b["after_repair"]).max() < 1e-12
+    r.arap(3, line_search=False)
+    span = np.ptp(b["after_free"])
+    assert np.abs(r.U - b["after_free"]).max() < 1e-5 * span      # reference CG stops at 1e-6
+    r.arap(3, line_search=True)
+    assert np.abs(r.U - b["after_ls"]).max() < 1e-5 * span

@@ -1,0 +1,342 @@
+"""Pins of the CPU oracle (oracle/param_cpu.py, oracle/batch_cpu.py).
+
+Each check is chosen so that a plausible restatement mistake (a dropped
+factor, a wrong sign, a swapped index or corner) fails it: closed forms,
+identities that hold only for the correct formula (Dirichlet energy, energy
+gradients by finite differences), library routines (SVD polar factor, dense
+solves) and brute force on tiny inputs.  Trajectory-level behaviour that
+only the reference itself pins is covered by tests/test_golden.py.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spla
+
+from oracle import param_cpu as pc
+from oracle.batch_cpu import BatchOracle
+from ultimate_uv_artist_amd import workloads
+
+
+def planar_disk(seed=0, t=400):
+    P, T = workloads.disk_island(np.random.default_rng(seed), t)
+    P = P.copy()
+    P[:, 2] = 0.0
+    return P, T
+
+
+@pytest.fixture(scope="module")
+def bumpy():
+    return workloads.disk_island(np.random.default_rng(1), 600)
+
+
+# ------------------------------------------------------------------ geometry
+def test_local_frames_preserve_edges_and_area(bumpy):
+    P, T = bumpy
+    X, area = pc.local_frames(P, T)
+    for a, b in ((0, 1), (0, 2), (1, 2)):
+        l3 = np.linalg.norm(P[T[:, b]] - P[T[:, a]], axis=1)
+        l2 = np.linalg.norm(X[:, b] - X[:, a], axis=1)
+        assert np.allclose(l2, l3, rtol=1e-12)
+    cr = np.cross(P[T[:, 1]] - P[T[:, 0]], P[T[:, 2]] - P[T[:, 0]])
+    assert np.allclose(area, 0.5 * np.linalg.norm(cr, axis=1), rtol=1e-12)
+    assert np.allclose(area, 0.5 * X[:, 1, 0] * X[:, 2, 1], rtol=1e-10)
+    assert (X[:, 2, 1] > 0).all() and np.all(X[:, 0] == 0) and np.all(X[:, 1, 1] == 0)
+
+
+def test_cot_weights_are_cotangents_of_opposite_angles(bumpy):
+    P, T = bumpy
+    X, _ = pc.local_frames(P, T)
+    c = pc.cot_weights(X)
+    for k in range(3):
+        u = P[T[:, (k + 1) % 3]] - P[T[:, k]]
+        v = P[T[:, (k + 2) % 3]] - P[T[:, k]]
+        ang = np.arccos(np.clip((u * v).sum(1) / np.linalg.norm(u, axis=1) / np.linalg.norm(v, axis=1), -1, 1))
+        assert np.allclose(c[:, k], 1.0 / np.tan(ang), rtol=1e-7, atol=1e-9)
+
+
+def test_laplacian_is_the_dirichlet_energy(bumpy):
+    """x^T L x equals the integral of |grad x|^2 of the piecewise-linear x."""
+    P, T = bumpy
+    n = len(P)
+    X, area = pc.local_frames(P, T)
+    L = pc.cot_laplacian(n, T, pc.cot_weights(X))
+    assert abs(L - L.T).max() < 1e-12
+    assert np.abs(np.asarray(L.sum(1))).max() < 1e-9
+    rng = np.random.default_rng(0)
+    x = rng.normal(size=n)
+    x1, x2, y2 = X[:, 1, 0], X[:, 2, 0], X[:, 2, 1]
+    gx = (x[T[:, 1]] - x[T[:, 0]]) / x1
+    gy = ((x[T[:, 2]] - x[T[:, 0]]) - gx * x2) / y2
+    assert x @ (L @ x) == pytest.approx(float((area * (gx * gx + gy * gy)).sum()), rel=1e-10)
+
+
+def test_laplacian_linear_precision_on_planar_mesh():
+    P, T = planar_disk()
+    n = len(P)
+    X, _ = pc.local_frames(P, T)
+    L = pc.cot_laplacian(n, T, pc.cot_weights(X))
+    bi, bj = pc.boundary_edges_oriented(T)
+    interior = np.ones(n, bool)
+    interior[bi] = interior[bj] = False
+    f = 0.3 * P[:, 0] - 1.7 * P[:, 1] + 0.5
+    assert np.abs((L @ f)[interior]).max() < 1e-10
+
+
+def test_boundary_edges_and_loops_on_grid():
+    P, T = workloads.grid_island(nx=6)
+    bi, bj = pc.boundary_edges_oriented(T)
+    assert len(bi) == 4 * 5
+    loops = pc.boundary_loops(T)
+    assert len(loops) == 1 and len(loops[0]) == 20
+    lp = loops[0]
+    pairs = set(zip(bi.tolist(), bj.tolist()))
+    assert all((lp[i], lp[(i + 1) % len(lp)]) in pairs for i in range(len(lp)))
+
+
+def test_signed_areas_known_triangle():
+    U = np.array([[0.0, 0.0], [2.0, 0.0], [0.0, 3.0]])
+    assert pc.signed_areas(U, np.array([[0, 1, 2]]))[0] == 3.0
+    assert pc.signed_areas(U, np.array([[0, 2, 1]]))[0] == -3.0
+
+
+# ------------------------------------------------------------------ LSCM / solver
+def test_lscm_reproduces_a_planar_disk():
+    """A planar chart has zero conformal energy: LSCM returns it up to the
+    rigid motion fixed by the pins p0 -> (0,0), p1 -> (|p1-p0|, 0)."""
+    P, T = planar_disk(3, 500)
+    U = pc.lscm(P, T)
+    bi, _ = pc.boundary_edges_oriented(T)
+    p0, p1 = pc.far_pair(P, np.unique(bi))
+    d = P[p1, :2] - P[p0, :2]
+    c, s = d / np.linalg.norm(d)
+    R = np.array([[c, s], [-s, c]])
+    Uref = (P[:, :2] - P[p0, :2]) @ R.T
+    assert np.abs(U - Uref).max() < 1e-9
+
+
+def test_lscm_matches_dense_solve(bumpy):
+    P, T = bumpy
+    H, rhs, keep, pin_val, _ = pc.lscm_system(P, T)
+    x = np.linalg.solve(H.toarray(), rhs)
+    U = pc.lscm(P, T)
+    n = len(P)
+    full = pin_val.copy()
+    full[keep] = x
+    assert np.abs(U - np.stack([full[:n], full[n:]], 1)).max() < 1e-9
+    # H is the conformal energy: symmetric positive definite after pinning
+    Hd = H.toarray()
+    assert np.abs(Hd - Hd.T).max() < 1e-12
+    assert np.linalg.eigvalsh(Hd).min() > 0
+
+
+def test_lscm_raises_on_closed_chart():
+    P = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1.0]])
+    T = np.array([[0, 2, 1], [0, 1, 3], [0, 3, 2], [1, 2, 3]])
+    with pytest.raises(ValueError):
+        pc.lscm(P, T)
+
+
+def test_spdsolver_direct_and_cg_agree(bumpy):
+    P, T = bumpy
+    n = len(P)
+    X, _ = pc.local_frames(P, T)
+    L = pc.cot_laplacian(n, T, pc.cot_weights(X)).tolil()
+    L[0, 0] += 1.0
+    L = L.tocsc()
+    b = np.random.default_rng(2).normal(size=(n, 2))
+    s = pc.SPDSolver(L)
+    x = s.solve(b)
+    assert np.abs(L @ x - b).max() < 1e-9
+    old = pc.DIRECT_LIMIT
+    try:
+        pc.DIRECT_LIMIT = 0
+        xc = pc.SPDSolver(L).solve(b, tol=1e-10, maxiter=20000)
+    finally:
+        pc.DIRECT_LIMIT = old
+    assert np.abs(xc - x).max() < 1e-6 * np.abs(x).max()
+    assert np.abs(spla.spsolve(L, b) - x).max() < 1e-9 * np.abs(x).max()
+
+
+# ------------------------------------------------------------------ local step pieces
+def _rand_J(rng, m):
+    J = rng.normal(size=(m, 2, 2))
+    neg = np.linalg.det(J) < 0
+    J[neg, :, 0] *= -1
+    return J
+
+
+def _layout_with_jacobians(J):
+    """One triangle per Jacobian with frame x1=1, x2=0.3, y2=0.8 and U = J X."""
+    m = len(J)
+    X = np.zeros((m, 3, 2))
+    X[:, 1, 0] = 1.0
+    X[:, 2] = [0.3, 0.8]
+    U = np.einsum("tij,tkj->tki", J, X).reshape(-1, 2)
+    T = np.arange(3 * m).reshape(m, 3)
+    return U, T, X
+
+
+def test_rotation_is_the_polar_factor():
+    rng = np.random.default_rng(4)
+    J = _rand_J(rng, 200)
+    U, T, X = _layout_with_jacobians(J)
+    R, Wm = pc.arap_rotations(U, T, X, None, 0)
+    Uu, _, Vt = np.linalg.svd(J)
+    assert np.abs(R - Uu @ Vt).max() < 1e-10
+    assert np.all(Wm == np.eye(2))
+    th = rng.uniform(0, 2 * np.pi, 20)
+    Jr = np.stack([np.stack([np.cos(th), -np.sin(th)], 1), np.stack([np.sin(th), np.cos(th)], 1)], 1)
+    U, T, X = _layout_with_jacobians(Jr)
+    assert np.abs(pc.arap_rotations(U, T, X, None, 0)[0] - Jr).max() < 1e-12
+
+
+def test_slim_weights_from_singular_values():
+    rng = np.random.default_rng(5)
+    J = _rand_J(rng, 100)
+    U, T, X = _layout_with_jacobians(J)
+    R, Wm = pc.arap_rotations(U, T, X, None, 1)
+    Uu, S, Vt = np.linalg.svd(J)
+    assert np.abs(R - Uu @ Vt).max() < 1e-10
+    w = np.sqrt((S - S ** -3) / (S - 1))
+    # W has the left singular vectors as eigenvectors and w(s) as eigenvalues
+    for f in range(len(J)):
+        for k in range(2):
+            assert np.allclose(Wm[f] @ Uu[f][:, k], w[f, k] * Uu[f][:, k], rtol=1e-8, atol=1e-10)
+
+
+def test_rhs_is_the_negative_energy_gradient(bumpy):
+    """With R fixed, E(U) = 1/2 sum w_k |u_i - u_j - R d_k|^2 has gradient L U - b."""
+    P, T = bumpy
+    n = len(P)
+    X, _ = pc.local_frames(P, T)
+    cots = pc.cot_weights(X)
+    rng = np.random.default_rng(6)
+    U = pc.lscm(P, T) + rng.normal(0, 0.01, (n, 2))
+    R, _ = pc.arap_rotations(U, T, X, cots, 0)
+    b = pc.arap_rhs(n, T, X, cots, R)
+    L = pc.cot_laplacian(n, T, cots)
+
+    def E(V):
+        e = 0.0
+        for k in range(3):
+            i, j = (k + 1) % 3, (k + 2) % 3
+            r = V[T[:, i]] - V[T[:, j]] - np.einsum("tab,tb->ta", R, X[:, i] - X[:, j])
+            e += 0.5 * (0.5 * cots[:, k] * (r * r).sum(1)).sum()
+        return e
+
+    g = L @ U - b
+    for v, c in [(3, 0), (17, 1), (int(T[5, 2]), 0)]:
+        h = 1e-6
+        Up, Um = U.copy(), U.copy()
+        Up[v, c] += h
+        Um[v, c] -= h
+        assert (E(Up) - E(Um)) / (2 * h) == pytest.approx(g[v, c], rel=1e-5, abs=1e-8)
+
+
+def test_max_step_brute_force():
+    rng = np.random.default_rng(8)
+    P, T = planar_disk(8, 150)
+    U = P[:, :2].copy()
+    for trial in range(5):
+        D = rng.normal(0, 0.05, U.shape)
+        tm = pc.max_step(U, D, T)
+        ts = np.linspace(0, min(tm, 50.0) * 1.05, 20001)
+        first = None
+        for t in ts:
+            if (pc.signed_areas(U + t * D, T) <= 0).any():
+                first = t
+                break
+        if tm < 50.0:
+            assert first is not None and abs(first - tm) <= (ts[1] - ts[0]) * 1.01
+        else:
+            assert first is None
+    assert pc.max_step(U, np.ones_like(U), T) == 1e30                   # translation never flips
+    c = U.mean(0)
+    assert pc.max_step(U, -(U - c), T) == pytest.approx(1.0, rel=1e-6)  # double root at 1
+
+
+def test_energies_at_isometry_and_flip():
+    P, T = planar_disk(9, 200)
+    X, area = pc.local_frames(P, T)
+    U = P[:, :2]
+    assert pc.energy(U, T, X, area, 0) < 1e-20
+    assert pc.energy(U, T, X, area, 1) < 1e-10
+    assert pc.energy(U * 2.0, T, X, area, 0) == pytest.approx(2 * area.sum(), rel=1e-12)   # s1=s2=2
+    Uf = U.copy()
+    Uf[:, 1] *= -1
+    assert pc.energy(Uf, T, X, area, 1) == np.inf
+
+
+# ------------------------------------------------------------------ refine / repair / quality
+def test_refine_isometry_is_a_fixed_point_and_pin_holds():
+    P, T = planar_disk(10, 300)
+    U0 = P[:, :2] + 0.5
+    U, _ = pc.refine(P, T, U0, iters=3, tol=0.0, pin=7)
+    assert np.abs(U - U0).max() < 1e-10
+
+
+def test_refine_energy_decreases_and_stays_flip_free(bumpy):
+    P, T = bumpy
+    U0 = pc.lscm(P, T)
+    trace = []
+    U, _ = pc.refine(P, T, U0, iters=15, tol=0.0, trace=trace)
+    e = [t[0] for t in trace]
+    assert all(e[i + 1] <= e[i] * (1 + 1e-12) for i in range(len(e) - 1))
+    assert (pc.signed_areas(U, T) > 0).all()
+    trace2 = []
+    pc.refine(P, T, U0, iters=50, tol=1e-3, trace=trace2)
+    assert len(trace2) < 50
+    assert abs(trace2[-1][0] - trace2[-2][0]) <= 1e-3 * trace2[-2][0]
+
+
+def test_repair_moves_interior_vertex_to_ring_mean():
+    # a fan around vertex 0 with vertex 0 pushed outside: one pass fixes it
+    ang = np.linspace(0, 2 * np.pi, 7)[:-1]
+    P = np.concatenate([[[0.0, 0.0, 0.0]], np.stack([np.cos(ang), np.sin(ang), 0 * ang], 1)])
+    T = np.array([[0, 1 + k, 1 + (k + 1) % 6] for k in range(6)])
+    U = P[:, :2].copy()
+    U[0] = [1.5, 0.2]
+    assert (pc.signed_areas(U, T) <= 0).any()
+    out = pc.repair_flips(U, T, rounds=5)
+    assert np.allclose(out[0], U[1:].mean(0))
+    assert (pc.signed_areas(out, T) > 0).all()
+    assert np.array_equal(out[1:], U[1:])
+
+
+def test_weighted_quantile_and_quality_at_isometry():
+    x = np.array([5.0, 1.0, 3.0, 2.0])
+    w = np.array([0.1, 0.4, 0.3, 0.2])
+    assert pc.weighted_quantile(x, w, 0.5) == 2.0
+    assert pc.weighted_quantile(x, w, 0.95) == 5.0
+    assert pc.weighted_quantile(x, w, 0.0) == 1.0
+    P, T = planar_disk(11, 200)
+    q = pc.chart_quality(P, T, P[:, :2] * 3.0)
+    assert q["flips"] == 0
+    assert q["aniso_p95"] == pytest.approx(1.0, abs=1e-9)
+    assert q["area_p95"] == pytest.approx(1.0, abs=1e-9)
+    assert q["scale"] == pytest.approx(3.0, rel=1e-12)
+
+
+# ------------------------------------------------------------------ Batch semantics
+def test_batch_oracle_equals_refine_semantics_on_one_chart(bumpy):
+    """With exact solves, a single-chart Batch.arap with line search is
+    param.refine without early exit (the batch-global pin weight changes the
+    matrix, not the solution: the ARAP rhs sums to zero)."""
+    P, T = bumpy
+    U0 = pc.lscm(P, T)
+    b = BatchOracle([(P, T, U0)])
+    b.arap(5, line_search=True)
+    U, _ = pc.refine(P, T, U0, iters=5, tol=-1.0)
+    assert np.abs(b.U - U).max() < 1e-9 * np.ptp(U)
+
+
+def test_batch_orient_and_flip_count():
+    P, T = planar_disk(12, 200)
+    U = P[:, :2].copy()
+    U[:, 1] *= -1
+    b = BatchOracle([(P, T, U)])
+    assert b.flips_per_chart()[0] == len(T)
+    b.orient_positive()
+    assert b.flips_per_chart()[0] == 0
+    assert np.allclose(b.U, P[:, :2])
