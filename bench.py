@@ -42,6 +42,8 @@ def parse():
     ap.add_argument("--precision", default="mixed", choices=["mixed", "f32", "f64"])
     ap.add_argument("--cpu-sample", type=int, default=16, help="islands in the CPU baseline sample (0 = skip)")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-plain", dest="plain", action="store_false",
+                    help="skip the plain per-island GPU baseline")
     return ap.parse_args()
 
 
@@ -135,6 +137,36 @@ def main():
           f"step mean {kpi.mean():.1f} p95 {np.percentile(kpi, 95):.1f}; converged {int(st['converged'].sum())}/"
           f"{len(charts)}; max rel resid {st['rel_resid'].max():.2e}", flush=True)
 
+    plain = None
+    if args.plain:
+        # the same kernels, one island per launch sequence, islands one after another
+        plans = [IslandPlan([c], dev=dev) for c in charts]
+        U0h = U0.cpu().numpy()
+        vo = plan.vert_off
+        Us0 = [torch.as_tensor(U0h[vo[i]:vo[i + 1]], device=dev).contiguous() for i in range(len(charts))]
+        Us = [u.clone() for u in Us0]
+
+        def plain_step():
+            for p, u, u0 in zip(plans, Us, Us0):
+                u.copy_(u0)
+                p.arap(u, args.iters, line_search=True, mode=_lib.UVA_MODE_REFINE, tol=0.0, cg_tol=args.cg_tol,
+                       precision=prec, use_graph=not args.no_graph)
+
+        plain_step()
+        torch.cuda.synchronize()
+        ev0.record(stream)
+        plain_step()
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        ms_plain = ev0.elapsed_time(ev1)
+        same = all(np.array_equal(u.cpu().numpy(), U[vo[i]:vo[i + 1]].cpu().numpy()) for i, u in enumerate(Us))
+        plain = {"value": round(float(tri.sum()) * args.iters / (ms_plain * 1e-3) / 1e6, 3),
+                 "ms_per_step": round(ms_plain, 3), "speedup_batched": round(ms_plain / ms, 2),
+                 "identical_layouts": bool(same)}
+        print(f"bench: plain per-island {plain['value']} Mtri*iter/s ({ms_plain:.1f} ms), batched is "
+              f"{plain['speedup_batched']}x faster, identical layouts: {same}", flush=True)
+        del plans
+
     cpu = None
     parity = None
     if args.cpu_sample > 0:
@@ -185,6 +217,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None},
         "cpu_baseline": cpu,
+        "plain_per_island": plain,
         "cg_iters_per_step": {"mean": round(float(kpi.mean()), 2), "p95": round(float(np.percentile(kpi, 95)), 2),
                               "max": int(st["cg_iters_max"].max())},
         "converged_islands": int(st["converged"].sum()),

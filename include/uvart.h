@@ -48,6 +48,8 @@ extern "C" {
 /* plan flags */
 #define UVA_PLAN_BATCH 1u   /* gpu.Batch conventions: thin-triangle mask,
                                1e-30 clamps, batch-global pin weight */
+#define UVA_PLAN_NO_WAVE 2u /* solve every island with a whole workgroup
+                               (no single-wavefront tasks) */
 
 /* ARAP semantics */
 #define UVA_MODE_REFINE 0   /* param.refine: per-island pin weight, energy early

@@ -21,6 +21,7 @@ UVA_ENOMEM = -3
 UVA_ENOBOUNDARY = -5
 
 UVA_PLAN_BATCH = 1
+UVA_PLAN_NO_WAVE = 2
 UVA_MODE_REFINE = 0
 UVA_MODE_BATCH = 1
 UVA_PREC_MIXED = 0

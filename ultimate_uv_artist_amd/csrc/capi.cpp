@@ -251,7 +251,8 @@ int uva_plan_create(uva_plan** out, int64_t n_islands, const int64_t* vert_off, 
     std::string err;
     try {
         err = uva::build_host_plan(hp, n_islands, vert_off, tri_off, P, T, pins, X, area, cots,
-                                   (flags & UVA_PLAN_BATCH) != 0, nullptr);
+                                   (flags & UVA_PLAN_BATCH) != 0, nullptr,
+                                   (flags & UVA_PLAN_NO_WAVE) ? -1 : uva::kWaveMaxRows);
     } catch (const std::bad_alloc&) {
         return fail(UVA_ENOMEM, "host allocation failed while building the plan");
     }

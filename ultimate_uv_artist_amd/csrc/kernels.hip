@@ -16,6 +16,26 @@
 
 namespace uva {
 
+// ---------------------------------------------------------------- uniform values
+// Every value that steers control flow (task ids, reduction results, island
+// flags) goes through readfirstlane: it lands in a scalar register, so the
+// compiler emits scalar branches and never splits a wavefront's lanes over
+// different paths of a team loop.  (A task id broadcast with a lane shuffle
+// is a vector value to the compiler; it restructured the task loop so lanes
+// 1-63 re-entered it without lane 0's fresh task id.)
+__device__ __forceinline__ int uniform_i(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+__device__ __forceinline__ double uniform_d(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffffLL));
+    const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+__device__ __forceinline__ double4 uniform_d4(double4 v) {
+    return make_double4(uniform_d(v.x), uniform_d(v.y), uniform_d(v.z), uniform_d(v.w));
+}
+
 // ---------------------------------------------------------------- reductions
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -44,12 +64,12 @@ struct WaveTeam {
     __device__ double4 sum4(double4 v) const {
         v.x = wave_sum(v.x); v.y = wave_sum(v.y); v.z = wave_sum(v.z); v.w = wave_sum(v.w);
         sync();
-        return v;
+        return uniform_d4(v);
     }
     __device__ double min1(double v) const {
         v = wave_min(v);
         sync();
-        return v;
+        return uniform_d(v);
     }
 };
 
@@ -78,7 +98,7 @@ struct BlockTeam {
         double4 s = make_double4(0.0, 0.0, 0.0, 0.0);
 #pragma unroll
         for (int w = 0; w < kWaves; ++w) { s.x += buf[4 * w]; s.y += buf[4 * w + 1]; s.z += buf[4 * w + 2]; s.w += buf[4 * w + 3]; }
-        return s;
+        return uniform_d4(s);
     }
     __device__ double min1(double v) {
         v = wave_min(v);
@@ -89,7 +109,7 @@ struct BlockTeam {
         double m = buf[0];
 #pragma unroll
         for (int w = 1; w < kWaves; ++w) m = fmin(m, buf[4 * w]);
-        return m;
+        return uniform_d(m);
     }
 };
 
@@ -101,8 +121,10 @@ struct Isl {
 
 __device__ __forceinline__ Isl load_isl(const DevPlan& P, int i) {
     Isl d;
-    d.v0 = P.isl_v0[i]; d.nv = P.isl_nv[i]; d.t0 = P.isl_t0[i]; d.nt = P.isl_nt[i];
-    d.s0 = P.isl_s0[i]; d.ns = P.isl_ns[i]; d.pin = P.isl_pin[i]; d.wpin = P.isl_wpin[i];
+    d.v0 = uniform_i(P.isl_v0[i]); d.nv = uniform_i(P.isl_nv[i]);
+    d.t0 = uniform_i(P.isl_t0[i]); d.nt = uniform_i(P.isl_nt[i]);
+    d.s0 = uniform_i(P.isl_s0[i]); d.ns = uniform_i(P.isl_ns[i]);
+    d.pin = uniform_i(P.isl_pin[i]); d.wpin = uniform_d(P.isl_wpin[i]);
     return d;
 }
 
@@ -123,6 +145,8 @@ struct Vec2<double> { using type = double2; };
 // ---------------------------------------------------------------- sparse rows
 enum { OP_ARAP = 0, OP_LSCM = 1 };
 
+constexpr int kGather = 8;   // matrix entries whose gathers are issued together
+
 // y = A x for one row: FP32 values (hi) for float vectors, hi + lo for double.
 template <typename V>
 __device__ __forceinline__ V mval(const DevPlan& P, int o);
@@ -141,17 +165,30 @@ __device__ __forceinline__ typename Vec2<V>::type spmv_row(const DevPlan& P, int
     typename Vec2<V>::type xr = x[row];
     V yx = dg * xr.x, yy = dg * xr.y;
     int o = base + lane;
-    for (int k = 0; k < width; ++k, o += kSlice) {
-        const int c = P.m_col[o];
-        const V v = mval<V>(P, o);
-        const typename Vec2<V>::type xc = x[c];
-        if (OP == OP_ARAP) {
-            yx = fma(v, xc.x, yx);
-            yy = fma(v, xc.y, yy);
-        } else {
-            const V bv = (V)P.m_b[o];
-            yx = fma(v, xc.x, fma(bv, xc.y, yx));
-            yy = fma(v, xc.y, -fma(bv, xc.x, -yy));
+    // Entries go in groups of kGather: all column indices and values of a
+    // group are loaded first, then all gathers are issued, so a row costs two
+    // dependent memory round trips per group instead of two per entry.
+    for (int k0 = 0; k0 < width; k0 += kGather, o += kGather * kSlice) {
+        int c[kGather];
+        V v[kGather];
+        V bv[kGather];
+#pragma unroll
+        for (int u = 0; u < kGather; ++u) {
+            const bool in = k0 + u < width;
+            c[u] = in ? P.m_col[o + u * kSlice] : row;
+            v[u] = in ? mval<V>(P, o + u * kSlice) : (V)0;
+            bv[u] = (OP == OP_LSCM && in) ? (V)P.m_b[o + u * kSlice] : (V)0;
+        }
+#pragma unroll
+        for (int u = 0; u < kGather; ++u) {
+            const typename Vec2<V>::type xc = x[c[u]];
+            if (OP == OP_ARAP) {
+                yx = fma(v[u], xc.x, yx);
+                yy = fma(v[u], xc.y, yy);
+            } else {
+                yx = fma(v[u], xc.x, fma(bv[u], xc.y, yx));
+                yy = fma(v[u], xc.y, -fma(bv[u], xc.x, -yy));
+            }
         }
     }
     typename Vec2<V>::type y;
@@ -542,7 +579,7 @@ __device__ void run_task(Team& tm, const DevPlan& P, int isl, const TaskArgs& a)
             st->status = 0;
         }
     } else if (KIND == TASK_ARAP_ITER) {
-        if (st->done) return;
+        if (uniform_i(st->done)) return;
         const double2 b2 = rhs_step(tm, P, d, U);
         const CGOut cg = cg_island<PREC, OP_ARAP>(tm, P, d, b2, a.cg_tol, a.cg_max, -1, -1);
         if (a.batch) {
@@ -561,7 +598,7 @@ __device__ void run_task(Team& tm, const DevPlan& P, int isl, const TaskArgs& a)
                 tm.sync();
             }
         }
-        const int use_ls = st->use_ls;
+        const int use_ls = uniform_i(st->use_ls);
         double step = 1.0, tmax = 1e30;
         if (use_ls) {
             tmax = max_step_island(tm, P, d, U, P.X64, a.batch);
@@ -594,12 +631,12 @@ __device__ void run_task(Team& tm, const DevPlan& P, int isl, const TaskArgs& a)
             }
         }
     } else if (KIND == TASK_LSCM) {
-        const int p0 = P.isl_lpin0[isl], p1 = P.isl_lpin1[isl];
+        const int p0 = uniform_i(P.isl_lpin0[isl]), p1 = uniform_i(P.isl_lpin1[isl]);
         if (p0 < 0) {
             if (tm.rank() == 0) st->status = -5;
             return;
         }
-        const double dist = P.isl_ldist[isl];
+        const double dist = uniform_d(P.isl_ldist[isl]);
         // rhs = -H x_pin on the free rows (param.py:153-165)
         FOR_ROWS(tm, d, row, sg) {
             (void)sg;
@@ -765,19 +802,21 @@ __global__ __launch_bounds__(BS) void island_kernel(DevPlan P, TaskArgs a, int s
     for (;;) {
         if (threadIdx.x == 0) s_task = atomicAdd(P.queue + 2 * slot, 1);
         __syncthreads();
-        const int task = s_task;
+        const int task = uniform_i(s_task);
         __syncthreads();
         if (task >= P.n_group) break;
-        run_task<KIND, PREC>(bt, P, P.order_group[task], a);
+        run_task<KIND, PREC>(bt, P, uniform_i(P.order_group[task]), a);
         __syncthreads();
     }
     WaveTeam wt;
     for (;;) {
+        // the first active lane takes the next task for its wavefront; every
+        // lane is active here, so that is lane 0
         int task = 0;
-        if ((threadIdx.x & 63) == 0) task = atomicAdd(P.queue + 2 * slot + 1, 1);
-        task = __shfl(task, 0, 64);
+        if (__lane_id() == 0) task = atomicAdd(P.queue + 2 * slot + 1, 1);
+        task = uniform_i(task);
         if (task >= P.n_wave) break;
-        run_task<KIND, PREC>(wt, P, P.order_wave[task], a);
+        run_task<KIND, PREC>(wt, P, uniform_i(P.order_wave[task]), a);
     }
 }
 

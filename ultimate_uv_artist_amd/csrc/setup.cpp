@@ -261,7 +261,8 @@ void parallel_for(int n, F fn) {
 
 std::string build_host_plan(HostPlan& hp, int64_t n_islands, const int64_t* vert_off, const int64_t* tri_off,
                             const double* P, const int64_t* T, const int64_t* pins, const double* X,
-                            const double* area, const double* cots, bool batch, const int64_t* lscm_pins) {
+                            const double* area, const double* cots, bool batch, const int64_t* lscm_pins,
+                            int wave_max_rows) {
     if (n_islands <= 0) return "n_islands must be positive";
     if (vert_off[0] != 0 || tri_off[0] != 0) return "offsets must start at 0";
     for (int64_t i = 0; i < n_islands; ++i) {
@@ -416,7 +417,7 @@ std::string build_host_plan(HostPlan& hp, int64_t n_islands, const int64_t* vert
     std::vector<std::pair<double, int>> g, wv;
     for (int i = 0; i < n; ++i) {
         double cost = std::pow((double)isl[i].nv, 1.5);
-        (isl[i].nv > kWaveMaxRows ? g : wv).push_back({-cost, i});
+        (isl[i].nv > wave_max_rows ? g : wv).push_back({-cost, i});
     }
     std::sort(g.begin(), g.end());
     std::sort(wv.begin(), wv.end());

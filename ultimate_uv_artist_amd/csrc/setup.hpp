@@ -6,6 +6,8 @@
 #include <string>
 #include <vector>
 
+#include "plan.hpp"
+
 namespace uva {
 
 struct HostPlan {
@@ -31,7 +33,8 @@ struct HostPlan {
 // Returns an empty string on success, else an error message.
 std::string build_host_plan(HostPlan& hp, int64_t n_islands, const int64_t* vert_off, const int64_t* tri_off,
                             const double* P, const int64_t* T, const int64_t* pins, const double* X,
-                            const double* area, const double* cots, bool batch, const int64_t* lscm_pins);
+                            const double* area, const double* cots, bool batch, const int64_t* lscm_pins,
+                            int wave_max_rows = kWaveMaxRows);
 
 // One island holding a general symmetric matrix (no triangles, no pin).
 std::string build_matrix_plan(HostPlan& hp, int64_t n, const int64_t* indptr, const int64_t* indices,
