@@ -44,7 +44,22 @@ def parse():
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-plain", dest="plain", action="store_false",
                     help="skip the plain per-island GPU baseline")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB 5 times (HBM counter calibration)")
     return ap.parse_args()
+
+
+def copy_probe(mib):
+    """A plain copy of known size, for scaling FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_each_way": n * 4, "dispatches": 5}), flush=True)
 
 
 def unique_edge_count(T):
@@ -71,6 +86,9 @@ def orient_majority(U, plan, vo, to, T_local):
 
 def main():
     args = parse()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (islands would shard with no "
               "exchange, see DESIGN.md)", file=sys.stderr)
@@ -172,8 +190,12 @@ def main():
     if args.cpu_sample > 0:
         from oracle import param_cpu as oracle
 
+        # islands at evenly spaced quantiles of the triangle mass, so the sample
+        # has the batch's mix of small and large islands by work, not by count
         order = np.argsort(tri)
-        pick = order[np.linspace(0, len(order) - 1, min(args.cpu_sample, len(order))).astype(int)]
+        cum = np.cumsum(tri[order]) / tri.sum()
+        qs = (np.arange(min(args.cpu_sample, len(order))) + 0.5) / min(args.cpu_sample, len(order))
+        pick = np.unique(order[np.minimum(np.searchsorted(cum, qs), len(order) - 1)])
         Uh0 = U0.cpu().numpy()
         Uh = U.cpu().numpy()
         vo = plan.vert_off
@@ -191,8 +213,9 @@ def main():
             err.append(float(np.abs((g - lo) / s - (ur - lo) / s).max()))
         cpu_tri_iter = float(tri[pick].sum()) * args.iters
         cpu = {"value": cpu_tri_iter / t_cpu / 1e6, "unit": "Mtri*iter/s", "cores": 1, "kind": "port",
-               "sample": f"{len(pick)} islands spread over the size range ({int(tri[pick].sum())} triangles), "
-                         f"{args.iters} ARAP iterations each incl. SuperLU factorisation, {t_cpu:.1f}s"}
+               "sample": f"{len(pick)} islands at evenly spaced quantiles of the triangle mass "
+                         f"({int(tri[pick].sum())} triangles), {args.iters} ARAP iterations each from the same "
+                         f"LSCM layout, SuperLU factorisation included, {t_cpu:.1f}s on one core"}
         parity = {"islands": len(pick), "max_uv_err_unit_box": max(err)}
         print(f"bench: CPU oracle sample {cpu['value']:.3f} Mtri*iter/s; parity max err {max(err):.2e}", flush=True)
 

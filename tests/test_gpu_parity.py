@@ -131,8 +131,6 @@ def test_refine_folded_start_no_line_search(small_disks):
 
 def test_building_blocks(small_disks):
     """local step + rhs, max step, energy, signed areas against the oracle (FP64 both)."""
-    import scipy.sparse as sp
-
     from oracle import param_cpu as oracle
     from ultimate_uv_artist_amd.plan import IslandPlan
 
@@ -163,7 +161,6 @@ def test_building_blocks(small_disks):
         assert e0[i] == pytest.approx(oracle.energy(u, T, X, area, 0), rel=1e-9)
         assert e1[i] == pytest.approx(oracle.energy(u, T, X, area, 1), rel=1e-9)
         assert np.abs(sa[to[i]:to[i + 1]] - oracle.signed_areas(u, T)).max() <= 1e-12 * np.ptp(u) ** 2
-        del sp
 
 
 @pytest.mark.parametrize("precision", [0, 2])
