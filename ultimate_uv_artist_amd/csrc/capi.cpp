@@ -38,7 +38,7 @@ struct uva_plan {
     std::vector<void*> allocs;
     int64_t device_bytes = 0;
     int64_t nnz_pad = 0, nnz_true = 0;
-    int grid = 1;
+    int cus = 1;                          // compute units of the plan's device
     std::vector<double> P;                 // host positions (LSCM pins given later)
     std::vector<int> lpin0, lpin1, v0;     // host copies
     int* d_lpin0 = nullptr;
@@ -82,7 +82,7 @@ struct uva_plan {
 namespace {
 
 int launch(uva_plan* pl, int kind, int prec, const uva::TaskArgs& a, int slot, hipStream_t s) {
-    HIP_TRY(uva::launch_island_tasks(kind, prec, pl->dp, a, slot, pl->grid, s));
+    HIP_TRY(uva::launch_island_tasks(kind, prec, pl->dp, a, slot, pl->cus, s));
     return UVA_OK;
 }
 
@@ -199,7 +199,7 @@ int upload_plan(uva::HostPlan& hp, const double* P, hipStream_t s, uva_plan** ou
         d.b64 = p2;
         if ((e = pl->alloc(&p2, nv)) != hipSuccess) return bail(UVA_ENOMEM, "X64");
         d.X64 = p2;
-        void** vecs[4] = {&d.vr, &d.vp, &d.vAp, &d.ve};
+        void** vecs[6] = {&d.vr, &d.vp, &d.vAp, &d.ve, &d.vu, &d.vw};
         for (void** v : vecs) {
             double2* q = nullptr;
             if ((e = pl->alloc(&q, nv)) != hipSuccess) return bail(UVA_ENOMEM, "CG vectors");
@@ -223,9 +223,7 @@ int upload_plan(uva::HostPlan& hp, const double* P, hipStream_t s, uva_plan** ou
     int dev = 0, cus = 1;
     HIP_TRY(hipGetDevice(&dev));
     HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    int per_cu = uva::island_kernel_occupancy();
-    int want = d.n_group + (d.n_wave + uva::kBlock / 64 - 1) / (uva::kBlock / 64);
-    pl->grid = std::max(1, std::min(want, cus * per_cu));
+    pl->cus = cus;
     hipError_t se = hipStreamSynchronize(s);
     if (se != hipSuccess) return bail(UVA_EHIP, std::string("plan upload: ") + hipGetErrorString(se));
     *out = hold.release();

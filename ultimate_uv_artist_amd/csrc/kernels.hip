@@ -71,12 +71,18 @@ struct WaveTeam {
         sync();
         return uniform_d(v);
     }
+    __device__ void sum8(double4 a, double4 b, double4& sa, double4& sb) const {
+        sa = sum4(a);
+        b.x = wave_sum(b.x); b.y = wave_sum(b.y); b.z = wave_sum(b.z); b.w = wave_sum(b.w);
+        sb = uniform_d4(b);
+    }
 };
 
 template <int BS>
 struct BlockTeam {
     static constexpr int kWaves = BS / 64;
-    double* lds;  // 2 * kWaves * 4 doubles
+    static constexpr int kSlots = 8;   // doubles per wavefront slot
+    double* lds;  // 2 * kWaves * kSlots doubles
     int par;
     __device__ explicit BlockTeam(double* l) : lds(l), par(0) {}
     __device__ int lane() const { return threadIdx.x & 63; }
@@ -88,27 +94,40 @@ struct BlockTeam {
     // The two slot buffers alternate; a buffer is rewritten only after the
     // barrier of the following reduction, when every thread has read it.
     __device__ double4 sum4(double4 v) {
-        v.x = wave_sum(v.x); v.y = wave_sum(v.y); v.z = wave_sum(v.z); v.w = wave_sum(v.w);
-        double* buf = lds + par * kWaves * 4;
+        double4 s, unused;
+        sum8(v, make_double4(0.0, 0.0, 0.0, 0.0), s, unused);
+        return s;
+    }
+    __device__ void sum8(double4 a, double4 b, double4& sa, double4& sb) {
+        a.x = wave_sum(a.x); a.y = wave_sum(a.y); a.z = wave_sum(a.z); a.w = wave_sum(a.w);
+        b.x = wave_sum(b.x); b.y = wave_sum(b.y); b.z = wave_sum(b.z); b.w = wave_sum(b.w);
+        double* buf = lds + par * kWaves * kSlots;
         par ^= 1;
         if (lane() == 0) {
-            buf[4 * wave()] = v.x; buf[4 * wave() + 1] = v.y; buf[4 * wave() + 2] = v.z; buf[4 * wave() + 3] = v.w;
+            double* q = buf + kSlots * wave();
+            q[0] = a.x; q[1] = a.y; q[2] = a.z; q[3] = a.w;
+            q[4] = b.x; q[5] = b.y; q[6] = b.z; q[7] = b.w;
         }
         __syncthreads();
-        double4 s = make_double4(0.0, 0.0, 0.0, 0.0);
+        double4 s = make_double4(0.0, 0.0, 0.0, 0.0), t = make_double4(0.0, 0.0, 0.0, 0.0);
 #pragma unroll
-        for (int w = 0; w < kWaves; ++w) { s.x += buf[4 * w]; s.y += buf[4 * w + 1]; s.z += buf[4 * w + 2]; s.w += buf[4 * w + 3]; }
-        return uniform_d4(s);
+        for (int w = 0; w < kWaves; ++w) {
+            const double* q = buf + kSlots * w;
+            s.x += q[0]; s.y += q[1]; s.z += q[2]; s.w += q[3];
+            t.x += q[4]; t.y += q[5]; t.z += q[6]; t.w += q[7];
+        }
+        sa = uniform_d4(s);
+        sb = uniform_d4(t);
     }
     __device__ double min1(double v) {
         v = wave_min(v);
-        double* buf = lds + par * kWaves * 4;
+        double* buf = lds + par * kWaves * kSlots;
         par ^= 1;
-        if (lane() == 0) buf[4 * wave()] = v;
+        if (lane() == 0) buf[kSlots * wave()] = v;
         __syncthreads();
         double m = buf[0];
 #pragma unroll
-        for (int w = 1; w < kWaves; ++w) m = fmin(m, buf[4 * w]);
+        for (int w = 1; w < kWaves; ++w) m = fmin(m, buf[kSlots * w]);
         return uniform_d(m);
     }
 };
@@ -248,161 +267,7 @@ __device__ __forceinline__ double acc_mul(float a, float b) {
 template <int PREC>
 __device__ __forceinline__ double acc_mul(double a, double b) { return a * b; }
 
-struct CGOut {
-    int iters;
-    int conv;
-    double rel;
-};
-
-// Jacobi-preconditioned CG on the rows of one island, warm-started from X64,
-// right-hand side b64.  Two independent columns (u, v) for the ARAP operator,
-// one coupled 2n system for LSCM.  Mixed precision: FP32 inner iterations on
-// the correction e with FP64 dots; every outer pass recomputes the true
-// residual b - A X in FP64 and folds e into X64 (iterative refinement).
-template <int PREC, int OP, class Team>
-__device__ CGOut cg_island(Team& tm, const DevPlan& P, const Isl& d, double2 b2, double tol, int maxit, int lp0,
-                           int lp1) {
-    using PT = Prec<PREC>;
-    using V = typename PT::V;
-    using V2 = typename Vec2<V>::type;
-    V2* __restrict__ r = (V2*)P.vr;
-    V2* __restrict__ p = (V2*)P.vp;
-    V2* __restrict__ Ap = (V2*)P.vAp;
-    V2* __restrict__ e = (V2*)P.ve;
-    const bool coupled = (OP == OP_LSCM);
-    const double bx2 = coupled ? fmax(b2.x + b2.y, 1e-300) : fmax(b2.x, 1e-300);
-    const double by2 = coupled ? bx2 : fmax(b2.y, 1e-300);
-    const double t2 = tol * tol;
-    const int lane = tm.lane();
-    int it = 0;
-    CGOut out{0, 0, 0.0};
-    double relx = 0.0, rely = 0.0;
-    for (int outer = 0;; ++outer) {
-        // true residual and restart
-        double4 acc = make_double4(0.0, 0.0, 0.0, 0.0);
-        FOR_ROWS(tm, d, row, sg) {
-            double rx, ry;
-            const bool pinned = (OP == OP_LSCM) && (row == lp0 || row == lp1);
-            if (PT::kResid64) {
-                double2 y = spmv_row<double, OP>(P, row, sg, lane, P.X64, diag_of<OP>(P, d, row));
-                double2 bb = P.b64[row];
-                rx = bb.x - y.x;
-                ry = bb.y - y.y;
-            } else {
-                // all-FP32: residual from the FP32 matrix and the FP32-rounded iterate
-                const int base = P.sl_ptr[sg];
-                const int width = (P.sl_ptr[sg + 1] - base) >> 6;
-                double2 xr = P.X64[row];
-                float dg = (float)diag_of<OP>(P, d, row);
-                float yx = dg * (float)xr.x, yy = dg * (float)xr.y;
-                int o = base + lane;
-                for (int k = 0; k < width; ++k, o += kSlice) {
-                    double2 xc = P.X64[P.m_col[o]];
-                    yx = fmaf(P.m_hi[o], (float)xc.x, yx);
-                    yy = fmaf(P.m_hi[o], (float)xc.y, yy);
-                }
-                double2 bb = P.b64[row];
-                rx = (double)((float)bb.x - yx);
-                ry = (double)((float)bb.y - yy);
-            }
-            if (pinned) { rx = 0.0; ry = 0.0; }
-            V2 rv;
-            rv.x = (V)rx;
-            rv.y = (V)ry;
-            const V di = (V)dinv_of<OP>(P, d, row);
-            V2 z;
-            z.x = di * rv.x;
-            z.y = di * rv.y;
-            r[row] = rv;
-            p[row] = z;
-            V2 zero;
-            zero.x = 0;
-            zero.y = 0;
-            e[row] = zero;
-            acc.x += PT::kResid64 ? rx * rx : (double)((float)rx * (float)rx);
-            acc.y += PT::kResid64 ? ry * ry : (double)((float)ry * (float)ry);
-            acc.z += acc_mul<PREC>(rv.x, z.x);
-            acc.w += acc_mul<PREC>(rv.y, z.y);
-        }
-        double4 s = tm.sum4(acc);
-        if (coupled) {
-            relx = rely = (s.x + s.y) / bx2;
-        } else {
-            relx = s.x / bx2;
-            rely = s.y / by2;
-        }
-        if (relx <= t2 && rely <= t2) { out.conv = 1; break; }
-        if (outer >= PT::kMaxOuter || it >= maxit || !isfinite(relx) || !isfinite(rely)) break;
-        double rzx = s.z, rzy = s.w;
-        if (coupled) { rzx = rzy = s.z + s.w; }
-        for (;;) {
-            // Ap = A p and p.Ap
-            double4 pa = make_double4(0.0, 0.0, 0.0, 0.0);
-            FOR_ROWS(tm, d, row, sg) {
-                V2 ap = spmv_row<V, OP>(P, row, sg, lane, p, (V)diag_of<OP>(P, d, row));
-                if (OP == OP_LSCM && (row == lp0 || row == lp1)) { ap.x = 0; ap.y = 0; }
-                Ap[row] = ap;
-                V2 pr = p[row];
-                pa.x += acc_mul<PREC>(pr.x, ap.x);
-                pa.y += acc_mul<PREC>(pr.y, ap.y);
-            }
-            double4 ps = tm.sum4(pa);
-            double pax = ps.x, pay = ps.y;
-            if (coupled) pax = pay = ps.x + ps.y;
-            const V ax = (V)(pax > 0.0 ? rzx / pax : 0.0);
-            const V ay = (V)(pay > 0.0 ? rzy / pay : 0.0);
-            // e += alpha p, r -= alpha Ap, z = D^-1 r
-            double4 acc2 = make_double4(0.0, 0.0, 0.0, 0.0);
-            FOR_ROWS(tm, d, row, sg) {
-                V2 pr = p[row], apr = Ap[row], rr = r[row], ee = e[row];
-                ee.x = fma(ax, pr.x, ee.x);
-                ee.y = fma(ay, pr.y, ee.y);
-                rr.x = fma(-ax, apr.x, rr.x);
-                rr.y = fma(-ay, apr.y, rr.y);
-                e[row] = ee;
-                r[row] = rr;
-                const V di = (V)dinv_of<OP>(P, d, row);
-                acc2.x += acc_mul<PREC>(rr.x, rr.x);
-                acc2.y += acc_mul<PREC>(rr.y, rr.y);
-                acc2.z += acc_mul<PREC>(rr.x, di * rr.x);
-                acc2.w += acc_mul<PREC>(rr.y, di * rr.y);
-            }
-            double4 s2 = tm.sum4(acc2);
-            ++it;
-            double rrx = s2.x, rry = s2.y;
-            if (coupled) rrx = rry = s2.x + s2.y;
-            if ((rrx <= PT::kInnerFrac2 * t2 * bx2 && rry <= PT::kInnerFrac2 * t2 * by2) || it >= maxit) break;
-            if (!isfinite(rrx) || !isfinite(rry)) break;   // a non-finite island stops (it is not converged)
-            double nzx = s2.z, nzy = s2.w;
-            if (coupled) nzx = nzy = s2.z + s2.w;
-            const V bex = (V)(rzx > 0.0 ? nzx / rzx : 0.0);
-            const V bey = (V)(rzy > 0.0 ? nzy / rzy : 0.0);
-            rzx = nzx;
-            rzy = nzy;
-            FOR_ROWS(tm, d, row, sg) {
-                V2 pr = p[row], rr = r[row];
-                const V di = (V)dinv_of<OP>(P, d, row);
-                pr.x = fma(bex, pr.x, di * rr.x);
-                pr.y = fma(bey, pr.y, di * rr.y);
-                p[row] = pr;
-            }
-            tm.sync();
-        }
-        // fold the correction into the FP64 iterate
-        FOR_ROWS(tm, d, row, sg) {
-            double2 x = P.X64[row];
-            V2 ee = e[row];
-            x.x += (double)ee.x;
-            x.y += (double)ee.y;
-            if (PT::kRoundX) { x.x = (double)(float)x.x; x.y = (double)(float)x.y; }
-            P.X64[row] = x;
-        }
-        tm.sync();
-    }
-    out.iters = it;
-    out.rel = sqrt(fmax(relx, rely));
-    return out;
-}
+#include "cg_island.hpp"
 
 // ---------------------------------------------------------------- per-triangle pieces
 struct Jac {
@@ -796,7 +661,7 @@ __device__ void run_task(Team& tm, const DevPlan& P, int isl, const TaskArgs& a)
 
 template <int BS, int KIND, int PREC>
 __global__ __launch_bounds__(BS) void island_kernel(DevPlan P, TaskArgs a, int slot) {
-    __shared__ double red[2 * (BS / 64) * 4];
+    __shared__ double red[2 * (BS / 64) * BlockTeam<BS>::kSlots];
     __shared__ int s_task;
     BlockTeam<BS> bt(red);
     for (;;) {
@@ -832,31 +697,44 @@ __global__ void signed_areas_kernel(DevPlan P, const double2* __restrict__ U, do
 
 // ---------------------------------------------------------------- launchers
 template <int KIND, int PREC>
-static hipError_t launch_kind(const DevPlan& P, const TaskArgs& a, int slot, int grid, hipStream_t s) {
-    hipLaunchKernelGGL((island_kernel<kBlock, KIND, PREC>), dim3(grid), dim3(kBlock), 0, s, P, a, slot);
+static hipError_t launch_kind(const DevPlan& P, const TaskArgs& a, int slot, int cus, hipStream_t s) {
+    // FP64 kernels hold more registers per lane: half the workgroup size
+    constexpr int BS = PREC == UVA_PREC_F64 ? kBlockF64 : kBlock;
+    static int occ = 0;   // resident workgroups per CU of this instantiation (same on every call)
+    if (occ == 0) {
+        int nb = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, island_kernel<BS, KIND, PREC>, BS, 0) != hipSuccess)
+            nb = 1;
+        occ = nb > 0 ? nb : 1;
+    }
+    // enough workgroups for every workgroup task and the wavefront tasks, at
+    // most what the device keeps resident
+    const int want = P.n_group + (P.n_wave + BS / 64 - 1) / (BS / 64);
+    const int grid = want < cus * occ ? (want > 0 ? want : 1) : cus * occ;
+    hipLaunchKernelGGL((island_kernel<BS, KIND, PREC>), dim3(grid), dim3(BS), 0, s, P, a, slot);
     return hipGetLastError();
 }
 
-hipError_t launch_island_tasks(int kind, int prec, const DevPlan& P, const TaskArgs& a, int slot, int grid,
+hipError_t launch_island_tasks(int kind, int prec, const DevPlan& P, const TaskArgs& a, int slot, int cus,
                                hipStream_t s) {
 #define UVA_K(K)                                                                 \
     case K:                                                                      \
         switch (prec) {                                                          \
-            case UVA_PREC_F32: return launch_kind<K, UVA_PREC_F32>(P, a, slot, grid, s); \
-            case UVA_PREC_F64: return launch_kind<K, UVA_PREC_F64>(P, a, slot, grid, s); \
-            default: return launch_kind<K, UVA_PREC_MIXED>(P, a, slot, grid, s);   \
+            case UVA_PREC_F32: return launch_kind<K, UVA_PREC_F32>(P, a, slot, cus, s); \
+            case UVA_PREC_F64: return launch_kind<K, UVA_PREC_F64>(P, a, slot, cus, s); \
+            default: return launch_kind<K, UVA_PREC_MIXED>(P, a, slot, cus, s);   \
         }
     switch (kind) {
         UVA_K(TASK_ARAP_ITER)
         UVA_K(TASK_CG)
-        case TASK_PROLOGUE: return launch_kind<TASK_PROLOGUE, UVA_PREC_MIXED>(P, a, slot, grid, s);
-        case TASK_LSCM: return launch_kind<TASK_LSCM, UVA_PREC_F64>(P, a, slot, grid, s);
-        case TASK_LOCAL_RHS: return launch_kind<TASK_LOCAL_RHS, UVA_PREC_MIXED>(P, a, slot, grid, s);
-        case TASK_MAX_STEP: return launch_kind<TASK_MAX_STEP, UVA_PREC_MIXED>(P, a, slot, grid, s);
-        case TASK_ENERGY: return launch_kind<TASK_ENERGY, UVA_PREC_MIXED>(P, a, slot, grid, s);
-        case TASK_REPAIR: return launch_kind<TASK_REPAIR, UVA_PREC_MIXED>(P, a, slot, grid, s);
-        case TASK_ORIENT: return launch_kind<TASK_ORIENT, UVA_PREC_MIXED>(P, a, slot, grid, s);
-        case TASK_FLIPS: return launch_kind<TASK_FLIPS, UVA_PREC_MIXED>(P, a, slot, grid, s);
+        case TASK_PROLOGUE: return launch_kind<TASK_PROLOGUE, UVA_PREC_MIXED>(P, a, slot, cus, s);
+        case TASK_LSCM: return launch_kind<TASK_LSCM, UVA_PREC_F64>(P, a, slot, cus, s);
+        case TASK_LOCAL_RHS: return launch_kind<TASK_LOCAL_RHS, UVA_PREC_MIXED>(P, a, slot, cus, s);
+        case TASK_MAX_STEP: return launch_kind<TASK_MAX_STEP, UVA_PREC_MIXED>(P, a, slot, cus, s);
+        case TASK_ENERGY: return launch_kind<TASK_ENERGY, UVA_PREC_MIXED>(P, a, slot, cus, s);
+        case TASK_REPAIR: return launch_kind<TASK_REPAIR, UVA_PREC_MIXED>(P, a, slot, cus, s);
+        case TASK_ORIENT: return launch_kind<TASK_ORIENT, UVA_PREC_MIXED>(P, a, slot, cus, s);
+        case TASK_FLIPS: return launch_kind<TASK_FLIPS, UVA_PREC_MIXED>(P, a, slot, cus, s);
         default: return hipErrorInvalidValue;
     }
 #undef UVA_K
@@ -874,12 +752,5 @@ hipError_t launch_signed_areas(const DevPlan& P, const double2* U, double* sa, h
     return hipGetLastError();
 }
 
-int island_kernel_occupancy() {
-    int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, island_kernel<kBlock, TASK_ARAP_ITER, UVA_PREC_MIXED>,
-                                                     kBlock, 0) != hipSuccess)
-        return 1;
-    return nb > 0 ? nb : 1;
-}
 
 }  // namespace uva

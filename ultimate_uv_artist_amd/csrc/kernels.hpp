@@ -7,7 +7,8 @@
 
 namespace uva {
 
-constexpr int kBlock = 512;   // workgroup size of the island kernel (8 wavefronts)
+constexpr int kBlock = 1024;    // workgroup size of the island kernel (16 wavefronts)
+constexpr int kBlockF64 = 512;  // FP64 instantiations (more registers per lane)
 
 enum TaskKind {
     TASK_PROLOGUE = 0,   // flips of the start, line-search decision, rotations + energy
@@ -36,10 +37,9 @@ struct TaskArgs {
     int rounds;              // repair rounds
 };
 
-hipError_t launch_island_tasks(int kind, int prec, const DevPlan& P, const TaskArgs& a, int slot, int grid,
+hipError_t launch_island_tasks(int kind, int prec, const DevPlan& P, const TaskArgs& a, int slot, int cus,
                                hipStream_t s);
 hipError_t launch_fill_int(int* p, int n, int v, hipStream_t s);
 hipError_t launch_signed_areas(const DevPlan& P, const double2* U, double* sa, hipStream_t s);
-int island_kernel_occupancy();
 
 }  // namespace uva

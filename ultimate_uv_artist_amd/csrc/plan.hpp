@@ -82,8 +82,10 @@ struct DevPlan {
     double2* X64;                // CG iterate (warm start = U)
     void* vr;
     void* vp;
-    void* vAp;
+    void* vAp;                   // s = A p in the CG recurrence
     void* ve;
+    void* vu;                    // u = D^-1 r
+    void* vw;                    // w = A u
     double* red;                 // per-island scalars for multi-launch paths
     IslandState* st;
     int* queue;                  // task counters
