@@ -151,6 +151,13 @@ def main():
     gbs = bytes_step / (ms * 1e-3) / 1e9
     value = float(tri.sum()) * args.iters / (ms * 1e-3) / 1e6
     kpi = k / np.maximum(iters_run, 1)
+    busy = st["busy_ms"]
+    n_cu = torch.cuda.get_device_properties(dev).multi_processor_count
+    slow = np.argsort(-busy)[:5]
+    print(f"bench: island work {busy.sum():.0f} team-ms per step = {busy.sum() / ms:.1f} teams busy on average "
+          f"({n_cu} CUs); slowest islands (n, CG iters, ms, us per CG iter): " +
+          ", ".join(f"({nv[i]}, {int(k[i])}, {busy[i]:.1f}, {1e3 * busy[i] / max(k[i], 1):.1f})" for i in slow),
+          flush=True)
     print(f"bench: {ms:.2f} ms/step, {value:.1f} Mtri*iter/s, {gbs:.0f} GB/s algorithmic; CG iters per global "
           f"step mean {kpi.mean():.1f} p95 {np.percentile(kpi, 95):.1f}; converged {int(st['converged'].sum())}/"
           f"{len(charts)}; max rel resid {st['rel_resid'].max():.2e}", flush=True)

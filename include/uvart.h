@@ -50,6 +50,8 @@ extern "C" {
                                1e-30 clamps, batch-global pin weight */
 #define UVA_PLAN_NO_WAVE 2u /* solve every island with a whole workgroup
                                (no single-wavefront tasks) */
+#define UVA_PLAN_NO_TEAMS 4u /* one workgroup per island at most (no
+                               multi-workgroup teams for large islands) */
 
 /* ARAP semantics */
 #define UVA_MODE_REFINE 0   /* param.refine: per-island pin weight, energy early
@@ -88,6 +90,7 @@ typedef struct {
     double rel_resid;        /* last true relative residual (max over u, v) */
     double energy;           /* ARAP energy of the returned layout (REFINE) */
     double step;             /* last line-search step */
+    double busy_ms;          /* time a team spent on this island in the call */
 } uva_island_stats;
 
 typedef struct {

@@ -111,7 +111,8 @@ def test_refine_early_exit_and_pin(small_disks):
         U = Ud.cpu().numpy()
         assert st["arap_iters"][0] == len(trace)
         assert unit_box_err(U, Ur) <= 1e-4
-        assert np.allclose(U[5], U0[5], atol=1e-9 * np.ptp(U0))   # the soft pin holds exactly
+        # the soft pin holds to the accuracy of the global solves (exactly only for a direct solve)
+        assert np.abs(U[5] - U0[5]).max() <= 1e-5 * np.ptp(U0)
         assert abs(st["energy"][0] / trace[-1][0] - 1) <= 1e-6
 
 
@@ -281,6 +282,37 @@ def test_batched_equals_single_island_plans(small_disks):
         u = p1.device_uv(U0[s])
         p1.arap(u, 5, line_search=True, mode=_lib.UVA_MODE_REFINE, tol=0.0)
         assert np.array_equal(u.cpu().numpy(), Ub[s])
+
+
+@pytest.mark.parametrize("no_wave", [False, True])
+def test_team_wave_and_workgroup_paths_agree(no_wave):
+    """The same islands solved by multi-workgroup teams, single workgroups and
+    single wavefronts give the same layout to CG accuracy, and match the oracle."""
+    from oracle import param_cpu as oracle
+    from ultimate_uv_artist_amd import _lib, workloads
+    from ultimate_uv_artist_amd.plan import IslandPlan
+
+    rng = np.random.default_rng(31)
+    charts = [workloads.disk_island(rng, t) for t in (30000, 14000, 600, 2500)]
+    U0 = np.concatenate([oracle.lscm(P, T) for P, T in charts])
+    out = []
+    for no_teams in (False, True):
+        plan = IslandPlan(charts, no_teams=no_teams, no_wave=no_wave)
+        info = plan.info()
+        print(f"paths: no_teams={no_teams} no_wave={no_wave} {info['n_group_tasks']} group tasks, "
+              f"{info['n_wave_tasks']} wave tasks", flush=True)
+        if not no_teams:
+            assert info["n_group_tasks"] > len(charts) - (0 if no_wave else 2)   # some island has a team
+        U = plan.device_uv(U0)
+        plan.arap(U, 3, mode=_lib.UVA_MODE_REFINE, tol=0.0)
+        assert plan.stats()["converged"].all()
+        out.append(U.cpu().numpy())
+    vo = plan.vert_off
+    for i, (P, T) in enumerate(charts):
+        s = slice(vo[i], vo[i + 1])
+        assert unit_box_err(out[0][s], out[1][s]) <= 1e-5
+    Ur, _ = oracle.refine(*charts[0], U0[vo[0]:vo[1]], iters=3, tol=0.0)
+    assert unit_box_err(out[0][vo[0]:vo[1]], Ur) <= 1e-4
 
 
 def test_graph_replays_match_eager(small_disks):

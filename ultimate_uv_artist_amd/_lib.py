@@ -22,6 +22,7 @@ UVA_ENOBOUNDARY = -5
 
 UVA_PLAN_BATCH = 1
 UVA_PLAN_NO_WAVE = 2
+UVA_PLAN_NO_TEAMS = 4
 UVA_MODE_REFINE = 0
 UVA_MODE_BATCH = 1
 UVA_PREC_MIXED = 0
@@ -48,7 +49,8 @@ class PlanInfo(C.Structure):
 class IslandStats(C.Structure):
     _fields_ = [("cg_iters", C.c_int32), ("cg_iters_max", C.c_int32), ("arap_iters", C.c_int32),
                 ("converged", C.c_int32), ("flips0", C.c_int32), ("use_ls", C.c_int32), ("status", C.c_int32),
-                ("pad", C.c_int32), ("rel_resid", C.c_double), ("energy", C.c_double), ("step", C.c_double)]
+                ("pad", C.c_int32), ("rel_resid", C.c_double), ("energy", C.c_double), ("step", C.c_double),
+                ("busy_ms", C.c_double)]
 
 
 class ArapOpts(C.Structure):

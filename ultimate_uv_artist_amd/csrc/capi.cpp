@@ -39,6 +39,7 @@ struct uva_plan {
     int64_t device_bytes = 0;
     int64_t nnz_pad = 0, nnz_true = 0;
     int cus = 1;                          // compute units of the plan's device
+    int clock_khz = 0;                    // device wall-clock rate (wall_clock64 ticks per ms)
     std::vector<double> P;                 // host positions (LSCM pins given later)
     std::vector<int> lpin0, lpin1, v0;     // host copies
     int* d_lpin0 = nullptr;
@@ -88,6 +89,7 @@ int launch(uva_plan* pl, int kind, int prec, const uva::TaskArgs& a, int slot, h
 
 int reset_queue(uva_plan* pl, int slots, hipStream_t s) {
     HIP_TRY(uva::launch_fill_int(pl->dp.queue, 2 * slots, 0, s));
+    if (pl->dp.n_teams > 0) HIP_TRY(uva::launch_fill_int(pl->dp.team_cnt, slots * pl->dp.n_teams, 0, s));
     return UVA_OK;
 }
 
@@ -156,6 +158,7 @@ int upload_plan(uva::HostPlan& hp, const double* P, hipStream_t s, uva_plan** ou
     UP(diag, diag); UP(dinv32, dinv32);
     UP(adj_ptr, adj_ptr); UP(adj_idx, adj_idx); UP(vbnd, vbnd);
     UP(order_group, order_group); UP(order_wave, order_wave);
+    UP(grp_mem, grp_mem); UP(grp_m, grp_m); UP(grp_team, grp_team);
 #undef UP
     {
         const double* g = nullptr;
@@ -183,6 +186,13 @@ int upload_plan(uva::HostPlan& hp, const double* P, hipStream_t s, uva_plan** ou
     }
     d.n_group = (int)hp.order_group.size();
     d.n_wave = (int)hp.order_wave.size();
+    d.n_teams = hp.n_teams;
+    {
+        hipError_t e;
+        if ((e = pl->alloc(&d.team_cnt, (size_t)kMaxSlots * std::max(1, hp.n_teams))) != hipSuccess ||
+            (e = pl->alloc(&d.team_part, (size_t)std::max(1, hp.n_teams) * 2 * uva::kMaxTeam * 8)) != hipSuccess)
+            return bail(UVA_ENOMEM, "team arrays");
+    }
     // scratch
     {
         hipError_t e = hipSuccess;
@@ -223,6 +233,7 @@ int upload_plan(uva::HostPlan& hp, const double* P, hipStream_t s, uva_plan** ou
     int dev = 0, cus = 1;
     HIP_TRY(hipGetDevice(&dev));
     HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    if (hipDeviceGetAttribute(&pl->clock_khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess) pl->clock_khz = 0;
     pl->cus = cus;
     hipError_t se = hipStreamSynchronize(s);
     if (se != hipSuccess) return bail(UVA_EHIP, std::string("plan upload: ") + hipGetErrorString(se));
@@ -245,12 +256,22 @@ int uva_plan_create(uva_plan** out, int64_t n_islands, const int64_t* vert_off, 
     *out = nullptr;
     if ((X || area || cots) && !(X && area && cots)) return fail(UVA_EINVAL, "X, area and cots go together");
     hipStream_t s = (hipStream_t)stream;
+    // one 1024-thread workgroup per CU is resident: the team plan balances
+    // the islands over that many concurrent workgroups
+    int team_slots = 0;
+    if (!(flags & UVA_PLAN_NO_TEAMS)) {
+        int dev = 0, c = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess)
+            team_slots = c;
+    }
     uva::HostPlan hp;
     std::string err;
     try {
         err = uva::build_host_plan(hp, n_islands, vert_off, tri_off, P, T, pins, X, area, cots,
                                    (flags & UVA_PLAN_BATCH) != 0, nullptr,
-                                   (flags & UVA_PLAN_NO_WAVE) ? -1 : uva::kWaveMaxRows);
+                                   (flags & UVA_PLAN_NO_WAVE) ? -1 : uva::kWaveMaxRows,
+                                   (flags & UVA_PLAN_NO_TEAMS) ? 0 : team_slots);
     } catch (const std::bad_alloc&) {
         return fail(UVA_ENOMEM, "host allocation failed while building the plan");
     }
@@ -312,6 +333,7 @@ int uva_plan_get_stats(uva_plan* plan, uva_island_stats* stats, void* stream) {
         o.rel_resid = st[i].rel_resid;
         o.energy = st[i].e_prev;
         o.step = st[i].step;
+        o.busy_ms = plan->clock_khz > 0 ? (double)st[i].ticks / (double)plan->clock_khz : 0.0;
     }
     return UVA_OK;
 }

@@ -26,9 +26,10 @@ struct CGOut {
 template <class Team, class F>
 __device__ __forceinline__ void for_row_pairs(const Team& tm, const Isl& d, F&& f) {
     const int nw = tm.nwaves(), lane = tm.lane();
-    for (int s = tm.wave(); s < d.ns; s += 2 * nw) {
+    const int hi = tm.sl_hi(d.ns);
+    for (int s = tm.sl_lo(d.ns) + tm.wave(); s < hi; s += 2 * nw) {
         const int sb = s + nw;
-        const bool sbv = sb < d.ns;
+        const bool sbv = sb < hi;
         const bool oka = s * kSlice + lane < d.nv;
         const bool okb = sbv && sb * kSlice + lane < d.nv;
         f(d.v0 + s * kSlice + lane, d.s0 + s, oka, d.v0 + sb * kSlice + lane, d.s0 + sb, sbv, okb);

@@ -49,12 +49,23 @@ __device__ __forceinline__ double wave_min(double v) {
     return v;
 }
 
+// A team solves one island.  Its interface: lane()/wave()/nwaves() map rows
+// (wavefront w of the team takes slices lo+w, lo+w+W, ... of its slice range
+// [sl_lo, sl_hi)), tid()/tstride() and [tri_lo, tri_hi) map triangles,
+// leader() is the one thread that writes island state, sync() makes the
+// team's global-memory writes visible to the whole team, and the reductions
+// return identical (scalar) values in every thread.
 struct WaveTeam {
     __device__ int lane() const { return threadIdx.x & 63; }
     __device__ int wave() const { return 0; }
     __device__ int nwaves() const { return 1; }
-    __device__ int rank() const { return threadIdx.x & 63; }
-    __device__ int size() const { return 64; }
+    __device__ int tid() const { return threadIdx.x & 63; }
+    __device__ int tstride() const { return 64; }
+    __device__ bool leader() const { return (threadIdx.x & 63) == 0; }
+    __device__ int sl_lo(int) const { return 0; }
+    __device__ int sl_hi(int ns) const { return ns; }
+    __device__ int tri_lo(int) const { return 0; }
+    __device__ int tri_hi(int nt) const { return nt; }
     // Global-memory writes of this wavefront become visible to its other lanes.
     __device__ void sync() const {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
@@ -88,8 +99,13 @@ struct BlockTeam {
     __device__ int lane() const { return threadIdx.x & 63; }
     __device__ int wave() const { return threadIdx.x >> 6; }
     __device__ int nwaves() const { return kWaves; }
-    __device__ int rank() const { return threadIdx.x; }
-    __device__ int size() const { return BS; }
+    __device__ int tid() const { return threadIdx.x; }
+    __device__ int tstride() const { return BS; }
+    __device__ bool leader() const { return threadIdx.x == 0; }
+    __device__ int sl_lo(int) const { return 0; }
+    __device__ int sl_hi(int ns) const { return ns; }
+    __device__ int tri_lo(int) const { return 0; }
+    __device__ int tri_hi(int nt) const { return nt; }
     __device__ void sync() const { __syncthreads(); }
     // The two slot buffers alternate; a buffer is rewritten only after the
     // barrier of the following reduction, when every thread has read it.
@@ -132,6 +148,95 @@ struct BlockTeam {
     }
 };
 
+// m workgroups on one island.  Member k owns a contiguous block of the
+// island's slices and triangles (halo traffic only at block edges).  sync()
+// is a counter barrier with agent-scope release/acquire (every member's writes
+// become visible to the others); a reduction combines each member's partial
+// sums, written to global memory, in member order after one barrier, so the
+// result is identical in every member.  All members are resident (the grid
+// never exceeds the resident capacity) and take their member tasks from the
+// queue in order, so every member eventually arrives.
+template <int BS>
+struct TeamWG {
+    static constexpr int kWaves = BS / 64;
+    static constexpr int kSlots = 8;
+    double* lds;        // kWaves * kSlots doubles
+    int* cnt;           // arrival counter of this team (zero at launch)
+    double* part;       // 2 * kMaxTeam * kSlots partial sums
+    int m, mem;
+    int rounds;         // barriers passed
+    int nred;           // reductions done (selects the partial buffer)
+    __device__ TeamWG(double* l, int* c, double* pt, int m_, int k_)
+        : lds(l), cnt(c), part(pt), m(m_), mem(k_), rounds(0), nred(0) {}
+    __device__ int lane() const { return threadIdx.x & 63; }
+    __device__ int wave() const { return threadIdx.x >> 6; }
+    __device__ int nwaves() const { return kWaves; }
+    __device__ int tid() const { return threadIdx.x; }
+    __device__ int tstride() const { return BS; }
+    __device__ bool leader() const { return mem == 0 && threadIdx.x == 0; }
+    __device__ int sl_lo(int ns) const { return (int)((long long)ns * mem / m); }
+    __device__ int sl_hi(int ns) const { return (int)((long long)ns * (mem + 1) / m); }
+    __device__ int tri_lo(int nt) const { return (int)((long long)nt * mem / m); }
+    __device__ int tri_hi(int nt) const { return (int)((long long)nt * (mem + 1) / m); }
+    __device__ void sync() {
+        __syncthreads();
+        ++rounds;
+        if (threadIdx.x == 0) {
+            __hip_atomic_fetch_add(cnt, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            while (__hip_atomic_load(cnt, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < rounds * m)
+                __builtin_amdgcn_s_sleep(1);
+        }
+        __syncthreads();
+    }
+    __device__ void sum8(double4 a, double4 b, double4& sa, double4& sb) {
+        a.x = wave_sum(a.x); a.y = wave_sum(a.y); a.z = wave_sum(a.z); a.w = wave_sum(a.w);
+        b.x = wave_sum(b.x); b.y = wave_sum(b.y); b.z = wave_sum(b.z); b.w = wave_sum(b.w);
+        if (lane() == 0) {
+            double* q = lds + kSlots * wave();
+            q[0] = a.x; q[1] = a.y; q[2] = a.z; q[3] = a.w;
+            q[4] = b.x; q[5] = b.y; q[6] = b.z; q[7] = b.w;
+        }
+        __syncthreads();
+        double* pb = part + (nred & 1) * kMaxTeam * kSlots;
+        ++nred;
+        if (threadIdx.x < kSlots) {
+            double v = 0.0;
+            for (int w = 0; w < kWaves; ++w) v += lds[kSlots * w + threadIdx.x];
+            pb[kSlots * mem + threadIdx.x] = v;
+        }
+        sync();
+        double4 s = make_double4(0.0, 0.0, 0.0, 0.0), t = make_double4(0.0, 0.0, 0.0, 0.0);
+        for (int k = 0; k < m; ++k) {
+            const double* q = pb + kSlots * k;
+            s.x += q[0]; s.y += q[1]; s.z += q[2]; s.w += q[3];
+            t.x += q[4]; t.y += q[5]; t.z += q[6]; t.w += q[7];
+        }
+        sa = uniform_d4(s);
+        sb = uniform_d4(t);
+    }
+    __device__ double4 sum4(double4 v) {
+        double4 s, unused;
+        sum8(v, make_double4(0.0, 0.0, 0.0, 0.0), s, unused);
+        return s;
+    }
+    __device__ double min1(double v) {
+        v = wave_min(v);
+        if (lane() == 0) lds[kSlots * wave()] = v;
+        __syncthreads();
+        double* pb = part + (nred & 1) * kMaxTeam * kSlots;
+        ++nred;
+        if (threadIdx.x == 0) {
+            double mn = lds[0];
+            for (int w = 1; w < kWaves; ++w) mn = fmin(mn, lds[kSlots * w]);
+            pb[kSlots * mem] = mn;
+        }
+        sync();
+        double mn = pb[0];
+        for (int k = 1; k < m; ++k) mn = fmin(mn, pb[kSlots * k]);
+        return uniform_d(mn);
+    }
+};
+
 // ---------------------------------------------------------------- island view
 struct Isl {
     int v0, nv, t0, nt, s0, ns, pin;
@@ -148,11 +253,17 @@ __device__ __forceinline__ Isl load_isl(const DevPlan& P, int i) {
 }
 
 // Iterate the rows owned by this lane: wavefront w of the team takes slices
-// w, w + W, ...; lane l takes row l of the slice.
-#define FOR_ROWS(tm, d, row, sg)                                                           \
-    for (int _s = (tm).wave(); _s < (d).ns; _s += (tm).nwaves())                            \
-        for (int row = (d).v0 + _s * kSlice + (tm).lane(), sg = (d).s0 + _s, _once = 1;     \
+// lo+w, lo+w+W, ... of its slice range; lane l takes row l of the slice.
+#define FOR_ROWS(tm, d, row, sg)                                                                   \
+    for (int _s = (tm).sl_lo((d).ns) + (tm).wave(), _se = (tm).sl_hi((d).ns); _s < _se;            \
+         _s += (tm).nwaves())                                                                      \
+        for (int row = (d).v0 + _s * kSlice + (tm).lane(), sg = (d).s0 + _s, _once = 1;             \
              _once && (_s * kSlice + (tm).lane() < (d).nv); _once = 0)
+
+// Iterate the triangles owned by this thread.
+#define FOR_TRIS(tm, d, f)                                                                         \
+    for (int f = (d).t0 + (tm).tri_lo((d).nt) + (tm).tid(), _fe = (d).t0 + (tm).tri_hi((d).nt);   \
+         f < _fe; f += (tm).tstride())
 
 template <class T>
 struct Vec2;
@@ -311,7 +422,7 @@ template <class Team>
 __device__ double local_step(Team& tm, const DevPlan& P, const Isl& d, const double2* __restrict__ U, bool batch,
                              bool want_energy) {
     double en = 0.0;
-    for (int f = d.t0 + tm.rank(); f < d.t0 + d.nt; f += tm.size()) {
+    FOR_TRIS(tm, d, f) {
         const bool ok = P.fok[f];
         double ca = 1.0, cb = 0.0;
         if (ok || batch) {
@@ -369,7 +480,7 @@ template <class Team>
 __device__ double max_step_island(Team& tm, const DevPlan& P, const Isl& d, const double2* __restrict__ U,
                                   const double2* __restrict__ Xn, bool batch) {
     double best = 1e30;
-    for (int f = d.t0 + tm.rank(); f < d.t0 + d.nt; f += tm.size()) {
+    FOR_TRIS(tm, d, f) {
         const int a = P.tv0[f], b = P.tv1[f], c = P.tv2[f];
         const double2 ua = U[a], ub = U[b], uc = U[c];
         const double2 xa = Xn[a], xb = Xn[b], xc = Xn[c];
@@ -426,10 +537,10 @@ __device__ void run_task(Team& tm, const DevPlan& P, int isl, const TaskArgs& a)
     if (KIND == TASK_PROLOGUE) {
         // flips of the start, line-search decision, rotations and energy of U
         double nflip = 0.0;
-        for (int f = d.t0 + tm.rank(); f < d.t0 + d.nt; f += tm.size()) nflip += signed_area(P, f, U) <= 0.0 ? 1.0 : 0.0;
+        FOR_TRIS(tm, d, f) nflip += signed_area(P, f, U) <= 0.0 ? 1.0 : 0.0;
         const double fl = tm.sum4(make_double4(nflip, 0.0, 0.0, 0.0)).x;
         const double en = local_step(tm, P, d, U, a.batch, !a.batch);
-        if (tm.rank() == 0) {
+        if (tm.leader()) {
             st->flips0 = (int)fl;
             st->use_ls = a.line_search && (a.batch || fl == 0.0);
             st->e_prev = en;
@@ -442,9 +553,11 @@ __device__ void run_task(Team& tm, const DevPlan& P, int isl, const TaskArgs& a)
             st->step = 0.0;
             st->tmax = 1e30;
             st->status = 0;
+            st->ticks = 0;
         }
     } else if (KIND == TASK_ARAP_ITER) {
         if (uniform_i(st->done)) return;
+        const long long t_start = wall_clock64();
         const double2 b2 = rhs_step(tm, P, d, U);
         const CGOut cg = cg_island<PREC, OP_ARAP>(tm, P, d, b2, a.cg_tol, a.cg_max, -1, -1);
         if (a.batch) {
@@ -481,7 +594,7 @@ __device__ void run_task(Team& tm, const DevPlan& P, int isl, const TaskArgs& a)
         }
         tm.sync();
         const double en = local_step(tm, P, d, U, a.batch, !a.batch);
-        if (tm.rank() == 0) {
+        if (tm.leader()) {
             st->cg_iters += cg.iters;
             st->cg_iters_max = max(st->cg_iters_max, cg.iters);
             st->arap_iters += 1;
@@ -489,6 +602,7 @@ __device__ void run_task(Team& tm, const DevPlan& P, int isl, const TaskArgs& a)
             st->rel_resid = cg.rel;
             st->step = step;
             st->tmax = tmax;
+            st->ticks += wall_clock64() - t_start;
             if (!a.batch) {
                 const double prev = st->e_prev;
                 if (fabs(prev - en) <= a.tol * fmax(prev, 1e-300)) st->done = 1;
@@ -498,7 +612,7 @@ __device__ void run_task(Team& tm, const DevPlan& P, int isl, const TaskArgs& a)
     } else if (KIND == TASK_LSCM) {
         const int p0 = uniform_i(P.isl_lpin0[isl]), p1 = uniform_i(P.isl_lpin1[isl]);
         if (p0 < 0) {
-            if (tm.rank() == 0) st->status = -5;
+            if (tm.leader()) st->status = -5;
             return;
         }
         const double dist = uniform_d(P.isl_ldist[isl]);
@@ -530,7 +644,7 @@ __device__ void run_task(Team& tm, const DevPlan& P, int isl, const TaskArgs& a)
             if (row == p1) x = make_double2(dist, 0.0);
             U[row] = x;
         }
-        if (tm.rank() == 0) {
+        if (tm.leader()) {
             st->cg_iters = cg.iters;
             st->cg_iters_max = cg.iters;
             st->converged = cg.conv;
@@ -563,7 +677,7 @@ __device__ void run_task(Team& tm, const DevPlan& P, int isl, const TaskArgs& a)
             (void)sg;
             U[row] = P.X64[row];
         }
-        if (tm.rank() == 0) {
+        if (tm.leader()) {
             st->cg_iters = cg.iters;
             st->cg_iters_max = cg.iters;
             st->converged = cg.conv;
@@ -579,16 +693,16 @@ __device__ void run_task(Team& tm, const DevPlan& P, int isl, const TaskArgs& a)
         }
         tm.sync();
         const double t = max_step_island(tm, P, d, U, P.X64, a.batch);
-        if (tm.rank() == 0) a.out_isl[isl] = t;
+        if (tm.leader()) a.out_isl[isl] = t;
     } else if (KIND == TASK_ENERGY) {
         double en = 0.0;
-        for (int f = d.t0 + tm.rank(); f < d.t0 + d.nt; f += tm.size()) {
+        FOR_TRIS(tm, d, f) {
             if (!P.fok[f]) continue;
             const Jac J = jacobian(P, f, U);
             en += a.kind == 0 ? arap_energy_tri(J, P.farea[f]) : sd_energy_tri(J, P.farea[f]);
         }
         const double s = tm.sum4(make_double4(en, 0.0, 0.0, 0.0)).x;
-        if (tm.rank() == 0) a.out_isl[isl] = s;
+        if (tm.leader()) a.out_isl[isl] = s;
     } else if (KIND == TASK_REPAIR) {
         // Jacobi 1-ring relaxation of the free vertices of folded thick triangles
         // (gpu.py:251-273); the batch-wide "no fold left" stop is per island here,
@@ -601,7 +715,7 @@ __device__ void run_task(Team& tm, const DevPlan& P, int isl, const TaskArgs& a)
         tm.sync();
         for (int round = 0; round < a.rounds; ++round) {
             double nbad = 0.0;
-            for (int f = d.t0 + tm.rank(); f < d.t0 + d.nt; f += tm.size()) {
+            FOR_TRIS(tm, d, f) {
                 if (signed_area(P, f, U) < 0.0 && P.fok[f]) {
                     nbad += 1.0;
                     mark[P.tv0[f]] = 1;
@@ -636,7 +750,7 @@ __device__ void run_task(Team& tm, const DevPlan& P, int isl, const TaskArgs& a)
         }
     } else if (KIND == TASK_ORIENT) {
         double s = 0.0;
-        for (int f = d.t0 + tm.rank(); f < d.t0 + d.nt; f += tm.size()) {
+        FOR_TRIS(tm, d, f) {
             const double sa = signed_area(P, f, U);
             const double sg = sa > 0.0 ? 1.0 : (sa < 0.0 ? -1.0 : 0.0);
             s += sg * P.farea[f];
@@ -652,10 +766,10 @@ __device__ void run_task(Team& tm, const DevPlan& P, int isl, const TaskArgs& a)
         tm.sync();
     } else if (KIND == TASK_FLIPS) {
         double c = 0.0;
-        for (int f = d.t0 + tm.rank(); f < d.t0 + d.nt; f += tm.size())
+        FOR_TRIS(tm, d, f)
             c += (signed_area(P, f, U) < 0.0 && P.fok[f]) ? 1.0 : 0.0;
         const double s = tm.sum4(make_double4(c, 0.0, 0.0, 0.0)).x;
-        if (tm.rank() == 0) a.out_isl[isl] = s;
+        if (tm.leader()) a.out_isl[isl] = s;
     }
 }
 
@@ -670,7 +784,17 @@ __global__ __launch_bounds__(BS) void island_kernel(DevPlan P, TaskArgs a, int s
         const int task = uniform_i(s_task);
         __syncthreads();
         if (task >= P.n_group) break;
-        run_task<KIND, PREC>(bt, P, uniform_i(P.order_group[task]), a);
+        const int isl = uniform_i(P.order_group[task]);
+        const int m = uniform_i(P.grp_m[task]);
+        if (m <= 1) {
+            run_task<KIND, PREC>(bt, P, isl, a);
+        } else {
+            const int team = uniform_i(P.grp_team[task]);
+            TeamWG<BS> tw(red, P.team_cnt + slot * P.n_teams + team,
+                          P.team_part + (size_t)team * 2 * kMaxTeam * TeamWG<BS>::kSlots, m,
+                          uniform_i(P.grp_mem[task]));
+            run_task<KIND, PREC>(tw, P, isl, a);
+        }
         __syncthreads();
     }
     WaveTeam wt;

@@ -19,6 +19,8 @@ namespace uva {
 
 constexpr int kSlice = 64;          // rows per slice (= wavefront width)
 constexpr int kWaveMaxRows = 1536;  // islands up to this many vertices run on one wavefront
+constexpr int kMaxTeam = 16;        // most workgroups on one island
+constexpr int kTeamMinRows = 2048;  // fewest rows per team member
 
 struct IslandState {
     double e_prev;       // ARAP energy of the current layout
@@ -33,6 +35,7 @@ struct IslandState {
     int converged;
     int flips0;
     int status;
+    long long ticks;     // device wall-clock ticks spent on this island in the call
 };
 
 struct DevPlan {
@@ -89,9 +92,15 @@ struct DevPlan {
     double* red;                 // per-island scalars for multi-launch paths
     IslandState* st;
     int* queue;                  // task counters
-    const int* order_group;      // island ids, most expensive first
+    const int* order_group;      // group tasks: island ids, longest first
+    const int* grp_mem;          // member index of the task in its team
+    const int* grp_m;            // team size (1 = one workgroup, no cross-workgroup barrier)
+    const int* grp_team;         // team slot (teams of two or more)
     const int* order_wave;
     int n_group, n_wave;
+    int n_teams;
+    int* team_cnt;               // [launch slot][team] barrier counters, zeroed per call sequence
+    double* team_part;           // [team][2][kMaxTeam][8] partial sums
 };
 
 }  // namespace uva

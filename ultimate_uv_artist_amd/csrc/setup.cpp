@@ -262,7 +262,7 @@ void parallel_for(int n, F fn) {
 std::string build_host_plan(HostPlan& hp, int64_t n_islands, const int64_t* vert_off, const int64_t* tri_off,
                             const double* P, const int64_t* T, const int64_t* pins, const double* X,
                             const double* area, const double* cots, bool batch, const int64_t* lscm_pins,
-                            int wave_max_rows) {
+                            int wave_max_rows, int team_slots) {
     if (n_islands <= 0) return "n_islands must be positive";
     if (vert_off[0] != 0 || tri_off[0] != 0) return "offsets must start at 0";
     for (int64_t i = 0; i < n_islands; ++i) {
@@ -414,16 +414,47 @@ std::string build_host_plan(HostPlan& hp, int64_t n_islands, const int64_t* vert
         }
     });
     // task order: most expensive first (cost ~ rows^1.5 for Jacobi-PCG on a 2-D Laplacian)
-    std::vector<std::pair<double, int>> g, wv;
+    std::vector<std::pair<double, int>> wv;
+    std::vector<int> g;
     for (int i = 0; i < n; ++i) {
-        double cost = std::pow((double)isl[i].nv, 1.5);
-        (isl[i].nv > wave_max_rows ? g : wv).push_back({-cost, i});
+        if (isl[i].nv > wave_max_rows) g.push_back(i);
+        else wv.push_back({-std::pow((double)isl[i].nv, 1.5), i});
     }
-    std::sort(g.begin(), g.end());
     std::sort(wv.begin(), wv.end());
-    for (auto& p : g) hp.order_group.push_back(p.second);
     for (auto& p : wv) hp.order_wave.push_back(p.second);
+    plan_teams(hp, g, team_slots);
     return "";
+}
+
+void plan_teams(HostPlan& hp, const std::vector<int>& group_islands, int team_slots) {
+    hp.order_group.clear(); hp.grp_mem.clear(); hp.grp_m.clear(); hp.grp_team.clear();
+    hp.n_teams = 0;
+    auto cost = [&](int i) { return std::pow((double)hp.isl_nv[i], 1.5); };
+    double total = 0.0;
+    for (int i = 0; i < hp.n_isl; ++i) total += cost(i);
+    // an island longer than the ideal per-workgroup share gets more workgroups
+    const double share = team_slots > 0 ? total / team_slots : 0.0;
+    std::vector<std::pair<double, std::pair<int, int>>> order;   // (-time per member, (island, m))
+    for (int i : group_islands) {
+        int m = 1;
+        if (share > 0.0) {
+            m = (int)std::ceil(cost(i) / share);
+            m = std::min(m, std::min(kMaxTeam, std::max(1, hp.isl_nv[i] / kTeamMinRows)));
+            m = std::max(m, 1);
+        }
+        order.push_back({-cost(i) / m, {i, m}});
+    }
+    std::sort(order.begin(), order.end());
+    for (auto& o : order) {
+        const int i = o.second.first, m = o.second.second;
+        const int team = m > 1 ? hp.n_teams++ : -1;
+        for (int k = 0; k < m; ++k) {
+            hp.order_group.push_back(i);
+            hp.grp_mem.push_back(k);
+            hp.grp_m.push_back(m);
+            hp.grp_team.push_back(team);
+        }
+    }
 }
 
 std::string build_matrix_plan(HostPlan& hp, int64_t n, const int64_t* indptr, const int64_t* indices,
@@ -508,7 +539,8 @@ std::string build_matrix_plan(HostPlan& hp, int64_t n, const int64_t* indptr, co
     for (int i = 0; i < nv; ++i)
         for (auto& e : rows[i]) hp.adj_idx.push_back(e.first);
     hp.inc_ptr.assign(nv + 1, 0);
-    (nv > kWaveMaxRows ? hp.order_group : hp.order_wave).push_back(0);
+    if (nv > kWaveMaxRows) plan_teams(hp, {0}, 0);
+    else hp.order_wave.push_back(0);
     return "";
 }
 

@@ -27,6 +27,11 @@ struct HostPlan {
     std::vector<int> adj_ptr, adj_idx;
     std::vector<unsigned char> vbnd;
     std::vector<int> order_group, order_wave;
+    // group tasks: island order_group[k] is solved by member grp_mem[k] of a
+    // team of grp_m[k] workgroups; teams with more than one member have a
+    // slot grp_team[k] for their barrier counter and partial sums
+    std::vector<int> grp_mem, grp_m, grp_team;
+    int n_teams = 0;
     int64_t nnz_true = 0;
 };
 
@@ -34,7 +39,12 @@ struct HostPlan {
 std::string build_host_plan(HostPlan& hp, int64_t n_islands, const int64_t* vert_off, const int64_t* tri_off,
                             const double* P, const int64_t* T, const int64_t* pins, const double* X,
                             const double* area, const double* cots, bool batch, const int64_t* lscm_pins,
-                            int wave_max_rows = kWaveMaxRows);
+                            int wave_max_rows = kWaveMaxRows, int team_slots = 0);
+
+// Group-task order and team sizes: islands whose cost exceeds their share of
+// `team_slots` concurrent workgroups get several workgroups (at most kMaxTeam,
+// at least kTeamMinRows rows each).  team_slots = 0 gives every island one.
+void plan_teams(HostPlan& hp, const std::vector<int>& group_islands, int team_slots);
 
 // One island holding a general symmetric matrix (no triangles, no pin).
 std::string build_matrix_plan(HostPlan& hp, int64_t n, const int64_t* indptr, const int64_t* indices,

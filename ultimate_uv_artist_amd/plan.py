@@ -54,7 +54,7 @@ class IslandPlan:
     batch   gpu.Batch conventions instead of param.refine conventions
     """
 
-    def __init__(self, charts, pins=None, frames=None, batch=False, dev=None, no_wave=False):
+    def __init__(self, charts, pins=None, frames=None, batch=False, dev=None, no_wave=False, no_teams=False):
         self.dev = require_gpu(dev)
         L = _lib.lib()
         self.vert_off, self.tri_off, P, T = concat_charts(charts)
@@ -71,7 +71,8 @@ class IslandPlan:
         with torch.cuda.device(self.dev):
             rc = L.uva_plan_create(C.byref(self._h), self.n_islands, i64p(self.vert_off), i64p(self.tri_off),
                                    f64p(P), i64p(T), i64p(pins_a), f64p(X), f64p(area), f64p(cots),
-                                   (_lib.UVA_PLAN_BATCH if batch else 0) | (_lib.UVA_PLAN_NO_WAVE if no_wave else 0),
+                                   (_lib.UVA_PLAN_BATCH if batch else 0) | (_lib.UVA_PLAN_NO_WAVE if no_wave else 0)
+                                   | (_lib.UVA_PLAN_NO_TEAMS if no_teams else 0),
                                    stream_of(self.dev))
         check(rc, "uva_plan_create")
         self.T_local = T
