@@ -1,0 +1,92 @@
+"""ctypes binding of libuvart_emu.so, the CPU emulation of the island tasks
+(the kernels' run_task code compiled for the host, one thread per lane).
+Test infrastructure: checks kernel logic on machines without a GPU."""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+LIB = Path(__file__).resolve().parent.parent / "ultimate_uv_artist_amd" / "libuvart_emu.so"
+
+_i64p = C.POINTER(C.c_int64)
+_dp = C.POINTER(C.c_double)
+_lib = None
+
+
+def _stale():
+    if not LIB.exists():
+        return True
+    csrc = LIB.parent / "csrc"
+    newest = max(p.stat().st_mtime for p in csrc.iterdir() if p.suffix in (".hip", ".hpp", ".cpp", ".inc", ".h"))
+    return newest > LIB.stat().st_mtime
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        from ultimate_uv_artist_amd import _lib as L
+        from ultimate_uv_artist_amd import build
+
+        if _stale():
+            build.build_emulator()
+        _lib = C.CDLL(str(LIB))
+        _lib.uva_emu_create.restype = C.c_void_p
+        _lib.uva_emu_create.argtypes = [C.c_int64, _i64p, _i64p, _dp, _i64p, _i64p, C.c_uint32, C.c_int32, C.c_int32]
+        _lib.uva_emu_destroy.argtypes = [C.c_void_p]
+        _lib.uva_emu_lscm.argtypes = [C.c_void_p, _dp, C.c_double, C.c_int32]
+        _lib.uva_emu_arap.argtypes = [C.c_void_p, _dp, C.POINTER(L.ArapOpts)]
+        _lib.uva_emu_cg.argtypes = [C.c_void_p, _dp, _dp, C.c_double, C.c_int32, C.c_int32]
+        _lib.uva_emu_stats.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), _dp]
+    return _lib
+
+
+class EmuPlan:
+    """A batch of islands run by the CPU emulation with teams of
+    ``members`` x ``waves`` wavefronts of 64 lane threads."""
+
+    def __init__(self, charts, members=1, waves=1, batch=False, pins=None):
+        from ultimate_uv_artist_amd.plan import concat_charts
+
+        self.vo, self.to, P, T = concat_charts(charts)
+        self.n = int(self.vo[-1])
+        self.k = len(charts)
+        pa = None if pins is None else np.ascontiguousarray(np.asarray(pins, np.int64))
+        self.h = lib().uva_emu_create(self.k, self.vo.ctypes.data_as(_i64p), self.to.ctypes.data_as(_i64p),
+                                      P.ctypes.data_as(_dp), T.ctypes.data_as(_i64p),
+                                      None if pa is None else pa.ctypes.data_as(_i64p), 1 if batch else 0,
+                                      members, waves)
+        assert self.h, "emulator plan creation failed"
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().uva_emu_destroy(self.h)
+
+    def lscm(self, tol=1e-8, max_iters=0):
+        U = np.zeros((self.n, 2))
+        rc = lib().uva_emu_lscm(self.h, U.ctypes.data_as(_dp), tol, max_iters)
+        if rc != 0:
+            raise ValueError("island without boundary")
+        return U
+
+    def arap(self, U, iters, line_search=True, batch=False, tol=0.0, cg_tol=1e-6, cg_max_iters=0, precision=0):
+        from ultimate_uv_artist_amd import _lib as L
+
+        U = np.ascontiguousarray(np.array(U, np.float64))
+        o = L.ArapOpts(iters, int(line_search), 1 if batch else 0, precision, tol, cg_tol, cg_max_iters, 0)
+        lib().uva_emu_arap(self.h, U.ctypes.data_as(_dp), C.byref(o))
+        return U
+
+    def cg(self, b, x0, tol=1e-6, max_iters=0, precision=0):
+        b = np.ascontiguousarray(np.array(b, np.float64))
+        x = np.ascontiguousarray(np.array(x0, np.float64))
+        lib().uva_emu_cg(self.h, b.ctypes.data_as(_dp), x.ctypes.data_as(_dp), tol, max_iters, precision)
+        return x
+
+    def stats(self):
+        it = (C.c_int32 * self.k)()
+        cv = (C.c_int32 * self.k)()
+        rr = (C.c_double * self.k)()
+        lib().uva_emu_stats(self.h, it, cv, rr)
+        return {"cg_iters": np.array(it[:]), "converged": np.array(cv[:]), "rel_resid": np.array(rr[:])}

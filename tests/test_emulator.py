@@ -1,0 +1,114 @@
+"""The island tasks' own code (run_task, cg_island, the team interface) run on
+the CPU by the emulator, one thread per lane, against the oracle.
+
+This covers the kernel logic -- recurrences, per-member slice and triangle
+ranges, barrier placement, reductions -- for team shapes the GPU uses
+(one wavefront, one workgroup of several wavefronts, several workgroups),
+on machines without a GPU.  Tolerances are the GPU parity tests' own.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from emu import EmuPlan
+
+# (members, wavefronts per member): a wavefront team, a workgroup team, and
+# multi-workgroup teams (3 members leave uneven slice ranges; 5 members on the
+# smallest island leave some members without slices)
+SHAPES = [(1, 1), (1, 4), (3, 2), (5, 1)]
+
+
+def unit_box_err(U, Uref):
+    lo = Uref.min(0)
+    s = float((Uref.max(0) - lo).max())
+    return float(np.abs((U - lo) / s - (Uref - lo) / s).max())
+
+
+@pytest.fixture(scope="module")
+def disks():
+    from ultimate_uv_artist_amd import workloads
+
+    rng = np.random.default_rng(5)
+    return [workloads.disk_island(rng, t) for t in (180, 700, 1500)]
+
+
+def _pinned_laplacian(P, T):
+    import scipy.sparse as sp
+
+    from oracle import param_cpu as oracle
+
+    X, _ = oracle.local_frames(P, T)
+    I, J, W = oracle.lap_triplets(T, oracle.cot_weights(X))
+    n = len(P)
+    return sp.csc_matrix((np.r_[W, np.abs(W).max()], (np.r_[I, 0], np.r_[J, 0])), shape=(n, n))
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("precision", [0, 2])
+def test_emu_cg_vs_direct(disks, shape, precision):
+    """Mixed (FP32 inner, FP64 residual) and FP64 CG against SuperLU."""
+    import scipy.sparse.linalg as spla
+
+    e = EmuPlan(disks, *shape)
+    rng = np.random.default_rng(7)
+    b = rng.normal(size=(e.n, 2))
+    for i in range(e.k):
+        s = slice(e.vo[i], e.vo[i + 1])
+        b[s] -= b[s].mean(0)
+    tol = 1e-10 if precision == 2 else 1e-6
+    x = e.cg(b, np.zeros_like(b), tol=tol, precision=precision)
+    st = e.stats()
+    for i, (P, T) in enumerate(disks):
+        s = slice(e.vo[i], e.vo[i + 1])
+        L = _pinned_laplacian(P, T)
+        xr = spla.splu(L).solve(b[s])
+        r = np.linalg.norm(b[s] - L @ x[s], axis=0) / np.linalg.norm(b[s], axis=0)
+        assert st["converged"][i] == 1
+        assert r.max() <= tol * 1.0001
+        assert abs(st["rel_resid"][i] - r.max()) <= 1e-3 * r.max() + 1e-15
+        assert np.abs(x[s] - xr).max() <= (1e-6 if precision == 2 else 1e-2) * np.abs(xr).max()
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_emu_lscm_vs_oracle(disks, shape):
+    from oracle import param_cpu as oracle
+
+    e = EmuPlan(disks, *shape)
+    U = e.lscm(tol=1e-10)
+    assert e.stats()["converged"].all()
+    for i, (P, T) in enumerate(disks):
+        s = slice(e.vo[i], e.vo[i + 1])
+        assert unit_box_err(U[s], oracle.lscm(P, T)) <= 1e-7
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_emu_arap_refine_vs_oracle(disks, shape):
+    """param.refine semantics: soft pin, flip-safe line search, 4 iterations."""
+    from oracle import param_cpu as oracle
+
+    e = EmuPlan(disks, *shape)
+    U0 = np.concatenate([oracle.lscm(P, T) for P, T in disks])
+    U = e.arap(U0, 4, tol=0.0, cg_tol=1e-8)
+    assert e.stats()["converged"].all()
+    for i, (P, T) in enumerate(disks):
+        s = slice(e.vo[i], e.vo[i + 1])
+        Ur, _ = oracle.refine(P, T, U0[s], iters=4, tol=0.0)
+        assert unit_box_err(U[s], Ur) <= 1e-5
+
+
+def test_emu_arap_batch_mode_vs_batch_oracle(disks):
+    """gpu.Batch.arap semantics (free system, per-island frames)."""
+    from oracle import param_cpu as oracle
+    from oracle.batch_cpu import BatchOracle
+
+    items = [(P, T, oracle.lscm(P, T)) for P, T in disks]
+    r = BatchOracle(items)
+    r.orient_positive()
+    U0 = r.U.copy()
+    e = EmuPlan(disks, 3, 2, batch=True)
+    U = e.arap(U0, 3, line_search=True, batch=True, cg_tol=1e-8)
+    r.arap(3, line_search=True)
+    for i in range(len(disks)):
+        s = slice(e.vo[i], e.vo[i + 1])
+        assert unit_box_err(U[s], r.U[s]) <= 1e-4

@@ -40,5 +40,26 @@ def build(verbose=False):
     return LIB
 
 
+EMU_LIB = HERE / "libuvart_emu.so"
+
+
+def build_emulator(verbose=False):
+    """libuvart_emu.so: the island tasks compiled for the host with one thread
+    per lane (test-only CPU emulation of the kernels; no GPU code inside)."""
+    tmp = EMU_LIB.with_suffix(".so.tmp")
+    cmd = [hipcc(), "--cuda-host-only", "-O2", "-std=c++17", "-fPIC", "-shared", "-DUVA_EMULATOR", "-pthread",
+           "-Wno-unused-command-line-argument", "-I", str(HERE.parent / "include"),
+           str(CSRC / "kernels.hip"), str(CSRC / "setup.cpp"), "-o", str(tmp)]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("hipcc failed building libuvart_emu.so")
+    os.replace(tmp, EMU_LIB)
+    return EMU_LIB
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_emulator(verbose=True))

@@ -209,10 +209,12 @@ int upload_plan(uva::HostPlan& hp, const double* P, hipStream_t s, uva_plan** ou
         d.b64 = p2;
         if ((e = pl->alloc(&p2, nv)) != hipSuccess) return bail(UVA_ENOMEM, "X64");
         d.X64 = p2;
-        void** vecs[8] = {&d.vr, &d.vp, &d.vAp, &d.ve, &d.vu, &d.vw, &d.vm0, &d.vm1};
+        void** vecs[6] = {&d.vr, &d.vp, &d.vAp, &d.ve, &d.vu, &d.vw};
         for (void** v : vecs) {
             double2* q = nullptr;
             if ((e = pl->alloc(&q, nv)) != hipSuccess) return bail(UVA_ENOMEM, "CG vectors");
+            // defined contents from the start (plan creation, never inside a graph)
+            HIP_TRY(hipMemsetAsync(q, 0, std::max<size_t>(nv, 1) * sizeof(double2), s));
             *v = q;
         }
         double* red = nullptr;

@@ -1,22 +1,21 @@
 // Island-level preconditioned conjugate gradients (included by kernels.hip).
 //
-// Pipelined Jacobi-PCG (Ghysels & Vanroose): every iteration is ONE pass over
-// the island's rows and ONE team barrier.  The pass computes the SpMV
-// n = A m of the vector published by the previous barrier and, row by row,
-// all recurrences with the scalars of that barrier's reduction:
-//   z = n + beta z,  s = w + beta s,  p = u + beta p,
-//   e += alpha p,  r -= alpha s,  w -= alpha z,  m' = D^-1 w,
-//   gamma' = r.u', delta' = w.u', r.r   (u = D^-1 r computed on the fly)
-//   beta = gamma'/gamma,  alpha = gamma' / (delta' - beta gamma'/alpha)
-// The barrier both reduces (gamma', delta', r.r) for both columns and
-// publishes m' (double-buffered, so the pass never overwrites the vector
-// other rows are still gathering).
+// Chronopoulos-Gear form of Jacobi-PCG: each iteration is one pass of vector
+// updates over the island's rows, one barrier, one SpMV pass that also forms
+// the dot products, and ONE team reduction (r.r, r.z, w.z for both columns),
+// instead of three passes and two reductions.  With u = D^-1 r, w = A u:
+//   p = u + beta p,  s = w + beta s (= A p),  e += alpha p,  r -= alpha s,
+//   u = D^-1 r,  w = A u,  gamma' = r.u,  delta' = w.u,
+//   beta = gamma'/gamma,  alpha = gamma' / (delta' - beta gamma'/alpha).
+// (The pipelined Ghysels-Vanroose form, one barrier per iteration, amplifies
+// FP32 rounding in its extra recurrences: on cotangent Laplacians of a few
+// thousand rows it needs 10-100x the iterations or diverges, so it is not
+// used with FP32 vectors.)
 // Mixed precision: the inner iterations work in FP32 on the correction e
 // with FP64 dots; every outer pass recomputes the true residual b - A X in
-// FP64 (hi+lo matrix values, FP64 iterate), restarts the recurrences and
-// folds e into X (iterative refinement, which also bounds the pipelined
-// recurrences' drift).  Each lane handles two rows per step (slices s and
-// s + W) so two rows' loads are in flight together.
+// FP64 (hi+lo matrix values, FP64 iterate) and folds e into X (iterative
+// refinement).  Each lane handles two rows per step (slices s and s + W) so
+// two rows' loads are in flight together.
 #pragma once
 
 struct CGOut {
@@ -29,7 +28,7 @@ struct CGOut {
 // lane's wavefront.  okA/okB say whether each row exists; sbv says whether
 // slice B exists (uniform across the wavefront).
 template <class Team, class F>
-__device__ __forceinline__ void for_row_pairs(const Team& tm, const Isl& d, F&& f) {
+UVA_HD __forceinline__ void for_row_pairs(const Team& tm, const Isl& d, F&& f) {
     const int nw = tm.nwaves(), lane = tm.lane();
     const int hi = tm.sl_hi(d.ns);
     for (int s = tm.sl_lo(d.ns) + tm.wave(); s < hi; s += 2 * nw) {
@@ -43,7 +42,7 @@ __device__ __forceinline__ void for_row_pairs(const Team& tm, const Isl& d, F&& 
 
 // y = A x for the two rows of a pair (see spmv_row); absent rows give 0.
 template <typename V, int OP>
-__device__ __forceinline__ void spmv_row2(const DevPlan& P, int ra, int sa, bool oka, int rb, int sb, bool sbv,
+UVA_HD __forceinline__ void spmv_row2(const DevPlan& P, int ra, int sa, bool oka, int rb, int sb, bool sbv,
                                           bool okb, int lane, const typename Vec2<V>::type* __restrict__ x, V dga,
                                           V dgb, typename Vec2<V>::type& ya, typename Vec2<V>::type& yb) {
     using V2 = typename Vec2<V>::type;
@@ -76,20 +75,24 @@ __device__ __forceinline__ void spmv_row2(const DevPlan& P, int ra, int sa, bool
             ba[q] = (OP == OP_LSCM && ia) ? (V)P.m_b[oa] : (V)0;
             bb[q] = (OP == OP_LSCM && ib) ? (V)P.m_b[ob] : (V)0;
         }
+        // entries past a row's width gather row 0 and are left out by select
+        // (a multiply by zero would let a non-finite value of row 0 through)
 #pragma unroll
         for (int q = 0; q < kGather; ++q) {
+            const bool ia = oka && k0 + q < wA;
+            const bool ib = okb && k0 + q < wB;
             const V2 ga = x[ca[q]];
             const V2 gb = x[cb[q]];
             if (OP == OP_ARAP) {
-                yax = fma(va[q], ga.x, yax);
-                yay = fma(va[q], ga.y, yay);
-                ybx = fma(vb[q], gb.x, ybx);
-                yby = fma(vb[q], gb.y, yby);
+                yax = ia ? fma(va[q], ga.x, yax) : yax;
+                yay = ia ? fma(va[q], ga.y, yay) : yay;
+                ybx = ib ? fma(vb[q], gb.x, ybx) : ybx;
+                yby = ib ? fma(vb[q], gb.y, yby) : yby;
             } else {
-                yax = fma(va[q], ga.x, fma(ba[q], ga.y, yax));
-                yay = fma(va[q], ga.y, -fma(ba[q], ga.x, -yay));
-                ybx = fma(vb[q], gb.x, fma(bb[q], gb.y, ybx));
-                yby = fma(vb[q], gb.y, -fma(bb[q], gb.x, -yby));
+                yax = ia ? fma(va[q], ga.x, fma(ba[q], ga.y, yax)) : yax;
+                yay = ia ? fma(va[q], ga.y, -fma(ba[q], ga.x, -yay)) : yay;
+                ybx = ib ? fma(vb[q], gb.x, fma(bb[q], gb.y, ybx)) : ybx;
+                yby = ib ? fma(vb[q], gb.y, -fma(bb[q], gb.x, -yby)) : yby;
             }
         }
     }
@@ -98,7 +101,7 @@ __device__ __forceinline__ void spmv_row2(const DevPlan& P, int ra, int sa, bool
 }
 
 template <int PREC, int OP, class Team>
-__device__ CGOut cg_island(Team& tm, const DevPlan& P, const Isl& d, double2 b2, double tol, int maxit, int lp0,
+UVA_HD CGOut cg_island(Team& tm, const DevPlan& P, const Isl& d, double2 b2, double tol, int maxit, int lp0,
                            int lp1) {
     using PT = Prec<PREC>;
     using V = typename PT::V;
@@ -107,23 +110,19 @@ __device__ CGOut cg_island(Team& tm, const DevPlan& P, const Isl& d, double2 b2,
     V2* __restrict__ p = (V2*)P.vp;
     V2* __restrict__ s = (V2*)P.vAp;
     V2* __restrict__ e = (V2*)P.ve;
-    V2* __restrict__ z = (V2*)P.vu;
+    V2* __restrict__ u = (V2*)P.vu;
     V2* __restrict__ w = (V2*)P.vw;
-    V2* __restrict__ mbuf[2] = {(V2*)P.vm0, (V2*)P.vm1};
     const bool coupled = (OP == OP_LSCM);
     const double bx2 = coupled ? fmax(b2.x + b2.y, 1e-300) : fmax(b2.x, 1e-300);
     const double by2 = coupled ? bx2 : fmax(b2.y, 1e-300);
     const double t2 = tol * tol;
     const int lane = tm.lane();
     auto is_pin = [&](int row) { return OP == OP_LSCM && (row == lp0 || row == lp1); };
-    V2 zero;
-    zero.x = 0;
-    zero.y = 0;
     int it = 0;
     CGOut out{0, 0, 0.0};
     double relx = 0.0, rely = 0.0;
     for (int outer = 0;; ++outer) {
-        // --- restart: r = b - A X (true residual), e = 0, u = D^-1 r (kept in z)
+        // true residual r = b - A X, u = D^-1 r, e = 0
         double4 a1 = make_double4(0.0, 0.0, 0.0, 0.0);
         FOR_ROWS(tm, d, row, sg) {
             double rx, ry;
@@ -158,7 +157,10 @@ __device__ CGOut cg_island(Team& tm, const DevPlan& P, const Isl& d, double2 b2,
             uv.x = di * rv.x;
             uv.y = di * rv.y;
             r[row] = rv;
-            z[row] = uv;
+            u[row] = uv;
+            V2 zero;
+            zero.x = 0;
+            zero.y = 0;
             e[row] = zero;
             a1.x += PT::kResid64 ? rx * rx : (double)((float)rx * (float)rx);
             a1.y += PT::kResid64 ? ry * ry : (double)((float)ry * (float)ry);
@@ -166,21 +168,26 @@ __device__ CGOut cg_island(Team& tm, const DevPlan& P, const Isl& d, double2 b2,
             a1.w += acc_mul<PREC>(rv.y, uv.y);
         }
         tm.sync();
-        // --- w = A u, m = D^-1 w, delta = w.u
         double4 a2 = make_double4(0.0, 0.0, 0.0, 0.0);
-        FOR_ROWS(tm, d, row, sg) {
-            V2 wv = spmv_row<V, OP>(P, row, sg, lane, z, (V)diag_of<OP>(P, d, row));
-            if (is_pin(row)) wv = zero;
-            const V di = (V)dinv_of<OP>(P, d, row);
-            V2 mv;
-            mv.x = di * wv.x;
-            mv.y = di * wv.y;
-            w[row] = wv;
-            mbuf[0][row] = mv;
-            const V2 uv = z[row];
-            a2.x += acc_mul<PREC>(wv.x, uv.x);
-            a2.y += acc_mul<PREC>(wv.y, uv.y);
-        }
+        for_row_pairs(tm, d, [&](int ra, int sa, bool oka, int rb, int sb, bool sbv, bool okb) {
+            V2 wa, wb;
+            spmv_row2<V, OP>(P, ra, sa, oka, rb, sb, sbv, okb, lane, u, oka ? (V)diag_of<OP>(P, d, ra) : (V)0,
+                             okb ? (V)diag_of<OP>(P, d, rb) : (V)0, wa, wb);
+            if (oka) {
+                if (is_pin(ra)) { wa.x = 0; wa.y = 0; }
+                w[ra] = wa;
+                const V2 ua = u[ra];
+                a2.x += acc_mul<PREC>(wa.x, ua.x);
+                a2.y += acc_mul<PREC>(wa.y, ua.y);
+            }
+            if (okb) {
+                if (is_pin(rb)) { wb.x = 0; wb.y = 0; }
+                w[rb] = wb;
+                const V2 ub = u[rb];
+                a2.x += acc_mul<PREC>(wb.x, ub.x);
+                a2.y += acc_mul<PREC>(wb.y, ub.y);
+            }
+        });
         double4 S1, S2;
         tm.sum8(a1, a2, S1, S2);
         if (coupled) {
@@ -190,64 +197,87 @@ __device__ CGOut cg_island(Team& tm, const DevPlan& P, const Isl& d, double2 b2,
             rely = S1.y / by2;
         }
         if (relx <= t2 && rely <= t2) { out.conv = 1; break; }
-        if (outer >= PT::kMaxOuter || it >= maxit || !isfinite(relx) || !isfinite(rely)) break;
+        if (outer >= PT::kMaxOuter || it >= maxit || !__builtin_isfinite(relx) || !__builtin_isfinite(rely)) break;
         double gx = S1.z, gy = S1.w, dx = S2.x, dy = S2.y;
         if (coupled) { gx = gy = S1.z + S1.w; dx = dy = S2.x + S2.y; }
         double alx = dx > 0.0 ? gx / dx : 0.0, aly = dy > 0.0 ? gy / dy : 0.0;
         double bex = 0.0, bey = 0.0;
         bool first = true;
-        int cur = 0;
-        // --- pipelined iterations: one pass and one barrier each
         for (;;) {
             const V ax = (V)alx, ay = (V)aly, bx = (V)bex, by = (V)bey;
-            const V2* __restrict__ mc = mbuf[cur];
-            V2* __restrict__ mn = mbuf[cur ^ 1];
-            double4 g1 = make_double4(0.0, 0.0, 0.0, 0.0), g2 = make_double4(0.0, 0.0, 0.0, 0.0);
-            auto step_row = [&](int row, const V2 nv) {
-                const V di = (V)dinv_of<OP>(P, d, row);
-                V2 rv = r[row], wv = w[row], ev = e[row], zv, sv, pv;
-                const V2 uv = V2{di * rv.x, di * rv.y};
-                if (first) {
-                    zv = nv;
-                    sv = wv;
-                    pv = uv;
-                } else {
-                    zv = z[row];
-                    sv = s[row];
-                    pv = p[row];
-                    zv.x = fma(bx, zv.x, nv.x); zv.y = fma(by, zv.y, nv.y);
-                    sv.x = fma(bx, sv.x, wv.x); sv.y = fma(by, sv.y, wv.y);
-                    pv.x = fma(bx, pv.x, uv.x); pv.y = fma(by, pv.y, uv.y);
-                }
-                ev.x = fma(ax, pv.x, ev.x); ev.y = fma(ay, pv.y, ev.y);
-                rv.x = fma(-ax, sv.x, rv.x); rv.y = fma(-ay, sv.y, rv.y);
-                wv.x = fma(-ax, zv.x, wv.x); wv.y = fma(-ay, zv.y, wv.y);
-                const V2 un = V2{di * rv.x, di * rv.y};
-                z[row] = zv; s[row] = sv; p[row] = pv; e[row] = ev; r[row] = rv; w[row] = wv;
-                mn[row] = V2{di * wv.x, di * wv.y};
-                g1.x += acc_mul<PREC>(rv.x, rv.x);
-                g1.y += acc_mul<PREC>(rv.y, rv.y);
-                g1.z += acc_mul<PREC>(rv.x, un.x);
-                g1.w += acc_mul<PREC>(rv.y, un.y);
-                g2.x += acc_mul<PREC>(wv.x, un.x);
-                g2.y += acc_mul<PREC>(wv.y, un.y);
-            };
+            double4 g1 = make_double4(0.0, 0.0, 0.0, 0.0);
             for_row_pairs(tm, d, [&](int ra, int sa, bool oka, int rb, int sb, bool sbv, bool okb) {
-                V2 na, nb;
-                spmv_row2<V, OP>(P, ra, sa, oka, rb, sb, sbv, okb, lane, mc, oka ? (V)diag_of<OP>(P, d, ra) : (V)0,
-                                 okb ? (V)diag_of<OP>(P, d, rb) : (V)0, na, nb);
-                if (oka) step_row(ra, is_pin(ra) ? zero : na);
-                if (okb) step_row(rb, is_pin(rb) ? zero : nb);
+                (void)sa; (void)sb; (void)sbv;
+                V2 uA, wA, rA, eA, pA, sA, uB, wB, rB, eB, pB, sB;
+                V dA = 0, dB = 0;
+                if (oka) {
+                    uA = u[ra]; wA = w[ra]; rA = r[ra]; eA = e[ra]; dA = (V)dinv_of<OP>(P, d, ra);
+                    if (!first) { pA = p[ra]; sA = s[ra]; }
+                }
+                if (okb) {
+                    uB = u[rb]; wB = w[rb]; rB = r[rb]; eB = e[rb]; dB = (V)dinv_of<OP>(P, d, rb);
+                    if (!first) { pB = p[rb]; sB = s[rb]; }
+                }
+                if (oka) {
+                    if (first) { pA = uA; sA = wA; }
+                    else {
+                        pA.x = fma(bx, pA.x, uA.x); pA.y = fma(by, pA.y, uA.y);
+                        sA.x = fma(bx, sA.x, wA.x); sA.y = fma(by, sA.y, wA.y);
+                    }
+                    eA.x = fma(ax, pA.x, eA.x); eA.y = fma(ay, pA.y, eA.y);
+                    rA.x = fma(-ax, sA.x, rA.x); rA.y = fma(-ay, sA.y, rA.y);
+                    uA.x = dA * rA.x; uA.y = dA * rA.y;
+                    p[ra] = pA; s[ra] = sA; e[ra] = eA; r[ra] = rA; u[ra] = uA;
+                    g1.x += acc_mul<PREC>(rA.x, rA.x);
+                    g1.y += acc_mul<PREC>(rA.y, rA.y);
+                    g1.z += acc_mul<PREC>(rA.x, uA.x);
+                    g1.w += acc_mul<PREC>(rA.y, uA.y);
+                }
+                if (okb) {
+                    if (first) { pB = uB; sB = wB; }
+                    else {
+                        pB.x = fma(bx, pB.x, uB.x); pB.y = fma(by, pB.y, uB.y);
+                        sB.x = fma(bx, sB.x, wB.x); sB.y = fma(by, sB.y, wB.y);
+                    }
+                    eB.x = fma(ax, pB.x, eB.x); eB.y = fma(ay, pB.y, eB.y);
+                    rB.x = fma(-ax, sB.x, rB.x); rB.y = fma(-ay, sB.y, rB.y);
+                    uB.x = dB * rB.x; uB.y = dB * rB.y;
+                    p[rb] = pB; s[rb] = sB; e[rb] = eB; r[rb] = rB; u[rb] = uB;
+                    g1.x += acc_mul<PREC>(rB.x, rB.x);
+                    g1.y += acc_mul<PREC>(rB.y, rB.y);
+                    g1.z += acc_mul<PREC>(rB.x, uB.x);
+                    g1.w += acc_mul<PREC>(rB.y, uB.y);
+                }
+            });
+            tm.sync();
+            double4 g2 = make_double4(0.0, 0.0, 0.0, 0.0);
+            for_row_pairs(tm, d, [&](int ra, int sa, bool oka, int rb, int sb, bool sbv, bool okb) {
+                V2 wa, wb;
+                spmv_row2<V, OP>(P, ra, sa, oka, rb, sb, sbv, okb, lane, u, oka ? (V)diag_of<OP>(P, d, ra) : (V)0,
+                                 okb ? (V)diag_of<OP>(P, d, rb) : (V)0, wa, wb);
+                if (oka) {
+                    if (is_pin(ra)) { wa.x = 0; wa.y = 0; }
+                    w[ra] = wa;
+                    const V2 ua = u[ra];
+                    g2.x += acc_mul<PREC>(wa.x, ua.x);
+                    g2.y += acc_mul<PREC>(wa.y, ua.y);
+                }
+                if (okb) {
+                    if (is_pin(rb)) { wb.x = 0; wb.y = 0; }
+                    w[rb] = wb;
+                    const V2 ub = u[rb];
+                    g2.x += acc_mul<PREC>(wb.x, ub.x);
+                    g2.y += acc_mul<PREC>(wb.y, ub.y);
+                }
             });
             double4 T1, T2;
             tm.sum8(g1, g2, T1, T2);
             ++it;
             first = false;
-            cur ^= 1;
             double rrx = T1.x, rry = T1.y;
             if (coupled) rrx = rry = T1.x + T1.y;
             if ((rrx <= PT::kInnerFrac2 * t2 * bx2 && rry <= PT::kInnerFrac2 * t2 * by2) || it >= maxit) break;
-            if (!isfinite(rrx) || !isfinite(rry)) break;   // a non-finite island stops (it is not converged)
+            if (!__builtin_isfinite(rrx) || !__builtin_isfinite(rry)) break;   // a non-finite island stops (it is not converged)
             double ngx = T1.z, ngy = T1.w, ndx = T2.x, ndy = T2.y;
             if (coupled) { ngx = ngy = T1.z + T1.w; ndx = ndy = T2.x + T2.y; }
             bex = gx > 0.0 ? ngx / gx : 0.0;

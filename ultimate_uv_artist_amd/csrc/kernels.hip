@@ -23,17 +23,40 @@ namespace uva {
 // different paths of a team loop.  (A task id broadcast with a lane shuffle
 // is a vector value to the compiler; it restructured the task loop so lanes
 // 1-63 re-entered it without lane 0's fresh task id.)
-__device__ __forceinline__ int uniform_i(int v) { return __builtin_amdgcn_readfirstlane(v); }
+// The per-island code (UVA_HD functions) also compiles for the host, where
+// the CPU emulator (emu.inc) runs it with one thread per lane; there a value
+// is already uniform and the device clock reads 0.
+#define UVA_HD __host__ __device__
 
-__device__ __forceinline__ double uniform_d(double v) {
+UVA_HD __forceinline__ int uniform_i(int v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_readfirstlane(v);
+#else
+    return v;
+#endif
+}
+
+UVA_HD __forceinline__ double uniform_d(double v) {
+#if defined(__HIP_DEVICE_COMPILE__)
     const long long b = __double_as_longlong(v);
     const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffffLL));
     const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
     return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+#else
+    return v;
+#endif
 }
 
-__device__ __forceinline__ double4 uniform_d4(double4 v) {
+UVA_HD __forceinline__ double4 uniform_d4(double4 v) {
     return make_double4(uniform_d(v.x), uniform_d(v.y), uniform_d(v.z), uniform_d(v.w));
+}
+
+UVA_HD __forceinline__ long long uva_clock() {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return wall_clock64();
+#else
+    return 0;
+#endif
 }
 
 // ---------------------------------------------------------------- reductions
@@ -243,7 +266,7 @@ struct Isl {
     double wpin;
 };
 
-__device__ __forceinline__ Isl load_isl(const DevPlan& P, int i) {
+UVA_HD __forceinline__ Isl load_isl(const DevPlan& P, int i) {
     Isl d;
     d.v0 = uniform_i(P.isl_v0[i]); d.nv = uniform_i(P.isl_nv[i]);
     d.t0 = uniform_i(P.isl_t0[i]); d.nt = uniform_i(P.isl_nt[i]);
@@ -279,16 +302,16 @@ constexpr int kGather = 8;   // matrix entries whose gathers are issued together
 
 // y = A x for one row: FP32 values (hi) for float vectors, hi + lo for double.
 template <typename V>
-__device__ __forceinline__ V mval(const DevPlan& P, int o);
+UVA_HD __forceinline__ V mval(const DevPlan& P, int o);
 template <>
-__device__ __forceinline__ float mval<float>(const DevPlan& P, int o) { return P.m_hi[o]; }
+UVA_HD __forceinline__ float mval<float>(const DevPlan& P, int o) { return P.m_hi[o]; }
 template <>
-__device__ __forceinline__ double mval<double>(const DevPlan& P, int o) {
+UVA_HD __forceinline__ double mval<double>(const DevPlan& P, int o) {
     return (double)P.m_hi[o] + (double)P.m_lo[o];
 }
 
 template <typename V, int OP>
-__device__ __forceinline__ typename Vec2<V>::type spmv_row(const DevPlan& P, int row, int sg, int lane,
+UVA_HD __forceinline__ typename Vec2<V>::type spmv_row(const DevPlan& P, int row, int sg, int lane,
                                                            const typename Vec2<V>::type* __restrict__ x, V dg) {
     const int base = P.sl_ptr[sg];
     const int width = (P.sl_ptr[sg + 1] - base) >> 6;
@@ -311,13 +334,14 @@ __device__ __forceinline__ typename Vec2<V>::type spmv_row(const DevPlan& P, int
         }
 #pragma unroll
         for (int u = 0; u < kGather; ++u) {
+            const bool in = k0 + u < width;          // padding entries are left out by select
             const typename Vec2<V>::type xc = x[c[u]];
             if (OP == OP_ARAP) {
-                yx = fma(v[u], xc.x, yx);
-                yy = fma(v[u], xc.y, yy);
+                yx = in ? fma(v[u], xc.x, yx) : yx;
+                yy = in ? fma(v[u], xc.y, yy) : yy;
             } else {
-                yx = fma(v[u], xc.x, fma(bv[u], xc.y, yx));
-                yy = fma(v[u], xc.y, -fma(bv[u], xc.x, -yy));
+                yx = in ? fma(v[u], xc.x, fma(bv[u], xc.y, yx)) : yx;
+                yy = in ? fma(v[u], xc.y, -fma(bv[u], xc.x, -yy)) : yy;
             }
         }
     }
@@ -328,14 +352,14 @@ __device__ __forceinline__ typename Vec2<V>::type spmv_row(const DevPlan& P, int
 }
 
 template <int OP>
-__device__ __forceinline__ double diag_of(const DevPlan& P, const Isl& d, int row) {
+UVA_HD __forceinline__ double diag_of(const DevPlan& P, const Isl& d, int row) {
     double g = P.diag[row];
     if (OP == OP_ARAP && row == d.pin) g += d.wpin;
     return g;
 }
 
 template <int OP>
-__device__ __forceinline__ double dinv_of(const DevPlan& P, const Isl& d, int row) {
+UVA_HD __forceinline__ double dinv_of(const DevPlan& P, const Isl& d, int row) {
     if (OP == OP_ARAP) return (double)P.dinv32[row];
     double g = P.diag[row];
     return g > 1e-30 ? 1.0 / g : 1.0;
@@ -371,12 +395,12 @@ struct Prec<UVA_PREC_F64> {
 
 // FP32-only accumulation for the all-FP32 variant, FP64 otherwise.
 template <int PREC>
-__device__ __forceinline__ double acc_mul(float a, float b) {
+UVA_HD __forceinline__ double acc_mul(float a, float b) {
     if (PREC == UVA_PREC_F32) return (double)(a * b);
     return (double)a * (double)b;
 }
 template <int PREC>
-__device__ __forceinline__ double acc_mul(double a, double b) { return a * b; }
+UVA_HD __forceinline__ double acc_mul(double a, double b) { return a * b; }
 
 #include "cg_island.hpp"
 
@@ -385,7 +409,7 @@ struct Jac {
     double j00, j01, j10, j11;
 };
 
-__device__ __forceinline__ Jac jacobian(const DevPlan& P, int f, const double2* __restrict__ U) {
+UVA_HD __forceinline__ Jac jacobian(const DevPlan& P, int f, const double2* __restrict__ U) {
     const double2 ua = U[P.tv0[f]], ub = U[P.tv1[f]], uc = U[P.tv2[f]];
     const double x1 = P.fx1[f], x2 = P.fx2[f], y2 = P.fy2[f];
     Jac J;
@@ -397,7 +421,7 @@ __device__ __forceinline__ Jac jacobian(const DevPlan& P, int f, const double2* 
 }
 
 // ARAP energy of one triangle (param.py:306-311): signed singular values q +- r.
-__device__ __forceinline__ double arap_energy_tri(const Jac& J, double area) {
+UVA_HD __forceinline__ double arap_energy_tri(const Jac& J, double area) {
     const double qe = 0.5 * (J.j00 + J.j11), qh = 0.5 * (J.j10 - J.j01);
     const double rf = 0.5 * (J.j00 - J.j11), rg = 0.5 * (J.j10 + J.j01);
     const double q = sqrt(qe * qe + qh * qh), r = sqrt(rf * rf + rg * rg);
@@ -406,9 +430,9 @@ __device__ __forceinline__ double arap_energy_tri(const Jac& J, double area) {
 }
 
 // Symmetric Dirichlet energy of one triangle (param.py:313-318).
-__device__ __forceinline__ double sd_energy_tri(const Jac& J, double area) {
+UVA_HD __forceinline__ double sd_energy_tri(const Jac& J, double area) {
     const double dj = J.j00 * J.j11 - J.j01 * J.j10;
-    if (!(dj > 0.0)) return __longlong_as_double(0x7ff0000000000000LL);  // +inf (also for NaN)
+    if (!(dj > 0.0)) return __builtin_huge_val();  // +inf (also for NaN)
     const double fro = J.j00 * J.j00 + J.j01 * J.j01 + J.j10 * J.j10 + J.j11 * J.j11;
     return area * (fro + fro / (dj * dj) - 4.0);
 }
@@ -419,7 +443,7 @@ __device__ __forceinline__ double sd_energy_tri(const Jac& J, double area) {
 // rotation (= angle of atan2) for thick finite triangles, identity otherwise,
 // times the triangle's target scale (gpu.py:166-170).
 template <class Team>
-__device__ double local_step(Team& tm, const DevPlan& P, const Isl& d, const double2* __restrict__ U, bool batch,
+UVA_HD double local_step(Team& tm, const DevPlan& P, const Isl& d, const double2* __restrict__ U, bool batch,
                              bool want_energy) {
     double en = 0.0;
     FOR_TRIS(tm, d, f) {
@@ -430,7 +454,7 @@ __device__ double local_step(Team& tm, const DevPlan& P, const Isl& d, const dou
             const double A = J.j00 + J.j11, B = J.j10 - J.j01;
             const double h = sqrt(A * A + B * B);
             if (batch) {
-                if (ok && isfinite(h) && h > 0.0) { ca = A / h; cb = B / h; }
+                if (ok && __builtin_isfinite(h) && h > 0.0) { ca = A / h; cb = B / h; }
                 const double sc = P.fscale[f];
                 ca *= sc;
                 cb *= sc;
@@ -449,7 +473,7 @@ __device__ double local_step(Team& tm, const DevPlan& P, const Isl& d, const dou
 // rhs: b[v] = sum over incident corners of R(tri) g (param.py:457-468, gpu.py:171-181)
 // plus the soft-pin term; X64 = U (warm start).  Returns |b|^2 per column.
 template <class Team>
-__device__ double2 rhs_step(Team& tm, const DevPlan& P, const Isl& d, const double2* __restrict__ U) {
+UVA_HD double2 rhs_step(Team& tm, const DevPlan& P, const Isl& d, const double2* __restrict__ U) {
     double4 acc = make_double4(0.0, 0.0, 0.0, 0.0);
     FOR_ROWS(tm, d, row, sg) {
         (void)sg;
@@ -477,7 +501,7 @@ __device__ double2 rhs_step(Team& tm, const DevPlan& P, const Isl& d, const doub
 // Flip-preventing step along D = X64 - U (param.py:259-291 for refine;
 // gpu.py:204-227 for batch), minimum over the island's triangles.
 template <class Team>
-__device__ double max_step_island(Team& tm, const DevPlan& P, const Isl& d, const double2* __restrict__ U,
+UVA_HD double max_step_island(Team& tm, const DevPlan& P, const Isl& d, const double2* __restrict__ U,
                                   const double2* __restrict__ Xn, bool batch) {
     double best = 1e30;
     FOR_TRIS(tm, d, f) {
@@ -523,14 +547,14 @@ __device__ double max_step_island(Team& tm, const DevPlan& P, const Isl& d, cons
     return tm.min1(best);
 }
 
-__device__ __forceinline__ double signed_area(const DevPlan& P, int f, const double2* __restrict__ U) {
+UVA_HD __forceinline__ double signed_area(const DevPlan& P, int f, const double2* __restrict__ U) {
     const double2 a = U[P.tv0[f]], b = U[P.tv1[f]], c = U[P.tv2[f]];
     return 0.5 * ((b.x - a.x) * (c.y - a.y) - (c.x - a.x) * (b.y - a.y));
 }
 
 // ---------------------------------------------------------------- island tasks
 template <int KIND, int PREC, class Team>
-__device__ void run_task(Team& tm, const DevPlan& P, int isl, const TaskArgs& a) {
+UVA_HD void run_task(Team& tm, const DevPlan& P, int isl, const TaskArgs& a) {
     IslandState* st = P.st + isl;
     const Isl d = load_isl(P, isl);
     double2* U = a.U;
@@ -557,7 +581,7 @@ __device__ void run_task(Team& tm, const DevPlan& P, int isl, const TaskArgs& a)
         }
     } else if (KIND == TASK_ARAP_ITER) {
         if (uniform_i(st->done)) return;
-        const long long t_start = wall_clock64();
+        const long long t_start = uva_clock();
         const double2 b2 = rhs_step(tm, P, d, U);
         const CGOut cg = cg_island<PREC, OP_ARAP>(tm, P, d, b2, a.cg_tol, a.cg_max, -1, -1);
         if (a.batch) {
@@ -566,7 +590,7 @@ __device__ void run_task(Team& tm, const DevPlan& P, int isl, const TaskArgs& a)
             FOR_ROWS(tm, d, row, sg) {
                 (void)sg;
                 const double2 x = P.X64[row];
-                bad += (isfinite(x.x) && isfinite(x.y)) ? 0.0 : 1.0;
+                bad += (__builtin_isfinite(x.x) && __builtin_isfinite(x.y)) ? 0.0 : 1.0;
             }
             if (tm.sum4(make_double4(bad, 0.0, 0.0, 0.0)).x > 0.0) {
                 FOR_ROWS(tm, d, row, sg) {
@@ -602,7 +626,7 @@ __device__ void run_task(Team& tm, const DevPlan& P, int isl, const TaskArgs& a)
             st->rel_resid = cg.rel;
             st->step = step;
             st->tmax = tmax;
-            st->ticks += wall_clock64() - t_start;
+            st->ticks += uva_clock() - t_start;
             if (!a.batch) {
                 const double prev = st->e_prev;
                 if (fabs(prev - en) <= a.tol * fmax(prev, 1e-300)) st->done = 1;
@@ -773,6 +797,7 @@ __device__ void run_task(Team& tm, const DevPlan& P, int isl, const TaskArgs& a)
     }
 }
 
+#if !defined(UVA_EMULATOR)
 template <int BS, int KIND, int PREC>
 __global__ __launch_bounds__(BS) void island_kernel(DevPlan P, TaskArgs a, int slot) {
     __shared__ double red[2 * (BS / 64) * BlockTeam<BS>::kSlots];
@@ -875,6 +900,10 @@ hipError_t launch_signed_areas(const DevPlan& P, const double2* U, double* sa, h
     hipLaunchKernelGGL(signed_areas_kernel, dim3((P.n_tri + 255) / 256), dim3(256), 0, s, P, U, sa);
     return hipGetLastError();
 }
-
+#endif  // !UVA_EMULATOR
 
 }  // namespace uva
+
+#if defined(UVA_EMULATOR) && !defined(__HIP_DEVICE_COMPILE__)
+#include "emu.inc"
+#endif

@@ -87,10 +87,8 @@ struct DevPlan {
     void* vp;
     void* vAp;                   // s = A p in the CG recurrence
     void* ve;
-    void* vu;                    // z = A m recurrence (u = D^-1 r at a restart)
+    void* vu;                    // u = D^-1 r
     void* vw;                    // w = A u
-    void* vm0;                   // m = D^-1 w, double-buffered
-    void* vm1;
     double* red;                 // per-island scalars for multi-launch paths
     IslandState* st;
     int* queue;                  // task counters
