@@ -5,6 +5,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <memory>
 #include <string>
 #include <vector>
@@ -88,7 +90,7 @@ int launch(uva_plan* pl, int kind, int prec, const uva::TaskArgs& a, int slot, h
 }
 
 int reset_queue(uva_plan* pl, int slots, hipStream_t s) {
-    HIP_TRY(uva::launch_fill_int(pl->dp.queue, 2 * slots, 0, s));
+    HIP_TRY(uva::launch_fill_int(pl->dp.queue, uva::kQueueStride * slots, 0, s));
     if (pl->dp.n_teams > 0) HIP_TRY(uva::launch_fill_int(pl->dp.team_cnt, slots * pl->dp.n_teams, 0, s));
     return UVA_OK;
 }
@@ -131,6 +133,48 @@ bool valid_plan(const uva_plan* p) { return p != nullptr; }
 }  // namespace
 
 namespace {
+
+// XCDs of the current device as the dispatcher uses them: the distinct
+// hardware ids seen by the blocks of a probe launch, in ascending order
+// (probed once per device; n = 1 if the probe fails or sees one XCD).
+struct XccInfo {
+    int n = 1;
+    int map[16];
+};
+
+XccInfo xcc_info() {
+    static std::mutex mu;
+    static std::map<int, XccInfo> cache;
+    int dev = 0;
+    XccInfo xi;
+    for (int k = 0; k < 16; ++k) xi.map[k] = -1;
+    if (hipGetDevice(&dev) != hipSuccess) return xi;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(dev);
+    if (it != cache.end()) return it->second;
+    constexpr int kBlocks = 1024;
+    int* d = nullptr;
+    std::vector<int> h(kBlocks, -1);
+    bool ok = hipMalloc(&d, kBlocks * sizeof(int)) == hipSuccess;
+    ok = ok && uva::launch_xcc_probe(d, kBlocks, nullptr) == hipSuccess;
+    ok = ok && hipMemcpy(h.data(), d, kBlocks * sizeof(int), hipMemcpyDeviceToHost) == hipSuccess;
+    if (d) (void)hipFree(d);
+    if (ok) {
+        bool seen[16] = {false};
+        for (int v : h) {
+            if (v < 0 || v > 15) { ok = false; break; }
+            seen[v] = true;
+        }
+        int n = 0;
+        for (int k = 0; k < 16 && ok; ++k)
+            if (seen[k]) xi.map[k] = n++;
+        if (ok && n >= 1 && n <= uva::kMaxXcd) xi.n = n;
+        else for (int k = 0; k < 16; ++k) xi.map[k] = -1;
+    }
+    if (xi.n == 1) for (int k = 0; k < 16; ++k) xi.map[k] = 0;
+    cache[dev] = xi;
+    return xi;
+}
 
 // Upload a host plan and allocate the scratch of a new device plan.
 int upload_plan(uva::HostPlan& hp, const double* P, hipStream_t s, uva_plan** out) {
@@ -225,9 +269,20 @@ int upload_plan(uva::HostPlan& hp, const double* P, hipStream_t s, uva_plan** ou
         HIP_TRY(hipMemsetAsync(st, 0, hp.n_isl * sizeof(uva::IslandState), s));
         d.st = st;
         int* q = nullptr;
-        if ((e = pl->alloc(&q, 2 * (size_t)kMaxSlots)) != hipSuccess) return bail(UVA_ENOMEM, "queue");
+        if ((e = pl->alloc(&q, uva::kQueueStride * (size_t)kMaxSlots)) != hipSuccess) return bail(UVA_ENOMEM, "queue");
         d.queue = q;
+        if ((e = pl->alloc(&d.fault, 1)) != hipSuccess) return bail(UVA_ENOMEM, "fault flag");
+        HIP_TRY(hipMemsetAsync(d.fault, 0, sizeof(int), s));
     }
+    // XCD lists: the probed hardware ids map to lists 0..n_xcd-1 when the plan
+    // was built for them, else everything maps to the single list
+    d.n_xcd = hp.n_xcd;
+    d.max_m = hp.max_m;
+    for (int x = 0; x <= uva::kMaxXcd; ++x) d.grp_xoff[x] = hp.grp_xoff[std::min(x, hp.n_xcd)];
+    const XccInfo xi = xcc_info();
+    d.xcd_local = (hp.n_xcd > 1 && hp.n_xcd == xi.n) ? 1 : 0;
+    // an XCD the probe never saw has no list (its workgroups only take wavefront tasks)
+    for (int k = 0; k < 16; ++k) d.xcc_map[k] = d.xcd_local ? xi.map[k] : 0;
     pl->nnz_pad = hp.sl_ptr.empty() ? 0 : hp.sl_ptr.back();
     pl->nnz_true = hp.nnz_true;
     // grid: enough workgroups to hold every group task and the wave tasks,
@@ -237,6 +292,8 @@ int upload_plan(uva::HostPlan& hp, const double* P, hipStream_t s, uva_plan** ou
     HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     if (hipDeviceGetAttribute(&pl->clock_khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess) pl->clock_khz = 0;
     pl->cus = cus;
+    // a team barrier gives up after 20 s (hang guard only)
+    d.spin_ticks = (long long)(pl->clock_khz > 0 ? pl->clock_khz : 100000) * 20000;
     hipError_t se = hipStreamSynchronize(s);
     if (se != hipSuccess) return bail(UVA_EHIP, std::string("plan upload: ") + hipGetErrorString(se));
     *out = hold.release();
@@ -267,13 +324,14 @@ int uva_plan_create(uva_plan** out, int64_t n_islands, const int64_t* vert_off, 
             hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess)
             team_slots = c;
     }
+    const int n_xcd = xcc_info().n;
     uva::HostPlan hp;
     std::string err;
     try {
         err = uva::build_host_plan(hp, n_islands, vert_off, tri_off, P, T, pins, X, area, cots,
                                    (flags & UVA_PLAN_BATCH) != 0, nullptr,
                                    (flags & UVA_PLAN_NO_WAVE) ? -1 : uva::kWaveMaxRows,
-                                   (flags & UVA_PLAN_NO_TEAMS) ? 0 : team_slots);
+                                   (flags & UVA_PLAN_NO_TEAMS) ? 0 : team_slots, n_xcd);
     } catch (const std::bad_alloc&) {
         return fail(UVA_ENOMEM, "host allocation failed while building the plan");
     }
@@ -320,8 +378,11 @@ int uva_plan_get_stats(uva_plan* plan, uva_island_stats* stats, void* stream) {
     if (!valid_plan(plan) || !stats) return fail(UVA_EINVAL, "null argument");
     hipStream_t s = (hipStream_t)stream;
     std::vector<uva::IslandState> st(plan->dp.n_isl);
+    int fault = 0;
     HIP_TRY(hipMemcpyAsync(st.data(), plan->dp.st, st.size() * sizeof(uva::IslandState), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(&fault, plan->dp.fault, sizeof(int), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
+    if (fault) return fail(UVA_EHIP, "a multi-workgroup island team gave up waiting at a barrier; results are invalid");
     for (size_t i = 0; i < st.size(); ++i) {
         uva_island_stats& o = stats[i];
         o.cg_iters = st[i].cg_iters;

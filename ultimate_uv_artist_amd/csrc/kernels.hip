@@ -171,14 +171,24 @@ struct BlockTeam {
     }
 };
 
+// Hardware id of the XCD this wavefront runs on (HW_REG_XCC_ID).
+__device__ __forceinline__ int xcc_id() { return __builtin_amdgcn_s_getreg((15 << 11) | 20) & 15; }
+
 // m workgroups on one island.  Member k owns a contiguous block of the
 // island's slices and triangles (halo traffic only at block edges).  sync()
-// is a counter barrier with agent-scope release/acquire (every member's writes
-// become visible to the others); a reduction combines each member's partial
-// sums, written to global memory, in member order after one barrier, so the
-// result is identical in every member.  All members are resident (the grid
-// never exceeds the resident capacity) and take their member tasks from the
-// queue in order, so every member eventually arrives.
+// is a counter barrier; a reduction combines each member's partial sums,
+// written to global memory, in member order after one barrier, so the result
+// is identical in every member.  All members are resident (the grid never
+// exceeds the resident capacity) and take their member tasks from the queue
+// in order, so every member eventually arrives.
+//
+// Barrier visibility.  When all members run on one XCD (xcd_local), they
+// share its L2: every wavefront waits until its own stores have reached L2
+// (vmcnt 0), the counter is an L2 atomic, and after the wait the L1 of the CU
+// is invalidated so loads see the other members' data.  Otherwise the counter
+// uses agent-scope release/acquire, which writes back and invalidates L2.
+// A wait longer than spin_ticks of the device clock gives up and sets *fault
+// (a hang guard: the island's result is then invalid and the host reports it).
 template <int BS>
 struct TeamWG {
     static constexpr int kWaves = BS / 64;
@@ -189,8 +199,12 @@ struct TeamWG {
     int m, mem;
     int rounds;         // barriers passed
     int nred;           // reductions done (selects the partial buffer)
-    __device__ TeamWG(double* l, int* c, double* pt, int m_, int k_)
-        : lds(l), cnt(c), part(pt), m(m_), mem(k_), rounds(0), nred(0) {}
+    int local;          // members share one XCD
+    long long spin;
+    int* fault;
+    __device__ TeamWG(double* l, int* c, double* pt, int m_, int k_, int local_, long long spin_, int* fault_)
+        : lds(l), cnt(c), part(pt), m(m_), mem(k_), rounds(0), nred(0), local(local_), spin(spin_),
+          fault(fault_) {}
     __device__ int lane() const { return threadIdx.x & 63; }
     __device__ int wave() const { return threadIdx.x >> 6; }
     __device__ int nwaves() const { return kWaves; }
@@ -202,12 +216,28 @@ struct TeamWG {
     __device__ int tri_lo(int nt) const { return (int)((long long)nt * mem / m); }
     __device__ int tri_hi(int nt) const { return (int)((long long)nt * (mem + 1) / m); }
     __device__ void sync() {
+        if (local) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         __syncthreads();
         ++rounds;
         if (threadIdx.x == 0) {
-            __hip_atomic_fetch_add(cnt, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-            while (__hip_atomic_load(cnt, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < rounds * m)
-                __builtin_amdgcn_s_sleep(1);
+            const int target = rounds * m;
+            const long long t0 = wall_clock64();
+            bool late = false;
+            if (local) {
+                __hip_atomic_fetch_add(cnt, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                while (__hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+                    __builtin_amdgcn_s_sleep(1);
+                    if (wall_clock64() - t0 > spin) { late = true; break; }
+                }
+                asm volatile("buffer_inv sc0\n\ts_waitcnt vmcnt(0)" ::: "memory");
+            } else {
+                __hip_atomic_fetch_add(cnt, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                while (__hip_atomic_load(cnt, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+                    __builtin_amdgcn_s_sleep(1);
+                    if (wall_clock64() - t0 > spin) { late = true; break; }
+                }
+            }
+            if (late) atomicOr(fault, 1);
         }
         __syncthreads();
     }
@@ -803,12 +833,18 @@ __global__ __launch_bounds__(BS) void island_kernel(DevPlan P, TaskArgs a, int s
     __shared__ double red[2 * (BS / 64) * BlockTeam<BS>::kSlots];
     __shared__ int s_task;
     BlockTeam<BS> bt(red);
-    for (;;) {
-        if (threadIdx.x == 0) s_task = atomicAdd(P.queue + 2 * slot, 1);
+    // the workgroup tasks of this XCD's list, then wavefront tasks of any island
+    const int list = uniform_i(P.xcc_map[xcc_id()]);
+    const int t_lo = list >= 0 ? uniform_i(P.grp_xoff[list]) : 0;
+    const int t_n = list >= 0 ? uniform_i(P.grp_xoff[list + 1]) - t_lo : 0;
+    int* const q = P.queue + kQueueStride * slot;
+    for (; t_n > 0;) {
+        if (threadIdx.x == 0) s_task = atomicAdd(q + list, 1);
         __syncthreads();
-        const int task = uniform_i(s_task);
+        int task = uniform_i(s_task);
         __syncthreads();
-        if (task >= P.n_group) break;
+        if (task >= t_n) break;
+        task += t_lo;
         const int isl = uniform_i(P.order_group[task]);
         const int m = uniform_i(P.grp_m[task]);
         if (m <= 1) {
@@ -817,7 +853,7 @@ __global__ __launch_bounds__(BS) void island_kernel(DevPlan P, TaskArgs a, int s
             const int team = uniform_i(P.grp_team[task]);
             TeamWG<BS> tw(red, P.team_cnt + slot * P.n_teams + team,
                           P.team_part + (size_t)team * 2 * kMaxTeam * TeamWG<BS>::kSlots, m,
-                          uniform_i(P.grp_mem[task]));
+                          uniform_i(P.grp_mem[task]), P.xcd_local, P.spin_ticks, P.fault);
             run_task<KIND, PREC>(tw, P, isl, a);
         }
         __syncthreads();
@@ -827,11 +863,15 @@ __global__ __launch_bounds__(BS) void island_kernel(DevPlan P, TaskArgs a, int s
         // the first active lane takes the next task for its wavefront; every
         // lane is active here, so that is lane 0
         int task = 0;
-        if (__lane_id() == 0) task = atomicAdd(P.queue + 2 * slot + 1, 1);
+        if (__lane_id() == 0) task = atomicAdd(q + kMaxXcd, 1);
         task = uniform_i(task);
         if (task >= P.n_wave) break;
         run_task<KIND, PREC>(wt, P, uniform_i(P.order_wave[task]), a);
     }
+}
+
+__global__ void xcc_probe_kernel(int* out) {
+    if (threadIdx.x == 0) out[blockIdx.x] = xcc_id();
 }
 
 __global__ void fill_int_kernel(int* p, int n, int v) {
@@ -857,9 +897,13 @@ static hipError_t launch_kind(const DevPlan& P, const TaskArgs& a, int slot, int
         occ = nb > 0 ? nb : 1;
     }
     // enough workgroups for every workgroup task and the wavefront tasks, at
-    // most what the device keeps resident
-    const int want = P.n_group + (P.n_wave + BS / 64 - 1) / (BS / 64);
-    const int grid = want < cus * occ ? (want > 0 ? want : 1) : cus * occ;
+    // most what the device keeps resident; blocks go to the XCDs round-robin,
+    // so every XCD list gets at least as many workgroups as its largest team
+    const int cap = cus * occ;
+    int grid = P.n_group + (P.n_wave + BS / 64 - 1) / (BS / 64);
+    if (P.n_group > 0) grid = grid > P.n_xcd * P.max_m ? grid : P.n_xcd * P.max_m;
+    grid = (grid + P.n_xcd - 1) / P.n_xcd * P.n_xcd;
+    grid = grid < cap ? (grid > 0 ? grid : 1) : cap;
     hipLaunchKernelGGL((island_kernel<BS, KIND, PREC>), dim3(grid), dim3(BS), 0, s, P, a, slot);
     return hipGetLastError();
 }
@@ -887,6 +931,11 @@ hipError_t launch_island_tasks(int kind, int prec, const DevPlan& P, const TaskA
         default: return hipErrorInvalidValue;
     }
 #undef UVA_K
+}
+
+hipError_t launch_xcc_probe(int* out, int blocks, hipStream_t s) {
+    hipLaunchKernelGGL(xcc_probe_kernel, dim3(blocks), dim3(64), 0, s, out);
+    return hipGetLastError();
 }
 
 hipError_t launch_fill_int(int* p, int n, int v, hipStream_t s) {

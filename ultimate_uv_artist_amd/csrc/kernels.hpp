@@ -40,6 +40,8 @@ struct TaskArgs {
 hipError_t launch_island_tasks(int kind, int prec, const DevPlan& P, const TaskArgs& a, int slot, int cus,
                                hipStream_t s);
 hipError_t launch_fill_int(int* p, int n, int v, hipStream_t s);
+// out[b] = hardware XCD id of block b (blocks of 64 threads)
+hipError_t launch_xcc_probe(int* out, int blocks, hipStream_t s);
 hipError_t launch_signed_areas(const DevPlan& P, const double2* U, double* sa, hipStream_t s);
 
 }  // namespace uva

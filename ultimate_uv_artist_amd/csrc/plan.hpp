@@ -21,6 +21,10 @@ constexpr int kSlice = 64;          // rows per slice (= wavefront width)
 constexpr int kWaveMaxRows = 1536;  // islands up to this many vertices run on one wavefront
 constexpr int kMaxTeam = 16;        // most workgroups on one island
 constexpr int kTeamMinRows = 2048;  // fewest rows per team member
+constexpr int kMaxXcd = 8;          // XCDs (own L2 each) a device may have
+// Task counters of one launch: one per XCD list of workgroup tasks, then one
+// for the wavefront tasks.
+constexpr int kQueueStride = kMaxXcd + 1;
 
 struct IslandState {
     double e_prev;       // ARAP energy of the current layout
@@ -101,6 +105,19 @@ struct DevPlan {
     int n_teams;
     int* team_cnt;               // [launch slot][team] barrier counters, zeroed per call sequence
     double* team_part;           // [team][2][kMaxTeam][8] partial sums
+    // Workgroup tasks are split into one list per XCD: [grp_xoff[x],
+    // grp_xoff[x+1]) of order_group.  A workgroup takes tasks from the list
+    // of the XCD it runs on (hardware id -> list through xcc_map), so all
+    // members of a team share one L2 and their barrier needs no L2
+    // write-back or invalidation (xcd_local).  Without a probed XCD map there
+    // is one list and teams use agent-scope release/acquire.
+    int n_xcd;
+    int xcd_local;
+    int max_m;                   // largest team
+    int xcc_map[16];
+    int grp_xoff[kMaxXcd + 1];
+    long long spin_ticks;        // hang guard of a team barrier (device clock ticks)
+    int* fault;                  // set when a team barrier gave up waiting
 };
 
 }  // namespace uva

@@ -262,7 +262,7 @@ void parallel_for(int n, F fn) {
 std::string build_host_plan(HostPlan& hp, int64_t n_islands, const int64_t* vert_off, const int64_t* tri_off,
                             const double* P, const int64_t* T, const int64_t* pins, const double* X,
                             const double* area, const double* cots, bool batch, const int64_t* lscm_pins,
-                            int wave_max_rows, int team_slots) {
+                            int wave_max_rows, int team_slots, int n_xcd) {
     if (n_islands <= 0) return "n_islands must be positive";
     if (vert_off[0] != 0 || tri_off[0] != 0) return "offsets must start at 0";
     for (int64_t i = 0; i < n_islands; ++i) {
@@ -422,19 +422,26 @@ std::string build_host_plan(HostPlan& hp, int64_t n_islands, const int64_t* vert
     }
     std::sort(wv.begin(), wv.end());
     for (auto& p : wv) hp.order_wave.push_back(p.second);
-    plan_teams(hp, g, team_slots);
+    plan_teams(hp, g, team_slots, n_xcd);
     return "";
 }
 
-void plan_teams(HostPlan& hp, const std::vector<int>& group_islands, int team_slots) {
+void plan_teams(HostPlan& hp, const std::vector<int>& group_islands, int team_slots, int n_xcd) {
     hp.order_group.clear(); hp.grp_mem.clear(); hp.grp_m.clear(); hp.grp_team.clear();
     hp.n_teams = 0;
+    hp.max_m = 1;
+    n_xcd = std::max(1, std::min(n_xcd, kMaxXcd));
+    hp.n_xcd = n_xcd;
     auto cost = [&](int i) { return std::pow((double)hp.isl_nv[i], 1.5); };
     double total = 0.0;
     for (int i = 0; i < hp.n_isl; ++i) total += cost(i);
     // an island longer than the ideal per-workgroup share gets more workgroups
     const double share = team_slots > 0 ? total / team_slots : 0.0;
-    std::vector<std::pair<double, std::pair<int, int>>> order;   // (-time per member, (island, m))
+    struct Item {
+        double key;   // -(time per member)
+        int isl, m;
+    };
+    std::vector<Item> items;
     for (int i : group_islands) {
         int m = 1;
         if (share > 0.0) {
@@ -442,18 +449,32 @@ void plan_teams(HostPlan& hp, const std::vector<int>& group_islands, int team_sl
             m = std::min(m, std::min(kMaxTeam, std::max(1, hp.isl_nv[i] / kTeamMinRows)));
             m = std::max(m, 1);
         }
-        order.push_back({-cost(i) / m, {i, m}});
+        items.push_back({-cost(i) / m, i, m});
     }
-    std::sort(order.begin(), order.end());
-    for (auto& o : order) {
-        const int i = o.second.first, m = o.second.second;
-        const int team = m > 1 ? hp.n_teams++ : -1;
-        for (int k = 0; k < m; ++k) {
-            hp.order_group.push_back(i);
-            hp.grp_mem.push_back(k);
-            hp.grp_m.push_back(m);
-            hp.grp_team.push_back(team);
+    std::sort(items.begin(), items.end(), [](const Item& a, const Item& b) {
+        return a.key < b.key || (a.key == b.key && a.isl < b.isl);
+    });
+    // deal to the least-loaded XCD list (load = workgroup time = cost)
+    std::vector<double> load(n_xcd, 0.0);
+    std::vector<std::vector<Item>> lists(n_xcd);
+    for (const Item& it : items) {
+        int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        load[x] += cost(it.isl);
+        lists[x].push_back(it);
+    }
+    hp.grp_xoff.assign(n_xcd + 1, 0);
+    for (int x = 0; x < n_xcd; ++x) {
+        for (const Item& it : lists[x]) {
+            const int team = it.m > 1 ? hp.n_teams++ : -1;
+            hp.max_m = std::max(hp.max_m, it.m);
+            for (int k = 0; k < it.m; ++k) {
+                hp.order_group.push_back(it.isl);
+                hp.grp_mem.push_back(k);
+                hp.grp_m.push_back(it.m);
+                hp.grp_team.push_back(team);
+            }
         }
+        hp.grp_xoff[x + 1] = (int)hp.order_group.size();
     }
 }
 

@@ -32,6 +32,10 @@ struct HostPlan {
     // slot grp_team[k] for their barrier counter and partial sums
     std::vector<int> grp_mem, grp_m, grp_team;
     int n_teams = 0;
+    int max_m = 1;
+    // order_group is split into n_xcd lists: list x is [grp_xoff[x], grp_xoff[x+1])
+    int n_xcd = 1;
+    std::vector<int> grp_xoff{0, 0};
     int64_t nnz_true = 0;
 };
 
@@ -39,12 +43,14 @@ struct HostPlan {
 std::string build_host_plan(HostPlan& hp, int64_t n_islands, const int64_t* vert_off, const int64_t* tri_off,
                             const double* P, const int64_t* T, const int64_t* pins, const double* X,
                             const double* area, const double* cots, bool batch, const int64_t* lscm_pins,
-                            int wave_max_rows = kWaveMaxRows, int team_slots = 0);
+                            int wave_max_rows = kWaveMaxRows, int team_slots = 0, int n_xcd = 1);
 
 // Group-task order and team sizes: islands whose cost exceeds their share of
 // `team_slots` concurrent workgroups get several workgroups (at most kMaxTeam,
 // at least kTeamMinRows rows each).  team_slots = 0 gives every island one.
-void plan_teams(HostPlan& hp, const std::vector<int>& group_islands, int team_slots);
+// Islands are dealt to n_xcd lists (one per XCD), largest first onto the
+// least-loaded list; within a list, longest time per member first.
+void plan_teams(HostPlan& hp, const std::vector<int>& group_islands, int team_slots, int n_xcd = 1);
 
 // One island holding a general symmetric matrix (no triangles, no pin).
 std::string build_matrix_plan(HostPlan& hp, int64_t n, const int64_t* indptr, const int64_t* indices,
