@@ -19,6 +19,9 @@
  *   uva_lscm            param.py:136-179 lscm (+ _far_pair)
  *   uva_arap            param.py:379-414 refine(method="arap") and
  *                       gpu.py:230-249 Batch.arap
+ *   uva_slim            param.py:415-454 refine(method="slim") with
+ *                       param.py:234-256 _arap_rotations (mode 1) and
+ *                       param.py:341-376 _assemble_weighted / _slim_system
  *   uva_local_rhs       param.py:206-233 _arap_rotations (mode 0) +
  *                       param.py:457-474 _arap_rhs; gpu.py:165-182 Batch._rhs
  *   uva_cg_solve        param.py:103-132 SPDSolver.solve; gpu.py:184-202 Batch._cg
@@ -137,6 +140,20 @@ int uva_lscm(uva_plan *plan, double *U, const int64_t *pins, double tol, int32_t
 
 /* ARAP iterations on U in place (device, n x 2 float64). */
 int uva_arap(uva_plan *plan, double *U, const uva_arap_opts *opts, void *stream);
+
+/* SLIM (symmetric Dirichlet) options: param.refine(method="slim"). */
+typedef struct uva_slim_opts {
+    int32_t iters;           /* SLIM iterations (refine's iters) */
+    int32_t cg_max_iters;    /* PCG cap per global step (0 = 20000) */
+    double tol;              /* energy early exit |prev - e| <= tol prev */
+    double cg_tol;           /* relative residual of the normal equations;
+                                the reference solves them directly, so keep
+                                it tight (1e-10) */
+} uva_slim_opts;
+
+/* SLIM iterations on U in place (device, n x 2 float64).  Plans built
+ * without UVA_PLAN_BATCH only (refine conventions). */
+int uva_slim(uva_plan *plan, double *U, const uva_slim_opts *opts, void *stream);
 
 /* Building blocks (one launch each). */
 /* local step + rhs: rotations of U and b (device, n x 2 float64); mode picks

@@ -39,6 +39,8 @@ def lib():
         _lib.uva_emu_arap.argtypes = [C.c_void_p, _dp, C.POINTER(L.ArapOpts)]
         _lib.uva_emu_cg.argtypes = [C.c_void_p, _dp, _dp, C.c_double, C.c_int32, C.c_int32]
         _lib.uva_emu_stats.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), _dp]
+        _lib.uva_emu_stats2.argtypes = [C.c_void_p, _dp, C.POINTER(C.c_int32), _dp]
+        _lib.uva_emu_slim.argtypes = [C.c_void_p, _dp, C.POINTER(L.SlimOpts)]
     return _lib
 
 
@@ -78,6 +80,14 @@ class EmuPlan:
         lib().uva_emu_arap(self.h, U.ctypes.data_as(_dp), C.byref(o))
         return U
 
+    def slim(self, U, iters, tol=1e-5, cg_tol=1e-10, cg_max_iters=0):
+        from ultimate_uv_artist_amd import _lib as L
+
+        U = np.ascontiguousarray(np.array(U, np.float64))
+        o = L.SlimOpts(iters, cg_max_iters, tol, cg_tol)
+        lib().uva_emu_slim(self.h, U.ctypes.data_as(_dp), C.byref(o))
+        return U
+
     def cg(self, b, x0, tol=1e-6, max_iters=0, precision=0):
         b = np.ascontiguousarray(np.array(b, np.float64))
         x = np.ascontiguousarray(np.array(x0, np.float64))
@@ -89,4 +99,9 @@ class EmuPlan:
         cv = (C.c_int32 * self.k)()
         rr = (C.c_double * self.k)()
         lib().uva_emu_stats(self.h, it, cv, rr)
-        return {"cg_iters": np.array(it[:]), "converged": np.array(cv[:]), "rel_resid": np.array(rr[:])}
+        en = (C.c_double * self.k)()
+        ni = (C.c_int32 * self.k)()
+        sp = (C.c_double * self.k)()
+        lib().uva_emu_stats2(self.h, en, ni, sp)
+        return {"cg_iters": np.array(it[:]), "converged": np.array(cv[:]), "rel_resid": np.array(rr[:]),
+                "energy": np.array(en[:]), "iters": np.array(ni[:]), "step": np.array(sp[:])}

@@ -15,8 +15,11 @@ Only behaviour that the oracle's own checks cannot pin is recorded:
   * the order-dependent sequential repair_flips on a folded chart,
   * chart_quality scalars,
   * gpu.Batch on three islands (orient, repair, free and flip-safe ARAP) with
-    a CG cap high enough to converge.
-Output: tests/golden/param_golden.npz and tests/golden/batch_golden.npz.
+    a CG cap high enough to converge,
+  * refine(method="slim") trajectories (every energy evaluation, including
+    the backtracking ones) on two disk islands.
+Output: tests/golden/param_golden.npz, tests/golden/batch_golden.npz and
+tests/golden/slim_golden.npz (``--only slim`` writes the last one alone).
 """
 from __future__ import annotations
 
@@ -49,6 +52,38 @@ def load_reference(root):
         spec.loader.exec_module(m)
         mods[name] = m
     return mods["param"], mods["gpu"]
+
+
+def slim_fixtures(rp):
+    """refine(method="slim") from an ARAP layout and from a stretched LSCM
+    layout (which needs backtracking), every energy evaluation recorded."""
+    from ultimate_uv_artist_amd import workloads
+
+    rng = np.random.default_rng(400)
+    out = {}
+    real_energy = rp._energy
+    for c, t in enumerate((700, 1200)):
+        P, T = workloads.disk_island(rng, t)
+        U0 = rp.lscm(P, T)
+        if c == 0:
+            U0, _ = rp.refine(P, T, U0, iters=3, tol=0.0)
+        else:
+            U0 = U0 * np.array([2.5, 0.6])
+        energies = []
+
+        def rec(U, T_, X_, area_, kind):
+            e = real_energy(U, T_, X_, area_, kind)
+            energies.append(e)
+            return e
+
+        rp._energy = rec
+        U8, _ = rp.refine(P, T, U0, iters=8, method="slim", tol=1e-6)
+        rp._energy = real_energy
+        out[f"P{c}"], out[f"T{c}"], out[f"U0_{c}"] = P, T.astype(np.int32), U0
+        out[f"slim{c}"] = U8
+        out[f"slim{c}_energy"] = np.array(energies)
+    np.savez_compressed(HERE / "slim_golden.npz", **out)
+    print("slim_golden.npz", (HERE / "slim_golden.npz").stat().st_size, "bytes")
 
 
 def main(root):
@@ -118,7 +153,13 @@ def main(root):
     np.savez_compressed(HERE / "batch_golden.npz", **gold)
     for f in ("param_golden.npz", "batch_golden.npz"):
         print(f, (HERE / f).stat().st_size, "bytes")
+    slim_fixtures(rp)
 
 
 if __name__ == "__main__":
-    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")
+    args = [a for a in sys.argv[1:] if a != "--only" and a != "slim"]
+    root = args[0] if args else "/root/reference"
+    if "--only" in sys.argv and "slim" in sys.argv:
+        slim_fixtures(load_reference(root)[0])
+    else:
+        main(root)

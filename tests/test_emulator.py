@@ -112,3 +112,56 @@ def test_emu_arap_batch_mode_vs_batch_oracle(disks):
     for i in range(len(disks)):
         s = slice(e.vo[i], e.vo[i + 1])
         assert unit_box_err(U[s], r.U[s]) <= 1e-4
+
+
+def own_box(U):
+    lo = U.min(0)
+    return (U - lo) / float((U.max(0) - lo).max())
+
+
+@pytest.mark.parametrize("shape", [(1, 1), (3, 2)])
+def test_emu_slim_vs_reference_trajectory(shape):
+    """SLIM task code against the reference's trajectory (golden fixture):
+    energies rel 1e-9, layout 1e-8 in its own unit box, and the translation
+    kept exactly (the mean of the layout does not move)."""
+    from pathlib import Path
+
+    d = dict(np.load(Path(__file__).parent / "golden" / "slim_golden.npz"))
+    charts = [(d[f"P{c}"], d[f"T{c}"].astype(np.int64)) for c in (0, 1)]
+    U0 = np.concatenate([d["U0_0"], d["U0_1"]])
+    e = EmuPlan(charts, *shape)
+    U = e.slim(U0, 8, tol=1e-6)
+    st = e.stats()
+    for c in (0, 1):
+        s = slice(e.vo[c], e.vo[c + 1])
+        ref_e = d[f"slim{c}_energy"]
+        assert st["iters"][c] == len(ref_e) - 1
+        assert abs(st["energy"][c] - ref_e[-1]) <= 1e-9 * ref_e[-1]
+        assert np.abs(own_box(U[s]) - own_box(d[f"slim{c}"])).max() <= 1e-8
+        assert np.abs((U[s] - U0[s]).mean(0)).max() <= 1e-12
+        assert st["converged"][c] == 1 and st["rel_resid"][c] <= 1e-10
+
+
+def test_emu_slim_flipped_start_and_early_exit(disks):
+    """A start with a flipped triangle has infinite symmetric Dirichlet energy
+    and is returned unchanged; a loose tol exits after the iteration whose
+    energy change falls under it (both as in param.py:418-453)."""
+    from oracle import param_cpu as oracle
+
+    P, T = disks[1]
+    U0 = oracle.refine(P, T, oracle.lscm(P, T), iters=2, tol=0.0)[0]
+    Uf = U0.copy()
+    a, b, c = (Uf[v].copy() for v in T[len(T) // 2])
+    edge = c - b
+    foot = b + edge * ((a - b) @ edge) / (edge @ edge)
+    Uf[T[len(T) // 2][0]] = 2.0 * foot - a        # mirror the corner across its opposite edge
+    assert (oracle.signed_areas(Uf, T) <= 0).any()
+    e = EmuPlan([(P, T)])
+    assert np.array_equal(e.slim(Uf, 5), Uf)
+    assert e.stats()["iters"][0] == 0
+    for tol in (1e-1, 1e-3):
+        trace = []
+        Ur, _ = oracle.refine(P, T, U0, iters=20, method="slim", tol=tol, trace=trace)
+        U = e.slim(U0, 20, tol=tol)
+        assert e.stats()["iters"][0] == len(trace) < 20
+        assert np.abs(own_box(U) - own_box(Ur)).max() <= 1e-8

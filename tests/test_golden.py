@@ -68,3 +68,26 @@ def test_batch_semantics():
     assert np.abs(r.U - b["after_free"]).max() < 1e-5 * span      # reference CG stops at 1e-6
     r.arap(3, line_search=True)
     assert np.abs(r.U - b["after_ls"]).max() < 1e-5 * span
+
+
+def own_box(U):
+    """Layout scaled into the unit box by its own bounding box (SLIM's
+    translation is fixed only by a 1e-12-weighted regulariser, so the
+    reference's direct solve leaves ~1e-4 of drift there)."""
+    lo = U.min(0)
+    return (U - lo) / float((U.max(0) - lo).max())
+
+
+@pytest.mark.parametrize("case", [0, 1])
+def test_slim_trajectory(case):
+    """Oracle refine(method="slim") against the reference's own trajectory:
+    every accepted energy (rel 1e-9) and the final layout (own unit box 1e-9)."""
+    d = dict(np.load(GOLD / "slim_golden.npz"))
+    P, T, U0 = d[f"P{case}"], d[f"T{case}"].astype(np.int64), d[f"U0_{case}"]
+    trace = []
+    U, solver = pc.refine(P, T, U0, iters=8, method="slim", tol=1e-6, trace=trace)
+    assert solver is None
+    ref_e = d[f"slim{case}_energy"]
+    assert len(trace) == len(ref_e) - 1          # no backtracking in these cases: one energy per iteration
+    np.testing.assert_allclose([e for e, _ in trace], ref_e[1:], rtol=1e-9)
+    assert np.abs(own_box(U) - own_box(d[f"slim{case}"])).max() <= 1e-9

@@ -356,3 +356,36 @@ def test_nonfinite_island_reverts_in_batch(small_disks):
     for i in (0, 2, 3):
         assert np.array_equal(a[i], c[i])
     assert np.isnan(c[1][3]).all()
+
+
+def _own_box(U):
+    lo = U.min(0)
+    return (U - lo) / float((U.max(0) - lo).max())
+
+
+def test_refine_slim_matches_reference_trajectory():
+    """param.refine(method="slim") on the GPU against the reference's SLIM
+    trajectory (golden fixture): final energy rel 1e-9, layout 1e-8 in its own
+    unit box (translation is fixed only by a 1e-12 regulariser; the GPU keeps
+    the start's mean exactly), and a flipped start comes back unchanged."""
+    from pathlib import Path
+
+    from oracle import param_cpu as oracle
+    from ultimate_uv_artist_amd import param
+
+    d = dict(np.load(Path(__file__).parent / "golden" / "slim_golden.npz"))
+    for c in (0, 1):
+        P, T, U0 = d[f"P{c}"], d[f"T{c}"].astype(np.int64), d[f"U0_{c}"]
+        print(f"slim case {c}: {len(T)} triangles", flush=True)
+        U, solver = param.refine(P, T, U0, iters=8, method="slim", tol=1e-6)
+        assert solver is None
+        X, area = oracle.local_frames(P, T)
+        e = oracle.energy(U, T, X, area, 1)
+        assert abs(e - d[f"slim{c}_energy"][-1]) <= 1e-9 * e
+        assert np.abs(_own_box(U) - _own_box(d[f"slim{c}"])).max() <= 1e-8
+        assert np.abs((U - U0).mean(0)).max() <= 1e-12
+    Uf = U0.copy()
+    a, b, cc = (Uf[v].copy() for v in T[0])
+    edge = cc - b
+    Uf[T[0][0]] = 2.0 * (b + edge * ((a - b) @ edge) / (edge @ edge)) - a
+    assert np.array_equal(param.refine(P, T, Uf, iters=3, method="slim")[0], Uf)

@@ -58,6 +58,10 @@ class ArapOpts(C.Structure):
                 ("tol", C.c_double), ("cg_tol", C.c_double), ("cg_max_iters", C.c_int32), ("use_graph", C.c_int32)]
 
 
+class SlimOpts(C.Structure):
+    _fields_ = [("iters", C.c_int32), ("cg_max_iters", C.c_int32), ("tol", C.c_double), ("cg_tol", C.c_double)]
+
+
 _vp = C.c_void_p
 _i64p = C.POINTER(C.c_int64)
 _dp = C.POINTER(C.c_double)
@@ -74,6 +78,7 @@ SIGNATURES = {
     "uva_plan_set_scale": (C.c_int, [_vp, _vp, _vp]),
     "uva_lscm": (C.c_int, [_vp, _vp, _i64p, C.c_double, C.c_int32, _vp]),
     "uva_arap": (C.c_int, [_vp, _vp, C.POINTER(ArapOpts), _vp]),
+    "uva_slim": (C.c_int, [_vp, _vp, C.POINTER(SlimOpts), _vp]),
     "uva_local_rhs": (C.c_int, [_vp, _vp, _vp, C.c_int32, _vp]),
     "uva_cg_solve": (C.c_int, [_vp, _vp, _vp, C.c_double, C.c_int32, C.c_int32, _vp]),
     "uva_max_step": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int32, _vp]),

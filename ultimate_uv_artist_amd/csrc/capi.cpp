@@ -476,6 +476,37 @@ int uva_arap(uva_plan* plan, double* U, const uva_arap_opts* o, void* stream) {
     return UVA_OK;
 }
 
+int uva_slim(uva_plan* plan, double* U, const uva_slim_opts* o, void* stream) {
+    if (!valid_plan(plan) || !U || !o) return fail(UVA_EINVAL, "null argument");
+    if (o->iters < 0) return fail(UVA_EINVAL, "iters must be >= 0");
+    if (!(o->cg_tol > 0.0)) return fail(UVA_EINVAL, "cg_tol must be positive");
+    if (plan->dp.mode_batch) return fail(UVA_EINVAL, "SLIM needs a plan built with refine conventions");
+    if (plan->dp.n_tri == 0) return fail(UVA_EINVAL, "SLIM needs triangles");
+    hipStream_t s = (hipStream_t)stream;
+    uva::DevPlan& d = plan->dp;
+    if (!d.sw) {
+        if (plan->alloc(&d.sw, d.n_tri) != hipSuccess || plan->alloc(&d.sy, 3 * (size_t)d.n_tri) != hipSuccess ||
+            plan->alloc(&d.sdi, d.n_vert) != hipSuccess)
+            return fail(UVA_ENOMEM, "SLIM scratch");
+    }
+    uva::TaskArgs a = base_args(U);
+    a.cg_tol = o->cg_tol;
+    a.cg_max = o->cg_max_iters > 0 ? o->cg_max_iters : kDefaultMaxIters;
+    a.tol = o->tol;
+    const int total = 1 + o->iters;
+    for (int done = 0; done < total;) {
+        const int chunk = std::min(total - done, kMaxSlots);
+        int rc = reset_queue(plan, chunk, s);
+        if (rc) return rc;
+        for (int k = 0; k < chunk; ++k) {
+            rc = launch(plan, done + k == 0 ? uva::TASK_SLIM_PROLOGUE : uva::TASK_SLIM_ITER, UVA_PREC_F64, a, k, s);
+            if (rc) return rc;
+        }
+        done += chunk;
+    }
+    return UVA_OK;
+}
+
 int uva_local_rhs(uva_plan* plan, const double* U, double* b, int32_t mode, void* stream) {
     if (!valid_plan(plan) || !U || !b) return fail(UVA_EINVAL, "null argument");
     hipStream_t s = (hipStream_t)stream;

@@ -582,6 +582,8 @@ UVA_HD __forceinline__ double signed_area(const DevPlan& P, int f, const double2
     return 0.5 * ((b.x - a.x) * (c.y - a.y) - (c.x - a.x) * (b.y - a.y));
 }
 
+#include "slim_island.hpp"
+
 // ---------------------------------------------------------------- island tasks
 template <int KIND, int PREC, class Team>
 UVA_HD void run_task(Team& tm, const DevPlan& P, int isl, const TaskArgs& a) {
@@ -824,6 +826,73 @@ UVA_HD void run_task(Team& tm, const DevPlan& P, int isl, const TaskArgs& a) {
             c += (signed_area(P, f, U) < 0.0 && P.fok[f]) ? 1.0 : 0.0;
         const double s = tm.sum4(make_double4(c, 0.0, 0.0, 0.0)).x;
         if (tm.leader()) a.out_isl[isl] = s;
+    } else if (KIND == TASK_SLIM_PROLOGUE) {
+        // param.py:418-421: symmetric Dirichlet energy of the start (a
+        // non-finite start leaves U as it is), regularisation 1e-12 * area
+        double en = 0.0, ar = 0.0;
+        FOR_TRIS(tm, d, f) {
+            ar += P.farea[f];
+            if (P.fok[f]) en += sd_energy_tri(jacobian(P, f, U), P.farea[f]);
+        }
+        const double4 s = tm.sum4(make_double4(en, ar, 0.0, 0.0));
+        if (tm.leader()) {
+            st->e_prev = s.x;
+            st->reg = 1e-6 * s.y * 1e-6;
+            st->done = __builtin_isfinite(s.x) ? 0 : 1;
+            st->cg_iters = 0;
+            st->cg_iters_max = 0;
+            st->arap_iters = 0;
+            st->converged = 1;
+            st->rel_resid = 0.0;
+            st->step = 0.0;
+            st->tmax = 1e30;
+            st->status = 0;
+            st->flips0 = 0;
+            st->use_ls = 1;
+            st->ticks = 0;
+        }
+        tm.sync();
+    } else if (KIND == TASK_SLIM_ITER) {
+        // one iteration of param.py:423-453
+        if (uniform_i(st->done)) return;
+        const long long t_start = uva_clock();
+        const double reg = uniform_d(st->reg), prev = uniform_d(st->e_prev);
+        int conv = 0;
+        double rel = 0.0;
+        const int its = slim_solve(tm, P, d, U, reg, a.cg_tol, a.cg_max, conv, rel);
+        const double tm_ = max_step_island(tm, P, d, U, P.X64, false);
+        double step = fmin(1.0, 0.9 * tm_);
+        double e = slim_energy_at(tm, P, d, U, step);
+        for (int k = 0; k < 12 && (!__builtin_isfinite(e) || e > prev); ++k) {
+            step *= 0.5;
+            e = slim_energy_at(tm, P, d, U, step);
+        }
+        const bool accept = __builtin_isfinite(e) && !(e > prev);
+        if (accept) {
+            FOR_ROWS(tm, d, row, sg) {
+                (void)sg;
+                const double2 u = U[row], xn = P.X64[row];
+                const double dx = xn.x - u.x, dy = xn.y - u.y;
+                U[row] = make_double2(u.x + step * dx, u.y + step * dy);
+            }
+        }
+        if (tm.leader()) {
+            st->cg_iters += its;
+            st->cg_iters_max = its > st->cg_iters_max ? its : st->cg_iters_max;
+            st->converged = st->converged && conv;
+            st->rel_resid = rel;
+            st->tmax = tm_;
+            if (accept) {
+                st->arap_iters += 1;
+                st->step = step;
+                st->done = fabs(prev - e) <= a.tol * prev ? 1 : 0;
+                st->e_prev = e;
+            } else {
+                st->done = 1;
+            }
+            st->ticks += uva_clock() - t_start;
+        }
+        tm.sync();
     }
 }
 
@@ -928,6 +997,8 @@ hipError_t launch_island_tasks(int kind, int prec, const DevPlan& P, const TaskA
         case TASK_REPAIR: return launch_kind<TASK_REPAIR, UVA_PREC_MIXED>(P, a, slot, cus, s);
         case TASK_ORIENT: return launch_kind<TASK_ORIENT, UVA_PREC_MIXED>(P, a, slot, cus, s);
         case TASK_FLIPS: return launch_kind<TASK_FLIPS, UVA_PREC_MIXED>(P, a, slot, cus, s);
+        case TASK_SLIM_PROLOGUE: return launch_kind<TASK_SLIM_PROLOGUE, UVA_PREC_F64>(P, a, slot, cus, s);
+        case TASK_SLIM_ITER: return launch_kind<TASK_SLIM_ITER, UVA_PREC_F64>(P, a, slot, cus, s);
         default: return hipErrorInvalidValue;
     }
 #undef UVA_K

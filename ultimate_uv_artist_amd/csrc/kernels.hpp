@@ -21,6 +21,8 @@ enum TaskKind {
     TASK_REPAIR = 7,
     TASK_ORIENT = 8,
     TASK_FLIPS = 9,
+    TASK_SLIM_PROLOGUE = 10,  // symmetric Dirichlet energy of the start, regularisation weight
+    TASK_SLIM_ITER = 11,      // SLIM local step -> normal equations by PCG -> step with backtracking
 };
 
 struct TaskArgs {

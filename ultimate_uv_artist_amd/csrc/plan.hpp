@@ -40,6 +40,7 @@ struct IslandState {
     int flips0;
     int status;
     long long ticks;     // device wall-clock ticks spent on this island in the call
+    double reg;          // SLIM regularisation weight 1e-12 * island area
 };
 
 struct DevPlan {
@@ -93,6 +94,12 @@ struct DevPlan {
     void* ve;
     void* vu;                    // u = D^-1 r
     void* vw;                    // w = A u
+    // SLIM scratch (allocated on the first SLIM call): per triangle the
+    // weight matrix W^T W (xx, xy, yy) and the 2x3 operator output, per
+    // vertex the inverse of the 2x2 diagonal block (xx, xy, yy)
+    double4* sw;
+    double2* sy;
+    double4* sdi;
     double* red;                 // per-island scalars for multi-launch paths
     IslandState* st;
     int* queue;                  // task counters

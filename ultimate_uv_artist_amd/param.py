@@ -355,21 +355,30 @@ def refine(P, T, U, iters=10, method="arap", X=None, area=None, cots=None, solve
 
     ``method="arap"`` runs every iteration on the GPU: rotations, rhs, CG to
     relative residual 1e-6 warm-started at U, flip-safe step (only when the
-    start has no flips), energy early exit at ``tol``.  Returns (U, solver)."""
-    if method != "arap":
-        raise NotImplementedError("refine(method='slim') is not available on the GPU path yet")
-    import torch
+    start has no flips), energy early exit at ``tol``.  Returns (U, solver).
 
+    ``method="slim"`` (symmetric Dirichlet, param.py:415-454) runs every
+    iteration on the GPU too: SLIM weights from the closed-form 2x2 SVD, the
+    normal equations by block-Jacobi PCG to relative residual 1e-10 (the
+    reference solves them directly), flip-safe step halved up to 12 times
+    until the energy does not increase.  Returns (U, None)."""
     P = np.asarray(P, np.float64)
     T = np.asarray(T, np.int64)
+    frames = None
+    if X is not None:   # caller's frames (the reference uses them instead of P's)
+        frames = (X, area if area is not None else local_frames(P, T)[1], cots if cots is not None else cot_weights(X))
+    if method == "slim":
+        plan = IslandPlan([(P, T)], frames=frames)
+        Ud = plan.device_uv(U)
+        plan.slim(Ud, iters, tol=tol)
+        return Ud.cpu().numpy(), None
+    if method != "arap":
+        raise ValueError(f"unknown refine method {method!r}")
     pin = 0 if pin is None else int(pin)
     plan = None
     if isinstance(solver, ArapSolver) and solver.n == len(P) and solver.t == len(T) and solver.pin == pin:
         plan = solver.plan
     if plan is None:
-        frames = None
-        if X is not None and cots is not None:
-            frames = (X, area if area is not None else local_frames(P, T)[1], cots)
         plan = IslandPlan([(P, T)], pins=[pin], frames=frames)
         solver = ArapSolver(plan, len(P), len(T), pin)
     Ud = plan.device_uv(U)

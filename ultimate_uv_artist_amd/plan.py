@@ -16,6 +16,9 @@ from ._lib import check, f64p, i64p, ptr
 
 CG_TOL = 1e-6        # relative residual of every ARAP global step
 LSCM_TOL = 1e-8      # relative residual of the LSCM solve (FP64)
+# SLIM's normal equations are solved directly by the reference (param.py:432),
+# so their CG runs to a tight residual
+SLIM_CG_TOL = 1e-10
 
 
 def require_gpu(dev=None):
@@ -132,6 +135,13 @@ class IslandPlan:
         o = _lib.ArapOpts(int(iters), int(bool(line_search)), int(mode), int(precision), float(tol), float(cg_tol),
                           int(cg_max_iters), int(bool(use_graph)))
         check(_lib.lib().uva_arap(self._h, ptr(U), C.byref(o), stream_of(self.dev)), "uva_arap")
+        return U
+
+    def slim(self, U, iters, tol=1e-5, cg_tol=SLIM_CG_TOL, cg_max_iters=0):
+        """SLIM (symmetric Dirichlet) iterations on U (device tensor) in place."""
+        self._check_uv(U)
+        o = _lib.SlimOpts(int(iters), int(cg_max_iters), float(tol), float(cg_tol))
+        check(_lib.lib().uva_slim(self._h, ptr(U), C.byref(o), stream_of(self.dev)), "uva_slim")
         return U
 
     def local_rhs(self, U, mode=_lib.UVA_MODE_REFINE):
