@@ -27,6 +27,10 @@
  *   uva_cg_solve        param.py:103-132 SPDSolver.solve; gpu.py:184-202 Batch._cg
  *   uva_max_step        param.py:259-291 _max_step; gpu.py:204-227 Batch._max_step
  *   uva_energy          param.py:294-338 _energy
+ *   uva_energy_fwd      param.py:294-338 _energy with per-triangle weights
+ *   uva_energy_bwd,     no counterpart in the reference (it has no
+ *   uva_adjoint_cg,     gradients): the backward of _energy and of one
+ *   uva_arap_step_bwd   refine global step (param.py:391-401, 457-474)
  *   uva_signed_areas    param.py:97-100 signed_areas; gpu.py:153-156
  *   uva_flips_per_chart gpu.py:158-163 Batch.flips_per_chart
  *   uva_repair_jacobi   gpu.py:251-273 Batch.repair_flips
@@ -166,6 +170,20 @@ int uva_cg_solve(uva_plan *plan, const double *b, double *x, double tol, int32_t
 int uva_max_step(uva_plan *plan, const double *U, const double *D, double *tmax, int32_t mode, void *stream);
 /* per-island energy (kind 0 ARAP, 1 symmetric Dirichlet) -> e (device) */
 int uva_energy(uva_plan *plan, const double *U, int32_t kind, double *e, void *stream);
+/* Differentiation (refine plans).  Per-island weighted energy
+ * E_i = sum_f w_f e_f(U) (w: device, n_triangles, or NULL = 1) -> e. */
+int uva_energy_fwd(uva_plan *plan, const double *U, const double *w, int32_t kind, double *e, void *stream);
+/* gU = ge_i dE_i/dU (device, n x 2) and, if gw is not NULL, gw_f = ge_i e_f
+ * (device, n_triangles), for the per-island upstream gradient ge (device). */
+int uva_energy_bwd(uva_plan *plan, const double *U, const double *w, int32_t kind, const double *ge, double *gU,
+                   double *gw, void *stream);
+/* Adjoint of the global step: lam = L^-1 g per island (same SPD operator as
+ * the ARAP solve, started from zero). */
+int uva_adjoint_cg(uva_plan *plan, const double *g, double *lam, double tol, int32_t max_iters, int32_t precision,
+                   void *stream);
+/* Backward of one global step U' = L^-1 b(U): gU = (db/dU)^T lam, with lam
+ * from uva_adjoint_cg on dLoss/dU'. */
+int uva_arap_step_bwd(uva_plan *plan, const double *U, const double *lam, double *gU, void *stream);
 /* per-triangle signed UV area -> sa (device, n_triangles) */
 int uva_signed_areas(uva_plan *plan, const double *U, double *sa, void *stream);
 /* Batch.flips_per_chart -> counts (device, n_islands float64) */

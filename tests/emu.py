@@ -41,6 +41,11 @@ def lib():
         _lib.uva_emu_stats.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), _dp]
         _lib.uva_emu_stats2.argtypes = [C.c_void_p, _dp, C.POINTER(C.c_int32), _dp]
         _lib.uva_emu_slim.argtypes = [C.c_void_p, _dp, C.POINTER(L.SlimOpts)]
+        _lib.uva_emu_local_rhs.argtypes = [C.c_void_p, _dp, _dp]
+        _lib.uva_emu_energy_fwd.argtypes = [C.c_void_p, _dp, _dp, C.c_int32, _dp]
+        _lib.uva_emu_energy_bwd.argtypes = [C.c_void_p, _dp, _dp, C.c_int32, _dp, _dp, _dp]
+        _lib.uva_emu_step_bwd.argtypes = [C.c_void_p, _dp, _dp, _dp]
+        _lib.uva_emu_adjoint_cg.argtypes = [C.c_void_p, _dp, _dp, C.c_double, C.c_int32]
     return _lib
 
 
@@ -93,6 +98,45 @@ class EmuPlan:
         x = np.ascontiguousarray(np.array(x0, np.float64))
         lib().uva_emu_cg(self.h, b.ctypes.data_as(_dp), x.ctypes.data_as(_dp), tol, max_iters, precision)
         return x
+
+    @staticmethod
+    def _p(a):
+        return None if a is None else a.ctypes.data_as(_dp)
+
+    def local_rhs(self, U):
+        U = np.ascontiguousarray(np.array(U, np.float64))
+        b = np.zeros_like(U)
+        lib().uva_emu_local_rhs(self.h, self._p(U), self._p(b))
+        return b
+
+    def energy_fwd(self, U, w=None, kind=0):
+        U = np.ascontiguousarray(np.array(U, np.float64))
+        w = None if w is None else np.ascontiguousarray(np.asarray(w, np.float64))
+        e = np.zeros(self.k)
+        lib().uva_emu_energy_fwd(self.h, self._p(U), self._p(w), kind, self._p(e))
+        return e
+
+    def energy_bwd(self, U, ge, w=None, kind=0):
+        U = np.ascontiguousarray(np.array(U, np.float64))
+        w = None if w is None else np.ascontiguousarray(np.asarray(w, np.float64))
+        ge = np.ascontiguousarray(np.asarray(ge, np.float64))
+        gU = np.zeros_like(U)
+        gw = np.zeros(int(self.to[-1]))
+        lib().uva_emu_energy_bwd(self.h, self._p(U), self._p(w), kind, self._p(ge), self._p(gU), self._p(gw))
+        return gU, gw
+
+    def step_bwd(self, U, lam):
+        U = np.ascontiguousarray(np.array(U, np.float64))
+        lam = np.ascontiguousarray(np.array(lam, np.float64))
+        gU = np.zeros_like(U)
+        lib().uva_emu_step_bwd(self.h, self._p(U), self._p(lam), self._p(gU))
+        return gU
+
+    def adjoint_cg(self, g, tol=1e-13, max_iters=0):
+        g = np.ascontiguousarray(np.array(g, np.float64))
+        lam = np.zeros_like(g)
+        lib().uva_emu_adjoint_cg(self.h, self._p(g), self._p(lam), tol, max_iters)
+        return lam
 
     def stats(self):
         it = (C.c_int32 * self.k)()

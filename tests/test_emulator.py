@@ -165,3 +165,79 @@ def test_emu_slim_flipped_start_and_early_exit(disks):
         U = e.slim(U0, 20, tol=tol)
         assert e.stats()["iters"][0] == len(trace) < 20
         assert np.abs(own_box(U) - own_box(Ur)).max() <= 1e-8
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+@pytest.mark.parametrize("shape", [(1, 1), (3, 2)])
+def test_emu_energy_backward(disks, kind, shape):
+    """Weighted energy and its hand-written backward (task code) against the
+    float64 torch autograd reference (rel 1e-12) and central differences
+    (h = 1e-3 along a mesh-scaled direction; their O(h^2) truncation on the
+    symmetric Dirichlet energy is ~1e-5 relative, so rel 1e-4)."""
+    import torch
+
+    import diffref
+    from oracle import param_cpu as oracle
+
+    rng = np.random.default_rng(11 + kind)
+    e = EmuPlan(disks[:2], *shape)
+    U0 = np.concatenate([oracle.refine(P, T, oracle.lscm(P, T), iters=2, tol=0.0)[0] for P, T in disks[:2]])
+    w = rng.uniform(0.5, 2.0, int(e.to[-1]))
+    ge = rng.uniform(0.5, 1.5, e.k)
+    E = e.energy_fwd(U0, w, kind)
+    gU, gw = e.energy_bwd(U0, ge, w, kind)
+    for i, (P, T) in enumerate(disks[:2]):
+        sv, st = slice(e.vo[i], e.vo[i + 1]), slice(e.to[i], e.to[i + 1])
+        X, area = oracle.local_frames(P, T)
+        Ut = torch.tensor(U0[sv], requires_grad=True)
+        wt = torch.tensor(w[st], requires_grad=True)
+        Et = diffref.energy(Ut, T, X, area, wt, kind)
+        (ge[i] * Et).backward()
+        assert abs(E[i] - Et.item()) <= 1e-12 * abs(Et.item())
+        assert np.abs(gU[sv] - Ut.grad.numpy()).max() <= 1e-12 * np.abs(Ut.grad.numpy()).max()
+        assert np.abs(gw[st] - wt.grad.numpy()).max() <= 1e-12 * np.abs(wt.grad.numpy()).max()
+        D = np.zeros_like(U0)
+        D[sv] = diffref.mesh_direction(U0[sv], T, rng)
+        h = 1e-3
+        fd = (e.energy_fwd(U0 + h * D, w, kind)[i] - e.energy_fwd(U0 - h * D, w, kind)[i]) / (2 * h)
+        an = (gU[sv] * D[sv]).sum() / ge[i]
+        assert abs(fd - an) <= 1e-4 * abs(an)
+
+
+def test_emu_global_step_backward(disks):
+    """Backward of two unrolled refine global steps (adjoint CG + rhs/rotation
+    backward, task code) against float64 torch autograd through dense solves
+    and against central differences of the emulated steps."""
+    import torch
+
+    import diffref
+    from oracle import param_cpu as oracle
+
+    rng = np.random.default_rng(21)
+    P, T = disks[1]
+    U0 = oracle.refine(P, T, oracle.lscm(P, T), iters=2, tol=0.0)[0]
+    e = EmuPlan([(P, T)])
+    step = diffref.Step(P, T)
+    gbar = rng.normal(size=U0.shape)
+
+    def emu_steps(U):
+        for _ in range(2):
+            b = e.local_rhs(U)
+            U = e.cg(b, U, tol=1e-13, precision=2)
+        return U
+
+    U1 = emu_steps(U0)
+    Ut = torch.tensor(U0, requires_grad=True)
+    Ur = step(step(Ut))
+    assert np.abs(U1 - Ur.detach().numpy()).max() <= 1e-9 * np.abs(U1).max()
+    (Ur * torch.as_tensor(gbar)).sum().backward()
+    # backward: per step, the adjoint solve then the step backward at that step's input
+    U_1 = e.cg(e.local_rhs(U0), U0, tol=1e-13, precision=2)
+    g1 = e.step_bwd(U_1, e.adjoint_cg(gbar))
+    g0 = e.step_bwd(U0, e.adjoint_cg(g1))
+    ref = Ut.grad.numpy()
+    assert np.abs(g0 - ref).max() <= 1e-9 * np.abs(ref).max()
+    D = diffref.mesh_direction(U0, T, rng)
+    h = 1e-3
+    fd = ((emu_steps(U0 + h * D) - emu_steps(U0 - h * D)) * gbar).sum() / (2 * h)
+    assert abs(fd - (g0 * D).sum()) <= 1e-5 * abs(fd)

@@ -79,6 +79,10 @@ SIGNATURES = {
     "uva_lscm": (C.c_int, [_vp, _vp, _i64p, C.c_double, C.c_int32, _vp]),
     "uva_arap": (C.c_int, [_vp, _vp, C.POINTER(ArapOpts), _vp]),
     "uva_slim": (C.c_int, [_vp, _vp, C.POINTER(SlimOpts), _vp]),
+    "uva_energy_fwd": (C.c_int, [_vp, _vp, _vp, C.c_int32, _vp, _vp]),
+    "uva_energy_bwd": (C.c_int, [_vp, _vp, _vp, C.c_int32, _vp, _vp, _vp, _vp]),
+    "uva_adjoint_cg": (C.c_int, [_vp, _vp, _vp, C.c_double, C.c_int32, C.c_int32, _vp]),
+    "uva_arap_step_bwd": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
     "uva_local_rhs": (C.c_int, [_vp, _vp, _vp, C.c_int32, _vp]),
     "uva_cg_solve": (C.c_int, [_vp, _vp, _vp, C.c_double, C.c_int32, C.c_int32, _vp]),
     "uva_max_step": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int32, _vp]),

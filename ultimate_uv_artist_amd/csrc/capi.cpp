@@ -209,6 +209,8 @@ int upload_plan(uva::HostPlan& hp, const double* P, hipStream_t s, uva_plan** ou
         hipError_t e = pl->upload(&g, hp.inc_g, s);
         if (e != hipSuccess) return bail(UVA_EHIP, "upload inc_g");
         d.inc_g = (const double2*)g;
+        if ((e = pl->upload(&g, hp.tri_g, s)) != hipSuccess) return bail(UVA_EHIP, "upload tri_g");
+        d.tri_g = (const double2*)g;
     }
     // LSCM pins (mutable through uva_lscm)
     pl->lpin0 = hp.isl_lpin0;
@@ -253,6 +255,9 @@ int upload_plan(uva::HostPlan& hp, const double* P, hipStream_t s, uva_plan** ou
         d.b64 = p2;
         if ((e = pl->alloc(&p2, nv)) != hipSuccess) return bail(UVA_ENOMEM, "X64");
         d.X64 = p2;
+        // per-triangle corner values of the matrix-free passes (SLIM, backward)
+        if ((e = pl->alloc(&p2, 3 * nt)) != hipSuccess) return bail(UVA_ENOMEM, "corner scratch");
+        d.sy = p2;
         void** vecs[6] = {&d.vr, &d.vp, &d.vAp, &d.ve, &d.vu, &d.vw};
         for (void** v : vecs) {
             double2* q = nullptr;
@@ -485,8 +490,7 @@ int uva_slim(uva_plan* plan, double* U, const uva_slim_opts* o, void* stream) {
     hipStream_t s = (hipStream_t)stream;
     uva::DevPlan& d = plan->dp;
     if (!d.sw) {
-        if (plan->alloc(&d.sw, d.n_tri) != hipSuccess || plan->alloc(&d.sy, 3 * (size_t)d.n_tri) != hipSuccess ||
-            plan->alloc(&d.sdi, d.n_vert) != hipSuccess)
+        if (plan->alloc(&d.sw, d.n_tri) != hipSuccess || plan->alloc(&d.sdi, d.n_vert) != hipSuccess)
             return fail(UVA_ENOMEM, "SLIM scratch");
     }
     uva::TaskArgs a = base_args(U);
@@ -555,6 +559,65 @@ int uva_energy(uva_plan* plan, const double* U, int32_t kind, double* e, void* s
     int rc = reset_queue(plan, 1, s);
     if (rc) return rc;
     return launch(plan, uva::TASK_ENERGY, UVA_PREC_MIXED, a, 0, s);
+}
+
+int uva_energy_fwd(uva_plan* plan, const double* U, const double* w, int32_t kind, double* e, void* stream) {
+    if (!valid_plan(plan) || !U || !e) return fail(UVA_EINVAL, "null argument");
+    if (kind != 0 && kind != 1) return fail(UVA_EINVAL, "kind must be 0 (ARAP) or 1 (symmetric Dirichlet)");
+    if (plan->dp.mode_batch) return fail(UVA_EINVAL, "the energy gradients need a plan built with refine conventions");
+    hipStream_t s = (hipStream_t)stream;
+    uva::TaskArgs a = base_args(const_cast<double*>(U));
+    a.out_isl = e;
+    a.kind = kind;
+    a.w = w;
+    int rc = reset_queue(plan, 1, s);
+    if (rc) return rc;
+    return launch(plan, uva::TASK_ENERGY, UVA_PREC_MIXED, a, 0, s);
+}
+
+int uva_energy_bwd(uva_plan* plan, const double* U, const double* w, int32_t kind, const double* ge, double* gU,
+                   double* gw, void* stream) {
+    if (!valid_plan(plan) || !U || !ge || !gU) return fail(UVA_EINVAL, "null argument");
+    if (kind != 0 && kind != 1) return fail(UVA_EINVAL, "kind must be 0 (ARAP) or 1 (symmetric Dirichlet)");
+    if (plan->dp.mode_batch) return fail(UVA_EINVAL, "the energy gradients need a plan built with refine conventions");
+    hipStream_t s = (hipStream_t)stream;
+    uva::TaskArgs a = base_args(const_cast<double*>(U));
+    a.kind = kind;
+    a.w = w;
+    a.gin = ge;
+    a.out = (double2*)gU;
+    a.gw = gw;
+    int rc = reset_queue(plan, 1, s);
+    if (rc) return rc;
+    return launch(plan, uva::TASK_ENERGY_BWD, UVA_PREC_F64, a, 0, s);
+}
+
+int uva_adjoint_cg(uva_plan* plan, const double* g, double* lam, double tol, int32_t max_iters, int32_t precision,
+                   void* stream) {
+    if (!valid_plan(plan) || !g || !lam) return fail(UVA_EINVAL, "null argument");
+    if (!(tol > 0.0)) return fail(UVA_EINVAL, "tol must be positive");
+    if (precision < UVA_PREC_MIXED || precision > UVA_PREC_F64) return fail(UVA_EINVAL, "bad precision");
+    hipStream_t s = (hipStream_t)stream;
+    uva::TaskArgs a = base_args(lam);
+    a.in = (const double2*)g;
+    a.cg_tol = tol;
+    a.cg_max = max_iters > 0 ? max_iters : kDefaultMaxIters;
+    a.zero_x = 1;
+    int rc = reset_queue(plan, 1, s);
+    if (rc) return rc;
+    return launch(plan, uva::TASK_CG, precision, a, 0, s);
+}
+
+int uva_arap_step_bwd(uva_plan* plan, const double* U, const double* lam, double* gU, void* stream) {
+    if (!valid_plan(plan) || !U || !lam || !gU) return fail(UVA_EINVAL, "null argument");
+    if (plan->dp.mode_batch) return fail(UVA_EINVAL, "the step backward needs a plan built with refine conventions");
+    hipStream_t s = (hipStream_t)stream;
+    uva::TaskArgs a = base_args(const_cast<double*>(U));
+    a.in = (const double2*)lam;
+    a.out = (double2*)gU;
+    int rc = reset_queue(plan, 1, s);
+    if (rc) return rc;
+    return launch(plan, uva::TASK_STEP_BWD, UVA_PREC_F64, a, 0, s);
 }
 
 int uva_signed_areas(uva_plan* plan, const double* U, double* sa, void* stream) {

@@ -583,6 +583,7 @@ UVA_HD __forceinline__ double signed_area(const DevPlan& P, int f, const double2
 }
 
 #include "slim_island.hpp"
+#include "diff_island.hpp"
 
 // ---------------------------------------------------------------- island tasks
 template <int KIND, int PREC, class Team>
@@ -717,13 +718,13 @@ UVA_HD void run_task(Team& tm, const DevPlan& P, int isl, const TaskArgs& a) {
             a.out[row] = P.b64[row];
         }
     } else if (KIND == TASK_CG) {
-        // b in a.in, warm start and result in U
+        // b in a.in, warm start (or zero start) and result in U
         double4 acc = make_double4(0.0, 0.0, 0.0, 0.0);
         FOR_ROWS(tm, d, row, sg) {
             (void)sg;
             const double2 bb = a.in[row];
             P.b64[row] = bb;
-            P.X64[row] = U[row];
+            P.X64[row] = a.zero_x ? make_double2(0.0, 0.0) : U[row];
             acc.x += bb.x * bb.x;
             acc.y += bb.y * bb.y;
         }
@@ -751,14 +752,12 @@ UVA_HD void run_task(Team& tm, const DevPlan& P, int isl, const TaskArgs& a) {
         const double t = max_step_island(tm, P, d, U, P.X64, a.batch);
         if (tm.leader()) a.out_isl[isl] = t;
     } else if (KIND == TASK_ENERGY) {
-        double en = 0.0;
-        FOR_TRIS(tm, d, f) {
-            if (!P.fok[f]) continue;
-            const Jac J = jacobian(P, f, U);
-            en += a.kind == 0 ? arap_energy_tri(J, P.farea[f]) : sd_energy_tri(J, P.farea[f]);
-        }
-        const double s = tm.sum4(make_double4(en, 0.0, 0.0, 0.0)).x;
+        const double s = energy_fwd_island(tm, P, d, U, a.w, a.kind);
         if (tm.leader()) a.out_isl[isl] = s;
+    } else if (KIND == TASK_ENERGY_BWD) {
+        energy_bwd_island(tm, P, d, U, a.w, a.kind, uniform_d(a.gin[isl]), a.out, a.gw);
+    } else if (KIND == TASK_STEP_BWD) {
+        step_bwd_island(tm, P, d, U, a.in, a.out);
     } else if (KIND == TASK_REPAIR) {
         // Jacobi 1-ring relaxation of the free vertices of folded thick triangles
         // (gpu.py:251-273); the batch-wide "no fold left" stop is per island here,
@@ -999,6 +998,8 @@ hipError_t launch_island_tasks(int kind, int prec, const DevPlan& P, const TaskA
         case TASK_FLIPS: return launch_kind<TASK_FLIPS, UVA_PREC_MIXED>(P, a, slot, cus, s);
         case TASK_SLIM_PROLOGUE: return launch_kind<TASK_SLIM_PROLOGUE, UVA_PREC_F64>(P, a, slot, cus, s);
         case TASK_SLIM_ITER: return launch_kind<TASK_SLIM_ITER, UVA_PREC_F64>(P, a, slot, cus, s);
+        case TASK_ENERGY_BWD: return launch_kind<TASK_ENERGY_BWD, UVA_PREC_F64>(P, a, slot, cus, s);
+        case TASK_STEP_BWD: return launch_kind<TASK_STEP_BWD, UVA_PREC_F64>(P, a, slot, cus, s);
         default: return hipErrorInvalidValue;
     }
 #undef UVA_K

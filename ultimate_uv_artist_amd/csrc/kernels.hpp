@@ -23,6 +23,8 @@ enum TaskKind {
     TASK_FLIPS = 9,
     TASK_SLIM_PROLOGUE = 10,  // symmetric Dirichlet energy of the start, regularisation weight
     TASK_SLIM_ITER = 11,      // SLIM local step -> normal equations by PCG -> step with backtracking
+    TASK_ENERGY_BWD = 12,     // dE/dU and dE/dw of the weighted energy
+    TASK_STEP_BWD = 13,       // backward of one ARAP global step given the adjoint solution
 };
 
 struct TaskArgs {
@@ -37,6 +39,10 @@ struct TaskArgs {
     int batch;
     int kind;                // energy kind
     int rounds;              // repair rounds
+    const double* w;         // per-triangle energy weights (NULL = 1)
+    const double* gin;       // per-island upstream gradient (energy backward)
+    double* gw;              // per-triangle weight gradient (NULL = not wanted)
+    int zero_x;              // CG starts from x = 0 instead of U (adjoint solve)
 };
 
 hipError_t launch_island_tasks(int kind, int prec, const DevPlan& P, const TaskArgs& a, int slot, int cus,

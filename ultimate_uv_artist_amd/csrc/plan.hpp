@@ -94,6 +94,7 @@ struct DevPlan {
     void* ve;
     void* vu;                    // u = D^-1 r
     void* vw;                    // w = A u
+    const double2* tri_g;        // rhs corner vectors per triangle (3 per triangle)
     // SLIM scratch (allocated on the first SLIM call): per triangle the
     // weight matrix W^T W (xx, xy, yy) and the 2x3 operator output, per
     // vertex the inverse of the 2x2 diagonal block (xx, xy, yy)

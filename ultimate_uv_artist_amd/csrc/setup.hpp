@@ -20,6 +20,7 @@ struct HostPlan {
     std::vector<unsigned char> fok;
     std::vector<int> inc_ptr, inc_tri;
     std::vector<double> inc_g;          // 2 per incidence
+    std::vector<double> tri_g;          // the same corner vectors triangle-major: 6 per triangle
     std::vector<int> sl_ptr, m_col;
     std::vector<float> m_hi, m_lo, m_b;
     std::vector<double> diag;

@@ -171,6 +171,50 @@ class IslandPlan:
         check(_lib.lib().uva_energy(self._h, ptr(U), int(kind), ptr(out), stream_of(self.dev)), "uva_energy")
         return out
 
+    def _check_tri(self, w):
+        if not (isinstance(w, torch.Tensor) and w.dtype == torch.float64 and w.is_contiguous()
+                and w.device == self.dev and tuple(w.shape) == (self.t,)):
+            raise ValueError(f"expected a contiguous ({self.t},) float64 tensor on {self.dev}")
+
+    def energy_fwd(self, U, w=None, kind=0):
+        """Per-island weighted energy sum_f w_f e_f(U) (w None = 1)."""
+        self._check_uv(U)
+        if w is not None:
+            self._check_tri(w)
+        out = torch.empty(self.n_islands, dtype=torch.float64, device=self.dev)
+        check(_lib.lib().uva_energy_fwd(self._h, ptr(U), None if w is None else ptr(w), int(kind), ptr(out),
+                                        stream_of(self.dev)), "uva_energy_fwd")
+        return out
+
+    def energy_bwd(self, U, ge, w=None, kind=0, want_w=True):
+        """(ge_i dE_i/dU, ge_i e_f or None) for the per-island upstream gradient ge."""
+        self._check_uv(U)
+        if w is not None:
+            self._check_tri(w)
+        ge = ge.to(device=self.dev, dtype=torch.float64).contiguous().reshape(self.n_islands)
+        gU = torch.empty_like(U)
+        gw = torch.empty(self.t, dtype=torch.float64, device=self.dev) if want_w else None
+        check(_lib.lib().uva_energy_bwd(self._h, ptr(U), None if w is None else ptr(w), int(kind), ptr(ge), ptr(gU),
+                                        None if gw is None else ptr(gw), stream_of(self.dev)), "uva_energy_bwd")
+        return gU, gw
+
+    def adjoint_cg(self, g, tol=1e-10, max_iters=0, precision=_lib.UVA_PREC_F64):
+        """lam = L^-1 g per island (the global step's operator, zero start)."""
+        self._check_uv(g)
+        lam = torch.empty_like(g)
+        check(_lib.lib().uva_adjoint_cg(self._h, ptr(g), ptr(lam), float(tol), int(max_iters), int(precision),
+                                        stream_of(self.dev)), "uva_adjoint_cg")
+        return lam
+
+    def step_bwd(self, U, lam):
+        """(db/dU)^T lam for the refine global step's right-hand side b(U)."""
+        self._check_uv(U)
+        self._check_uv(lam)
+        gU = torch.empty_like(U)
+        check(_lib.lib().uva_arap_step_bwd(self._h, ptr(U), ptr(lam), ptr(gU), stream_of(self.dev)),
+              "uva_arap_step_bwd")
+        return gU
+
     def signed_areas(self, U):
         self._check_uv(U)
         out = torch.empty(self.t, dtype=torch.float64, device=self.dev)
