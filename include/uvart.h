@@ -145,6 +145,12 @@ int uva_lscm(uva_plan *plan, double *U, const int64_t *pins, double tol, int32_t
 /* ARAP iterations on U in place (device, n x 2 float64). */
 int uva_arap(uva_plan *plan, double *U, const uva_arap_opts *opts, void *stream);
 
+/* uva_arap run eagerly with a HIP event pair around every launch on the
+ * stream: launch_ms[0] is the prologue, launch_ms[k] ARAP iteration k
+ * (n_launch = iters + 1).  Measurement only; no counterpart in the reference. */
+int uva_arap_profile(uva_plan *plan, double *U, const uva_arap_opts *opts, double *launch_ms, int32_t n_launch,
+                     void *stream);
+
 /* SLIM (symmetric Dirichlet) options: param.refine(method="slim"). */
 typedef struct uva_slim_opts {
     int32_t iters;           /* SLIM iterations (refine's iters) */

@@ -11,6 +11,7 @@ import numpy as np
 LIB = Path(__file__).resolve().parent.parent / "ultimate_uv_artist_amd" / "libuvart_emu.so"
 
 _i64p = C.POINTER(C.c_int64)
+NO_RESIDENT = 1 << 8   # emulator plan flag: the ARAP CG streams instead of keeping rows resident
 _dp = C.POINTER(C.c_double)
 _lib = None
 
@@ -53,7 +54,7 @@ class EmuPlan:
     """A batch of islands run by the CPU emulation with teams of
     ``members`` x ``waves`` wavefronts of 64 lane threads."""
 
-    def __init__(self, charts, members=1, waves=1, batch=False, pins=None):
+    def __init__(self, charts, members=1, waves=1, batch=False, pins=None, resident=True):
         from ultimate_uv_artist_amd.plan import concat_charts
 
         self.vo, self.to, P, T = concat_charts(charts)
@@ -62,8 +63,8 @@ class EmuPlan:
         pa = None if pins is None else np.ascontiguousarray(np.asarray(pins, np.int64))
         self.h = lib().uva_emu_create(self.k, self.vo.ctypes.data_as(_i64p), self.to.ctypes.data_as(_i64p),
                                       P.ctypes.data_as(_dp), T.ctypes.data_as(_i64p),
-                                      None if pa is None else pa.ctypes.data_as(_i64p), 1 if batch else 0,
-                                      members, waves)
+                                      None if pa is None else pa.ctypes.data_as(_i64p),
+                                      (1 if batch else 0) | (0 if resident else NO_RESIDENT), members, waves)
         assert self.h, "emulator plan creation failed"
 
     def __del__(self):

@@ -97,6 +97,27 @@ def test_emu_arap_refine_vs_oracle(disks, shape):
         assert unit_box_err(U[s], Ur) <= 1e-5
 
 
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("precision", [0, 1, 2])
+def test_emu_resident_cg_equals_streaming_cg(disks, shape, precision):
+    """The resident ARAP CG (rows in registers, u and e in LDS, halo through
+    global memory) performs the streaming CG's arithmetic in the same order:
+    both give the same layout bit for bit, for every team shape and
+    precision.  The islands span 2 to 12 slices, so some members hold their
+    rows resident and (with one wavefront per member) the largest island
+    exceeds a lane's register slots and streams in both runs."""
+    from oracle import param_cpu as oracle
+
+    U0 = np.concatenate([oracle.lscm(P, T) for P, T in disks])
+    out = []
+    for resident in (True, False):
+        e = EmuPlan(disks, *shape, resident=resident)
+        # all-FP32 stalls near its rounding floor, so it gets a looser tolerance
+        out.append(e.arap(U0, 3, tol=0.0, cg_tol=1e-4 if precision == 1 else 1e-7, precision=precision))
+        assert e.stats()["converged"].all()
+    assert np.array_equal(out[0], out[1])
+
+
 def test_emu_arap_batch_mode_vs_batch_oracle(disks):
     """gpu.Batch.arap semantics (free system, per-island frames)."""
     from oracle import param_cpu as oracle

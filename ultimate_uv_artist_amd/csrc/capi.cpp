@@ -104,8 +104,10 @@ uva::TaskArgs base_args(double* U) {
 }
 
 // One sequence of launches (queue reset, prologue, iterations) with slots
-// reused in chunks of kMaxSlots.
-int arap_sequence(uva_plan* pl, double* U, const uva_arap_opts* o, hipStream_t s) {
+// reused in chunks of kMaxSlots.  With ev (2 events per launch), ev[2k] and
+// ev[2k+1] bracket launch k (0 = prologue) on the stream.
+int arap_sequence(uva_plan* pl, double* U, const uva_arap_opts* o, hipStream_t s,
+                  const std::vector<hipEvent_t>* ev = nullptr) {
     uva::TaskArgs a = base_args(U);
     a.cg_tol = o->cg_tol;
     a.cg_max = o->cg_max_iters > 0 ? o->cg_max_iters : kDefaultMaxIters;
@@ -120,8 +122,10 @@ int arap_sequence(uva_plan* pl, double* U, const uva_arap_opts* o, hipStream_t s
         if (rc) return rc;
         for (int k = 0; k < chunk; ++k) {
             int step = done + k;
+            if (ev) HIP_TRY(hipEventRecord((*ev)[2 * step], s));
             rc = launch(pl, step == 0 ? uva::TASK_PROLOGUE : uva::TASK_ARAP_ITER, o->precision, a, k, s);
             if (rc) return rc;
+            if (ev) HIP_TRY(hipEventRecord((*ev)[2 * step + 1], s));
         }
         done += chunk;
     }
@@ -479,6 +483,31 @@ int uva_arap(uva_plan* plan, double* U, const uva_arap_opts* o, void* stream) {
     }
     HIP_TRY(hipGraphLaunch(plan->gexec, s));
     return UVA_OK;
+}
+
+int uva_arap_profile(uva_plan* plan, double* U, const uva_arap_opts* o, double* launch_ms, int32_t n_launch,
+                     void* stream) {
+    if (!valid_plan(plan) || !U || !o || !launch_ms) return fail(UVA_EINVAL, "null argument");
+    if (o->iters < 0 || n_launch != 1 + o->iters) return fail(UVA_EINVAL, "n_launch must be iters + 1");
+    if (!(o->cg_tol > 0.0)) return fail(UVA_EINVAL, "cg_tol must be positive");
+    if (o->precision < UVA_PREC_MIXED || o->precision > UVA_PREC_F64) return fail(UVA_EINVAL, "bad precision");
+    if (o->mode != UVA_MODE_REFINE && o->mode != UVA_MODE_BATCH) return fail(UVA_EINVAL, "bad mode");
+    hipStream_t s = (hipStream_t)stream;
+    std::vector<hipEvent_t> ev(2 * (size_t)n_launch, nullptr);
+    int rc = UVA_OK;
+    for (auto& e : ev) {
+        if (hipEventCreate(&e) != hipSuccess) { rc = fail(UVA_EHIP, "hipEventCreate"); break; }
+    }
+    if (rc == UVA_OK) rc = arap_sequence(plan, U, o, s, &ev);
+    if (rc == UVA_OK && hipStreamSynchronize(s) != hipSuccess) rc = fail(UVA_EHIP, "hipStreamSynchronize");
+    for (int k = 0; rc == UVA_OK && k < n_launch; ++k) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]) != hipSuccess) rc = fail(UVA_EHIP, "hipEventElapsedTime");
+        launch_ms[k] = ms;
+    }
+    for (auto& e : ev)
+        if (e) (void)hipEventDestroy(e);
+    return rc;
 }
 
 int uva_slim(uva_plan* plan, double* U, const uva_slim_opts* o, void* stream) {

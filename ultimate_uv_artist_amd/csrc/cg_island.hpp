@@ -101,7 +101,7 @@ UVA_HD __forceinline__ void spmv_row2(const DevPlan& P, int ra, int sa, bool oka
 }
 
 template <int PREC, int OP, class Team>
-UVA_HD CGOut cg_island(Team& tm, const DevPlan& P, const Isl& d, double2 b2, double tol, int maxit, int lp0,
+UVA_HD __forceinline__ CGOut cg_island(Team& tm, const DevPlan& P, const Isl& d, double2 b2, double tol, int maxit, int lp0,
                            int lp1) {
     using PT = Prec<PREC>;
     using V = typename PT::V;

@@ -79,6 +79,12 @@ __device__ __forceinline__ double wave_min(double v) {
 // team's global-memory writes visible to the whole team, and the reductions
 // return identical (scalar) values in every thread.
 struct WaveTeam {
+    char* lv;   // this wavefront's share of the LDS vector buffer (or NULL)
+    int lvb;    // its size in bytes
+    __device__ WaveTeam(char* l, int b) : lv(l), lvb(b) {}
+    __device__ void* lvec() const { return lv; }
+    __device__ int lvec_bytes() const { return lvb; }
+    __device__ bool multi() const { return false; }
     __device__ int lane() const { return threadIdx.x & 63; }
     __device__ int wave() const { return 0; }
     __device__ int nwaves() const { return 1; }
@@ -118,7 +124,12 @@ struct BlockTeam {
     static constexpr int kSlots = 8;   // doubles per wavefront slot
     double* lds;  // 2 * kWaves * kSlots doubles
     int par;
-    __device__ explicit BlockTeam(double* l) : lds(l), par(0) {}
+    char* lv;     // LDS vector buffer of the workgroup (or NULL)
+    int lvb;
+    __device__ BlockTeam(double* l, char* v, int vb) : lds(l), par(0), lv(v), lvb(vb) {}
+    __device__ void* lvec() const { return lv; }
+    __device__ int lvec_bytes() const { return lvb; }
+    __device__ bool multi() const { return false; }
     __device__ int lane() const { return threadIdx.x & 63; }
     __device__ int wave() const { return threadIdx.x >> 6; }
     __device__ int nwaves() const { return kWaves; }
@@ -202,9 +213,15 @@ struct TeamWG {
     int local;          // members share one XCD
     long long spin;
     int* fault;
-    __device__ TeamWG(double* l, int* c, double* pt, int m_, int k_, int local_, long long spin_, int* fault_)
+    char* lv;           // LDS vector buffer of this member (or NULL)
+    int lvb;
+    __device__ TeamWG(double* l, int* c, double* pt, int m_, int k_, int local_, long long spin_, int* fault_,
+                      char* v, int vb)
         : lds(l), cnt(c), part(pt), m(m_), mem(k_), rounds(0), nred(0), local(local_), spin(spin_),
-          fault(fault_) {}
+          fault(fault_), lv(v), lvb(vb) {}
+    __device__ void* lvec() const { return lv; }
+    __device__ int lvec_bytes() const { return lvb; }
+    __device__ bool multi() const { return true; }
     __device__ int lane() const { return threadIdx.x & 63; }
     __device__ int wave() const { return threadIdx.x >> 6; }
     __device__ int nwaves() const { return kWaves; }
@@ -433,6 +450,7 @@ template <int PREC>
 UVA_HD __forceinline__ double acc_mul(double a, double b) { return a * b; }
 
 #include "cg_island.hpp"
+#include "cg_resident.hpp"
 
 // ---------------------------------------------------------------- per-triangle pieces
 struct Jac {
@@ -473,7 +491,7 @@ UVA_HD __forceinline__ double sd_energy_tri(const Jac& J, double area) {
 // rotation (= angle of atan2) for thick finite triangles, identity otherwise,
 // times the triangle's target scale (gpu.py:166-170).
 template <class Team>
-UVA_HD double local_step(Team& tm, const DevPlan& P, const Isl& d, const double2* __restrict__ U, bool batch,
+UVA_HD __forceinline__ double local_step(Team& tm, const DevPlan& P, const Isl& d, const double2* __restrict__ U, bool batch,
                              bool want_energy) {
     double en = 0.0;
     FOR_TRIS(tm, d, f) {
@@ -503,7 +521,7 @@ UVA_HD double local_step(Team& tm, const DevPlan& P, const Isl& d, const double2
 // rhs: b[v] = sum over incident corners of R(tri) g (param.py:457-468, gpu.py:171-181)
 // plus the soft-pin term; X64 = U (warm start).  Returns |b|^2 per column.
 template <class Team>
-UVA_HD double2 rhs_step(Team& tm, const DevPlan& P, const Isl& d, const double2* __restrict__ U) {
+UVA_HD __forceinline__ double2 rhs_step(Team& tm, const DevPlan& P, const Isl& d, const double2* __restrict__ U) {
     double4 acc = make_double4(0.0, 0.0, 0.0, 0.0);
     FOR_ROWS(tm, d, row, sg) {
         (void)sg;
@@ -531,7 +549,7 @@ UVA_HD double2 rhs_step(Team& tm, const DevPlan& P, const Isl& d, const double2*
 // Flip-preventing step along D = X64 - U (param.py:259-291 for refine;
 // gpu.py:204-227 for batch), minimum over the island's triangles.
 template <class Team>
-UVA_HD double max_step_island(Team& tm, const DevPlan& P, const Isl& d, const double2* __restrict__ U,
+UVA_HD __forceinline__ double max_step_island(Team& tm, const DevPlan& P, const Isl& d, const double2* __restrict__ U,
                                   const double2* __restrict__ Xn, bool batch) {
     double best = 1e30;
     FOR_TRIS(tm, d, f) {
@@ -587,7 +605,7 @@ UVA_HD __forceinline__ double signed_area(const DevPlan& P, int f, const double2
 
 // ---------------------------------------------------------------- island tasks
 template <int KIND, int PREC, class Team>
-UVA_HD void run_task(Team& tm, const DevPlan& P, int isl, const TaskArgs& a) {
+UVA_HD __forceinline__ void run_task(Team& tm, const DevPlan& P, int isl, const TaskArgs& a) {
     IslandState* st = P.st + isl;
     const Isl d = load_isl(P, isl);
     double2* U = a.U;
@@ -616,7 +634,7 @@ UVA_HD void run_task(Team& tm, const DevPlan& P, int isl, const TaskArgs& a) {
         if (uniform_i(st->done)) return;
         const long long t_start = uva_clock();
         const double2 b2 = rhs_step(tm, P, d, U);
-        const CGOut cg = cg_island<PREC, OP_ARAP>(tm, P, d, b2, a.cg_tol, a.cg_max, -1, -1);
+        const CGOut cg = cg_arap<PREC>(tm, P, d, b2, a.cg_tol, a.cg_max);
         if (a.batch) {
             // islands whose solve went non-finite keep their layout (gpu.py:235-240)
             double bad = 0.0;
@@ -729,7 +747,7 @@ UVA_HD void run_task(Team& tm, const DevPlan& P, int isl, const TaskArgs& a) {
             acc.y += bb.y * bb.y;
         }
         const double4 s = tm.sum4(acc);
-        const CGOut cg = cg_island<PREC, OP_ARAP>(tm, P, d, make_double2(s.x, s.y), a.cg_tol, a.cg_max, -1, -1);
+        const CGOut cg = cg_arap<PREC>(tm, P, d, make_double2(s.x, s.y), a.cg_tol, a.cg_max);
         FOR_ROWS(tm, d, row, sg) {
             (void)sg;
             U[row] = P.X64[row];
@@ -900,7 +918,11 @@ template <int BS, int KIND, int PREC>
 __global__ __launch_bounds__(BS) void island_kernel(DevPlan P, TaskArgs a, int slot) {
     __shared__ double red[2 * (BS / 64) * BlockTeam<BS>::kSlots];
     __shared__ int s_task;
-    BlockTeam<BS> bt(red);
+    // vectors of the resident CG (kinds that solve the ARAP system only)
+    constexpr int kLv = lds_vec_bytes(KIND);
+    __shared__ double lvec[kLv > 0 ? kLv / 8 : 1];
+    char* const lv = kLv > 0 ? (char*)lvec : nullptr;
+    BlockTeam<BS> bt(red, lv, kLv);
     // the workgroup tasks of this XCD's list, then wavefront tasks of any island
     const int list = uniform_i(P.xcc_map[xcc_id()]);
     const int t_lo = list >= 0 ? uniform_i(P.grp_xoff[list]) : 0;
@@ -921,12 +943,13 @@ __global__ __launch_bounds__(BS) void island_kernel(DevPlan P, TaskArgs a, int s
             const int team = uniform_i(P.grp_team[task]);
             TeamWG<BS> tw(red, P.team_cnt + slot * P.n_teams + team,
                           P.team_part + (size_t)team * 2 * kMaxTeam * TeamWG<BS>::kSlots, m,
-                          uniform_i(P.grp_mem[task]), P.xcd_local, P.spin_ticks, P.fault);
+                          uniform_i(P.grp_mem[task]), P.xcd_local, P.spin_ticks, P.fault, lv, kLv);
             run_task<KIND, PREC>(tw, P, isl, a);
         }
         __syncthreads();
     }
-    WaveTeam wt;
+    constexpr int kLvWave = kLv / (BS / 64);
+    WaveTeam wt(kLv > 0 ? lv + (threadIdx.x >> 6) * kLvWave : nullptr, kLvWave);
     for (;;) {
         // the first active lane takes the next task for its wavefront; every
         // lane is active here, so that is lane 0
@@ -955,8 +978,7 @@ __global__ void signed_areas_kernel(DevPlan P, const double2* __restrict__ U, do
 // ---------------------------------------------------------------- launchers
 template <int KIND, int PREC>
 static hipError_t launch_kind(const DevPlan& P, const TaskArgs& a, int slot, int cus, hipStream_t s) {
-    // FP64 kernels hold more registers per lane: half the workgroup size
-    constexpr int BS = PREC == UVA_PREC_F64 ? kBlockF64 : kBlock;
+    constexpr int BS = block_size(KIND, PREC);
     static int occ = 0;   // resident workgroups per CU of this instantiation (same on every call)
     if (occ == 0) {
         int nb = 0;

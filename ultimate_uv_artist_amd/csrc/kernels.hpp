@@ -9,6 +9,8 @@ namespace uva {
 
 constexpr int kBlock = 1024;    // workgroup size of the island kernel (16 wavefronts)
 constexpr int kBlockF64 = 512;  // FP64 instantiations (more registers per lane)
+constexpr int kBlockRes = 512;  // kinds with the resident CG (up to 8 rows per lane in registers)
+constexpr int kLdsVec = 65536;  // LDS bytes of the resident CG vectors u and e per workgroup
 
 enum TaskKind {
     TASK_PROLOGUE = 0,   // flips of the start, line-search decision, rotations + energy
@@ -26,6 +28,13 @@ enum TaskKind {
     TASK_ENERGY_BWD = 12,     // dE/dU and dE/dw of the weighted energy
     TASK_STEP_BWD = 13,       // backward of one ARAP global step given the adjoint solution
 };
+
+// Kinds whose tasks solve the ARAP system (resident CG when the island fits).
+constexpr bool solves_arap(int kind) { return kind == TASK_ARAP_ITER || kind == TASK_CG; }
+constexpr int lds_vec_bytes(int kind) { return solves_arap(kind) ? kLdsVec : 0; }
+constexpr int block_size(int kind, int prec) {
+    return solves_arap(kind) ? kBlockRes : (prec == UVA_PREC_F64 ? kBlockF64 : kBlock);
+}
 
 struct TaskArgs {
     double2* U;              // layout (in/out)

@@ -18,7 +18,8 @@
 namespace uva {
 
 constexpr int kSlice = 64;          // rows per slice (= wavefront width)
-constexpr int kWaveMaxRows = 1536;  // islands up to this many vertices run on one wavefront
+constexpr int kWaveMaxRows = 512;   // islands up to this many vertices run on one wavefront
+constexpr int kResSlices = 64;      // slices one workgroup keeps resident in the FP32 ARAP CG (8 waves x 8)
 constexpr int kMaxTeam = 16;        // most workgroups on one island
 constexpr int kTeamMinRows = 2048;  // fewest rows per team member
 constexpr int kMaxXcd = 8;          // XCDs (own L2 each) a device may have

@@ -449,6 +449,9 @@ void plan_teams(HostPlan& hp, const std::vector<int>& group_islands, int team_sl
         if (share > 0.0) {
             m = (int)std::ceil(cost(i) / share);
             m = std::min(m, std::min(kMaxTeam, std::max(1, hp.isl_nv[i] / kTeamMinRows)));
+            // enough members that each keeps its rows resident in the ARAP CG
+            const int need = (hp.isl_ns[i] + kResSlices - 1) / kResSlices;
+            m = std::max(m, std::min(need, kMaxTeam));
             m = std::max(m, 1);
         }
         items.push_back({-cost(i) / m, i, m});

@@ -137,6 +137,18 @@ class IslandPlan:
         check(_lib.lib().uva_arap(self._h, ptr(U), C.byref(o), stream_of(self.dev)), "uva_arap")
         return U
 
+    def arap_profile(self, U, iters, line_search=True, mode=_lib.UVA_MODE_REFINE, tol=1e-5, cg_tol=CG_TOL,
+                     cg_max_iters=0, precision=_lib.UVA_PREC_MIXED):
+        """``arap`` run eagerly with HIP events around every launch: returns the
+        launch durations in ms (index 0 the prologue, k the ARAP iteration k)."""
+        self._check_uv(U)
+        o = _lib.ArapOpts(int(iters), int(bool(line_search)), int(mode), int(precision), float(tol), float(cg_tol),
+                          int(cg_max_iters), 0)
+        ms = np.zeros(int(iters) + 1, np.float64)
+        check(_lib.lib().uva_arap_profile(self._h, ptr(U), C.byref(o), f64p(ms), int(iters) + 1,
+                                          stream_of(self.dev)), "uva_arap_profile")
+        return ms
+
     def slim(self, U, iters, tol=1e-5, cg_tol=SLIM_CG_TOL, cg_max_iters=0):
         """SLIM (symmetric Dirichlet) iterations on U (device tensor) in place."""
         self._check_uv(U)
