@@ -59,6 +59,10 @@ extern "C" {
                                (no single-wavefront tasks) */
 #define UVA_PLAN_NO_TEAMS 4u /* one workgroup per island at most (no
                                multi-workgroup teams for large islands) */
+#define UVA_PLAN_KEEP_ORDER 8u /* keep the caller's vertex order inside the
+                               plan (default: Morton order per island) */
+#define UVA_PLAN_NO_COARSE 16u /* Jacobi-preconditioned CG only (no coarse
+                               level over 64-vertex aggregates) */
 
 /* ARAP semantics */
 #define UVA_MODE_REFINE 0   /* param.refine: per-island pin weight, energy early

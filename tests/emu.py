@@ -54,7 +54,8 @@ class EmuPlan:
     """A batch of islands run by the CPU emulation with teams of
     ``members`` x ``waves`` wavefronts of 64 lane threads."""
 
-    def __init__(self, charts, members=1, waves=1, batch=False, pins=None, resident=True):
+    def __init__(self, charts, members=1, waves=1, batch=False, pins=None, resident=True, keep_order=False,
+                 no_coarse=False):
         from ultimate_uv_artist_amd.plan import concat_charts
 
         self.vo, self.to, P, T = concat_charts(charts)
@@ -64,7 +65,8 @@ class EmuPlan:
         self.h = lib().uva_emu_create(self.k, self.vo.ctypes.data_as(_i64p), self.to.ctypes.data_as(_i64p),
                                       P.ctypes.data_as(_dp), T.ctypes.data_as(_i64p),
                                       None if pa is None else pa.ctypes.data_as(_i64p),
-                                      (1 if batch else 0) | (0 if resident else NO_RESIDENT), members, waves)
+                                      (1 if batch else 0) | (0 if resident else NO_RESIDENT)
+                                      | (8 if keep_order else 0) | (16 if no_coarse else 0), members, waves)
         assert self.h, "emulator plan creation failed"
 
     def __del__(self):

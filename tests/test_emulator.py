@@ -97,21 +97,35 @@ def test_emu_arap_refine_vs_oracle(disks, shape):
         assert unit_box_err(U[s], Ur) <= 1e-5
 
 
+def _shuffled(P, T, seed):
+    """The same island with its vertices in random order (a matrix bandwidth
+    close to the vertex count)."""
+    perm = np.random.default_rng(seed).permutation(len(P))
+    inv = np.empty_like(perm)
+    inv[perm] = np.arange(len(P))
+    return P[perm], inv[T]
+
+
 @pytest.mark.parametrize("shape", SHAPES)
 @pytest.mark.parametrize("precision", [0, 1, 2])
-def test_emu_resident_cg_equals_streaming_cg(disks, shape, precision):
-    """The resident ARAP CG (rows in registers, u and e in LDS, halo through
-    global memory) performs the streaming CG's arithmetic in the same order:
-    both give the same layout bit for bit, for every team shape and
-    precision.  The islands span 2 to 12 slices, so some members hold their
-    rows resident and (with one wavefront per member) the largest island
-    exceeds a lane's register slots and streams in both runs."""
+@pytest.mark.parametrize("order", ["scan", "shuffled"])
+def test_emu_resident_cg_equals_streaming_cg(disks, shape, precision, order):
+    """The resident ARAP CG (rows in registers, u and e in LDS, halo windows
+    filled from global memory) performs the streaming CG's arithmetic in the
+    same order: both give the same layout bit for bit, for every team shape
+    and precision.  The islands span 2 to 12 slices, so some members hold
+    their rows resident and (with one wavefront per member) the largest island
+    exceeds a lane's register slots and streams in both runs.  With shuffled
+    vertices the largest island's bandwidth exceeds the halo window, so team
+    members also gather from global memory."""
     from oracle import param_cpu as oracle
 
+    if order == "shuffled":
+        disks = [_shuffled(P, T, 40 + i) for i, (P, T) in enumerate(disks)]
     U0 = np.concatenate([oracle.lscm(P, T) for P, T in disks])
     out = []
     for resident in (True, False):
-        e = EmuPlan(disks, *shape, resident=resident)
+        e = EmuPlan(disks, *shape, resident=resident, no_coarse=True)
         # all-FP32 stalls near its rounding floor, so it gets a looser tolerance
         out.append(e.arap(U0, 3, tol=0.0, cg_tol=1e-4 if precision == 1 else 1e-7, precision=precision))
         assert e.stats()["converged"].all()

@@ -23,6 +23,8 @@ UVA_ENOBOUNDARY = -5
 UVA_PLAN_BATCH = 1
 UVA_PLAN_NO_WAVE = 2
 UVA_PLAN_NO_TEAMS = 4
+UVA_PLAN_KEEP_ORDER = 8
+UVA_PLAN_NO_COARSE = 16
 UVA_MODE_REFINE = 0
 UVA_MODE_BATCH = 1
 UVA_PREC_MIXED = 0

@@ -44,6 +44,8 @@ struct uva_plan {
     int clock_khz = 0;                    // device wall-clock rate (wall_clock64 ticks per ms)
     std::vector<double> P;                 // host positions (LSCM pins given later)
     std::vector<int> lpin0, lpin1, v0;     // host copies
+    std::vector<int> vinv;                 // caller's vertex id -> plan vertex
+    double2* ubuf[3] = {nullptr, nullptr, nullptr};  // plan-order copies of the caller's vertex arrays
     int* d_lpin0 = nullptr;
     int* d_lpin1 = nullptr;
     double* d_ldist = nullptr;
@@ -89,6 +91,22 @@ int launch(uva_plan* pl, int kind, int prec, const uva::TaskArgs& a, int slot, h
     return UVA_OK;
 }
 
+// The plan keeps its vertices in its own order (Morton order inside each
+// island); every entry point copies the caller's vertex arrays (n x 2
+// float64) into plan order on the stream and the results back.
+double2* to_plan(uva_plan* pl, const double* ext, int k, hipStream_t s, int& rc) {
+    if (rc) return nullptr;
+    hipError_t e = uva::launch_permute(pl->dp, (const double2*)ext, pl->ubuf[k], true, s);
+    if (e != hipSuccess) rc = fail(UVA_EHIP, std::string("permute: ") + hipGetErrorString(e));
+    return pl->ubuf[k];
+}
+
+int from_plan(uva_plan* pl, int k, double* ext, hipStream_t s, int rc) {
+    if (rc) return rc;
+    HIP_TRY(uva::launch_permute(pl->dp, pl->ubuf[k], (double2*)ext, false, s));
+    return UVA_OK;
+}
+
 int reset_queue(uva_plan* pl, int slots, hipStream_t s) {
     HIP_TRY(uva::launch_fill_int(pl->dp.queue, uva::kQueueStride * slots, 0, s));
     if (pl->dp.n_teams > 0) HIP_TRY(uva::launch_fill_int(pl->dp.team_cnt, slots * pl->dp.n_teams, 0, s));
@@ -106,9 +124,12 @@ uva::TaskArgs base_args(double* U) {
 // One sequence of launches (queue reset, prologue, iterations) with slots
 // reused in chunks of kMaxSlots.  With ev (2 events per launch), ev[2k] and
 // ev[2k+1] bracket launch k (0 = prologue) on the stream.
-int arap_sequence(uva_plan* pl, double* U, const uva_arap_opts* o, hipStream_t s,
+int arap_sequence(uva_plan* pl, double* Uext, const uva_arap_opts* o, hipStream_t s,
                   const std::vector<hipEvent_t>* ev = nullptr) {
-    uva::TaskArgs a = base_args(U);
+    int prc = UVA_OK;
+    double2* U = to_plan(pl, Uext, 0, s, prc);
+    if (prc) return prc;
+    uva::TaskArgs a = base_args((double*)U);
     a.cg_tol = o->cg_tol;
     a.cg_max = o->cg_max_iters > 0 ? o->cg_max_iters : kDefaultMaxIters;
     a.tol = o->tol;
@@ -129,7 +150,7 @@ int arap_sequence(uva_plan* pl, double* U, const uva_arap_opts* o, hipStream_t s
         }
         done += chunk;
     }
-    return UVA_OK;
+    return from_plan(pl, 0, Uext, s, UVA_OK);
 }
 
 bool valid_plan(const uva_plan* p) { return p != nullptr; }
@@ -198,7 +219,7 @@ int upload_plan(uva::HostPlan& hp, const double* P, hipStream_t s, uva_plan** ou
     d.n_tri = hp.n_tri;
     d.mode_batch = hp.batch;
     UP(isl_v0, isl_v0); UP(isl_nv, isl_nv); UP(isl_t0, isl_t0); UP(isl_nt, isl_nt);
-    UP(isl_s0, isl_s0); UP(isl_ns, isl_ns); UP(isl_pin, isl_pin); UP(isl_wpin, isl_wpin);
+    UP(isl_s0, isl_s0); UP(isl_ns, isl_ns); UP(isl_pin, isl_pin); UP(isl_wpin, isl_wpin); UP(isl_bw, isl_bw);
     UP(tv0, tv0); UP(tv1, tv1); UP(tv2, tv2);
     UP(fx1, fx1); UP(fx2, fx2); UP(fy2, fy2); UP(farea, farea); UP(fok, fok);
     UP(inc_ptr, inc_ptr); UP(inc_tri, inc_tri);
@@ -207,7 +228,17 @@ int upload_plan(uva::HostPlan& hp, const double* P, hipStream_t s, uva_plan** ou
     UP(adj_ptr, adj_ptr); UP(adj_idx, adj_idx); UP(vbnd, vbnd);
     UP(order_group, order_group); UP(order_wave, order_wave);
     UP(grp_mem, grp_mem); UP(grp_m, grp_m); UP(grp_team, grp_team);
+    UP(vperm, vperm); UP(isl_cofs, isl_cofs); UP(isl_cg, isl_cg); UP(cinv, cinv);
 #undef UP
+    pl->vinv.assign(hp.n_vert, 0);
+    for (int v = 0; v < hp.n_vert; ++v) pl->vinv[hp.vperm[v]] = v;
+    {
+        hipError_t e;
+        const size_t nsl = hp.sl_ptr.empty() ? 1 : hp.sl_ptr.size();
+        if ((e = pl->alloc(&d.crc, nsl)) != hipSuccess) return bail(UVA_ENOMEM, "coarse vector");
+        for (int k = 0; k < 3; ++k)
+            if ((e = pl->alloc(&pl->ubuf[k], (size_t)hp.n_vert)) != hipSuccess) return bail(UVA_ENOMEM, "vertex buffers");
+    }
     {
         const double* g = nullptr;
         hipError_t e = pl->upload(&g, hp.inc_g, s);
@@ -340,7 +371,8 @@ int uva_plan_create(uva_plan** out, int64_t n_islands, const int64_t* vert_off, 
         err = uva::build_host_plan(hp, n_islands, vert_off, tri_off, P, T, pins, X, area, cots,
                                    (flags & UVA_PLAN_BATCH) != 0, nullptr,
                                    (flags & UVA_PLAN_NO_WAVE) ? -1 : uva::kWaveMaxRows,
-                                   (flags & UVA_PLAN_NO_TEAMS) ? 0 : team_slots, n_xcd);
+                                   (flags & UVA_PLAN_NO_TEAMS) ? 0 : team_slots, n_xcd,
+                                   (flags & UVA_PLAN_KEEP_ORDER) == 0, (flags & UVA_PLAN_NO_COARSE) == 0);
     } catch (const std::bad_alloc&) {
         return fail(UVA_ENOMEM, "host allocation failed while building the plan");
     }
@@ -430,10 +462,11 @@ int uva_lscm(uva_plan* plan, double* U, const int64_t* pins, double tol, int32_t
             int nv = (i + 1 < d.n_isl ? plan->v0[i + 1] : d.n_vert) - plan->v0[i];
             if (pins[2 * i] < 0 || pins[2 * i] >= nv || pins[2 * i + 1] < 0 || pins[2 * i + 1] >= nv)
                 return fail(UVA_EINVAL, "LSCM pin outside its island");
-            a[i] = plan->v0[i] + (int)pins[2 * i];
-            b[i] = plan->v0[i] + (int)pins[2 * i + 1];
-            const double* p0 = &plan->P[3 * (size_t)a[i]];
-            const double* p1 = &plan->P[3 * (size_t)b[i]];
+            const int ca = plan->v0[i] + (int)pins[2 * i], cb = plan->v0[i] + (int)pins[2 * i + 1];
+            a[i] = plan->vinv[ca];
+            b[i] = plan->vinv[cb];
+            const double* p0 = &plan->P[3 * (size_t)ca];
+            const double* p1 = &plan->P[3 * (size_t)cb];
             dist[i] = std::sqrt((p1[0] - p0[0]) * (p1[0] - p0[0]) + (p1[1] - p0[1]) * (p1[1] - p0[1]) +
                                 (p1[2] - p0[2]) * (p1[2] - p0[2]));
         }
@@ -447,12 +480,12 @@ int uva_lscm(uva_plan* plan, double* U, const int64_t* pins, double tol, int32_t
                 return fail(UVA_ENOBOUNDARY, "island " + std::to_string(i) +
                                                  " has no boundary (closed surface): needs a cut");
     }
-    uva::TaskArgs a = base_args(U);
+    uva::TaskArgs a = base_args((double*)plan->ubuf[0]);
     a.cg_tol = tol;
     a.cg_max = max_iters > 0 ? max_iters : kDefaultMaxIters;
     int rc = reset_queue(plan, 1, s);
-    if (rc) return rc;
-    return launch(plan, uva::TASK_LSCM, UVA_PREC_F64, a, 0, s);
+    if (!rc) rc = launch(plan, uva::TASK_LSCM, UVA_PREC_F64, a, 0, s);
+    return from_plan(plan, 0, U, s, rc);
 }
 
 int uva_arap(uva_plan* plan, double* U, const uva_arap_opts* o, void* stream) {
@@ -522,33 +555,32 @@ int uva_slim(uva_plan* plan, double* U, const uva_slim_opts* o, void* stream) {
         if (plan->alloc(&d.sw, d.n_tri) != hipSuccess || plan->alloc(&d.sdi, d.n_vert) != hipSuccess)
             return fail(UVA_ENOMEM, "SLIM scratch");
     }
-    uva::TaskArgs a = base_args(U);
+    int rc = UVA_OK;
+    uva::TaskArgs a = base_args((double*)to_plan(plan, U, 0, s, rc));
     a.cg_tol = o->cg_tol;
     a.cg_max = o->cg_max_iters > 0 ? o->cg_max_iters : kDefaultMaxIters;
     a.tol = o->tol;
     const int total = 1 + o->iters;
-    for (int done = 0; done < total;) {
+    for (int done = 0; done < total && !rc;) {
         const int chunk = std::min(total - done, kMaxSlots);
-        int rc = reset_queue(plan, chunk, s);
-        if (rc) return rc;
-        for (int k = 0; k < chunk; ++k) {
+        rc = reset_queue(plan, chunk, s);
+        for (int k = 0; k < chunk && !rc; ++k)
             rc = launch(plan, done + k == 0 ? uva::TASK_SLIM_PROLOGUE : uva::TASK_SLIM_ITER, UVA_PREC_F64, a, k, s);
-            if (rc) return rc;
-        }
         done += chunk;
     }
-    return UVA_OK;
+    return from_plan(plan, 0, U, s, rc);
 }
 
 int uva_local_rhs(uva_plan* plan, const double* U, double* b, int32_t mode, void* stream) {
     if (!valid_plan(plan) || !U || !b) return fail(UVA_EINVAL, "null argument");
     hipStream_t s = (hipStream_t)stream;
-    uva::TaskArgs a = base_args(const_cast<double*>(U));
-    a.out = (double2*)b;
+    int rc = UVA_OK;
+    uva::TaskArgs a = base_args((double*)to_plan(plan, U, 0, s, rc));
+    a.out = plan->ubuf[1];
     a.batch = mode == UVA_MODE_BATCH;
-    int rc = reset_queue(plan, 1, s);
-    if (rc) return rc;
-    return launch(plan, uva::TASK_LOCAL_RHS, UVA_PREC_MIXED, a, 0, s);
+    if (!rc) rc = reset_queue(plan, 1, s);
+    if (!rc) rc = launch(plan, uva::TASK_LOCAL_RHS, UVA_PREC_MIXED, a, 0, s);
+    return from_plan(plan, 1, b, s, rc);
 }
 
 int uva_cg_solve(uva_plan* plan, const double* b, double* x, double tol, int32_t max_iters, int32_t precision,
@@ -557,37 +589,40 @@ int uva_cg_solve(uva_plan* plan, const double* b, double* x, double tol, int32_t
     if (!(tol > 0.0)) return fail(UVA_EINVAL, "tol must be positive");
     if (precision < UVA_PREC_MIXED || precision > UVA_PREC_F64) return fail(UVA_EINVAL, "bad precision");
     hipStream_t s = (hipStream_t)stream;
-    uva::TaskArgs a = base_args(x);
-    a.in = (const double2*)b;
+    int rc = UVA_OK;
+    uva::TaskArgs a = base_args((double*)to_plan(plan, x, 0, s, rc));
+    a.in = to_plan(plan, b, 1, s, rc);
     a.cg_tol = tol;
     a.cg_max = max_iters > 0 ? max_iters : kDefaultMaxIters;
-    int rc = reset_queue(plan, 1, s);
-    if (rc) return rc;
-    return launch(plan, uva::TASK_CG, precision, a, 0, s);
+    if (!rc) rc = reset_queue(plan, 1, s);
+    if (!rc) rc = launch(plan, uva::TASK_CG, precision, a, 0, s);
+    return from_plan(plan, 0, x, s, rc);
 }
 
 int uva_max_step(uva_plan* plan, const double* U, const double* D, double* tmax, int32_t mode, void* stream) {
     if (!valid_plan(plan) || !U || !D || !tmax) return fail(UVA_EINVAL, "null argument");
     hipStream_t s = (hipStream_t)stream;
-    uva::TaskArgs a = base_args(const_cast<double*>(U));
-    a.in = (const double2*)D;
+    int rc = UVA_OK;
+    uva::TaskArgs a = base_args((double*)to_plan(plan, U, 0, s, rc));
+    a.in = to_plan(plan, D, 1, s, rc);
     a.out_isl = tmax;
     a.batch = mode == UVA_MODE_BATCH;
-    int rc = reset_queue(plan, 1, s);
-    if (rc) return rc;
-    return launch(plan, uva::TASK_MAX_STEP, UVA_PREC_MIXED, a, 0, s);
+    if (!rc) rc = reset_queue(plan, 1, s);
+    if (!rc) rc = launch(plan, uva::TASK_MAX_STEP, UVA_PREC_MIXED, a, 0, s);
+    return rc;
 }
 
 int uva_energy(uva_plan* plan, const double* U, int32_t kind, double* e, void* stream) {
     if (!valid_plan(plan) || !U || !e) return fail(UVA_EINVAL, "null argument");
     if (kind != 0 && kind != 1) return fail(UVA_EINVAL, "kind must be 0 (ARAP) or 1 (symmetric Dirichlet)");
     hipStream_t s = (hipStream_t)stream;
-    uva::TaskArgs a = base_args(const_cast<double*>(U));
+    int rc = UVA_OK;
+    uva::TaskArgs a = base_args((double*)to_plan(plan, U, 0, s, rc));
     a.out_isl = e;
     a.kind = kind;
-    int rc = reset_queue(plan, 1, s);
-    if (rc) return rc;
-    return launch(plan, uva::TASK_ENERGY, UVA_PREC_MIXED, a, 0, s);
+    if (!rc) rc = reset_queue(plan, 1, s);
+    if (!rc) rc = launch(plan, uva::TASK_ENERGY, UVA_PREC_MIXED, a, 0, s);
+    return rc;
 }
 
 int uva_energy_fwd(uva_plan* plan, const double* U, const double* w, int32_t kind, double* e, void* stream) {
@@ -595,13 +630,14 @@ int uva_energy_fwd(uva_plan* plan, const double* U, const double* w, int32_t kin
     if (kind != 0 && kind != 1) return fail(UVA_EINVAL, "kind must be 0 (ARAP) or 1 (symmetric Dirichlet)");
     if (plan->dp.mode_batch) return fail(UVA_EINVAL, "the energy gradients need a plan built with refine conventions");
     hipStream_t s = (hipStream_t)stream;
-    uva::TaskArgs a = base_args(const_cast<double*>(U));
+    int rc = UVA_OK;
+    uva::TaskArgs a = base_args((double*)to_plan(plan, U, 0, s, rc));
     a.out_isl = e;
     a.kind = kind;
     a.w = w;
-    int rc = reset_queue(plan, 1, s);
-    if (rc) return rc;
-    return launch(plan, uva::TASK_ENERGY, UVA_PREC_MIXED, a, 0, s);
+    if (!rc) rc = reset_queue(plan, 1, s);
+    if (!rc) rc = launch(plan, uva::TASK_ENERGY, UVA_PREC_MIXED, a, 0, s);
+    return rc;
 }
 
 int uva_energy_bwd(uva_plan* plan, const double* U, const double* w, int32_t kind, const double* ge, double* gU,
@@ -610,15 +646,16 @@ int uva_energy_bwd(uva_plan* plan, const double* U, const double* w, int32_t kin
     if (kind != 0 && kind != 1) return fail(UVA_EINVAL, "kind must be 0 (ARAP) or 1 (symmetric Dirichlet)");
     if (plan->dp.mode_batch) return fail(UVA_EINVAL, "the energy gradients need a plan built with refine conventions");
     hipStream_t s = (hipStream_t)stream;
-    uva::TaskArgs a = base_args(const_cast<double*>(U));
+    int rc = UVA_OK;
+    uva::TaskArgs a = base_args((double*)to_plan(plan, U, 0, s, rc));
     a.kind = kind;
     a.w = w;
     a.gin = ge;
-    a.out = (double2*)gU;
+    a.out = plan->ubuf[1];
     a.gw = gw;
-    int rc = reset_queue(plan, 1, s);
-    if (rc) return rc;
-    return launch(plan, uva::TASK_ENERGY_BWD, UVA_PREC_F64, a, 0, s);
+    if (!rc) rc = reset_queue(plan, 1, s);
+    if (!rc) rc = launch(plan, uva::TASK_ENERGY_BWD, UVA_PREC_F64, a, 0, s);
+    return from_plan(plan, 1, gU, s, rc);
 }
 
 int uva_adjoint_cg(uva_plan* plan, const double* g, double* lam, double tol, int32_t max_iters, int32_t precision,
@@ -627,62 +664,70 @@ int uva_adjoint_cg(uva_plan* plan, const double* g, double* lam, double tol, int
     if (!(tol > 0.0)) return fail(UVA_EINVAL, "tol must be positive");
     if (precision < UVA_PREC_MIXED || precision > UVA_PREC_F64) return fail(UVA_EINVAL, "bad precision");
     hipStream_t s = (hipStream_t)stream;
-    uva::TaskArgs a = base_args(lam);
-    a.in = (const double2*)g;
+    int rc = UVA_OK;
+    uva::TaskArgs a = base_args((double*)plan->ubuf[0]);
+    a.in = to_plan(plan, g, 1, s, rc);
     a.cg_tol = tol;
     a.cg_max = max_iters > 0 ? max_iters : kDefaultMaxIters;
     a.zero_x = 1;
-    int rc = reset_queue(plan, 1, s);
-    if (rc) return rc;
-    return launch(plan, uva::TASK_CG, precision, a, 0, s);
+    if (!rc) rc = reset_queue(plan, 1, s);
+    if (!rc) rc = launch(plan, uva::TASK_CG, precision, a, 0, s);
+    return from_plan(plan, 0, lam, s, rc);
 }
 
 int uva_arap_step_bwd(uva_plan* plan, const double* U, const double* lam, double* gU, void* stream) {
     if (!valid_plan(plan) || !U || !lam || !gU) return fail(UVA_EINVAL, "null argument");
     if (plan->dp.mode_batch) return fail(UVA_EINVAL, "the step backward needs a plan built with refine conventions");
     hipStream_t s = (hipStream_t)stream;
-    uva::TaskArgs a = base_args(const_cast<double*>(U));
-    a.in = (const double2*)lam;
-    a.out = (double2*)gU;
-    int rc = reset_queue(plan, 1, s);
-    if (rc) return rc;
-    return launch(plan, uva::TASK_STEP_BWD, UVA_PREC_F64, a, 0, s);
+    int rc = UVA_OK;
+    uva::TaskArgs a = base_args((double*)to_plan(plan, U, 0, s, rc));
+    a.in = to_plan(plan, lam, 1, s, rc);
+    a.out = plan->ubuf[2];
+    if (!rc) rc = reset_queue(plan, 1, s);
+    if (!rc) rc = launch(plan, uva::TASK_STEP_BWD, UVA_PREC_F64, a, 0, s);
+    return from_plan(plan, 2, gU, s, rc);
 }
 
 int uva_signed_areas(uva_plan* plan, const double* U, double* sa, void* stream) {
     if (!valid_plan(plan) || !U || !sa) return fail(UVA_EINVAL, "null argument");
-    HIP_TRY(uva::launch_signed_areas(plan->dp, (const double2*)U, sa, (hipStream_t)stream));
+    int rc = UVA_OK;
+    const double2* Up = to_plan(plan, U, 0, (hipStream_t)stream, rc);
+    if (rc) return rc;
+    HIP_TRY(uva::launch_signed_areas(plan->dp, Up, sa, (hipStream_t)stream));
     return UVA_OK;
 }
 
 int uva_flips_per_chart(uva_plan* plan, const double* U, double* counts, void* stream) {
     if (!valid_plan(plan) || !U || !counts) return fail(UVA_EINVAL, "null argument");
     hipStream_t s = (hipStream_t)stream;
-    uva::TaskArgs a = base_args(const_cast<double*>(U));
+    int rc = UVA_OK;
+    uva::TaskArgs a = base_args((double*)to_plan(plan, U, 0, s, rc));
     a.out_isl = counts;
-    int rc = reset_queue(plan, 1, s);
-    if (rc) return rc;
-    return launch(plan, uva::TASK_FLIPS, UVA_PREC_MIXED, a, 0, s);
+    if (!rc) rc = reset_queue(plan, 1, s);
+    if (!rc) rc = launch(plan, uva::TASK_FLIPS, UVA_PREC_MIXED, a, 0, s);
+    return rc;
 }
 
 int uva_repair_jacobi(uva_plan* plan, double* U, int32_t rounds, void* stream) {
     if (!valid_plan(plan) || !U) return fail(UVA_EINVAL, "null argument");
     if (rounds < 0) return fail(UVA_EINVAL, "rounds must be >= 0");
     hipStream_t s = (hipStream_t)stream;
-    uva::TaskArgs a = base_args(U);
+    int rc = UVA_OK;
+    uva::TaskArgs a = base_args((double*)to_plan(plan, U, 0, s, rc));
     a.rounds = rounds;
-    int rc = reset_queue(plan, 1, s);
-    if (rc) return rc;
-    return launch(plan, uva::TASK_REPAIR, UVA_PREC_MIXED, a, 0, s);
+    if (!rc) rc = reset_queue(plan, 1, s);
+    if (!rc) rc = launch(plan, uva::TASK_REPAIR, UVA_PREC_MIXED, a, 0, s);
+    return from_plan(plan, 0, U, s, rc);
 }
 
 int uva_orient_positive(uva_plan* plan, double* U, void* stream) {
     if (!valid_plan(plan) || !U) return fail(UVA_EINVAL, "null argument");
     hipStream_t s = (hipStream_t)stream;
-    uva::TaskArgs a = base_args(U);
-    int rc = reset_queue(plan, 1, s);
-    if (rc) return rc;
-    return launch(plan, uva::TASK_ORIENT, UVA_PREC_MIXED, a, 0, s);
+    int rc = UVA_OK;
+    uva::TaskArgs a = base_args((double*)to_plan(plan, U, 0, s, rc));
+    if (!rc) rc = reset_queue(plan, 1, s);
+    if (!rc) rc = launch(plan, uva::TASK_ORIENT, UVA_PREC_MIXED, a, 0, s);
+    return from_plan(plan, 0, U, s, rc);
 }
 
 int uva_repair_seq(double* U, int64_t n, const int64_t* T, int64_t t, int32_t rounds) {

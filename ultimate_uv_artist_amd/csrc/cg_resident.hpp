@@ -1,37 +1,75 @@
 // Island-resident preconditioned CG (included by kernels.hip).
 //
-// The same Chronopoulos-Gear Jacobi-PCG as cg_island (same recurrences, same
+// The same Chronopoulos-Gear PCG as cg_island (same recurrences, same
 // per-lane accumulation order), for islands whose rows fit on the team's
-// compute units.  Each lane keeps r, p, s = A p, w = A u, the diagonal and
-// its inverse for its rows in registers (at most 8 rows in FP32, 4 in FP64 at
-// 512 threads): slot j holds row lane of the local slice j*W + wave of the
-// member's slice range.  The preconditioned residual
-// u = D^-1 r, which the SpMV gathers, and the correction e live in LDS, so a
-// CG iteration reads only the matrix from memory.  Members of a
-// multi-workgroup team also publish u to global memory; their SpMV gathers
-// rows owned by other members from there (the halo) and their own rows from
-// LDS.
+// compute units.  Each lane keeps r, p, s = A p and w = A u for its rows in
+// registers (at most 6 rows in FP32, 4 in FP64 at 512 threads): slot j holds
+// row lane of the local slice j*W + wave of the member's slice range.  The
+// preconditioned residual u = M^-1 r, which the SpMV gathers, the correction
+// e and the diagonal D and D^-1 live in LDS, so a CG iteration reads only the
+// matrix from memory.
+//
+// Preconditioner: Jacobi, plus (when the plan has one for the island) an
+// additive coarse correction
+//     M^-1 r = D^-1 r + Z Ac^-1 Z^T r,   Ac = Z^T A Z,
+// over aggregates of g consecutive 64-row slices.  The plan orders vertices
+// by Morton code, so slices are compact surface patches and the coarse level
+// removes the smooth error that Jacobi-PCG needs O(sqrt(n)) iterations for.
+// Z^T r is a wavefront sum per slice; the slices' sums go through global
+// memory to the whole team, and each member applies the rows of Ac^-1 (kept
+// in LDS for the whole solve) that its own aggregates need.
+//
+// Members of a multi-workgroup team keep u for a window of rows that extends
+// H = min(bandwidth, kHalo) rows beyond each end of their own range.  They
+// publish the rows other members' windows cover to global memory, and after
+// each team barrier copy their window's foreign rows from there into LDS in
+// one coalesced pass, so every gather of the SpMV reads LDS.  (An island whose
+// bandwidth exceeds kHalo publishes all rows and gathers the columns outside
+// the window from global memory.)
 #pragma once
+
+UVA_HD __forceinline__ int align16(int b) { return (b + 15) & ~15; }
+
+// LDS bytes of a member holding `rows` rows (its slice range): the u window
+// (rows + 2 * halo), e, D^-1 and D; with a coarse level of nc aggregates, of
+// which the member's slices touch na: na rows of Ac^-1 (FP32), the
+// restricted residual of all aggregates and the correction of the member's.
+template <class V2, class V>
+UVA_HD __forceinline__ int res_vec_bytes(int rows, int halo) {
+    return align16((rows + 2 * halo) * (int)sizeof(V2) + rows * ((int)sizeof(V2) + 2 * (int)sizeof(V)));
+}
+UVA_HD __forceinline__ int res_coarse_bytes(int nc, int na) {
+    return nc > 0 ? align16(na * nc * 4) + (nc + na) * 16 : 0;
+}
 
 template <int PREC, int RPT, class Team>
 struct ResCG {
     using PT = Prec<PREC>;
     using V = typename PT::V;
     using V2 = typename Vec2<V>::type;
-    static constexpr int kG = 4;   // matrix entries whose loads are issued together
+    static constexpr int kG = 2;   // matrix entries per slot whose loads are issued together
 
     Team& tm;
     const DevPlan& P;
     const Isl& d;
     int W, wv, lane, slo, shi, rlo, rhi;
-    bool multi;
-    V2* lu;     // LDS: u of the member's rows
+    int H, wlo, whi;   // u window [wlo, whi) in LDS covers [rlo, rhi) and H rows each side
+    bool multi;        // several members (halo exchange through global memory)
+    bool gfull;        // the bandwidth exceeds the window: publish every row
+    bool coarse;       // additive coarse correction
+    int g, nc, alo, na;   // g slices per aggregate, nc aggregates; the member's are [alo, alo + na)
+    V2* lu;     // LDS: u of the window, row - wlo
     V2* le;     // LDS: e of the member's rows
-    V2* gu;     // global: u of every row (teams only)
+    V* ldv;     // LDS: D^-1
+    V* ldg;     // LDS: D (with the pin weight)
+    float* lci;     // LDS: rows alo .. alo + na - 1 of Ac^-1
+    double2* lrc;   // LDS: Z^T r of every aggregate
+    double2* ly;    // LDS: the member's aggregates' coarse correction
+    V2* gu;     // global: u of the published rows (teams only)
     V2 r[RPT], p[RPT], s[RPT], w[RPT];
-    V dv[RPT], dg[RPT];
 
-    UVA_HD __forceinline__ ResCG(Team& t, const DevPlan& P_, const Isl& d_) : tm(t), P(P_), d(d_) {
+    UVA_HD __forceinline__ ResCG(Team& t, const DevPlan& P_, const Isl& d_, bool use_coarse)
+        : tm(t), P(P_), d(d_) {
         W = tm.nwaves();
         wv = tm.wave();
         lane = tm.lane();
@@ -40,19 +78,42 @@ struct ResCG {
         rlo = d.v0 + slo * kSlice;
         rhi = min(d.v0 + shi * kSlice, d.v0 + d.nv);
         multi = tm.multi();
-        lu = (V2*)tm.lvec();
-        le = lu + tm.lvec_bytes() / (2 * (int)sizeof(V2));
+        H = multi ? min(d.bw, kHalo) : 0;
+        gfull = multi && d.bw > kHalo;
+        wlo = max(rlo - H, d.v0);
+        whi = min(rhi + H, d.v0 + d.nv);
+        coarse = use_coarse;
+        g = d.cg;
+        nc = (d.ns + g - 1) / g;
+        alo = slo / g;
+        na = shi > slo ? (shi - 1) / g - alo + 1 : 0;
+        const int rows = (shi - slo) * kSlice;
+        char* base = (char*)tm.lvec();
+        lu = (V2*)base;
+        le = lu + rows + 2 * (multi ? kHalo : 0);
+        ldv = (V*)(le + rows);
+        ldg = ldv + rows;
+        lci = (float*)(base + res_vec_bytes<V2, V>(rows, multi ? kHalo : 0));
+        lrc = (double2*)((char*)lci + align16(na * nc * 4));
+        ly = lrc + nc;
         gu = (V2*)P.vu;
 #pragma unroll
         for (int j = 0; j < RPT; ++j) {
             const int l = (j * W + wv) * kSlice + lane;
             const int row = rlo + l;
-            const bool ok = slot(j) && row < rhi;
-            dv[j] = ok ? (V)dinv_of<OP_ARAP>(P, d, row) : (V)0;
-            dg[j] = ok ? (V)diag_of<OP_ARAP>(P, d, row) : (V)0;
+            if (slot(j) && row < rhi) {
+                ldv[l] = (V)dinv_of<OP_ARAP>(P, d, row);
+                ldg[l] = (V)diag_of<OP_ARAP>(P, d, row);
+            }
             r[j].x = 0; r[j].y = 0;
             p[j] = r[j]; s[j] = r[j]; w[j] = r[j];
         }
+        if (coarse) {
+            const float* gci = P.cinv + d.cofs + (long long)alo * nc;
+            for (int i = tm.tid(); i < na * nc; i += tm.tstride()) lci[i] = gci[i];
+        }
+        // each pass reads D^-1 and D of its own rows only, and the coarse
+        // rows after a team barrier
     }
     // slot j exists for this wavefront (uniform)
     UVA_HD __forceinline__ bool slot(int j) const { return slo + j * W + wv < shi; }
@@ -65,55 +126,152 @@ struct ResCG {
         asm volatile("" : "+v"(lane));
 #endif
     }
-
-    UVA_HD __forceinline__ V2 gather(int c) const {
-        if (multi && (c < rlo || c >= rhi)) return gu[c];
-        return lu[c - rlo];
+    // u of an own row into the window (and to global memory when another
+    // member's window or gathers may need it)
+    UVA_HD __forceinline__ void publish(int row, V2 u) {
+        lu[row - wlo] = u;
+        if (multi && (gfull || row < rlo + H || row >= rhi - H)) gu[row] = u;
     }
-
-    // w = A u over the member's rows (u = D^-1 r from registers for the own
-    // row, LDS / global for the columns) and the dot w.u.
-    UVA_HD __forceinline__ double2 spmv() {
+    // After the team barrier: the window's foreign rows from global memory.
+    UVA_HD __forceinline__ void fill_halo() {
+        if (!multi) return;
+        const int nlo = rlo - wlo, n = nlo + (whi - rhi);
+        for (int i = tm.tid(); i < n; i += tm.tstride()) {
+            const int row = i < nlo ? wlo + i : rhi + (i - nlo);
+            lu[row - wlo] = gu[row];
+        }
+        tm.bsync();
+    }
+    // Z^T r of slot j's slice (a wavefront sum, FP64) to global memory.
+    UVA_HD __forceinline__ void restrict_slot(int j, bool ok) {
+        const double2 rc = tm.wsum2(make_double2(ok ? (double)r[j].x : 0.0, ok ? (double)r[j].y : 0.0));
+        if (lane == 0) P.crc[d.s0 + slo + j * W + wv] = rc;
+    }
+    // After every slice's Z^T r is in global memory: Z^T r per aggregate,
+    // then y = Ac^-1 Z^T r for the member's aggregates, into LDS.
+    UVA_HD __forceinline__ void coarse_solve() {
+        tm.sync();
+        for (int b = tm.tid(); b < nc; b += tm.tstride()) {
+            double2 acc = make_double2(0.0, 0.0);
+            const int s1 = min((b + 1) * g, d.ns);
+            for (int sl = b * g; sl < s1; ++sl) {
+                const double2 v = P.crc[d.s0 + sl];
+                acc.x += v.x;
+                acc.y += v.y;
+            }
+            lrc[b] = acc;
+        }
+        tm.bsync();
+        for (int a = wv; a < na; a += W) {
+            double2 part = make_double2(0.0, 0.0);
+            for (int b = lane; b < nc; b += kSlice) {
+                const double c = (double)lci[a * nc + b];
+                const double2 v = lrc[b];
+                part.x = fma(c, v.x, part.x);
+                part.y = fma(c, v.y, part.y);
+            }
+            const double2 y = tm.wsum2(part);
+            if (lane == 0) ly[a] = y;
+        }
+        tm.bsync();
+    }
+    // u = M^-1 r of the member's rows into the window; returns the dots r.u
+    UVA_HD __forceinline__ double2 precondition() {
+        if (coarse) coarse_solve();
         refresh();
-        double2 acc = make_double2(0.0, 0.0);
+        double2 ru = make_double2(0.0, 0.0);
 #pragma unroll
         for (int j = 0; j < RPT; ++j) {
             if (!slot(j)) continue;
             const int l = lrow(j);
             const int row = rlo + l;
-            const bool ok = row < rhi;
-            const int sg = d.s0 + slo + j * W + wv;
-            const int base = P.sl_ptr[sg];
-            const int width = (P.sl_ptr[sg + 1] - base) >> 6;
+            if (row >= rhi) continue;
+            const V dvj = ldv[l];
+            V2 u;
+            u.x = dvj * r[j].x;
+            u.y = dvj * r[j].y;
+            if (coarse) {
+                const double2 y = ly[(slo + j * W + wv) / g - alo];
+                u.x += (V)y.x;
+                u.y += (V)y.y;
+            }
+            publish(row, u);
+            ru.x += acc_mul<PREC>(r[j].x, u.x);
+            ru.y += acc_mul<PREC>(r[j].y, u.y);
+        }
+        return ru;
+    }
+
+    UVA_HD __forceinline__ V2 gather(int c) const {
+        if (gfull && (c < wlo || c >= whi)) return gu[c];
+        return lu[c - wlo];
+    }
+
+    // w = A u over the member's rows (own u from the window, columns from the
+    // window or, beyond it, global memory) and the dot w.u.  The matrix loads
+    // of every slot are issued together, kG entries per slot at a time, so a
+    // wavefront waits for one memory round trip per kG entries of its widest
+    // slice rather than one per slot.  Entries are summed in row order, as in
+    // spmv_row2.
+    UVA_HD __forceinline__ double2 spmv() {
+        refresh();
+        int base[RPT], wid[RPT];    // uniform per wavefront
+        bool ok[RPT];               // the lane's row of slot j exists
+        int wmax = 0;
+#pragma unroll
+        for (int j = 0; j < RPT; ++j) {
+            base[j] = 0;
+            wid[j] = 0;
+            if (slot(j)) {
+                const int sg = d.s0 + slo + j * W + wv;
+                base[j] = uniform_i(P.sl_ptr[sg]);
+                wid[j] = (uniform_i(P.sl_ptr[sg + 1]) - base[j]) >> 6;
+                wmax = wid[j] > wmax ? wid[j] : wmax;
+            }
+            const int l = lrow(j);
+            ok[j] = slot(j) && rlo + l < rhi;
             V2 uo;
-            uo.x = dv[j] * r[j].x;
-            uo.y = dv[j] * r[j].y;
-            V yx = dg[j] * uo.x, yy = dg[j] * uo.y;
-            for (int k0 = 0; k0 < width; k0 += kG) {
-                int c[kG];
-                V v[kG];
+            uo.x = 0; uo.y = 0;
+            if (ok[j]) uo = lu[rlo + l - wlo];
+            const V dgj = ok[j] ? ldg[l] : (V)0;
+            w[j].x = dgj * uo.x;
+            w[j].y = dgj * uo.y;
+        }
+        for (int k0 = 0; k0 < wmax; k0 += kG) {
+            // every lane loads (entries past a row's width read entry 0 of the
+            // matrix and are left out by select), so the loads of all slots
+            // form one straight run the compiler issues back to back
+            int c[RPT][kG];
+            V v[RPT][kG];
+#pragma unroll
+            for (int j = 0; j < RPT; ++j) {
 #pragma unroll
                 for (int q = 0; q < kG; ++q) {
-                    const bool in = ok && k0 + q < width;
-                    const int o = base + (k0 + q) * kSlice + lane;
-                    c[q] = in ? P.m_col[o] : rlo;
-                    v[q] = in ? mval<V>(P, o) : (V)0;
+                    const bool in = ok[j] && k0 + q < wid[j];
+                    const int o = in ? base[j] + (k0 + q) * kSlice + lane : 0;
+                    c[j][q] = P.m_col[o];
+                    v[j][q] = mval<V>(P, o);
+                    c[j][q] = in ? c[j][q] : wlo;
                 }
+            }
+#pragma unroll
+            for (int j = 0; j < RPT; ++j) {
 #pragma unroll
                 for (int q = 0; q < kG; ++q) {
-                    const bool in = ok && k0 + q < width;
-                    const V2 x = gather(c[q]);
-                    yx = in ? fma(v[q], x.x, yx) : yx;
-                    yy = in ? fma(v[q], x.y, yy) : yy;
+                    const bool in = ok[j] && k0 + q < wid[j];
+                    const V2 x = gather(c[j][q]);
+                    w[j].x = in ? fma(v[j][q], x.x, w[j].x) : w[j].x;
+                    w[j].y = in ? fma(v[j][q], x.y, w[j].y) : w[j].y;
                 }
             }
-            if (!ok) { yx = 0; yy = 0; }
-            w[j].x = yx;
-            w[j].y = yy;
-            if (ok) {
-                acc.x += acc_mul<PREC>(yx, uo.x);
-                acc.y += acc_mul<PREC>(yy, uo.y);
-            }
+        }
+        double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int j = 0; j < RPT; ++j) {
+            if (!ok[j]) continue;
+            const V2 uo = lu[rlo + lrow(j) - wlo];
+            acc.x += acc_mul<PREC>(w[j].x, uo.x);
+            acc.y += acc_mul<PREC>(w[j].y, uo.y);
         }
         return acc;
     }
@@ -126,7 +284,7 @@ struct ResCG {
         double relx = 0.0, rely = 0.0;
         for (int outer = 0;; ++outer) {
             // true residual r = b - A X (FP64 from the hi+lo matrix, or the
-            // all-FP32 form), u = D^-1 r, e = 0
+            // all-FP32 form), e = 0, u = M^-1 r
             double4 a1 = make_double4(0.0, 0.0, 0.0, 0.0);
             refresh();
 #pragma unroll
@@ -134,47 +292,48 @@ struct ResCG {
                 if (!slot(j)) continue;
                 const int l = lrow(j);
                 const int row = rlo + l;
-                if (row >= rhi) continue;
-                const int sg = d.s0 + slo + j * W + wv;
-                double rx, ry;
-                if (PT::kResid64) {
-                    const double2 y = spmv_row<double, OP_ARAP>(P, row, sg, lane, P.X64, diag_of<OP_ARAP>(P, d, row));
-                    const double2 bb = P.b64[row];
-                    rx = bb.x - y.x;
-                    ry = bb.y - y.y;
-                } else {
-                    const int base = P.sl_ptr[sg];
-                    const int width = (P.sl_ptr[sg + 1] - base) >> 6;
-                    const double2 xr = P.X64[row];
-                    const float dgf = (float)diag_of<OP_ARAP>(P, d, row);
-                    float yx = dgf * (float)xr.x, yy = dgf * (float)xr.y;
-                    int o = base + lane;
-                    for (int k = 0; k < width; ++k, o += kSlice) {
-                        const double2 xc = P.X64[P.m_col[o]];
-                        yx = fmaf(P.m_hi[o], (float)xc.x, yx);
-                        yy = fmaf(P.m_hi[o], (float)xc.y, yy);
+                const bool ok = row < rhi;
+                if (ok) {
+                    const int sg = d.s0 + slo + j * W + wv;
+                    double rx, ry;
+                    if (PT::kResid64) {
+                        const double2 y =
+                            spmv_row<double, OP_ARAP>(P, row, sg, lane, P.X64, diag_of<OP_ARAP>(P, d, row));
+                        const double2 bb = P.b64[row];
+                        rx = bb.x - y.x;
+                        ry = bb.y - y.y;
+                    } else {
+                        const int base = P.sl_ptr[sg];
+                        const int width = (P.sl_ptr[sg + 1] - base) >> 6;
+                        const double2 xr = P.X64[row];
+                        const float dgf = (float)diag_of<OP_ARAP>(P, d, row);
+                        float yx = dgf * (float)xr.x, yy = dgf * (float)xr.y;
+                        int o = base + lane;
+                        for (int k = 0; k < width; ++k, o += kSlice) {
+                            const double2 xc = P.X64[P.m_col[o]];
+                            yx = fmaf(P.m_hi[o], (float)xc.x, yx);
+                            yy = fmaf(P.m_hi[o], (float)xc.y, yy);
+                        }
+                        const double2 bb = P.b64[row];
+                        rx = (double)((float)bb.x - yx);
+                        ry = (double)((float)bb.y - yy);
                     }
-                    const double2 bb = P.b64[row];
-                    rx = (double)((float)bb.x - yx);
-                    ry = (double)((float)bb.y - yy);
+                    r[j].x = (V)rx;
+                    r[j].y = (V)ry;
+                    V2 zero;
+                    zero.x = 0;
+                    zero.y = 0;
+                    le[l] = zero;
+                    a1.x += PT::kResid64 ? rx * rx : (double)((float)rx * (float)rx);
+                    a1.y += PT::kResid64 ? ry * ry : (double)((float)ry * (float)ry);
                 }
-                r[j].x = (V)rx;
-                r[j].y = (V)ry;
-                V2 uv;
-                uv.x = dv[j] * r[j].x;
-                uv.y = dv[j] * r[j].y;
-                lu[l] = uv;
-                if (multi) gu[row] = uv;
-                V2 zero;
-                zero.x = 0;
-                zero.y = 0;
-                le[l] = zero;
-                a1.x += PT::kResid64 ? rx * rx : (double)((float)rx * (float)rx);
-                a1.y += PT::kResid64 ? ry * ry : (double)((float)ry * (float)ry);
-                a1.z += acc_mul<PREC>(r[j].x, uv.x);
-                a1.w += acc_mul<PREC>(r[j].y, uv.y);
+                if (coarse) restrict_slot(j, ok);
             }
+            const double2 ru = precondition();
+            a1.z += ru.x;
+            a1.w += ru.y;
             tm.sync();
+            fill_halo();
             const double2 wu = spmv();
             double4 S1, S2;
             tm.sum8(a1, make_double4(wu.x, wu.y, 0.0, 0.0), S1, S2);
@@ -194,34 +353,34 @@ struct ResCG {
                 for (int j = 0; j < RPT; ++j) {
                     if (!slot(j)) continue;
                     const int l = lrow(j);
-                    const int row = rlo + l;
-                    if (row >= rhi) continue;
-                    V2 uA;
-                    uA.x = dv[j] * r[j].x;
-                    uA.y = dv[j] * r[j].y;
-                    if (first) {
-                        p[j] = uA;
-                        s[j] = w[j];
-                    } else {
-                        p[j].x = fma(bx, p[j].x, uA.x); p[j].y = fma(by, p[j].y, uA.y);
-                        s[j].x = fma(bx, s[j].x, w[j].x); s[j].y = fma(by, s[j].y, w[j].y);
+                    const bool ok = rlo + l < rhi;
+                    if (ok) {
+                        // u of the last precondition step (each lane reads
+                        // only its own rows, rewritten by itself below)
+                        const V2 uA = lu[rlo + l - wlo];
+                        if (first) {
+                            p[j] = uA;
+                            s[j] = w[j];
+                        } else {
+                            p[j].x = fma(bx, p[j].x, uA.x); p[j].y = fma(by, p[j].y, uA.y);
+                            s[j].x = fma(bx, s[j].x, w[j].x); s[j].y = fma(by, s[j].y, w[j].y);
+                        }
+                        V2 eA = le[l];
+                        eA.x = fma(ax, p[j].x, eA.x);
+                        eA.y = fma(ay, p[j].y, eA.y);
+                        le[l] = eA;
+                        r[j].x = fma(-ax, s[j].x, r[j].x);
+                        r[j].y = fma(-ay, s[j].y, r[j].y);
+                        g1.x += acc_mul<PREC>(r[j].x, r[j].x);
+                        g1.y += acc_mul<PREC>(r[j].y, r[j].y);
                     }
-                    V2 eA = le[l];
-                    eA.x = fma(ax, p[j].x, eA.x);
-                    eA.y = fma(ay, p[j].y, eA.y);
-                    le[l] = eA;
-                    r[j].x = fma(-ax, s[j].x, r[j].x);
-                    r[j].y = fma(-ay, s[j].y, r[j].y);
-                    uA.x = dv[j] * r[j].x;
-                    uA.y = dv[j] * r[j].y;
-                    lu[l] = uA;
-                    if (multi) gu[row] = uA;
-                    g1.x += acc_mul<PREC>(r[j].x, r[j].x);
-                    g1.y += acc_mul<PREC>(r[j].y, r[j].y);
-                    g1.z += acc_mul<PREC>(r[j].x, uA.x);
-                    g1.w += acc_mul<PREC>(r[j].y, uA.y);
+                    if (coarse) restrict_slot(j, ok);
                 }
+                const double2 ru2 = precondition();
+                g1.z += ru2.x;
+                g1.w += ru2.y;
                 tm.sync();
+                fill_halo();
                 const double2 wu2 = spmv();
                 double4 T1, T2;
                 tm.sum8(g1, make_double4(wu2.x, wu2.y, 0.0, 0.0), T1, T2);
@@ -265,20 +424,27 @@ struct ResCG {
 
 // ARAP global-step CG of one island: the resident form when the member's
 // rows fit in the team's registers and LDS, the streaming form otherwise.
+// The coarse level is used when the plan has one for the island, `coarse` is
+// set and its rows fit in LDS beside the vectors.
 template <int PREC, class Team>
-UVA_HD __forceinline__ CGOut cg_arap(Team& tm, const DevPlan& P, const Isl& d, double2 b2, double tol, int maxit) {
+UVA_HD __forceinline__ CGOut cg_arap(Team& tm, const DevPlan& P, const Isl& d, double2 b2, double tol, int maxit,
+                                     bool coarse) {
     using V = typename Prec<PREC>::V;
     using V2 = typename Vec2<V>::type;
     // rows per lane the registers hold without spilling at 512 threads
-    constexpr int kMaxRpt = sizeof(V) == 4 ? 8 : 4;
-    const int nsl = tm.sl_hi(d.ns) - tm.sl_lo(d.ns);
+    constexpr int kMaxRpt = sizeof(V) == 4 ? 6 : 4;
+    const int slo = tm.sl_lo(d.ns), shi = tm.sl_hi(d.ns);
+    const int nsl = shi - slo;
     const int W = tm.nwaves();
     const int rpt = (nsl + W - 1) / W;
-    const int cap = tm.lvec_bytes() / (2 * (int)sizeof(V2));
-    if (tm.lvec() != nullptr && nsl * kSlice <= cap && rpt <= kMaxRpt) {
-        if (rpt <= 2) return ResCG<PREC, 2, Team>(tm, P, d).solve(b2, tol, maxit);
-        if (kMaxRpt == 4 || rpt <= 4) return ResCG<PREC, 4, Team>(tm, P, d).solve(b2, tol, maxit);
-        return ResCG<PREC, kMaxRpt, Team>(tm, P, d).solve(b2, tol, maxit);
+    const int vec = res_vec_bytes<V2, V>(nsl * kSlice, tm.multi() ? kHalo : 0);
+    if (tm.lvec() != nullptr && vec <= tm.lvec_bytes() && rpt <= kMaxRpt) {
+        const int nc = (d.ns + d.cg - 1) / d.cg;
+        const int na = nsl > 0 ? (shi - 1) / d.cg - slo / d.cg + 1 : 0;
+        const bool co = coarse && d.cofs >= 0 && vec + res_coarse_bytes(nc, na) <= tm.lvec_bytes();
+        if (rpt <= 2) return ResCG<PREC, 2, Team>(tm, P, d, co).solve(b2, tol, maxit);
+        if (rpt <= 4) return ResCG<PREC, 4, Team>(tm, P, d, co).solve(b2, tol, maxit);
+        return ResCG<PREC, kMaxRpt, Team>(tm, P, d, co).solve(b2, tol, maxit);
     }
     return cg_island<PREC, OP_ARAP>(tm, P, d, b2, tol, maxit, -1, -1);
 }

@@ -95,6 +95,10 @@ struct WaveTeam {
     __device__ int sl_hi(int ns) const { return ns; }
     __device__ int tri_lo(int) const { return 0; }
     __device__ int tri_hi(int nt) const { return nt; }
+    // LDS writes of the workgroup (here: wavefront) become visible to it.
+    __device__ void bsync() const { sync(); }
+    // sum over the 64 lanes of the calling wavefront (the same in every lane)
+    __device__ double2 wsum2(double2 v) const { return make_double2(wave_sum(v.x), wave_sum(v.y)); }
     // Global-memory writes of this wavefront become visible to its other lanes.
     __device__ void sync() const {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
@@ -141,6 +145,8 @@ struct BlockTeam {
     __device__ int tri_lo(int) const { return 0; }
     __device__ int tri_hi(int nt) const { return nt; }
     __device__ void sync() const { __syncthreads(); }
+    __device__ void bsync() const { __syncthreads(); }
+    __device__ double2 wsum2(double2 v) const { return make_double2(wave_sum(v.x), wave_sum(v.y)); }
     // The two slot buffers alternate; a buffer is rewritten only after the
     // barrier of the following reduction, when every thread has read it.
     __device__ double4 sum4(double4 v) {
@@ -189,7 +195,9 @@ __device__ __forceinline__ int xcc_id() { return __builtin_amdgcn_s_getreg((15 <
 // island's slices and triangles (halo traffic only at block edges).  sync()
 // is a counter barrier; a reduction combines each member's partial sums,
 // written to global memory, in member order after one barrier, so the result
-// is identical in every member.  All members are resident (the grid never
+// is identical in every member.  After the barrier the m x 8 partials are
+// fetched with one load per thread (all in flight together) into LDS, and
+// eight threads sum them member by member.  All members are resident (the grid never
 // exceeds the resident capacity) and take their member tasks from the queue
 // in order, so every member eventually arrives.
 //
@@ -204,7 +212,10 @@ template <int BS>
 struct TeamWG {
     static constexpr int kWaves = BS / 64;
     static constexpr int kSlots = 8;
-    double* lds;        // kWaves * kSlots doubles
+    // LDS doubles a team reduction needs: wavefront slots, the members'
+    // partials and the eight results
+    static constexpr int kLdsDoubles = kWaves * kSlots + kMaxTeam * kSlots + kSlots;
+    double* lds;        // kLdsDoubles doubles
     int* cnt;           // arrival counter of this team (zero at launch)
     double* part;       // 2 * kMaxTeam * kSlots partial sums
     int m, mem;
@@ -232,6 +243,9 @@ struct TeamWG {
     __device__ int sl_hi(int ns) const { return (int)((long long)ns * (mem + 1) / m); }
     __device__ int tri_lo(int nt) const { return (int)((long long)nt * mem / m); }
     __device__ int tri_hi(int nt) const { return (int)((long long)nt * (mem + 1) / m); }
+    // LDS writes of this member become visible to the member's threads.
+    __device__ void bsync() const { __syncthreads(); }
+    __device__ double2 wsum2(double2 v) const { return make_double2(wave_sum(v.x), wave_sum(v.y)); }
     __device__ void sync() {
         if (local) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         __syncthreads();
@@ -275,14 +289,18 @@ struct TeamWG {
             pb[kSlots * mem + threadIdx.x] = v;
         }
         sync();
-        double4 s = make_double4(0.0, 0.0, 0.0, 0.0), t = make_double4(0.0, 0.0, 0.0, 0.0);
-        for (int k = 0; k < m; ++k) {
-            const double* q = pb + kSlots * k;
-            s.x += q[0]; s.y += q[1]; s.z += q[2]; s.w += q[3];
-            t.x += q[4]; t.y += q[5]; t.z += q[6]; t.w += q[7];
+        double* mp = lds + kWaves * kSlots;          // members' partials
+        double* res = mp + kMaxTeam * kSlots;        // the eight sums
+        if (threadIdx.x < m * kSlots) mp[threadIdx.x] = pb[threadIdx.x];
+        __syncthreads();
+        if (threadIdx.x < kSlots) {
+            double v = 0.0;
+            for (int k = 0; k < m; ++k) v += mp[kSlots * k + threadIdx.x];
+            res[threadIdx.x] = v;
         }
-        sa = uniform_d4(s);
-        sb = uniform_d4(t);
+        __syncthreads();
+        sa = uniform_d4(make_double4(res[0], res[1], res[2], res[3]));
+        sb = uniform_d4(make_double4(res[4], res[5], res[6], res[7]));
     }
     __device__ double4 sum4(double4 v) {
         double4 s, unused;
@@ -301,15 +319,25 @@ struct TeamWG {
             pb[kSlots * mem] = mn;
         }
         sync();
-        double mn = pb[0];
-        for (int k = 1; k < m; ++k) mn = fmin(mn, pb[kSlots * k]);
-        return uniform_d(mn);
+        double* mp = lds + kWaves * kSlots;
+        double* res = mp + kMaxTeam * kSlots;
+        if (threadIdx.x < m) mp[threadIdx.x] = pb[kSlots * threadIdx.x];
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double mn = mp[0];
+            for (int k = 1; k < m; ++k) mn = fmin(mn, mp[k]);
+            res[0] = mn;
+        }
+        __syncthreads();
+        return uniform_d(res[0]);
     }
 };
 
 // ---------------------------------------------------------------- island view
 struct Isl {
-    int v0, nv, t0, nt, s0, ns, pin;
+    int v0, nv, t0, nt, s0, ns, pin, bw;
+    int cg;               // slices per coarse aggregate
+    long long cofs;       // offset of the inverse coarse operator, -1 without one
     double wpin;
 };
 
@@ -319,6 +347,10 @@ UVA_HD __forceinline__ Isl load_isl(const DevPlan& P, int i) {
     d.t0 = uniform_i(P.isl_t0[i]); d.nt = uniform_i(P.isl_nt[i]);
     d.s0 = uniform_i(P.isl_s0[i]); d.ns = uniform_i(P.isl_ns[i]);
     d.pin = uniform_i(P.isl_pin[i]); d.wpin = uniform_d(P.isl_wpin[i]);
+    d.bw = uniform_i(P.isl_bw[i]);
+    d.cg = uniform_i(P.isl_cg[i]);
+    const long long co = P.isl_cofs[i];
+    d.cofs = ((long long)uniform_i((int)(co >> 32)) << 32) | (long long)(unsigned)uniform_i((int)(co & 0xffffffffLL));
     return d;
 }
 
@@ -634,7 +666,7 @@ UVA_HD __forceinline__ void run_task(Team& tm, const DevPlan& P, int isl, const 
         if (uniform_i(st->done)) return;
         const long long t_start = uva_clock();
         const double2 b2 = rhs_step(tm, P, d, U);
-        const CGOut cg = cg_arap<PREC>(tm, P, d, b2, a.cg_tol, a.cg_max);
+        const CGOut cg = cg_arap<PREC>(tm, P, d, b2, a.cg_tol, a.cg_max, true);
         if (a.batch) {
             // islands whose solve went non-finite keep their layout (gpu.py:235-240)
             double bad = 0.0;
@@ -747,7 +779,7 @@ UVA_HD __forceinline__ void run_task(Team& tm, const DevPlan& P, int isl, const 
             acc.y += bb.y * bb.y;
         }
         const double4 s = tm.sum4(acc);
-        const CGOut cg = cg_arap<PREC>(tm, P, d, make_double2(s.x, s.y), a.cg_tol, a.cg_max);
+        const CGOut cg = cg_arap<PREC>(tm, P, d, make_double2(s.x, s.y), a.cg_tol, a.cg_max, true);
         FOR_ROWS(tm, d, row, sg) {
             (void)sg;
             U[row] = P.X64[row];
@@ -916,7 +948,9 @@ UVA_HD __forceinline__ void run_task(Team& tm, const DevPlan& P, int isl, const 
 #if !defined(UVA_EMULATOR)
 template <int BS, int KIND, int PREC>
 __global__ __launch_bounds__(BS) void island_kernel(DevPlan P, TaskArgs a, int slot) {
-    __shared__ double red[2 * (BS / 64) * BlockTeam<BS>::kSlots];
+    constexpr int kRed = 2 * (BS / 64) * BlockTeam<BS>::kSlots > TeamWG<BS>::kLdsDoubles
+                             ? 2 * (BS / 64) * BlockTeam<BS>::kSlots : TeamWG<BS>::kLdsDoubles;
+    __shared__ double red[kRed];
     __shared__ int s_task;
     // vectors of the resident CG (kinds that solve the ARAP system only)
     constexpr int kLv = lds_vec_bytes(KIND);
@@ -968,6 +1002,14 @@ __global__ void xcc_probe_kernel(int* out) {
 __global__ void fill_int_kernel(int* p, int n, int v) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
+}
+
+__global__ void permute_kernel(const int* __restrict__ vperm, const double2* __restrict__ src,
+                               double2* __restrict__ dst, int n, int to_plan) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    if (to_plan) dst[v] = src[vperm[v]];
+    else dst[vperm[v]] = src[v];
 }
 
 __global__ void signed_areas_kernel(DevPlan P, const double2* __restrict__ U, double* __restrict__ sa) {
@@ -1035,6 +1077,13 @@ hipError_t launch_xcc_probe(int* out, int blocks, hipStream_t s) {
 hipError_t launch_fill_int(int* p, int n, int v, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(fill_int_kernel, dim3((n + 255) / 256), dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_permute(const DevPlan& P, const double2* src, double2* dst, bool to_plan, hipStream_t s) {
+    if (P.n_vert <= 0) return hipSuccess;
+    hipLaunchKernelGGL(permute_kernel, dim3((P.n_vert + 255) / 256), dim3(256), 0, s, P.vperm, src, dst, P.n_vert,
+                       to_plan ? 1 : 0);
     return hipGetLastError();
 }
 

@@ -10,7 +10,11 @@ namespace uva {
 constexpr int kBlock = 1024;    // workgroup size of the island kernel (16 wavefronts)
 constexpr int kBlockF64 = 512;  // FP64 instantiations (more registers per lane)
 constexpr int kBlockRes = 512;  // kinds with the resident CG (up to 8 rows per lane in registers)
-constexpr int kLdsVec = 65536;  // LDS bytes of the resident CG vectors u and e per workgroup
+// LDS bytes of the resident CG per workgroup: u with halo, e, D^-1, D (80 KB
+// for 3072 FP32 rows) and the coarse level (inverse rows and two vectors of
+// up to kCoarseN aggregates, 68 KB)
+constexpr int kLdsVec = 151552;
+constexpr int kHalo = 512;      // rows of u a team member keeps beyond each end of its range
 
 enum TaskKind {
     TASK_PROLOGUE = 0,   // flips of the start, line-search decision, rotations + energy
@@ -60,5 +64,7 @@ hipError_t launch_fill_int(int* p, int n, int v, hipStream_t s);
 // out[b] = hardware XCD id of block b (blocks of 64 threads)
 hipError_t launch_xcc_probe(int* out, int blocks, hipStream_t s);
 hipError_t launch_signed_areas(const DevPlan& P, const double2* U, double* sa, hipStream_t s);
+// caller-order vertex array -> plan order (gather), or back (scatter)
+hipError_t launch_permute(const DevPlan& P, const double2* src, double2* dst, bool to_plan, hipStream_t s);
 
 }  // namespace uva

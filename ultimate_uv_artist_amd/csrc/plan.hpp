@@ -18,9 +18,10 @@
 namespace uva {
 
 constexpr int kSlice = 64;          // rows per slice (= wavefront width)
-constexpr int kWaveMaxRows = 512;   // islands up to this many vertices run on one wavefront
-constexpr int kResSlices = 64;      // slices one workgroup keeps resident in the FP32 ARAP CG (8 waves x 8)
+constexpr int kWaveMaxRows = 384;   // islands up to this many vertices run on one wavefront
+constexpr int kResSlices = 48;      // slices one workgroup keeps resident in the FP32 ARAP CG (8 waves x 6)
 constexpr int kMaxTeam = 16;        // most workgroups on one island
+constexpr int kCoarseN = 128;       // most aggregates of an island's coarse level (64 KB inverse)
 constexpr int kTeamMinRows = 2048;  // fewest rows per team member
 constexpr int kMaxXcd = 8;          // XCDs (own L2 each) a device may have
 // Task counters of one launch: one per XCD list of workgroup tasks, then one
@@ -55,6 +56,7 @@ struct DevPlan {
     const int* isl_s0;           // first slice
     const int* isl_ns;           // number of slices
     const int* isl_pin;          // global id of the soft pin
+    const int* isl_bw;           // matrix bandwidth: largest |column - row| of the island
     const double* isl_wpin;
     const int* isl_lpin0;        // LSCM pins (global ids), -1 when the island has no boundary
     const int* isl_lpin1;
@@ -104,6 +106,16 @@ struct DevPlan {
     double4* sdi;
     double* red;                 // per-island scalars for multi-launch paths
     IslandState* st;
+    // vertex order: plan vertex v holds the caller's vertex vperm[v]
+    const int* vperm;
+    // coarse level of the ARAP CG (aggregates of isl_cg[i] consecutive 64-row
+    // slices): isl_cofs[i] is the offset of island i's inverse coarse operator
+    // (nc x nc FP32, row-major) in cinv, or -1; crc holds the restricted
+    // residual (FP64, two columns) of every slice during a solve
+    const int64_t* isl_cofs;
+    const int* isl_cg;           // slices per aggregate
+    const float* cinv;
+    double2* crc;
     int* queue;                  // task counters
     const int* order_group;      // group tasks: island ids, longest first
     const int* grp_mem;          // member index of the task in its team

@@ -46,8 +46,11 @@ inline void cross3(const double* a, const double* b, double* o) {
     o[2] = a[0] * b[1] - a[1] * b[0];
 }
 
+// perm (or NULL): the caller's local id of each vertex of P/T, which fixes the
+// order in which the farthest-boundary-pair search visits vertices (the
+// reference's ascending vertex order).
 void build_island(Island& I, int nv, int nt, const double* P, const int64_t* T, const double* X,
-                  const double* area, const double* cots, bool batch, const int64_t* lpins) {
+                  const double* area, const double* cots, bool batch, const int64_t* lpins, const int* perm) {
     I.nv = nv;
     I.nt = nt;
     I.x1.resize(nt); I.x2.resize(nt); I.y2.resize(nt); I.area.resize(nt); I.ok.resize(nt);
@@ -210,9 +213,11 @@ void build_island(Island& I, int nv, int nt, const double* P, const int64_t* T, 
             I.inc_corner[ip[v]] = c;
             ip[v]++;
         }
-    // LSCM pins: farthest boundary pair (param.py:172-179) unless given
+    // LSCM pins: farthest boundary pair (param.py:172-179) unless given;
+    // candidates in the caller's ascending vertex order
     std::sort(bsrc.begin(), bsrc.end());
     bsrc.erase(std::unique(bsrc.begin(), bsrc.end()), bsrc.end());
+    if (perm) std::sort(bsrc.begin(), bsrc.end(), [&](int a, int b) { return perm[a] < perm[b]; });
     if (lpins) {
         I.lp0 = (int)lpins[0];
         I.lp1 = (int)lpins[1];
@@ -243,6 +248,94 @@ void build_island(Island& I, int nv, int nt, const double* P, const int64_t* T, 
     }
 }
 
+// Vertex order of an island inside the plan: ascending 3-D Morton code of
+// the positions (21 bits per axis over the island's bounding box), ties in
+// the caller's order.  Consecutive 64-vertex slices are then compact patches
+// of the surface, which makes them the aggregates of the coarse level.
+std::vector<int> morton_order(int nv, const double* P) {
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int v = 0; v < nv; ++v)
+        for (int k = 0; k < 3; ++k) {
+            const double x = P[3 * (size_t)v + k];
+            if (std::isfinite(x)) { lo[k] = std::min(lo[k], x); hi[k] = std::max(hi[k], x); }
+        }
+    double span = 0.0;
+    for (int k = 0; k < 3; ++k) span = std::max(span, hi[k] - lo[k]);
+    const double sc = span > 0.0 ? 2097151.0 / span : 0.0;
+    auto spread = [](uint64_t v) {
+        v &= 0x1fffff;
+        v = (v | v << 32) & 0x1f00000000ffffULL;
+        v = (v | v << 16) & 0x1f0000ff0000ffULL;
+        v = (v | v << 8) & 0x100f00f00f00f00fULL;
+        v = (v | v << 4) & 0x10c30c30c30c30c3ULL;
+        v = (v | v << 2) & 0x1249249249249249ULL;
+        return v;
+    };
+    std::vector<std::pair<uint64_t, int>> key(nv);
+    for (int v = 0; v < nv; ++v) {
+        uint64_t code = 0;
+        for (int k = 0; k < 3; ++k) {
+            double q = (P[3 * (size_t)v + k] - lo[k]) * sc;
+            q = std::isfinite(q) ? std::min(std::max(q, 0.0), 2097151.0) : 0.0;
+            code |= spread((uint64_t)q) << k;
+        }
+        key[v] = {code, v};
+    }
+    std::sort(key.begin(), key.end());
+    std::vector<int> perm(nv);
+    for (int v = 0; v < nv; ++v) perm[v] = key[v].second;
+    return perm;
+}
+
+// Inverse of the coarse operator Ac = Z^T A Z of one island (A = Laplacian
+// + pin weight; Z = indicator of the aggregates, g consecutive 64-row slices
+// each), symmetric, in FP32.  Returns false if Ac is not numerically positive
+// definite.
+bool coarse_inverse(int nv, int g, const std::vector<int>& row_ptr, const std::vector<int>& row_col,
+                    const std::vector<double>& row_val, const std::vector<double>& diag, int pin, double wpin,
+                    float* out) {
+    const int rows = kSlice * g;
+    const int ns = (nv + rows - 1) / rows;
+    std::vector<double> A((size_t)ns * ns, 0.0);
+    for (int i = 0; i < nv; ++i) {
+        const int a = i / rows;
+        A[(size_t)a * ns + a] += diag[i] + (i == pin ? wpin : 0.0);
+        for (int q = row_ptr[i]; q < row_ptr[i + 1]; ++q) A[(size_t)a * ns + row_col[q] / rows] += row_val[q];
+    }
+    // Cholesky A = L L^T (lower, in place)
+    for (int j = 0; j < ns; ++j) {
+        double dj = A[(size_t)j * ns + j];
+        for (int k = 0; k < j; ++k) dj -= A[(size_t)j * ns + k] * A[(size_t)j * ns + k];
+        if (!(dj > 0.0) || !std::isfinite(dj)) return false;
+        const double ljj = std::sqrt(dj);
+        A[(size_t)j * ns + j] = ljj;
+        for (int i = j + 1; i < ns; ++i) {
+            double v = A[(size_t)i * ns + j];
+            for (int k = 0; k < j; ++k) v -= A[(size_t)i * ns + k] * A[(size_t)j * ns + k];
+            A[(size_t)i * ns + j] = v / ljj;
+        }
+    }
+    // M = L^-1 (lower), then Ac^-1 = M^T M
+    std::vector<double> M((size_t)ns * ns, 0.0);
+    for (int c = 0; c < ns; ++c) {
+        M[(size_t)c * ns + c] = 1.0 / A[(size_t)c * ns + c];
+        for (int i = c + 1; i < ns; ++i) {
+            double v = 0.0;
+            for (int k = c; k < i; ++k) v -= A[(size_t)i * ns + k] * M[(size_t)k * ns + c];
+            M[(size_t)i * ns + c] = v / A[(size_t)i * ns + i];
+        }
+    }
+    for (int a = 0; a < ns; ++a)
+        for (int b = 0; b <= a; ++b) {
+            double v = 0.0;
+            for (int k = a; k < ns; ++k) v += M[(size_t)k * ns + a] * M[(size_t)k * ns + b];
+            if (!std::isfinite(v)) return false;
+            out[(size_t)a * ns + b] = (float)v;
+            out[(size_t)b * ns + a] = (float)v;
+        }
+    return true;
+}
+
 template <class F>
 void parallel_for(int n, F fn) {
     int nth = (int)std::min<unsigned>(std::max(1u, std::thread::hardware_concurrency()), 16u);
@@ -262,7 +355,7 @@ void parallel_for(int n, F fn) {
 std::string build_host_plan(HostPlan& hp, int64_t n_islands, const int64_t* vert_off, const int64_t* tri_off,
                             const double* P, const int64_t* T, const int64_t* pins, const double* X,
                             const double* area, const double* cots, bool batch, const int64_t* lscm_pins,
-                            int wave_max_rows, int team_slots, int n_xcd) {
+                            int wave_max_rows, int team_slots, int n_xcd, bool reorder, bool coarse) {
     if (n_islands <= 0) return "n_islands must be positive";
     if (vert_off[0] != 0 || tri_off[0] != 0) return "offsets must start at 0";
     for (int64_t i = 0; i < n_islands; ++i) {
@@ -285,11 +378,24 @@ std::string build_host_plan(HostPlan& hp, int64_t n_islands, const int64_t* vert
     }
     int n = (int)n_islands;
     std::vector<Island> isl(n);
+    // per island: perm[v] = caller's local id of plan vertex v, inv its inverse
+    std::vector<std::vector<int>> perm(n), inv(n);
     parallel_for(n, [&](int i) {
         int64_t v0 = vert_off[i], t0 = tri_off[i];
-        build_island(isl[i], (int)(vert_off[i + 1] - v0), (int)(tri_off[i + 1] - t0), P + 3 * v0, T + 3 * t0,
-                     X ? X + 6 * t0 : nullptr, area ? area + t0 : nullptr, cots ? cots + 3 * t0 : nullptr, batch,
-                     lscm_pins ? lscm_pins + 2 * i : nullptr);
+        const int nv = (int)(vert_off[i + 1] - v0), nt = (int)(tri_off[i + 1] - t0);
+        if (reorder) perm[i] = morton_order(nv, P + 3 * v0);
+        else { perm[i].resize(nv); for (int v = 0; v < nv; ++v) perm[i][v] = v; }
+        inv[i].resize(nv);
+        for (int v = 0; v < nv; ++v) inv[i][perm[i][v]] = v;
+        std::vector<double> Pp(3 * (size_t)nv);
+        for (int v = 0; v < nv; ++v)
+            for (int k = 0; k < 3; ++k) Pp[3 * (size_t)v + k] = P[3 * (v0 + perm[i][v]) + k];
+        std::vector<int64_t> Tp(3 * (size_t)nt);
+        for (size_t q = 0; q < 3 * (size_t)nt; ++q) Tp[q] = inv[i][T[3 * t0 + (int64_t)q]];
+        int64_t lp[2];
+        if (lscm_pins) { lp[0] = inv[i][lscm_pins[2 * i]]; lp[1] = inv[i][lscm_pins[2 * i + 1]]; }
+        build_island(isl[i], nv, nt, Pp.data(), Tp.data(), X ? X + 6 * t0 : nullptr, area ? area + t0 : nullptr,
+                     cots ? cots + 3 * t0 : nullptr, batch, lscm_pins ? lp : nullptr, perm[i].data());
     });
     hp.n_isl = n;
     hp.n_vert = (int)nv_all;
@@ -301,7 +407,7 @@ std::string build_host_plan(HostPlan& hp, int64_t n_islands, const int64_t* vert
     // island table and slice layout
     hp.isl_v0.resize(n); hp.isl_nv.resize(n); hp.isl_t0.resize(n); hp.isl_nt.resize(n);
     hp.isl_s0.resize(n); hp.isl_ns.resize(n); hp.isl_pin.resize(n); hp.isl_wpin.resize(n);
-    hp.isl_lpin0.resize(n); hp.isl_lpin1.resize(n); hp.isl_ldist.resize(n);
+    hp.isl_lpin0.resize(n); hp.isl_lpin1.resize(n); hp.isl_ldist.resize(n); hp.isl_bw.assign(n, 0);
     int64_t nslices = 0;
     std::vector<int64_t> slice_start(n);
     std::vector<std::vector<int>> widths(n);
@@ -324,7 +430,21 @@ std::string build_host_plan(HostPlan& hp, int64_t n_islands, const int64_t* vert
     if (nnz_pad >= (int64_t(1) << 31)) return "batch too large for 32-bit matrix offsets";
     hp.sl_ptr.assign(nslices + 1, 0);
     hp.m_col.resize(nnz_pad); hp.m_hi.resize(nnz_pad); hp.m_lo.resize(nnz_pad); hp.m_b.resize(nnz_pad);
-    hp.diag.resize(nv_all); hp.dinv32.resize(nv_all); hp.vbnd.resize(nv_all);
+    hp.diag.resize(nv_all); hp.dinv32.resize(nv_all); hp.vbnd.resize(nv_all); hp.vperm.resize(nv_all);
+    // coarse level: aggregates of g consecutive slices, at most kCoarseN of them
+    hp.isl_cofs.assign(n, -1);
+    hp.isl_cg.assign(n, 1);
+    {
+        int64_t c = 0;
+        for (int i = 0; i < n; ++i) {
+            const int ns = (int)widths[i].size();
+            const int g = (ns + kCoarseN - 1) / kCoarseN;
+            const int64_t nc = (ns + g - 1) / g;
+            hp.isl_cg[i] = g;
+            if (coarse && nc >= 2) { hp.isl_cofs[i] = c; c += nc * nc; }
+        }
+        hp.cinv.assign(std::max<int64_t>(c, 1), 0.0f);
+    }
     hp.tv0.resize(nt_all); hp.tv1.resize(nt_all); hp.tv2.resize(nt_all);
     hp.fx1.resize(nt_all); hp.fx2.resize(nt_all); hp.fy2.resize(nt_all); hp.farea.resize(nt_all); hp.fok.resize(nt_all);
     hp.inc_ptr.assign(nv_all + 1, 0);
@@ -358,17 +478,18 @@ std::string build_host_plan(HostPlan& hp, int64_t n_islands, const int64_t* vert
         int v0 = (int)vert_off[i], t0 = (int)tri_off[i];
         hp.isl_v0[i] = v0; hp.isl_nv[i] = I.nv; hp.isl_t0[i] = t0; hp.isl_nt[i] = I.nt;
         hp.isl_s0[i] = (int)slice_start[i]; hp.isl_ns[i] = (int)widths[i].size();
-        int lp = pins ? (int)pins[i] : 0;
+        int lp = inv[i][pins ? (int)pins[i] : 0];
         hp.isl_pin[i] = v0 + lp;
+        for (int v = 0; v < I.nv; ++v) hp.vperm[v0 + v] = v0 + perm[i][v];
         double wp = batch ? wglob : (I.nt ? I.wmax : 1.0);
         hp.isl_wpin[i] = wp;
         hp.isl_lpin0[i] = I.lp0 >= 0 ? v0 + I.lp0 : -1;
         hp.isl_lpin1[i] = I.lp1 >= 0 ? v0 + I.lp1 : -1;
         hp.isl_ldist[i] = I.ldist;
         for (int f = 0; f < I.nt; ++f) {
-            hp.tv0[t0 + f] = v0 + (int)T[3 * ((size_t)t0 + f)];
-            hp.tv1[t0 + f] = v0 + (int)T[3 * ((size_t)t0 + f) + 1];
-            hp.tv2[t0 + f] = v0 + (int)T[3 * ((size_t)t0 + f) + 2];
+            hp.tv0[t0 + f] = v0 + inv[i][T[3 * ((size_t)t0 + f)]];
+            hp.tv1[t0 + f] = v0 + inv[i][T[3 * ((size_t)t0 + f) + 1]];
+            hp.tv2[t0 + f] = v0 + inv[i][T[3 * ((size_t)t0 + f) + 2]];
             hp.fx1[t0 + f] = I.x1[f]; hp.fx2[t0 + f] = I.x2[f]; hp.fy2[t0 + f] = I.y2[f];
             hp.farea[t0 + f] = I.area[f]; hp.fok[t0 + f] = I.ok[f];
             for (int j = 0; j < 6; ++j) hp.tri_g[6 * ((size_t)t0 + f) + j] = I.g[6 * (size_t)f + j];
@@ -387,9 +508,20 @@ std::string build_host_plan(HostPlan& hp, int64_t n_islands, const int64_t* vert
             double dp = dg + (v == lp ? wp : 0.0);
             hp.dinv32[v0 + v] = (float)(dp > 1e-30 ? 1.0 / dp : 1.0);
             hp.vbnd[v0 + v] = I.bnd[v];
-            int a = hp.adj_ptr[v0 + v];
-            for (int q = I.row_ptr[v]; q < I.row_ptr[v + 1]; ++q) hp.adj_idx[a++] = v0 + I.row_col[q];
+            // neighbours in the caller's ascending order (the Jacobi repair sums them)
+            const int a0 = hp.adj_ptr[v0 + v];
+            int a = a0;
+            for (int q = I.row_ptr[v]; q < I.row_ptr[v + 1]; ++q) {
+                hp.adj_idx[a++] = v0 + I.row_col[q];
+                hp.isl_bw[i] = std::max(hp.isl_bw[i], std::abs(I.row_col[q] - v));
+            }
+            std::sort(hp.adj_idx.begin() + a0, hp.adj_idx.begin() + a, [&](int x, int y) {
+                return perm[i][x - v0] < perm[i][y - v0];
+            });
         }
+        if (hp.isl_cofs[i] >= 0 && !coarse_inverse(I.nv, hp.isl_cg[i], I.row_ptr, I.row_col, I.row_val, I.diag, lp,
+                                                   wp, hp.cinv.data() + hp.isl_cofs[i]))
+            hp.isl_cofs[i] = -1;
         // sliced ELL
         for (int s = 0; s < (int)widths[i].size(); ++s) {
             int base = hp.sl_ptr[slice_start[i] + s];
@@ -501,6 +633,7 @@ std::string build_matrix_plan(HostPlan& hp, int64_t n, const int64_t* indptr, co
     int ns = (nv + kSlice - 1) / kSlice;
     hp.isl_ns = {ns};
     hp.isl_pin = {-1}; hp.isl_wpin = {0.0}; hp.isl_lpin0 = {-1}; hp.isl_lpin1 = {-1}; hp.isl_ldist = {0.0};
+    hp.isl_bw = {0};
     // rows: off-diagonal entries with duplicates summed, columns ascending
     std::vector<std::vector<std::pair<int, double>>> rows(nv);
     hp.diag.assign(nv, 0.0);
@@ -521,6 +654,7 @@ std::string build_matrix_plan(HostPlan& hp, int64_t n, const int64_t* indptr, co
         }
         r.resize(w);
         hp.nnz_true += (int64_t)w;
+        for (const auto& e : r) hp.isl_bw[0] = std::max(hp.isl_bw[0], std::abs(e.first - i));
     }
     hp.sl_ptr.assign(ns + 1, 0);
     int64_t off = 0;
@@ -565,6 +699,21 @@ std::string build_matrix_plan(HostPlan& hp, int64_t n, const int64_t* indptr, co
     for (int i = 0; i < nv; ++i)
         for (auto& e : rows[i]) hp.adj_idx.push_back(e.first);
     hp.inc_ptr.assign(nv + 1, 0);
+    hp.vperm.resize(nv);
+    for (int i = 0; i < nv; ++i) hp.vperm[i] = i;
+    // coarse level over the 64-row slices of the given order
+    hp.isl_cofs.assign(1, -1);
+    const int g = (ns + kCoarseN - 1) / kCoarseN;
+    const int nc = (ns + g - 1) / g;
+    hp.isl_cg.assign(1, g);
+    hp.cinv.assign(1, 0.0f);
+    if (nc >= 2) {
+        std::vector<double> val(hp.adj_idx.size());
+        for (int i = 0, q = 0; i < nv; ++i)
+            for (auto& e : rows[i]) val[q++] = e.second;
+        hp.cinv.assign((size_t)nc * nc, 0.0f);
+        if (coarse_inverse(nv, g, hp.adj_ptr, hp.adj_idx, val, hp.diag, -1, 0.0, hp.cinv.data())) hp.isl_cofs[0] = 0;
+    }
     if (nv > kWaveMaxRows) plan_teams(hp, {0}, 0);
     else hp.order_wave.push_back(0);
     return "";

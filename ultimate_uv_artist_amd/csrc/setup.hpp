@@ -14,6 +14,7 @@ struct HostPlan {
     int n_isl = 0, n_vert = 0, n_tri = 0;
     bool batch = false;
     std::vector<int> isl_v0, isl_nv, isl_t0, isl_nt, isl_s0, isl_ns, isl_pin, isl_lpin0, isl_lpin1;
+    std::vector<int> isl_bw;     // matrix bandwidth: largest |column - row| of the island
     std::vector<double> isl_wpin, isl_ldist;
     std::vector<int> tv0, tv1, tv2;
     std::vector<double> fx1, fx2, fy2, farea;
@@ -38,13 +39,23 @@ struct HostPlan {
     int n_xcd = 1;
     std::vector<int> grp_xoff{0, 0};
     int64_t nnz_true = 0;
+    // plan vertex -> caller's global vertex id (islands keep their offsets;
+    // inside an island the plan orders vertices by Morton code)
+    std::vector<int> vperm;
+    // coarse level: isl_cofs[i] is the offset of island i's inverse coarse
+    // operator (nc x nc, FP32, nc = ceil(ns / isl_cg[i]) aggregates of
+    // isl_cg[i] slices) in cinv, or -1 without a coarse level
+    std::vector<int64_t> isl_cofs;
+    std::vector<int> isl_cg;      // slices per aggregate
+    std::vector<float> cinv;
 };
 
 // Returns an empty string on success, else an error message.
 std::string build_host_plan(HostPlan& hp, int64_t n_islands, const int64_t* vert_off, const int64_t* tri_off,
                             const double* P, const int64_t* T, const int64_t* pins, const double* X,
                             const double* area, const double* cots, bool batch, const int64_t* lscm_pins,
-                            int wave_max_rows = kWaveMaxRows, int team_slots = 0, int n_xcd = 1);
+                            int wave_max_rows = kWaveMaxRows, int team_slots = 0, int n_xcd = 1,
+                            bool reorder = true, bool coarse = true);
 
 // Group-task order and team sizes: islands whose cost exceeds their share of
 // `team_slots` concurrent workgroups get several workgroups (at most kMaxTeam,

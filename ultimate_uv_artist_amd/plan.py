@@ -57,7 +57,8 @@ class IslandPlan:
     batch   gpu.Batch conventions instead of param.refine conventions
     """
 
-    def __init__(self, charts, pins=None, frames=None, batch=False, dev=None, no_wave=False, no_teams=False):
+    def __init__(self, charts, pins=None, frames=None, batch=False, dev=None, no_wave=False, no_teams=False,
+                 keep_order=False, no_coarse=False):
         self.dev = require_gpu(dev)
         L = _lib.lib()
         self.vert_off, self.tri_off, P, T = concat_charts(charts)
@@ -75,7 +76,9 @@ class IslandPlan:
             rc = L.uva_plan_create(C.byref(self._h), self.n_islands, i64p(self.vert_off), i64p(self.tri_off),
                                    f64p(P), i64p(T), i64p(pins_a), f64p(X), f64p(area), f64p(cots),
                                    (_lib.UVA_PLAN_BATCH if batch else 0) | (_lib.UVA_PLAN_NO_WAVE if no_wave else 0)
-                                   | (_lib.UVA_PLAN_NO_TEAMS if no_teams else 0),
+                                   | (_lib.UVA_PLAN_NO_TEAMS if no_teams else 0)
+                                   | (_lib.UVA_PLAN_KEEP_ORDER if keep_order else 0)
+                                   | (_lib.UVA_PLAN_NO_COARSE if no_coarse else 0),
                                    stream_of(self.dev))
         check(rc, "uva_plan_create")
         self.T_local = T
