@@ -42,6 +42,8 @@ def parse():
     ap.add_argument("--precision", default="mixed", choices=["mixed", "f32", "f64"])
     ap.add_argument("--cpu-sample", type=int, default=16, help="islands in the CPU baseline sample (0 = skip)")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-tune", dest="tune", action="store_false",
+                    help="keep the size-based schedule (no re-planning from measured iterations)")
     ap.add_argument("--no-plain", dest="plain", action="store_false",
                     help="skip the plain per-island GPU baseline")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -145,6 +147,13 @@ def main():
         plan.arap(U, args.iters, line_search=True, mode=_lib.UVA_MODE_REFINE, tol=0.0, cg_tol=args.cg_tol,
                   precision=prec, use_graph=not args.no_graph)
 
+    if args.tune:
+        # autotune: one step measures every island's CG iterations, the plan
+        # re-plans team sizes and the workgroup schedule from them
+        step()
+        torch.cuda.synchronize()
+        plan.tune()
+        print("bench: schedule re-planned from the measured CG iterations per island", flush=True)
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()

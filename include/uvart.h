@@ -138,6 +138,12 @@ int uva_plan_destroy(uva_plan *plan);
 int uva_plan_get_info(const uva_plan *plan, uva_plan_info *info);
 /* Per-island statistics of the last solve call (host array of n_islands). */
 int uva_plan_get_stats(uva_plan *plan, uva_island_stats *stats, void *stream);
+/* Autotune the schedule: re-plan team sizes and the workgroup schedule from
+ * the CG iterations per global step each island took in the last solve call
+ * on the plan (the iteration counts of a repeated solve are reproducible).
+ * Later calls run the new schedule; results are unchanged up to the order of
+ * FP64 reductions across team members.  No counterpart in the reference. */
+int uva_plan_tune(uva_plan *plan, void *stream);
 /* Set the per-triangle target scale (gpu.py:123 Batch.scale); device array. */
 int uva_plan_set_scale(uva_plan *plan, const double *scale_dev, void *stream);
 

@@ -389,3 +389,35 @@ def test_refine_slim_matches_reference_trajectory():
     edge = cc - b
     Uf[T[0][0]] = 2.0 * (b + edge * ((a - b) @ edge) / (edge @ edge)) - a
     assert np.array_equal(param.refine(P, T, Uf, iters=3, method="slim")[0], Uf)
+
+
+def test_tuned_schedule_matches_default():
+    """uva_plan_tune re-plans team sizes and the workgroup schedule from the
+    measured CG iterations; the solve gives the same layout to CG accuracy,
+    repeated tuned runs are bitwise identical, and graph replays after the
+    re-plan (the cached graph is rebuilt) match eager runs."""
+    from oracle import param_cpu as oracle
+    from ultimate_uv_artist_amd import _lib, workloads
+    from ultimate_uv_artist_amd.plan import IslandPlan
+
+    torch = _torch()
+    rng = np.random.default_rng(57)
+    charts = [workloads.disk_island(rng, t) for t in (26000, 9000, 3000, 600, 300)]
+    U0 = np.concatenate([oracle.lscm(P, T) for P, T in charts])
+    plan = IslandPlan(charts)
+    ref = plan.device_uv(U0)
+    plan.arap(ref, 3, mode=_lib.UVA_MODE_REFINE, tol=0.0, use_graph=True)
+    ref = ref.cpu().numpy()
+    plan.tune()
+    outs = []
+    for use_graph in (False, True, True):
+        U = plan.device_uv(U0)
+        plan.arap(U, 3, mode=_lib.UVA_MODE_REFINE, tol=0.0, use_graph=use_graph)
+        torch.cuda.synchronize()
+        assert plan.stats()["converged"].all()
+        outs.append(U.cpu().numpy())
+    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[1], outs[2])
+    vo = plan.vert_off
+    for i in range(len(charts)):
+        s = slice(vo[i], vo[i + 1])
+        assert unit_box_err(outs[0][s], ref[s]) <= 1e-5

@@ -45,6 +45,8 @@ struct uva_plan {
     std::vector<double> P;                 // host positions (LSCM pins given later)
     std::vector<int> lpin0, lpin1, v0;     // host copies
     std::vector<int> vinv;                 // caller's vertex id -> plan vertex
+    uva::HostPlan sched;                   // island sizes and schedule inputs (for uva_plan_tune)
+    int team_alloc = 0;                    // teams the barrier arrays hold
     double2* ubuf[3] = {nullptr, nullptr, nullptr};  // plan-order copies of the caller's vertex arrays
     int* d_lpin0 = nullptr;
     int* d_lpin1 = nullptr;
@@ -227,7 +229,7 @@ int upload_plan(uva::HostPlan& hp, const double* P, hipStream_t s, uva_plan** ou
     UP(diag, diag); UP(dinv32, dinv32);
     UP(adj_ptr, adj_ptr); UP(adj_idx, adj_idx); UP(vbnd, vbnd);
     UP(order_group, order_group); UP(order_wave, order_wave);
-    UP(grp_mem, grp_mem); UP(grp_m, grp_m); UP(grp_team, grp_team);
+    UP(grp_mem, grp_mem); UP(grp_m, grp_m); UP(grp_team, grp_team); UP(slot_ptr, slot_ptr);
     UP(vperm, vperm); UP(isl_cofs, isl_cofs); UP(isl_cg, isl_cg); UP(cinv, cinv);
 #undef UP
     pl->vinv.assign(hp.n_vert, 0);
@@ -273,6 +275,17 @@ int upload_plan(uva::HostPlan& hp, const double* P, hipStream_t s, uva_plan** ou
         if ((e = pl->alloc(&d.team_cnt, (size_t)kMaxSlots * std::max(1, hp.n_teams))) != hipSuccess ||
             (e = pl->alloc(&d.team_part, (size_t)std::max(1, hp.n_teams) * 2 * uva::kMaxTeam * 8)) != hipSuccess)
             return bail(UVA_ENOMEM, "team arrays");
+        pl->team_alloc = std::max(1, hp.n_teams);
+    }
+    {
+        uva::HostPlan& sc = pl->sched;
+        sc.n_isl = hp.n_isl;
+        sc.isl_nv = hp.isl_nv;
+        sc.isl_ns = hp.isl_ns;
+        sc.group_isl = hp.group_isl;
+        sc.sched_slots = hp.sched_slots;
+        sc.sched_teams = hp.sched_teams;
+        sc.n_xcd = hp.n_xcd;
     }
     // scratch
     {
@@ -318,7 +331,7 @@ int upload_plan(uva::HostPlan& hp, const double* P, hipStream_t s, uva_plan** ou
     // was built for them, else everything maps to the single list
     d.n_xcd = hp.n_xcd;
     d.max_m = hp.max_m;
-    for (int x = 0; x <= uva::kMaxXcd; ++x) d.grp_xoff[x] = hp.grp_xoff[std::min(x, hp.n_xcd)];
+    d.slots_per_xcd = hp.slots_per_xcd;
     const XccInfo xi = xcc_info();
     d.xcd_local = (hp.n_xcd > 1 && hp.n_xcd == xi.n) ? 1 : 0;
     // an XCD the probe never saw has no list (its workgroups only take wavefront tasks)
@@ -355,14 +368,14 @@ int uva_plan_create(uva_plan** out, int64_t n_islands, const int64_t* vert_off, 
     *out = nullptr;
     if ((X || area || cots) && !(X && area && cots)) return fail(UVA_EINVAL, "X, area and cots go together");
     hipStream_t s = (hipStream_t)stream;
-    // one 1024-thread workgroup per CU is resident: the team plan balances
-    // the islands over that many concurrent workgroups
-    int team_slots = 0;
-    if (!(flags & UVA_PLAN_NO_TEAMS)) {
+    // one CG workgroup per CU is resident: the schedule has one workgroup
+    // slot per CU
+    int slots = 1;
+    {
         int dev = 0, c = 0;
         if (hipGetDevice(&dev) == hipSuccess &&
-            hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess)
-            team_slots = c;
+            hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && c > 0)
+            slots = c;
     }
     const int n_xcd = xcc_info().n;
     uva::HostPlan hp;
@@ -371,7 +384,7 @@ int uva_plan_create(uva_plan** out, int64_t n_islands, const int64_t* vert_off, 
         err = uva::build_host_plan(hp, n_islands, vert_off, tri_off, P, T, pins, X, area, cots,
                                    (flags & UVA_PLAN_BATCH) != 0, nullptr,
                                    (flags & UVA_PLAN_NO_WAVE) ? -1 : uva::kWaveMaxRows,
-                                   (flags & UVA_PLAN_NO_TEAMS) ? 0 : team_slots, n_xcd,
+                                   slots, n_xcd, (flags & UVA_PLAN_NO_TEAMS) == 0,
                                    (flags & UVA_PLAN_KEEP_ORDER) == 0, (flags & UVA_PLAN_NO_COARSE) == 0);
     } catch (const std::bad_alloc&) {
         return fail(UVA_ENOMEM, "host allocation failed while building the plan");
@@ -439,6 +452,54 @@ int uva_plan_get_stats(uva_plan* plan, uva_island_stats* stats, void* stream) {
         o.step = st[i].step;
         o.busy_ms = plan->clock_khz > 0 ? (double)st[i].ticks / (double)plan->clock_khz : 0.0;
     }
+    return UVA_OK;
+}
+
+int uva_plan_tune(uva_plan* plan, void* stream) {
+    if (!valid_plan(plan)) return fail(UVA_EINVAL, "null argument");
+    hipStream_t s = (hipStream_t)stream;
+    uva::DevPlan& d = plan->dp;
+    std::vector<uva::IslandState> st(d.n_isl);
+    HIP_TRY(hipMemcpyAsync(st.data(), d.st, st.size() * sizeof(uva::IslandState), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    std::vector<double> est(d.n_isl);
+    bool any = false;
+    for (int i = 0; i < d.n_isl; ++i) {
+        est[i] = (double)st[i].cg_iters / std::max(1, st[i].arap_iters);
+        any = any || st[i].cg_iters > 0;
+    }
+    if (!any) return fail(UVA_EINVAL, "uva_plan_tune needs a finished solve on the plan (no CG iterations recorded)");
+    uva::HostPlan& sc = plan->sched;
+    try {
+        uva::plan_teams(sc, sc.group_isl, sc.sched_slots, sc.n_xcd, sc.sched_teams, est.data());
+    } catch (const std::bad_alloc&) {
+        return fail(UVA_ENOMEM, "host allocation failed while re-planning");
+    }
+    // new task arrays (the old ones stay allocated until the plan goes away:
+    // a cached graph or work in flight on another stream may still read them)
+#define UPS(field, vec)                                                                          \
+    do {                                                                                         \
+        hipError_t _e = plan->upload(&d.field, sc.vec, s);                                        \
+        if (_e != hipSuccess) return fail(UVA_EHIP, std::string("upload " #vec ": ") + hipGetErrorString(_e)); \
+    } while (0)
+    UPS(order_group, order_group); UPS(grp_mem, grp_mem); UPS(grp_m, grp_m); UPS(grp_team, grp_team);
+    UPS(slot_ptr, slot_ptr);
+#undef UPS
+    if (sc.n_teams > plan->team_alloc) {
+        if (plan->alloc(&d.team_cnt, (size_t)kMaxSlots * sc.n_teams) != hipSuccess ||
+            plan->alloc(&d.team_part, (size_t)sc.n_teams * 2 * uva::kMaxTeam * 8) != hipSuccess)
+            return fail(UVA_ENOMEM, "team arrays");
+        plan->team_alloc = sc.n_teams;
+    }
+    d.n_group = (int)sc.order_group.size();
+    d.n_teams = sc.n_teams;
+    d.max_m = sc.max_m;
+    d.slots_per_xcd = sc.slots_per_xcd;
+    // a cached graph holds the old launch parameters
+    if (plan->gexec) { (void)hipGraphExecDestroy(plan->gexec); plan->gexec = nullptr; }
+    if (plan->graph) { (void)hipGraphDestroy(plan->graph); plan->graph = nullptr; }
+    plan->gkey.clear();
+    HIP_TRY(hipStreamSynchronize(s));
     return UVA_OK;
 }
 

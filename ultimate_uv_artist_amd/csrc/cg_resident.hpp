@@ -15,9 +15,11 @@
 // over aggregates of g consecutive 64-row slices.  The plan orders vertices
 // by Morton code, so slices are compact surface patches and the coarse level
 // removes the smooth error that Jacobi-PCG needs O(sqrt(n)) iterations for.
-// Z^T r is a wavefront sum per slice; the slices' sums go through global
-// memory to the whole team, and each member applies the rows of Ac^-1 (kept
-// in LDS for the whole solve) that its own aggregates need.
+// Z^T r: each lane parks r of its rows in their u slots in LDS (free until the
+// new u is written), and one thread per slice sums the slice's 64 rows; the
+// slices' sums go through global memory to the whole team, and one thread per
+// aggregate of the member applies the aggregate's row of Ac^-1 (kept in LDS
+// for the whole solve).
 //
 // Members of a multi-workgroup team keep u for a window of rows that extends
 // H = min(bandwidth, kHalo) rows beyond each end of their own range.  They
@@ -38,8 +40,11 @@ template <class V2, class V>
 UVA_HD __forceinline__ int res_vec_bytes(int rows, int halo) {
     return align16((rows + 2 * halo) * (int)sizeof(V2) + rows * ((int)sizeof(V2) + 2 * (int)sizeof(V)));
 }
+// rows of Ac^-1 are stored with an odd stride, so the threads of different
+// aggregates read different LDS banks
+UVA_HD __forceinline__ int coarse_stride(int nc) { return nc | 1; }
 UVA_HD __forceinline__ int res_coarse_bytes(int nc, int na) {
-    return nc > 0 ? align16(na * nc * 4) + (nc + na) * 16 : 0;
+    return nc > 0 ? align16(na * coarse_stride(nc) * 4) + (nc + na) * 16 : 0;
 }
 
 template <int PREC, int RPT, class Team>
@@ -47,7 +52,9 @@ struct ResCG {
     using PT = Prec<PREC>;
     using V = typename PT::V;
     using V2 = typename Vec2<V>::type;
-    static constexpr int kG = 2;   // matrix entries per slot whose loads are issued together
+    // matrix entries per slot whose loads are issued together (as many as the
+    // registers hold beside the slots' vectors)
+    static constexpr int kG = RPT * (int)sizeof(V) <= 16 ? 2 : 1;
 
     Team& tm;
     const DevPlan& P;
@@ -57,7 +64,7 @@ struct ResCG {
     bool multi;        // several members (halo exchange through global memory)
     bool gfull;        // the bandwidth exceeds the window: publish every row
     bool coarse;       // additive coarse correction
-    int g, nc, alo, na;   // g slices per aggregate, nc aggregates; the member's are [alo, alo + na)
+    int g, nc, ncp, alo, na;   // g slices per aggregate, nc aggregates (row stride ncp); the member's are [alo, alo + na)
     V2* lu;     // LDS: u of the window, row - wlo
     V2* le;     // LDS: e of the member's rows
     V* ldv;     // LDS: D^-1
@@ -85,6 +92,7 @@ struct ResCG {
         coarse = use_coarse;
         g = d.cg;
         nc = (d.ns + g - 1) / g;
+        ncp = coarse_stride(nc);
         alo = slo / g;
         na = shi > slo ? (shi - 1) / g - alo + 1 : 0;
         const int rows = (shi - slo) * kSlice;
@@ -94,7 +102,7 @@ struct ResCG {
         ldv = (V*)(le + rows);
         ldg = ldv + rows;
         lci = (float*)(base + res_vec_bytes<V2, V>(rows, multi ? kHalo : 0));
-        lrc = (double2*)((char*)lci + align16(na * nc * 4));
+        lrc = (double2*)((char*)lci + align16(na * ncp * 4));
         ly = lrc + nc;
         gu = (V2*)P.vu;
 #pragma unroll
@@ -110,7 +118,7 @@ struct ResCG {
         }
         if (coarse) {
             const float* gci = P.cinv + d.cofs + (long long)alo * nc;
-            for (int i = tm.tid(); i < na * nc; i += tm.tstride()) lci[i] = gci[i];
+            for (int i = tm.tid(); i < na * nc; i += tm.tstride()) lci[(i / nc) * ncp + i % nc] = gci[i];
         }
         // each pass reads D^-1 and D of its own rows only, and the coarse
         // rows after a team barrier
@@ -142,10 +150,19 @@ struct ResCG {
         }
         tm.bsync();
     }
-    // Z^T r of slot j's slice (a wavefront sum, FP64) to global memory.
-    UVA_HD __forceinline__ void restrict_slot(int j, bool ok) {
-        const double2 rc = tm.wsum2(make_double2(ok ? (double)r[j].x : 0.0, ok ? (double)r[j].y : 0.0));
-        if (lane == 0) P.crc[d.s0 + slo + j * W + wv] = rc;
+    // Z^T r of the member's slices (r parked in the u slots) to global memory.
+    UVA_HD __forceinline__ void restrict_slices() {
+        tm.bsync();
+        for (int t = tm.tid(); t < shi - slo; t += tm.tstride()) {
+            double2 acc = make_double2(0.0, 0.0);
+            const int r0 = rlo + t * kSlice, r1 = min(r0 + kSlice, rhi);
+            for (int row = r0; row < r1; ++row) {
+                const V2 v = lu[row - wlo];
+                acc.x += (double)v.x;
+                acc.y += (double)v.y;
+            }
+            P.crc[d.s0 + slo + t] = acc;
+        }
     }
     // After every slice's Z^T r is in global memory: Z^T r per aggregate,
     // then y = Ac^-1 Z^T r for the member's aggregates, into LDS.
@@ -162,16 +179,25 @@ struct ResCG {
             lrc[b] = acc;
         }
         tm.bsync();
-        for (int a = wv; a < na; a += W) {
-            double2 part = make_double2(0.0, 0.0);
-            for (int b = lane; b < nc; b += kSlice) {
-                const double c = (double)lci[a * nc + b];
-                const double2 v = lrc[b];
-                part.x = fma(c, v.x, part.x);
-                part.y = fma(c, v.y, part.y);
+        for (int a = tm.tid(); a < na; a += tm.tstride()) {
+            const float* ci = lci + a * ncp;
+            double yx0 = 0.0, yy0 = 0.0, yx1 = 0.0, yy1 = 0.0;
+            int b = 0;
+            for (; b + 1 < nc; b += 2) {
+                const double c0 = (double)ci[b], c1 = (double)ci[b + 1];
+                const double2 v0 = lrc[b], v1 = lrc[b + 1];
+                yx0 = fma(c0, v0.x, yx0);
+                yy0 = fma(c0, v0.y, yy0);
+                yx1 = fma(c1, v1.x, yx1);
+                yy1 = fma(c1, v1.y, yy1);
             }
-            const double2 y = tm.wsum2(part);
-            if (lane == 0) ly[a] = y;
+            if (b < nc) {
+                const double c0 = (double)ci[b];
+                const double2 v0 = lrc[b];
+                yx0 = fma(c0, v0.x, yx0);
+                yy0 = fma(c0, v0.y, yy0);
+            }
+            ly[a] = make_double2(yx0 + yx1, yy0 + yy1);
         }
         tm.bsync();
     }
@@ -326,9 +352,10 @@ struct ResCG {
                     le[l] = zero;
                     a1.x += PT::kResid64 ? rx * rx : (double)((float)rx * (float)rx);
                     a1.y += PT::kResid64 ? ry * ry : (double)((float)ry * (float)ry);
+                    if (coarse) lu[row - wlo] = r[j];
                 }
-                if (coarse) restrict_slot(j, ok);
             }
+            if (coarse) restrict_slices();
             const double2 ru = precondition();
             a1.z += ru.x;
             a1.w += ru.y;
@@ -356,7 +383,7 @@ struct ResCG {
                     const bool ok = rlo + l < rhi;
                     if (ok) {
                         // u of the last precondition step (each lane reads
-                        // only its own rows, rewritten by itself below)
+                        // and rewrites only its own rows' slots here)
                         const V2 uA = lu[rlo + l - wlo];
                         if (first) {
                             p[j] = uA;
@@ -373,9 +400,10 @@ struct ResCG {
                         r[j].y = fma(-ay, s[j].y, r[j].y);
                         g1.x += acc_mul<PREC>(r[j].x, r[j].x);
                         g1.y += acc_mul<PREC>(r[j].y, r[j].y);
+                        if (coarse) lu[rlo + l - wlo] = r[j];
                     }
-                    if (coarse) restrict_slot(j, ok);
                 }
+                if (coarse) restrict_slices();
                 const double2 ru2 = precondition();
                 g1.z += ru2.x;
                 g1.w += ru2.y;

@@ -97,8 +97,6 @@ struct WaveTeam {
     __device__ int tri_hi(int nt) const { return nt; }
     // LDS writes of the workgroup (here: wavefront) become visible to it.
     __device__ void bsync() const { sync(); }
-    // sum over the 64 lanes of the calling wavefront (the same in every lane)
-    __device__ double2 wsum2(double2 v) const { return make_double2(wave_sum(v.x), wave_sum(v.y)); }
     // Global-memory writes of this wavefront become visible to its other lanes.
     __device__ void sync() const {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
@@ -146,7 +144,6 @@ struct BlockTeam {
     __device__ int tri_hi(int nt) const { return nt; }
     __device__ void sync() const { __syncthreads(); }
     __device__ void bsync() const { __syncthreads(); }
-    __device__ double2 wsum2(double2 v) const { return make_double2(wave_sum(v.x), wave_sum(v.y)); }
     // The two slot buffers alternate; a buffer is rewritten only after the
     // barrier of the following reduction, when every thread has read it.
     __device__ double4 sum4(double4 v) {
@@ -245,7 +242,6 @@ struct TeamWG {
     __device__ int tri_hi(int nt) const { return (int)((long long)nt * (mem + 1) / m); }
     // LDS writes of this member become visible to the member's threads.
     __device__ void bsync() const { __syncthreads(); }
-    __device__ double2 wsum2(double2 v) const { return make_double2(wave_sum(v.x), wave_sum(v.y)); }
     __device__ void sync() {
         if (local) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         __syncthreads();
@@ -957,18 +953,23 @@ __global__ __launch_bounds__(BS) void island_kernel(DevPlan P, TaskArgs a, int s
     __shared__ double lvec[kLv > 0 ? kLv / 8 : 1];
     char* const lv = kLv > 0 ? (char*)lvec : nullptr;
     BlockTeam<BS> bt(red, lv, kLv);
-    // the workgroup tasks of this XCD's list, then wavefront tasks of any island
+    // the workgroup slots of this XCD (each slot's tasks in order), then
+    // wavefront tasks of any island
     const int list = uniform_i(P.xcc_map[xcc_id()]);
-    const int t_lo = list >= 0 ? uniform_i(P.grp_xoff[list]) : 0;
-    const int t_n = list >= 0 ? uniform_i(P.grp_xoff[list + 1]) - t_lo : 0;
     int* const q = P.queue + kQueueStride * slot;
-    for (; t_n > 0;) {
-        if (threadIdx.x == 0) s_task = atomicAdd(q + list, 1);
-        __syncthreads();
-        int task = uniform_i(s_task);
-        __syncthreads();
-        if (task >= t_n) break;
-        task += t_lo;
+    for (int t_cur = 0, t_end = 0; list >= 0;) {
+        if (t_cur == t_end) {
+            // claim the next unclaimed slot of the XCD
+            if (threadIdx.x == 0) s_task = atomicAdd(q + list, 1);
+            __syncthreads();
+            const int k = uniform_i(s_task);
+            __syncthreads();
+            if (k >= P.slots_per_xcd) break;
+            t_cur = uniform_i(P.slot_ptr[list * P.slots_per_xcd + k]);
+            t_end = uniform_i(P.slot_ptr[list * P.slots_per_xcd + k + 1]);
+            continue;
+        }
+        const int task = t_cur++;
         const int isl = uniform_i(P.order_group[task]);
         const int m = uniform_i(P.grp_m[task]);
         if (m <= 1) {
@@ -1033,7 +1034,7 @@ static hipError_t launch_kind(const DevPlan& P, const TaskArgs& a, int slot, int
     // so every XCD list gets at least as many workgroups as its largest team
     const int cap = cus * occ;
     int grid = P.n_group + (P.n_wave + BS / 64 - 1) / (BS / 64);
-    if (P.n_group > 0) grid = grid > P.n_xcd * P.max_m ? grid : P.n_xcd * P.max_m;
+    if (P.n_group > 0) grid = grid > P.n_xcd * P.slots_per_xcd ? grid : P.n_xcd * P.slots_per_xcd;
     grid = (grid + P.n_xcd - 1) / P.n_xcd * P.n_xcd;
     grid = grid < cap ? (grid > 0 ? grid : 1) : cap;
     hipLaunchKernelGGL((island_kernel<BS, KIND, PREC>), dim3(grid), dim3(BS), 0, s, P, a, slot);

@@ -126,17 +126,19 @@ struct DevPlan {
     int n_teams;
     int* team_cnt;               // [launch slot][team] barrier counters, zeroed per call sequence
     double* team_part;           // [team][2][kMaxTeam][8] partial sums
-    // Workgroup tasks are split into one list per XCD: [grp_xoff[x],
-    // grp_xoff[x+1]) of order_group.  A workgroup takes tasks from the list
-    // of the XCD it runs on (hardware id -> list through xcc_map), so all
-    // members of a team share one L2 and their barrier needs no L2
-    // write-back or invalidation (xcd_local).  Without a probed XCD map there
-    // is one list and teams use agent-scope release/acquire.
+    // Static schedule of the workgroup tasks: slots_per_xcd slots per XCD;
+    // slot q = x * slots_per_xcd + k runs tasks [slot_ptr[q], slot_ptr[q+1])
+    // of order_group in order.  A workgroup claims the slots of the XCD it
+    // runs on (hardware id -> list x through xcc_map), so all members of a
+    // team share one L2 and their barrier needs no L2 write-back or
+    // invalidation (xcd_local).  Without a probed XCD map there is one list
+    // and teams use agent-scope release/acquire.
     int n_xcd;
     int xcd_local;
     int max_m;                   // largest team
+    int slots_per_xcd;
+    const int* slot_ptr;
     int xcc_map[16];
-    int grp_xoff[kMaxXcd + 1];
     long long spin_ticks;        // hang guard of a team barrier (device clock ticks)
     int* fault;                  // set when a team barrier gave up waiting
 };

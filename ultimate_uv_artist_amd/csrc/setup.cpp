@@ -355,7 +355,7 @@ void parallel_for(int n, F fn) {
 std::string build_host_plan(HostPlan& hp, int64_t n_islands, const int64_t* vert_off, const int64_t* tri_off,
                             const double* P, const int64_t* T, const int64_t* pins, const double* X,
                             const double* area, const double* cots, bool batch, const int64_t* lscm_pins,
-                            int wave_max_rows, int team_slots, int n_xcd, bool reorder, bool coarse) {
+                            int wave_max_rows, int slots, int n_xcd, bool teams, bool reorder, bool coarse) {
     if (n_islands <= 0) return "n_islands must be positive";
     if (vert_off[0] != 0 || tri_off[0] != 0) return "offsets must start at 0";
     for (int64_t i = 0; i < n_islands; ++i) {
@@ -556,62 +556,100 @@ std::string build_host_plan(HostPlan& hp, int64_t n_islands, const int64_t* vert
     }
     std::sort(wv.begin(), wv.end());
     for (auto& p : wv) hp.order_wave.push_back(p.second);
-    plan_teams(hp, g, team_slots, n_xcd);
+    hp.group_isl = g;
+    hp.sched_slots = slots;
+    hp.sched_teams = teams;
+    plan_teams(hp, g, slots, n_xcd, teams, nullptr);
     return "";
 }
 
-void plan_teams(HostPlan& hp, const std::vector<int>& group_islands, int team_slots, int n_xcd) {
-    hp.order_group.clear(); hp.grp_mem.clear(); hp.grp_m.clear(); hp.grp_team.clear();
+void plan_teams(HostPlan& hp, const std::vector<int>& group_islands, int slots, int n_xcd, bool teams,
+                const double* est_iters) {
+    hp.order_group.clear(); hp.grp_mem.clear(); hp.grp_m.clear(); hp.grp_team.clear(); hp.slot_ptr.clear();
     hp.n_teams = 0;
     hp.max_m = 1;
     n_xcd = std::max(1, std::min(n_xcd, kMaxXcd));
     hp.n_xcd = n_xcd;
-    auto cost = [&](int i) { return std::pow((double)hp.isl_nv[i], 1.5); };
-    double total = 0.0;
-    for (int i = 0; i < hp.n_isl; ++i) total += cost(i);
-    // an island longer than the ideal per-workgroup share gets more workgroups
-    const double share = team_slots > 0 ? total / team_slots : 0.0;
+    const int S = std::max(1, slots / n_xcd);   // workgroup slots per XCD
+    hp.slots_per_xcd = S;
+    const int mcap = std::max(1, std::min(kMaxTeam, S));
+    // CG iterations per global step (measured ones when given) and the time
+    // of one island task on m members: per CG iteration a fixed part (team
+    // barriers and latency) and a part per slice of the member
+    auto iters = [&](int i) {
+        return est_iters ? std::max(1.0, est_iters[i]) : 3.2 * std::pow((double)hp.isl_nv[i], 0.4);
+    };
+    auto t_of = [&](int i, int m) {
+        const int per = (hp.isl_ns[i] + m - 1) / m;
+        return iters(i) * ((m > 1 ? 30.0 : 15.0) + 0.1 * per);
+    };
+    const int n = (int)group_islands.size();
+    std::vector<int> m(n, 1);
+    if (teams) {
+        double work = 0.0;
+        for (int k = 0; k < n; ++k) {
+            const int i = group_islands[k];
+            m[k] = std::max(1, std::min(mcap, (hp.isl_ns[i] + kResSlices - 1) / kResSlices));
+            work += m[k] * t_of(i, m[k]);
+        }
+        // islands longer than the ideal makespan get members while that pays
+        const double target = work / (double)(S * n_xcd);
+        for (int k = 0; k < n; ++k) {
+            const int i = group_islands[k];
+            while (m[k] < mcap && t_of(i, m[k]) > target && t_of(i, m[k] + 1) < 0.9 * t_of(i, m[k])) ++m[k];
+        }
+    }
     struct Item {
-        double key;   // -(time per member)
+        double t;   // estimated time on its members
         int isl, m;
     };
-    std::vector<Item> items;
-    for (int i : group_islands) {
-        int m = 1;
-        if (share > 0.0) {
-            m = (int)std::ceil(cost(i) / share);
-            m = std::min(m, std::min(kMaxTeam, std::max(1, hp.isl_nv[i] / kTeamMinRows)));
-            // enough members that each keeps its rows resident in the ARAP CG
-            const int need = (hp.isl_ns[i] + kResSlices - 1) / kResSlices;
-            m = std::max(m, std::min(need, kMaxTeam));
-            m = std::max(m, 1);
-        }
-        items.push_back({-cost(i) / m, i, m});
-    }
+    std::vector<Item> items(n);
+    for (int k = 0; k < n; ++k) items[k] = {t_of(group_islands[k], m[k]), group_islands[k], m[k]};
     std::sort(items.begin(), items.end(), [](const Item& a, const Item& b) {
-        return a.key < b.key || (a.key == b.key && a.isl < b.isl);
+        return a.t > b.t || (a.t == b.t && a.isl < b.isl);
     });
-    // deal to the least-loaded XCD list (load = workgroup time = cost)
+    // islands to the least-loaded XCD (load = workgroup time)
     std::vector<double> load(n_xcd, 0.0);
     std::vector<std::vector<Item>> lists(n_xcd);
     for (const Item& it : items) {
-        int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-        load[x] += cost(it.isl);
+        const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        load[x] += it.t * it.m;
         lists[x].push_back(it);
     }
-    hp.grp_xoff.assign(n_xcd + 1, 0);
+    // per XCD: longest first, each island's members on the slots that free
+    // up first, starting together; every slot keeps the global order
+    struct Task {
+        int isl, mem, m, team;
+    };
+    std::vector<std::vector<Task>> st((size_t)S * n_xcd);
+    std::vector<int> idx(S);
     for (int x = 0; x < n_xcd; ++x) {
+        std::vector<double> free(S, 0.0);
         for (const Item& it : lists[x]) {
+            for (int k = 0; k < S; ++k) idx[k] = k;
+            std::partial_sort(idx.begin(), idx.begin() + it.m, idx.end(), [&](int a, int b) {
+                return free[a] < free[b] || (free[a] == free[b] && a < b);
+            });
+            std::sort(idx.begin(), idx.begin() + it.m);
+            double start = 0.0;
+            for (int k = 0; k < it.m; ++k) start = std::max(start, free[idx[k]]);
             const int team = it.m > 1 ? hp.n_teams++ : -1;
             hp.max_m = std::max(hp.max_m, it.m);
             for (int k = 0; k < it.m; ++k) {
-                hp.order_group.push_back(it.isl);
-                hp.grp_mem.push_back(k);
-                hp.grp_m.push_back(it.m);
-                hp.grp_team.push_back(team);
+                st[(size_t)x * S + idx[k]].push_back({it.isl, k, it.m, team});
+                free[idx[k]] = start + it.t;
             }
         }
-        hp.grp_xoff[x + 1] = (int)hp.order_group.size();
+    }
+    hp.slot_ptr.assign(st.size() + 1, 0);
+    for (size_t q = 0; q < st.size(); ++q) {
+        for (const Task& t : st[q]) {
+            hp.order_group.push_back(t.isl);
+            hp.grp_mem.push_back(t.mem);
+            hp.grp_m.push_back(t.m);
+            hp.grp_team.push_back(t.team);
+        }
+        hp.slot_ptr[q + 1] = (int)hp.order_group.size();
     }
 }
 
@@ -714,7 +752,10 @@ std::string build_matrix_plan(HostPlan& hp, int64_t n, const int64_t* indptr, co
         hp.cinv.assign((size_t)nc * nc, 0.0f);
         if (coarse_inverse(nv, g, hp.adj_ptr, hp.adj_idx, val, hp.diag, -1, 0.0, hp.cinv.data())) hp.isl_cofs[0] = 0;
     }
-    if (nv > kWaveMaxRows) plan_teams(hp, {0}, 0);
+    if (nv > kWaveMaxRows) {
+        hp.group_isl = {0};
+        plan_teams(hp, {0}, 1, 1, false, nullptr);
+    }
     else hp.order_wave.push_back(0);
     return "";
 }

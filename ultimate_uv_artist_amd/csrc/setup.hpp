@@ -35,9 +35,15 @@ struct HostPlan {
     std::vector<int> grp_mem, grp_m, grp_team;
     int n_teams = 0;
     int max_m = 1;
-    // order_group is split into n_xcd lists: list x is [grp_xoff[x], grp_xoff[x+1])
+    // static schedule: n_xcd x slots_per_xcd workgroup slots; slot q runs the
+    // tasks [slot_ptr[q], slot_ptr[q+1]) of order_group in order
     int n_xcd = 1;
-    std::vector<int> grp_xoff{0, 0};
+    int slots_per_xcd = 1;
+    std::vector<int> slot_ptr{0, 0};
+    // inputs of the schedule, kept to re-plan it from measured iterations
+    std::vector<int> group_isl;
+    int sched_slots = 1;
+    bool sched_teams = false;
     int64_t nnz_true = 0;
     // plan vertex -> caller's global vertex id (islands keep their offsets;
     // inside an island the plan orders vertices by Morton code)
@@ -54,15 +60,20 @@ struct HostPlan {
 std::string build_host_plan(HostPlan& hp, int64_t n_islands, const int64_t* vert_off, const int64_t* tri_off,
                             const double* P, const int64_t* T, const int64_t* pins, const double* X,
                             const double* area, const double* cots, bool batch, const int64_t* lscm_pins,
-                            int wave_max_rows = kWaveMaxRows, int team_slots = 0, int n_xcd = 1,
+                            int wave_max_rows = kWaveMaxRows, int slots = 1, int n_xcd = 1, bool teams = false,
                             bool reorder = true, bool coarse = true);
 
-// Group-task order and team sizes: islands whose cost exceeds their share of
-// `team_slots` concurrent workgroups get several workgroups (at most kMaxTeam,
-// at least kTeamMinRows rows each).  team_slots = 0 gives every island one.
-// Islands are dealt to n_xcd lists (one per XCD), largest first onto the
-// least-loaded list; within a list, longest time per member first.
-void plan_teams(HostPlan& hp, const std::vector<int>& group_islands, int team_slots, int n_xcd = 1);
+// Static gang schedule of the workgroup tasks.  `slots` workgroups run at
+// once (one per CU for the CG kernels), slots / n_xcd per XCD.  Team sizes
+// (teams = true): enough members that each member's rows stay resident in the
+// ARAP CG, more for islands whose estimated time exceeds the batch's ideal
+// makespan.  The estimate is est_iters[i] CG iterations per global step (or a
+// size-based guess when NULL) times a per-iteration cost.  Islands go to the
+// least-loaded XCD; within an XCD, longest first, an island's m members take
+// the m slots that free up first and start together.  Every slot lists its
+// tasks in one global order, so no two teams wait on each other.
+void plan_teams(HostPlan& hp, const std::vector<int>& group_islands, int slots, int n_xcd, bool teams,
+                const double* est_iters);
 
 // One island holding a general symmetric matrix (no triangles, no pin).
 std::string build_matrix_plan(HostPlan& hp, int64_t n, const int64_t* indptr, const int64_t* indices,

@@ -107,6 +107,11 @@ class IslandPlan:
         names = [k for k, _ in _lib.IslandStats._fields_ if k != "pad"]
         return {k: np.array([getattr(s, k) for s in arr]) for k in names}
 
+    def tune(self):
+        """Re-plan team sizes and the workgroup schedule from the CG iterations
+        each island took in the last solve call (see uva_plan_tune)."""
+        check(_lib.lib().uva_plan_tune(self._h, stream_of(self.dev)), "uva_plan_tune")
+
     # ------------------------------------------------------------- helpers
     def device_uv(self, U):
         """(n,2) float64 contiguous tensor on the plan's GPU (a copy of numpy input)."""
