@@ -41,6 +41,8 @@ def parse():
     ap.add_argument("--cg-tol", type=float, default=1e-6)
     ap.add_argument("--precision", default="mixed", choices=["mixed", "f32", "f64"])
     ap.add_argument("--cpu-sample", type=int, default=160, help="islands in the CPU baseline sample (0 = skip)")
+    ap.add_argument("--verdict-sample", type=int, default=16,
+                    help="islands of the CPU sample whose GPU and oracle layouts are validated (0 = skip)")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-tune", dest="tune", action="store_false",
                     help="keep the size-based schedule (no re-planning from measured iterations)")
@@ -244,7 +246,7 @@ def main():
         Uh = U.cpu().numpy()
         vo = plan.vert_off
         t_cpu = 0.0
-        err = []
+        err, lay = [], []
         for i in pick:
             P, T = charts[i]
             u0 = Uh0[vo[i]:vo[i + 1]]
@@ -255,13 +257,30 @@ def main():
             lo = ur.min(0)
             s = float((ur.max(0) - lo).max())
             err.append(float(np.abs((g - lo) / s - (ur - lo) / s).max()))
+            lay.append((i, g, ur))
+        # validate.py verdicts and distortion quantiles of the GPU and oracle
+        # layouts on islands spread over the sample's sizes
+        from oracle import validate_cpu as vc
+
+        nvs = min(args.verdict_sample, len(lay))
+        sel = np.unique(np.linspace(0, len(lay) - 1, nvs).round().astype(int)) if nvs > 0 else []
+        nvs, same, dq = len(sel), 0, 0.0
+        for j in sel:
+            i, g, ur = lay[j]
+            P, T = charts[i]
+            same += int(vc.island_verdict(P, T, g) == vc.island_verdict(P, T, ur))
+            q, qr = oracle.chart_quality(P, T, g), oracle.chart_quality(P, T, ur)
+            dq = max(dq, abs(q["aniso_p95"] / qr["aniso_p95"] - 1), abs(q["area_p95"] / qr["area_p95"] - 1))
         cpu_tri_iter = float(tri[pick].sum()) * args.iters
         cpu = {"value": cpu_tri_iter / t_cpu / 1e6, "unit": "Mtri*iter/s", "cores": 1, "kind": "port",
                "sample": f"{len(pick)} islands at evenly spaced quantiles of the triangle mass "
                          f"({int(tri[pick].sum())} triangles), {args.iters} ARAP iterations each from the same "
                          f"LSCM layout, SuperLU factorisation included, {t_cpu:.1f}s on one core"}
-        parity = {"islands": len(pick), "max_uv_err_unit_box": max(err)}
-        print(f"bench: CPU oracle sample {cpu['value']:.3f} Mtri*iter/s; parity max err {max(err):.2e}", flush=True)
+        parity = {"islands": len(pick), "max_uv_err_unit_box": max(err), "verdicts_checked": nvs,
+                  "verdicts_identical": same, "max_rel_diff_p95_aniso_area": dq}
+        print(f"bench: CPU oracle sample {cpu['value']:.3f} Mtri*iter/s; parity max err {max(err):.2e}; "
+              f"validate verdicts identical on {same}/{nvs} islands; p95 anisotropy/area within {dq:.1e}",
+              flush=True)
 
     rec = {
         "metric": "batched ARAP Mtri*iter/s at relative residual 1e-6",

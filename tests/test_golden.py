@@ -1,6 +1,7 @@
 """The oracle against golden vectors written by the reference itself
 (tests/golden/make_golden.py): LSCM, refine trajectories with and without the
-energy early exit, chart_quality, the sequential flip repair, and gpu.Batch."""
+energy early exit, chart_quality, the sequential flip repair, gpu.Batch, SLIM
+trajectories, and the validation verdicts (validate_uvs, chart_overlaps)."""
 from __future__ import annotations
 
 from pathlib import Path
@@ -91,3 +92,34 @@ def test_slim_trajectory(case):
     assert len(trace) == len(ref_e) - 1          # no backtracking in these cases: one energy per iteration
     np.testing.assert_allclose([e for e, _ in trace], ref_e[1:], rtol=1e-9)
     assert np.abs(own_box(U) - own_box(d[f"slim{case}"])).max() <= 1e-9
+
+
+@pytest.fixture(scope="module")
+def vg():
+    return dict(np.load(GOLD / "validate_golden.npz"))
+
+
+def _vcases():
+    d = np.load(GOLD / "validate_golden.npz")
+    return list(d["v_names"]), list(d["c_names"])
+
+
+@pytest.mark.parametrize("i", range(len(_vcases()[0])), ids=_vcases()[0])
+def test_validate_uvs_report(vg, i):
+    """validate_uvs(gutter=False): every counter, the verdict and the
+    per-triangle problem mask as the reference reports them."""
+    from oracle import validate_cpu as vc
+
+    rep, prob = vc.validate_uvs(vg[f"v{i}_P"], vg[f"v{i}_F"].astype(np.int64), vg[f"v{i}_U"])
+    keys = [str(k) for k in vg["report_keys"]]
+    got = np.array([float(rep.get(k, -1)) for k in keys])
+    assert dict(zip(keys, got)) == dict(zip(keys, vg[f"v{i}_report"]))
+    assert np.array_equal(prob, vg[f"v{i}_problems"])
+
+
+@pytest.mark.parametrize("i", range(len(_vcases()[1])), ids=_vcases()[1])
+def test_chart_overlaps(vg, i):
+    from oracle import validate_cpu as vc
+
+    m = vc.chart_overlaps(vg[f"c{i}_U"], vg[f"c{i}_F"].astype(np.int64))
+    assert np.array_equal(m, vg[f"c{i}_mask"])

@@ -421,3 +421,62 @@ def test_tuned_schedule_matches_default():
     for i in range(len(charts)):
         s = slice(vo[i], vo[i + 1])
         assert unit_box_err(outs[0][s], ref[s]) <= 1e-5
+
+
+def _atlas(charts, layouts, mirror=()):
+    """Islands in the cells of a k x k grid of the unit square (each scaled
+    into 0.9 of its cell, keeping its shape), as one mesh for validate_uvs;
+    islands in ``mirror`` are reflected."""
+    k = int(np.ceil(np.sqrt(len(charts))))
+    Ps, Ts, Us, off = [], [], [], 0
+    for i, ((P, T), U) in enumerate(zip(charts, layouts)):
+        lo = U.min(0)
+        u = (U - lo) / float((U.max(0) - lo).max()) * 0.9 / k
+        if i in mirror:
+            u[:, 0] = 0.9 / k - u[:, 0]
+        Ps.append(P + 100.0 * i)
+        Ts.append(np.asarray(T, np.int64) + off)
+        Us.append(u + np.array([i % k, i // k]) / k)
+        off += len(P)
+    return np.concatenate(Ps), np.concatenate(Ts), np.concatenate(Us)
+
+
+def test_validation_verdicts_match_oracle():
+    """The reference's validate.py verdicts (validate_uvs counters, the valid
+    flag and chart_overlaps) of the GPU layouts equal those of the oracle's
+    layouts, per island and for an atlas of all islands (one of them
+    mirrored), after LSCM and after 10 ARAP iterations (CG to 1e-6), on disk
+    islands and on islands with 10% needle triangles (aspect 50-500).  One
+    needle island keeps a flipped triangle and six overlapping ones after
+    ARAP in the oracle's layout too: the verdicts must agree on it as well."""
+    from oracle import param_cpu as oracle
+    from oracle import validate_cpu as vc
+    from ultimate_uv_artist_amd import _lib, workloads
+    from ultimate_uv_artist_amd.plan import IslandPlan
+
+    rng = np.random.default_rng(61)
+    charts = [workloads.disk_island(rng, t) for t in (300, 800, 2000, 4500, 9000)]
+    charts += workloads.sliver_islands(seed=7, n_islands=3, tmin=1500, tmax=4000)
+    plan = IslandPlan(charts)
+    U = plan.device_uv(np.zeros((plan.n, 2)))
+    plan.lscm(U)
+    Ul = U.cpu().numpy()
+    plan.arap(U, 10, line_search=True, mode=_lib.UVA_MODE_REFINE, tol=0.0, cg_tol=1e-6)
+    Ua = U.cpu().numpy()
+    assert plan.stats()["converged"].all()
+    vo = plan.vert_off
+    gl, ga, rl, ra = [], [], [], []
+    for i, (P, T) in enumerate(charts):
+        s = slice(vo[i], vo[i + 1])
+        url = oracle.lscm(P, T)
+        ura, _ = oracle.refine(P, T, url, iters=10, tol=0.0)
+        gl.append(Ul[s]), ga.append(Ua[s]), rl.append(url), ra.append(ura)
+        for g, r in ((Ul[s], url), (Ua[s], ura)):
+            vg, vr = vc.island_verdict(P, T, g), vc.island_verdict(P, T, r)
+            print(f"island {i} ({len(T)} triangles): {vg}", flush=True)
+            assert vg == vr
+    for g, r in ((gl, rl), (ga, ra)):
+        a = vc.validate_uvs(*_atlas(charts, g, mirror=(2,)))
+        b = vc.validate_uvs(*_atlas(charts, r, mirror=(2,)))
+        assert a[0] == b[0] and np.array_equal(a[1], b[1])
+        assert a[0]["islands"] == len(charts) and a[0]["mirrored_islands"] == 1

@@ -1,4 +1,5 @@
-"""Pins of the CPU oracle (oracle/param_cpu.py, oracle/batch_cpu.py).
+"""Pins of the CPU oracle (oracle/param_cpu.py, oracle/batch_cpu.py,
+oracle/validate_cpu.py).
 
 Each check is chosen so that a plausible restatement mistake (a dropped
 factor, a wrong sign, a swapped index or corner) fails it: closed forms,
@@ -340,3 +341,127 @@ def test_batch_orient_and_flip_count():
     b.orient_positive()
     assert b.flips_per_chart()[0] == 0
     assert np.allclose(b.U, P[:, :2])
+
+
+# ------------------------------------------------------------ validation checks
+def _clip_area(a, b):
+    """Area of the intersection of triangles a, b (3x2 arrays) by
+    Sutherland-Hodgman clipping of a against b's half-planes."""
+    if (b[1, 0] - b[0, 0]) * (b[2, 1] - b[0, 1]) - (b[2, 0] - b[0, 0]) * (b[1, 1] - b[0, 1]) < 0:
+        b = b[::-1]
+    poly = [tuple(p) for p in a]
+    for k in range(3):
+        p, q = b[k], b[(k + 1) % 3]
+        side = lambda x: (q[0] - p[0]) * (x[1] - p[1]) - (q[1] - p[1]) * (x[0] - p[0])
+        out = []
+        for i in range(len(poly)):
+            c, d = np.array(poly[i]), np.array(poly[(i + 1) % len(poly)])
+            sc, sd = side(c), side(d)
+            if sc >= 0:
+                out.append(tuple(c))
+            if (sc >= 0) != (sd >= 0):
+                out.append(tuple(c + (d - c) * (sc / (sc - sd))))
+        poly = out
+        if not poly:
+            return 0.0
+    x = np.array([p[0] for p in poly])
+    y = np.array([p[1] for p in poly])
+    return 0.5 * abs(np.dot(x, np.roll(y, -1)) - np.dot(y, np.roll(x, -1)))
+
+
+def test_triangle_overlap_equals_positive_intersection_area():
+    """On integer corners (orientation tolerances far below 1) and no shared
+    corner positions: a reported overlap always has positive intersection
+    area (computed independently by clipping), and in general position (no
+    corner on the line of an edge of the other triangle) the test is exactly
+    'positive intersection area'.  With corners on edges the reference's test
+    can miss an overlap, e.g. (6,3),(2,3),(6,6) against (4,3),(6,4),(6,2),
+    which share the area-1 triangle (4,3),(6,3),(6,4): no crossing is proper,
+    no corner or centroid is strictly inside."""
+    from oracle import validate_cpu as vc
+
+    rng = np.random.default_rng(41)
+    U = rng.integers(0, 7, (6 * 6000, 2)).astype(np.float64)
+    F = np.arange(len(U)).reshape(-1, 3)
+    u = U[F]
+    sa = (u[:, 1, 0] - u[:, 0, 0]) * (u[:, 2, 1] - u[:, 0, 1]) - (u[:, 2, 0] - u[:, 0, 0]) * (u[:, 1, 1] - u[:, 0, 1])
+    F = F[sa != 0]
+    s, t = 2 * np.arange(len(F) // 2), 2 * np.arange(len(F) // 2) + 1
+    Fp = F[: 2 * len(s)]
+    same = (U[Fp[s]][:, :, None, :] == U[Fp[t]][:, None, :, :]).all(-1).any((1, 2))
+    s, t = s[~same], t[~same]
+    got = vc.tri_overlap_pairs(U, Fp, s, t, 1e-6)
+    want = np.array([_clip_area(U[Fp[i]], U[Fp[j]]) > 1e-9 for i, j in zip(s, t)])
+    assert not (got & ~want).any()
+    ua, ub = U[Fp[s]], U[Fp[t]]
+
+    def on_line(x, y):             # any corner of y on the line of an edge of x
+        hit = np.zeros(len(x), bool)
+        for i in range(3):
+            a, b = x[:, i], x[:, (i + 1) % 3]
+            for j in range(3):
+                hit |= vc._orient(a[:, 0], a[:, 1], b[:, 0], b[:, 1], y[:, j, 0], y[:, j, 1]) == 0
+        return hit
+
+    gen = ~on_line(ua, ub) & ~on_line(ub, ua)
+    assert got[gen].sum() > 200 and (~got[gen]).sum() > 200
+    assert np.array_equal(got[gen], want[gen])
+    A = np.array([[6.0, 3], [2, 3], [6, 6], [4, 3], [6, 4], [6, 2]])
+    assert _clip_area(A[:3], A[3:]) == 1.0
+    assert not vc.tri_overlap_pairs(A, np.array([[0, 1, 2], [3, 4, 5]]), [0], [1], 1e-6)[0]
+    # symmetric in the pair and in the corner order of either triangle
+    assert np.array_equal(vc.tri_overlap_pairs(U, Fp, t, s, 1e-6), got)
+    Fr = Fp[:, [1, 2, 0]]
+    assert np.array_equal(vc.tri_overlap_pairs(U, Fr, s, t, 1e-6), got)
+
+
+def test_candidate_pairs_are_all_meeting_bounding_boxes():
+    from oracle import validate_cpu as vc
+
+    rng = np.random.default_rng(42)
+    U = rng.random((300, 2))
+    F = rng.integers(0, 300, (250, 3))
+    skip = rng.random(250) < 0.1
+    s, t = vc.candidate_pairs(U, F, skip)
+    lo, hi = U[F].min(1), U[F].max(1)
+    want = {(i, j) for i in range(250) for j in range(i + 1, 250)
+            if not skip[i] and not skip[j] and (lo[i] <= hi[j]).all() and (lo[j] <= hi[i]).all()}
+    got = set(zip(s.tolist(), t.tolist()))
+    assert got == want and len(got) == len(s)
+
+
+def test_chart_overlaps_invariant_under_similarity_and_reorder():
+    from oracle import validate_cpu as vc
+
+    rng = np.random.default_rng(43)
+    P, T = planar_disk(44, 300)
+    U = P[:, :2] + rng.normal(0, 0.03 * np.ptp(P[:, :2]), (len(P), 2))
+    m = vc.chart_overlaps(U, T)
+    assert m.any() and not m.all()
+    perm = rng.permutation(len(T))
+    assert np.array_equal(vc.chart_overlaps(U * 3.7 + np.array([5.0, -2.0]), T), m)
+    assert np.array_equal(vc.chart_overlaps(U, T[perm]), m[perm])
+    assert not vc.chart_overlaps(P[:, :2], T).any()          # a planar embedding
+
+
+def test_validate_orientation_and_island_counts():
+    from oracle import validate_cpu as vc
+
+    P, T = planar_disk(45, 200)
+    U = (P[:, :2] - P[:, :2].min(0)) / np.ptp(P[:, :2]).max() * 0.4 + 0.05
+    rep, _ = vc.validate_uvs(P, T, U)
+    assert rep["valid"] and rep["islands"] == 1 and rep["flipped_triangles"] == 0
+    Um = U * np.array([-1.0, 1.0]) + np.array([1.0, 0.0])
+    rep, _ = vc.validate_uvs(P, T, Um)
+    assert rep["mirrored_islands"] == 1 and rep["flipped_triangles"] == 0 and rep["valid"]
+    # a second copy of the chart beside the first: two islands; one triangle reversed
+    P2 = np.r_[P, P + 10.0]
+    T2 = np.r_[T, T[:, [0, 2, 1]] + len(P)]
+    T2[len(T)] = T2[len(T)][[0, 2, 1]]
+    U2 = np.r_[U, U + np.array([0.5, 0.0])]
+    rep, prob = vc.validate_uvs(P2, T2, U2)
+    assert rep["islands"] == 2 and rep["mirrored_islands"] == 1
+    # the second copy has reversed winding everywhere except the one triangle
+    assert rep["flipped_triangles"] == 1 and prob[len(T)] and not rep["valid"]
+    isl, n = vc.uv_islands(T2, len(P2), U2, P2)
+    assert n == 2 and len(set(isl[: len(T)])) == 1 and len(set(isl[len(T):])) == 1
