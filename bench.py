@@ -12,7 +12,8 @@ Mtri*iter/s.  The roofline figure uses the algorithmic byte model
     B_i = 124 t_i + 72 n_i + 8 nnz_i + k_i (100 n_i + 8 nnz_i)
 per island and ARAP iteration (k_i = CG iterations executed, nnz_i with the
 diagonal).  The CPU baseline is the float64 CPU restatement of the reference
-path (SuperLU) on a bounded sample of the same islands, one core.
+path (SuperLU) on a bounded sample of the same islands, over all host cores
+(worker processes, up to 16) and on one core.
 """
 from __future__ import annotations
 
@@ -48,6 +49,11 @@ def parse():
                     help="keep the size-based schedule (no re-planning from measured iterations)")
     ap.add_argument("--no-plain", dest="plain", action="store_false",
                     help="skip the plain per-island GPU baseline")
+    ap.add_argument("--workload", default="disks", choices=["disks", "heightfield", "needles"],
+                    help="disks: the throughput workload; heightfield: 3,988 islands of 50-1,946 triangles "
+                         "cut from a 2M-triangle heightfield (use --iters 15); needles: the precision study on "
+                         "256 islands of 5k-60k triangles with 10%% needle triangles (one JSON line with every "
+                         "precision variant)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB 5 times (HBM counter calibration)")
     return ap.parse_args()
@@ -101,6 +107,101 @@ def orient_majority(U, plan, vo, to, T_local):
     return U
 
 
+def _one_thread():
+    from threadpoolctl import threadpool_limits
+
+    threadpool_limits(1)
+
+
+def _oracle_refine(job):
+    """One island of the CPU baseline: the oracle's ARAP from the given
+    layout, timed inside the worker."""
+    from oracle import param_cpu as oracle
+
+    P, T, u0, iters = job
+    c0 = time.perf_counter()
+    ur, _ = oracle.refine(P, T, u0, iters=iters, tol=0.0)
+    return ur, time.perf_counter() - c0
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown CPU"
+
+
+def precision_study(args):
+    """The three precision variants of the ARAP global step on ill-conditioned
+    islands: CG iterations to the relative residual, time, and the UV error
+    against the float64 oracle (SuperLU) on a sample of islands."""
+    import torch
+
+    from oracle import param_cpu as oracle
+    from ultimate_uv_artist_amd import _lib, workloads
+    from ultimate_uv_artist_amd.plan import IslandPlan
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    t0 = time.perf_counter()
+    charts = workloads.sliver_islands(seed=3, n_islands=256)
+    tri = np.array([len(T) for _, T in charts])
+    print(f"bench: {len(charts)} needle islands, {tri.sum()} triangles (generated in "
+          f"{time.perf_counter() - t0:.1f}s)", flush=True)
+    plan = IslandPlan(charts, dev=dev)
+    U0 = torch.zeros(plan.n, 2, dtype=torch.float64, device=dev)
+    plan.lscm(U0)
+    torch.cuda.synchronize()
+    vo = plan.vert_off
+    order = np.argsort(tri)
+    sample = order[np.linspace(0, len(order) - 1, 8).round().astype(int)]
+    Uh0 = U0.cpu().numpy()
+    ref = {}
+    for i in sample:
+        P, T = charts[i]
+        ref[i], _ = oracle.refine(P, T, Uh0[vo[i]:vo[i + 1]], iters=args.iters, tol=0.0)
+    print(f"bench: oracle layouts of {len(sample)} sample islands done", flush=True)
+    out = {}
+    for name, prec in (("mixed", _lib.UVA_PREC_MIXED), ("f32", _lib.UVA_PREC_F32), ("f64", _lib.UVA_PREC_F64)):
+        U = U0.clone()
+
+        def step():
+            U.copy_(U0)
+            plan.arap(U, args.iters, line_search=True, mode=_lib.UVA_MODE_REFINE, tol=0.0, cg_tol=args.cg_tol,
+                      precision=prec, use_graph=not args.no_graph)
+
+        step()
+        torch.cuda.synchronize()
+        st = plan.stats()
+        t0 = time.perf_counter()
+        for _ in range(max(args.steps, 1)):
+            step()
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / max(args.steps, 1)
+        Uh = U.cpu().numpy()
+        err = []
+        for i in sample:
+            g, r = Uh[vo[i]:vo[i + 1]], ref[i]
+            lo = r.min(0)
+            err.append(float(np.abs(g - r).max() / (r.max(0) - lo).max()))
+        kpi = st["cg_iters"] / np.maximum(st["arap_iters"], 1)
+        out[name] = {"ms_per_step": round(ms, 2), "mtri_iter_per_s": round(float(tri.sum()) * args.iters / ms / 1e3, 2),
+                     "cg_iters_per_global_step_mean": round(float(kpi.mean()), 1),
+                     "cg_iters_per_global_step_p95": round(float(np.percentile(kpi, 95)), 1),
+                     "converged_islands": int(st["converged"].sum()),
+                     "max_rel_resid": float(st["rel_resid"].max()),
+                     "uv_err_unit_box_per_sample_island": [round(e, 9) for e in err]}
+        print(f"bench: {name}: {ms:.1f} ms/step, CG iters per global step mean {kpi.mean():.1f}, converged "
+              f"{int(st['converged'].sum())}/{len(charts)}, max UV err vs FP64 oracle {max(err):.2e}", flush=True)
+    print(json.dumps({"workload": f"{len(charts)} needle islands, t uniform [5000, 60000], 10% needles of aspect "
+                                  f"U(50, 500), LSCM + {args.iters} ARAP, CG rel resid {args.cg_tol:g}",
+                      "triangles": int(tri.sum()), "mesh_sha256": workloads.mesh_digest(charts)[:16],
+                      "sample_islands_triangles": [int(tri[i]) for i in sample], "variants": out}), flush=True)
+
+
 def main():
     args = parse()
     if args.copy_probe:
@@ -110,16 +211,33 @@ def main():
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (islands would shard with no "
               "exchange, see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "needles":
+        precision_study(args)
+        return
     import torch
 
     from ultimate_uv_artist_amd import _lib, workloads
     from ultimate_uv_artist_amd.plan import IslandPlan
 
     prec = {"mixed": _lib.UVA_PREC_MIXED, "f32": _lib.UVA_PREC_F32, "f64": _lib.UVA_PREC_F64}[args.precision]
+    pool, n_cores = None, 1
+    if args.cpu_sample > 0:
+        # worker processes for the all-core CPU baseline, forked before the GPU is touched
+        import multiprocessing as mp
+
+        n_cores = min(16, len(os.sched_getaffinity(0)))
+        pool = mp.get_context("fork").Pool(n_cores, initializer=_one_thread)
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
     t0 = time.perf_counter()
-    charts = workloads.disk_islands(seed=2, n_islands=args.islands)
+    if args.workload == "heightfield":
+        charts = workloads.heightfield_islands()
+        wl = (f"{len(charts)} heightfield islands (1001x1001 fBm grid, Voronoi cells), LSCM + {args.iters} ARAP, "
+              f"CG rel resid {args.cg_tol:g}")
+    else:
+        charts = workloads.disk_islands(seed=2, n_islands=args.islands)
+        wl = (f"{len(charts)} disk islands, t log-uniform [200,40000], LSCM + {args.iters} ARAP, "
+              f"CG rel resid {args.cg_tol:g}")
     t_gen = time.perf_counter() - t0
     tri = np.array([len(T) for _, T in charts])
     nv = np.array([len(P) for P, _ in charts])
@@ -272,13 +390,26 @@ def main():
             q, qr = oracle.chart_quality(P, T, g), oracle.chart_quality(P, T, ur)
             dq = max(dq, abs(q["aniso_p95"] / qr["aniso_p95"] - 1), abs(q["area_p95"] / qr["area_p95"] - 1))
         cpu_tri_iter = float(tri[pick].sum()) * args.iters
-        cpu = {"value": cpu_tri_iter / t_cpu / 1e6, "unit": "Mtri*iter/s", "cores": 1, "kind": "port",
+        # the same sample over all cores, islands spread over worker processes
+        # (largest first), as the reference spreads charts over a pool
+        jobs = sorted(pick, key=lambda i: -tri[i])
+        c0 = time.perf_counter()
+        res = pool.map(_oracle_refine, [(*charts[i], Uh0[vo[i]:vo[i + 1]], args.iters) for i in jobs], chunksize=1)
+        t_all = time.perf_counter() - c0
+        pool.close()
+        d_all = max(float(np.abs(r[0] - l[2]).max() / np.ptp(l[2], 0).max())
+                    for r, l in zip(res, sorted(lay, key=lambda x: -tri[x[0]])))
+        cpu = {"value": cpu_tri_iter / t_all / 1e6, "unit": "Mtri*iter/s", "cores": n_cores, "kind": "port",
+               "value_1core": cpu_tri_iter / t_cpu / 1e6,
                "sample": f"{len(pick)} islands at evenly spaced quantiles of the triangle mass "
                          f"({int(tri[pick].sum())} triangles), {args.iters} ARAP iterations each from the same "
-                         f"LSCM layout, SuperLU factorisation included, {t_cpu:.1f}s on one core"}
+                         f"LSCM layout, SuperLU factorisation included: {t_all:.1f}s over {n_cores} worker "
+                         f"processes, {t_cpu:.1f}s on one core; {cpu_model()}"}
         parity = {"islands": len(pick), "max_uv_err_unit_box": max(err), "verdicts_checked": nvs,
-                  "verdicts_identical": same, "max_rel_diff_p95_aniso_area": dq}
-        print(f"bench: CPU oracle sample {cpu['value']:.3f} Mtri*iter/s; parity max err {max(err):.2e}; "
+                  "verdicts_identical": same, "max_rel_diff_p95_aniso_area": dq,
+                  "cpu_workers_vs_serial_unit_box": d_all}
+        print(f"bench: CPU oracle sample {cpu['value']:.3f} Mtri*iter/s on {n_cores} cores, "
+              f"{cpu['value_1core']:.3f} on one; parity max err {max(err):.2e}; "
               f"validate verdicts identical on {same}/{nvs} islands; p95 anisotropy/area within {dq:.1e}",
               flush=True)
 
@@ -295,8 +426,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32 SpMV/vectors, f64 dots+residual" if prec == _lib.UVA_PREC_MIXED else args.precision,
         "data": "synthetic",
-        "config": {"workload": f"{len(charts)} disk islands, t log-uniform [200,40000], LSCM + {args.iters} ARAP, "
-                               f"CG rel resid {args.cg_tol:g}",
+        "config": {"workload": wl,
                    "islands": len(charts), "triangles": int(tri.sum()), "vertices": int(nv.sum()),
                    "arap_iters": args.iters, "mesh_sha256": workloads.mesh_digest(charts)[:16],
                    "parallelism": "single GPU"},

@@ -446,9 +446,10 @@ def test_validation_verdicts_match_oracle():
     flag and chart_overlaps) of the GPU layouts equal those of the oracle's
     layouts, per island and for an atlas of all islands (one of them
     mirrored), after LSCM and after 10 ARAP iterations (CG to 1e-6), on disk
-    islands and on islands with 10% needle triangles (aspect 50-500).  One
-    needle island keeps a flipped triangle and six overlapping ones after
-    ARAP in the oracle's layout too: the verdicts must agree on it as well."""
+    islands and on islands with 10% needle triangles (aspect 50-500).  The
+    needle islands keep one or two flipped and 5-12 overlapping triangles
+    after LSCM and after ARAP in the oracle's layouts too: the verdicts must
+    agree on them as well."""
     from oracle import param_cpu as oracle
     from oracle import validate_cpu as vc
     from ultimate_uv_artist_amd import _lib, workloads
