@@ -369,7 +369,7 @@ struct ResCG {
             if (relx <= t2 && rely <= t2) { out.conv = 1; break; }
             if (outer >= PT::kMaxOuter || it >= maxit || !__builtin_isfinite(relx) || !__builtin_isfinite(rely)) break;
             double gx = S1.z, gy = S1.w, dx = S2.x, dy = S2.y;
-            double alx = dx > 0.0 ? gx / dx : 0.0, aly = dy > 0.0 ? gy / dy : 0.0;
+            double alx = cg_alpha<PREC>(gx, dx), aly = cg_alpha<PREC>(gy, dy);
             double bex = 0.0, bey = 0.0;
             bool first = true;
             for (;;) {
@@ -422,8 +422,8 @@ struct ResCG {
                 bey = gy > 0.0 ? ngy / gy : 0.0;
                 const double denx = ndx - (alx != 0.0 ? bex * ngx / alx : 0.0);
                 const double deny = ndy - (aly != 0.0 ? bey * ngy / aly : 0.0);
-                alx = denx > 0.0 ? ngx / denx : 0.0;
-                aly = deny > 0.0 ? ngy / deny : 0.0;
+                alx = cg_alpha<PREC>(ngx, denx, ndx);
+                aly = cg_alpha<PREC>(ngy, deny, ndy);
                 gx = ngx;
                 gy = ngy;
             }

@@ -477,6 +477,25 @@ UVA_HD __forceinline__ double acc_mul(float a, float b) {
 template <int PREC>
 UVA_HD __forceinline__ double acc_mul(double a, double b) { return a * b; }
 
+// CG step length alpha = gamma / (p, A p).  The curvature (p, A p) may be
+// negative: the reference clips cotangent weights to +-1e4, and a clipped
+// triangle's quadratic form can be indefinite, which makes the whole operator
+// indefinite (a needle island with one clipped weight has an eigenvalue near
+// -38).  CG still converges on such systems as long as it keeps the signed
+// step, as the direct solve the reference uses would.  A curvature that the
+// vectors' precision cannot resolve (|den| below that precision times the
+// uncorrected term nd) stops the step instead: alpha = 0.
+template <int PREC>
+UVA_HD __forceinline__ double cg_alpha(double g, double den, double nd) {
+    constexpr double kRes = sizeof(typename Prec<PREC>::V) == 4 ? 1e-6 : 1e-13;
+    if (!(fabs(den) > kRes * fabs(nd)) || !__builtin_isfinite(den)) return 0.0;
+    return g / den;
+}
+template <int PREC>
+UVA_HD __forceinline__ double cg_alpha(double g, double den) {
+    return cg_alpha<PREC>(g, den, den);
+}
+
 #include "cg_island.hpp"
 #include "cg_resident.hpp"
 
