@@ -97,6 +97,27 @@ def test_emu_arap_refine_vs_oracle(disks, shape):
         assert unit_box_err(U[s], Ur) <= 1e-5
 
 
+@pytest.mark.timeout(120)
+@pytest.mark.parametrize("precision", [0, 1, 2])
+def test_emu_team_members_take_one_path(precision):
+    """A 65-slice island on a two-member team: the members hold 32 and 33
+    slices, on either side of the FP64 resident limit (4 rows per lane).
+    Every member must take the same CG path (their barrier sequences differ),
+    so the solve finishes and matches the oracle."""
+    from oracle import param_cpu as oracle
+    from ultimate_uv_artist_amd import workloads
+
+    P, T = workloads.disk_island(np.random.default_rng(7800), 7800)
+    assert -(-len(P) // 64) == 65
+    U0 = oracle.lscm(P, T)
+    e = EmuPlan([(P, T)], members=2, waves=8)
+    U = e.arap(U0, 2, tol=0.0, cg_tol=1e-8, precision=precision)
+    if precision != 1:
+        assert e.stats()["converged"].all()
+        Ur, _ = oracle.refine(P, T, U0, iters=2, tol=0.0)
+        assert unit_box_err(U, Ur) <= 1e-5
+
+
 def _shuffled(P, T, seed):
     """The same island with its vertices in random order (a matrix bandwidth
     close to the vertex count)."""

@@ -461,14 +461,25 @@ UVA_HD __forceinline__ CGOut cg_arap(Team& tm, const DevPlan& P, const Isl& d, d
     using V2 = typename Vec2<V>::type;
     // rows per lane the registers hold without spilling at 512 threads
     constexpr int kMaxRpt = sizeof(V) == 4 ? 6 : 4;
-    const int slo = tm.sl_lo(d.ns), shi = tm.sl_hi(d.ns);
-    const int nsl = shi - slo;
+    // Every member of a team must take the same path (resident or streaming,
+    // with or without the coarse level): the paths pass different sequences
+    // of team barriers.  The choice is made for the largest member share
+    // (members' shares differ by one slice), which every member computes alike.
+    const int m = tm.members();
+    int nsl = 0, na = 0;
+    for (int k = 0; k < m; ++k) {
+        const int lo = (int)((long long)d.ns * k / m), hi = (int)((long long)d.ns * (k + 1) / m);
+        nsl = hi - lo > nsl ? hi - lo : nsl;
+        if (d.cg > 0 && hi > lo) {
+            const int a = (hi - 1) / d.cg - lo / d.cg + 1;
+            na = a > na ? a : na;
+        }
+    }
     const int W = tm.nwaves();
     const int rpt = (nsl + W - 1) / W;
     const int vec = res_vec_bytes<V2, V>(nsl * kSlice, tm.multi() ? kHalo : 0);
     if (tm.lvec() != nullptr && vec <= tm.lvec_bytes() && rpt <= kMaxRpt) {
-        const int nc = (d.ns + d.cg - 1) / d.cg;
-        const int na = nsl > 0 ? (shi - 1) / d.cg - slo / d.cg + 1 : 0;
+        const int nc = d.cg > 0 ? (d.ns + d.cg - 1) / d.cg : 0;
         const bool co = coarse && d.cofs >= 0 && vec + res_coarse_bytes(nc, na) <= tm.lvec_bytes();
         if (rpt <= 2) return ResCG<PREC, 2, Team>(tm, P, d, co).solve(b2, tol, maxit);
         if (rpt <= 4) return ResCG<PREC, 4, Team>(tm, P, d, co).solve(b2, tol, maxit);

@@ -91,6 +91,7 @@ struct WaveTeam {
     __device__ int tid() const { return threadIdx.x & 63; }
     __device__ int tstride() const { return 64; }
     __device__ bool leader() const { return (threadIdx.x & 63) == 0; }
+    __device__ int members() const { return 1; }
     __device__ int sl_lo(int) const { return 0; }
     __device__ int sl_hi(int ns) const { return ns; }
     __device__ int tri_lo(int) const { return 0; }
@@ -138,6 +139,7 @@ struct BlockTeam {
     __device__ int tid() const { return threadIdx.x; }
     __device__ int tstride() const { return BS; }
     __device__ bool leader() const { return threadIdx.x == 0; }
+    __device__ int members() const { return 1; }
     __device__ int sl_lo(int) const { return 0; }
     __device__ int sl_hi(int ns) const { return ns; }
     __device__ int tri_lo(int) const { return 0; }
@@ -236,6 +238,7 @@ struct TeamWG {
     __device__ int tid() const { return threadIdx.x; }
     __device__ int tstride() const { return BS; }
     __device__ bool leader() const { return mem == 0 && threadIdx.x == 0; }
+    __device__ int members() const { return m; }
     __device__ int sl_lo(int ns) const { return (int)((long long)ns * mem / m); }
     __device__ int sl_hi(int ns) const { return (int)((long long)ns * (mem + 1) / m); }
     __device__ int tri_lo(int nt) const { return (int)((long long)nt * mem / m); }
