@@ -17,8 +17,9 @@
  *   uva_plan_create     gpu.py:69-142 Batch.__init__ / _build_laplacian,
  *                       param.py:19-56 local_frames, cot_weights, cot_laplacian
  *   uva_lscm            param.py:136-179 lscm (+ _far_pair)
- *   uva_arap            param.py:379-414 refine(method="arap") and
- *                       gpu.py:230-249 Batch.arap
+ *   uva_arap            param.py:379-414 refine(method="arap"),
+ *                       gpu.py:230-249 Batch.arap and the iterations of
+ *                       hier.py:394-509 Solver.relax (UVA_MODE_HIER)
  *   uva_slim            param.py:415-454 refine(method="slim") with
  *                       param.py:234-256 _arap_rotations (mode 1) and
  *                       param.py:341-376 _assemble_weighted / _slim_system
@@ -69,7 +70,12 @@ extern "C" {
                                exit, line search only if the start has no
                                flips, exact-root step rule */
 #define UVA_MODE_BATCH 1    /* Batch.arap: no early exit, line search when
-                               asked, non-finite islands keep their layout */
+                               asked, non-finite islands keep their layout,
+                               Jacobi-preconditioned CG */
+#define UVA_MODE_HIER 2     /* hier.Solver.relax (hier.py:498-509): Batch
+                               conventions, and each iteration a chart with a
+                               folded usable triangle takes the plain step;
+                               only usable triangles limit the others' step */
 
 /* precision of the CG global step */
 #define UVA_PREC_MIXED 0    /* FP32 SpMV/vectors, FP64 dots, FP64 true residual */

@@ -20,9 +20,12 @@ Only behaviour that the oracle's own checks cannot pin is recorded:
     the backtracking ones) on two disk islands.
   * validate_uvs (without the gutter) and chart_overlaps verdicts of
     hand-built contacts and real layouts (engine/validate.py, with
-    engine/metrics.py loaded as its package sibling).
-Output: tests/golden/param_golden.npz, batch_golden.npz, slim_golden.npz and
-validate_golden.npz (``--only slim`` / ``--only validate`` writes one alone).
+    engine/metrics.py loaded as its package sibling),
+  * hier.Solver.relax and bad_fraction on two subsets of a four-island mesh
+    (engine/hier.py with engine/gpu.py, CPU torch).
+Output: tests/golden/param_golden.npz, batch_golden.npz, slim_golden.npz,
+validate_golden.npz and hier_golden.npz (``--only slim|validate|hier``
+writes one alone).
 """
 from __future__ import annotations
 
@@ -291,6 +294,83 @@ def validate_fixtures(root, rp):
     print("validate_golden.npz", (HERE / "validate_golden.npz").stat().st_size, "bytes")
 
 
+def load_hier(root):
+    """engine/hier.py with engine/gpu.py as its package sibling (the numba
+    stand-in of load_reference; the Solver runs on CPU torch)."""
+    load_validate(root)
+    pkg = sys.modules["ref_engine"]
+    for name in ("gpu", "hier"):
+        spec = importlib.util.spec_from_file_location(f"ref_engine.{name}", Path(root) / "engine" / f"{name}.py")
+        m = importlib.util.module_from_spec(spec)
+        sys.modules[f"ref_engine.{name}"] = m
+        spec.loader.exec_module(m)
+        setattr(pkg, name, m)
+    return sys.modules["ref_engine.hier"]
+
+
+def hier_cases(rp):
+    """A mesh of four disk islands and two subsets for Solver.relax: three
+    charts with shuffled faces, shuffled local vertex ids and permuted chart
+    ids (one chart mirrored, one folded by noise) at the default settings, and
+    one chart with other iteration counts."""
+    from ultimate_uv_artist_amd import workloads
+
+    rng = np.random.default_rng(600)
+    isl = [workloads.disk_island(rng, t) for t in (300, 800, 1500, 500)]
+    voff = np.cumsum([0] + [len(P) for P, _ in isl])
+    V = np.concatenate([P + np.array([3.0 * k, 0.0, 0.0]) for k, (P, _) in enumerate(isl)])
+    F = np.concatenate([T + voff[k] for k, (_, T) in enumerate(isl)])
+    foff = np.cumsum([0] + [len(T) for _, T in isl])
+    U_isl = []
+    for k, (P, T) in enumerate(isl):
+        U = rp.lscm(P, T)
+        if k == 1:
+            U[:, 0] = -U[:, 0]
+        if k == 2:
+            U = U + rng.normal(0, 0.02 * np.ptp(U), U.shape)
+        U_isl.append(U)
+    cases = []
+    for members, chart_of, kw in (([0, 1, 2], {0: 2, 1: 0, 2: 1}, {}),
+                                  ([3], {3: 0}, {"iters": 3, "cg_iters": 10, "repair_rounds": 5})):
+        faces = np.concatenate([np.arange(foff[k], foff[k + 1]) for k in members])
+        faces = faces[rng.permutation(len(faces))]
+        gverts = np.unique(F[faces])
+        nv = len(gverts)
+        lid = np.empty(len(V), np.int64)
+        lid[gverts] = rng.permutation(nv)
+        CV = lid[F[faces]]
+        owner = np.searchsorted(foff, faces, side="right") - 1
+        chart = np.array([chart_of[o] for o in owner])
+        U0 = np.zeros((nv, 2))
+        for k in members:
+            U0[lid[voff[k] + np.arange(len(isl[k][0]))]] = U_isl[k]
+        cases.append(dict(faces=faces, CV=CV, nv=nv, chart=chart, nchart=len(members), U0=U0, **kw))
+    return V, F, cases
+
+
+def hier_fixtures(root, rp):
+    """hier.Solver.relax and bad_fraction on the hier_cases subsets, run by
+    the reference on CPU torch.  Output: tests/golden/hier_golden.npz."""
+    import types as _t
+
+    rh = load_hier(root)
+    V, F, cases = hier_cases(rp)
+    S = rh.Solver(_t.SimpleNamespace(V=V, F=F))
+    out = {"V": V, "F": F.astype(np.int32), "n": np.array(len(cases))}
+    for i, c in enumerate(cases):
+        kw = {k: c[k] for k in ("iters", "cg_iters", "repair_rounds") if k in c}
+        U, flips, scale = S.relax(c["faces"], c["CV"], c["nv"], c["chart"], c["nchart"], c["U0"], **kw)
+        bA, bR = S.bad_fraction(2.0, 1.5)
+        for k in ("faces", "CV", "chart", "U0"):
+            out[f"c{i}_{k}"] = c[k]
+        out[f"c{i}_nv"], out[f"c{i}_nchart"] = np.array(c["nv"]), np.array(c["nchart"])
+        out[f"c{i}_kw"] = np.array([kw.get("iters", 6), kw.get("cg_iters", 20), kw.get("repair_rounds", 24)])
+        out[f"c{i}_U"], out[f"c{i}_flips"], out[f"c{i}_scale"] = U, flips, scale
+        out[f"c{i}_badA"], out[f"c{i}_badR"] = bA, bR
+    np.savez_compressed(HERE / "hier_golden.npz", **out)
+    print("hier_golden.npz", (HERE / "hier_golden.npz").stat().st_size, "bytes")
+
+
 if __name__ == "__main__":
     only = sys.argv[sys.argv.index("--only") + 1] if "--only" in sys.argv else None
     args = [a for a in sys.argv[1:] if a != "--only" and a != only]
@@ -299,6 +379,9 @@ if __name__ == "__main__":
         slim_fixtures(load_reference(root)[0])
     elif only == "validate":
         validate_fixtures(root, load_reference(root)[0])
+    elif only == "hier":
+        hier_fixtures(root, load_reference(root)[0])
     else:
         main(root)
         validate_fixtures(root, load_reference(root)[0])
+        hier_fixtures(root, load_reference(root)[0])

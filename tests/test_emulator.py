@@ -297,3 +297,27 @@ def test_emu_global_step_backward(disks):
     h = 1e-3
     fd = ((emu_steps(U0 + h * D) - emu_steps(U0 - h * D)) * gbar).sum() / (2 * h)
     assert abs(fd - (g0 * D).sum()) <= 1e-5 * abs(fd)
+
+
+def test_emu_hier_iterations_vs_oracle():
+    """UVA_MODE_HIER (hier.Solver.relax's iterations: Jacobi-PCG of exactly
+    cg_iters iterations, plain step for folded charts, flip-safe step over
+    usable faces otherwise) against the restatement of hier.py with the
+    repairs switched off, on one chart with a folded start."""
+    from oracle import hier_cpu
+
+    rng = np.random.default_rng(61)
+    from oracle import param_cpu as oracle
+    from ultimate_uv_artist_amd import workloads
+
+    P, T = workloads.disk_island(rng, 900)
+    U0 = oracle.lscm(P, T)
+    U0 = U0 + rng.normal(0, 0.01 * np.ptp(U0), U0.shape)
+    assert (oracle.signed_areas(U0, T) < 0).any()
+    S = hier_cpu.Solver(P, T)
+    Ur, _, _ = S.relax(np.arange(len(T)), T, len(P), np.zeros(len(T), np.int64), 1, U0, iters=4, cg_iters=20,
+                       repair_rounds=0)
+    e = EmuPlan([(P, T)], batch=True)
+    U = e.arap(U0.astype(np.float32).astype(np.float64), 4, line_search=True, tol=0.0, cg_tol=0.0,
+               cg_max_iters=20, mode=2)
+    assert unit_box_err(U, Ur) <= 1e-5

@@ -123,3 +123,41 @@ def test_chart_overlaps(vg, i):
 
     m = vc.chart_overlaps(vg[f"c{i}_U"], vg[f"c{i}_F"].astype(np.int64))
     assert np.array_equal(m, vg[f"c{i}_mask"])
+
+
+def _hier_case(h, i):
+    kw = dict(zip(("iters", "cg_iters", "repair_rounds"), (int(x) for x in h[f"c{i}_kw"])))
+    args = (h[f"c{i}_faces"], h[f"c{i}_CV"], int(h[f"c{i}_nv"]), h[f"c{i}_chart"], int(h[f"c{i}_nchart"]),
+            h[f"c{i}_U0"])
+    return args, kw
+
+
+def _chart_unit_err(U, Ur, CV, chart, nchart):
+    cv = np.zeros(len(U), np.int64)
+    cv[CV.reshape(-1)] = np.repeat(chart, 3)
+    err = 0.0
+    for c in range(nchart):
+        m = cv == c
+        lo = Ur[m].min(0)
+        err = max(err, float(np.abs(U[m] - Ur[m]).max() / (Ur[m].max(0) - lo).max()))
+    return err
+
+
+@pytest.mark.parametrize("i", [0, 1])
+def test_hier_relax(i):
+    """hier.Solver.relax and bad_fraction: the float64 restatement against
+    the reference's float32 run (three charts with shuffled faces, vertex
+    ids and chart ids, one mirrored and one folded start; one chart with
+    other iteration counts)."""
+    from oracle import hier_cpu
+
+    h = dict(np.load(GOLD / "hier_golden.npz"))
+    S = hier_cpu.Solver(h["V"], h["F"].astype(np.int64))
+    args, kw = _hier_case(h, i)
+    U, flips, scale = S.relax(*args, **kw)
+    assert _chart_unit_err(U, h[f"c{i}_U"], args[1], args[3], args[4]) <= 2e-5
+    assert np.array_equal(flips, h[f"c{i}_flips"])
+    assert np.abs(scale / h[f"c{i}_scale"] - 1).max() <= 1e-5
+    bA, bR = S.bad_fraction(2.0, 1.5)
+    assert np.abs(bA - h[f"c{i}_badA"]).max() <= 1e-6 and np.abs(bR - h[f"c{i}_badR"]).max() <= 1e-6
+

@@ -481,3 +481,37 @@ def test_validation_verdicts_match_oracle():
         b = vc.validate_uvs(*_atlas(charts, r, mirror=(2,)))
         assert a[0] == b[0] and np.array_equal(a[1], b[1])
         assert a[0]["islands"] == len(charts) and a[0]["mirrored_islands"] == 1
+
+
+@pytest.mark.parametrize("i", [0, 1])
+def test_hier_solver_matches_reference(i):
+    """hier.Solver.relax / bad_fraction on the GPU against the reference's own
+    run (tests/golden/hier_golden.npz) and the float64 restatement: layouts
+    within 1e-4 per chart (unit box), identical flip counts, scale and bad
+    fractions to float32 rounding."""
+    import types
+    from pathlib import Path
+
+    from oracle import hier_cpu
+    from ultimate_uv_artist_amd import hier
+
+    h = dict(np.load(Path(__file__).resolve().parent / "golden" / "hier_golden.npz"))
+    V, F = h["V"], h["F"].astype(np.int64)
+    kw = dict(zip(("iters", "cg_iters", "repair_rounds"), (int(x) for x in h[f"c{i}_kw"])))
+    args = (h[f"c{i}_faces"], h[f"c{i}_CV"], int(h[f"c{i}_nv"]), h[f"c{i}_chart"], int(h[f"c{i}_nchart"]),
+            h[f"c{i}_U0"])
+    S = hier.Solver(types.SimpleNamespace(V=V, F=F))
+    U, flips, scale = S.relax(*args, **kw)
+    bA, bR = S.bad_fraction(2.0, 1.5)
+    O = hier_cpu.Solver(V, F)
+    Uo, flo, sco = O.relax(*args, **kw)
+    cv = np.zeros(len(U), np.int64)
+    cv[args[1].reshape(-1)] = np.repeat(args[3], 3)
+    for ref in (h[f"c{i}_U"], Uo):
+        for c in range(args[4]):
+            m = cv == c
+            assert unit_box_err(U[m], ref[m]) <= 1e-4
+    assert np.array_equal(flips, h[f"c{i}_flips"]) and np.array_equal(flips, flo)
+    assert np.abs(scale / h[f"c{i}_scale"] - 1).max() <= 1e-5
+    assert np.abs(bA - h[f"c{i}_badA"]).max() <= 1e-4 and np.abs(bR - h[f"c{i}_badR"]).max() <= 1e-4
+    assert U.dtype == np.float64 and np.array_equal(U, U.astype(np.float32).astype(np.float64))

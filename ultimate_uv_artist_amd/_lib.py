@@ -27,6 +27,7 @@ UVA_PLAN_KEEP_ORDER = 8
 UVA_PLAN_NO_COARSE = 16
 UVA_MODE_REFINE = 0
 UVA_MODE_BATCH = 1
+UVA_MODE_HIER = 2
 UVA_PREC_MIXED = 0
 UVA_PREC_F32 = 1
 UVA_PREC_F64 = 2

@@ -136,7 +136,8 @@ int arap_sequence(uva_plan* pl, double* Uext, const uva_arap_opts* o, hipStream_
     a.cg_max = o->cg_max_iters > 0 ? o->cg_max_iters : kDefaultMaxIters;
     a.tol = o->tol;
     a.line_search = o->line_search;
-    a.batch = o->mode == UVA_MODE_BATCH;
+    a.batch = o->mode == UVA_MODE_BATCH || o->mode == UVA_MODE_HIER;
+    a.hier = o->mode == UVA_MODE_HIER;
     int total = 1 + o->iters;
     int done = 0;
     while (done < total) {
@@ -554,7 +555,8 @@ int uva_arap(uva_plan* plan, double* U, const uva_arap_opts* o, void* stream) {
     if (o->iters < 0) return fail(UVA_EINVAL, "iters must be >= 0");
     if (!(o->cg_tol > 0.0)) return fail(UVA_EINVAL, "cg_tol must be positive");
     if (o->precision < UVA_PREC_MIXED || o->precision > UVA_PREC_F64) return fail(UVA_EINVAL, "bad precision");
-    if (o->mode != UVA_MODE_REFINE && o->mode != UVA_MODE_BATCH) return fail(UVA_EINVAL, "bad mode");
+    if (o->mode != UVA_MODE_REFINE && o->mode != UVA_MODE_BATCH && o->mode != UVA_MODE_HIER)
+        return fail(UVA_EINVAL, "bad mode");
     hipStream_t s = (hipStream_t)stream;
     if (!o->use_graph) return arap_sequence(plan, U, o, s);
     // graph of the whole sequence, cached per (U, options)
@@ -585,7 +587,8 @@ int uva_arap_profile(uva_plan* plan, double* U, const uva_arap_opts* o, double* 
     if (o->iters < 0 || n_launch != 1 + o->iters) return fail(UVA_EINVAL, "n_launch must be iters + 1");
     if (!(o->cg_tol > 0.0)) return fail(UVA_EINVAL, "cg_tol must be positive");
     if (o->precision < UVA_PREC_MIXED || o->precision > UVA_PREC_F64) return fail(UVA_EINVAL, "bad precision");
-    if (o->mode != UVA_MODE_REFINE && o->mode != UVA_MODE_BATCH) return fail(UVA_EINVAL, "bad mode");
+    if (o->mode != UVA_MODE_REFINE && o->mode != UVA_MODE_BATCH && o->mode != UVA_MODE_HIER)
+        return fail(UVA_EINVAL, "bad mode");
     hipStream_t s = (hipStream_t)stream;
     std::vector<hipEvent_t> ev(2 * (size_t)n_launch, nullptr);
     int rc = UVA_OK;

@@ -600,7 +600,7 @@ UVA_HD __forceinline__ double2 rhs_step(Team& tm, const DevPlan& P, const Isl& d
 // gpu.py:204-227 for batch), minimum over the island's triangles.
 template <class Team>
 UVA_HD __forceinline__ double max_step_island(Team& tm, const DevPlan& P, const Isl& d, const double2* __restrict__ U,
-                                  const double2* __restrict__ Xn, bool batch) {
+                                  const double2* __restrict__ Xn, bool batch, bool usable_only = false) {
     double best = 1e30;
     FOR_TRIS(tm, d, f) {
         const int a = P.tv0[f], b = P.tv1[f], c = P.tv2[f];
@@ -638,7 +638,8 @@ UVA_HD __forceinline__ double max_step_island(Team& tm, const DevPlan& P, const 
             const double r1 = (-B - sq) / A2, r2 = (-B + sq) / A2;
             double quad = fmin(r1 > 0.0 ? r1 : big, r2 > 0.0 ? r2 : big);
             quad = (disc >= 0.0 && fabs(A) >= 1e-30) ? quad : big;
-            const double t = C > 0.0 ? fmin(lin, quad) : big;
+            // hier.py:494 also leaves out triangles without a usable frame
+            const double t = (C > 0.0 && (!usable_only || P.fok[f])) ? fmin(lin, quad) : big;
             best = fmin(best, t);
         }
     }
@@ -684,7 +685,10 @@ UVA_HD __forceinline__ void run_task(Team& tm, const DevPlan& P, int isl, const 
         if (uniform_i(st->done)) return;
         const long long t_start = uva_clock();
         const double2 b2 = rhs_step(tm, P, d, U);
-        const CGOut cg = cg_arap<PREC>(tm, P, d, b2, a.cg_tol, a.cg_max, true);
+        // Batch and hier solves cap the CG iterations (40 and 20 by default), so
+        // their result depends on the preconditioner: they keep the
+        // reference's Jacobi one (gpu.py:183-201, hier.py:458-467)
+        const CGOut cg = cg_arap<PREC>(tm, P, d, b2, a.cg_tol, a.cg_max, !a.batch);
         if (a.batch) {
             // islands whose solve went non-finite keep their layout (gpu.py:235-240)
             double bad = 0.0;
@@ -701,10 +705,17 @@ UVA_HD __forceinline__ void run_task(Team& tm, const DevPlan& P, int isl, const 
                 tm.sync();
             }
         }
-        const int use_ls = uniform_i(st->use_ls);
+        int use_ls = uniform_i(st->use_ls);
+        if (a.hier && use_ls) {
+            // hier.py:500-505: a chart with a folded usable triangle takes the
+            // plain ARAP step this iteration
+            double nf = 0.0;
+            FOR_TRIS(tm, d, f) nf += (P.fok[f] && signed_area(P, f, U) < 0.0) ? 1.0 : 0.0;
+            if (tm.sum4(make_double4(nf, 0.0, 0.0, 0.0)).x > 0.0) use_ls = 0;
+        }
         double step = 1.0, tmax = 1e30;
         if (use_ls) {
-            tmax = max_step_island(tm, P, d, U, P.X64, a.batch);
+            tmax = max_step_island(tm, P, d, U, P.X64, a.batch, a.hier != 0);
             step = fmin(1.0, 0.9 * tmax);
         }
         FOR_ROWS(tm, d, row, sg) {

@@ -49,7 +49,8 @@ struct TaskArgs {
     double tol;              // energy early exit
     int cg_max;
     int line_search;
-    int batch;
+    int batch;               // Batch conventions (gpu.py): batch-global pin weight, non-finite revert
+    int hier;                // hier.Solver.relax step rule (hier.py:498-509)
     int kind;                // energy kind
     int rounds;              // repair rounds
     const double* w;         // per-triangle energy weights (NULL = 1)

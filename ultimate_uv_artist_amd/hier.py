@@ -1,0 +1,167 @@
+"""hier.Solver on the GPU: ARAP relaxation of any subset of a mesh's charts
+(hier.py:357-593 of the reference), with the same call signatures.
+
+``relax`` regroups the subset into islands (each chart's vertices in
+ascending local id, so the chart's soft pin -- its smallest vertex id,
+hier.py:407-409 -- is the island's first vertex), builds a Batch-convention
+plan and runs the reference's sequence on the GPU kernels:
+
+  orient every chart by area-weighted sign         uva_orient_positive
+  Jacobi flip repair (``repair_rounds``)            uva_repair_jacobi
+  ``iters`` ARAP iterations, each a Jacobi-PCG      uva_arap(UVA_MODE_HIER)
+  of exactly ``cg_iters`` iterations from the
+  current layout, non-finite charts reverted, a
+  folded chart taking the plain step and the
+  others 0.9 of the largest fold-free step
+  (usable triangles only), with a repair after
+  iteration ``iters // 2``
+  final repair, per-chart quality                   torch on the plan's GPU
+
+Every vertex must belong to a face of the subset (the reference's callers
+pass compacted subsets).  ``lscm`` (the free-boundary soft-pin LSCM that the
+reference keeps behind ``part_lscm=False``) is not part of this build.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import _lib
+from .plan import IslandPlan, require_gpu
+
+
+class Solver:
+    """Frames come from the mesh's positions ``topo.V`` and faces ``topo.F``
+    (hier.py:361-389); the GPU plan of a subset is built per ``relax`` call."""
+
+    def __init__(self, topo, dev=None):
+        self.dev = require_gpu(dev)
+        self.V = np.asarray(topo.V, np.float64)
+        self.F = np.asarray(topo.F, np.int64)
+        self._q = None
+
+    def _charts(self, faces, CV, nv, chart, nchart):
+        """Islands of the subset in chart order: per chart its vertices in
+        ascending local id and its faces in subset order."""
+        if CV.min(initial=0) < 0 or CV.max(initial=-1) >= nv:
+            raise ValueError("CV holds vertex ids outside [0, nv)")
+        if len(chart) and (chart.min() < 0 or chart.max() >= nchart):
+            raise ValueError("chart ids outside [0, nchart)")
+        cv = np.full(nv, -1, np.int64)
+        cv[CV.reshape(-1)] = np.repeat(chart, 3)
+        if (cv < 0).any():
+            raise ValueError("every vertex of the subset must belong to one of its faces")
+        pos = np.zeros((nv, 3))
+        pos[CV.reshape(-1)] = self.V[self.F[faces]].reshape(-1, 3)
+        vorder = np.argsort(cv, kind="stable")            # chart-major, ascending id inside a chart
+        local = np.empty(nv, np.int64)
+        vcount = np.bincount(cv, minlength=nchart)
+        voff = np.concatenate([[0], np.cumsum(vcount)])
+        local[vorder] = np.arange(nv) - np.repeat(voff[:-1], vcount)
+        forder = np.argsort(chart, kind="stable")
+        fcount = np.bincount(chart, minlength=nchart)
+        foff = np.concatenate([[0], np.cumsum(fcount)])
+        charts, keep = [], []
+        for c in range(nchart):
+            if fcount[c] == 0:
+                continue
+            fs = forder[foff[c]:foff[c + 1]]
+            charts.append((pos[vorder[voff[c]:voff[c + 1]]], local[CV[fs]]))
+            keep.append(c)
+        return charts, np.array(keep, np.int64), vorder, forder
+
+    def relax(self, faces, CV, nv, chart, nchart, U0, iters=6, cg_iters=20, repair_rounds=24):
+        """faces (n,) global face ids, CV (n,3) local vertex ids, chart (n,)
+        local chart ids, U0 (nv,2) start layout.  Returns the layout (nv,2)
+        float64 and per-chart flips and UV/3-D area scale (hier.py:394-530)."""
+        faces = np.asarray(faces, np.int64).reshape(-1)
+        CV = np.asarray(CV, np.int64).reshape(-1, 3)
+        chart = np.asarray(chart, np.int64).reshape(-1)
+        nv, nchart = int(nv), int(nchart)
+        U0 = np.asarray(U0, np.float64).reshape(nv, 2)
+        charts, keep, vorder, forder = self._charts(faces, CV, nv, chart, nchart)
+        flips = np.zeros(nchart)
+        scale = np.zeros(nchart)
+        if not charts:
+            self._q = None
+            return U0.astype(np.float32).astype(np.float64), flips, scale
+        plan = IslandPlan(charts, batch=True, dev=self.dev)
+        # the reference holds the layout in float32 (hier.py:400)
+        U = plan.device_uv(U0[vorder].astype(np.float32).astype(np.float64))
+        plan.orient_positive(U)
+        plan.repair_jacobi(U, repair_rounds)
+        first = min(iters, iters // 2 + 1)
+        if first > 0:
+            plan.arap(U, first, line_search=True, mode=_lib.UVA_MODE_HIER, tol=0.0, cg_tol=0.0,
+                      cg_max_iters=int(cg_iters))
+            plan.repair_jacobi(U, repair_rounds)            # after iteration iters // 2
+        if iters - first > 0:
+            plan.arap(U, iters - first, line_search=True, mode=_lib.UVA_MODE_HIER, tol=0.0, cg_tol=0.0,
+                      cg_max_iters=int(cg_iters))
+        plan.repair_jacobi(U, repair_rounds)
+        Pall = np.concatenate([c[0] for c in charts])
+        U.copy_(U.float().double())                          # the reference returns float32 values
+        q = self._quality(plan, Pall, U, len(charts))
+        out = np.empty((nv, 2))
+        out[vorder] = U.cpu().numpy()
+        flips[keep] = q["flips"]
+        scale[keep] = q["scale"]
+        self._q = (q, keep, nchart)
+        return out, flips, scale
+
+    def _quality(self, plan, Pall, U, nisl):
+        """hier.py:511-530 per face and per chart, float64 on the plan's GPU."""
+        d = self.dev
+        P = torch.as_tensor(Pall, device=d)
+        T = torch.as_tensor(plan.T_local + np.repeat(plan.vert_off[:-1], np.diff(plan.tri_off))[:, None], device=d)
+        p0, p1, p2 = P[T[:, 0]], P[T[:, 1]], P[T[:, 2]]
+        e1, e2 = p1 - p0, p2 - p0
+        l1 = e1.norm(dim=1)
+        nrm = torch.cross(e1, e2, dim=1)
+        dbl = nrm.norm(dim=1)
+        ok = (dbl > 1e-14 * (l1 * l1).clamp_min(1e-300)) & torch.isfinite(dbl)
+        ex = e1 / l1.clamp_min(1e-30)[:, None]
+        ey = torch.cross(nrm / dbl.clamp_min(1e-30)[:, None], ex, dim=1)
+        x1 = torch.where(ok, l1, torch.zeros_like(l1)).clamp_min(1e-30)
+        x2 = torch.where(ok, (e2 * ex).sum(1), torch.zeros_like(l1))
+        y2 = torch.where(ok, (e2 * ey).sum(1), torch.zeros_like(l1))
+        y2 = torch.where(y2.abs() > 1e-30, y2, torch.full_like(y2, 1e-30))
+        a0, a1, a2 = U[T[:, 0]], U[T[:, 1]], U[T[:, 2]]
+        j00 = (a1[:, 0] - a0[:, 0]) / x1
+        j10 = (a1[:, 1] - a0[:, 1]) / x1
+        j01 = ((a2[:, 0] - a0[:, 0]) - j00 * x2) / y2
+        j11 = ((a2[:, 1] - a0[:, 1]) - j10 * x2) / y2
+        E, Fq, G, H = (j00 + j11) / 2, (j00 - j11) / 2, (j10 + j01) / 2, (j10 - j01) / 2
+        Q, R = (E * E + H * H).sqrt(), (Fq * Fq + G * G).sqrt()
+        s1, s2 = Q + R, (Q - R).abs()
+        ar3 = 0.5 * dbl * ok
+        sgn = 0.5 * ((a1[:, 0] - a0[:, 0]) * (a2[:, 1] - a0[:, 1]) - (a2[:, 0] - a0[:, 0]) * (a1[:, 1] - a0[:, 1]))
+        ct = torch.as_tensor(np.repeat(np.arange(nisl), np.diff(plan.tri_off)), device=d)
+        uva = torch.zeros(nisl, dtype=torch.float64, device=d).index_add_(0, ct, sgn.abs() * ok)
+        a3 = torch.zeros(nisl, dtype=torch.float64, device=d).index_add_(0, ct, ar3)
+        scale = uva / a3.clamp_min(1e-30)
+        flips = torch.zeros(nisl, dtype=torch.float64, device=d).index_add_(0, ct, ((sgn < 0) & ok).double())
+        return {"flips": flips.cpu().numpy(), "scale": scale.cpu().numpy(), "aniso": s1 / s2.clamp_min(1e-12),
+                "arat": (s1 * s2) / scale[ct].clamp_min(1e-30), "ar3": ar3, "ct": ct, "a3": a3}
+
+    def bad_fraction(self, max_aniso, max_area):
+        """Per chart of the last ``relax``: area fraction of triangles with
+        anisotropy above ``max_aniso`` and with area ratio outside
+        [1/max_area, max_area] (hier.py:584-590)."""
+        if self._q is None:
+            raise ValueError("bad_fraction needs a relax call first")
+        q, keep, nchart = self._q
+        n = len(keep)
+        d = q["ar3"].device
+        badA = torch.zeros(n, dtype=torch.float64, device=d).index_add_(0, q["ct"], q["ar3"] * (q["aniso"] > max_aniso))
+        badR = torch.zeros(n, dtype=torch.float64, device=d).index_add_(
+            0, q["ct"], q["ar3"] * ((q["arat"] > max_area) | (q["arat"] < 1.0 / max_area)))
+        a3 = q["a3"].clamp_min(1e-30)
+        outA, outR = np.zeros(nchart), np.zeros(nchart)
+        outA[keep] = (badA / a3).cpu().numpy()
+        outR[keep] = (badR / a3).cpu().numpy()
+        return outA, outR
+
+    def lscm(self, *args, **kwargs):
+        raise NotImplementedError("hier.Solver.lscm (soft-pin free-boundary LSCM, off by default in the reference) "
+                                  "is not part of this build; use param.lscm")
