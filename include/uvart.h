@@ -116,7 +116,9 @@ typedef struct {
     int32_t mode;            /* UVA_MODE_* */
     int32_t precision;       /* UVA_PREC_* */
     double tol;              /* energy early-exit tolerance (REFINE) */
-    double cg_tol;           /* relative residual per global step */
+    double cg_tol;           /* relative residual per global step; 0 runs
+                                exactly cg_max_iters CG iterations (the
+                                fixed budgets of Batch and hier.Solver) */
     int32_t cg_max_iters;    /* cap per global step (Batch's cg_iters) */
     int32_t use_graph;       /* capture the ARAP iterations in a hipGraph */
 } uva_arap_opts;
@@ -150,6 +152,13 @@ int uva_plan_get_stats(uva_plan *plan, uva_island_stats *stats, void *stream);
  * Later calls run the new schedule; results are unchanged up to the order of
  * FP64 reductions across team members.  No counterpart in the reference. */
 int uva_plan_tune(uva_plan *plan, void *stream);
+/* Soft pins of weight w (w = 0: the island's own pin weight, max |W| of the
+ * island) at the given vertices (caller ids) in place of each island's own
+ * pin, for a plan with Batch conventions whose single island holds several
+ * charts as one block-diagonal system: hier.py:407-412 and gpu.py:133-141 pin
+ * every chart of a batch that one CG solves together.  Turns the coarse level
+ * off (Jacobi-preconditioned CG). */
+int uva_plan_set_pins(uva_plan *plan, const int64_t *pins, int64_t n_pins, double w, void *stream);
 /* Set the per-triangle target scale (gpu.py:123 Batch.scale); device array. */
 int uva_plan_set_scale(uva_plan *plan, const double *scale_dev, void *stream);
 

@@ -47,6 +47,7 @@ def lib():
         _lib.uva_emu_energy_bwd.argtypes = [C.c_void_p, _dp, _dp, C.c_int32, _dp, _dp, _dp]
         _lib.uva_emu_step_bwd.argtypes = [C.c_void_p, _dp, _dp, _dp]
         _lib.uva_emu_adjoint_cg.argtypes = [C.c_void_p, _dp, _dp, C.c_double, C.c_int32]
+        _lib.uva_emu_set_pins.argtypes = [C.c_void_p, _i64p, C.c_int64, C.c_double]
     return _lib
 
 
@@ -97,6 +98,10 @@ class EmuPlan:
         o = L.SlimOpts(iters, cg_max_iters, tol, cg_tol)
         lib().uva_emu_slim(self.h, U.ctypes.data_as(_dp), C.byref(o))
         return U
+
+    def set_pins(self, pins, w):
+        pins = np.ascontiguousarray(np.asarray(pins, np.int64))
+        lib().uva_emu_set_pins(self.h, pins.ctypes.data_as(_i64p), len(pins), float(w))
 
     def cg(self, b, x0, tol=1e-6, max_iters=0, precision=0):
         b = np.ascontiguousarray(np.array(b, np.float64))

@@ -22,10 +22,11 @@ Only behaviour that the oracle's own checks cannot pin is recorded:
     hand-built contacts and real layouts (engine/validate.py, with
     engine/metrics.py loaded as its package sibling),
   * hier.Solver.relax and bad_fraction on two subsets of a four-island mesh
-    (engine/hier.py with engine/gpu.py, CPU torch).
+    (engine/hier.py with engine/gpu.py, CPU torch),
+  * gpu.Batch.arap at its default CG budget (40 iterations: not converged).
 Output: tests/golden/param_golden.npz, batch_golden.npz, slim_golden.npz,
-validate_golden.npz and hier_golden.npz (``--only slim|validate|hier``
-writes one alone).
+validate_golden.npz, hier_golden.npz and batch40_golden.npz
+(``--only slim|validate|hier|batch40`` writes one alone).
 """
 from __future__ import annotations
 
@@ -160,6 +161,32 @@ def main(root):
     for f in ("param_golden.npz", "batch_golden.npz"):
         print(f, (HERE / f).stat().st_size, "bytes")
     slim_fixtures(rp)
+
+
+def batch40_fixtures(rp, rg):
+    """gpu.Batch.arap at its default CG budget (40 iterations, tol 1e-6, one
+    CG over all charts): the iterate the product must reproduce without
+    converging.  Output: tests/golden/batch40_golden.npz."""
+    import torch
+
+    from ultimate_uv_artist_amd import workloads
+
+    rng = np.random.default_rng(310)
+    items = []
+    for t in (400, 900, 1600, 250):
+        Pb, Tb = workloads.disk_island(rng, t)
+        Ub = rp.lscm(Pb, Tb)
+        Ub = Ub + rng.normal(0, 0.003 * np.ptp(Ub), Ub.shape)
+        items.append((Pb, Tb, Ub))
+    b = rg.Batch(items, dev=torch.device("cpu"), dtype=torch.float64)
+    b.arap(4, line_search=True)
+    u_ls = b.U.clone()
+    b.arap(2, line_search=False)
+    gold = {"n": np.array(len(items)), "after_ls": u_ls.numpy(), "after_free": b.U.numpy()}
+    for i, (Pb, Tb, Ub) in enumerate(items):
+        gold[f"P{i}"], gold[f"T{i}"], gold[f"U{i}"] = Pb, Tb.astype(np.int32), Ub
+    np.savez_compressed(HERE / "batch40_golden.npz", **gold)
+    print("batch40_golden.npz", (HERE / "batch40_golden.npz").stat().st_size, "bytes")
 
 
 def load_validate(root):
@@ -381,7 +408,10 @@ if __name__ == "__main__":
         validate_fixtures(root, load_reference(root)[0])
     elif only == "hier":
         hier_fixtures(root, load_reference(root)[0])
+    elif only == "batch40":
+        batch40_fixtures(*load_reference(root))
     else:
         main(root)
         validate_fixtures(root, load_reference(root)[0])
         hier_fixtures(root, load_reference(root)[0])
+        batch40_fixtures(*load_reference(root))

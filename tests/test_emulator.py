@@ -321,3 +321,55 @@ def test_emu_hier_iterations_vs_oracle():
     U = e.arap(U0.astype(np.float32).astype(np.float64), 4, line_search=True, tol=0.0, cg_tol=0.0,
                cg_max_iters=20, mode=2)
     assert unit_box_err(U, Ur) <= 1e-5
+
+
+def test_emu_chart_pins_coupled_cg():
+    """Several charts solved as one system (hier.Solver.relax / Batch): a
+    one-island plan holding two charts, soft pins at each chart's first
+    vertex with the island's weight max |W| (uva_plan_set_pins), and exactly
+    20 Jacobi-PCG iterations shared by both charts, against plain PCG on the
+    same block-diagonal matrix (hier.py:405-417, 465-474)."""
+    import scipy.sparse as sp
+
+    from oracle import hier_cpu
+    from oracle import param_cpu as oracle
+    from ultimate_uv_artist_amd import workloads
+
+    rng = np.random.default_rng(62)
+    a, b = workloads.disk_island(rng, 500), workloads.disk_island(rng, 900)
+    P = np.concatenate([a[0], b[0] + 5.0])
+    T = np.concatenate([a[1], b[1] + len(a[0])])
+    starts = np.array([0, len(a[0])])
+    U0 = np.concatenate([oracle.lscm(*a), oracle.lscm(*b)])
+    e = EmuPlan([(P, T)], batch=True, keep_order=True)
+    e.set_pins(starts, 0.0)
+    rhs = e.local_rhs(U0)
+    x = e.cg(rhs, U0, tol=0.0, max_iters=20, precision=2)
+    X, w, ok, area, x1, x2, y2 = hier_cpu.frames(P, T)
+    W = np.concatenate([w[:, 0], w[:, 1], w[:, 2]])
+    I = np.concatenate([T[:, 1], T[:, 2], T[:, 0]])
+    J = np.concatenate([T[:, 2], T[:, 0], T[:, 1]])
+    wpin = np.abs(W).max()
+    n = len(P)
+    L = sp.csr_matrix((np.concatenate([-W, -W, W, W, [wpin, wpin]]),
+                       (np.concatenate([I, J, I, J, starts]), np.concatenate([J, I, I, J, starts]))), shape=(n, n))
+    dinv = 1.0 / L.diagonal()[:, None]
+    xr = U0.copy()
+    r = rhs - L @ xr
+    z = r * dinv
+    p = z.copy()
+    rz = (r * z).sum(0)
+    for _ in range(20):
+        Ap = L @ p
+        al = rz / (p * Ap).sum(0)
+        xr = xr + p * al
+        r = r - Ap * al
+        z = r * dinv
+        rz2 = (r * z).sum(0)
+        p = z + p * (rz2 / rz)
+        rz = rz2
+    assert np.abs(x - xr).max() <= 1e-9 * np.abs(xr).max()
+    # one shared CG differs from per-chart CGs after 20 iterations
+    e2 = EmuPlan([a, (b[0] + 5.0, b[1])], batch=True, keep_order=True)
+    x2 = e2.cg(rhs, U0, tol=0.0, max_iters=20, precision=2)
+    assert np.abs(x2 - xr).max() > 1e-6 * np.abs(xr).max()

@@ -515,3 +515,26 @@ def test_hier_solver_matches_reference(i):
     assert np.abs(scale / h[f"c{i}_scale"] - 1).max() <= 1e-5
     assert np.abs(bA - h[f"c{i}_badA"]).max() <= 1e-4 and np.abs(bR - h[f"c{i}_badR"]).max() <= 1e-4
     assert U.dtype == np.float64 and np.array_equal(U, U.astype(np.float32).astype(np.float64))
+
+
+def test_batch_default_budget_matches_reference():
+    """gpu.Batch.arap at its default CG budget (40 iterations, tolerance
+    1e-6, one CG shared by all charts) against the reference's own run of the
+    same calls (tests/golden/batch40_golden.npz, float64 CPU torch): the
+    unconverged iterates agree per chart to 1e-6 (unit box)."""
+    from pathlib import Path
+
+    from ultimate_uv_artist_amd import gpu
+
+    g = dict(np.load(Path(__file__).resolve().parent / "golden" / "batch40_golden.npz"))
+    items = [(g[f"P{i}"], g[f"T{i}"].astype(np.int64), g[f"U{i}"]) for i in range(int(g["n"]))]
+    b = gpu.Batch(items, dtype=_torch().float64)
+    b.arap(4, line_search=True)
+    u_ls = b.U.cpu().numpy()
+    b.arap(2, line_search=False)
+    u_free = b.U.cpu().numpy()
+    off = b.offsets
+    for got, ref in ((u_ls, g["after_ls"]), (u_free, g["after_free"])):
+        for i in range(len(items)):
+            s = slice(off[i], off[i + 1])
+            assert unit_box_err(got[s], ref[s]) <= 1e-6

@@ -2,6 +2,7 @@
 // hipGraph capture of the ARAP iterations.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -512,6 +513,55 @@ int uva_plan_set_scale(uva_plan* plan, const double* scale_dev, void* stream) {
     return UVA_OK;
 }
 
+int uva_plan_set_pins(uva_plan* plan, const int64_t* pins, int64_t n_pins, double w, void* stream) {
+    if (!valid_plan(plan) || (n_pins > 0 && !pins)) return fail(UVA_EINVAL, "null argument");
+    uva::DevPlan& d = plan->dp;
+    if (!d.mode_batch) return fail(UVA_EINVAL, "per-vertex pins need a plan with Batch conventions");
+    if (!(w >= 0.0) || !std::isfinite(w)) return fail(UVA_EINVAL, "pin weight must be >= 0 and finite");
+    if (n_pins < 0) return fail(UVA_EINVAL, "n_pins must be >= 0");
+    hipStream_t s = (hipStream_t)stream;
+    const int n = d.n_vert;
+    // w = 0: each pin takes the soft-pin weight of its island
+    std::vector<double> wisl((size_t)std::max(d.n_isl, 1), w);
+    if (w == 0.0 && d.n_isl) {
+        HIP_TRY(hipMemcpyAsync(wisl.data(), d.isl_wpin, d.n_isl * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+    }
+    std::vector<double> rp((size_t)std::max(n, 1), 0.0);
+    for (int64_t i = 0; i < n_pins; ++i) {
+        if (pins[i] < 0 || pins[i] >= n) return fail(UVA_EINVAL, "pin outside the plan's vertices");
+        const int r = plan->vinv[pins[i]];
+        const int isl = (int)(std::upper_bound(plan->v0.begin(), plan->v0.end(), r) - plan->v0.begin()) - 1;
+        rp[r] += wisl[std::max(isl, 0)];
+    }
+    std::vector<double> diag((size_t)std::max(n, 1), 0.0);
+    std::vector<float> dinv((size_t)std::max(n, 1), 1.0f);
+    if (n) HIP_TRY(hipMemcpyAsync(diag.data(), d.diag, n * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    for (int r = 0; r < n; ++r) {
+        const double g = diag[r] + rp[r];
+        dinv[r] = (float)(g > 1e-30 ? 1.0 / g : 1.0);
+    }
+    if (!d.rpin) {
+        double* p = nullptr;
+        if (plan->alloc(&p, (size_t)std::max(n, 1)) != hipSuccess) return fail(UVA_ENOMEM, "pin weights");
+        d.rpin = p;
+    }
+    if (n) {
+        HIP_TRY(hipMemcpyAsync((void*)d.rpin, rp.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync((void*)d.dinv32, dinv.data(), n * sizeof(float), hipMemcpyHostToDevice, s));
+    }
+    // the coarse inverses hold the islands' own pins: Jacobi only from here on
+    std::vector<int64_t> none((size_t)std::max(d.n_isl, 1), -1);
+    if (d.n_isl)
+        HIP_TRY(hipMemcpyAsync((void*)d.isl_cofs, none.data(), d.n_isl * sizeof(int64_t), hipMemcpyHostToDevice, s));
+    if (plan->gexec) { (void)hipGraphExecDestroy(plan->gexec); plan->gexec = nullptr; }
+    if (plan->graph) { (void)hipGraphDestroy(plan->graph); plan->graph = nullptr; }
+    plan->gkey.clear();
+    HIP_TRY(hipStreamSynchronize(s));
+    return UVA_OK;
+}
+
 int uva_lscm(uva_plan* plan, double* U, const int64_t* pins, double tol, int32_t max_iters, void* stream) {
     if (!valid_plan(plan) || !U) return fail(UVA_EINVAL, "null argument");
     if (!(tol > 0.0)) return fail(UVA_EINVAL, "tol must be positive");
@@ -553,7 +603,8 @@ int uva_lscm(uva_plan* plan, double* U, const int64_t* pins, double tol, int32_t
 int uva_arap(uva_plan* plan, double* U, const uva_arap_opts* o, void* stream) {
     if (!valid_plan(plan) || !U || !o) return fail(UVA_EINVAL, "null argument");
     if (o->iters < 0) return fail(UVA_EINVAL, "iters must be >= 0");
-    if (!(o->cg_tol > 0.0)) return fail(UVA_EINVAL, "cg_tol must be positive");
+    if (!(o->cg_tol >= 0.0) || (o->cg_tol == 0.0 && o->cg_max_iters <= 0))
+        return fail(UVA_EINVAL, "cg_tol must be positive, or 0 with cg_max_iters > 0");
     if (o->precision < UVA_PREC_MIXED || o->precision > UVA_PREC_F64) return fail(UVA_EINVAL, "bad precision");
     if (o->mode != UVA_MODE_REFINE && o->mode != UVA_MODE_BATCH && o->mode != UVA_MODE_HIER)
         return fail(UVA_EINVAL, "bad mode");
@@ -585,7 +636,8 @@ int uva_arap_profile(uva_plan* plan, double* U, const uva_arap_opts* o, double* 
                      void* stream) {
     if (!valid_plan(plan) || !U || !o || !launch_ms) return fail(UVA_EINVAL, "null argument");
     if (o->iters < 0 || n_launch != 1 + o->iters) return fail(UVA_EINVAL, "n_launch must be iters + 1");
-    if (!(o->cg_tol > 0.0)) return fail(UVA_EINVAL, "cg_tol must be positive");
+    if (!(o->cg_tol >= 0.0) || (o->cg_tol == 0.0 && o->cg_max_iters <= 0))
+        return fail(UVA_EINVAL, "cg_tol must be positive, or 0 with cg_max_iters > 0");
     if (o->precision < UVA_PREC_MIXED || o->precision > UVA_PREC_F64) return fail(UVA_EINVAL, "bad precision");
     if (o->mode != UVA_MODE_REFINE && o->mode != UVA_MODE_BATCH && o->mode != UVA_MODE_HIER)
         return fail(UVA_EINVAL, "bad mode");
@@ -650,7 +702,8 @@ int uva_local_rhs(uva_plan* plan, const double* U, double* b, int32_t mode, void
 int uva_cg_solve(uva_plan* plan, const double* b, double* x, double tol, int32_t max_iters, int32_t precision,
                  void* stream) {
     if (!valid_plan(plan) || !b || !x) return fail(UVA_EINVAL, "null argument");
-    if (!(tol > 0.0)) return fail(UVA_EINVAL, "tol must be positive");
+    if (!(tol >= 0.0) || (tol == 0.0 && max_iters <= 0))
+        return fail(UVA_EINVAL, "tol must be positive, or 0 with max_iters > 0");
     if (precision < UVA_PREC_MIXED || precision > UVA_PREC_F64) return fail(UVA_EINVAL, "bad precision");
     hipStream_t s = (hipStream_t)stream;
     int rc = UVA_OK;
@@ -670,7 +723,8 @@ int uva_max_step(uva_plan* plan, const double* U, const double* D, double* tmax,
     uva::TaskArgs a = base_args((double*)to_plan(plan, U, 0, s, rc));
     a.in = to_plan(plan, D, 1, s, rc);
     a.out_isl = tmax;
-    a.batch = mode == UVA_MODE_BATCH;
+    a.batch = mode == UVA_MODE_BATCH || mode == UVA_MODE_HIER;
+    a.hier = mode == UVA_MODE_HIER;
     if (!rc) rc = reset_queue(plan, 1, s);
     if (!rc) rc = launch(plan, uva::TASK_MAX_STEP, UVA_PREC_MIXED, a, 0, s);
     return rc;

@@ -80,10 +80,13 @@ UVA_HD void gather_corner_grads(Team& tm, const DevPlan& P, const Isl& d, double
             gx += y.x;
             gy += y.y;
         }
-        if (lam && row == d.pin) {   // soft-pin term of b: wpin U[pin]
+        if (lam) {   // soft-pin term of b: wpin U[pin]
             const double2 l = lam[row];
-            gx += d.wpin * l.x;
-            gy += d.wpin * l.y;
+            const double pw = pin_weight(P, d, row);
+            if (pw != 0.0) {
+                gx += pw * l.x;
+                gy += pw * l.y;
+            }
         }
         out[row] = make_double2(gx, gy);
     }

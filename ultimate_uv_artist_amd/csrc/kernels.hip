@@ -429,10 +429,17 @@ UVA_HD __forceinline__ typename Vec2<V>::type spmv_row(const DevPlan& P, int row
     return y;
 }
 
+// Soft-pin weight of a row: the island's pin, or the per-row weights that
+// replace it when the plan's islands are pinned chart by chart.
+UVA_HD __forceinline__ double pin_weight(const DevPlan& P, const Isl& d, int row) {
+    if (P.rpin) return P.rpin[row];
+    return row == d.pin ? d.wpin : 0.0;
+}
+
 template <int OP>
 UVA_HD __forceinline__ double diag_of(const DevPlan& P, const Isl& d, int row) {
     double g = P.diag[row];
-    if (OP == OP_ARAP && row == d.pin) g += d.wpin;
+    if (OP == OP_ARAP) g += pin_weight(P, d, row);
     return g;
 }
 
@@ -583,9 +590,10 @@ UVA_HD __forceinline__ double2 rhs_step(Team& tm, const DevPlan& P, const Isl& d
             by += R.y * g.x + R.x * g.y;
         }
         const double2 u = U[row];
-        if (row == d.pin) {
-            bx += d.wpin * u.x;
-            by += d.wpin * u.y;
+        const double pw = pin_weight(P, d, row);
+        if (pw != 0.0) {
+            bx += pw * u.x;
+            by += pw * u.y;
         }
         P.b64[row] = make_double2(bx, by);
         P.X64[row] = u;
@@ -828,7 +836,7 @@ UVA_HD __forceinline__ void run_task(Team& tm, const DevPlan& P, int isl, const 
             P.X64[row] = make_double2(u.x + dd.x, u.y + dd.y);
         }
         tm.sync();
-        const double t = max_step_island(tm, P, d, U, P.X64, a.batch);
+        const double t = max_step_island(tm, P, d, U, P.X64, a.batch, a.hier != 0);
         if (tm.leader()) a.out_isl[isl] = t;
     } else if (KIND == TASK_ENERGY) {
         const double s = energy_fwd_island(tm, P, d, U, a.w, a.kind);

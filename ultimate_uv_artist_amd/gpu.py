@@ -18,7 +18,9 @@ import numpy as np
 import torch
 
 from . import _lib
-from .plan import CG_TOL, IslandPlan, require_gpu
+from .plan import IslandPlan, require_gpu
+
+BATCH_CG_TOL = 1e-6   # gpu.py:184 Batch._cg(tol=1e-6)
 
 
 def csr32(A):
@@ -83,6 +85,8 @@ class Batch:
         self.offsets = np.cumsum([0] + [len(p) for p, _, _ in charts])
         self.n = int(self.offsets[-1])
         self._plan = IslandPlan([(p, t) for p, t, _ in charts], batch=True, dev=self.dev)
+        self._P = np.concatenate([np.asarray(p, np.float64).reshape(-1, 3) for p, _, _ in charts])
+        self._whole = None
         U = np.concatenate([np.asarray(u, np.float64).reshape(-1, 2) for _, _, u in charts])
         self._U = torch.as_tensor(U, device=self.dev).contiguous()
         toff = np.cumsum([0] + [len(t) for _, t, _ in charts])
@@ -111,6 +115,8 @@ class Batch:
     def scale(self, value):
         self._scale = value.to(self.dev, torch.float64).contiguous()
         self._plan.set_scale(self._scale)
+        if self._whole is not None:
+            self._whole.set_scale(self._scale)
 
     # ---------------------------------------------------------------- pieces
     def signed_area(self, U):
@@ -125,13 +131,39 @@ class Batch:
         return self._plan.flips_per_chart(U).to(self.dt)
 
     # ---------------------------------------------------------------- solvers
-    def arap(self, iters=8, line_search=True, cg_iters=40, cg_tol=CG_TOL):
-        """ARAP iterations of every chart (gpu.py:230-249): rotations x scale,
-        rhs, CG to relative residual ``cg_tol`` per chart (at most ``cg_iters``
-        iterations per global step), revert of non-finite charts, per-chart
-        flip-safe step when ``line_search``."""
-        self._plan.arap(self._U, iters, line_search=line_search, mode=_lib.UVA_MODE_BATCH, cg_tol=cg_tol,
-                        cg_max_iters=cg_iters)
+    def _whole_plan(self):
+        """One-island plan of the whole batch (block-diagonal), soft pins at
+        every chart's first vertex with the batch-global weight: the global
+        step as the reference solves it, one CG over all charts together."""
+        if getattr(self, "_whole", None) is None:
+            self._whole = IslandPlan([(self._P, self.T.cpu().numpy())], batch=True, dev=self.dev)
+            self._whole.set_pins(self.offsets[:-1], 0.0)
+            self._whole.set_scale(self._scale)
+        return self._whole
+
+    def arap(self, iters=8, line_search=True, cg_iters=40, cg_tol=BATCH_CG_TOL):
+        """ARAP iterations of every chart (gpu.py:230-249): rotations x scale
+        and rhs, one Jacobi-PCG over all charts together from the current
+        layout, stopped when both columns reach relative residual ``cg_tol``
+        or after ``cg_iters`` iterations (gpu.py:184-202), revert of
+        non-finite charts, per-chart flip-safe step when ``line_search``."""
+        whole = self._whole_plan()
+        U = self._U
+        for _ in range(int(iters)):
+            b = whole.local_rhs(U, _lib.UVA_MODE_BATCH)
+            Un = U.clone()
+            whole.cg_solve(b, Un, tol=cg_tol, max_iters=int(cg_iters), precision=_lib.UVA_PREC_F64)
+            bad = ~torch.isfinite(Un).all(1)
+            if bool(bad.any()):
+                badc = torch.zeros(self.nc, dtype=torch.bool, device=self.dev)
+                badc[self.cv[bad]] = True
+                Un = torch.where(badc[self.cv][:, None], U, Un)
+            if line_search:
+                D = (Un - U).contiguous()
+                step = torch.clamp(0.9 * self._plan.max_step(U, D, mode=_lib.UVA_MODE_BATCH), max=1.0)
+                U.add_(D * step[self.cv][:, None])
+            else:
+                U.copy_(Un)
         return self.U
 
     def repair_flips(self, rounds=60):

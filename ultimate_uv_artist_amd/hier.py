@@ -8,13 +8,18 @@ plan and runs the reference's sequence on the GPU kernels:
 
   orient every chart by area-weighted sign         uva_orient_positive
   Jacobi flip repair (``repair_rounds``)            uva_repair_jacobi
-  ``iters`` ARAP iterations, each a Jacobi-PCG      uva_arap(UVA_MODE_HIER)
-  of exactly ``cg_iters`` iterations from the
-  current layout, non-finite charts reverted, a
-  folded chart taking the plain step and the
-  others 0.9 of the largest fold-free step
-  (usable triangles only), with a repair after
-  iteration ``iters // 2``
+  ``iters`` ARAP iterations:
+    rotations and rhs of the whole subset           uva_local_rhs (one-island plan)
+    one Jacobi-PCG of exactly ``cg_iters``          uva_cg_solve (one-island plan,
+    iterations over all charts together               per-chart pins: uva_plan_set_pins)
+    (the reference's batched CG shares its
+    step lengths between charts), from the
+    current layout
+    non-finite charts keep their layout             torch
+    folded charts take the plain step, the others  uva_flips_per_chart,
+    0.9 of the largest fold-free step over          uva_max_step(UVA_MODE_HIER)
+    usable triangles
+    a repair after iteration ``iters // 2``         uva_repair_jacobi
   final repair, per-chart quality                   torch on the plan's GPU
 
 Every vertex must belong to a face of the subset (the reference's callers
@@ -32,7 +37,7 @@ from .plan import IslandPlan, require_gpu
 
 class Solver:
     """Frames come from the mesh's positions ``topo.V`` and faces ``topo.F``
-    (hier.py:361-389); the GPU plan of a subset is built per ``relax`` call."""
+    (hier.py:361-389); the GPU plans of a subset are built per ``relax`` call."""
 
     def __init__(self, topo, dev=None):
         self.dev = require_gpu(dev)
@@ -85,21 +90,38 @@ class Solver:
         if not charts:
             self._q = None
             return U0.astype(np.float32).astype(np.float64), flips, scale
+        # per-chart plan: orientation, repair, flips, step limits; one-island
+        # plan of the whole subset: the global step, one CG over every chart
+        # as the reference solves them (shared step lengths), one soft pin
+        # per chart with the subset's weight max |W|
         plan = IslandPlan(charts, batch=True, dev=self.dev)
+        voff = plan.vert_off
+        Pall = np.concatenate([c[0] for c in charts])
+        Tall = plan.T_local + np.repeat(voff[:-1], np.diff(plan.tri_off))[:, None]
+        whole = IslandPlan([(Pall, Tall)], batch=True, dev=self.dev)
+        whole.set_pins(voff[:-1], 0.0)
+        cv = torch.as_tensor(np.repeat(np.arange(len(charts)), np.diff(voff)), device=self.dev)
         # the reference holds the layout in float32 (hier.py:400)
         U = plan.device_uv(U0[vorder].astype(np.float32).astype(np.float64))
         plan.orient_positive(U)
         plan.repair_jacobi(U, repair_rounds)
-        first = min(iters, iters // 2 + 1)
-        if first > 0:
-            plan.arap(U, first, line_search=True, mode=_lib.UVA_MODE_HIER, tol=0.0, cg_tol=0.0,
-                      cg_max_iters=int(cg_iters))
-            plan.repair_jacobi(U, repair_rounds)            # after iteration iters // 2
-        if iters - first > 0:
-            plan.arap(U, iters - first, line_search=True, mode=_lib.UVA_MODE_HIER, tol=0.0, cg_tol=0.0,
-                      cg_max_iters=int(cg_iters))
+        for it in range(int(iters)):
+            b = whole.local_rhs(U, _lib.UVA_MODE_BATCH)
+            Un = U.clone()
+            whole.cg_solve(b, Un, tol=0.0, max_iters=int(cg_iters), precision=_lib.UVA_PREC_F64)
+            bad = ~torch.isfinite(Un).all(1)
+            if bool(bad.any()):                              # hier.py:500-503
+                badc = torch.zeros(len(charts), dtype=torch.bool, device=self.dev)
+                badc[cv[bad]] = True
+                Un = torch.where(badc[cv][:, None], U, Un)
+            folded = plan.flips_per_chart(U) > 0
+            D = (Un - U).contiguous()
+            step = torch.clamp(0.9 * plan.max_step(U, D, mode=_lib.UVA_MODE_HIER), max=1.0)
+            step = torch.where(folded, torch.ones_like(step), step)
+            U.add_(D * step[cv][:, None])
+            if it == iters // 2:
+                plan.repair_jacobi(U, repair_rounds)
         plan.repair_jacobi(U, repair_rounds)
-        Pall = np.concatenate([c[0] for c in charts])
         U.copy_(U.float().double())                          # the reference returns float32 values
         q = self._quality(plan, Pall, U, len(charts))
         out = np.empty((nv, 2))

@@ -257,6 +257,13 @@ class IslandPlan:
         check(_lib.lib().uva_orient_positive(self._h, ptr(U), stream_of(self.dev)), "uva_orient_positive")
         return U
 
+    def set_pins(self, pins, w=0.0):
+        """Soft pins at the given vertices in place of each island's own pin
+        (w = 0: the island's pin weight); see uva_plan_set_pins."""
+        p = np.ascontiguousarray(np.asarray(pins, np.int64).reshape(-1))
+        check(_lib.lib().uva_plan_set_pins(self._h, i64p(p), len(p), float(w), stream_of(self.dev)),
+              "uva_plan_set_pins")
+
     def set_scale(self, scale):
         s = scale.to(self.dev, torch.float64).contiguous()
         if s.numel() != self.t:
