@@ -340,22 +340,40 @@ def test_graph_replays_match_eager(small_disks):
 
 
 def test_nonfinite_island_reverts_in_batch(small_disks):
+    """A chart with a non-finite vertex: the one CG of the batch carries the
+    NaN into every chart's iterate (its dot products are shared), so every
+    chart reverts and keeps its layout, as in the reference (gpu.py:184-202,
+    235-240); the per-island solve of param.refine semantics leaves the other
+    islands' results as they are without the bad one."""
     from oracle import param_cpu as oracle
-    from ultimate_uv_artist_amd import gpu
+    from ultimate_uv_artist_amd import _lib, gpu
+    from ultimate_uv_artist_amd.plan import IslandPlan
 
     items = [(P, T, oracle.lscm(P, T)) for P, T in small_disks[:4]]
-    good = gpu.Batch(items)
-    good.arap(2, line_search=True, cg_iters=100000)
     bad_items = list(items)
     Ubad = items[1][2].copy()
     Ubad[3] = np.nan
     bad_items[1] = (items[1][0], items[1][1], Ubad)
     b = gpu.Batch(bad_items)
     b.arap(2, line_search=True, cg_iters=100000)
-    a, c = good.chart_uvs(), b.chart_uvs()
+    c = b.chart_uvs()
     for i in (0, 2, 3):
-        assert np.array_equal(a[i], c[i])
+        assert np.array_equal(c[i], items[i][2])
     assert np.isnan(c[1][3]).all()
+    # per-island solves (UVA_MODE_BATCH through uva_arap): the bad island
+    # reverts, the others are solved as without it
+    charts = [(P, T) for P, T, _ in items]
+    U0 = np.concatenate([u for _, _, u in bad_items])
+    good0 = np.concatenate([u for _, _, u in items])
+    plan = IslandPlan(charts, batch=True)
+    Ug, Ub = plan.device_uv(good0), plan.device_uv(U0)
+    plan.arap(Ug, 2, line_search=True, mode=_lib.UVA_MODE_BATCH, cg_max_iters=100000)
+    plan.arap(Ub, 2, line_search=True, mode=_lib.UVA_MODE_BATCH, cg_max_iters=100000)
+    Ug, Ub = Ug.cpu().numpy(), Ub.cpu().numpy()
+    vo = plan.vert_off
+    for i in (0, 2, 3):
+        assert np.array_equal(Ug[vo[i]:vo[i + 1]], Ub[vo[i]:vo[i + 1]])
+    assert np.isnan(Ub[vo[1] + 3]).all()
 
 
 def _own_box(U):
