@@ -572,7 +572,7 @@ def test_iterations_in_one_launch_equal_one_launch_each(small_disks):
     outs = []
     for per, graph in ((1, False), (0, False), (3, False), (0, True)):
         U = plan.device_uv(U0)
-        plan.arap(U, 7, line_search=True, mode=_lib.UVA_MODE_REFINE, tol=3e-2, use_graph=graph,
+        plan.arap(U, 7, line_search=True, mode=_lib.UVA_MODE_REFINE, tol=1e-4, use_graph=graph,
                   iters_per_launch=per)
         outs.append((U.cpu().numpy(), plan.stats()["arap_iters"].copy()))
     for u, n in outs[1:]:
