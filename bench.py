@@ -44,8 +44,6 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, default=160, help="islands in the CPU baseline sample (0 = skip)")
     ap.add_argument("--verdict-sample", type=int, default=16,
                     help="islands of the CPU sample whose GPU and oracle layouts are validated (0 = skip)")
-    ap.add_argument("--iters-per-launch", type=int, default=0,
-                    help="ARAP iterations per kernel launch (0 = all in one launch)")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-tune", dest="tune", action="store_false",
                     help="keep the size-based schedule (no re-planning from measured iterations)")
@@ -74,7 +72,7 @@ def copy_probe(mib):
     print(json.dumps({"copy_probe_bytes_each_way": n * 4, "dispatches": 5}), flush=True)
 
 
-def measured_traffic(digest, per_launch):
+def measured_traffic(digest):
     """HBM bytes per ARAP-iteration launch from the committed rocprofv3 counter
     summary of this workload (FETCH_SIZE + WRITE_SIZE, scaled by a plain copy
     of known size), or None when no summary for this mesh is committed."""
@@ -82,7 +80,7 @@ def measured_traffic(digest, per_launch):
     if not f.exists():
         return None
     d = json.loads(f.read_text())
-    if d.get("mesh_sha256") != digest[:16] or d.get("iters_per_launch", 1) != per_launch:
+    if d.get("mesh_sha256") != digest[:16]:
         return None
     return d.get("hbm_bytes_per_launch")
 
@@ -264,7 +262,7 @@ def main():
     def step():
         U.copy_(U0)
         plan.arap(U, args.iters, line_search=True, mode=_lib.UVA_MODE_REFINE, tol=0.0, cg_tol=args.cg_tol,
-                  precision=prec, use_graph=not args.no_graph, iters_per_launch=args.iters_per_launch)
+                  precision=prec, use_graph=not args.no_graph)
 
     if args.tune:
         # autotune: one step measures every island's CG iterations, the plan
@@ -286,23 +284,21 @@ def main():
     torch.cuda.synchronize()
     ms = ev0.elapsed_time(ev1) / max(args.steps, 1)
     value = float(tri.sum()) * args.iters / (ms * 1e-3) / 1e6
-    # The dominant kernel (one launch = every island's ARAP iterations, all of
-    # them by default): one more step run eagerly with a HIP event pair around
-    # every launch on the launch stream.  The solves are deterministic, so
-    # this step repeats the timed steps' work (the same CG iterations per island).
+    # The dominant kernel (one launch = one ARAP iteration of every island):
+    # one more step run eagerly with a HIP event pair around every launch on
+    # the launch stream.  The solves are deterministic, so this step repeats
+    # the timed steps' work (the same CG iterations per island).
     U.copy_(U0)
     launch_ms = plan.arap_profile(U, args.iters, line_search=True, mode=_lib.UVA_MODE_REFINE, tol=0.0,
-                                  cg_tol=args.cg_tol, precision=prec, iters_per_launch=args.iters_per_launch)
+                                  cg_tol=args.cg_tol, precision=prec)
     arap_ms = launch_ms[1:]
-    per_launch = args.iters // len(arap_ms) if len(arap_ms) else 0
     st = plan.stats()
     k = st["cg_iters"].astype(np.float64)                    # CG iterations over the step, per island
     iters_run = st["arap_iters"]
     bytes_step = float(np.sum(iters_run * (124.0 * tri + 72.0 * nv + 8.0 * nnz) + k * (100.0 * nv + 8.0 * nnz)))
     gbs = bytes_step / (arap_ms.sum() * 1e-3) / 1e9
-    print(f"bench: ARAP kernel {arap_ms.mean():.3f} ms per launch, {len(arap_ms)} launch(es) of {per_launch} "
-          f"iteration(s) (eager, HIP events; prologue {launch_ms[0]:.3f} ms); "
-          f"{bytes_step / len(arap_ms) / 1e9:.2f} GB algorithmic per launch", flush=True)
+    print(f"bench: ARAP-iteration kernel {arap_ms.mean():.3f} ms per launch (eager, HIP events; prologue "
+          f"{launch_ms[0]:.3f} ms); {bytes_step / args.iters / 1e9:.2f} GB algorithmic per launch", flush=True)
     kpi = k / np.maximum(iters_run, 1)
     busy = st["busy_ms"]
     n_cu = torch.cuda.get_device_properties(dev).multi_processor_count
@@ -432,10 +428,9 @@ def main():
                    "arap_iters": args.iters, "mesh_sha256": workloads.mesh_digest(charts)[:16],
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": measured_traffic(workloads.mesh_digest(charts), per_launch),
-                     "kernel": f"island_kernel ARAP iterations ({per_launch} per launch)",
-                     "launch_ms": round(float(arap_ms.mean()), 4),
-                     "algorithmic_bytes_per_launch": round(bytes_step / len(arap_ms))},
+                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": measured_traffic(workloads.mesh_digest(charts)),
+                     "kernel": "island_kernel ARAP iteration", "launch_ms": round(float(arap_ms.mean()), 4),
+                     "algorithmic_bytes_per_launch": round(bytes_step / args.iters)},
         "cpu_baseline": cpu,
         "plain_per_island": plain,
         "cg_iters_per_step": {"mean": round(float(kpi.mean()), 2), "p95": round(float(np.percentile(kpi, 95)), 2),

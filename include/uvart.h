@@ -121,9 +121,6 @@ typedef struct {
                                 fixed budgets of Batch and hier.Solver) */
     int32_t cg_max_iters;    /* cap per global step (Batch's cg_iters) */
     int32_t use_graph;       /* capture the ARAP iterations in a hipGraph */
-    int32_t iters_per_launch;/* ARAP iterations per kernel launch; 0 = all of
-                                them in one launch (islands do not wait for
-                                each other between iterations) */
 } uva_arap_opts;
 
 int uva_version(void);
@@ -174,10 +171,8 @@ int uva_lscm(uva_plan *plan, double *U, const int64_t *pins, double tol, int32_t
 int uva_arap(uva_plan *plan, double *U, const uva_arap_opts *opts, void *stream);
 
 /* uva_arap run eagerly with a HIP event pair around every launch on the
- * stream: launch_ms[0] is the prologue, launch_ms[k] the k-th ARAP launch
- * (n_launch = 1 + ceil(iters / iters_per_launch), 2 with the default of one
- * launch for all iterations).  Measurement only; no counterpart in the
- * reference. */
+ * stream: launch_ms[0] is the prologue, launch_ms[k] ARAP iteration k
+ * (n_launch = iters + 1).  Measurement only; no counterpart in the reference. */
 int uva_arap_profile(uva_plan *plan, double *U, const uva_arap_opts *opts, double *launch_ms, int32_t n_launch,
                      void *stream);
 

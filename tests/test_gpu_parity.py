@@ -556,26 +556,3 @@ def test_batch_default_budget_matches_reference():
         for i in range(len(items)):
             s = slice(off[i], off[i + 1])
             assert unit_box_err(got[s], ref[s]) <= 1e-6
-
-
-def test_iterations_in_one_launch_equal_one_launch_each(small_disks):
-    """All ARAP iterations in one launch (the default: islands do not wait for
-    each other between iterations) give bitwise the layout of one launch per
-    iteration, eagerly and from a captured graph, with early exits."""
-    from oracle import param_cpu as oracle
-    from ultimate_uv_artist_amd import _lib
-    from ultimate_uv_artist_amd.plan import IslandPlan
-
-    charts = small_disks[:6]
-    U0 = np.concatenate([oracle.lscm(P, T) for P, T in charts])
-    plan = IslandPlan(charts)
-    outs = []
-    for per, graph in ((1, False), (0, False), (3, False), (0, True)):
-        U = plan.device_uv(U0)
-        plan.arap(U, 7, line_search=True, mode=_lib.UVA_MODE_REFINE, tol=1e-4, use_graph=graph,
-                  iters_per_launch=per)
-        outs.append((U.cpu().numpy(), plan.stats()["arap_iters"].copy()))
-    for u, n in outs[1:]:
-        assert np.array_equal(u, outs[0][0]) and np.array_equal(n, outs[0][1])
-    assert (outs[0][1] < 7).any()          # some islands stopped early
-    assert len(plan.arap_profile(plan.device_uv(U0), 7, iters_per_launch=3)) == 1 + 3

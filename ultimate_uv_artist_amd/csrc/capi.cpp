@@ -127,15 +127,6 @@ uva::TaskArgs base_args(double* U) {
 // One sequence of launches (queue reset, prologue, iterations) with slots
 // reused in chunks of kMaxSlots.  With ev (2 events per launch), ev[2k] and
 // ev[2k+1] bracket launch k (0 = prologue) on the stream.
-// ARAP iterations per launch, and launches of a uva_arap call (prologue included)
-int iters_per_launch(const uva_arap_opts* o) {
-    return o->iters_per_launch > 0 ? std::min(o->iters_per_launch, std::max(o->iters, 1)) : std::max(o->iters, 1);
-}
-int arap_launches(const uva_arap_opts* o) {
-    const int per = iters_per_launch(o);
-    return 1 + (o->iters + per - 1) / per;
-}
-
 int arap_sequence(uva_plan* pl, double* Uext, const uva_arap_opts* o, hipStream_t s,
                   const std::vector<hipEvent_t>* ev = nullptr) {
     int prc = UVA_OK;
@@ -148,8 +139,7 @@ int arap_sequence(uva_plan* pl, double* Uext, const uva_arap_opts* o, hipStream_
     a.line_search = o->line_search;
     a.batch = o->mode == UVA_MODE_BATCH || o->mode == UVA_MODE_HIER;
     a.hier = o->mode == UVA_MODE_HIER;
-    const int per = iters_per_launch(o);
-    int total = arap_launches(o);
+    int total = 1 + o->iters;
     int done = 0;
     while (done < total) {
         int chunk = std::min(total - done, kMaxSlots);
@@ -158,9 +148,6 @@ int arap_sequence(uva_plan* pl, double* Uext, const uva_arap_opts* o, hipStream_
         for (int k = 0; k < chunk; ++k) {
             int step = done + k;
             if (ev) HIP_TRY(hipEventRecord((*ev)[2 * step], s));
-            // launch 0 is the prologue; launch j >= 1 runs iterations
-            // (j-1)*per .. min(j*per, iters) - 1
-            a.reps = step == 0 ? 1 : std::min(per, o->iters - (step - 1) * per);
             rc = launch(pl, step == 0 ? uva::TASK_PROLOGUE : uva::TASK_ARAP_ITER, o->precision, a, k, s);
             if (rc) return rc;
             if (ev) HIP_TRY(hipEventRecord((*ev)[2 * step + 1], s));
@@ -648,9 +635,7 @@ int uva_arap(uva_plan* plan, double* U, const uva_arap_opts* o, void* stream) {
 int uva_arap_profile(uva_plan* plan, double* U, const uva_arap_opts* o, double* launch_ms, int32_t n_launch,
                      void* stream) {
     if (!valid_plan(plan) || !U || !o || !launch_ms) return fail(UVA_EINVAL, "null argument");
-    if (o->iters < 0 || o->iters_per_launch < 0) return fail(UVA_EINVAL, "iters must be >= 0");
-    if (n_launch != arap_launches(o))
-        return fail(UVA_EINVAL, "n_launch must be 1 + the number of ARAP launches (ceil(iters / iters_per_launch))");
+    if (o->iters < 0 || n_launch != 1 + o->iters) return fail(UVA_EINVAL, "n_launch must be iters + 1");
     if (!(o->cg_tol >= 0.0) || (o->cg_tol == 0.0 && o->cg_max_iters <= 0))
         return fail(UVA_EINVAL, "cg_tol must be positive, or 0 with cg_max_iters > 0");
     if (o->precision < UVA_PREC_MIXED || o->precision > UVA_PREC_F64) return fail(UVA_EINVAL, "bad precision");

@@ -690,74 +690,68 @@ UVA_HD __forceinline__ void run_task(Team& tm, const DevPlan& P, int isl, const 
             st->ticks = 0;
         }
     } else if (KIND == TASK_ARAP_ITER) {
-        // a.reps ARAP iterations in a row in one launch: islands run their
-        // iterations without waiting for each other between them
-        const int reps = a.reps > 0 ? a.reps : 1;
-        for (int rep = 0; rep < reps; ++rep) {
-            if (uniform_i(st->done)) return;
-            const long long t_start = uva_clock();
-            const double2 b2 = rhs_step(tm, P, d, U);
-            // Batch and hier solves cap the CG iterations (40 and 20 by default), so
-            // their result depends on the preconditioner: they keep the
-            // reference's Jacobi one (gpu.py:183-201, hier.py:458-467)
-            const CGOut cg = cg_arap<PREC>(tm, P, d, b2, a.cg_tol, a.cg_max, !a.batch);
-            if (a.batch) {
-                // islands whose solve went non-finite keep their layout (gpu.py:235-240)
-                double bad = 0.0;
-                FOR_ROWS(tm, d, row, sg) {
-                    (void)sg;
-                    const double2 x = P.X64[row];
-                    bad += (__builtin_isfinite(x.x) && __builtin_isfinite(x.y)) ? 0.0 : 1.0;
-                }
-                if (tm.sum4(make_double4(bad, 0.0, 0.0, 0.0)).x > 0.0) {
-                    FOR_ROWS(tm, d, row, sg) {
-                        (void)sg;
-                        P.X64[row] = U[row];
-                    }
-                    tm.sync();
-                }
-            }
-            int use_ls = uniform_i(st->use_ls);
-            if (a.hier && use_ls) {
-                // hier.py:500-505: a chart with a folded usable triangle takes the
-                // plain ARAP step this iteration
-                double nf = 0.0;
-                FOR_TRIS(tm, d, f) nf += (P.fok[f] && signed_area(P, f, U) < 0.0) ? 1.0 : 0.0;
-                if (tm.sum4(make_double4(nf, 0.0, 0.0, 0.0)).x > 0.0) use_ls = 0;
-            }
-            double step = 1.0, tmax = 1e30;
-            if (use_ls) {
-                tmax = max_step_island(tm, P, d, U, P.X64, a.batch, a.hier != 0);
-                step = fmin(1.0, 0.9 * tmax);
-            }
+        if (uniform_i(st->done)) return;
+        const long long t_start = uva_clock();
+        const double2 b2 = rhs_step(tm, P, d, U);
+        // Batch and hier solves cap the CG iterations (40 and 20 by default), so
+        // their result depends on the preconditioner: they keep the
+        // reference's Jacobi one (gpu.py:183-201, hier.py:458-467)
+        const CGOut cg = cg_arap<PREC>(tm, P, d, b2, a.cg_tol, a.cg_max, !a.batch);
+        if (a.batch) {
+            // islands whose solve went non-finite keep their layout (gpu.py:235-240)
+            double bad = 0.0;
             FOR_ROWS(tm, d, row, sg) {
                 (void)sg;
                 const double2 x = P.X64[row];
-                if (use_ls) {
-                    const double2 u = U[row];
-                    U[row] = make_double2(u.x + step * (x.x - u.x), u.y + step * (x.y - u.y));
-                } else {
-                    U[row] = x;
-                }
+                bad += (__builtin_isfinite(x.x) && __builtin_isfinite(x.y)) ? 0.0 : 1.0;
             }
-            tm.sync();
-            const double en = local_step(tm, P, d, U, a.batch, !a.batch);
-            if (tm.leader()) {
-                st->cg_iters += cg.iters;
-                st->cg_iters_max = max(st->cg_iters_max, cg.iters);
-                st->arap_iters += 1;
-                st->converged = cg.conv;
-                st->rel_resid = cg.rel;
-                st->step = step;
-                st->tmax = tmax;
-                st->ticks += uva_clock() - t_start;
-                if (!a.batch) {
-                    const double prev = st->e_prev;
-                    if (fabs(prev - en) <= a.tol * fmax(prev, 1e-300)) st->done = 1;
-                    st->e_prev = en;
+            if (tm.sum4(make_double4(bad, 0.0, 0.0, 0.0)).x > 0.0) {
+                FOR_ROWS(tm, d, row, sg) {
+                    (void)sg;
+                    P.X64[row] = U[row];
                 }
+                tm.sync();
             }
-            tm.sync();   // the leader's island state (done) before the next iteration
+        }
+        int use_ls = uniform_i(st->use_ls);
+        if (a.hier && use_ls) {
+            // hier.py:500-505: a chart with a folded usable triangle takes the
+            // plain ARAP step this iteration
+            double nf = 0.0;
+            FOR_TRIS(tm, d, f) nf += (P.fok[f] && signed_area(P, f, U) < 0.0) ? 1.0 : 0.0;
+            if (tm.sum4(make_double4(nf, 0.0, 0.0, 0.0)).x > 0.0) use_ls = 0;
+        }
+        double step = 1.0, tmax = 1e30;
+        if (use_ls) {
+            tmax = max_step_island(tm, P, d, U, P.X64, a.batch, a.hier != 0);
+            step = fmin(1.0, 0.9 * tmax);
+        }
+        FOR_ROWS(tm, d, row, sg) {
+            (void)sg;
+            const double2 x = P.X64[row];
+            if (use_ls) {
+                const double2 u = U[row];
+                U[row] = make_double2(u.x + step * (x.x - u.x), u.y + step * (x.y - u.y));
+            } else {
+                U[row] = x;
+            }
+        }
+        tm.sync();
+        const double en = local_step(tm, P, d, U, a.batch, !a.batch);
+        if (tm.leader()) {
+            st->cg_iters += cg.iters;
+            st->cg_iters_max = max(st->cg_iters_max, cg.iters);
+            st->arap_iters += 1;
+            st->converged = cg.conv;
+            st->rel_resid = cg.rel;
+            st->step = step;
+            st->tmax = tmax;
+            st->ticks += uva_clock() - t_start;
+            if (!a.batch) {
+                const double prev = st->e_prev;
+                if (fabs(prev - en) <= a.tol * fmax(prev, 1e-300)) st->done = 1;
+                st->e_prev = en;
+            }
         }
     } else if (KIND == TASK_LSCM) {
         const int p0 = uniform_i(P.isl_lpin0[isl]), p1 = uniform_i(P.isl_lpin1[isl]);

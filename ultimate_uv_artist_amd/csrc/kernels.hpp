@@ -57,7 +57,6 @@ struct TaskArgs {
     const double* gin;       // per-island upstream gradient (energy backward)
     double* gw;              // per-triangle weight gradient (NULL = not wanted)
     int zero_x;              // CG starts from x = 0 instead of U (adjoint solve)
-    int reps;                // ARAP iterations per launch of TASK_ARAP_ITER (<= 0: one)
 };
 
 hipError_t launch_island_tasks(int kind, int prec, const DevPlan& P, const TaskArgs& a, int slot, int cus,

@@ -137,32 +137,24 @@ class IslandPlan:
         return U
 
     def arap(self, U, iters, line_search=True, mode=_lib.UVA_MODE_REFINE, tol=1e-5, cg_tol=CG_TOL,
-             cg_max_iters=0, precision=_lib.UVA_PREC_MIXED, use_graph=False, iters_per_launch=0):
-        """ARAP iterations on U (device tensor) in place; by default all of them
-        in one kernel launch (``iters_per_launch`` > 0 splits them)."""
+             cg_max_iters=0, precision=_lib.UVA_PREC_MIXED, use_graph=False):
+        """ARAP iterations on U (device tensor) in place."""
         self._check_uv(U)
         o = _lib.ArapOpts(int(iters), int(bool(line_search)), int(mode), int(precision), float(tol), float(cg_tol),
-                          int(cg_max_iters), int(bool(use_graph)), int(iters_per_launch))
+                          int(cg_max_iters), int(bool(use_graph)))
         check(_lib.lib().uva_arap(self._h, ptr(U), C.byref(o), stream_of(self.dev)), "uva_arap")
         return U
 
-    @staticmethod
-    def arap_launches(iters, iters_per_launch=0):
-        """Kernel launches of an ``arap`` call, the prologue included."""
-        per = min(iters_per_launch, max(iters, 1)) if iters_per_launch > 0 else max(iters, 1)
-        return 1 + (iters + per - 1) // per
-
     def arap_profile(self, U, iters, line_search=True, mode=_lib.UVA_MODE_REFINE, tol=1e-5, cg_tol=CG_TOL,
-                     cg_max_iters=0, precision=_lib.UVA_PREC_MIXED, iters_per_launch=0):
+                     cg_max_iters=0, precision=_lib.UVA_PREC_MIXED):
         """``arap`` run eagerly with HIP events around every launch: returns the
-        launch durations in ms (index 0 the prologue, then the ARAP launches)."""
+        launch durations in ms (index 0 the prologue, k the ARAP iteration k)."""
         self._check_uv(U)
         o = _lib.ArapOpts(int(iters), int(bool(line_search)), int(mode), int(precision), float(tol), float(cg_tol),
-                          int(cg_max_iters), 0, int(iters_per_launch))
-        n = self.arap_launches(int(iters), int(iters_per_launch))
-        ms = np.zeros(n, np.float64)
-        check(_lib.lib().uva_arap_profile(self._h, ptr(U), C.byref(o), f64p(ms), n, stream_of(self.dev)),
-              "uva_arap_profile")
+                          int(cg_max_iters), 0)
+        ms = np.zeros(int(iters) + 1, np.float64)
+        check(_lib.lib().uva_arap_profile(self._h, ptr(U), C.byref(o), f64p(ms), int(iters) + 1,
+                                          stream_of(self.dev)), "uva_arap_profile")
         return ms
 
     def slim(self, U, iters, tol=1e-5, cg_tol=SLIM_CG_TOL, cg_max_iters=0):
