@@ -121,6 +121,11 @@ typedef struct {
                                 fixed budgets of Batch and hier.Solver) */
     int32_t cg_max_iters;    /* cap per global step (Batch's cg_iters) */
     int32_t use_graph;       /* capture the ARAP iterations in a hipGraph */
+    int32_t iters_folded;    /* 0: every island runs iters iterations; > 0:
+                                islands whose start has a folded triangle
+                                run iters_folded, the others iters
+                                (charts.py:411-415 doubles the iterations of
+                                folded starts) */
 } uva_arap_opts;
 
 int uva_version(void);

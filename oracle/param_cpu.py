@@ -601,3 +601,20 @@ def flatten_lscm_arap(P, T, iters=8, tol=1e-4):
         U, _ = refine(P, T, U, iters=iters * (2 if folded else 1), method="arap", X=X, area=area, cots=cots,
                       line_search=not folded, tol=tol)
     return U
+
+
+def flatten(P, T, iters=8):
+    """charts.py:401-421 -- flatten_lscm_arap, then, if folds remain, the
+    sequential repair and, when it removed them, a flip-safe ARAP of
+    max(2, iters // 2) iterations (method "arap", no SLIM; the boundary
+    untangle of charts.py:428-431 is not restated)."""
+    T = np.asarray(T, np.int64)
+    X, area = local_frames(P, T)
+    cots = cot_weights(X)
+    U = flatten_lscm_arap(P, T, iters=iters, tol=1e-4)
+    if (signed_areas(U, T) <= 0).any():
+        U = repair_flips(U, T)
+        if not (signed_areas(U, T) <= 0).any() and iters:
+            U, _ = refine(P, T, U, iters=max(2, iters // 2), method="arap", X=X, area=area, cots=cots,
+                          line_search=True, tol=1e-4)
+    return U

@@ -82,12 +82,12 @@ class EmuPlan:
         return U
 
     def arap(self, U, iters, line_search=True, batch=False, tol=0.0, cg_tol=1e-6, cg_max_iters=0, precision=0,
-             mode=None):
+             mode=None, iters_folded=0):
         from ultimate_uv_artist_amd import _lib as L
 
         U = np.ascontiguousarray(np.array(U, np.float64))
         mode = (1 if batch else 0) if mode is None else mode
-        o = L.ArapOpts(iters, int(line_search), mode, precision, tol, cg_tol, cg_max_iters, 0)
+        o = L.ArapOpts(iters, int(line_search), mode, precision, tol, cg_tol, cg_max_iters, 0, iters_folded)
         lib().uva_emu_arap(self.h, U.ctypes.data_as(_dp), C.byref(o))
         return U
 

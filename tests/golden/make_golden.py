@@ -23,10 +23,13 @@ Only behaviour that the oracle's own checks cannot pin is recorded:
     engine/metrics.py loaded as its package sibling),
   * hier.Solver.relax and bad_fraction on two subsets of a four-island mesh
     (engine/hier.py with engine/gpu.py, CPU torch),
-  * gpu.Batch.arap at its default CG budget (40 iterations: not converged).
+  * gpu.Batch.arap at its default CG budget (40 iterations: not converged),
+  * charts.flatten on small charts (engine/charts.py with param.py and
+    metrics.py).
 Output: tests/golden/param_golden.npz, batch_golden.npz, slim_golden.npz,
 validate_golden.npz, hier_golden.npz and batch40_golden.npz
-(``--only slim|validate|hier|batch40`` writes one alone).
+and flatten_golden.npz (``--only slim|validate|hier|batch40|flatten``
+writes one alone).
 """
 from __future__ import annotations
 
@@ -187,6 +190,50 @@ def batch40_fixtures(rp, rg):
         gold[f"P{i}"], gold[f"T{i}"], gold[f"U{i}"] = Pb, Tb.astype(np.int32), Ub
     np.savez_compressed(HERE / "batch40_golden.npz", **gold)
     print("batch40_golden.npz", (HERE / "batch40_golden.npz").stat().st_size, "bytes")
+
+
+def flatten_cases():
+    """Small charts like the ones grow.validate and merge_islands flatten:
+    disk islands of 40-120 triangles, some with needle triangles (a folded
+    LSCM start), and ruffled ones (the centre lifted in alternating sectors,
+    so ARAP keeps a fold that the sequential repair removes)."""
+    from ultimate_uv_artist_amd import workloads
+
+    rng = np.random.default_rng(700)
+    charts = [workloads.disk_island(rng, int(t)) for t in rng.integers(40, 120, 16)]
+    charts += workloads.sliver_islands(seed=701, n_islands=10, tmin=60, tmax=140, frac=0.3, aspect=(200.0, 2000.0))
+    rng = np.random.default_rng(7)
+    for _ in range(40):
+        P, T = workloads.disk_island(rng, int(rng.integers(60, 140)))
+        r, th = np.linalg.norm(P[:, :2], axis=1), np.arctan2(P[:, 1], P[:, 0])
+        k, h = int(rng.integers(3, 7)), rng.uniform(0.3, 1.2)
+        P = P.copy()
+        P[:, 2] += h * np.exp(-(r / 0.45) ** 2) * np.sign(np.sin(k * th)) * (r > 0.05)
+        charts.append((P, T))
+    return charts
+
+
+def flatten_fixtures(root):
+    """charts.flatten (engine/charts.py with param.py and metrics.py as its
+    package siblings) on flatten_cases.  Output: tests/golden/flatten_golden.npz."""
+    load_validate(root)
+    pkg = sys.modules["ref_engine"]
+    for name in ("param", "charts"):
+        spec = importlib.util.spec_from_file_location(f"ref_engine.{name}", Path(root) / "engine" / f"{name}.py")
+        m = importlib.util.module_from_spec(spec)
+        sys.modules[f"ref_engine.{name}"] = m
+        spec.loader.exec_module(m)
+        setattr(pkg, name, m)
+    rc = sys.modules["ref_engine.charts"]
+    out = {}
+    charts = flatten_cases()
+    out["n"] = np.array(len(charts))
+    for i, (P, T) in enumerate(charts):
+        U, q = rc.flatten(P, T)
+        out[f"P{i}"], out[f"T{i}"], out[f"U{i}"] = P, T.astype(np.int32), U
+        out[f"q{i}"] = np.array([q["flips"], q["aniso_p95"], q["area_p95"], q["boundary_hits"]], np.float64)
+    np.savez_compressed(HERE / "flatten_golden.npz", **out)
+    print("flatten_golden.npz", (HERE / "flatten_golden.npz").stat().st_size, "bytes")
 
 
 def load_validate(root):
@@ -410,8 +457,11 @@ if __name__ == "__main__":
         hier_fixtures(root, load_reference(root)[0])
     elif only == "batch40":
         batch40_fixtures(*load_reference(root))
+    elif only == "flatten":
+        flatten_fixtures(root)
     else:
         main(root)
         validate_fixtures(root, load_reference(root)[0])
         hier_fixtures(root, load_reference(root)[0])
         batch40_fixtures(*load_reference(root))
+        flatten_fixtures(root)

@@ -373,3 +373,36 @@ def test_emu_chart_pins_coupled_cg():
     e2 = EmuPlan([a, (b[0] + 5.0, b[1])], batch=True, keep_order=True)
     x2 = e2.cg(rhs, U0, tol=0.0, max_iters=20, precision=2)
     assert np.abs(x2 - xr).max() > 1e-6 * np.abs(xr).max()
+
+
+def test_emu_folded_starts_run_more_iterations():
+    """charts.flatten's ARAP (charts.py:411-415) for several charts in one
+    call: a chart whose start is folded runs iters_folded iterations without
+    line search, the others iters with it (per-island cap and use_ls), as the
+    oracle's flatten_lscm_arap does chart by chart."""
+    from pathlib import Path
+
+    from oracle import param_cpu as oracle
+
+    g = dict(np.load(Path(__file__).resolve().parent / "golden" / "flatten_golden.npz"))
+    pick = [0, 17, 5]                          # chart 17 has a folded LSCM start
+    charts = [(g[f"P{i}"], g[f"T{i}"].astype(np.int64)) for i in pick]
+    U0 = []
+    for P, T in charts:
+        U = oracle.lscm(P, T)
+        sa = oracle.signed_areas(U, T)
+        if (sa < 0).sum() > (sa > 0).sum():
+            U[:, 1] = -U[:, 1]
+        U0.append(U)
+    folded = [bool((oracle.signed_areas(u, T) <= 0).any()) for u, (_, T) in zip(U0, charts)]
+    assert folded == [False, True, False]
+    e = EmuPlan(charts)
+    U = e.arap(np.concatenate(U0), 8, line_search=True, tol=1e-4, cg_tol=1e-10, iters_folded=16)
+    st = e.stats()
+    for i, (P, T) in enumerate(charts):
+        s = slice(e.vo[i], e.vo[i + 1])
+        tr = []
+        Ur, _ = oracle.refine(P, T, U0[i], iters=16 if folded[i] else 8, line_search=not folded[i], tol=1e-4,
+                              trace=tr)
+        assert st["iters"][i] == len(tr)
+        assert unit_box_err(U[s], Ur) <= 1e-6

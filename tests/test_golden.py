@@ -161,3 +161,17 @@ def test_hier_relax(i):
     bA, bR = S.bad_fraction(2.0, 1.5)
     assert np.abs(bA - h[f"c{i}_badA"]).max() <= 1e-6 and np.abs(bR - h[f"c{i}_badR"]).max() <= 1e-6
 
+
+
+def test_flatten_small_charts():
+    """charts.flatten (charts.py:401-421): LSCM, mirror, ARAP with doubled
+    iterations for folded starts, sequential repair and a second ARAP, on 66
+    small charts (plain, needle and ruffled)."""
+    g = dict(np.load(GOLD / "flatten_golden.npz"))
+    worst = 0.0
+    for i in range(int(g["n"])):
+        P, T, Ur = g[f"P{i}"], g[f"T{i}"].astype(np.int64), g[f"U{i}"]
+        U = pc.flatten(P, T)
+        lo = Ur.min(0)
+        worst = max(worst, float(np.abs(U - Ur).max() / (Ur.max(0) - lo).max()))
+    assert worst <= 1e-10

@@ -556,3 +556,29 @@ def test_batch_default_budget_matches_reference():
         for i in range(len(items)):
             s = slice(off[i], off[i + 1])
             assert unit_box_err(got[s], ref[s]) <= 1e-6
+
+
+def test_flatten_many_and_queue_match_reference():
+    """charts.flatten for 66 small charts as one batch (charts.flatten_many)
+    and through FlattenQueue from four threads, against the reference's own
+    charts.flatten (tests/golden/flatten_golden.npz): layouts within 1e-4
+    (unit box), identical fold counts, p95 anisotropy and area within 1 %."""
+    from concurrent.futures import ThreadPoolExecutor
+    from pathlib import Path
+
+    from ultimate_uv_artist_amd import charts as gch
+
+    g = dict(np.load(Path(__file__).resolve().parent / "golden" / "flatten_golden.npz"))
+    items = [(g[f"P{i}"], g[f"T{i}"].astype(np.int64)) for i in range(int(g["n"]))]
+    res = gch.flatten_many(items)
+    for i, (U, q) in enumerate(res):
+        Ur, qr = g[f"U{i}"], g[f"q{i}"]
+        assert unit_box_err(U, Ur) <= 1e-4
+        assert q["flips"] == qr[0]
+        assert abs(q["aniso_p95"] / qr[1] - 1) <= 0.01 and abs(q["area_p95"] / qr[2] - 1) <= 0.01
+    with gch.FlattenQueue(max_wait_ms=20.0) as fq, ThreadPoolExecutor(4) as ex:
+        out = list(ex.map(lambda it: fq.flatten(*it), items))
+    for (U, _), (Uq, _) in zip(res, out):
+        assert np.abs(U - Uq).max() <= 1e-9 * np.abs(U).max()
+    with pytest.raises(ValueError):
+        gch.flatten_many([(np.eye(4, 3), np.array([[0, 1, 2], [0, 2, 1]]))])

@@ -58,7 +58,8 @@ class IslandStats(C.Structure):
 
 class ArapOpts(C.Structure):
     _fields_ = [("iters", C.c_int32), ("line_search", C.c_int32), ("mode", C.c_int32), ("precision", C.c_int32),
-                ("tol", C.c_double), ("cg_tol", C.c_double), ("cg_max_iters", C.c_int32), ("use_graph", C.c_int32)]
+                ("tol", C.c_double), ("cg_tol", C.c_double), ("cg_max_iters", C.c_int32), ("use_graph", C.c_int32),
+                ("iters_folded", C.c_int32)]
 
 
 class SlimOpts(C.Structure):

@@ -139,7 +139,9 @@ int arap_sequence(uva_plan* pl, double* Uext, const uva_arap_opts* o, hipStream_
     a.line_search = o->line_search;
     a.batch = o->mode == UVA_MODE_BATCH || o->mode == UVA_MODE_HIER;
     a.hier = o->mode == UVA_MODE_HIER;
-    int total = 1 + o->iters;
+    a.iters_plain = o->iters_folded > 0 ? std::max(o->iters, 1) : 0;
+    a.iters_folded = o->iters_folded;
+    int total = 1 + std::max(o->iters, o->iters_folded);
     int done = 0;
     while (done < total) {
         int chunk = std::min(total - done, kMaxSlots);
@@ -602,7 +604,7 @@ int uva_lscm(uva_plan* plan, double* U, const int64_t* pins, double tol, int32_t
 
 int uva_arap(uva_plan* plan, double* U, const uva_arap_opts* o, void* stream) {
     if (!valid_plan(plan) || !U || !o) return fail(UVA_EINVAL, "null argument");
-    if (o->iters < 0) return fail(UVA_EINVAL, "iters must be >= 0");
+    if (o->iters < 0 || o->iters_folded < 0) return fail(UVA_EINVAL, "iters must be >= 0");
     if (!(o->cg_tol >= 0.0) || (o->cg_tol == 0.0 && o->cg_max_iters <= 0))
         return fail(UVA_EINVAL, "cg_tol must be positive, or 0 with cg_max_iters > 0");
     if (o->precision < UVA_PREC_MIXED || o->precision > UVA_PREC_F64) return fail(UVA_EINVAL, "bad precision");
@@ -635,7 +637,9 @@ int uva_arap(uva_plan* plan, double* U, const uva_arap_opts* o, void* stream) {
 int uva_arap_profile(uva_plan* plan, double* U, const uva_arap_opts* o, double* launch_ms, int32_t n_launch,
                      void* stream) {
     if (!valid_plan(plan) || !U || !o || !launch_ms) return fail(UVA_EINVAL, "null argument");
-    if (o->iters < 0 || n_launch != 1 + o->iters) return fail(UVA_EINVAL, "n_launch must be iters + 1");
+    if (o->iters < 0 || o->iters_folded < 0) return fail(UVA_EINVAL, "iters must be >= 0");
+    if (n_launch != 1 + std::max(o->iters, o->iters_folded))
+        return fail(UVA_EINVAL, "n_launch must be 1 + max(iters, iters_folded)");
     if (!(o->cg_tol >= 0.0) || (o->cg_tol == 0.0 && o->cg_max_iters <= 0))
         return fail(UVA_EINVAL, "cg_tol must be positive, or 0 with cg_max_iters > 0");
     if (o->precision < UVA_PREC_MIXED || o->precision > UVA_PREC_F64) return fail(UVA_EINVAL, "bad precision");

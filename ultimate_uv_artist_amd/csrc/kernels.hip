@@ -677,6 +677,7 @@ UVA_HD __forceinline__ void run_task(Team& tm, const DevPlan& P, int isl, const 
         if (tm.leader()) {
             st->flips0 = (int)fl;
             st->use_ls = a.line_search && (a.batch || fl == 0.0);
+            st->iters_cap = fl > 0.0 ? a.iters_folded : a.iters_plain;
             st->e_prev = en;
             st->done = 0;
             st->cg_iters = 0;
@@ -691,6 +692,8 @@ UVA_HD __forceinline__ void run_task(Team& tm, const DevPlan& P, int isl, const 
         }
     } else if (KIND == TASK_ARAP_ITER) {
         if (uniform_i(st->done)) return;
+        const int cap = uniform_i(st->iters_cap);
+        if (cap > 0 && uniform_i(st->arap_iters) >= cap) return;
         const long long t_start = uva_clock();
         const double2 b2 = rhs_step(tm, P, d, U);
         // Batch and hier solves cap the CG iterations (40 and 20 by default), so

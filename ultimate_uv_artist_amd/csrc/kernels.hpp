@@ -57,6 +57,8 @@ struct TaskArgs {
     const double* gin;       // per-island upstream gradient (energy backward)
     double* gw;              // per-triangle weight gradient (NULL = not wanted)
     int zero_x;              // CG starts from x = 0 instead of U (adjoint solve)
+    int iters_plain;         // ARAP iterations of islands whose start has no fold (0 = all launched)
+    int iters_folded;        // ... and of islands whose start has one (charts.py:411-415)
 };
 
 hipError_t launch_island_tasks(int kind, int prec, const DevPlan& P, const TaskArgs& a, int slot, int cus,

@@ -43,6 +43,8 @@ struct IslandState {
     int status;
     long long ticks;     // device wall-clock ticks spent on this island in the call
     double reg;          // SLIM regularisation weight 1e-12 * island area
+    int iters_cap;       // ARAP iterations this island runs in the call (0 = all launched)
+    int pad_;
 };
 
 struct DevPlan {

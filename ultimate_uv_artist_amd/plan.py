@@ -137,11 +137,12 @@ class IslandPlan:
         return U
 
     def arap(self, U, iters, line_search=True, mode=_lib.UVA_MODE_REFINE, tol=1e-5, cg_tol=CG_TOL,
-             cg_max_iters=0, precision=_lib.UVA_PREC_MIXED, use_graph=False):
-        """ARAP iterations on U (device tensor) in place."""
+             cg_max_iters=0, precision=_lib.UVA_PREC_MIXED, use_graph=False, iters_folded=0):
+        """ARAP iterations on U (device tensor) in place; with ``iters_folded``
+        > 0, islands whose start has a folded triangle run that many instead."""
         self._check_uv(U)
         o = _lib.ArapOpts(int(iters), int(bool(line_search)), int(mode), int(precision), float(tol), float(cg_tol),
-                          int(cg_max_iters), int(bool(use_graph)))
+                          int(cg_max_iters), int(bool(use_graph)), int(iters_folded))
         check(_lib.lib().uva_arap(self._h, ptr(U), C.byref(o), stream_of(self.dev)), "uva_arap")
         return U
 
