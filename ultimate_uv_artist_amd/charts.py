@@ -34,6 +34,13 @@ from . import _lib, param
 from .plan import IslandPlan, require_gpu
 
 
+# Relative residual of the global steps.  The reference solves them exactly
+# (SuperLU); small charts reach 1e-10 in a few dozen CG iterations, and a
+# folded start's unconstrained iterations amplify a looser solve (1e-6 left
+# 1.8e-4 on a needle chart).
+FLATTEN_CG_TOL = 1e-10
+
+
 def _has_boundary(T):
     return len(param.boundary_edges_oriented(np.asarray(T, np.int64))[0]) > 0
 
@@ -62,7 +69,8 @@ def flatten_many(charts, iters=8, dev=None):
     flip = (neg > pos)[vert_isl]
     U[:, 1] = torch.where(flip, -U[:, 1], U[:, 1])
     if iters:
-        plan.arap(U, iters, line_search=True, mode=_lib.UVA_MODE_REFINE, tol=1e-4, iters_folded=2 * iters)
+        plan.arap(U, iters, line_search=True, mode=_lib.UVA_MODE_REFINE, tol=1e-4, cg_tol=FLATTEN_CG_TOL,
+                  iters_folded=2 * iters)
     Uh = U.cpu().numpy()
     out = [Uh[vo[i]:vo[i + 1]].copy() for i in range(len(charts))]
     # charts that keep folds: sequential repair, then a flip-safe ARAP if it removed them
@@ -75,7 +83,8 @@ def flatten_many(charts, iters=8, dev=None):
     if redo:
         sub = IslandPlan([charts[i] for i in redo], dev=dev)
         U2 = sub.device_uv(np.concatenate([out[i] for i in redo]))
-        sub.arap(U2, max(2, iters // 2), line_search=True, mode=_lib.UVA_MODE_REFINE, tol=1e-4)
+        sub.arap(U2, max(2, iters // 2), line_search=True, mode=_lib.UVA_MODE_REFINE, tol=1e-4,
+                 cg_tol=FLATTEN_CG_TOL)
         U2 = U2.cpu().numpy()
         for k, i in enumerate(redo):
             out[i] = U2[sub.vert_off[k]:sub.vert_off[k + 1]].copy()
