@@ -25,11 +25,12 @@ Only behaviour that the oracle's own checks cannot pin is recorded:
     (engine/hier.py with engine/gpu.py, CPU torch),
   * gpu.Batch.arap at its default CG budget (40 iterations: not converged),
   * charts.flatten on small charts (engine/charts.py with param.py and
-    metrics.py).
+    metrics.py), and charts.boundary_intersections / untangle_boundary
+    (with overlapfix.py and segment.py).
 Output: tests/golden/param_golden.npz, batch_golden.npz, slim_golden.npz,
 validate_golden.npz, hier_golden.npz and batch40_golden.npz
-and flatten_golden.npz (``--only slim|validate|hier|batch40|flatten``
-writes one alone).
+flatten_golden.npz and boundary_golden.npz
+(``--only slim|validate|hier|batch40|flatten|boundary`` writes one alone).
 """
 from __future__ import annotations
 
@@ -234,6 +235,69 @@ def flatten_fixtures(root):
         out[f"q{i}"] = np.array([q["flips"], q["aniso_p95"], q["area_p95"], q["boundary_hits"]], np.float64)
     np.savez_compressed(HERE / "flatten_golden.npz", **out)
     print("flatten_golden.npz", (HERE / "flatten_golden.npz").stat().st_size, "bytes")
+
+
+def _strip(turns, n=60, m=3, width=0.15):
+    """A strip of n x m vertices bent into a spiral of ``turns`` turns
+    (positively oriented triangles; its ends overlap for turns > 1)."""
+    s_, t_ = np.meshgrid(np.arange(n), np.arange(m), indexing="ij")
+    ang = s_.ravel() / (n - 1) * 2 * np.pi * turns
+    rad = 1.0 + width * t_.ravel()
+    U = np.c_[rad * np.cos(ang), rad * np.sin(ang)]
+    v = s_ * m + t_
+    T = []
+    for i in range(n - 1):
+        for j in range(m - 1):
+            a, b, c, d = v[i, j], v[i + 1, j], v[i + 1, j + 1], v[i, j + 1]
+            T += [[a, c, b], [a, d, c]]
+    return U, np.array(T, np.int64)
+
+
+def boundary_fixtures(root):
+    """charts.boundary_intersections and charts.untangle_boundary on spiral
+    strips (global boundary overlaps), disk layouts with a boundary vertex
+    pushed across its neighbours (local crossings) and crossing-free layouts.
+    Output: tests/golden/boundary_golden.npz."""
+    flatten_fixtures_load = sys.modules.get("ref_engine.charts")
+    if flatten_fixtures_load is None:
+        load_validate(root)
+        pkg = sys.modules["ref_engine"]
+        for name in ("param", "charts"):
+            spec = importlib.util.spec_from_file_location(f"ref_engine.{name}", Path(root) / "engine" / f"{name}.py")
+            m = importlib.util.module_from_spec(spec)
+            sys.modules[f"ref_engine.{name}"] = m
+            spec.loader.exec_module(m)
+            setattr(pkg, name, m)
+    pkg = sys.modules["ref_engine"]
+    for name in ("segment", "overlapfix"):
+        spec = importlib.util.spec_from_file_location(f"ref_engine.{name}", Path(root) / "engine" / f"{name}.py")
+        m = importlib.util.module_from_spec(spec)
+        sys.modules[f"ref_engine.{name}"] = m
+        spec.loader.exec_module(m)
+        setattr(pkg, name, m)
+    rc = sys.modules["ref_engine.charts"]
+    cases = [_strip(t) for t in (0.8, 1.03, 1.1, 1.3)]
+    g = np.load(HERE / "flatten_golden.npz")
+    rng = np.random.default_rng(800)
+    for i in (0, 3, 9, 20, 30, 41):
+        T = g[f"T{i}"].astype(np.int64)
+        U = g[f"U{i}"].copy()
+        cases.append((U.copy(), T))
+        bi, bj = rc.param.boundary_edges_oriented(T)
+        k = int(rng.integers(len(bi)))
+        v, w = bi[k], bj[(k + 2) % len(bj)]
+        c = U.mean(0)
+        U[v] = U[v] + 1.6 * (U[w] - U[v]) + 0.05 * (U[v] - c)      # over a neighbour's edge
+        cases.append((U, T))
+    out = {"n": np.array(len(cases))}
+    for i, (U, T) in enumerate(cases):
+        out[f"U{i}"], out[f"T{i}"] = U, T.astype(np.int32)
+        out[f"hits{i}"] = np.array(rc.boundary_intersections(U, T))
+        U2, ok = rc.untangle_boundary(U, T)
+        out[f"untangled{i}"], out[f"ok{i}"] = U2, np.array(bool(ok))
+    np.savez_compressed(HERE / "boundary_golden.npz", **out)
+    print("boundary_golden.npz", (HERE / "boundary_golden.npz").stat().st_size, "bytes",
+          [int(out[f"hits{i}"]) for i in range(len(cases))], [bool(out[f"ok{i}"]) for i in range(len(cases))])
 
 
 def load_validate(root):
@@ -459,9 +523,12 @@ if __name__ == "__main__":
         batch40_fixtures(*load_reference(root))
     elif only == "flatten":
         flatten_fixtures(root)
+    elif only == "boundary":
+        boundary_fixtures(root)
     else:
         main(root)
         validate_fixtures(root, load_reference(root)[0])
         hier_fixtures(root, load_reference(root)[0])
         batch40_fixtures(*load_reference(root))
         flatten_fixtures(root)
+        boundary_fixtures(root)

@@ -175,3 +175,20 @@ def test_flatten_small_charts():
         lo = Ur.min(0)
         worst = max(worst, float(np.abs(U - Ur).max() / (Ur.max(0) - lo).max()))
     assert worst <= 1e-10
+
+
+def test_boundary_intersections_and_untangle():
+    """charts.boundary_intersections (crossings counted once per grid cell)
+    and charts.untangle_boundary (local ring smoothing without folds) of the
+    batched flatten against the reference's own results: spiral strips whose
+    ends overlap, disk layouts with a boundary vertex pushed over a
+    neighbour's edge, crossing-free layouts."""
+    from ultimate_uv_artist_amd import charts as gch
+
+    g = dict(np.load(GOLD / "boundary_golden.npz"))
+    for i in range(int(g["n"])):
+        U, T = g[f"U{i}"], g[f"T{i}"].astype(np.int64)
+        assert gch.boundary_intersections(U, T) == int(g[f"hits{i}"])
+        U2, ok = gch.untangle_boundary(U, T)
+        assert ok == bool(g[f"ok{i}"])
+        assert np.abs(U2 - g[f"untangled{i}"]).max() <= 1e-12 * np.abs(U).max()

@@ -574,7 +574,7 @@ def test_flatten_many_and_queue_match_reference():
     for i, (U, q) in enumerate(res):
         Ur, qr = g[f"U{i}"], g[f"q{i}"]
         assert unit_box_err(U, Ur) <= 1e-4
-        assert q["flips"] == qr[0]
+        assert q["flips"] == qr[0] and q["boundary_hits"] == qr[3]
         assert abs(q["aniso_p95"] / qr[1] - 1) <= 0.01 and abs(q["area_p95"] / qr[2] - 1) <= 0.01
     with gch.FlattenQueue(max_wait_ms=20.0) as fq, ThreadPoolExecutor(4) as ex:
         out = list(ex.map(lambda it: fq.flatten(*it), items))

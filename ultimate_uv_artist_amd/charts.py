@@ -14,11 +14,9 @@ several threads (grow.validate, merge_islands, chart_hier; charts.py:401-438).
   a second flip-safe ARAP of max(2, iters // 2)
   iterations when the repair removed them (charts.py:416-421)
                                                          uva_repair_seq, uva_arap
-  chart_quality (charts.py:432)                          host
+  boundary crossings and their local untangle,
+  chart_quality (charts.py:428-437)                      host (small charts)
 
-The boundary-intersection test and its local untangle (charts.py:428-431)
-are not part of this build: ``q["boundary_hits"]`` is None and callers that
-need them apply the reference's functions to the returned layouts.
 ``FlattenQueue`` gathers flatten requests from many threads into such batches.
 """
 from __future__ import annotations
@@ -43,6 +41,128 @@ FLATTEN_CG_TOL = 1e-10
 
 def _has_boundary(T):
     return len(param.boundary_edges_oriented(np.asarray(T, np.int64))[0]) > 0
+
+
+# ------------------------------------------------------------ boundary crossings
+def _cells(A, B, cell, lo, nx, ny):
+    """Grid cells each segment's box covers, as (segment, cell) pairs in
+    segment order (charts.py:555-570)."""
+    mn, mx = np.minimum(A, B), np.maximum(A, B)
+    x0 = np.maximum(((mn[:, 0] - lo[0]) / cell).astype(np.int64), 0)
+    x1 = np.minimum(((mx[:, 0] - lo[0]) / cell).astype(np.int64), nx - 1)
+    y0 = np.maximum(((mn[:, 1] - lo[1]) / cell).astype(np.int64), 0)
+    y1 = np.minimum(((mx[:, 1] - lo[1]) / cell).astype(np.int64), ny - 1)
+    wx, wy = np.maximum(x1 - x0 + 1, 0), np.maximum(y1 - y0 + 1, 0)
+    cnt = wx * wy
+    seg = np.repeat(np.arange(len(A)), cnt)
+    k = np.arange(len(seg)) - np.repeat(np.cumsum(cnt) - cnt, cnt)
+    gx = x0[seg] + k // np.maximum(wy[seg], 1)
+    gy = y0[seg] + k % np.maximum(wy[seg], 1)
+    return seg, gy * nx + gx
+
+
+def _crossing_pairs(A, B, cell, lo, nx, ny, maxk):
+    """Pairs (i, j) of segments that cross properly, tested once per grid cell
+    both cover (a pair in two cells is listed twice), in cell order, then
+    by i, then by j; at most ``maxk`` (charts.py:553-594, overlapfix.py:88-127)."""
+    seg, c = _cells(A, B, cell, lo, nx, ny)
+    o = np.lexsort((seg, c))
+    seg, c = seg[o], c[o]
+    _, start, cnt = np.unique(c, return_index=True, return_counts=True)
+    r = np.arange(len(seg)) - np.repeat(start, cnt)
+    npair = np.repeat(cnt, cnt) - 1 - r
+    first = np.repeat(np.arange(len(seg)), npair)
+    second = first + 1 + np.arange(len(first)) - np.repeat(np.cumsum(npair) - npair, npair)
+    i, j = seg[first], seg[second]
+    Ai, Bi, Aj, Bj = A[i], B[i], A[j], B[j]
+    touch = ((Ai == Aj).all(1) | (Ai == Bj).all(1) | (Bi == Aj).all(1) | (Bi == Bj).all(1))
+    d1, d2 = Bi - Ai, Bj - Aj
+    o1 = d1[:, 0] * (Aj[:, 1] - Ai[:, 1]) - d1[:, 1] * (Aj[:, 0] - Ai[:, 0])
+    o2 = d1[:, 0] * (Bj[:, 1] - Ai[:, 1]) - d1[:, 1] * (Bj[:, 0] - Ai[:, 0])
+    o3 = d2[:, 0] * (Ai[:, 1] - Aj[:, 1]) - d2[:, 1] * (Ai[:, 0] - Aj[:, 0])
+    o4 = d2[:, 0] * (Bi[:, 1] - Aj[:, 1]) - d2[:, 1] * (Bi[:, 0] - Aj[:, 0])
+    t1, t2 = 1e-9 * (d1 * d1).sum(1), 1e-9 * (d2 * d2).sum(1)
+    hit = ~touch & (((o1 > t1) & (o2 < -t1)) | ((o1 < -t1) & (o2 > t1))) & \
+        (((o3 > t2) & (o4 < -t2)) | ((o3 < -t2) & (o4 > t2)))
+    return np.stack([i[hit], j[hit]], 1)[:maxk]
+
+
+def _grid(U, bi, bj):
+    A, B = U[bi], U[bj]
+    lo = U.min(0)
+    ext = U.max(0) - lo
+    L = np.linalg.norm(B - A, axis=1)
+    cell = max(float(np.median(L)) * 4, float(ext.max()) / 512, 1e-30)
+    return A, B, cell, lo, int(ext[0] / cell) + 1, int(ext[1] / cell) + 1
+
+
+def boundary_intersections(U, T, max_report=1000):
+    """Proper crossings between non-adjacent boundary edges of the chart's
+    UV layout, counted once per grid cell both cover (charts.py:597-608)."""
+    bi, bj = param.boundary_edges_oriented(np.asarray(T, np.int64))
+    if len(bi) < 3:
+        return 0
+    U = np.asarray(U, np.float64)
+    return int(len(_crossing_pairs(*_grid(U, bi, bj), max_report)))
+
+
+def untangle_boundary(U, T, rounds=40):
+    """Remove boundary self-crossings by moving the vertices around each
+    crossing toward their 1-ring centroid, never folding a triangle and never
+    shrinking one below 20 % of its area (charts.py:440-544).  Returns
+    (U, success)."""
+    T = np.asarray(T, np.int64)
+    U = np.array(U, np.float64)
+    bi, bj = param.boundary_edges_oriented(T)
+    if len(bi) < 3:
+        return U, True
+    n = len(U)
+    I = np.r_[T[:, 0], T[:, 1], T[:, 2], T[:, 1], T[:, 2], T[:, 0]]
+    J = np.r_[T[:, 1], T[:, 2], T[:, 0], T[:, 0], T[:, 1], T[:, 2]]
+    key = np.unique(I * n + J)
+    I, J = key // n, key % n
+    ptr = np.searchsorted(I, np.arange(n + 1))
+    deg = np.diff(ptr)
+    vtri = [[] for _ in range(n)]
+    for t, row in enumerate(T.tolist()):
+        for v in row:
+            vtri[v].append(t)
+    for r in range(rounds):
+        hits = _crossing_pairs(*_grid(U, bi, bj), 512)
+        if len(hits) == 0:
+            return U, True
+        moved = False
+        for sel in (hits, hits[:1]):
+            seeds = np.unique(np.r_[bi[sel[:, 0]], bj[sel[:, 0]], bi[sel[:, 1]], bj[sel[:, 1]]])
+            reg = np.zeros(n, bool)
+            reg[seeds] = True
+            cur = seeds
+            for _ in range(1 + r // 8):                      # rings around the seeds
+                nb = np.unique(np.concatenate([J[ptr[v]:ptr[v + 1]] for v in cur])) if len(cur) else cur
+                cur = nb[~reg[nb]]
+                reg[cur] = True
+            rv = np.nonzero(reg)[0]
+            tris = np.unique(np.concatenate([vtri[v] for v in rv]).astype(np.int64))
+            Tt = T[tris]
+            sa0 = param.signed_areas(U, Tt)
+            was_ok = sa0 > 0
+            mean = np.zeros((n, 2))
+            has = deg > 0
+            if has.any():
+                mean[has] = np.add.reduceat(U[J], ptr[:-1][has], axis=0) / deg[has][:, None]
+            for step in (0.5, 0.25, 0.1):
+                Un = U.copy()
+                m = reg & (deg > 0)
+                Un[m] = (1 - step) * U[m] + step * mean[m]
+                if ((param.signed_areas(Un, Tt) > 0.2 * sa0) | ~was_ok).all():
+                    U = Un
+                    moved = True
+                    break
+            if moved:
+                break
+        if not moved:
+            return U, False
+    return U, False
 
 
 def flatten_many(charts, iters=8, dev=None):
@@ -90,9 +210,15 @@ def flatten_many(charts, iters=8, dev=None):
             out[i] = U2[sub.vert_off[k]:sub.vert_off[k + 1]].copy()
     res = []
     for (P, T), Ui in zip(charts, out):
+        # boundary crossings and their local untangle (charts.py:428-437), on the host
+        hits = boundary_intersections(Ui, T)
+        if hits and not (param.signed_areas(Ui, T) <= 0).any():
+            U2, ok = untangle_boundary(Ui, T)
+            if ok:
+                Ui, hits = U2, 0
         q = param.chart_quality(P, T, Ui)
-        q["boundary_hits"] = None
-        q["overlap"] = None
+        q["boundary_hits"] = hits
+        q["overlap"] = 1.0 if hits else 0.0
         res.append((Ui, q))
     return res
 
