@@ -302,9 +302,13 @@ def main():
     kpi = k / np.maximum(iters_run, 1)
     busy = st["busy_ms"]
     n_cu = torch.cuda.get_device_properties(dev).multi_processor_count
+    # workgroup-slot time the islands used: a team holds m slots, a wavefront
+    # task one eighth of one (8 wavefronts per 512-thread workgroup)
+    slot_ms = float(np.sum(np.where(st["members"] > 0, st["members"], 1.0 / 8.0) * busy))
     slow = np.argsort(-busy)[:5]
     print(f"bench: island work {busy.sum():.0f} team-ms per step = {busy.sum() / ms:.1f} teams busy on average "
-          f"({n_cu} CUs); slowest islands (n, CG iters, ms, us per CG iter): " +
+          f"({n_cu} CUs); workgroup slots busy {100.0 * slot_ms / (n_cu * ms):.0f}% of the step; "
+          f"slowest islands (n, CG iters, ms, us per CG iter): " +
           ", ".join(f"({nv[i]}, {int(k[i])}, {busy[i]:.1f}, {1e3 * busy[i] / max(k[i], 1):.1f})" for i in slow),
           flush=True)
     print(f"bench: {ms:.2f} ms/step, {value:.1f} Mtri*iter/s, {gbs:.0f} GB/s algorithmic; CG iters per global "

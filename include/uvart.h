@@ -103,7 +103,8 @@ typedef struct {
     int32_t flips0;          /* triangles with signed area <= 0 at the start */
     int32_t use_ls;          /* line search was active */
     int32_t status;          /* 0, or UVA_ENOBOUNDARY for LSCM */
-    int32_t pad;
+    int32_t members;         /* workgroups of the island's team in the
+                                schedule (0: one wavefront) */
     double rel_resid;        /* last true relative residual (max over u, v) */
     double energy;           /* ARAP energy of the returned layout (REFINE) */
     double step;             /* last line-search step */

@@ -52,7 +52,7 @@ class PlanInfo(C.Structure):
 class IslandStats(C.Structure):
     _fields_ = [("cg_iters", C.c_int32), ("cg_iters_max", C.c_int32), ("arap_iters", C.c_int32),
                 ("converged", C.c_int32), ("flips0", C.c_int32), ("use_ls", C.c_int32), ("status", C.c_int32),
-                ("pad", C.c_int32), ("rel_resid", C.c_double), ("energy", C.c_double), ("step", C.c_double),
+                ("members", C.c_int32), ("rel_resid", C.c_double), ("energy", C.c_double), ("step", C.c_double),
                 ("busy_ms", C.c_double)]
 
 

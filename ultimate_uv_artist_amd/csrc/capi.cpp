@@ -48,6 +48,7 @@ struct uva_plan {
     std::vector<int> vinv;                 // caller's vertex id -> plan vertex
     uva::HostPlan sched;                   // island sizes and schedule inputs (for uva_plan_tune)
     int team_alloc = 0;                    // teams the barrier arrays hold
+    std::vector<int> members;              // per island: workgroups of its team (0 = one wavefront)
     double2* ubuf[3] = {nullptr, nullptr, nullptr};  // plan-order copies of the caller's vertex arrays
     int* d_lpin0 = nullptr;
     int* d_lpin1 = nullptr;
@@ -208,6 +209,13 @@ XccInfo xcc_info() {
 }
 
 // Upload a host plan and allocate the scratch of a new device plan.
+// Per island: workgroups of its team in the schedule (0 = a wavefront task).
+std::vector<int> island_members(const uva::HostPlan& hp) {
+    std::vector<int> m((size_t)std::max(hp.n_isl, 0), 0);
+    for (size_t k = 0; k < hp.order_group.size(); ++k) m[hp.order_group[k]] = hp.grp_m[k];
+    return m;
+}
+
 int upload_plan(uva::HostPlan& hp, const double* P, hipStream_t s, uva_plan** out) {
     std::unique_ptr<uva_plan> hold(new (std::nothrow) uva_plan());
     uva_plan* pl = hold.get();
@@ -281,6 +289,7 @@ int upload_plan(uva::HostPlan& hp, const double* P, hipStream_t s, uva_plan** ou
             return bail(UVA_ENOMEM, "team arrays");
         pl->team_alloc = std::max(1, hp.n_teams);
     }
+    pl->members = island_members(hp);
     {
         uva::HostPlan& sc = pl->sched;
         sc.n_isl = hp.n_isl;
@@ -450,7 +459,7 @@ int uva_plan_get_stats(uva_plan* plan, uva_island_stats* stats, void* stream) {
         o.flips0 = st[i].flips0;
         o.use_ls = st[i].use_ls;
         o.status = st[i].status;
-        o.pad = 0;
+        o.members = i < plan->members.size() ? plan->members[i] : 0;
         o.rel_resid = st[i].rel_resid;
         o.energy = st[i].e_prev;
         o.step = st[i].step;
@@ -495,6 +504,7 @@ int uva_plan_tune(uva_plan* plan, void* stream) {
             return fail(UVA_ENOMEM, "team arrays");
         plan->team_alloc = sc.n_teams;
     }
+    plan->members = island_members(sc);
     d.n_group = (int)sc.order_group.size();
     d.n_teams = sc.n_teams;
     d.max_m = sc.max_m;
