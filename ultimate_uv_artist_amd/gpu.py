@@ -152,7 +152,8 @@ class Batch:
         for _ in range(int(iters)):
             b = whole.local_rhs(U, _lib.UVA_MODE_BATCH)
             Un = U.clone()
-            whole.cg_solve(b, Un, tol=cg_tol, max_iters=int(cg_iters), precision=_lib.UVA_PREC_F64)
+            if cg_iters > 0:                                   # no CG iteration: the layout stays
+                whole.cg_solve(b, Un, tol=cg_tol, max_iters=int(cg_iters), precision=_lib.UVA_PREC_F64)
             bad = ~torch.isfinite(Un).all(1)
             if bool(bad.any()):
                 badc = torch.zeros(self.nc, dtype=torch.bool, device=self.dev)

@@ -108,7 +108,8 @@ class Solver:
         for it in range(int(iters)):
             b = whole.local_rhs(U, _lib.UVA_MODE_BATCH)
             Un = U.clone()
-            whole.cg_solve(b, Un, tol=0.0, max_iters=int(cg_iters), precision=_lib.UVA_PREC_F64)
+            if cg_iters > 0:                                   # no CG iteration: the layout stays
+                whole.cg_solve(b, Un, tol=0.0, max_iters=int(cg_iters), precision=_lib.UVA_PREC_F64)
             bad = ~torch.isfinite(Un).all(1)
             if bool(bad.any()):                              # hier.py:500-503
                 badc = torch.zeros(len(charts), dtype=torch.bool, device=self.dev)
