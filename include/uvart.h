@@ -182,6 +182,18 @@ int uva_arap(uva_plan *plan, double *U, const uva_arap_opts *opts, void *stream)
 int uva_arap_profile(uva_plan *plan, double *U, const uva_arap_opts *opts, double *launch_ms, int32_t n_launch,
                      void *stream);
 
+/* Exact UV overlap flag per triangle (validate.py:58-136 _overlaps; with
+ * strict = 1 the zero-tolerance crossings of validate.py:229-299
+ * _overlaps_strict).  U (n_vert x 2 float64), F (n_tri x 3 int64), skip
+ * (n_tri bytes, 1 = left out; may be NULL) and flags (n_tri bytes, out) are
+ * device arrays; grid is the g of the g x g candidate grid over [0,1]^2 and
+ * rel the orientation tolerance relative to squared edge lengths.  Pairs in
+ * one crowded cell are tested one by one, so a layout that stacks most
+ * triangles in one cell costs time quadratic in their number, as in the
+ * reference.  Synchronises the stream. */
+int uva_overlap_flags(const double *U, int64_t n_vert, const int64_t *F, int64_t n_tri, const uint8_t *skip,
+                      double rel, int32_t strict, int32_t grid, uint8_t *flags, void *stream);
+
 /* SLIM (symmetric Dirichlet) options: param.refine(method="slim"). */
 typedef struct uva_slim_opts {
     int32_t iters;           /* SLIM iterations (refine's iters) */

@@ -48,7 +48,21 @@ def lib():
         _lib.uva_emu_step_bwd.argtypes = [C.c_void_p, _dp, _dp, _dp]
         _lib.uva_emu_adjoint_cg.argtypes = [C.c_void_p, _dp, _dp, C.c_double, C.c_int32]
         _lib.uva_emu_set_pins.argtypes = [C.c_void_p, _i64p, C.c_int64, C.c_double]
+        _lib.uva_emu_overlap_flags.argtypes = [_dp, C.c_int64, _i64p, C.c_int64, C.c_void_p, C.c_double, C.c_int32,
+                                               C.c_int32, C.c_void_p]
     return _lib
+
+
+def overlap_flags(U, F, skip, rel, strict, grid):
+    """uva_overlap_flags' algorithm run on the host (same pair test code)."""
+    U = np.ascontiguousarray(U, np.float64)
+    F = np.ascontiguousarray(F, np.int64)
+    sk = None if skip is None else np.ascontiguousarray(skip, np.uint8)
+    out = np.zeros(len(F), np.uint8)
+    lib().uva_emu_overlap_flags(U.ctypes.data_as(_dp), len(U), F.ctypes.data_as(_i64p), len(F),
+                                None if sk is None else sk.ctypes.data_as(C.c_void_p), float(rel), int(strict),
+                                int(grid), out.ctypes.data_as(C.c_void_p))
+    return out.astype(bool)
 
 
 class EmuPlan:

@@ -406,3 +406,38 @@ def test_emu_folded_starts_run_more_iterations():
                               trace=tr)
         assert st["iters"][i] == len(tr)
         assert unit_box_err(U[s], Ur) <= 1e-6
+
+
+def test_emu_overlap_flags_equal_restatement():
+    """The GPU overlap test's pair code (validate.hip, run on the host)
+    flags exactly the triangles the CPU restatement of validate.py flags, on
+    the reference's own validation cases (chart_overlaps grids) and on noisy
+    layouts in both modes (tolerant exact test and zero-tolerance crossings)."""
+    from pathlib import Path
+
+    from emu import overlap_flags
+    from oracle import validate_cpu as vc
+
+    g = dict(np.load(Path(__file__).resolve().parent / "golden" / "validate_golden.npz"))
+    for i in range(len(g["c_names"])):
+        U, T = g[f"c{i}_U"], g[f"c{i}_F"].astype(np.int64)
+        lo = U.min(0)
+        ext = float((U.max(0) - lo).max())
+        Un = (U - lo) / ext * 0.999999
+        u = Un[T]
+        sa = 0.5 * ((u[:, 1, 0] - u[:, 0, 0]) * (u[:, 2, 1] - u[:, 0, 1]) - (u[:, 2, 0] - u[:, 0, 0]) * (u[:, 1, 1] - u[:, 0, 1]))
+        skip = np.abs(sa) <= 1e-14
+        G = int(np.clip(np.sqrt(len(T)) / 2, 4, 2048))
+        assert np.array_equal(overlap_flags(Un, T, skip, 1e-6, 0, G), g[f"c{i}_mask"])
+    from ultimate_uv_artist_amd import workloads
+
+    rng = np.random.default_rng(71)
+    for k in range(4):
+        P, T = workloads.disk_island(rng, 1500)
+        U = (P[:, :2] - P[:, :2].min(0)) / np.ptp(P[:, :2]).max() * 0.999
+        U = U + rng.normal(0, 0.002 * (k + 1), U.shape)
+        U = np.clip(U, 0.0, 1.0)
+        skip = rng.random(len(T)) < 0.05
+        for G in (8, 64):
+            assert np.array_equal(overlap_flags(U, T, skip, 1e-6, 0, G), vc.overlaps(U, T, skip, 1e-6))
+            assert np.array_equal(overlap_flags(U, T, skip, 0.0, 1, G), vc.overlaps_strict(U, T, skip))

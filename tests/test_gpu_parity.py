@@ -582,3 +582,42 @@ def test_flatten_many_and_queue_match_reference():
         assert np.abs(U - Uq).max() <= 1e-9 * np.abs(U).max()
     with pytest.raises(ValueError):
         gch.flatten_many([(np.eye(4, 3), np.array([[0, 1, 2], [0, 2, 1]]))])
+
+
+def test_validation_on_gpu_matches_reference_and_oracle():
+    """validate.validate_uvs (gutter=False) and chart_overlaps with the GPU
+    overlap test: the reference's own reports and masks on its validation
+    cases (tests/golden/validate_golden.npz), and the same flags as the CPU
+    restatement on a 4-island atlas of 60k triangles with noise (both the
+    tolerant test and the zero-tolerance crossings)."""
+    from pathlib import Path
+
+    from oracle import validate_cpu as vc
+    from ultimate_uv_artist_amd import validate, workloads
+
+    g = dict(np.load(Path(__file__).resolve().parent / "golden" / "validate_golden.npz"))
+    keys = [str(k) for k in g["report_keys"]]
+    for i in range(len(g["v_names"])):
+        rep, prob = validate.validate_uvs(g[f"v{i}_P"], g[f"v{i}_F"].astype(np.int64), g[f"v{i}_U"], gutter=False)
+        assert dict(zip(keys, [float(rep.get(k, -1)) for k in keys])) == dict(zip(keys, g[f"v{i}_report"]))
+        assert np.array_equal(prob, g[f"v{i}_problems"])
+    for i in range(len(g["c_names"])):
+        assert np.array_equal(validate.chart_overlaps(g[f"c{i}_U"], g[f"c{i}_F"].astype(np.int64)), g[f"c{i}_mask"])
+    rng = np.random.default_rng(72)
+    Ps, Ts, Us, off = [], [], [], 0
+    for k in range(4):
+        P, T = workloads.disk_island(rng, 15000)
+        U = (P[:, :2] - P[:, :2].min(0)) / np.ptp(P[:, :2]).max() * 0.55 + np.array([0.4 * (k % 2), 0.4 * (k // 2)])
+        Ps.append(P + 5.0 * k)
+        Ts.append(T + off)
+        Us.append(U + rng.normal(0, 0.0005, U.shape))
+        off += len(P)
+    P, T, U = np.concatenate(Ps), np.concatenate(Ts), np.clip(np.concatenate(Us), 0, 1)
+    skip = rng.random(len(T)) < 0.02
+    G = int(np.clip(np.sqrt(len(T)) / 2, 64, 4096))
+    a = validate.overlap_flags(U, T, skip, 1e-6, False, G)
+    assert a.sum() > 100 and np.array_equal(a, vc.overlaps(U, T, skip, 1e-6))
+    assert np.array_equal(validate.overlap_flags(U, T, skip, 0.0, True, G), vc.overlaps_strict(U, T, skip))
+    r1, p1 = validate.validate_uvs(P, T, U, gutter=False)
+    r2, p2 = vc.validate_uvs(P, T, U)
+    assert r1 == r2 and np.array_equal(p1, p2)

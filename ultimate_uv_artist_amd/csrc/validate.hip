@@ -1,0 +1,317 @@
+// Exact UV triangle overlap test of the reference's validation
+// (engine/validate.py:58-136) on the GPU.
+//
+// A pair of triangles overlaps with positive area when an edge of one
+// properly crosses an edge of the other, or a corner or the centroid of one
+// lies strictly inside the other, with orientation tolerances relative to the
+// squared edge lengths and never below float32 storage resolution; corners at
+// identical UV positions count as shared, and three shared corners (a stacked
+// copy) do not overlap.  Candidate pairs come from a uniform g x g grid over
+// the unit square (cells clamped to the grid): each triangle is entered in the
+// cells its box covers and a pair is tested in the first cell both cover, so
+// every pair whose boxes meet is tested exactly once.
+//
+// Kernels: cell ranges and counts per cell (atomics), one-workgroup exclusive
+// scan, fill (atomics; the order inside a cell does not change the result),
+// pair tests with one thread per cell entry.  Arithmetic is plain IEEE double
+// without contraction, in the reference's expression order, so flags match
+// the CPU restatement bit for bit.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <vector>
+
+#include "../../include/uvart.h"
+
+#pragma clang fp contract(off)
+
+namespace uva {
+namespace ov {
+
+constexpr double kUlp = 1.0 / 16777216.0;   // 2^-24: float32 spacing just below 1.0
+
+__host__ __device__ inline double orient(double ax, double ay, double bx, double by, double cx, double cy) {
+    return (bx - ax) * (cy - ay) - (by - ay) * (cx - ax);
+}
+
+// Python's max(a, b): b only when b > a
+__host__ __device__ inline double dmax(double a, double b) { return b > a ? b : a; }
+
+__host__ __device__ inline bool strict_inside(double px, double py, double ax, double ay, double bx, double by,
+                                              double cx, double cy, double rel) {
+    const double l = dmax((bx - ax) * (bx - ax) + (by - ay) * (by - ay),
+                          dmax((cx - bx) * (cx - bx) + (cy - by) * (cy - by), (ax - cx) * (ax - cx) + (ay - cy) * (ay - cy)));
+    const double eps = rel > 0.0 ? dmax(rel * l, 4.0 * kUlp * sqrt(l)) : 0.0;
+    const double d1 = orient(ax, ay, bx, by, px, py);
+    const double d2 = orient(bx, by, cx, cy, px, py);
+    const double d3 = orient(cx, cy, ax, ay, px, py);
+    return (d1 > eps && d2 > eps && d3 > eps) || (d1 < -eps && d2 < -eps && d3 < -eps);
+}
+
+__host__ __device__ inline bool proper_cross(double ax, double ay, double bx, double by, double cx, double cy,
+                                             double dx, double dy, double rel) {
+    const double l1 = (bx - ax) * (bx - ax) + (by - ay) * (by - ay);
+    const double l2 = (dx - cx) * (dx - cx) + (dy - cy) * (dy - cy);
+    const double e1 = dmax(rel * l1, 4.0 * kUlp * sqrt(l1));
+    const double e2 = dmax(rel * l2, 4.0 * kUlp * sqrt(l2));
+    const double o1 = orient(ax, ay, bx, by, cx, cy), o2 = orient(ax, ay, bx, by, dx, dy);
+    const double o3 = orient(cx, cy, dx, dy, ax, ay), o4 = orient(cx, cy, dx, dy, bx, by);
+    return ((o1 > e1 && o2 < -e1) || (o1 < -e1 && o2 > e1)) && ((o3 > e2 && o4 < -e2) || (o3 < -e2 && o4 > e2));
+}
+
+// validate.py:58-91 for triangles s, t
+__host__ __device__ inline bool tri_overlap(const double2* U, const int64_t* F, int64_t s, int64_t t, double rel) {
+    double2 a[3], b[3];
+    for (int k = 0; k < 3; ++k) {
+        a[k] = U[F[3 * s + k]];
+        b[k] = U[F[3 * t + k]];
+    }
+    bool same[3][3];
+    int shared = 0;
+    for (int i = 0; i < 3; ++i) {
+        bool any = false;
+        for (int j = 0; j < 3; ++j) {
+            same[i][j] = a[i].x == b[j].x && a[i].y == b[j].y;
+            any = any || same[i][j];
+        }
+        shared += any ? 1 : 0;
+    }
+    if (shared == 3) return false;
+    for (int i = 0; i < 3; ++i) {
+        const int i1 = (i + 1) % 3;
+        for (int j = 0; j < 3; ++j) {
+            const int j1 = (j + 1) % 3;
+            if (same[i][j] || same[i][j1] || same[i1][j] || same[i1][j1]) continue;
+            if (proper_cross(a[i].x, a[i].y, a[i1].x, a[i1].y, b[j].x, b[j].y, b[j1].x, b[j1].y, rel)) return true;
+        }
+    }
+    for (int i = 0; i < 3; ++i) {
+        if (!(same[i][0] || same[i][1] || same[i][2]) &&
+            strict_inside(a[i].x, a[i].y, b[0].x, b[0].y, b[1].x, b[1].y, b[2].x, b[2].y, rel))
+            return true;
+        if (!(same[0][i] || same[1][i] || same[2][i]) &&
+            strict_inside(b[i].x, b[i].y, a[0].x, a[0].y, a[1].x, a[1].y, a[2].x, a[2].y, rel))
+            return true;
+    }
+    const double cax = (a[0].x + a[1].x + a[2].x) / 3.0, cay = (a[0].y + a[1].y + a[2].y) / 3.0;
+    if (strict_inside(cax, cay, b[0].x, b[0].y, b[1].x, b[1].y, b[2].x, b[2].y, rel)) return true;
+    const double cbx = (b[0].x + b[1].x + b[2].x) / 3.0, cby = (b[0].y + b[1].y + b[2].y) / 3.0;
+    return strict_inside(cbx, cby, a[0].x, a[0].y, a[1].x, a[1].y, a[2].x, a[2].y, rel);
+}
+
+// validate.py:229-254: a proper crossing with zero tolerance (float32-level
+// contacts), for triangles that are not stacked copies
+__host__ __device__ inline bool tri_cross_strict(const double2* U, const int64_t* F, int64_t s, int64_t t) {
+    double2 a[3], b[3];
+    for (int k = 0; k < 3; ++k) {
+        a[k] = U[F[3 * s + k]];
+        b[k] = U[F[3 * t + k]];
+    }
+    bool same[3][3];
+    int shared = 0;
+    for (int i = 0; i < 3; ++i) {
+        bool any = false;
+        for (int j = 0; j < 3; ++j) {
+            same[i][j] = a[i].x == b[j].x && a[i].y == b[j].y;
+            any = any || same[i][j];
+        }
+        shared += any ? 1 : 0;
+    }
+    if (shared == 3) return false;
+    for (int i = 0; i < 3; ++i) {
+        const int i1 = (i + 1) % 3;
+        for (int j = 0; j < 3; ++j) {
+            const int j1 = (j + 1) % 3;
+            if (same[i][j] || same[i][j1] || same[i1][j] || same[i1][j1]) continue;
+            const double o1 = orient(a[i].x, a[i].y, a[i1].x, a[i1].y, b[j].x, b[j].y);
+            const double o2 = orient(a[i].x, a[i].y, a[i1].x, a[i1].y, b[j1].x, b[j1].y);
+            const double o3 = orient(b[j].x, b[j].y, b[j1].x, b[j1].y, a[i].x, a[i].y);
+            const double o4 = orient(b[j].x, b[j].y, b[j1].x, b[j1].y, a[i1].x, a[i1].y);
+            if (o1 * o2 < 0.0 && o3 * o4 < 0.0) return true;
+        }
+    }
+    return false;
+}
+
+__host__ __device__ inline bool pair_hits(const double2* U, const int64_t* F, int64_t s, int64_t t, double rel,
+                                          int strict) {
+    return strict ? tri_cross_strict(U, F, s, t) : tri_overlap(U, F, s, t, rel);
+}
+
+// grid cell of a coordinate: int(x * g) truncated, clamped to [0, g-1]
+__host__ __device__ inline int cell_of(double x, int g) {
+    double v = x * (double)g;
+    if (!(v > 0.0)) return 0;                       // negatives and NaN
+    if (v >= (double)(g - 1)) return g - 1;
+    return (int)v;
+}
+
+__host__ __device__ inline void tri_box(const double2* U, const int64_t* F, int64_t f, int g, int& x0, int& x1,
+                                        int& y0, int& y1) {
+    const double2 a = U[F[3 * f]], b = U[F[3 * f + 1]], c = U[F[3 * f + 2]];
+    x0 = cell_of(fmin(a.x, fmin(b.x, c.x)), g);
+    x1 = cell_of(fmax(a.x, fmax(b.x, c.x)), g);
+    y0 = cell_of(fmin(a.y, fmin(b.y, c.y)), g);
+    y1 = cell_of(fmax(a.y, fmax(b.y, c.y)), g);
+}
+
+#if !defined(UVA_EMULATOR)
+__global__ void boxes_kernel(const double2* U, const int64_t* F, int64_t t, const uint8_t* skip, int g, int4* box,
+                             unsigned long long* cnt) {
+    const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= t) return;
+    int x0, x1, y0, y1;
+    tri_box(U, F, f, g, x0, x1, y0, y1);
+    box[f] = make_int4(x0, x1, y0, y1);
+    if (skip && skip[f]) return;
+    for (int gy = y0; gy <= y1; ++gy)
+        for (int gx = x0; gx <= x1; ++gx) atomicAdd(cnt + (size_t)gy * g + gx, 1ull);
+}
+
+// exclusive scan of n counts into start[0..n] with one workgroup of 1024 threads
+__global__ void scan_kernel(const unsigned long long* cnt, unsigned long long* start, int64_t n) {
+    __shared__ unsigned long long part[1024];
+    const int64_t chunk = (n + 1023) / 1024;
+    const int64_t lo = (int64_t)threadIdx.x * chunk, hi = lo + chunk < n ? lo + chunk : n;
+    unsigned long long s = 0;
+    for (int64_t i = lo; i < hi; ++i) s += cnt[i];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long run = 0;
+        for (int k = 0; k < 1024; ++k) {
+            const unsigned long long v = part[k];
+            part[k] = run;
+            run += v;
+        }
+        start[n] = run;
+    }
+    __syncthreads();
+    unsigned long long run = part[threadIdx.x];
+    for (int64_t i = lo; i < hi; ++i) {
+        start[i] = run;
+        run += cnt[i];
+    }
+}
+
+__global__ void fill_kernel(int64_t t, const uint8_t* skip, int g, const int4* box, unsigned long long* fill,
+                            int64_t* items, int* item_cell) {
+    const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= t || (skip && skip[f])) return;
+    const int4 bx = box[f];
+    for (int gy = bx.z; gy <= bx.w; ++gy)
+        for (int gx = bx.x; gx <= bx.y; ++gx) {
+            const int c = gy * g + gx;
+            const unsigned long long p = atomicAdd(fill + c, 1ull);
+            items[p] = f;
+            item_cell[p] = c;
+        }
+}
+
+__global__ void pairs_kernel(const double2* U, const int64_t* F, int g, const int4* box,
+                             const unsigned long long* start, const int64_t* items, const int* item_cell,
+                             int64_t n_items, double rel, int strict, uint8_t* flag) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n_items) return;
+    const int c = item_cell[p];
+    const int cy = c / g, cx = c % g;
+    const int64_t s = items[p];
+    const int4 bs = box[s];
+    const int64_t end = (int64_t)start[c + 1];
+    for (int64_t q = p + 1; q < end; ++q) {
+        const int64_t t = items[q];
+        const int4 bt = box[t];
+        // once per pair: the first cell both boxes cover
+        if ((bs.x > bt.x ? bs.x : bt.x) != cx || (bs.z > bt.z ? bs.z : bt.z) != cy) continue;
+        if (bs.x > bt.y || bt.x > bs.y || bs.z > bt.w || bt.z > bs.w) continue;
+        if (pair_hits(U, F, s, t, rel, strict)) {
+            flag[s] = 1;
+            flag[t] = 1;
+        }
+    }
+}
+#endif
+
+}  // namespace ov
+}  // namespace uva
+
+#if !defined(UVA_EMULATOR)
+extern "C" int uva_overlap_flags(const double* U, int64_t n_vert, const int64_t* F, int64_t n_tri,
+                                 const uint8_t* skip, double rel, int32_t strict, int32_t grid, uint8_t* flags,
+                                 void* stream) {
+    using namespace uva::ov;
+    if (!U || !F || !flags || n_vert < 0 || n_tri < 0) return UVA_EINVAL;
+    if (grid < 1 || grid > 8192 || !(rel >= 0.0)) return UVA_EINVAL;
+    hipStream_t s = (hipStream_t)stream;
+    if (n_tri == 0) return UVA_OK;
+    if (hipMemsetAsync(flags, 0, (size_t)n_tri, s) != hipSuccess) return UVA_EHIP;
+    const int64_t ncell = (int64_t)grid * grid;
+    int4* box = nullptr;
+    unsigned long long *cnt = nullptr, *start = nullptr;
+    int rc = UVA_OK;
+    auto ok = [&](hipError_t e) {
+        if (e != hipSuccess && rc == UVA_OK) rc = e == hipErrorOutOfMemory ? UVA_ENOMEM : UVA_EHIP;
+        return rc == UVA_OK;
+    };
+    const int B = 256;
+    const unsigned nb = (unsigned)((n_tri + B - 1) / B);
+    if (ok(hipMalloc(&box, (size_t)n_tri * sizeof(int4))) && ok(hipMalloc(&cnt, (size_t)ncell * 8)) &&
+        ok(hipMalloc(&start, (size_t)(ncell + 1) * 8)) && ok(hipMemsetAsync(cnt, 0, (size_t)ncell * 8, s))) {
+        hipLaunchKernelGGL(boxes_kernel, dim3(nb), dim3(B), 0, s, (const double2*)U, F, n_tri, skip, (int)grid, box,
+                           cnt);
+        hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, s, cnt, start, ncell);
+        unsigned long long total = 0;
+        if (ok(hipGetLastError()) &&
+            ok(hipMemcpyAsync(&total, start + ncell, 8, hipMemcpyDeviceToHost, s)) && ok(hipStreamSynchronize(s))) {
+            int64_t* items = nullptr;
+            int* item_cell = nullptr;
+            if (total > 0 && ok(hipMalloc(&items, (size_t)total * 8)) && ok(hipMalloc(&item_cell, (size_t)total * 4)) &&
+                ok(hipMemcpyAsync(cnt, start, (size_t)ncell * 8, hipMemcpyDeviceToDevice, s))) {
+                hipLaunchKernelGGL(fill_kernel, dim3(nb), dim3(B), 0, s, n_tri, skip, (int)grid, box, cnt, items,
+                                   item_cell);
+                const unsigned np = (unsigned)((total + B - 1) / B);
+                hipLaunchKernelGGL(pairs_kernel, dim3(np), dim3(B), 0, s, (const double2*)U, F, (int)grid, box, start,
+                                   items, item_cell, (int64_t)total, rel, strict, flags);
+                ok(hipGetLastError());
+                ok(hipStreamSynchronize(s));
+            }
+            if (items) (void)hipFree(items);
+            if (item_cell) (void)hipFree(item_cell);
+        }
+    }
+    if (box) (void)hipFree(box);
+    if (cnt) (void)hipFree(cnt);
+    if (start) (void)hipFree(start);
+    return rc;
+}
+#else
+// The same algorithm on the host, one pair at a time (test-only emulation).
+extern "C" int uva_emu_overlap_flags(const double* U, int64_t n_vert, const int64_t* F, int64_t n_tri,
+                                     const uint8_t* skip, double rel, int32_t strict, int32_t grid, uint8_t* flags) {
+    using namespace uva::ov;
+    (void)n_vert;
+    const int g = grid;
+    std::vector<int> x0(n_tri), x1(n_tri), y0(n_tri), y1(n_tri);
+    std::vector<std::vector<int64_t>> cells((size_t)g * g);
+    for (int64_t f = 0; f < n_tri; ++f) {
+        flags[f] = 0;
+        tri_box((const double2*)U, F, f, g, x0[f], x1[f], y0[f], y1[f]);
+        if (skip && skip[f]) continue;
+        for (int gy = y0[f]; gy <= y1[f]; ++gy)
+            for (int gx = x0[f]; gx <= x1[f]; ++gx) cells[(size_t)gy * g + gx].push_back(f);
+    }
+    for (int c = 0; c < g * g; ++c) {
+        const auto& it = cells[c];
+        for (size_t p = 0; p < it.size(); ++p)
+            for (size_t q = p + 1; q < it.size(); ++q) {
+                const int64_t s = it[p], t = it[q];
+                if (std::max(x0[s], x0[t]) != c % g || std::max(y0[s], y0[t]) != c / g) continue;
+                if (x0[s] > x1[t] || x0[t] > x1[s] || y0[s] > y1[t] || y0[t] > y1[s]) continue;
+                if (pair_hits((const double2*)U, F, s, t, rel, strict)) flags[s] = flags[t] = 1;
+            }
+    }
+    return 0;
+}
+#endif
