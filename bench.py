@@ -49,11 +49,12 @@ def parse():
                     help="keep the size-based schedule (no re-planning from measured iterations)")
     ap.add_argument("--no-plain", dest="plain", action="store_false",
                     help="skip the plain per-island GPU baseline")
-    ap.add_argument("--workload", default="disks", choices=["disks", "heightfield", "needles"],
+    ap.add_argument("--workload", default="disks", choices=["disks", "heightfield", "needles", "validate"],
                     help="disks: the throughput workload; heightfield: 3,988 islands of 50-1,946 triangles "
                          "cut from a 2M-triangle heightfield (use --iters 15); needles: the precision study on "
                          "256 islands of 5k-60k triangles with 10%% needle triangles (one JSON line with every "
-                         "precision variant)")
+                         "precision variant); validate: validate_uvs of the 512 solved islands packed "
+                         "into one atlas, GPU overlap test against the CPU restatement on a sample")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB 5 times (HBM counter calibration)")
     return ap.parse_args()
@@ -199,6 +200,64 @@ def precision_study(args):
                       "sample_islands_triangles": [int(tri[i]) for i in sample], "variants": out}), flush=True)
 
 
+def validate_study(args):
+    """validate.validate_uvs (gutter=False) of the 512 disk islands after LSCM
+    + ARAP, packed into a k x k grid of the unit square (3.8M triangles):
+    time of the whole check with the GPU overlap test, of the overlap kernel
+    alone, and of the CPU restatement's overlap test on a sample of cells."""
+    import torch
+
+    from oracle import validate_cpu as vc
+    from ultimate_uv_artist_amd import _lib, validate, workloads
+    from ultimate_uv_artist_amd.plan import IslandPlan
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    charts = workloads.disk_islands(seed=2, n_islands=args.islands)
+    plan = IslandPlan(charts, dev=dev)
+    U = torch.zeros(plan.n, 2, dtype=torch.float64, device=dev)
+    plan.lscm(U)
+    plan.arap(U, args.iters, line_search=True, mode=_lib.UVA_MODE_REFINE, tol=0.0, cg_tol=args.cg_tol)
+    Uh = U.cpu().numpy()
+    vo = plan.vert_off
+    k = int(np.ceil(np.sqrt(len(charts))))
+    Ps, Ts, Us, off = [], [], [], 0
+    for i, (P, T) in enumerate(charts):
+        u = Uh[vo[i]:vo[i + 1]]
+        lo = u.min(0)
+        u = (u - lo) / float((u.max(0) - lo).max()) * 0.9 / k + np.array([i % k, i // k]) / k
+        Ps.append(P + 10.0 * i)
+        Ts.append(T + off)
+        Us.append(u)
+        off += len(P)
+    P, T, Ua = np.concatenate(Ps), np.concatenate(Ts), np.concatenate(Us)
+    print(f"bench: atlas of {len(charts)} islands, {len(T)} triangles", flush=True)
+    validate.validate_uvs(P, T, Ua, gutter=False)           # warm-up
+    t0 = time.perf_counter()
+    rep, prob = validate.validate_uvs(P, T, Ua, gutter=False)
+    t_all = time.perf_counter() - t0
+    U32 = np.asarray(Ua, np.float32).astype(np.float64)
+    G = int(np.clip(np.sqrt(len(T)) / 2, 64, 4096))
+    t0 = time.perf_counter()
+    ov = validate.overlap_flags(U32, T, None, 1e-6, False, G)
+    t_ov = time.perf_counter() - t0
+    # the CPU restatement on the first 16 atlas cells' islands
+    nsmp = min(16, len(charts))
+    te = int(sum(len(T_) for _, T_ in charts[:nsmp]))
+    t0 = time.perf_counter()
+    ov_c = vc.overlaps(U32, T[:te], np.zeros(te, bool), 1e-6)
+    t_cpu = time.perf_counter() - t0
+    same = bool(np.array_equal(ov_c, ov[:te]))
+    print(f"bench: validate_uvs {t_all * 1e3:.1f} ms (valid {rep['valid']}, islands {rep['islands']}); overlap test "
+          f"{t_ov * 1e3:.1f} ms = {len(T) / t_ov / 1e6:.1f} Mtri/s on the GPU; CPU restatement {te / t_cpu / 1e6:.3f} "
+          f"Mtri/s on {te} triangles (one core), same flags {same}", flush=True)
+    print(json.dumps({"workload": f"validate_uvs(gutter=False) of {len(charts)} solved disk islands in one atlas",
+                      "triangles": int(len(T)), "validate_ms": round(t_all * 1e3, 1),
+                      "overlap_ms": round(t_ov * 1e3, 1), "overlap_mtri_per_s": round(len(T) / t_ov / 1e6, 2),
+                      "cpu_restatement_mtri_per_s": round(te / t_cpu / 1e6, 4), "cpu_sample_triangles": te,
+                      "flags_equal_on_sample": same, "report": rep}), flush=True)
+
+
 def main():
     args = parse()
     if args.copy_probe:
@@ -210,6 +269,9 @@ def main():
         sys.exit(2)
     if args.workload == "needles":
         precision_study(args)
+        return
+    if args.workload == "validate":
+        validate_study(args)
         return
     import torch
 
