@@ -1,0 +1,129 @@
+"""Host logic of validate.validate_uvs: the per-triangle checks and the island
+labels that the product computes with torch on the GPU, run here on CPU
+torch against the CPU restatement (the same torch operations run on either
+device).  The checks must equal numpy's values bit for bit; the island
+labels must give the same partition."""
+from __future__ import annotations
+
+from pathlib import Path
+
+import numpy as np
+import torch
+
+from oracle import validate_cpu as vc
+from ultimate_uv_artist_amd import validate, workloads
+
+
+def _same_partition(a, b):
+    """Two labelings name the same partition of the triangles."""
+    a, b = np.asarray(a), np.asarray(b)
+    if a.shape != b.shape:
+        return False
+    pairs = np.unique(np.stack([a, b], 1), axis=0)
+    return len(pairs) == len(np.unique(a)) == len(np.unique(b))
+
+
+def _numpy_checks(v, f, U):
+    p0, p1, p2 = v[f[:, 0]], v[f[:, 1]], v[f[:, 2]]
+    a3 = 0.5 * np.linalg.norm(np.cross(p1 - p0, p2 - p0), axis=1)
+    u = U[f]
+    sa = 0.5 * ((u[:, 1, 0] - u[:, 0, 0]) * (u[:, 2, 1] - u[:, 0, 1]) - (u[:, 2, 0] - u[:, 0, 0]) * (u[:, 1, 1] - u[:, 0, 1]))
+    r = 2.0 ** -24
+    edge = np.maximum(np.maximum(np.linalg.norm(u[:, 1] - u[:, 0], axis=1), np.linalg.norm(u[:, 2] - u[:, 1], axis=1)),
+                      np.linalg.norm(u[:, 0] - u[:, 2], axis=1))
+    sliver = np.abs(sa) < r * np.maximum(edge, r)
+    has3d = a3 > 1e-14 * max(float(a3.max()), 1e-30)
+    return a3, sa, sliver, has3d
+
+
+def _atlas(seed=5, k=6):
+    rng = np.random.default_rng(seed)
+    Ps, Ts, Us, off = [], [], [], 0
+    for i in range(k):
+        P, T = workloads.disk_island(rng, int(rng.integers(300, 3000)))
+        P = P * np.exp(rng.normal(0, 3))                    # areas over many decades
+        U = (P[:, :2] - P[:, :2].min(0)) / np.ptp(P[:, :2]).max() * 0.3 + rng.random(2) * 0.6
+        Ps.append(P)
+        Ts.append(T + off)
+        Us.append(U + rng.normal(0, 1e-3, U.shape))
+        off += len(P)
+    return np.concatenate(Ps), np.concatenate(Ts), np.concatenate(Us).astype(np.float32).astype(np.float64)
+
+
+def test_triangle_checks_equal_numpy_bitwise():
+    v, f, U = _atlas()
+    rng = np.random.default_rng(1)
+    U[rng.choice(len(U), 20)] = np.nan                     # non-finite corners
+    U[f[5]] = U[f[5, 0]]                                   # a zero-area triangle
+    v[f[7, 2]] = v[f[7, 0]]                                # a 3-D degenerate one
+    s3, sa, e2 = (t.numpy() for t in validate._triangle_terms(torch.as_tensor(v), torch.as_tensor(f),
+                                                                torch.as_tensor(U)))
+    a3, sliver, has3d = validate._triangle_checks(s3, sa, e2)
+    for g, w in zip((a3, sa, sliver, has3d), _numpy_checks(v, f, U)):
+        assert g.dtype == w.dtype and np.array_equal(g, w, equal_nan=g.dtype.kind == "f")
+    assert np.isnan(sa).sum() > 0 and sliver.sum() > 0 and (~has3d).sum() > 0
+
+
+def test_uv_islands_partition_equals_restatement():
+    v, f, U = _atlas()
+    U32 = U.astype(np.float32)
+    isl, n = validate._uv_islands_dev(torch.as_tensor(f), len(v), torch.as_tensor(U32), torch.as_tensor(v).float())
+    ref, nr = vc.uv_islands(f, len(v), U, v)
+    assert n == nr == 6 and _same_partition(isl.numpy(), ref)
+    # without positions and UVs: shared vertex ids only
+    isl, n = validate._uv_islands_dev(torch.as_tensor(f), len(v))
+    ref, nr = vc.uv_islands(f, len(v))
+    assert n == nr and _same_partition(isl.numpy(), ref)
+
+
+def test_uv_islands_merges_bitwise_equal_vertices_only():
+    # two triangles on distinct vertex ids that meet at one corner (vertex 0
+    # and vertex 3 have the same position and UV): one island; with -0.0
+    # against 0.0 in that corner's u, two (the reference compares bytes)
+    v = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 0], [-1, 0, 0], [0, -1, 0]], np.float64)
+    U = np.array([[0, 0], [1, 0], [0, 1], [0, 0], [0.5, 0.5], [0.2, 0.7]], np.float64)
+    f = np.array([[0, 1, 2], [3, 4, 5]], np.int64)
+    Um = U.copy()
+    Um[3, 0] = -0.0
+    for Uq, want in ((U, 1), (Um, 2)):
+        isl, n = validate._uv_islands_dev(torch.as_tensor(f), 6, torch.as_tensor(Uq.astype(np.float32)),
+                                          torch.as_tensor(v).float())
+        ref, nr = vc.uv_islands(f, 6, Uq, v)
+        assert n == nr == want and _same_partition(isl.numpy(), ref)
+
+
+def test_components_against_scipy_on_hard_graphs():
+    import scipy.sparse as sp
+    from scipy.sparse.csgraph import connected_components
+
+    rng = np.random.default_rng(3)
+    for n, kind in ((5000, "path"), (4000, "forest"), (3000, "random"), (10, "isolated")):
+        perm = rng.permutation(n)                          # labels in random order along the path
+        if kind == "path":
+            F = np.stack([perm[:-2], perm[1:-1], perm[2:]], 1)
+        elif kind == "forest":
+            F = np.stack([perm[:-2], perm[1:-1], perm[2:]], 1)[rng.random(n - 2) < 0.7]
+        elif kind == "random":
+            F = rng.integers(0, n, (n // 3, 3))
+        else:
+            F = np.array([[0, 1, 2]])
+        F = F.astype(np.int64)
+        lab = validate._components(torch.as_tensor(F), n).numpy()
+        a = np.concatenate([F[:, 0], F[:, 0]])
+        b = np.concatenate([F[:, 1], F[:, 2]])
+        G = sp.coo_matrix((np.ones(len(a)), (a, b)), shape=(n, n))
+        _, ref = connected_components(G, directed=False)
+        assert _same_partition(lab, ref)
+        # every label is the smallest vertex id of its component
+        assert np.array_equal(lab, np.array([np.flatnonzero(ref == ref[i]).min() for i in range(n)])) if n <= 10 \
+            else (lab <= np.arange(n)).all()
+
+
+def test_uv_islands_on_golden_layouts():
+    g = dict(np.load(Path(__file__).resolve().parent / "golden" / "validate_golden.npz"))
+    for i in range(len(g["v_names"])):
+        P, F, U = g[f"v{i}_P"], g[f"v{i}_F"].astype(np.int64), g[f"v{i}_U"]
+        isl, n = validate._uv_islands_dev(torch.as_tensor(F), len(P), torch.as_tensor(np.asarray(U, np.float32)),
+                                          torch.as_tensor(np.asarray(P, np.float64)).float())
+        ref, nr = vc.uv_islands(F, len(P), np.asarray(U, np.float32).astype(np.float64), P)
+        assert n == nr and _same_partition(isl.numpy(), ref)

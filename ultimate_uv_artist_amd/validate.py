@@ -22,6 +22,16 @@ from . import _lib
 from .plan import require_gpu, stream_of
 
 
+def _flags_dev(U, F, skip, rel, strict, grid, dev):
+    """Overlap flags (uint8 device tensor) of device tensors U (n,2) float64
+    and F (t,3) int64, skip (t,) uint8 or None."""
+    out = torch.empty(len(F), dtype=torch.uint8, device=dev)
+    _lib.check(_lib.lib().uva_overlap_flags(U.data_ptr(), len(U), F.data_ptr(), len(F),
+                                            None if skip is None else skip.data_ptr(), float(rel), int(bool(strict)),
+                                            int(grid), out.data_ptr(), stream_of(dev)), "uva_overlap_flags")
+    return out
+
+
 def overlap_flags(U, F, skip=None, rel=1e-6, strict=False, grid=64, dev=None):
     """Per-triangle flag: the triangle overlaps another with positive area
     (``strict``: crosses another with zero tolerance).  U (n,2) in the unit
@@ -33,14 +43,102 @@ def overlap_flags(U, F, skip=None, rel=1e-6, strict=False, grid=64, dev=None):
         raise ValueError("F holds vertex ids outside [0, len(U))")
     if skip is not None and len(skip) != len(F):
         raise ValueError("skip needs one entry per triangle")
-    U = torch.as_tensor(U, device=dev)
-    Ft = torch.as_tensor(F, device=dev)
     sk = None if skip is None else torch.as_tensor(np.ascontiguousarray(skip, np.uint8), device=dev)
-    out = torch.empty(len(Ft), dtype=torch.uint8, device=dev)
-    _lib.check(_lib.lib().uva_overlap_flags(U.data_ptr(), len(U), Ft.data_ptr(), len(Ft),
-                                            None if sk is None else sk.data_ptr(), float(rel), int(bool(strict)),
-                                            int(grid), out.data_ptr(), stream_of(dev)), "uva_overlap_flags")
+    out = _flags_dev(torch.as_tensor(U, device=dev), torch.as_tensor(F, device=dev), sk, rel, strict, grid, dev)
     return out.cpu().numpy().astype(bool)
+
+
+def _unique_rows(key):
+    """Group ids of equal rows of an integer (n, c) tensor: ids follow the
+    lexicographic order of the rows (stable argsort per column, last column
+    first)."""
+    n = key.shape[0]
+    idx = torch.arange(n, device=key.device)
+    for c in reversed(range(key.shape[1])):
+        idx = idx[torch.argsort(key[idx, c], stable=True)]
+    srt = key[idx]
+    new = torch.ones(n, dtype=torch.bool, device=key.device)
+    if n > 1:
+        new[1:] = (srt[1:] != srt[:-1]).any(1)
+    gid = torch.cumsum(new, 0) - 1
+    inv = torch.empty(n, dtype=torch.int64, device=key.device)
+    inv[idx] = gid
+    return inv, (int(gid[-1]) + 1 if n else 0)
+
+
+def _components(F, n):
+    """Connected components of the vertex graph of triangles F (t,3) int64
+    over n vertices: per vertex the smallest vertex id of its component.
+    Rounds of hooking every root to the smallest root across an edge, then
+    pointer jumping to the roots; each round at least halves the number of
+    components that still have an edge between them."""
+    lab = torch.arange(n, device=F.device)
+    a = torch.cat([F[:, 0], F[:, 0]])
+    b = torch.cat([F[:, 1], F[:, 2]])
+    while True:
+        la, lb = lab[a], lab[b]
+        d = la != lb
+        if not bool(d.any()):
+            return lab
+        la, lb = la[d], lb[d]
+        m = torch.minimum(la, lb)
+        lab.scatter_reduce_(0, la, m, "amin")      # every label is a root here: parents only decrease
+        lab.scatter_reduce_(0, lb, m, "amin")
+        while True:
+            nxt = lab[lab]
+            if torch.equal(nxt, lab):
+                break
+            lab = nxt
+
+
+def _uv_islands_dev(F, n_vertices, U32=None, P32=None):
+    """uv_islands on tensors (any device): F (t,3) int64, U32 (n,2) and P32
+    (n,3) float32.  Vertices with bitwise-equal float32 position and UV (the
+    reference compares the 20 bytes) are merged first."""
+    if U32 is not None and P32 is not None:
+        key = torch.cat([P32.contiguous().view(torch.int32), U32.contiguous().view(torch.int32)], 1)
+        inv, n_vertices = _unique_rows(key)
+        F = inv[F]
+    if len(F) == 0:
+        return torch.zeros(0, dtype=torch.int64, device=F.device), 0
+    root = _components(F, int(n_vertices))[F[:, 0]]
+    _, isl = torch.unique(root, return_inverse=True)
+    return isl, int(isl.max()) + 1
+
+
+def _triangle_terms(v, f, U):
+    """Per triangle, on tensors (any device): the squared norm of the 3-D
+    cross product, the signed UV area and the largest squared UV edge
+    length, in numpy's operation order (np.cross; np.linalg.norm sums the
+    squares left to right), so the values equal numpy's bit for bit.  The
+    square roots are left to the host (``_triangle_checks``): torch's CPU
+    sqrt is not correctly rounded."""
+    p0, p1, p2 = v[f[:, 0]], v[f[:, 1]], v[f[:, 2]]
+    e, g = p1 - p0, p2 - p0
+    c0 = e[:, 1] * g[:, 2] - e[:, 2] * g[:, 1]
+    c1 = e[:, 2] * g[:, 0] - e[:, 0] * g[:, 2]
+    c2 = e[:, 0] * g[:, 1] - e[:, 1] * g[:, 0]
+    s3 = c0 * c0 + c1 * c1 + c2 * c2
+    u0, u1, u2 = U[f[:, 0]], U[f[:, 1]], U[f[:, 2]]
+    sa = 0.5 * ((u1[:, 0] - u0[:, 0]) * (u2[:, 1] - u0[:, 1]) - (u2[:, 0] - u0[:, 0]) * (u1[:, 1] - u0[:, 1]))
+
+    def sq(d):
+        return d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]
+
+    e2 = torch.maximum(torch.maximum(sq(u1 - u0), sq(u2 - u1)), sq(u0 - u2))
+    return s3, sa, e2
+
+
+def _triangle_checks(s3, sa, e2):
+    """3-D area, sub-float32 sliver and has-3-D-area masks of
+    validate.py:148-160 from ``_triangle_terms`` (numpy arrays).  The longest
+    edge is sqrt of the largest squared length: a correctly rounded sqrt is
+    monotone, so this equals the largest of the three norms."""
+    a3 = 0.5 * np.sqrt(s3)
+    r = 2.0 ** -24
+    sliver = np.abs(sa) < r * np.maximum(np.sqrt(e2), r)
+    has3d = a3 > 1e-14 * max(float(a3.max()) if len(a3) else 0.0, 1e-30)
+    return a3, sliver, has3d
 
 
 def chart_overlaps(U, T):
@@ -61,27 +159,19 @@ def chart_overlaps(U, T):
     return overlap_flags(Un, T, np.abs(sa) <= 1e-14, 1e-6, False, G)
 
 
-def uv_islands(faces, n_vertices, uv=None, vertices=None):
+def uv_islands(faces, n_vertices, uv=None, vertices=None, dev=None):
     """Islands = faces connected through shared vertices; with ``uv`` and
     ``vertices``, vertices with identical float32 position and UV count as one
     (metrics.py:76-103).  Returns (island id per face, island count); ids are
     the reference's up to a renaming."""
-    import scipy.sparse as sp
-    from scipy.sparse.csgraph import connected_components
-
-    F = np.asarray(faces, np.int64)
+    dev = require_gpu(dev)
+    F = torch.as_tensor(np.ascontiguousarray(faces, np.int64).reshape(-1, 3), device=dev)
+    U32 = P32 = None
     if uv is not None and vertices is not None:
-        key = np.concatenate([np.asarray(vertices, np.float32), np.asarray(uv, np.float32)], 1)
-        key = np.ascontiguousarray(key).view(np.dtype((np.void, 20))).reshape(-1)
-        _, inv = np.unique(key, return_inverse=True)
-        F = inv.reshape(-1)[F]
-        n_vertices = int(inv.max()) + 1
-    if len(F) == 0:
-        return np.zeros(0, np.int32), 0
-    A = sp.csr_matrix((np.ones(3 * len(F)), (np.repeat(np.arange(len(F)), 3), F.reshape(-1))),
-                      shape=(len(F), int(n_vertices)))
-    n, lab = connected_components(A @ A.T, directed=False)
-    return lab.astype(np.int32), int(n)
+        U32 = torch.as_tensor(np.ascontiguousarray(uv, np.float32).reshape(-1, 2), device=dev)
+        P32 = torch.as_tensor(np.ascontiguousarray(vertices, np.float32).reshape(-1, 3), device=dev)
+    isl, n = _uv_islands_dev(F, n_vertices, U32, P32)
+    return isl.to(torch.int32).cpu().numpy(), n
 
 
 def validate_uvs(v, f, uv, atlas=4096, padding=None, in_v=None, in_f=None, in_normals=None, out_normals=None,
@@ -90,24 +180,33 @@ def validate_uvs(v, f, uv, atlas=4096, padding=None, in_v=None, in_f=None, in_no
     per-triangle problem mask).  ``gutter=True`` is not supported."""
     if gutter:
         raise NotImplementedError("the gutter measurement is not part of this build; call with gutter=False")
-    v = np.asarray(v, np.float64)
-    f = np.asarray(f, np.int64)
-    U = np.asarray(uv, np.float32).astype(np.float64)
+    dev = require_gpu(None)
+    v = np.ascontiguousarray(v, np.float64).reshape(-1, 3)
+    f = np.ascontiguousarray(f, np.int64).reshape(-1, 3)
+    U32 = np.ascontiguousarray(uv, np.float32).reshape(-1, 2)
+    if len(f) and (f.min() < 0 or f.max() >= len(v) or f.max() >= len(U32)):
+        raise ValueError("f holds vertex ids outside the vertex arrays")
+    vd = torch.as_tensor(v, device=dev)
+    fd = torch.as_tensor(f, device=dev)
+    U32d = torch.as_tensor(U32, device=dev)
+    Ud = U32d.double()
     rep = {}
-    rep["non_finite_uv"] = int((~np.isfinite(U)).any(1).sum())
-    rep["uv_outside_unit_square"] = int(((U < 0) | (U > 1)).any(1).sum())
-    p0, p1, p2 = v[f[:, 0]], v[f[:, 1]], v[f[:, 2]]
-    a3 = 0.5 * np.linalg.norm(np.cross(p1 - p0, p2 - p0), axis=1)
-    u = U[f]
-    sa = 0.5 * ((u[:, 1, 0] - u[:, 0, 0]) * (u[:, 2, 1] - u[:, 0, 1]) - (u[:, 2, 0] - u[:, 0, 0]) * (u[:, 1, 1] - u[:, 0, 1]))
-    f32_res = 2.0 ** -24
-    edge = np.maximum(np.maximum(np.linalg.norm(u[:, 1] - u[:, 0], axis=1), np.linalg.norm(u[:, 2] - u[:, 1], axis=1)),
-                      np.linalg.norm(u[:, 0] - u[:, 2], axis=1))
-    sliver = np.abs(sa) < f32_res * np.maximum(edge, f32_res)
-    has3d = a3 > 1e-14 * max(float(a3.max()) if len(a3) else 0.0, 1e-30)
+    rep["non_finite_uv"] = int((~torch.isfinite(Ud)).any(1).sum())
+    rep["uv_outside_unit_square"] = int(((Ud < 0) | (Ud > 1)).any(1).sum())
+    s3d, sad, e2d = _triangle_terms(vd, fd, Ud)
+    sa = sad.cpu().numpy()
+    _, sliver, has3d = _triangle_checks(s3d.cpu().numpy(), sa, e2d.cpu().numpy())
     rep["degenerate_uv_triangles"] = int((has3d & (sa == 0)).sum())
     rep["sub_float32_slivers"] = int((has3d & sliver).sum())
-    isl, n = uv_islands(f, len(v), U, v)
+    isld, n = _uv_islands_dev(fd, len(v), U32d, vd.float())
+    skipd = torch.as_tensor((~has3d | sliver | ~np.isfinite(sa)).astype(np.uint8), device=dev)
+    G = int(np.clip(np.sqrt(len(f)) / 2, 64, 4096))
+    Uc = torch.where(torch.isfinite(Ud), Ud, torch.zeros_like(Ud))   # only skipped triangles read a zeroed corner
+    ovd = _flags_dev(Uc, fd, skipd, 1e-6, False, G, dev)
+    ovsd = _flags_dev(Uc, fd, skipd, 0.0, True, G, dev)
+    # per-island sums in triangle order (np.bincount), as the reference adds them
+    isl = isld.cpu().numpy()
+    ov, ov_strict = ovd.cpu().numpy().astype(bool), ovsd.cpu().numpy().astype(bool)
     w = np.abs(sa)
     pos = np.bincount(isl, weights=np.where(sa > 0, w, 0), minlength=n)
     neg = np.bincount(isl, weights=np.where(sa < 0, w, 0), minlength=n)
@@ -116,12 +215,7 @@ def validate_uvs(v, f, uv, atlas=4096, padding=None, in_v=None, in_f=None, in_no
     rep["flipped_triangles"] = int(flipped.sum())
     rep["mirrored_islands"] = int((neg > pos).sum())
     rep["islands"] = int(n)
-    skip = ~has3d | sliver | ~np.isfinite(sa)
-    G = int(np.clip(np.sqrt(len(f)) / 2, 64, 4096))
-    Uc = np.where(np.isfinite(U), U, 0.0)            # skipped triangles only: their cells do not matter
-    ov = overlap_flags(Uc, f, skip, 1e-6, False, G)
     rep["overlapping_triangles_exact"] = int(ov.sum())
-    ov_strict = overlap_flags(Uc, f, skip, 0.0, True, G)
     rep["float32_contacts"] = int((ov_strict & ~ov).sum())
     if ov.any():
         rep["islands_with_overlap"] = int(len(np.unique(isl[ov])))
