@@ -37,6 +37,8 @@
  *   uva_repair_jacobi   gpu.py:251-273 Batch.repair_flips
  *   uva_orient_positive gpu.py:275-281 Batch.orient_positive
  *   uva_repair_seq      param.py:478-516 repair_flips (host, exact order)
+ *   uva_overlap_flags   validate.py:94-136 _overlaps, 257-299 _overlaps_strict
+ *   uva_jacobi_pcg      hier.py:565-577 (the CG of Solver.lscm)
  */
 #ifndef UVART_H
 #define UVART_H
@@ -193,6 +195,18 @@ int uva_arap_profile(uva_plan *plan, double *U, const uva_arap_opts *opts, doubl
  * reference.  Synchronises the stream. */
 int uva_overlap_flags(const double *U, int64_t n_vert, const int64_t *F, int64_t n_tri, const uint8_t *skip,
                       double rel, int32_t strict, int32_t grid, uint8_t *flags, void *stream);
+
+/* Jacobi-preconditioned CG on one symmetric n x n matrix, the solve of
+ * hier.py:534-582 Solver.lscm (one system over the u and v of every chart,
+ * coupled by the boundary area term): the CSR arrays (int64 indptr and
+ * indices, float64 data, repeated entries summed) are host memory, b and x
+ * (n float64, x the start on entry and the result on exit) device memory.
+ * At most max_iters iterations; after iteration k (from 0) with
+ * k % check_every == 0 it stops when r.r < tol2 * max(b.b, 1e-30) (the
+ * reference: check_every 50, tol2 1e-12).  *iters_out (may be NULL) gets the
+ * iterations run.  Synchronises the stream. */
+int uva_jacobi_pcg(int64_t n, const int64_t *indptr, const int64_t *indices, const double *data, const double *b,
+                   double *x, int32_t max_iters, int32_t check_every, double tol2, int32_t *iters_out, void *stream);
 
 /* SLIM (symmetric Dirichlet) options: param.refine(method="slim"). */
 typedef struct uva_slim_opts {

@@ -16,6 +16,14 @@ Restated (engine/hier.py):
   quality (flips, scale, per-face aniso,      hier.py:511-530
     area ratio)
   bad_fraction                                hier.py:584-590
+  lscm: Dirichlet on u and v minus the area   hier.py:534-582
+    term on directed boundary edges, two soft
+    pins per chart (extreme start u among its
+    boundary vertices, weight 10 max |W|),
+    Jacobi-PCG from the start layout, at most
+    ``iters`` iterations, stopping when
+    r.r < 1e-12 b.b at every ``check_every``-th
+    iteration (the reference: 50)
 The reference runs in float32 on the GPU; this restatement keeps every rule
 in float64, so it matches the reference to float32 rounding (pinned against
 the reference's own CPU-torch output in tests/golden/hier_golden.npz).
@@ -209,6 +217,68 @@ class Solver:
         flips = np.bincount(ct, weights=((sgn < 0) & ok).astype(float), minlength=nchart)
         self._q = (aniso, arat, ar3, ct, nchart, a3)
         return U, flips, scale
+
+    def lscm(self, faces, CV, nv, chart, nchart, U0, ba, bb, iters=600, check_every=50):
+        """hier.py:534-582.  Where several boundary vertices of a chart share
+        the extreme start u, the largest vertex id is the pin (the last write
+        of the reference's scatter on the CPU)."""
+        fi = np.asarray(faces, np.int64)
+        T = np.asarray(CV, np.int64).reshape(-1, 3)
+        ct = np.asarray(chart, np.int64)
+        w = self.w[fi]
+        I = np.concatenate([T[:, 1], T[:, 2], T[:, 0]])
+        J = np.concatenate([T[:, 2], T[:, 0], T[:, 1]])
+        W = np.concatenate([w[:, 0], w[:, 1], w[:, 2]])
+        bi = np.asarray(ba, np.int64)
+        bj = np.asarray(bb, np.int64)
+        h = np.full(len(bi), 0.5)
+        rows = np.concatenate([I, J, I, J, I + nv, J + nv, I + nv, J + nv, bi, bj, bi + nv, bj + nv])
+        cols = np.concatenate([J, I, I, J, J + nv, I + nv, I + nv, J + nv, bj + nv, bi + nv, bj, bi])
+        vals = np.concatenate([-W, -W, W, W, -W, -W, W, W, -h, h, h, -h])
+        U0f = np.asarray(U0, np.float64).reshape(nv, 2).astype(np.float32).astype(np.float64)
+        cv = np.zeros(nv, np.int64)
+        cv[T.reshape(-1)] = np.repeat(ct, 3)
+        bv = np.unique(bi)
+        key = U0f[bv, 0]
+        mn = np.full(nchart, 1e30)
+        mx = np.full(nchart, -1e30)
+        np.minimum.at(mn, cv[bv], key)
+        np.maximum.at(mx, cv[bv], key)
+        p0 = np.full(nchart, -1, np.int64)
+        p1 = np.full(nchart, -1, np.int64)
+        is_mn, is_mx = key == mn[cv[bv]], key == mx[cv[bv]]
+        np.maximum.at(p0, cv[bv][is_mn], bv[is_mn])
+        np.maximum.at(p1, cv[bv][is_mx], bv[is_mx])
+        pins = np.concatenate([p0[p0 >= 0], p1[p1 >= 0]])
+        wpin = max(float(np.abs(W).max()) if len(W) else 0.0, 1e-12) * 10
+        rows = np.concatenate([rows, pins, pins + nv])
+        cols = np.concatenate([cols, pins, pins + nv])
+        vals = np.concatenate([vals, np.full(2 * len(pins), wpin)])
+        H = sp.csr_matrix((vals, (rows, cols)), shape=(2 * nv, 2 * nv))
+        diag = np.bincount(rows[rows == cols], weights=vals[rows == cols], minlength=2 * nv)
+        dinv = 1.0 / np.maximum(diag, 1e-30)
+        b = np.zeros(2 * nv)
+        b[pins] += wpin * U0f[pins, 0]                  # no accumulation over repeated pins, as torch's b[pins] +=
+        b[pins + nv] += wpin * U0f[pins, 1]
+        x = np.concatenate([U0f[:, 0], U0f[:, 1]])
+        r = b - H @ x
+        z = r * dinv
+        p = z.copy()
+        rz = float(r @ z)
+        b2 = max(float(b @ b), 1e-30)
+        for it in range(int(iters)):
+            Ap = H @ p
+            al = rz / max(float(p @ Ap), 1e-30)
+            x = x + al * p
+            r = r - al * Ap
+            if it % check_every == 0 and float(r @ r) < 1e-12 * b2:
+                break
+            z = r * dinv
+            rz2 = float(r @ z)
+            p = z + (rz2 / max(rz, 1e-30)) * p
+            rz = rz2
+        U = np.stack([x[:nv], x[nv:]], 1)
+        return np.where(np.isfinite(U), U, U0f)
 
     def bad_fraction(self, max_aniso, max_area):
         aniso, arat, ar3, ct, nchart, a3 = self._q

@@ -509,6 +509,64 @@ def hier_fixtures(root, rp):
     print("hier_golden.npz", (HERE / "hier_golden.npz").stat().st_size, "bytes")
 
 
+def directed_boundary(CV, chart, nv):
+    """Directed boundary edges (a, b) of the local triangles: a triangle
+    edge whose reverse no triangle of the same chart holds (hier.py:157-178
+    for charts without cuts)."""
+    CV = np.asarray(CV, np.int64)
+    a = CV.reshape(-1)
+    b = CV[:, [1, 2, 0]].reshape(-1)
+    c = np.repeat(np.asarray(chart, np.int64), 3)
+    key = (c * nv + a) * nv + b
+    rev = (c * nv + b) * nv + a
+    m = ~np.isin(rev, key)
+    return a[m], b[m]
+
+
+def hier_lscm_cases(rp):
+    """Subsets of the hier_cases mesh for Solver.lscm: the three-chart subset
+    from a planar start (the xy projection, one chart rotated) at the default
+    budget and at 60 iterations, and the one-chart subset."""
+    V, F, cases = hier_cases(rp)
+    rng = np.random.default_rng(610)
+    out = []
+    for i, c in enumerate(cases):
+        nv = c["nv"]
+        U0 = np.zeros((nv, 2))
+        U0[c["CV"].reshape(-1)] = V[F[c["faces"]]].reshape(-1, 3)[:, :2]
+        vch = np.zeros(nv, np.int64)
+        vch[c["CV"].reshape(-1)] = np.repeat(c["chart"], 3)
+        th = rng.uniform(0, 2 * np.pi)
+        R = np.array([[np.cos(th), -np.sin(th)], [np.sin(th), np.cos(th)]])
+        U0 = np.where((vch == 0)[:, None], U0 @ R.T, U0)
+        ba, bb = directed_boundary(c["CV"], c["chart"], nv)
+        base = dict(faces=c["faces"], CV=c["CV"], nv=nv, chart=c["chart"], nchart=c["nchart"], U0=U0, ba=ba, bb=bb)
+        out.append(dict(base, iters=600))
+        if i == 0:
+            out.append(dict(base, iters=60))
+    return V, F, out
+
+
+def hier_lscm_fixtures(root, rp):
+    """hier.Solver.lscm on the hier_lscm_cases subsets, run by the reference
+    on CPU torch.  Output: tests/golden/hier_lscm_golden.npz."""
+    import types as _t
+
+    rh = load_hier(root)
+    V, F, cases = hier_lscm_cases(rp)
+    S = rh.Solver(_t.SimpleNamespace(V=V, F=F))
+    out = {"V": V, "F": F.astype(np.int32), "n": np.array(len(cases))}
+    for i, c in enumerate(cases):
+        U = S.lscm(c["faces"], c["CV"], c["nv"], c["chart"], c["nchart"], c["U0"], c["ba"], c["bb"], iters=c["iters"])
+        for k in ("faces", "CV", "chart", "U0", "ba", "bb"):
+            out[f"c{i}_{k}"] = c[k]
+        out[f"c{i}_nv"], out[f"c{i}_nchart"], out[f"c{i}_iters"] = np.array(c["nv"]), np.array(c["nchart"]), \
+            np.array(c["iters"])
+        out[f"c{i}_U"] = U
+    np.savez_compressed(HERE / "hier_lscm_golden.npz", **out)
+    print("hier_lscm_golden.npz", (HERE / "hier_lscm_golden.npz").stat().st_size, "bytes")
+
+
 if __name__ == "__main__":
     only = sys.argv[sys.argv.index("--only") + 1] if "--only" in sys.argv else None
     args = [a for a in sys.argv[1:] if a != "--only" and a != only]
@@ -519,6 +577,8 @@ if __name__ == "__main__":
         validate_fixtures(root, load_reference(root)[0])
     elif only == "hier":
         hier_fixtures(root, load_reference(root)[0])
+    elif only == "hier_lscm":
+        hier_lscm_fixtures(root, load_reference(root)[0])
     elif only == "batch40":
         batch40_fixtures(*load_reference(root))
     elif only == "flatten":

@@ -11,6 +11,7 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
+import emu
 from emu import EmuPlan
 
 # (members, wavefronts per member): a wavefront team, a workgroup team, and
@@ -441,3 +442,59 @@ def test_emu_overlap_flags_equal_restatement():
         for G in (8, 64):
             assert np.array_equal(overlap_flags(U, T, skip, 1e-6, 0, G), vc.overlaps(U, T, skip, 1e-6))
             assert np.array_equal(overlap_flags(U, T, skip, 0.0, 1, G), vc.overlaps_strict(U, T, skip))
+
+
+def test_emu_jacobi_pcg_iteration_and_stop_rule():
+    """uva_jacobi_pcg's row code run on the host: the Jacobi-PCG iterates of
+    the definition (alpha = rz / max(p.Ap, 1e-30), beta = rz' / max(rz,
+    1e-30), diagonal summed over repeated entries) on a random symmetric
+    positive definite matrix with repeated entries, stopping at the first
+    checked iteration (every check_every-th from 0) with r.r < tol2 b.b, or
+    after max_iters."""
+    import scipy.sparse as sp
+
+    rng = np.random.default_rng(77)
+    n = 300
+    i = rng.integers(0, n, 1500)
+    j = rng.integers(0, n, 1500)
+    v = rng.uniform(-1, 0, 1500)
+    A = sp.coo_matrix((np.r_[v, v], (np.r_[i, j], np.r_[j, i])), shape=(n, n)).tocsr()
+    d = np.asarray(abs(A).sum(1)).ravel() + rng.uniform(0.01, 0.1, n)
+    # the diagonal in two entries per row (the kernel sums repeated entries)
+    A = sp.csr_matrix((np.r_[A.tocoo().data, 0.5 * d, 0.5 * d],
+                       (np.r_[A.tocoo().row, np.arange(n), np.arange(n)],
+                        np.r_[A.tocoo().col, np.arange(n), np.arange(n)])), shape=(n, n))
+    A.has_canonical_format = False
+    assert A.nnz > 1500
+    b = rng.normal(size=n)
+    x0 = rng.normal(size=n)
+    As = A.copy()
+    As.sum_duplicates()
+    dinv = 1.0 / As.diagonal()
+
+    def pcg(iters, check, tol2):
+        x = x0.copy()
+        r = b - As @ x
+        z = r * dinv
+        p = z.copy()
+        rz = r @ z
+        for k in range(iters):
+            Ap = As @ p
+            al = rz / max(p @ Ap, 1e-30)
+            x, r = x + al * p, r - al * Ap
+            if k % check == 0 and r @ r < tol2 * max(b @ b, 1e-30):
+                return x, k + 1
+            z = r * dinv
+            rz2 = r @ z
+            p = z + rz2 / max(rz, 1e-30) * p
+            rz = rz2
+        return x, iters
+
+    for iters, check, tol2 in ((25, 50, 1e-12), (2000, 50, 1e-12), (2000, 1, 1e-12), (2000, 7, 1e-20)):
+        x, k = emu.jacobi_pcg(A, b, x0, iters, check, tol2)
+        xr, kr = pcg(iters, check, tol2)
+        assert k == kr and np.abs(x - xr).max() <= 1e-10 * np.abs(xr).max(), (iters, check, k, kr)
+        if iters == 2000 and check > 1 and tol2 > 1e-15:
+            assert k % check == 1 or check == 1
+    x, k = emu.jacobi_pcg(A, b, x0, 0)
+    assert k == 0 and np.array_equal(x, x0)

@@ -162,6 +162,34 @@ def test_hier_relax(i):
     assert np.abs(bA - h[f"c{i}_badA"]).max() <= 1e-6 and np.abs(bR - h[f"c{i}_badR"]).max() <= 1e-6
 
 
+def _lscm_case(g, i):
+    return ((g[f"c{i}_faces"], g[f"c{i}_CV"], int(g[f"c{i}_nv"]), g[f"c{i}_chart"], int(g[f"c{i}_nchart"]),
+             g[f"c{i}_U0"], g[f"c{i}_ba"], g[f"c{i}_bb"]), int(g[f"c{i}_iters"]))
+
+
+@pytest.mark.parametrize("i", [0, 1, 2])
+def test_hier_lscm(i):
+    """hier.Solver.lscm: the float64 restatement against the reference's
+    float32 run.  At 60 iterations the two follow the same iterates (per
+    chart within 1.9e-4 of its extent, float32 rounding, while the converged
+    map is 0.18 away); at the 600-iteration default the reference's float32
+    CG has stalled (its float32 residual never reaches the 1e-6 stop, and a
+    float32 replay of the restatement lands 2.8e-3 of the layout's extent
+    from the converged map, as the reference does), so the two agree to that
+    float32 floor (1.8e-3 and 7e-4 per chart)."""
+    from oracle import hier_cpu
+
+    g = dict(np.load(GOLD / "hier_lscm_golden.npz"))
+    S = hier_cpu.Solver(g["V"], g["F"].astype(np.int64))
+    args, iters = _lscm_case(g, i)
+    U = S.lscm(*args, iters=iters)
+    err = _chart_unit_err(U, g[f"c{i}_U"], args[1], args[3], args[4])
+    assert err <= (3e-4 if iters <= 60 else 3e-3), err
+    # the budget matters: the converged map is far from the 60-iteration layout
+    if iters <= 60:
+        Uc = S.lscm(*args, iters=20000, check_every=1)
+        assert _chart_unit_err(Uc, g[f"c{i}_U"], args[1], args[3], args[4]) > 100 * err
+
 
 def test_flatten_small_charts():
     """charts.flatten (charts.py:401-421): LSCM, mirror, ARAP with doubled

@@ -465,3 +465,56 @@ def test_validate_orientation_and_island_counts():
     assert rep["flipped_triangles"] == 1 and prob[len(T)] and not rep["valid"]
     isl, n = vc.uv_islands(T2, len(P2), U2, P2)
     assert n == 2 and len(set(isl[: len(T)])) == 1 and len(set(isl[len(T):])) == 1
+
+
+def test_hier_lscm_keeps_a_conformal_start_and_moves_a_mirrored_one():
+    """hier.Solver.lscm restatement: a planar chart whose start is a
+    similarity of its own positions is a minimum of the Dirichlet-minus-area
+    energy (zero energy, pins satisfied), so the converged solve returns it;
+    its mirror image is anti-conformal and is moved.  A flipped sign of the
+    area term, or an area term on the wrong boundary orientation, swaps the
+    two outcomes."""
+    from oracle import hier_cpu
+
+    rng = np.random.default_rng(41)
+    P, T = workloads.disk_island(rng, 600)
+    P = P.copy()
+    P[:, 2] = 0.0
+    th, s = 0.7, 1.9
+    R = np.array([[np.cos(th), -np.sin(th)], [np.sin(th), np.cos(th)]]) * s
+    a = T.reshape(-1)
+    b = T[:, [1, 2, 0]].reshape(-1)
+    key = a * len(P) + b
+    m = ~np.isin(b * len(P) + a, key)
+    S = hier_cpu.Solver(P, T)
+    args = (np.arange(len(T)), T, len(P), np.zeros(len(T), np.int64), 1)
+    for mirror, moves in ((False, False), (True, True)):
+        U0 = P[:, :2] @ R.T + np.array([0.3, -0.2])
+        if mirror:
+            U0[:, 0] = -U0[:, 0]
+        U0 = U0.astype(np.float32).astype(np.float64)
+        U = S.lscm(*args, U0, a[m], b[m], iters=20000, check_every=1)
+        err = np.abs(U - U0).max() / np.ptp(U0, 0).max()
+        assert (err > 1e-2) if moves else (err < 1e-7), (mirror, err)
+
+
+def test_hier_lscm_pins_are_extreme_boundary_vertices():
+    """The pins of each chart are the boundary vertices of smallest and
+    largest start u: with a start far from conformal, the converged layout
+    keeps those two vertices near their start (weight 10 max |W|) and no
+    other vertex pinned."""
+    from oracle import hier_cpu
+
+    rng = np.random.default_rng(42)
+    P, T = workloads.disk_island(rng, 400)
+    a = T.reshape(-1)
+    b = T[:, [1, 2, 0]].reshape(-1)
+    m = ~np.isin(b * len(P) + a, a * len(P) + b)
+    U0 = (P[:, :2] * np.array([3.0, 0.5])).astype(np.float32).astype(np.float64)
+    bv = np.unique(a[m])
+    lo, hi = bv[np.argmin(U0[bv, 0])], bv[np.argmax(U0[bv, 0])]
+    S = hier_cpu.Solver(P, T)
+    U = S.lscm(np.arange(len(T)), T, len(P), np.zeros(len(T), np.int64), 1, U0, a[m], b[m], iters=20000,
+               check_every=1)
+    d = np.linalg.norm(U - U0, axis=1) / np.ptp(U0, 0).max()
+    assert d[lo] < 0.02 and d[hi] < 0.02 and np.median(d) > 5 * max(d[lo], d[hi])

@@ -10,7 +10,7 @@ from pathlib import Path
 HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 LIB = HERE / "libuvart.so"
-SOURCES = ["kernels.hip", "capi.cpp", "setup.cpp", "validate.hip"]
+SOURCES = ["kernels.hip", "capi.cpp", "setup.cpp", "validate.hip", "pcg.hip"]
 ARCH = os.environ.get("UVA_OFFLOAD_ARCH", "gfx950")
 
 
@@ -49,7 +49,8 @@ def build_emulator(verbose=False):
     tmp = EMU_LIB.with_suffix(".so.tmp")
     cmd = [hipcc(), "--cuda-host-only", "-O2", "-std=c++17", "-fPIC", "-shared", "-DUVA_EMULATOR", "-pthread",
            "-Wno-unused-command-line-argument", "-I", str(HERE.parent / "include"),
-           str(CSRC / "kernels.hip"), str(CSRC / "setup.cpp"), str(CSRC / "validate.hip"), "-o", str(tmp)]
+           str(CSRC / "kernels.hip"), str(CSRC / "setup.cpp"), str(CSRC / "validate.hip"), str(CSRC / "pcg.hip"),
+           "-o", str(tmp)]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)

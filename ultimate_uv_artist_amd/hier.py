@@ -23,16 +23,53 @@ plan and runs the reference's sequence on the GPU kernels:
   final repair, per-chart quality                   torch on the plan's GPU
 
 Every vertex must belong to a face of the subset (the reference's callers
-pass compacted subsets).  ``lscm`` (the free-boundary soft-pin LSCM that the
-reference keeps behind ``part_lscm=False``) is not part of this build.
+pass compacted subsets).
+
+``lscm`` (hier.py:534-582, the free-boundary conformal start the reference
+uses behind ``part_lscm``) assembles one 2n-row system over the u and v of
+every chart on the host -- the cotangent Dirichlet energy minus the area term
+on the given directed boundary edges, plus two soft pins per chart -- and
+solves it on the GPU with ``uva_jacobi_pcg``, the reference's Jacobi-PCG
+with its stopping rule (600 iterations at most, r.r < 1e-12 b.b checked
+every 50).  That rule stops well short of the exact conformal map on charts
+of a few thousand vertices (the reference's layout is then a few 1e-3 of the
+chart's extent away from it), so the solve keeps the rule rather than
+converging further.
 """
 from __future__ import annotations
 
 import numpy as np
+import scipy.sparse as sp
 import torch
 
 from . import _lib
-from .plan import IslandPlan, require_gpu
+from .plan import IslandPlan, require_gpu, stream_of
+
+
+def _half_cots(V, F):
+    """Half cotangent weights per face (t,3), the weight of corner k on the
+    opposite edge (k+1, k+2), in isometric face frames; zero on faces without
+    area (hier.py:361-385)."""
+    p0, p1, p2 = V[F[:, 0]], V[F[:, 1]], V[F[:, 2]]
+    e1, e2 = p1 - p0, p2 - p0
+    l1 = np.linalg.norm(e1, axis=1)
+    nrm = np.cross(e1, e2)
+    dbl = np.linalg.norm(nrm, axis=1)
+    ok = (dbl > 1e-14 * np.maximum(l1 * l1, 1e-300)) & np.isfinite(dbl) & (l1 > 0)
+    ex = e1 / np.maximum(l1, 1e-30)[:, None]
+    ey = np.cross(nrm / np.maximum(dbl, 1e-30)[:, None], ex)
+    X = np.zeros((len(F), 3, 2))
+    X[:, 1, 0] = l1
+    X[:, 2, 0] = (e2 * ex).sum(1)
+    X[:, 2, 1] = (e2 * ey).sum(1)
+    X[~ok] = 0.0
+    w = np.empty((len(F), 3))
+    for k in range(3):
+        a, b = X[:, (k + 1) % 3] - X[:, k], X[:, (k + 2) % 3] - X[:, k]
+        cr = np.maximum(np.abs(a[:, 0] * b[:, 1] - a[:, 1] * b[:, 0]), 1e-30)
+        w[:, k] = 0.5 * np.clip((a * b).sum(1) / cr, -1e4, 1e4)
+    w[~ok] = 0.0
+    return w
 
 
 class Solver:
@@ -185,6 +222,74 @@ class Solver:
         outR[keep] = (badR / a3).cpu().numpy()
         return outA, outR
 
-    def lscm(self, *args, **kwargs):
-        raise NotImplementedError("hier.Solver.lscm (soft-pin free-boundary LSCM, off by default in the reference) "
-                                  "is not part of this build; use param.lscm")
+    def lscm(self, faces, CV, nv, chart, nchart, U0, ba, bb, iters=600):
+        """Free-boundary conformal map of every chart of the subset at once
+        (hier.py:534-582): ba, bb the directed boundary edges (local ids), U0
+        (nv,2) the start; the pins of a chart are its boundary vertices of
+        smallest and largest start u (on a tie the largest vertex id).
+        Returns (nv,2) float64 holding float32 values, as the reference."""
+        import ctypes as C
+
+        faces = np.asarray(faces, np.int64).reshape(-1)
+        CV = np.asarray(CV, np.int64).reshape(-1, 3)
+        chart = np.asarray(chart, np.int64).reshape(-1)
+        nv, nchart = int(nv), int(nchart)
+        U0f = np.asarray(U0, np.float64).reshape(nv, 2).astype(np.float32).astype(np.float64)
+        bi = np.asarray(ba, np.int64).reshape(-1)
+        bj = np.asarray(bb, np.int64).reshape(-1)
+        if len(bi) != len(bj):
+            raise ValueError("ba and bb need one entry per boundary edge")
+        for name, a, hi in (("CV", CV, nv), ("ba", bi, nv), ("bb", bj, nv), ("chart", chart, nchart),
+                            ("faces", faces, len(self.F))):
+            if len(a) and (a.min() < 0 or a.max() >= hi):
+                raise ValueError(f"{name} holds ids outside [0, {hi})")
+        if nv == 0 or len(faces) == 0:
+            return U0f
+        w = _half_cots(self.V, self.F[faces])
+        I = np.concatenate([CV[:, 1], CV[:, 2], CV[:, 0]])
+        J = np.concatenate([CV[:, 2], CV[:, 0], CV[:, 1]])
+        W = np.concatenate([w[:, 0], w[:, 1], w[:, 2]])
+        h = np.full(len(bi), 0.5)
+        # Dirichlet on u and on v, minus the area term on the boundary
+        rows = [I, J, I, J, I + nv, J + nv, I + nv, J + nv, bi, bj, bi + nv, bj + nv]
+        cols = [J, I, I, J, J + nv, I + nv, I + nv, J + nv, bj + nv, bi + nv, bj, bi]
+        vals = [-W, -W, W, W, -W, -W, W, W, -h, h, h, -h]
+        # two soft pins per chart with a boundary
+        cv = np.zeros(nv, np.int64)
+        cv[CV.reshape(-1)] = np.repeat(chart, 3)
+        bv = np.unique(bi)
+        key = U0f[bv, 0]
+        mn = np.full(nchart, 1e30)
+        mx = np.full(nchart, -1e30)
+        np.minimum.at(mn, cv[bv], key)
+        np.maximum.at(mx, cv[bv], key)
+        p0 = np.full(nchart, -1, np.int64)
+        p1 = np.full(nchart, -1, np.int64)
+        at_mn, at_mx = key == mn[cv[bv]], key == mx[cv[bv]]
+        np.maximum.at(p0, cv[bv][at_mn], bv[at_mn])
+        np.maximum.at(p1, cv[bv][at_mx], bv[at_mx])
+        pins = np.concatenate([p0[p0 >= 0], p1[p1 >= 0]])
+        wpin = max(float(np.abs(W).max()) if len(W) else 0.0, 1e-12) * 10
+        rows += [pins, pins + nv]
+        cols += [pins, pins + nv]
+        vals += [np.full(2 * len(pins), wpin)]
+        H = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(2 * nv, 2 * nv))
+        H.sum_duplicates()
+        b = np.zeros(2 * nv)
+        b[pins] += wpin * U0f[pins, 0]                 # no accumulation over a repeated pin, as the reference
+        b[pins + nv] += wpin * U0f[pins, 1]
+        ip = np.ascontiguousarray(H.indptr, np.int64)
+        ix = np.ascontiguousarray(H.indices, np.int64)
+        data = np.ascontiguousarray(H.data, np.float64)
+        bd = torch.as_tensor(b, device=self.dev)
+        x = torch.as_tensor(np.concatenate([U0f[:, 0], U0f[:, 1]]), device=self.dev)
+        its = C.c_int32(0)
+        with torch.cuda.device(self.dev):
+            _lib.check(_lib.lib().uva_jacobi_pcg(2 * nv, _lib.i64p(ip), _lib.i64p(ix), _lib.f64p(data), bd.data_ptr(),
+                                                 x.data_ptr(), int(iters), 50, 1e-12, C.byref(its),
+                                                 stream_of(self.dev)), "uva_jacobi_pcg")
+        self.lscm_iters = its.value
+        xs = x.cpu().numpy()
+        U = np.stack([xs[:nv], xs[nv:]], 1)
+        U = np.where(np.isfinite(U), U, U0f)
+        return U.astype(np.float32).astype(np.float64)
