@@ -603,11 +603,12 @@ def flatten_lscm_arap(P, T, iters=8, tol=1e-4):
     return U
 
 
-def flatten(P, T, iters=8):
-    """charts.py:401-421 -- flatten_lscm_arap, then, if folds remain, the
+def flatten(P, T, iters=8, slim_iters=0):
+    """charts.py:401-425 -- flatten_lscm_arap, then, if folds remain, the
     sequential repair and, when it removed them, a flip-safe ARAP of
-    max(2, iters // 2) iterations (method "arap", no SLIM; the boundary
-    untangle of charts.py:428-431 is not restated)."""
+    max(2, iters // 2) iterations; with ``slim_iters`` and no fold left,
+    that many SLIM iterations (kept when finite).  Method "arap"; the
+    boundary untangle of charts.py:426-431 is not restated."""
     T = np.asarray(T, np.int64)
     X, area = local_frames(P, T)
     cots = cot_weights(X)
@@ -617,4 +618,8 @@ def flatten(P, T, iters=8):
         if not (signed_areas(U, T) <= 0).any() and iters:
             U, _ = refine(P, T, U, iters=max(2, iters // 2), method="arap", X=X, area=area, cots=cots,
                           line_search=True, tol=1e-4)
+    if slim_iters and not (signed_areas(U, T) <= 0).any():
+        U2, _ = refine(P, T, U, iters=slim_iters, method="slim", X=X, area=area, cots=cots)
+        if np.isfinite(U2).all():
+            U = U2
     return U

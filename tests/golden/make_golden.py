@@ -237,6 +237,35 @@ def flatten_fixtures(root):
     print("flatten_golden.npz", (HERE / "flatten_golden.npz").stat().st_size, "bytes")
 
 
+def flatten_slim_fixtures(root):
+    """charts.flatten with the pipeline's other settings: plan_iters 4, 6
+    and 10 (pipeline.py:29-44) and the artist profile's final flatten
+    (final_iters 20, slim_iters 15, pipeline.py:45,260), on the first 24
+    flatten_cases.  Output: tests/golden/flatten_slim_golden.npz."""
+    load_validate(root)
+    pkg = sys.modules["ref_engine"]
+    for name in ("param", "charts"):
+        if f"ref_engine.{name}" in sys.modules:
+            continue
+        spec = importlib.util.spec_from_file_location(f"ref_engine.{name}", Path(root) / "engine" / f"{name}.py")
+        m = importlib.util.module_from_spec(spec)
+        sys.modules[f"ref_engine.{name}"] = m
+        spec.loader.exec_module(m)
+        setattr(pkg, name, m)
+    rc = sys.modules["ref_engine.charts"]
+    charts = flatten_cases()[:24]
+    settings = [(4, 0), (6, 0), (10, 0), (20, 0), (20, 15)]
+    out = {"n": np.array(len(charts)), "settings": np.array(settings)}
+    for i, (P, T) in enumerate(charts):
+        out[f"P{i}"], out[f"T{i}"] = P, T.astype(np.int32)
+        for k, (it, sl) in enumerate(settings):
+            U, q = rc.flatten(P, T, iters=it, slim_iters=sl)
+            out[f"U{i}_{k}"] = U
+            out[f"q{i}_{k}"] = np.array([q["flips"], q["aniso_p95"], q["area_p95"], q["boundary_hits"]], np.float64)
+    np.savez_compressed(HERE / "flatten_slim_golden.npz", **out)
+    print("flatten_slim_golden.npz", (HERE / "flatten_slim_golden.npz").stat().st_size, "bytes")
+
+
 def _strip(turns, n=60, m=3, width=0.15):
     """A strip of n x m vertices bent into a spiral of ``turns`` turns
     (positively oriented triangles; its ends overlap for turns > 1)."""
@@ -577,6 +606,8 @@ if __name__ == "__main__":
         validate_fixtures(root, load_reference(root)[0])
     elif only == "hier":
         hier_fixtures(root, load_reference(root)[0])
+    elif only == "flatten_slim":
+        flatten_slim_fixtures(root)
     elif only == "hier_lscm":
         hier_lscm_fixtures(root, load_reference(root)[0])
     elif only == "batch40":

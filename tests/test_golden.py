@@ -205,6 +205,46 @@ def test_flatten_small_charts():
     assert worst <= 1e-10
 
 
+def rigid_err(U, Ur):
+    """Largest distance between U and Ur after the best rotation and
+    translation of U onto Ur (Kabsch), over Ur's extent."""
+    a, b = U - U.mean(0), Ur - Ur.mean(0)
+    u, _, vt = np.linalg.svd(a.T @ b)
+    if np.linalg.det(u @ vt) < 0:
+        u[:, -1] *= -1
+    return float(np.abs(a @ (u @ vt) - b).max() / np.ptp(Ur, 0).max())
+
+
+def test_flatten_pipeline_settings_and_slim():
+    """charts.flatten with the pipeline's plan_iters 4, 6, 10, with 20
+    iterations, and the artist profile's final flatten (20 ARAP iterations,
+    then 15 SLIM iterations when no fold is left, charts.py:422-425), on 24
+    small charts.  The SLIM normal equations hold rigid motions only through
+    a 1e-12 x area regularisation (param.py:425-431), so rounding sets the
+    rotation of the result (the reference's own layouts carry up to 3.6e-2
+    of it): SLIM layouts are compared after a rigid alignment, together with
+    their symmetric Dirichlet energy."""
+    g = dict(np.load(GOLD / "flatten_slim_golden.npz"))
+    worst = worst_rigid = 0.0
+    moved = 0
+    for i in range(int(g["n"])):
+        P, T = g[f"P{i}"], g[f"T{i}"].astype(np.int64)
+        X, area = pc.local_frames(P, T)
+        for k, (it, sl) in enumerate(g["settings"]):
+            Ur = g[f"U{i}_{k}"]
+            U = pc.flatten(P, T, iters=int(it), slim_iters=int(sl))
+            if sl:
+                worst_rigid = max(worst_rigid, rigid_err(U, Ur))
+                e, er = pc.energy(U, T, X, area, 1), pc.energy(Ur, T, X, area, 1)
+                assert abs(e - er) <= 1e-12 * er
+                moved += int(rigid_err(U, pc.flatten(P, T, iters=int(it))) > 1e-6)
+            else:
+                lo = Ur.min(0)
+                worst = max(worst, float(np.abs(U - Ur).max() / (Ur.max(0) - lo).max()))
+    assert worst <= 1e-9 and worst_rigid <= 1e-9, (worst, worst_rigid)
+    assert moved >= 12                      # the SLIM branch changes most layouts beyond a rigid motion
+
+
 def test_boundary_intersections_and_untangle():
     """charts.boundary_intersections (crossings counted once per grid cell)
     and charts.untangle_boundary (local ring smoothing without folds) of the

@@ -165,9 +165,12 @@ def untangle_boundary(U, T, rounds=40):
     return U, False
 
 
-def flatten_many(charts, iters=8, dev=None):
+def flatten_many(charts, iters=8, dev=None, slim_iters=0):
     """charts: [(P (k,3), T (t,3))] -> [(U (k,2), quality dict)] in order, as
-    ``charts.flatten(P, T, iters)`` (method "arap", no SLIM) for each chart.
+    ``charts.flatten(P, T, iters, slim_iters=slim_iters)`` (method "arap")
+    for each chart: with ``slim_iters``, the charts without folds after the
+    repair take that many SLIM iterations in one batched plan
+    (charts.py:422-425; a layout with a non-finite value is not taken).
     A chart without boundary raises ValueError as param.lscm does."""
     dev = require_gpu(dev)
     charts = [(np.asarray(P, np.float64), np.asarray(T, np.int64)) for P, T in charts]
@@ -208,6 +211,17 @@ def flatten_many(charts, iters=8, dev=None):
         U2 = U2.cpu().numpy()
         for k, i in enumerate(redo):
             out[i] = U2[sub.vert_off[k]:sub.vert_off[k + 1]].copy()
+    if slim_iters:
+        sl = [i for i, (_, T) in enumerate(charts) if not (param.signed_areas(out[i], T) <= 0).any()]
+        if sl:
+            sub = IslandPlan([charts[i] for i in sl], dev=dev)
+            U3 = sub.device_uv(np.concatenate([out[i] for i in sl]))
+            sub.slim(U3, int(slim_iters), tol=1e-5)
+            U3 = U3.cpu().numpy()
+            for k, i in enumerate(sl):
+                Us = U3[sub.vert_off[k]:sub.vert_off[k + 1]]
+                if np.isfinite(Us).all():
+                    out[i] = Us.copy()
     res = []
     for (P, T), Ui in zip(charts, out):
         # boundary crossings and their local untangle (charts.py:428-437), on the host
@@ -226,8 +240,10 @@ def flatten_many(charts, iters=8, dev=None):
 class FlattenQueue:
     """Collects ``flatten`` requests from any number of threads and runs them
     in batches of up to ``max_batch`` charts: a batch starts when it is full
-    or ``max_wait_ms`` after its first request.  ``flatten(P, T)`` blocks and
-    returns what ``charts.flatten(P, T, iters)`` would (see flatten_many);
+    or ``max_wait_ms`` after its first request, and runs one flatten_many per
+    (iters, slim_iters) setting it holds.  ``flatten(P, T, iters=None,
+    slim_iters=0)`` blocks and returns what ``charts.flatten(P, T, iters,
+    slim_iters=slim_iters)`` would (``iters`` None: the queue's default);
     ``submit`` returns a Future."""
 
     def __init__(self, iters=8, max_batch=4096, max_wait_ms=2.0, dev=None):
@@ -241,17 +257,18 @@ class FlattenQueue:
         self._worker = threading.Thread(target=self._run, name="flatten-queue", daemon=True)
         self._worker.start()
 
-    def submit(self, P, T):
+    def submit(self, P, T, iters=None, slim_iters=0):
         fut = Future()
+        key = (self.iters if iters is None else int(iters), int(slim_iters))
         with self._cv:
             if self._closed:
                 raise RuntimeError("FlattenQueue is closed")
-            self._pending.append((P, T, fut))
+            self._pending.append((P, T, fut, key))
             self._cv.notify()
         return fut
 
-    def flatten(self, P, T):
-        return self.submit(P, T).result()
+    def flatten(self, P, T, iters=None, slim_iters=0):
+        return self.submit(P, T, iters, slim_iters).result()
 
     def close(self):
         with self._cv:
@@ -280,18 +297,20 @@ class FlattenQueue:
                         break
                     self._cv.wait(left)
                 batch, self._pending = self._pending[:self.max_batch], self._pending[self.max_batch:]
-            good = [(P, T, f) for P, T, f in batch if f.set_running_or_notify_cancel()]
-            ok = []
-            for P, T, f in good:
+            groups = {}
+            for P, T, f, key in batch:
+                if not f.set_running_or_notify_cancel():
+                    continue
                 if _has_boundary(T):
-                    ok.append((P, T, f))
+                    groups.setdefault(key, []).append((P, T, f))
                 else:
                     f.set_exception(ValueError("chart has no boundary (closed surface): needs a cut"))
-            try:
-                res = flatten_many([(P, T) for P, T, _ in ok], self.iters, self.dev)
-                for (_, _, f), r in zip(ok, res):
-                    f.set_result(r)
-            except Exception as e:            # the batch failed as a whole: every caller sees it
-                for _, _, f in ok:
-                    if not f.done():
-                        f.set_exception(e)
+            for (it, sl), ok in groups.items():
+                try:
+                    res = flatten_many([(P, T) for P, T, _ in ok], it, self.dev, slim_iters=sl)
+                    for (_, _, f), r in zip(ok, res):
+                        f.set_result(r)
+                except Exception as e:        # the group failed as a whole: each of its callers sees it
+                    for _, _, f in ok:
+                        if not f.done():
+                            f.set_exception(e)
