@@ -39,6 +39,7 @@
  *   uva_repair_seq      param.py:478-516 repair_flips (host, exact order)
  *   uva_overlap_flags   validate.py:94-136 _overlaps, 257-299 _overlaps_strict
  *   uva_jacobi_pcg      hier.py:565-577 (the CG of Solver.lscm)
+ *   uva_csr_pcg         gpu.py:184-202 Batch._cg; hier.py:465-474, 565-577
  */
 #ifndef UVART_H
 #define UVART_H
@@ -207,6 +208,24 @@ int uva_overlap_flags(const double *U, int64_t n_vert, const int64_t *F, int64_t
  * iterations run.  Synchronises the stream. */
 int uva_jacobi_pcg(int64_t n, const int64_t *indptr, const int64_t *indices, const double *data, const double *b,
                    double *x, int32_t max_iters, int32_t check_every, double tol2, int32_t *iters_out, void *stream);
+
+/* A symmetric matrix resident on the device for repeated coupled solves
+ * (CSR from host memory as above; the handle holds the matrix, its Jacobi
+ * weights and the work vectors of a two-column solve).  uva_csr_pcg: the
+ * same iteration for ncols = 1 or 2 columns with their own step lengths
+ * (b and x device arrays, n x ncols row-major); after iteration k with
+ * k % check_every == 0 it stops when every column has r.r < tol2 max(b.b,
+ * 1e-30) (strict = 1) or <= (strict = 0) -- gpu.py:184-202 Batch._cg is
+ * check_every 1, tol2 = tol^2, strict 0; hier.py:465-474 (a fixed count) is
+ * tol2 0, strict 1.  The stop test runs on the device; *iters_out (may be
+ * NULL, then the call does not wait for the stream) gets the iterations
+ * run.  One handle serves one solve at a time. */
+typedef struct uva_csr uva_csr;
+int uva_csr_create(uva_csr **out, int64_t n, const int64_t *indptr, const int64_t *indices, const double *data,
+                   void *stream);
+int uva_csr_destroy(uva_csr *m);
+int uva_csr_pcg(uva_csr *m, const double *b, double *x, int32_t ncols, int32_t max_iters, int32_t check_every,
+                double tol2, int32_t strict, int32_t *iters_out, void *stream);
 
 /* SLIM (symmetric Dirichlet) options: param.refine(method="slim"). */
 typedef struct uva_slim_opts {

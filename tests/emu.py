@@ -50,8 +50,8 @@ def lib():
         _lib.uva_emu_set_pins.argtypes = [C.c_void_p, _i64p, C.c_int64, C.c_double]
         _lib.uva_emu_overlap_flags.argtypes = [_dp, C.c_int64, _i64p, C.c_int64, C.c_void_p, C.c_double, C.c_int32,
                                                C.c_int32, C.c_void_p]
-        _lib.uva_emu_jacobi_pcg.argtypes = [C.c_int64, _i64p, _i64p, _dp, _dp, _dp, C.c_int32, C.c_int32,
-                                            C.c_double, C.POINTER(C.c_int32)]
+        _lib.uva_emu_csr_pcg.argtypes = [C.c_int64, _i64p, _i64p, _dp, _dp, _dp, C.c_int32, C.c_int32, C.c_int32,
+                                         C.c_double, C.c_int32, C.POINTER(C.c_int32)]
     return _lib
 
 
@@ -67,21 +67,22 @@ def overlap_flags(U, F, skip, rel, strict, grid):
     return out.astype(bool)
 
 
-def jacobi_pcg(A, b, x0, max_iters, check_every=50, tol2=1e-12):
-    """uva_jacobi_pcg's iteration run on the host (same row code): returns
-    (x, iterations run)."""
+def jacobi_pcg(A, b, x0, max_iters, check_every=50, tol2=1e-12, strict=True):
+    """uva_csr_pcg's iteration run on the host (same row code), for b and x0
+    of shape (n,) or (n, 2): returns (x, iterations run)."""
     A = A.tocsr()
     ip = np.ascontiguousarray(A.indptr, np.int64)
     ix = np.ascontiguousarray(A.indices, np.int64)
     a = np.ascontiguousarray(A.data, np.float64)
     b = np.ascontiguousarray(b, np.float64)
-    x = np.array(x0, np.float64, copy=True)
+    x = np.array(x0, np.float64, copy=True, order="C")
+    nc = 1 if b.ndim == 1 else b.shape[1]
     it = C.c_int32(0)
-    rc = lib().uva_emu_jacobi_pcg(A.shape[0], ip.ctypes.data_as(_i64p), ix.ctypes.data_as(_i64p),
-                                  a.ctypes.data_as(_dp), b.ctypes.data_as(_dp), x.ctypes.data_as(_dp),
-                                  int(max_iters), int(check_every), float(tol2), C.byref(it))
+    rc = lib().uva_emu_csr_pcg(A.shape[0], ip.ctypes.data_as(_i64p), ix.ctypes.data_as(_i64p),
+                               a.ctypes.data_as(_dp), b.ctypes.data_as(_dp), x.ctypes.data_as(_dp), int(nc),
+                               int(max_iters), int(check_every), float(tol2), int(bool(strict)), C.byref(it))
     if rc != 0:
-        raise ValueError(f"uva_emu_jacobi_pcg returned {rc}")
+        raise ValueError(f"uva_emu_csr_pcg returned {rc}")
     return x, it.value
 
 

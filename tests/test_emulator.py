@@ -472,29 +472,43 @@ def test_emu_jacobi_pcg_iteration_and_stop_rule():
     As.sum_duplicates()
     dinv = 1.0 / As.diagonal()
 
-    def pcg(iters, check, tol2):
-        x = x0.copy()
-        r = b - As @ x
-        z = r * dinv
+    def pcg(b, x0, iters, check, tol2, strict=True):
+        """the definition, per column (columns share only the stop test)"""
+        B = b.reshape(n, -1)
+        x = x0.reshape(n, -1).copy()
+        r = B - As @ x
+        z = r * dinv[:, None]
         p = z.copy()
-        rz = r @ z
+        rz = (r * z).sum(0)
+        b2 = np.maximum((B * B).sum(0), 1e-30)
         for k in range(iters):
             Ap = As @ p
-            al = rz / max(p @ Ap, 1e-30)
+            al = rz / np.maximum((p * Ap).sum(0), 1e-30)
             x, r = x + al * p, r - al * Ap
-            if k % check == 0 and r @ r < tol2 * max(b @ b, 1e-30):
-                return x, k + 1
-            z = r * dinv
-            rz2 = r @ z
-            p = z + rz2 / max(rz, 1e-30) * p
+            rr = (r * r).sum(0)
+            if k % check == 0 and ((rr < tol2 * b2) if strict else (rr <= tol2 * b2)).all():
+                return x.reshape(x0.shape), k + 1
+            z = r * dinv[:, None]
+            rz2 = (r * z).sum(0)
+            p = z + rz2 / np.maximum(rz, 1e-30) * p
             rz = rz2
-        return x, iters
+        return x.reshape(x0.shape), iters
 
     for iters, check, tol2 in ((25, 50, 1e-12), (2000, 50, 1e-12), (2000, 1, 1e-12), (2000, 7, 1e-20)):
         x, k = emu.jacobi_pcg(A, b, x0, iters, check, tol2)
-        xr, kr = pcg(iters, check, tol2)
+        xr, kr = pcg(b, x0, iters, check, tol2)
         assert k == kr and np.abs(x - xr).max() <= 1e-10 * np.abs(xr).max(), (iters, check, k, kr)
         if iters == 2000 and check > 1 and tol2 > 1e-15:
             assert k % check == 1 or check == 1
+    # two columns with their own step lengths, the Batch rule (every
+    # iteration, both columns at most tol^2 b.b), and a fixed count
+    b2c = np.stack([b, rng.normal(size=n) * 1e3], 1)
+    x2c = np.stack([x0, rng.normal(size=n)], 1)
+    for iters, tol2, strict in ((40, 1e-12, False), (2000, 1e-16, False), (20, 0.0, True)):
+        x, k = emu.jacobi_pcg(A, b2c, x2c, iters, 1, tol2, strict)
+        xr, kr = pcg(b2c, x2c, iters, 1, tol2, strict)
+        assert k == kr and np.abs(x - xr).max() <= 1e-10 * np.abs(xr).max(), (iters, k, kr)
+    x1, _ = emu.jacobi_pcg(A, b2c[:, 1], x2c[:, 1], 20, 1, 0.0, True)
+    assert np.abs(x1 - x[:, 1]).max() <= 1e-12 * np.abs(x1).max()      # columns do not mix
     x, k = emu.jacobi_pcg(A, b, x0, 0)
     assert k == 0 and np.array_equal(x, x0)

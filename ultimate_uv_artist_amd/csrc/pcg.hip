@@ -1,19 +1,27 @@
 // Jacobi-preconditioned conjugate gradients on one symmetric matrix in CSR
-// form, with the iteration and stopping rule of the reference's free-boundary
-// LSCM solve (engine/hier.py:565-577): from the start x, at most max_iters
-// iterations; after iteration k (0-based) with k % check_every == 0 the solve
-// stops when r.r < tol2 * max(b.b, 1e-30).  alpha = rz / max(p.Ap, 1e-30),
-// beta = rz' / max(rz, 1e-30), Jacobi weights 1 / max(diag, 1e-30) with the
-// diagonal summed over repeated entries.
+// form, for one or two right-hand-side columns with their own step lengths:
+// the coupled solves of the reference that span a whole batch of charts,
+//   engine/gpu.py:184-202 Batch._cg     two columns (u, v), both checked
+//                                       every iteration, stop when
+//                                       r.r <= tol^2 max(b.b, 1e-30) for both
+//   engine/hier.py:565-577 Solver.lscm  one 2n-row column (u and v coupled by
+//                                       the boundary area term), checked every
+//                                       50 iterations, stop when r.r < 1e-12 b.b
+//   engine/hier.py:465-474 Solver.relax two columns, a fixed iteration count
+// From the start x: alpha = rz / max(p.Ap, 1e-30), beta = rz' / max(rz,
+// 1e-30), Jacobi weights 1 / max(diag, 1e-30) with the diagonal summed over
+// repeated entries.
 //
-// The operator of that solve couples the u and v columns of every chart (the
-// area term on the boundary), so it is one 2n-row system, not the island
-// kernel's two-column Laplacian.  The matrix comes from the host (checked
-// there, then copied); b and x are device arrays.  Rows are independent per thread (one thread
-// per row, CSR gather); dot products are block partial sums in a fixed tree
-// order followed by a one-workgroup reduction in a fixed order, so repeated
-// runs are bitwise identical.  The scalars stay on the device; the host reads
-// the residual only at check iterations.
+// These systems span every chart of a batch (millions of rows, one shared
+// step length per column), so they run as ordinary grid-wide kernels over all
+// CUs, five launches per iteration: SpMV + partial p.Ap; a one-workgroup
+// reduction to alpha; the x, r, z update + partial r.r and r.z; a reduction
+// to beta and the stop test; the direction update.  The stop test is decided
+// on the device (a flag every later launch reads first), so a solve queues
+// all its launches without a host round trip.  Dot products are block
+// partial sums in a fixed tree order followed by a fixed-order reduction, so
+// repeated runs are bitwise identical.  Rows are one thread each (CSR
+// gather; the Laplacians here hold about 7 entries per row).
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -26,263 +34,376 @@ namespace uva {
 namespace pcg {
 
 constexpr int kB = 256;        // rows per workgroup
-constexpr int kR = 1024;       // threads of the final reduction
+constexpr int kR = 1024;       // threads of the final reductions
 
 struct Scalars {
-    double rz, b2, al, rr, beta;
+    double rz[2], b2[2], al[2], beta[2], rr[2];
+    int done, its;
 };
-
-__host__ __device__ inline double row_dot(const int64_t* ip, const int64_t* ix, const double* a, const double* v,
-                                          int64_t i) {
-    double s = 0.0;
-    for (int64_t q = ip[i]; q < ip[i + 1]; ++q) s += a[q] * v[ix[q]];
-    return s;
-}
-
-__host__ __device__ inline double row_diag(const int64_t* ip, const int64_t* ix, const double* a, int64_t i) {
-    double g = 0.0;
-    for (int64_t q = ip[i]; q < ip[i + 1]; ++q)
-        if (ix[q] == i) g += a[q];
-    return 1.0 / (g > 1e-30 ? g : 1e-30);
-}
 
 __host__ __device__ inline double clamp30(double v) { return v > 1e-30 ? v : 1e-30; }
 
-#if !defined(UVA_EMULATOR)
-// block sum of two values per thread into part[2*block], part[2*block+1]
-__device__ inline void block_sum2(double u, double v, double* part) {
-    __shared__ double su[kB], sv[kB];
-    su[threadIdx.x] = u;
-    sv[threadIdx.x] = v;
-    __syncthreads();
-    for (int h = kB / 2; h > 0; h >>= 1) {
-        if ((int)threadIdx.x < h) {
-            su[threadIdx.x] += su[threadIdx.x + h];
-            sv[threadIdx.x] += sv[threadIdx.x + h];
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        part[2 * blockIdx.x] = su[0];
-        part[2 * blockIdx.x + 1] = sv[0];
-    }
+__host__ __device__ inline double row_diag_inv(const int64_t* ip, const int64_t* ix, const double* a, int64_t i) {
+    double g = 0.0;
+    for (int64_t q = ip[i]; q < ip[i + 1]; ++q)
+        if (ix[q] == i) g += a[q];
+    return 1.0 / clamp30(g);
 }
 
-// r = b - A x, z = D r, p = z; partial r.z and b.b
-__global__ void init_kernel(int64_t n, const int64_t* ip, const int64_t* ix, const double* a, const double* b,
-                            const double* x, double* dinv, double* r, double* z, double* p, double* part) {
-    const int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x;
-    double rz = 0.0, bb = 0.0;
-    if (i < n) {
-        const double di = row_diag(ip, ix, a, i);
-        const double ri = b[i] - row_dot(ip, ix, a, x, i);
-        dinv[i] = di;
-        r[i] = ri;
-        z[i] = ri * di;
-        p[i] = ri * di;
-        rz = ri * (ri * di);
-        bb = b[i] * b[i];
+template <int NC>
+__host__ __device__ inline void row_mv(const int64_t* ip, const int64_t* ix, const double* a, const double* v,
+                                       int64_t i, double* out) {
+    double s[NC];
+    for (int c = 0; c < NC; ++c) s[c] = 0.0;
+    for (int64_t q = ip[i]; q < ip[i + 1]; ++q) {
+        const double w = a[q];
+        const int64_t j = ix[q];
+        for (int c = 0; c < NC; ++c) s[c] += w * v[j * NC + c];
     }
-    block_sum2(rz, bb, part);
+    for (int c = 0; c < NC; ++c) out[c] = s[c];
 }
 
-// Ap = A p; partial p.Ap
-__global__ void spmv_kernel(int64_t n, const int64_t* ip, const int64_t* ix, const double* a, const double* p,
-                            double* Ap, double* part) {
-    const int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x;
-    double pap = 0.0;
-    if (i < n) {
-        const double v = row_dot(ip, ix, a, p, i);
-        Ap[i] = v;
-        pap = p[i] * v;
-    }
-    block_sum2(pap, 0.0, part);
+// stop test of one check: every column below its threshold
+__host__ __device__ inline bool converged(const double* rr, const double* b2, int nc, double tol2, int strict) {
+    bool all = true;
+    for (int c = 0; c < nc; ++c) all = all && (strict ? rr[c] < tol2 * b2[c] : rr[c] <= tol2 * b2[c]);
+    return all;
 }
 
-// x += al p, r -= al Ap, z = D r; partial r.r and r.z
-__global__ void update_kernel(int64_t n, const Scalars* s, const double* dinv, const double* p, const double* Ap,
-                              double* x, double* r, double* z, double* part) {
-    const int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x;
-    double rr = 0.0, rz = 0.0;
-    if (i < n) {
-        const double al = s->al;
-        x[i] += al * p[i];
-        const double ri = r[i] - al * Ap[i];
-        r[i] = ri;
-        const double zi = ri * dinv[i];
-        z[i] = zi;
-        rr = ri * ri;
-        rz = ri * zi;
-    }
-    block_sum2(rr, rz, part);
+// host checks of a CSR matrix: "" when valid
+inline const char* check_csr(int64_t n, const int64_t* indptr, const int64_t* indices, const double* data) {
+    if (n < 1) return "matrix must have at least one row";
+    if (!indptr) return "null indptr";
+    if (indptr[0] != 0) return "indptr must start at 0";
+    for (int64_t i = 0; i < n; ++i)
+        if (indptr[i + 1] < indptr[i]) return "indptr must be non-decreasing";
+    if (indptr[n] > 0 && (!indices || !data)) return "null indices or data";
+    for (int64_t q = 0; q < indptr[n]; ++q)
+        if (indices[q] < 0 || indices[q] >= n) return "column index out of range";
+    return "";
 }
-
-// p = z + beta p
-__global__ void direction_kernel(int64_t n, const Scalars* s, const double* z, double* p) {
-    const int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x;
-    if (i < n) p[i] = z[i] + s->beta * p[i];
-}
-
-// sums of the nb block pairs in a fixed order, then the step scalars:
-// stage 0 (after init): rz, b2; stage 1 (after spmv): al; stage 2 (after
-// update): rr, beta, rz
-__global__ void reduce_kernel(const double* part, int64_t nb, int stage, Scalars* s) {
-    __shared__ double su[kR], sv[kR];
-    double u = 0.0, v = 0.0;
-    for (int64_t k = threadIdx.x; k < nb; k += kR) {
-        u += part[2 * k];
-        v += part[2 * k + 1];
-    }
-    su[threadIdx.x] = u;
-    sv[threadIdx.x] = v;
-    __syncthreads();
-    for (int h = kR / 2; h > 0; h >>= 1) {
-        if ((int)threadIdx.x < h) {
-            su[threadIdx.x] += su[threadIdx.x + h];
-            sv[threadIdx.x] += sv[threadIdx.x + h];
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x != 0) return;
-    if (stage == 0) {
-        s->rz = su[0];
-        s->b2 = clamp30(sv[0]);
-    } else if (stage == 1) {
-        s->al = s->rz / clamp30(su[0]);
-    } else {
-        s->rr = su[0];
-        s->beta = sv[0] / clamp30(s->rz);
-        s->rz = sv[0];
-    }
-}
-#endif
 
 }  // namespace pcg
 }  // namespace uva
 
 #if !defined(UVA_EMULATOR)
-extern "C" int uva_jacobi_pcg(int64_t n, const int64_t* indptr, const int64_t* indices, const double* data,
-                              const double* b, double* x, int32_t max_iters, int32_t check_every, double tol2,
-                              int32_t* iters_out, void* stream) {
+struct uva_csr {
+    int64_t n = 0, nnz = 0, nb = 0;
+    int64_t *ip = nullptr, *ix = nullptr;
+    double *a = nullptr, *dinv = nullptr, *work = nullptr, *part = nullptr;
+    uva::pcg::Scalars* sc = nullptr;
+};
+
+namespace uva {
+namespace pcg {
+
+// block sums of K values per thread into part[K*block + k]
+template <int K>
+__device__ inline void block_sum(const double (&v)[K], double* part) {
+    __shared__ double s[K][kB];
+    for (int k = 0; k < K; ++k) s[k][threadIdx.x] = v[k];
+    __syncthreads();
+    for (int h = kB / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h)
+            for (int k = 0; k < K; ++k) s[k][threadIdx.x] += s[k][threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < K; ++k) part[K * blockIdx.x + k] = s[k][0];
+}
+
+__global__ void diag_kernel(int64_t n, const int64_t* ip, const int64_t* ix, const double* a, double* dinv) {
+    const int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x;
+    if (i < n) dinv[i] = row_diag_inv(ip, ix, a, i);
+}
+
+// r = b - A x, z = D r, p = z; partial r.z and b.b per column
+template <int NC>
+__global__ void init_kernel(int64_t n, const int64_t* ip, const int64_t* ix, const double* a, const double* dinv,
+                            const double* b, const double* x, double* r, double* z, double* p, double* part) {
+    const int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x;
+    double v[2 * NC];
+    for (int k = 0; k < 2 * NC; ++k) v[k] = 0.0;
+    if (i < n) {
+        double ax[NC];
+        row_mv<NC>(ip, ix, a, x, i, ax);
+        const double di = dinv[i];
+        for (int c = 0; c < NC; ++c) {
+            const double bi = b[i * NC + c];
+            const double ri = bi - ax[c];
+            r[i * NC + c] = ri;
+            z[i * NC + c] = ri * di;
+            p[i * NC + c] = ri * di;
+            v[c] = ri * (ri * di);
+            v[NC + c] = bi * bi;
+        }
+    }
+    block_sum<2 * NC>(v, part);
+}
+
+// Ap = A p; partial p.Ap per column
+template <int NC>
+__global__ void spmv_kernel(int64_t n, const int64_t* ip, const int64_t* ix, const double* a, const double* p,
+                            double* Ap, double* part, const Scalars* s) {
+    if (s->done) return;
+    const int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x;
+    double v[NC];
+    for (int c = 0; c < NC; ++c) v[c] = 0.0;
+    if (i < n) {
+        double ap[NC];
+        row_mv<NC>(ip, ix, a, p, i, ap);
+        for (int c = 0; c < NC; ++c) {
+            Ap[i * NC + c] = ap[c];
+            v[c] = p[i * NC + c] * ap[c];
+        }
+    }
+    block_sum<NC>(v, part);
+}
+
+// x += al p, r -= al Ap, z = D r; partial r.r and r.z per column
+template <int NC>
+__global__ void update_kernel(int64_t n, const double* dinv, const double* p, const double* Ap, double* x, double* r,
+                              double* z, double* part, const Scalars* s) {
+    if (s->done) return;
+    const int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x;
+    double v[2 * NC];
+    for (int k = 0; k < 2 * NC; ++k) v[k] = 0.0;
+    if (i < n) {
+        const double di = dinv[i];
+        for (int c = 0; c < NC; ++c) {
+            const double al = s->al[c];
+            x[i * NC + c] += al * p[i * NC + c];
+            const double ri = r[i * NC + c] - al * Ap[i * NC + c];
+            const double zi = ri * di;
+            r[i * NC + c] = ri;
+            z[i * NC + c] = zi;
+            v[c] = ri * ri;
+            v[NC + c] = ri * zi;
+        }
+    }
+    block_sum<2 * NC>(v, part);
+}
+
+// p = z + beta p
+template <int NC>
+__global__ void direction_kernel(int64_t n, const double* z, double* p, const Scalars* s) {
+    if (s->done) return;
+    const int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x;
+    if (i < n)
+        for (int c = 0; c < NC; ++c) p[i * NC + c] = z[i * NC + c] + s->beta[c] * p[i * NC + c];
+}
+
+// fixed-order sums of the nb block partials (K per block) for NC columns:
+// stage 0 (after init, K = 2 NC):   rz, b2; done = 0; its = max_iters
+// stage 1 (after spmv, K = NC):     al
+// stage 2 (after update, K = 2 NC): rr, beta, rz; stop test at check iterations
+template <int NC, int K>
+__global__ void reduce_kernel(const double* part, int64_t nb, int stage, int it, int max_iters, int check_every,
+                              double tol2, int strict, Scalars* s) {
+    if (stage != 0 && s->done) return;
+    __shared__ double sh[K][kR];
+    double u[K];
+    for (int k = 0; k < K; ++k) u[k] = 0.0;
+    for (int64_t q = threadIdx.x; q < nb; q += kR)
+        for (int k = 0; k < K; ++k) u[k] += part[K * q + k];
+    for (int k = 0; k < K; ++k) sh[k][threadIdx.x] = u[k];
+    __syncthreads();
+    for (int h = kR / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h)
+            for (int k = 0; k < K; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    if (stage == 0) {
+        for (int c = 0; c < NC; ++c) {
+            s->rz[c] = sh[c][0];
+            s->b2[c] = clamp30(sh[NC + c][0]);
+        }
+        s->done = 0;
+        s->its = max_iters;
+    } else if (stage == 1) {
+        for (int c = 0; c < NC; ++c) s->al[c] = s->rz[c] / clamp30(sh[c][0]);
+    } else {
+        for (int c = 0; c < NC; ++c) {
+            s->rr[c] = sh[c][0];
+            s->beta[c] = sh[NC + c][0] / clamp30(s->rz[c]);
+            s->rz[c] = sh[NC + c][0];
+        }
+        if (it % check_every == 0 && converged(s->rr, s->b2, NC, tol2, strict)) {
+            s->done = 1;
+            s->its = it + 1;
+        }
+    }
+}
+
+template <int NC>
+int solve(uva_csr* m, const double* b, double* x, int max_iters, int check_every, double tol2, int strict,
+          hipStream_t st) {
+    const int64_t n = m->n, nb = m->nb;
+    double *r = m->work, *z = r + NC * n, *p = z + NC * n, *Ap = p + NC * n;
+    const dim3 grid((unsigned)nb), blk(kB), one(1), red(kR);
+    hipLaunchKernelGGL(init_kernel<NC>, grid, blk, 0, st, n, m->ip, m->ix, m->a, m->dinv, b, x, r, z, p, m->part);
+    hipLaunchKernelGGL((reduce_kernel<NC, 2 * NC>), one, red, 0, st, m->part, nb, 0, 0, max_iters, check_every,
+                       tol2, strict, m->sc);
+    for (int it = 0; it < max_iters; ++it) {
+        hipLaunchKernelGGL(spmv_kernel<NC>, grid, blk, 0, st, n, m->ip, m->ix, m->a, p, Ap, m->part, m->sc);
+        hipLaunchKernelGGL((reduce_kernel<NC, NC>), one, red, 0, st, m->part, nb, 1, it, max_iters, check_every,
+                           tol2, strict, m->sc);
+        hipLaunchKernelGGL(update_kernel<NC>, grid, blk, 0, st, n, m->dinv, p, Ap, x, r, z, m->part, m->sc);
+        hipLaunchKernelGGL((reduce_kernel<NC, 2 * NC>), one, red, 0, st, m->part, nb, 2, it, max_iters, check_every,
+                           tol2, strict, m->sc);
+        hipLaunchKernelGGL(direction_kernel<NC>, grid, blk, 0, st, n, z, p, m->sc);
+    }
+    return hipGetLastError() == hipSuccess ? UVA_OK : UVA_EHIP;
+}
+
+}  // namespace pcg
+}  // namespace uva
+
+extern "C" int uva_csr_create(uva_csr** out, int64_t n, const int64_t* indptr, const int64_t* indices,
+                              const double* data, void* stream) {
     using namespace uva::pcg;
-    if (n < 0 || max_iters < 0 || check_every < 1 || !(tol2 >= 0.0)) return UVA_EINVAL;
-    if (n > 0 && (!indptr || !b || !x)) return UVA_EINVAL;
-    if (iters_out) *iters_out = 0;
-    if (n == 0) return UVA_OK;
-    if (indptr[0] != 0) return UVA_EINVAL;
-    for (int64_t i = 0; i < n; ++i)
-        if (indptr[i + 1] < indptr[i]) return UVA_EINVAL;
-    const int64_t nnz = indptr[n];
-    if (nnz > 0 && (!indices || !data)) return UVA_EINVAL;
-    for (int64_t q = 0; q < nnz; ++q)
-        if (indices[q] < 0 || indices[q] >= n) return UVA_EINVAL;
-    if (max_iters == 0) return UVA_OK;
-    hipStream_t st = (hipStream_t)stream;
+    if (!out) return UVA_EINVAL;
+    *out = nullptr;
+    if (*check_csr(n, indptr, indices, data)) return UVA_EINVAL;
     const int64_t nb = (n + kB - 1) / kB;
     if (nb > (int64_t)0x7fffffff) return UVA_EINVAL;
-    double* buf = nullptr;
-    int64_t* idx = nullptr;
-    Scalars* sc = nullptr;
-    const size_t nvec = (size_t)n, nz = (size_t)nnz;
-    if (hipMalloc(&buf, (5 * nvec + 2 * (size_t)nb + nz + 1) * sizeof(double)) != hipSuccess) return UVA_ENOMEM;
-    if (hipMalloc(&idx, (nvec + 1 + nz + 1) * sizeof(int64_t)) != hipSuccess) {
-        (void)hipFree(buf);
-        return UVA_ENOMEM;
+    hipStream_t st = (hipStream_t)stream;
+    uva_csr* m = new uva_csr();
+    m->n = n;
+    m->nnz = indptr[n];
+    m->nb = nb;
+    const size_t nz = (size_t)m->nnz;
+    auto fail = [&](int rc) {
+        (void)hipStreamSynchronize(st);
+        (void)hipFree(m->ip);
+        (void)hipFree(m->ix);
+        (void)hipFree(m->a);
+        (void)hipFree(m->dinv);
+        (void)hipFree(m->work);
+        (void)hipFree(m->part);
+        (void)hipFree(m->sc);
+        delete m;
+        return rc;
+    };
+    // work: r, z, p, Ap for up to two columns
+    if (hipMalloc(&m->ip, ((size_t)n + 1) * 8) != hipSuccess || hipMalloc(&m->ix, (nz + 1) * 8) != hipSuccess ||
+        hipMalloc(&m->a, (nz + 1) * 8) != hipSuccess || hipMalloc(&m->dinv, (size_t)n * 8) != hipSuccess ||
+        hipMalloc(&m->work, (size_t)n * 8 * 8) != hipSuccess || hipMalloc(&m->part, (size_t)nb * 4 * 8) != hipSuccess ||
+        hipMalloc(&m->sc, sizeof(Scalars)) != hipSuccess)
+        return fail(UVA_ENOMEM);
+    if (hipMemcpyAsync(m->ip, indptr, ((size_t)n + 1) * 8, hipMemcpyHostToDevice, st) != hipSuccess ||
+        (nz && hipMemcpyAsync(m->ix, indices, nz * 8, hipMemcpyHostToDevice, st) != hipSuccess) ||
+        (nz && hipMemcpyAsync(m->a, data, nz * 8, hipMemcpyHostToDevice, st) != hipSuccess))
+        return fail(UVA_EHIP);
+    hipLaunchKernelGGL(diag_kernel, dim3((unsigned)nb), dim3(kB), 0, st, n, m->ip, m->ix, m->a, m->dinv);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess) return fail(UVA_EHIP);
+    *out = m;
+    return UVA_OK;
+}
+
+extern "C" int uva_csr_destroy(uva_csr* m) {
+    if (!m) return UVA_OK;
+    (void)hipDeviceSynchronize();
+    (void)hipFree(m->ip);
+    (void)hipFree(m->ix);
+    (void)hipFree(m->a);
+    (void)hipFree(m->dinv);
+    (void)hipFree(m->work);
+    (void)hipFree(m->part);
+    (void)hipFree(m->sc);
+    delete m;
+    return UVA_OK;
+}
+
+extern "C" int uva_csr_pcg(uva_csr* m, const double* b, double* x, int32_t ncols, int32_t max_iters,
+                           int32_t check_every, double tol2, int32_t strict, int32_t* iters_out, void* stream) {
+    using namespace uva::pcg;
+    if (!m || !b || !x || (ncols != 1 && ncols != 2) || max_iters < 0 || check_every < 1 || !(tol2 >= 0.0))
+        return UVA_EINVAL;
+    if (iters_out) *iters_out = 0;
+    if (max_iters == 0) return UVA_OK;
+    hipStream_t st = (hipStream_t)stream;
+    const int rc = ncols == 1 ? solve<1>(m, b, x, max_iters, check_every, tol2, strict, st)
+                              : solve<2>(m, b, x, max_iters, check_every, tol2, strict, st);
+    if (rc == UVA_OK && iters_out) {
+        int its = 0;
+        if (hipMemcpyAsync(&its, &m->sc->its, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess)
+            return UVA_EHIP;
+        *iters_out = its;
     }
-    if (hipMalloc(&sc, sizeof(Scalars)) != hipSuccess) {
-        (void)hipFree(buf);
-        (void)hipFree(idx);
-        return UVA_ENOMEM;
-    }
-    double *dinv = buf, *r = buf + nvec, *z = buf + 2 * nvec, *p = buf + 3 * nvec, *Ap = buf + 4 * nvec;
-    double* part = buf + 5 * nvec;
-    double* a = part + 2 * nb;
-    int64_t *ip = idx, *ix = idx + nvec + 1;
-    const dim3 grid((unsigned)nb), blk(kB);
-    int rc = UVA_OK;
-    int it_run = max_iters;
-    if (hipMemcpyAsync(ip, indptr, (nvec + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st) != hipSuccess ||
-        (nz && hipMemcpyAsync(ix, indices, nz * sizeof(int64_t), hipMemcpyHostToDevice, st) != hipSuccess) ||
-        (nz && hipMemcpyAsync(a, data, nz * sizeof(double), hipMemcpyHostToDevice, st) != hipSuccess))
-        rc = UVA_EHIP;
-    if (rc == UVA_OK)
-        hipLaunchKernelGGL(init_kernel, grid, blk, 0, st, n, ip, ix, a, b, x, dinv, r, z, p, part);
-    if (rc == UVA_OK) hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(kR), 0, st, part, nb, 0, sc);
-    for (int it = 0; it < max_iters && rc == UVA_OK; ++it) {
-        hipLaunchKernelGGL(spmv_kernel, grid, blk, 0, st, n, ip, ix, a, p, Ap, part);
-        hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(kR), 0, st, part, nb, 1, sc);
-        hipLaunchKernelGGL(update_kernel, grid, blk, 0, st, n, sc, dinv, p, Ap, x, r, z, part);
-        hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(kR), 0, st, part, nb, 2, sc);
-        if (hipGetLastError() != hipSuccess) rc = UVA_EHIP;
-        if (rc == UVA_OK && it % check_every == 0) {
-            Scalars h;
-            if (hipMemcpyAsync(&h, sc, sizeof(Scalars), hipMemcpyDeviceToHost, st) != hipSuccess ||
-                hipStreamSynchronize(st) != hipSuccess)
-                rc = UVA_EHIP;
-            else if (h.rr < tol2 * h.b2) {
-                it_run = it + 1;
-                break;
-            }
-        }
-        hipLaunchKernelGGL(direction_kernel, grid, blk, 0, st, n, sc, z, p);
-    }
-    if (hipStreamSynchronize(st) != hipSuccess && rc == UVA_OK) rc = UVA_EHIP;
-    (void)hipFree(buf);
-    (void)hipFree(idx);
-    (void)hipFree(sc);
-    if (rc == UVA_OK && iters_out) *iters_out = it_run;
     return rc;
 }
 
+extern "C" int uva_jacobi_pcg(int64_t n, const int64_t* indptr, const int64_t* indices, const double* data,
+                              const double* b, double* x, int32_t max_iters, int32_t check_every, double tol2,
+                              int32_t* iters_out, void* stream) {
+    if (n < 0 || max_iters < 0 || check_every < 1 || !(tol2 >= 0.0)) return UVA_EINVAL;
+    if (iters_out) *iters_out = 0;
+    if (n == 0) return UVA_OK;
+    if (!b || !x) return UVA_EINVAL;
+    uva_csr* m = nullptr;
+    int rc = uva_csr_create(&m, n, indptr, indices, data, stream);
+    if (rc) return rc;
+    rc = uva_csr_pcg(m, b, x, 1, max_iters, check_every, tol2, 1, iters_out, stream);
+    if (hipStreamSynchronize((hipStream_t)stream) != hipSuccess && rc == UVA_OK) rc = UVA_EHIP;
+    uva_csr_destroy(m);
+    return rc;
+}
 #else
 // The same iteration on the host, rows in order and sequential sums
 // (test-only emulation; equal to the device result up to the order of the
-// dot-product sums).
-extern "C" int uva_emu_jacobi_pcg(int64_t n, const int64_t* indptr, const int64_t* indices, const double* data,
-                                  const double* b, double* x, int32_t max_iters, int32_t check_every, double tol2,
-                                  int32_t* iters_out) {
+// dot-product sums).  b and x are host arrays, n x ncols.
+extern "C" int uva_emu_csr_pcg(int64_t n, const int64_t* indptr, const int64_t* indices, const double* data,
+                               const double* b, double* x, int32_t ncols, int32_t max_iters, int32_t check_every,
+                               double tol2, int32_t strict, int32_t* iters_out) {
     using namespace uva::pcg;
-    if (n < 0 || max_iters < 0 || check_every < 1 || !(tol2 >= 0.0)) return UVA_EINVAL;
+    if ((ncols != 1 && ncols != 2) || max_iters < 0 || check_every < 1 || !(tol2 >= 0.0)) return UVA_EINVAL;
+    if (*check_csr(n, indptr, indices, data) || !b || !x) return UVA_EINVAL;
     if (iters_out) *iters_out = 0;
-    if (n == 0 || max_iters == 0) return UVA_OK;
-    if (!indptr || indptr[0] != 0 || !b || !x) return UVA_EINVAL;
-    for (int64_t i = 0; i < n; ++i)
-        if (indptr[i + 1] < indptr[i]) return UVA_EINVAL;
-    for (int64_t q = 0; q < indptr[n]; ++q)
-        if (indices[q] < 0 || indices[q] >= n) return UVA_EINVAL;
-    std::vector<double> dinv(n), r(n), z(n), p(n), Ap(n);
-    double rz = 0.0, b2 = 0.0;
+    if (max_iters == 0) return UVA_OK;
+    const int nc = ncols;
+    std::vector<double> dinv(n), r(n * nc), z(n * nc), p(n * nc), Ap(n * nc);
+    double rz[2] = {0, 0}, b2[2] = {0, 0}, t[2];
     for (int64_t i = 0; i < n; ++i) {
-        dinv[i] = row_diag(indptr, indices, data, i);
-        r[i] = b[i] - row_dot(indptr, indices, data, x, i);
-        z[i] = p[i] = r[i] * dinv[i];
-        rz += r[i] * z[i];
-        b2 += b[i] * b[i];
+        dinv[i] = row_diag_inv(indptr, indices, data, i);
+        if (nc == 1) row_mv<1>(indptr, indices, data, x, i, t);
+        else row_mv<2>(indptr, indices, data, x, i, t);
+        for (int c = 0; c < nc; ++c) {
+            r[i * nc + c] = b[i * nc + c] - t[c];
+            z[i * nc + c] = p[i * nc + c] = r[i * nc + c] * dinv[i];
+            rz[c] += r[i * nc + c] * z[i * nc + c];
+            b2[c] += b[i * nc + c] * b[i * nc + c];
+        }
     }
-    b2 = clamp30(b2);
+    for (int c = 0; c < nc; ++c) b2[c] = clamp30(b2[c]);
     int it_run = max_iters;
     for (int it = 0; it < max_iters; ++it) {
-        double pap = 0.0;
+        double pap[2] = {0, 0}, rr[2] = {0, 0}, rz2[2] = {0, 0}, al[2];
         for (int64_t i = 0; i < n; ++i) {
-            Ap[i] = row_dot(indptr, indices, data, p.data(), i);
-            pap += p[i] * Ap[i];
+            if (nc == 1) row_mv<1>(indptr, indices, data, p.data(), i, &Ap[i]);
+            else row_mv<2>(indptr, indices, data, p.data(), i, &Ap[2 * i]);
+            for (int c = 0; c < nc; ++c) pap[c] += p[i * nc + c] * Ap[i * nc + c];
         }
-        const double al = rz / clamp30(pap);
-        double rr = 0.0, rz2 = 0.0;
-        for (int64_t i = 0; i < n; ++i) {
-            x[i] += al * p[i];
-            r[i] -= al * Ap[i];
-            z[i] = r[i] * dinv[i];
-            rr += r[i] * r[i];
-            rz2 += r[i] * z[i];
-        }
-        if (it % check_every == 0 && rr < tol2 * b2) {
+        for (int c = 0; c < nc; ++c) al[c] = rz[c] / clamp30(pap[c]);
+        for (int64_t i = 0; i < n; ++i)
+            for (int c = 0; c < nc; ++c) {
+                x[i * nc + c] += al[c] * p[i * nc + c];
+                r[i * nc + c] -= al[c] * Ap[i * nc + c];
+                z[i * nc + c] = r[i * nc + c] * dinv[i];
+                rr[c] += r[i * nc + c] * r[i * nc + c];
+                rz2[c] += r[i * nc + c] * z[i * nc + c];
+            }
+        if (it % check_every == 0 && converged(rr, b2, nc, tol2, strict)) {
             it_run = it + 1;
             break;
         }
-        const double beta = rz2 / clamp30(rz);
-        rz = rz2;
-        for (int64_t i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
+        for (int c = 0; c < nc; ++c) {
+            const double beta = rz2[c] / clamp30(rz[c]);
+            rz[c] = rz2[c];
+            for (int64_t i = 0; i < n; ++i) p[i * nc + c] = z[i * nc + c] + beta * p[i * nc + c];
+        }
     }
     if (iters_out) *iters_out = it_run;
     return UVA_OK;

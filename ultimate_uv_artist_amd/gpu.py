@@ -2,12 +2,14 @@
 charts on one MI355X.
 
 ``Batch`` keeps the reference's constructor and methods (``orient_positive``,
-``repair_flips``, ``arap``, ``chart_uvs`` ... , gpu.py:66-299) but every
-method is one persistent HIP kernel over all charts: islands are pulled from
-a device work queue, each solved by a workgroup or a single wavefront with its
-own convergence state, instead of ~40 torch launches and a host sync per CG
-iteration.  The layout is kept in FP64 on the device; ``Batch.U`` returns it
-in the requested dtype.
+``repair_flips``, ``arap``, ``chart_uvs`` ... , gpu.py:66-299).  The per-chart
+work (orientation, repair, rotations and rhs, step limits) is one persistent
+HIP kernel over all charts, islands pulled from a device work queue.  The
+global step is the reference's one CG over the whole batch (one step length
+per column, shared by every chart): grid-wide Jacobi-PCG kernels on the
+batch's pinned Laplacian, resident on the GPU (``uva_csr_pcg``), with the
+stop test on the device instead of a host sync per CG iteration.  The layout
+is kept in FP64 on the device; ``Batch.U`` returns it in the requested dtype.
 
 The small torch helpers (``csr32``, ``spmm``, ``device``, ``unique_edges``,
 ``sym_csr``) keep their reference behaviour because ``hier.py`` imports them.
@@ -18,7 +20,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .plan import IslandPlan, require_gpu
+from .plan import CsrSystem, IslandPlan, pinned_laplacian, require_gpu
 
 BATCH_CG_TOL = 1e-6   # gpu.py:184 Batch._cg(tol=1e-6)
 
@@ -86,12 +88,12 @@ class Batch:
         self.n = int(self.offsets[-1])
         self._plan = IslandPlan([(p, t) for p, t, _ in charts], batch=True, dev=self.dev)
         self._P = np.concatenate([np.asarray(p, np.float64).reshape(-1, 3) for p, _, _ in charts])
-        self._whole = None
+        self._sys = None
         U = np.concatenate([np.asarray(u, np.float64).reshape(-1, 2) for _, _, u in charts])
         self._U = torch.as_tensor(U, device=self.dev).contiguous()
         toff = np.cumsum([0] + [len(t) for _, t, _ in charts])
-        self.T = torch.as_tensor(self._plan.T_local + np.repeat(self.offsets[:-1], np.diff(toff))[:, None],
-                                 device=self.dev)
+        self._Tg = self._plan.T_local + np.repeat(self.offsets[:-1], np.diff(toff))[:, None]
+        self.T = torch.as_tensor(self._Tg, device=self.dev)
         self.cv = torch.as_tensor(np.repeat(np.arange(self.nc), np.diff(self.offsets)), device=self.dev)
         self.ct = torch.as_tensor(np.repeat(np.arange(self.nc), np.diff(toff)), device=self.dev)
         self.pins = torch.as_tensor(self.offsets[:-1], device=self.dev)
@@ -115,8 +117,6 @@ class Batch:
     def scale(self, value):
         self._scale = value.to(self.dev, torch.float64).contiguous()
         self._plan.set_scale(self._scale)
-        if self._whole is not None:
-            self._whole.set_scale(self._scale)
 
     # ---------------------------------------------------------------- pieces
     def signed_area(self, U):
@@ -131,15 +131,16 @@ class Batch:
         return self._plan.flips_per_chart(U).to(self.dt)
 
     # ---------------------------------------------------------------- solvers
-    def _whole_plan(self):
-        """One-island plan of the whole batch (block-diagonal), soft pins at
-        every chart's first vertex with the batch-global weight: the global
-        step as the reference solves it, one CG over all charts together."""
-        if getattr(self, "_whole", None) is None:
-            self._whole = IslandPlan([(self._P, self.T.cpu().numpy())], batch=True, dev=self.dev)
-            self._whole.set_pins(self.offsets[:-1], 0.0)
-            self._whole.set_scale(self._scale)
-        return self._whole
+    def _system(self):
+        """The batch Laplacian (block-diagonal over the charts) with a soft pin
+        at every chart's first vertex of the batch-global weight max |W|
+        (gpu.py:126-142), resident on the GPU for the coupled CG; the chart
+        plan's rhs takes the same pin weight."""
+        if self._sys is None:
+            A, wpin = pinned_laplacian(self._P, self._Tg, self.offsets[:-1])
+            self._sys = CsrSystem(A, self.dev)
+            self._plan.set_pins(self.offsets[:-1], wpin)
+        return self._sys
 
     def arap(self, iters=8, line_search=True, cg_iters=40, cg_tol=BATCH_CG_TOL):
         """ARAP iterations of every chart (gpu.py:230-249): rotations x scale
@@ -147,13 +148,13 @@ class Batch:
         layout, stopped when both columns reach relative residual ``cg_tol``
         or after ``cg_iters`` iterations (gpu.py:184-202), revert of
         non-finite charts, per-chart flip-safe step when ``line_search``."""
-        whole = self._whole_plan()
+        system = self._system()
         U = self._U
         for _ in range(int(iters)):
-            b = whole.local_rhs(U, _lib.UVA_MODE_BATCH)
+            b = self._plan.local_rhs(U, _lib.UVA_MODE_BATCH)
             Un = U.clone()
             if cg_iters > 0:                                   # no CG iteration: the layout stays
-                whole.cg_solve(b, Un, tol=cg_tol, max_iters=int(cg_iters), precision=_lib.UVA_PREC_F64)
+                system.pcg(b, Un, int(cg_iters), check_every=1, tol2=float(cg_tol) ** 2, strict=False)
             bad = ~torch.isfinite(Un).all(1)
             if bool(bad.any()):
                 badc = torch.zeros(self.nc, dtype=torch.bool, device=self.dev)

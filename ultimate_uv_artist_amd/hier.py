@@ -9,9 +9,10 @@ plan and runs the reference's sequence on the GPU kernels:
   orient every chart by area-weighted sign         uva_orient_positive
   Jacobi flip repair (``repair_rounds``)            uva_repair_jacobi
   ``iters`` ARAP iterations:
-    rotations and rhs of the whole subset           uva_local_rhs (one-island plan)
-    one Jacobi-PCG of exactly ``cg_iters``          uva_cg_solve (one-island plan,
-    iterations over all charts together               per-chart pins: uva_plan_set_pins)
+    rotations and rhs of every chart                uva_local_rhs (pin weight of the
+                                                      subset: uva_plan_set_pins)
+    one Jacobi-PCG of exactly ``cg_iters``          uva_csr_pcg on the subset's pinned
+    iterations over all charts together               Laplacian
     (the reference's batched CG shares its
     step lengths between charts), from the
     current layout
@@ -43,33 +44,7 @@ import scipy.sparse as sp
 import torch
 
 from . import _lib
-from .plan import IslandPlan, require_gpu, stream_of
-
-
-def _half_cots(V, F):
-    """Half cotangent weights per face (t,3), the weight of corner k on the
-    opposite edge (k+1, k+2), in isometric face frames; zero on faces without
-    area (hier.py:361-385)."""
-    p0, p1, p2 = V[F[:, 0]], V[F[:, 1]], V[F[:, 2]]
-    e1, e2 = p1 - p0, p2 - p0
-    l1 = np.linalg.norm(e1, axis=1)
-    nrm = np.cross(e1, e2)
-    dbl = np.linalg.norm(nrm, axis=1)
-    ok = (dbl > 1e-14 * np.maximum(l1 * l1, 1e-300)) & np.isfinite(dbl) & (l1 > 0)
-    ex = e1 / np.maximum(l1, 1e-30)[:, None]
-    ey = np.cross(nrm / np.maximum(dbl, 1e-30)[:, None], ex)
-    X = np.zeros((len(F), 3, 2))
-    X[:, 1, 0] = l1
-    X[:, 2, 0] = (e2 * ex).sum(1)
-    X[:, 2, 1] = (e2 * ey).sum(1)
-    X[~ok] = 0.0
-    w = np.empty((len(F), 3))
-    for k in range(3):
-        a, b = X[:, (k + 1) % 3] - X[:, k], X[:, (k + 2) % 3] - X[:, k]
-        cr = np.maximum(np.abs(a[:, 0] * b[:, 1] - a[:, 1] * b[:, 0]), 1e-30)
-        w[:, k] = 0.5 * np.clip((a * b).sum(1) / cr, -1e4, 1e4)
-    w[~ok] = 0.0
-    return w
+from .plan import CsrSystem, IslandPlan, half_cots, pinned_laplacian, require_gpu, stream_of
 
 
 class Solver:
@@ -127,26 +102,27 @@ class Solver:
         if not charts:
             self._q = None
             return U0.astype(np.float32).astype(np.float64), flips, scale
-        # per-chart plan: orientation, repair, flips, step limits; one-island
-        # plan of the whole subset: the global step, one CG over every chart
-        # as the reference solves them (shared step lengths), one soft pin
-        # per chart with the subset's weight max |W|
+        # per-chart plan: orientation, repair, flips, rotations and rhs, step
+        # limits; the subset's Laplacian on the GPU: the global step, one CG
+        # over every chart as the reference solves them (shared step lengths),
+        # one soft pin per chart with the subset's weight max |W|
         plan = IslandPlan(charts, batch=True, dev=self.dev)
         voff = plan.vert_off
         Pall = np.concatenate([c[0] for c in charts])
         Tall = plan.T_local + np.repeat(voff[:-1], np.diff(plan.tri_off))[:, None]
-        whole = IslandPlan([(Pall, Tall)], batch=True, dev=self.dev)
-        whole.set_pins(voff[:-1], 0.0)
+        A, wpin = pinned_laplacian(Pall, Tall, voff[:-1])
+        system = CsrSystem(A, self.dev)
+        plan.set_pins(voff[:-1], wpin)
         cv = torch.as_tensor(np.repeat(np.arange(len(charts)), np.diff(voff)), device=self.dev)
         # the reference holds the layout in float32 (hier.py:400)
         U = plan.device_uv(U0[vorder].astype(np.float32).astype(np.float64))
         plan.orient_positive(U)
         plan.repair_jacobi(U, repair_rounds)
         for it in range(int(iters)):
-            b = whole.local_rhs(U, _lib.UVA_MODE_BATCH)
+            b = plan.local_rhs(U, _lib.UVA_MODE_BATCH)
             Un = U.clone()
-            if cg_iters > 0:                                   # no CG iteration: the layout stays
-                whole.cg_solve(b, Un, tol=0.0, max_iters=int(cg_iters), precision=_lib.UVA_PREC_F64)
+            if cg_iters > 0:                                   # exactly cg_iters iterations; none: the layout stays
+                system.pcg(b, Un, int(cg_iters), check_every=1, tol2=0.0, strict=True)
             bad = ~torch.isfinite(Un).all(1)
             if bool(bad.any()):                              # hier.py:500-503
                 badc = torch.zeros(len(charts), dtype=torch.bool, device=self.dev)
@@ -245,7 +221,7 @@ class Solver:
                 raise ValueError(f"{name} holds ids outside [0, {hi})")
         if nv == 0 or len(faces) == 0:
             return U0f
-        w = _half_cots(self.V, self.F[faces])
+        w = half_cots(self.V, self.F[faces])
         I = np.concatenate([CV[:, 1], CV[:, 2], CV[:, 0]])
         J = np.concatenate([CV[:, 2], CV[:, 0], CV[:, 1]])
         W = np.concatenate([w[:, 0], w[:, 1], w[:, 2]])

@@ -47,6 +47,101 @@ def concat_charts(charts):
     return vo, to, P, T
 
 
+def half_cots(V, F):
+    """Half cotangent weights per face (t,3), the weight of corner k on the
+    opposite edge (k+1, k+2), in isometric face frames, clipped to
+    [-5e3, 5e3]; zero on faces without area (gpu.py:88-115, hier.py:361-385)."""
+    V = np.asarray(V, np.float64)
+    F = np.asarray(F, np.int64)
+    p0, p1, p2 = V[F[:, 0]], V[F[:, 1]], V[F[:, 2]]
+    e1, e2 = p1 - p0, p2 - p0
+    l1 = np.linalg.norm(e1, axis=1)
+    nrm = np.cross(e1, e2)
+    dbl = np.linalg.norm(nrm, axis=1)
+    ok = (dbl > 1e-14 * np.maximum(l1 * l1, 1e-300)) & np.isfinite(dbl) & (l1 > 0)
+    ex = e1 / np.maximum(l1, 1e-30)[:, None]
+    ey = np.cross(nrm / np.maximum(dbl, 1e-30)[:, None], ex)
+    X = np.zeros((len(F), 3, 2))
+    X[:, 1, 0] = l1
+    X[:, 2, 0] = (e2 * ex).sum(1)
+    X[:, 2, 1] = (e2 * ey).sum(1)
+    X[~ok] = 0.0
+    w = np.empty((len(F), 3))
+    for k in range(3):
+        a, b = X[:, (k + 1) % 3] - X[:, k], X[:, (k + 2) % 3] - X[:, k]
+        cr = np.maximum(np.abs(a[:, 0] * b[:, 1] - a[:, 1] * b[:, 0]), 1e-30)
+        w[:, k] = 0.5 * np.clip((a * b).sum(1) / cr, -1e4, 1e4)
+    w[~ok] = 0.0
+    return w
+
+
+def pinned_laplacian(V, F, pins):
+    """The cotangent Laplacian of a batch (F global vertex ids, rows of every
+    chart together) plus a soft pin of weight max |W| (at least 1e-12) at
+    each of ``pins``: the matrix of gpu.py:126-142 Batch._build_laplacian and
+    hier.py:394-412.  Returns (scipy CSR, pin weight)."""
+    import scipy.sparse as sp
+
+    F = np.asarray(F, np.int64)
+    n = len(V)
+    w = half_cots(V, F)
+    I = np.concatenate([F[:, 1], F[:, 2], F[:, 0]])
+    J = np.concatenate([F[:, 2], F[:, 0], F[:, 1]])
+    W = np.concatenate([w[:, 0], w[:, 1], w[:, 2]])
+    wpin = max(float(np.abs(W).max()) if len(W) else 0.0, 1e-12)
+    pins = np.asarray(pins, np.int64).reshape(-1)
+    rows = np.concatenate([I, J, I, J, pins])
+    cols = np.concatenate([J, I, I, J, pins])
+    vals = np.concatenate([-W, -W, W, W, np.full(len(pins), wpin)])
+    A = sp.csr_matrix((vals, (rows, cols)), shape=(n, n))
+    A.sum_duplicates()
+    return A, wpin
+
+
+class CsrSystem:
+    """A symmetric matrix resident on the GPU (``uva_csr``) for the coupled
+    Jacobi-PCG solves that span a whole batch (``uva_csr_pcg``)."""
+
+    def __init__(self, A, dev=None):
+        self.dev = require_gpu(dev)
+        A = A.tocsr()
+        self.n = A.shape[0]
+        ip = np.ascontiguousarray(A.indptr, np.int64)
+        ix = np.ascontiguousarray(A.indices, np.int64)
+        a = np.ascontiguousarray(A.data, np.float64)
+        self._h = C.c_void_p()
+        with torch.cuda.device(self.dev):
+            check(_lib.lib().uva_csr_create(C.byref(self._h), self.n, i64p(ip), i64p(ix), f64p(a),
+                                            stream_of(self.dev)), "uva_csr_create")
+
+    def pcg(self, b, x, max_iters, check_every=1, tol2=0.0, strict=False):
+        """x (n,) or (n,2) float64 device tensor: start in, result out, for
+        rhs b of the same shape.  Stops after iteration k with k %
+        check_every == 0 when every column has r.r below (``strict``) or at
+        most tol2 max(b.b, 1e-30); returns the iterations run."""
+        if b.shape != x.shape or b.shape[0] != self.n or b.dim() > 2 or (b.dim() == 2 and b.shape[1] > 2):
+            raise ValueError("b and x must both be (n,) or (n, 1..2)")
+        if x.dtype != torch.float64 or b.dtype != torch.float64 or not x.is_contiguous() or not b.is_contiguous():
+            raise ValueError("b and x must be contiguous float64 tensors")
+        nc = 1 if b.dim() == 1 else b.shape[1]
+        its = C.c_int32(0)
+        check(_lib.lib().uva_csr_pcg(self._h, ptr(b), ptr(x), int(nc), int(max_iters), int(check_every),
+                                     float(tol2), int(bool(strict)), C.byref(its), stream_of(self.dev)),
+              "uva_csr_pcg")
+        return its.value
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            _lib.lib().uva_csr_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class IslandPlan:
     """A batch of islands on one GPU.
 
