@@ -49,12 +49,14 @@ def parse():
                     help="keep the size-based schedule (no re-planning from measured iterations)")
     ap.add_argument("--no-plain", dest="plain", action="store_false",
                     help="skip the plain per-island GPU baseline")
-    ap.add_argument("--workload", default="disks", choices=["disks", "heightfield", "needles", "validate"],
+    ap.add_argument("--workload", default="disks", choices=["disks", "heightfield", "needles", "validate", "batch"],
                     help="disks: the throughput workload; heightfield: 3,988 islands of 50-1,946 triangles "
                          "cut from a 2M-triangle heightfield (use --iters 15); needles: the precision study on "
                          "256 islands of 5k-60k triangles with 10%% needle triangles (one JSON line with every "
                          "precision variant); validate: validate_uvs of the 512 solved islands packed "
-                         "into one atlas, GPU overlap test against the CPU restatement on a sample")
+                         "into one atlas, GPU overlap test against the CPU restatement on a sample; batch: "
+                         "gpu.Batch.arap(8) of the 512 islands as one batch (one coupled 40-iteration CG "
+                         "per global step), against the oracle's Batch on a sample")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB 5 times (HBM counter calibration)")
     return ap.parse_args()
@@ -258,6 +260,104 @@ def validate_study(args):
                       "flags_equal_on_sample": same, "report": rep}), flush=True)
 
 
+def batch_study(args):
+    """gpu.Batch on the 512 disk islands as one batch (3.83M triangles) from
+    their LSCM layouts: orient, repair, then the timed Batch.arap(8,
+    line_search=True) of the reference's defaults (one Jacobi-PCG of at most
+    40 iterations over every chart per global step).  Reports Mtri*iter/s,
+    the coupled CG's HBM roofline (uva_csr_pcg, HIP events around 40
+    iterations), and the oracle's Batch with the same coupled CG on a sample
+    of 16 islands (one core), whose GPU result it also checks."""
+    import torch
+
+    from oracle.batch_cpu import BatchOracle
+    from ultimate_uv_artist_amd import gpu, workloads
+    from ultimate_uv_artist_amd.plan import IslandPlan
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    charts = workloads.disk_islands(seed=2, n_islands=args.islands)
+    plan = IslandPlan(charts, dev=dev)
+    U = torch.zeros(plan.n, 2, dtype=torch.float64, device=dev)
+    plan.lscm(U)
+    Uh = U.cpu().numpy()
+    vo = plan.vert_off
+    items = [(P, T, Uh[vo[i]:vo[i + 1]]) for i, (P, T) in enumerate(charts)]
+    del plan
+    nt = int(sum(len(T) for _, T in charts))
+    t0 = time.perf_counter()
+    b = gpu.Batch(items, dev=dev, dtype=torch.float64)
+    b.orient_positive()
+    b.repair_flips(80)
+    sysm = b._system()
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t0
+    U0 = b._U.clone()
+    print(f"bench: batch of {len(charts)} charts, {nt} triangles, setup {t_setup:.2f} s", flush=True)
+    b.arap(1, line_search=True)                      # warm-up
+    times = []
+    for k in range(max(args.steps, 1)):
+        b._U.copy_(U0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        b.arap(8, line_search=True)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+        print(f"bench: Batch.arap(8) step {k}: {times[-1] * 1e3:.1f} ms", flush=True)
+    t_step = float(np.median(times))
+    # the coupled CG alone: 40 iterations on the global step's rhs, HIP events
+    rhs = b._plan.local_rhs(U0, 1)
+    x = U0.clone()
+    s = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    cg_ms = []
+    for _ in range(5):
+        x.copy_(U0)
+        e0.record(s)
+        its = sysm.pcg(rhs, x, 40, check_every=1, tol2=0.0, strict=True)
+        e1.record(s)
+        torch.cuda.synchronize()
+        cg_ms.append(e0.elapsed_time(e1))
+    cg_t = float(np.median(cg_ms)) / 1e3
+    n, nnz = sysm.n, sysm.nnz
+    # algorithmic bytes of one iteration of this design (two float64 columns):
+    # SpMV 8 B value + 8 B index per entry, 8 B row pointer, p gathered once,
+    # Ap written (16 nnz + 8 (n+1) + 32 n); update reads D, p, Ap, x, r and
+    # writes x, r, z (120 n); direction reads z, p and writes p (48 n)
+    per_it = 16 * nnz + 8 * (n + 1) + 200 * n
+    achieved = per_it * its / cg_t / 1e9
+    # CPU: the oracle's Batch with the same coupled CG on 16 islands, one core
+    k = min(16, len(items))
+    sample = [(P, T, U0.cpu().numpy()[b.offsets[i]:b.offsets[i + 1]]) for i, (P, T, _) in enumerate(items[:k])]
+    _one_thread()
+    o = BatchOracle(sample)
+    c0 = time.perf_counter()
+    Uo = o.arap(8, line_search=True, cg_iters=40)
+    t_cpu = time.perf_counter() - c0
+    nt_s = int(sum(len(T) for _, T, _ in sample))
+    bs = gpu.Batch(sample, dev=dev, dtype=torch.float64)
+    Ug = bs.arap(8, line_search=True).cpu().numpy()
+    err = 0.0
+    for i in range(k):
+        sl = slice(o.offsets[i], o.offsets[i + 1])
+        err = max(err, float(np.abs(Ug[sl] - Uo[sl]).max() / np.ptp(Uo[sl], 0).max()))
+    line = {"workload": f"gpu.Batch.arap(8, line_search) of {len(charts)} disk islands as one batch, "
+                        f"one coupled CG of 40 iterations per global step",
+            "triangles": nt, "ms_per_step": round(t_step * 1e3, 2),
+            "mtri_iter_per_s": round(nt * 8 / t_step / 1e6, 1),
+            "coupled_cg": {"iterations": its, "ms": round(cg_t * 1e3, 3), "rows": n, "nnz": nnz,
+                           "bytes_per_iteration": per_it, "achieved_GBps": round(achieved, 1),
+                           "frac": round(achieved / 8000.0, 4)},
+            "cpu_baseline": {"value": round(nt_s * 8 / t_cpu / 1e6, 3), "unit": "Mtri*iter/s", "cores": 1,
+                             "kind": "port", "sample": f"BatchOracle with the coupled CG on the first {k} islands "
+                                                        f"({nt_s} triangles)"},
+            "parity": {"sample_islands": k, "max_unit_box_err": err}}
+    print(f"bench: Batch.arap(8) {t_step * 1e3:.1f} ms = {line['mtri_iter_per_s']} Mtri*iter/s; coupled CG "
+          f"{cg_t * 1e3:.2f} ms / 40 iterations = {achieved:.0f} GB/s; CPU oracle {line['cpu_baseline']['value']} "
+          f"Mtri*iter/s on {k} islands; sample error {err:.2e}", flush=True)
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     if args.copy_probe:
@@ -272,6 +372,9 @@ def main():
         return
     if args.workload == "validate":
         validate_study(args)
+        return
+    if args.workload == "batch":
+        batch_study(args)
         return
     import torch
 

@@ -151,11 +151,36 @@ class BatchOracle:
         np.minimum.at(out, self.ct, t)
         return out
 
-    def arap(self, iters=8, line_search=True):
-        """gpu.py:230-249 with exact global solves."""
+    def cg(self, b, x, iters=40, tol=1e-6):
+        """gpu.py:184-202 Batch._cg: one Jacobi-PCG over every chart, a step
+        length per column, stopped when both columns reach r.r <= tol^2
+        max(b.b, 1e-30) or after ``iters`` iterations."""
+        L = self.L.tocsr()
+        dinv = (1.0 / np.maximum(L.diagonal(), 1e-30))[:, None]
+        r = b - L @ x
+        z = r * dinv
+        p = z.copy()
+        rz = (r * z).sum(0)
+        b2 = np.maximum((b * b).sum(0), 1e-30)
+        for _ in range(iters):
+            Ap = L @ p
+            al = rz / np.maximum((p * Ap).sum(0), 1e-30)
+            x = x + p * al
+            r = r - Ap * al
+            if ((r * r).sum(0) <= tol * tol * b2).all():
+                break
+            z = r * dinv
+            rz2 = (r * z).sum(0)
+            p = z + p * (rz2 / np.maximum(rz, 1e-30))
+            rz = rz2
+        return x
+
+    def arap(self, iters=8, line_search=True, cg_iters=None):
+        """gpu.py:230-249; global steps solved exactly, or (``cg_iters``) by
+        the reference's coupled CG of that budget."""
         U = self.U
         for _ in range(iters):
-            Un = self._lu.solve(self.rhs(U))
+            Un = self._lu.solve(self.rhs(U)) if cg_iters is None else self.cg(self.rhs(U), U.copy(), cg_iters)
             badv = ~np.isfinite(Un).all(1)
             if badv.any():
                 badc = np.zeros(self.nc, bool)

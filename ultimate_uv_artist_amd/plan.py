@@ -106,6 +106,7 @@ class CsrSystem:
         self.dev = require_gpu(dev)
         A = A.tocsr()
         self.n = A.shape[0]
+        self.nnz = int(A.indptr[-1])
         ip = np.ascontiguousarray(A.indptr, np.int64)
         ix = np.ascontiguousarray(A.indices, np.int64)
         a = np.ascontiguousarray(A.data, np.float64)
