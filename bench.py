@@ -320,11 +320,11 @@ def batch_study(args):
         cg_ms.append(e0.elapsed_time(e1))
     cg_t = float(np.median(cg_ms)) / 1e3
     n, nnz = sysm.n, sysm.nnz
-    # algorithmic bytes of one iteration of this design (two float64 columns):
-    # SpMV 8 B value + 8 B index per entry, 8 B row pointer, p gathered once,
-    # Ap written (16 nnz + 8 (n+1) + 32 n); update reads D, p, Ap, x, r and
-    # writes x, r, z (120 n); direction reads z, p and writes p (48 n)
-    per_it = 16 * nnz + 8 * (n + 1) + 200 * n
+    # algorithmic bytes of one iteration (two float64 columns): SpMV 8 B value
+    # + 4 B column id per stored entry, p gathered once, Ap written (12 nnz +
+    # 32 n); update reads D, p, Ap, x, r and writes x, r, z (120 n); direction
+    # reads z, p and writes p (48 n).  The sliced-ELL padding is not counted.
+    per_it = 12 * nnz + 200 * n
     achieved = per_it * its / cg_t / 1e9
     # CPU: the oracle's Batch with the same coupled CG on 16 islands, one core
     k = min(16, len(items))

@@ -20,8 +20,16 @@
 // on the device (a flag every later launch reads first), so a solve queues
 // all its launches without a host round trip.  Dot products are block
 // partial sums in a fixed tree order followed by a fixed-order reduction, so
-// repeated runs are bitwise identical.  Rows are one thread each (CSR
-// gather; the Laplacians here hold about 7 entries per row).
+// repeated runs are bitwise identical.  (Folding the reductions into the
+// producing kernels with a last-workgroup-done counter needs a device-scope
+// fence per workgroup, which on this multi-XCD part writes back L2 and made
+// the solve three times slower.)
+//
+// The matrix is held in sliced ELL: per slice of 64 rows (one wavefront) the
+// entries of its rows column-major, width = the slice's longest row, 32-bit
+// column ids (-1 pads), so each load of a wavefront is one contiguous 256 or
+// 512-byte run.  Entries keep the CSR order of their row, so a row's sum is
+// the CSR sum.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -50,17 +58,47 @@ __host__ __device__ inline double row_diag_inv(const int64_t* ip, const int64_t*
     return 1.0 / clamp30(g);
 }
 
+constexpr int kSl = 64;       // rows per slice of the sliced-ELL matrix
+
+// y = A v of row i from the sliced-ELL arrays (slice offsets sp in entries)
 template <int NC>
-__host__ __device__ inline void row_mv(const int64_t* ip, const int64_t* ix, const double* a, const double* v,
-                                       int64_t i, double* out) {
+__host__ __device__ inline void row_mv_sell(const int64_t* sp, const int* col, const double* val, const double* v,
+                                            int64_t i, double* out) {
     double s[NC];
     for (int c = 0; c < NC; ++c) s[c] = 0.0;
-    for (int64_t q = ip[i]; q < ip[i + 1]; ++q) {
-        const double w = a[q];
-        const int64_t j = ix[q];
-        for (int c = 0; c < NC; ++c) s[c] += w * v[j * NC + c];
+    const int64_t sl = i / kSl, lane = i % kSl;
+    const int64_t w = (sp[sl + 1] - sp[sl]) / kSl;
+    for (int64_t k = 0; k < w; ++k) {
+        const int64_t q = sp[sl] + k * kSl + lane;
+        const int j = col[q];
+        if (j >= 0) {
+            const double a = val[q];
+            for (int c = 0; c < NC; ++c) s[c] += a * v[(int64_t)j * NC + c];
+        }
     }
     for (int c = 0; c < NC; ++c) out[c] = s[c];
+}
+
+// sliced ELL of a CSR matrix (n rows): slice offsets (ns + 1), columns, values
+inline void build_sell(int64_t n, const int64_t* ip, const int64_t* ix, const double* a, std::vector<int64_t>& sp,
+                       std::vector<int>& col, std::vector<double>& val) {
+    const int64_t ns = (n + kSl - 1) / kSl;
+    sp.assign(ns + 1, 0);
+    for (int64_t sl = 0; sl < ns; ++sl) {
+        int64_t w = 0;
+        for (int64_t i = sl * kSl; i < n && i < (sl + 1) * kSl; ++i) w = ip[i + 1] - ip[i] > w ? ip[i + 1] - ip[i] : w;
+        sp[sl + 1] = sp[sl] + w * kSl;
+    }
+    col.assign(sp[ns], -1);
+    val.assign(sp[ns], 0.0);
+    for (int64_t i = 0; i < n; ++i) {
+        const int64_t sl = i / kSl, lane = i % kSl;
+        for (int64_t q = ip[i]; q < ip[i + 1]; ++q) {
+            const int64_t o = sp[sl] + (q - ip[i]) * kSl + lane;
+            col[o] = (int)ix[q];
+            val[o] = a[q];
+        }
+    }
 }
 
 // stop test of one check: every column below its threshold
@@ -80,6 +118,7 @@ inline const char* check_csr(int64_t n, const int64_t* indptr, const int64_t* in
     if (indptr[n] > 0 && (!indices || !data)) return "null indices or data";
     for (int64_t q = 0; q < indptr[n]; ++q)
         if (indices[q] < 0 || indices[q] >= n) return "column index out of range";
+    if (n > 0x7fffffffLL) return "matrix too large for 32-bit column ids";
     return "";
 }
 
@@ -88,16 +127,17 @@ inline const char* check_csr(int64_t n, const int64_t* indptr, const int64_t* in
 
 #if !defined(UVA_EMULATOR)
 struct uva_csr {
-    int64_t n = 0, nnz = 0, nb = 0;
-    int64_t *ip = nullptr, *ix = nullptr;
-    double *a = nullptr, *dinv = nullptr, *work = nullptr, *part = nullptr;
+    int64_t n = 0, nnz = 0, nb = 0, nsell = 0;
+    int64_t* sp = nullptr;           // slice offsets (entries)
+    int* col = nullptr;              // column ids, -1 pads
+    double *val = nullptr, *dinv = nullptr, *work = nullptr, *part = nullptr;
     uva::pcg::Scalars* sc = nullptr;
 };
 
 namespace uva {
 namespace pcg {
 
-// block sums of K values per thread into part[K*block + k]
+// block sums of K values per thread into part[K * block + k]
 template <int K>
 __device__ inline void block_sum(const double (&v)[K], double* part) {
     __shared__ double s[K][kB];
@@ -110,87 +150,6 @@ __device__ inline void block_sum(const double (&v)[K], double* part) {
     }
     if (threadIdx.x == 0)
         for (int k = 0; k < K; ++k) part[K * blockIdx.x + k] = s[k][0];
-}
-
-__global__ void diag_kernel(int64_t n, const int64_t* ip, const int64_t* ix, const double* a, double* dinv) {
-    const int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x;
-    if (i < n) dinv[i] = row_diag_inv(ip, ix, a, i);
-}
-
-// r = b - A x, z = D r, p = z; partial r.z and b.b per column
-template <int NC>
-__global__ void init_kernel(int64_t n, const int64_t* ip, const int64_t* ix, const double* a, const double* dinv,
-                            const double* b, const double* x, double* r, double* z, double* p, double* part) {
-    const int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x;
-    double v[2 * NC];
-    for (int k = 0; k < 2 * NC; ++k) v[k] = 0.0;
-    if (i < n) {
-        double ax[NC];
-        row_mv<NC>(ip, ix, a, x, i, ax);
-        const double di = dinv[i];
-        for (int c = 0; c < NC; ++c) {
-            const double bi = b[i * NC + c];
-            const double ri = bi - ax[c];
-            r[i * NC + c] = ri;
-            z[i * NC + c] = ri * di;
-            p[i * NC + c] = ri * di;
-            v[c] = ri * (ri * di);
-            v[NC + c] = bi * bi;
-        }
-    }
-    block_sum<2 * NC>(v, part);
-}
-
-// Ap = A p; partial p.Ap per column
-template <int NC>
-__global__ void spmv_kernel(int64_t n, const int64_t* ip, const int64_t* ix, const double* a, const double* p,
-                            double* Ap, double* part, const Scalars* s) {
-    if (s->done) return;
-    const int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x;
-    double v[NC];
-    for (int c = 0; c < NC; ++c) v[c] = 0.0;
-    if (i < n) {
-        double ap[NC];
-        row_mv<NC>(ip, ix, a, p, i, ap);
-        for (int c = 0; c < NC; ++c) {
-            Ap[i * NC + c] = ap[c];
-            v[c] = p[i * NC + c] * ap[c];
-        }
-    }
-    block_sum<NC>(v, part);
-}
-
-// x += al p, r -= al Ap, z = D r; partial r.r and r.z per column
-template <int NC>
-__global__ void update_kernel(int64_t n, const double* dinv, const double* p, const double* Ap, double* x, double* r,
-                              double* z, double* part, const Scalars* s) {
-    if (s->done) return;
-    const int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x;
-    double v[2 * NC];
-    for (int k = 0; k < 2 * NC; ++k) v[k] = 0.0;
-    if (i < n) {
-        const double di = dinv[i];
-        for (int c = 0; c < NC; ++c) {
-            const double al = s->al[c];
-            x[i * NC + c] += al * p[i * NC + c];
-            const double ri = r[i * NC + c] - al * Ap[i * NC + c];
-            const double zi = ri * di;
-            r[i * NC + c] = ri;
-            z[i * NC + c] = zi;
-            v[c] = ri * ri;
-            v[NC + c] = ri * zi;
-        }
-    }
-    block_sum<2 * NC>(v, part);
-}
-
-// p = z + beta p
-template <int NC>
-__global__ void direction_kernel(int64_t n, const double* z, double* p, const Scalars* s) {
-    if (s->done) return;
-    const int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x;
-    if (i < n)
-        for (int c = 0; c < NC; ++c) p[i * NC + c] = z[i * NC + c] + s->beta[c] * p[i * NC + c];
 }
 
 // fixed-order sums of the nb block partials (K per block) for NC columns:
@@ -237,16 +196,91 @@ __global__ void reduce_kernel(const double* part, int64_t nb, int stage, int it,
 }
 
 template <int NC>
+__global__ void init_kernel(int64_t n, const int64_t* sp, const int* col, const double* val, const double* dinv,
+                            const double* b, const double* x, double* r, double* z, double* p, double* part) {
+    const int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x;
+    double v[2 * NC];
+    for (int k = 0; k < 2 * NC; ++k) v[k] = 0.0;
+    if (i < n) {
+        double ax[NC];
+        row_mv_sell<NC>(sp, col, val, x, i, ax);
+        const double di = dinv[i];
+        for (int c = 0; c < NC; ++c) {
+            const double bi = b[i * NC + c];
+            const double ri = bi - ax[c];
+            r[i * NC + c] = ri;
+            z[i * NC + c] = ri * di;
+            p[i * NC + c] = ri * di;
+            v[c] = ri * (ri * di);
+            v[NC + c] = bi * bi;
+        }
+    }
+    block_sum<2 * NC>(v, part);
+}
+
+// Ap = A p; partial p.Ap per column
+template <int NC>
+__global__ void spmv_kernel(int64_t n, const int64_t* sp, const int* col, const double* val, const double* p,
+                            double* Ap, double* part, const Scalars* s) {
+    if (s->done) return;
+    const int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x;
+    double v[NC];
+    for (int c = 0; c < NC; ++c) v[c] = 0.0;
+    if (i < n) {
+        double ap[NC];
+        row_mv_sell<NC>(sp, col, val, p, i, ap);
+        for (int c = 0; c < NC; ++c) {
+            Ap[i * NC + c] = ap[c];
+            v[c] = p[i * NC + c] * ap[c];
+        }
+    }
+    block_sum<NC>(v, part);
+}
+
+// x += al p, r -= al Ap, z = D r; partial r.r and r.z per column
+template <int NC>
+__global__ void update_kernel(int64_t n, const double* dinv, const double* p, const double* Ap, double* x, double* r,
+                              double* z, double* part, const Scalars* s) {
+    if (s->done) return;
+    const int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x;
+    double v[2 * NC];
+    for (int k = 0; k < 2 * NC; ++k) v[k] = 0.0;
+    if (i < n) {
+        const double di = dinv[i];
+        for (int c = 0; c < NC; ++c) {
+            const double al = s->al[c];
+            x[i * NC + c] += al * p[i * NC + c];
+            const double ri = r[i * NC + c] - al * Ap[i * NC + c];
+            const double zi = ri * di;
+            r[i * NC + c] = ri;
+            z[i * NC + c] = zi;
+            v[c] = ri * ri;
+            v[NC + c] = ri * zi;
+        }
+    }
+    block_sum<2 * NC>(v, part);
+}
+
+// p = z + beta p
+template <int NC>
+__global__ void direction_kernel(int64_t n, const double* z, double* p, const Scalars* s) {
+    if (s->done) return;
+    const int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x;
+    if (i < n)
+        for (int c = 0; c < NC; ++c) p[i * NC + c] = z[i * NC + c] + s->beta[c] * p[i * NC + c];
+}
+
+template <int NC>
 int solve(uva_csr* m, const double* b, double* x, int max_iters, int check_every, double tol2, int strict,
           hipStream_t st) {
     const int64_t n = m->n, nb = m->nb;
     double *r = m->work, *z = r + NC * n, *p = z + NC * n, *Ap = p + NC * n;
     const dim3 grid((unsigned)nb), blk(kB), one(1), red(kR);
-    hipLaunchKernelGGL(init_kernel<NC>, grid, blk, 0, st, n, m->ip, m->ix, m->a, m->dinv, b, x, r, z, p, m->part);
+    hipLaunchKernelGGL(init_kernel<NC>, grid, blk, 0, st, n, m->sp, m->col, m->val, m->dinv, b, x, r, z, p, m->part);
     hipLaunchKernelGGL((reduce_kernel<NC, 2 * NC>), one, red, 0, st, m->part, nb, 0, 0, max_iters, check_every,
                        tol2, strict, m->sc);
     for (int it = 0; it < max_iters; ++it) {
-        hipLaunchKernelGGL(spmv_kernel<NC>, grid, blk, 0, st, n, m->ip, m->ix, m->a, p, Ap, m->part, m->sc);
+        hipLaunchKernelGGL(spmv_kernel<NC>, grid, blk, 0, st, n, m->sp, m->col, m->val, p, Ap, m->part, m->sc);
         hipLaunchKernelGGL((reduce_kernel<NC, NC>), one, red, 0, st, m->part, nb, 1, it, max_iters, check_every,
                            tol2, strict, m->sc);
         hipLaunchKernelGGL(update_kernel<NC>, grid, blk, 0, st, n, m->dinv, p, Ap, x, r, z, m->part, m->sc);
@@ -260,6 +294,17 @@ int solve(uva_csr* m, const double* b, double* x, int max_iters, int check_every
 }  // namespace pcg
 }  // namespace uva
 
+static void csr_free(uva_csr* m) {
+    (void)hipFree(m->sp);
+    (void)hipFree(m->col);
+    (void)hipFree(m->val);
+    (void)hipFree(m->dinv);
+    (void)hipFree(m->work);
+    (void)hipFree(m->part);
+    (void)hipFree(m->sc);
+    delete m;
+}
+
 extern "C" int uva_csr_create(uva_csr** out, int64_t n, const int64_t* indptr, const int64_t* indices,
                               const double* data, void* stream) {
     using namespace uva::pcg;
@@ -269,35 +314,34 @@ extern "C" int uva_csr_create(uva_csr** out, int64_t n, const int64_t* indptr, c
     const int64_t nb = (n + kB - 1) / kB;
     if (nb > (int64_t)0x7fffffff) return UVA_EINVAL;
     hipStream_t st = (hipStream_t)stream;
+    std::vector<int64_t> sp;
+    std::vector<int> col;
+    std::vector<double> val, dinv((size_t)n);
+    build_sell(n, indptr, indices, data, sp, col, val);
+    for (int64_t i = 0; i < n; ++i) dinv[i] = row_diag_inv(indptr, indices, data, i);
     uva_csr* m = new uva_csr();
     m->n = n;
     m->nnz = indptr[n];
     m->nb = nb;
-    const size_t nz = (size_t)m->nnz;
-    auto fail = [&](int rc) {
-        (void)hipStreamSynchronize(st);
-        (void)hipFree(m->ip);
-        (void)hipFree(m->ix);
-        (void)hipFree(m->a);
-        (void)hipFree(m->dinv);
-        (void)hipFree(m->work);
-        (void)hipFree(m->part);
-        (void)hipFree(m->sc);
-        delete m;
-        return rc;
-    };
+    m->nsell = (int64_t)col.size();
+    const size_t ne = col.size() + 1;
     // work: r, z, p, Ap for up to two columns
-    if (hipMalloc(&m->ip, ((size_t)n + 1) * 8) != hipSuccess || hipMalloc(&m->ix, (nz + 1) * 8) != hipSuccess ||
-        hipMalloc(&m->a, (nz + 1) * 8) != hipSuccess || hipMalloc(&m->dinv, (size_t)n * 8) != hipSuccess ||
+    if (hipMalloc(&m->sp, sp.size() * 8) != hipSuccess || hipMalloc(&m->col, ne * 4) != hipSuccess ||
+        hipMalloc(&m->val, ne * 8) != hipSuccess || hipMalloc(&m->dinv, (size_t)n * 8) != hipSuccess ||
         hipMalloc(&m->work, (size_t)n * 8 * 8) != hipSuccess || hipMalloc(&m->part, (size_t)nb * 4 * 8) != hipSuccess ||
-        hipMalloc(&m->sc, sizeof(Scalars)) != hipSuccess)
-        return fail(UVA_ENOMEM);
-    if (hipMemcpyAsync(m->ip, indptr, ((size_t)n + 1) * 8, hipMemcpyHostToDevice, st) != hipSuccess ||
-        (nz && hipMemcpyAsync(m->ix, indices, nz * 8, hipMemcpyHostToDevice, st) != hipSuccess) ||
-        (nz && hipMemcpyAsync(m->a, data, nz * 8, hipMemcpyHostToDevice, st) != hipSuccess))
-        return fail(UVA_EHIP);
-    hipLaunchKernelGGL(diag_kernel, dim3((unsigned)nb), dim3(kB), 0, st, n, m->ip, m->ix, m->a, m->dinv);
-    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess) return fail(UVA_EHIP);
+        hipMalloc(&m->sc, sizeof(Scalars)) != hipSuccess) {
+        csr_free(m);
+        return UVA_ENOMEM;
+    }
+    if (hipMemcpyAsync(m->sp, sp.data(), sp.size() * 8, hipMemcpyHostToDevice, st) != hipSuccess ||
+        (!col.empty() && hipMemcpyAsync(m->col, col.data(), col.size() * 4, hipMemcpyHostToDevice, st) != hipSuccess) ||
+        (!val.empty() && hipMemcpyAsync(m->val, val.data(), val.size() * 8, hipMemcpyHostToDevice, st) != hipSuccess) ||
+        hipMemcpyAsync(m->dinv, dinv.data(), (size_t)n * 8, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess) {
+        (void)hipStreamSynchronize(st);
+        csr_free(m);
+        return UVA_EHIP;
+    }
     *out = m;
     return UVA_OK;
 }
@@ -305,14 +349,7 @@ extern "C" int uva_csr_create(uva_csr** out, int64_t n, const int64_t* indptr, c
 extern "C" int uva_csr_destroy(uva_csr* m) {
     if (!m) return UVA_OK;
     (void)hipDeviceSynchronize();
-    (void)hipFree(m->ip);
-    (void)hipFree(m->ix);
-    (void)hipFree(m->a);
-    (void)hipFree(m->dinv);
-    (void)hipFree(m->work);
-    (void)hipFree(m->part);
-    (void)hipFree(m->sc);
-    delete m;
+    csr_free(m);
     return UVA_OK;
 }
 
@@ -352,9 +389,9 @@ extern "C" int uva_jacobi_pcg(int64_t n, const int64_t* indptr, const int64_t* i
     return rc;
 }
 #else
-// The same iteration on the host, rows in order and sequential sums
-// (test-only emulation; equal to the device result up to the order of the
-// dot-product sums).  b and x are host arrays, n x ncols.
+// The same iteration on the host, on the same sliced-ELL arrays and row code,
+// rows in order and sequential sums (test-only emulation; equal to the device
+// result up to the order of the dot-product sums).  b and x are host arrays, n x ncols.
 extern "C" int uva_emu_csr_pcg(int64_t n, const int64_t* indptr, const int64_t* indices, const double* data,
                                const double* b, double* x, int32_t ncols, int32_t max_iters, int32_t check_every,
                                double tol2, int32_t strict, int32_t* iters_out) {
@@ -364,12 +401,16 @@ extern "C" int uva_emu_csr_pcg(int64_t n, const int64_t* indptr, const int64_t* 
     if (iters_out) *iters_out = 0;
     if (max_iters == 0) return UVA_OK;
     const int nc = ncols;
+    std::vector<int64_t> sp;
+    std::vector<int> col;
+    std::vector<double> val;
+    build_sell(n, indptr, indices, data, sp, col, val);
     std::vector<double> dinv(n), r(n * nc), z(n * nc), p(n * nc), Ap(n * nc);
     double rz[2] = {0, 0}, b2[2] = {0, 0}, t[2];
     for (int64_t i = 0; i < n; ++i) {
         dinv[i] = row_diag_inv(indptr, indices, data, i);
-        if (nc == 1) row_mv<1>(indptr, indices, data, x, i, t);
-        else row_mv<2>(indptr, indices, data, x, i, t);
+        if (nc == 1) row_mv_sell<1>(sp.data(), col.data(), val.data(), x, i, t);
+        else row_mv_sell<2>(sp.data(), col.data(), val.data(), x, i, t);
         for (int c = 0; c < nc; ++c) {
             r[i * nc + c] = b[i * nc + c] - t[c];
             z[i * nc + c] = p[i * nc + c] = r[i * nc + c] * dinv[i];
@@ -382,8 +423,8 @@ extern "C" int uva_emu_csr_pcg(int64_t n, const int64_t* indptr, const int64_t* 
     for (int it = 0; it < max_iters; ++it) {
         double pap[2] = {0, 0}, rr[2] = {0, 0}, rz2[2] = {0, 0}, al[2];
         for (int64_t i = 0; i < n; ++i) {
-            if (nc == 1) row_mv<1>(indptr, indices, data, p.data(), i, &Ap[i]);
-            else row_mv<2>(indptr, indices, data, p.data(), i, &Ap[2 * i]);
+            if (nc == 1) row_mv_sell<1>(sp.data(), col.data(), val.data(), p.data(), i, &Ap[i]);
+            else row_mv_sell<2>(sp.data(), col.data(), val.data(), p.data(), i, &Ap[2 * i]);
             for (int c = 0; c < nc; ++c) pap[c] += p[i * nc + c] * Ap[i * nc + c];
         }
         for (int c = 0; c < nc; ++c) al[c] = rz[c] / clamp30(pap[c]);
