@@ -704,3 +704,62 @@ def test_validation_on_gpu_matches_reference_and_oracle():
     r1, p1 = validate.validate_uvs(P, T, U, gutter=False)
     r2, p2 = vc.validate_uvs(P, T, U)
     assert r1 == r2 and np.array_equal(p1, p2)
+
+
+def test_csr_pcg_matches_the_definition():
+    """uva_csr_pcg (CsrSystem.pcg): solves in a row on one handle with other
+    right-hand sides and starts, eager torch work and synchronisations
+    between them, two columns at the Batch rule and one column at the lscm
+    rule, each against the Jacobi-PCG definition in numpy (iterations equal,
+    x to 1e-10 relative)."""
+    import scipy.sparse as sp
+    import torch
+
+    from ultimate_uv_artist_amd.plan import CsrSystem
+
+    rng = np.random.default_rng(91)
+    n = 5000
+    i = rng.integers(0, n, 20000)
+    j = rng.integers(0, n, 20000)
+    v = rng.uniform(-1, 0, 20000)
+    A = sp.coo_matrix((np.r_[v, v], (np.r_[i, j], np.r_[j, i])), shape=(n, n)).tocsr()
+    A = (A + sp.diags(np.asarray(abs(A).sum(1)).ravel() + rng.uniform(0.01, 0.1, n))).tocsr()
+    dinv = 1.0 / A.diagonal()
+
+    def pcg(B, X, iters, check, tol2, strict):
+        B, x = B.reshape(n, -1), X.reshape(n, -1).copy()
+        r = B - A @ x
+        z = r * dinv[:, None]
+        p = z.copy()
+        rz = (r * z).sum(0)
+        b2 = np.maximum((B * B).sum(0), 1e-30)
+        for k in range(iters):
+            Ap = A @ p
+            al = rz / np.maximum((p * Ap).sum(0), 1e-30)
+            x, r = x + al * p, r - al * Ap
+            rr = (r * r).sum(0)
+            if k % check == 0 and ((rr < tol2 * b2) if strict else (rr <= tol2 * b2)).all():
+                return x.reshape(X.shape), k + 1
+            z = r * dinv[:, None]
+            rz2 = (r * z).sum(0)
+            p = z + rz2 / np.maximum(rz, 1e-30) * p
+            rz = rz2
+        return x.reshape(X.shape), iters
+
+    dev = torch.device("cuda", 0)
+    S = CsrSystem(A, dev)
+    cases = [((n, 2), 40, 1, 1e-12, False)] * 4 + [((n,), 600, 50, 1e-12, True)] * 2 + [((n, 2), 400, 1, 1e-24, False)]
+    for k, (shape, iters, check, tol2, strict) in enumerate(cases):
+        print(f"csr pcg case {k}: shape {shape}, iters {iters}", flush=True)
+        B = rng.normal(size=shape) * 10.0 ** rng.integers(-2, 3)
+        X0 = rng.normal(size=shape)
+        b = torch.as_tensor(B, device=dev)
+        x = torch.as_tensor(X0, device=dev).clone()
+        its = S.pcg(b, x, iters, check, tol2, strict)
+        junk = (x * 2.0).sum()                           # eager work between replays
+        torch.cuda.synchronize()
+        assert torch.isfinite(junk)
+        xr, kr = pcg(B, X0, iters, check, tol2, strict)
+        assert its == kr, (k, its, kr)
+        assert np.abs(x.cpu().numpy() - xr).max() <= 1e-10 * np.abs(xr).max(), k
+    S.close()
