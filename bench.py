@@ -49,14 +49,15 @@ def parse():
                     help="keep the size-based schedule (no re-planning from measured iterations)")
     ap.add_argument("--no-plain", dest="plain", action="store_false",
                     help="skip the plain per-island GPU baseline")
-    ap.add_argument("--workload", default="disks", choices=["disks", "heightfield", "needles", "validate", "batch"],
+    ap.add_argument("--workload", default="disks", choices=["disks", "heightfield", "needles", "validate", "batch", "hier"],
                     help="disks: the throughput workload; heightfield: 3,988 islands of 50-1,946 triangles "
                          "cut from a 2M-triangle heightfield (use --iters 15); needles: the precision study on "
                          "256 islands of 5k-60k triangles with 10%% needle triangles (one JSON line with every "
                          "precision variant); validate: validate_uvs of the 512 solved islands packed "
                          "into one atlas, GPU overlap test against the CPU restatement on a sample; batch: "
                          "gpu.Batch.arap(8) of the 512 islands as one batch (one coupled 40-iteration CG "
-                         "per global step), against the oracle's Batch on a sample")
+                         "per global step), against the oracle's Batch on a sample; hier: hier.Solver.relax "
+                         "of the 512 islands as the charts of one subset, against the oracle on a sample")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB 5 times (HBM counter calibration)")
     return ap.parse_args()
@@ -358,6 +359,76 @@ def batch_study(args):
     print(json.dumps(line), flush=True)
 
 
+def hier_study(args):
+    """hier.Solver.relax (defaults: 6 iterations, 20 CG iterations, 24 repair
+    rounds) of the 512 disk islands as the charts of one subset of one mesh
+    (3.83M triangles) from their LSCM layouts.  The timed call includes what
+    the reference's call does besides the iterations: the per-subset set-up
+    (chart plan, pinned Laplacian) and the per-chart quality.  Reports
+    Mtri*iter/s of the whole call, the oracle's relax on a 16-chart subset
+    (one core) and its agreement with the GPU on that subset."""
+    import types
+
+    import torch
+
+    from oracle import hier_cpu
+    from ultimate_uv_artist_amd import hier, workloads
+    from ultimate_uv_artist_amd.plan import IslandPlan
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    charts = workloads.disk_islands(seed=2, n_islands=args.islands)
+    plan = IslandPlan(charts, dev=dev)
+    U = torch.zeros(plan.n, 2, dtype=torch.float64, device=dev)
+    plan.lscm(U)
+    U0 = U.cpu().numpy()
+    vo, to = plan.vert_off, plan.tri_off
+    del plan
+    V = np.concatenate([P for P, _ in charts])
+    F = np.concatenate([T + vo[i] for i, (_, T) in enumerate(charts)])
+    chart = np.repeat(np.arange(len(charts)), np.diff(to))
+    faces = np.arange(len(F))
+    nt = len(F)
+    S = hier.Solver(types.SimpleNamespace(V=V, F=F), dev=dev)
+    S.relax(faces, F, len(V), chart, len(charts), U0)          # warm-up
+    times = []
+    for k in range(max(args.steps, 1)):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        Uh, flips, scale = S.relax(faces, F, len(V), chart, len(charts), U0)
+        S.bad_fraction(2.0, 1.5)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+        print(f"bench: hier relax step {k}: {times[-1] * 1e3:.1f} ms", flush=True)
+    t_step = float(np.median(times))
+    # the oracle on the first 16 charts as their own subset, one core
+    k = min(16, len(charts))
+    nv_s, nt_s = int(vo[k]), int(to[k])
+    sub = (np.arange(nt_s), F[:nt_s], nv_s, chart[:nt_s], k, U0[:nv_s])
+    _one_thread()
+    O = hier_cpu.Solver(V[:nv_s], F[:nt_s])
+    c0 = time.perf_counter()
+    Uo, _, _ = O.relax(*sub)
+    t_cpu = time.perf_counter() - c0
+    Sg = hier.Solver(types.SimpleNamespace(V=V[:nv_s], F=F[:nt_s]), dev=dev)
+    Ug, flg, _ = Sg.relax(*sub)
+    err = 0.0
+    for i in range(k):
+        sl = slice(vo[i], vo[i + 1])
+        err = max(err, float(np.abs(Ug[sl] - Uo[sl]).max() / np.ptp(Uo[sl], 0).max()))
+    line = {"workload": f"hier.Solver.relax (6 iterations, 20 CG iterations) of {len(charts)} disk islands as the "
+                        f"charts of one subset, set-up and quality included",
+            "triangles": nt, "ms_per_call": round(t_step * 1e3, 2),
+            "mtri_iter_per_s": round(nt * 6 / t_step / 1e6, 1), "charts_with_flips": int((flips > 0).sum()),
+            "cpu_baseline": {"value": round(nt_s * 6 / t_cpu / 1e6, 3), "unit": "Mtri*iter/s", "cores": 1,
+                             "kind": "port", "sample": f"hier_cpu.Solver.relax of the first {k} charts "
+                                                        f"({nt_s} triangles)"},
+            "parity": {"sample_charts": k, "max_unit_box_err": err}}
+    print(f"bench: hier relax {t_step * 1e3:.1f} ms = {line['mtri_iter_per_s']} Mtri*iter/s; CPU oracle "
+          f"{line['cpu_baseline']['value']} Mtri*iter/s on {k} charts; sample error {err:.2e}", flush=True)
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     if args.copy_probe:
@@ -375,6 +446,9 @@ def main():
         return
     if args.workload == "batch":
         batch_study(args)
+        return
+    if args.workload == "hier":
+        hier_study(args)
         return
     import torch
 

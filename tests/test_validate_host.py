@@ -127,3 +127,29 @@ def test_uv_islands_on_golden_layouts():
                                           torch.as_tensor(np.asarray(P, np.float64)).float())
         ref, nr = vc.uv_islands(F, len(P), np.asarray(U, np.float32).astype(np.float64), P)
         assert n == nr and _same_partition(isl.numpy(), ref)
+
+
+def test_pinned_laplacian_device_assembly_equals_scipy():
+    """plan.pinned_laplacian with a device (the sorted, segment-summed
+    assembly gpu.Batch and hier.Solver run on the GPU, here on CPU torch)
+    gives the same CSR structure as the scipy assembly and the same entries
+    to rounding, with the same pin weight; rows sum to the pin weight at the
+    pins and to zero elsewhere (a Laplacian)."""
+    from ultimate_uv_artist_amd.plan import pinned_laplacian
+
+    rng = np.random.default_rng(8)
+    ch = [workloads.disk_island(rng, int(t)) for t in (300, 900, 2000)]
+    vo = np.cumsum([0] + [len(P) for P, _ in ch])
+    V = np.concatenate([P for P, _ in ch])
+    F = np.concatenate([T + vo[i] for i, (_, T) in enumerate(ch)])
+    A, w = pinned_laplacian(V, F, vo[:-1])
+    B, w2 = pinned_laplacian(V, F, vo[:-1], dev=torch.device("cpu"))
+    A.sort_indices()
+    B.sort_indices()
+    assert w == w2
+    assert np.array_equal(A.indptr, B.indptr) and np.array_equal(A.indices, B.indices)
+    assert np.abs(A.data - B.data).max() <= 1e-14 * np.abs(A.data).max()
+    rs = np.asarray(B.sum(1)).ravel()
+    want = np.zeros(len(V))
+    want[vo[:-1]] = w2
+    assert np.abs(rs - want).max() <= 1e-12 * w2

@@ -137,7 +137,7 @@ class Batch:
         (gpu.py:126-142), resident on the GPU for the coupled CG; the chart
         plan's rhs takes the same pin weight."""
         if self._sys is None:
-            A, wpin = pinned_laplacian(self._P, self._Tg, self.offsets[:-1])
+            A, wpin = pinned_laplacian(self._P, self._Tg, self.offsets[:-1], dev=self.dev)
             self._sys = CsrSystem(A, self.dev)
             self._plan.set_pins(self.offsets[:-1], wpin)
         return self._sys

@@ -110,7 +110,7 @@ class Solver:
         voff = plan.vert_off
         Pall = np.concatenate([c[0] for c in charts])
         Tall = plan.T_local + np.repeat(voff[:-1], np.diff(plan.tri_off))[:, None]
-        A, wpin = pinned_laplacian(Pall, Tall, voff[:-1])
+        A, wpin = pinned_laplacian(Pall, Tall, voff[:-1], dev=self.dev)
         system = CsrSystem(A, self.dev)
         plan.set_pins(voff[:-1], wpin)
         cv = torch.as_tensor(np.repeat(np.arange(len(charts)), np.diff(voff)), device=self.dev)

@@ -75,13 +75,64 @@ def half_cots(V, F):
     return w
 
 
-def pinned_laplacian(V, F, pins):
+def _half_cots_dev(V, F):
+    """half_cots on device tensors V (n,3) float64, F (t,3) int64."""
+    p0, p1, p2 = V[F[:, 0]], V[F[:, 1]], V[F[:, 2]]
+    e1, e2 = p1 - p0, p2 - p0
+    l1 = e1.norm(dim=1)
+    nrm = torch.cross(e1, e2, dim=1)
+    dbl = nrm.norm(dim=1)
+    ok = (dbl > 1e-14 * (l1 * l1).clamp_min(1e-300)) & torch.isfinite(dbl) & (l1 > 0)
+    ex = e1 / l1.clamp_min(1e-30)[:, None]
+    ey = torch.cross(nrm / dbl.clamp_min(1e-30)[:, None], ex, dim=1)
+    X = torch.zeros(len(F), 3, 2, dtype=torch.float64, device=V.device)
+    X[:, 1, 0] = l1
+    X[:, 2, 0] = (e2 * ex).sum(1)
+    X[:, 2, 1] = (e2 * ey).sum(1)
+    X[~ok] = 0.0
+    w = torch.empty(len(F), 3, dtype=torch.float64, device=V.device)
+    for k in range(3):
+        a, b = X[:, (k + 1) % 3] - X[:, k], X[:, (k + 2) % 3] - X[:, k]
+        cr = (a[:, 0] * b[:, 1] - a[:, 1] * b[:, 0]).abs().clamp_min(1e-30)
+        w[:, k] = 0.5 * ((a * b).sum(1) / cr).clamp(-1e4, 1e4)
+    w[~ok] = 0.0
+    return w
+
+
+def pinned_laplacian(V, F, pins, dev=None):
     """The cotangent Laplacian of a batch (F global vertex ids, rows of every
     chart together) plus a soft pin of weight max |W| (at least 1e-12) at
     each of ``pins``: the matrix of gpu.py:126-142 Batch._build_laplacian and
-    hier.py:394-412.  Returns (scipy CSR, pin weight)."""
+    hier.py:394-412.  Returns (scipy CSR, pin weight).  With ``dev`` the
+    weights and the assembly run on that GPU: a stable sort of the entries by
+    (row, column), then the repeated entries summed in their original order
+    (deterministic)."""
     import scipy.sparse as sp
 
+    if dev is not None:
+        n = len(V)
+        Vd = torch.as_tensor(np.asarray(V, np.float64), device=dev)
+        Fd = torch.as_tensor(np.asarray(F, np.int64), device=dev)
+        w = _half_cots_dev(Vd, Fd)
+        I = torch.cat([Fd[:, 1], Fd[:, 2], Fd[:, 0]])
+        J = torch.cat([Fd[:, 2], Fd[:, 0], Fd[:, 1]])
+        W = torch.cat([w[:, 0], w[:, 1], w[:, 2]])
+        wpin = max(float(W.abs().max()) if len(W) else 0.0, 1e-12)
+        pd = torch.as_tensor(np.asarray(pins, np.int64).reshape(-1), device=dev)
+        rows = torch.cat([I, J, I, J, pd])
+        cols = torch.cat([J, I, I, J, pd])
+        vals = torch.cat([-W, -W, W, W, torch.full((len(pd),), wpin, dtype=torch.float64, device=dev)])
+        del I, J, W, w
+        key, order = torch.sort(rows * n + cols, stable=True)
+        vals = vals[order]
+        del rows, cols, order
+        uk, counts = torch.unique_consecutive(key, return_counts=True)
+        data = torch.segment_reduce(vals, "sum", lengths=counts)
+        r = uk // n
+        indptr = torch.zeros(n + 1, dtype=torch.int64, device=dev)
+        indptr[1:] = torch.cumsum(torch.bincount(r, minlength=n), 0)
+        A = sp.csr_matrix((data.cpu().numpy(), (uk % n).cpu().numpy(), indptr.cpu().numpy()), shape=(n, n))
+        return A, wpin
     F = np.asarray(F, np.int64)
     n = len(V)
     w = half_cots(V, F)
