@@ -86,7 +86,9 @@ class Batch:
         self.nc = len(charts)
         self.offsets = np.cumsum([0] + [len(p) for p, _, _ in charts])
         self.n = int(self.offsets[-1])
-        self._plan = IslandPlan([(p, t) for p, t, _ in charts], batch=True, dev=self.dev)
+        # the per-chart work; the global step runs on the resident Laplacian, so
+        # the plan's own CG (and its coarse level) is not used
+        self._plan = IslandPlan([(p, t) for p, t, _ in charts], batch=True, dev=self.dev, no_coarse=True)
         self._P = np.concatenate([np.asarray(p, np.float64).reshape(-1, 3) for p, _, _ in charts])
         self._sys = None
         U = np.concatenate([np.asarray(u, np.float64).reshape(-1, 2) for _, _, u in charts])

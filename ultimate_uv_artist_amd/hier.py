@@ -68,8 +68,9 @@ class Solver:
         cv[CV.reshape(-1)] = np.repeat(chart, 3)
         if (cv < 0).any():
             raise ValueError("every vertex of the subset must belong to one of its faces")
-        pos = np.zeros((nv, 3))
-        pos[CV.reshape(-1)] = self.V[self.F[faces]].reshape(-1, 3)
+        corner = np.empty(nv, np.int64)                   # one mesh vertex per subset vertex
+        corner[CV.reshape(-1)] = self.F[faces].reshape(-1)
+        pos = self.V[corner]
         vorder = np.argsort(cv, kind="stable")            # chart-major, ascending id inside a chart
         local = np.empty(nv, np.int64)
         vcount = np.bincount(cv, minlength=nchart)
@@ -106,7 +107,7 @@ class Solver:
         # limits; the subset's Laplacian on the GPU: the global step, one CG
         # over every chart as the reference solves them (shared step lengths),
         # one soft pin per chart with the subset's weight max |W|
-        plan = IslandPlan(charts, batch=True, dev=self.dev)
+        plan = IslandPlan(charts, batch=True, dev=self.dev, no_coarse=True)    # its own CG is not used
         voff = plan.vert_off
         Pall = np.concatenate([c[0] for c in charts])
         Tall = plan.T_local + np.repeat(voff[:-1], np.diff(plan.tri_off))[:, None]
