@@ -91,6 +91,7 @@ class Batch:
         self._plan = IslandPlan([(p, t) for p, t, _ in charts], batch=True, dev=self.dev, no_coarse=True)
         self._P = np.concatenate([np.asarray(p, np.float64).reshape(-1, 3) for p, _, _ in charts])
         self._sys = None
+        self.last_cg_iters = 0
         U = np.concatenate([np.asarray(u, np.float64).reshape(-1, 2) for _, _, u in charts])
         self._U = torch.as_tensor(U, device=self.dev).contiguous()
         toff = np.cumsum([0] + [len(t) for _, t, _ in charts])
@@ -156,7 +157,8 @@ class Batch:
             b = self._plan.local_rhs(U, _lib.UVA_MODE_BATCH)
             Un = U.clone()
             if cg_iters > 0:                                   # no CG iteration: the layout stays
-                system.pcg(b, Un, int(cg_iters), check_every=1, tol2=float(cg_tol) ** 2, strict=False)
+                self.last_cg_iters = system.pcg(b, Un, int(cg_iters), check_every=1, tol2=float(cg_tol) ** 2,
+                                                strict=False)
             bad = ~torch.isfinite(Un).all(1)
             if bool(bad.any()):
                 badc = torch.zeros(self.nc, dtype=torch.bool, device=self.dev)
@@ -199,5 +201,7 @@ class Batch:
         return [U[a:b] for a, b in zip(self.offsets[:-1], self.offsets[1:])]
 
     def stats(self):
-        """Per-chart solver statistics of the last call."""
+        """Per-chart statistics of the chart plan's last call (orientation,
+        repair, rhs, step limits); the coupled CG of ``arap`` reports its
+        iteration count in ``last_cg_iters``."""
         return self._plan.stats()
