@@ -346,6 +346,7 @@ def batch_study(args):
                         f"one coupled CG of 40 iterations per global step",
             "triangles": nt, "ms_per_step": round(t_step * 1e3, 2),
             "mtri_iter_per_s": round(nt * 8 / t_step / 1e6, 1),
+            "setup_s": round(t_setup, 3),
             "coupled_cg": {"iterations": its, "ms": round(cg_t * 1e3, 3), "rows": n, "nnz": nnz,
                            "bytes_per_iteration": per_it, "achieved_GBps": round(achieved, 1),
                            "frac": round(achieved / 8000.0, 4)},
