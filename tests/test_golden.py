@@ -288,3 +288,18 @@ def test_batch_oracle_coupled_cg_matches_reference_budget():
         s = slice(off[i], off[i + 1])
         far = max(far, own_box_err(exact[s], g["after_ls"][s]))
     assert far > 1e-5
+
+
+def test_tutte_matches_reference():
+    """oracle.tutte against the reference's param.tutte (param.py:182-202) on
+    three charts (tests/golden/tutte_golden.npz): boundary loop in vertex-id
+    order, the same chart with its loop rotated against id order, and a grid
+    chart whose loop order differs from id order everywhere.  The reference
+    pairs boundary positions in loop order with boundary columns in id
+    order, so the last two embeddings are not harmonic: only its own output
+    pins that."""
+    g = dict(np.load(GOLD / "tutte_golden.npz"))
+    for name in ("ordered", "rotated", "grid"):
+        P, T = g[f"{name}_P"], g[f"{name}_T"].astype(np.int64)
+        U = pc.tutte(P, T)
+        assert np.abs(U - g[f"{name}_U"]).max() <= 1e-10 * np.abs(g[f"{name}_U"]).max(), name

@@ -253,9 +253,20 @@ def test_repair_flips_sequential_host():
 
 
 def test_spdsolver_and_tutte(small_disks):
+    """param.tutte (interior solve through SPDSolver on the GPU) against the
+    reference's own embeddings of three charts (tests/golden/tutte_golden.npz:
+    loop in id order, loop rotated against id order, grid chart) and against
+    the oracle on a disk chart."""
+    from pathlib import Path
+
     from oracle import param_cpu as oracle
     from ultimate_uv_artist_amd import param
 
+    g = dict(np.load(Path(__file__).resolve().parent / "golden" / "tutte_golden.npz"))
+    for name in ("ordered", "rotated", "grid"):
+        P, T = g[f"{name}_P"], g[f"{name}_T"].astype(np.int64)
+        U = param.tutte(P, T)
+        assert np.abs(U - g[f"{name}_U"]).max() <= 1e-8 * np.abs(g[f"{name}_U"]).max(), name
     P, T = small_disks[2]
     U = param.tutte(P, T)
     Ur = oracle.tutte(P, T)

@@ -518,3 +518,58 @@ def test_hier_lscm_pins_are_extreme_boundary_vertices():
                check_every=1)
     d = np.linalg.norm(U - U0, axis=1) / np.ptp(U0, 0).max()
     assert d[lo] < 0.02 and d[hi] < 0.02 and np.median(d) > 5 * max(d[lo], d[hi])
+
+
+def _neighbour_sets(T, n):
+    nb = [set() for _ in range(n)]
+    for a, b, c in np.asarray(T, np.int64).tolist():
+        nb[a] |= {b, c}
+        nb[b] |= {a, c}
+        nb[c] |= {a, b}
+    return nb
+
+
+def test_tutte_is_harmonic_with_the_boundary_on_a_circle():
+    """A chart whose boundary loop runs in vertex-id order (so the reference's
+    pairing of loop-order positions with id-order columns is consistent):
+    every interior vertex is the mean of its distinct neighbours, and the
+    boundary vertices sit on the circle of the perimeter's radius at their
+    arc-length angles, the first at angle 0."""
+    from pathlib import Path
+
+    g = dict(np.load(Path(__file__).resolve().parent / "golden" / "tutte_golden.npz"))
+    P, T = g["ordered_P"], g["ordered_T"].astype(np.int64)
+    U = pc.tutte(P, T)
+    loop = max(pc.boundary_loops(T), key=len)
+    assert np.array_equal(loop, np.sort(loop))
+    seg = np.linalg.norm(P[np.roll(loop, -1)] - P[loop], axis=1)
+    rad = seg.sum() / (2 * np.pi)
+    assert np.allclose(np.hypot(U[loop, 0], U[loop, 1]), rad, rtol=1e-12)
+    ang = np.mod(np.arctan2(U[loop, 1], U[loop, 0]), 2 * np.pi)
+    want = np.r_[0.0, np.cumsum(seg)[:-1]] / rad
+    assert np.allclose(ang[1:], want[1:], atol=1e-10) and abs(ang[0]) < 1e-12
+    nb = _neighbour_sets(T, len(P))
+    interior = np.setdiff1d(np.arange(len(P)), loop)
+    mean = np.array([U[sorted(nb[v])].mean(0) for v in interior])
+    assert np.abs(U[interior] - mean).max() <= 1e-10 * rad
+    # a disk chart's harmonic map onto a convex boundary has no fold (Tutte)
+    assert (pc.signed_areas(U, T) > 0).all() or (pc.signed_areas(U, T) < 0).all()
+
+
+def test_tutte_rotated_loop_is_not_harmonic():
+    """The same chart with its loop starting elsewhere: the boundary still lies
+    on the circle, but the interior solve uses the id-order pairing, so the
+    interior is no longer the mean of its neighbours' actual positions (the
+    behaviour the reference's own output pins)."""
+    from pathlib import Path
+
+    g = dict(np.load(Path(__file__).resolve().parent / "golden" / "tutte_golden.npz"))
+    P, T = g["rotated_P"], g["rotated_T"].astype(np.int64)
+    U = pc.tutte(P, T)
+    loop = max(pc.boundary_loops(T), key=len)
+    seg = np.linalg.norm(P[np.roll(loop, -1)] - P[loop], axis=1)
+    assert np.allclose(np.hypot(U[loop, 0], U[loop, 1]), seg.sum() / (2 * np.pi), rtol=1e-12)
+    nb = _neighbour_sets(T, len(P))
+    interior = np.setdiff1d(np.arange(len(P)), loop)
+    mean = np.array([U[sorted(nb[v])].mean(0) for v in interior])
+    assert np.abs(U[interior] - mean).max() > 1e-3 * seg.sum()

@@ -209,24 +209,53 @@ def _far_pair(P, cand):
 
 def tutte(P, T):
     """Uniform Tutte embedding onto a circle (param.py:182-202); the interior
-    solve runs on the GPU through SPDSolver."""
+    solve runs on the GPU through ``SPDSolver``.
+
+    The longest boundary loop goes onto the circle whose perimeter is the
+    loop's length, each vertex at its arc-length angle from the loop's first
+    vertex.  Every interior vertex is then the mean of its distinct
+    neighbours, with one property of the reference kept because it is part
+    of its result: its right-hand side takes the circle positions in loop
+    order against the boundary vertices in increasing id order (a boolean
+    column mask), so a boundary vertex enters its neighbours' equations with
+    the position of the loop vertex whose rank in the loop equals its rank
+    among the boundary ids.  The two orders agree when the loop runs in id
+    order from its smallest vertex; otherwise the interior is not harmonic
+    and may fold, as in the reference."""
+    P = np.asarray(P, np.float64)
     T = np.asarray(T, np.int64)
-    loop = max(boundary_loops(T), key=len)
     n = len(P)
+    loop = max(boundary_loops(T), key=len)
     seg = np.linalg.norm(P[np.roll(loop, -1)] - P[loop], axis=1)
-    ang = np.r_[0, np.cumsum(seg)[:-1]] / max(seg.sum(), 1e-300) * 2 * np.pi
-    Ub = np.stack([np.cos(ang), np.sin(ang)], 1) * (seg.sum() / (2 * np.pi))
-    I = np.concatenate([T[:, 0], T[:, 1], T[:, 2], T[:, 1], T[:, 2], T[:, 0]])
-    J = np.concatenate([T[:, 1], T[:, 2], T[:, 0], T[:, 0], T[:, 1], T[:, 2]])
-    A = sp.csr_matrix((np.ones(len(I)), (I, J)), shape=(n, n))
-    A.data[:] = 1.0
-    Lu = sp.diags(np.asarray(A.sum(1)).ravel()) - A
-    fixed = np.zeros(n, bool)
-    fixed[loop] = True
+    length = seg.sum()
+    theta = np.concatenate(([0.0], np.cumsum(seg)[:-1])) / max(length, 1e-300) * (2 * np.pi)
+    circle = (length / (2 * np.pi)) * np.stack([np.cos(theta), np.sin(theta)], 1)
+    on_loop = np.zeros(n, bool)
+    on_loop[loop] = True
+    # boundary value each boundary vertex contributes to its neighbours' rows
+    rhs_value = np.zeros((n, 2))
+    rhs_value[np.sort(loop)] = circle
+    # distinct undirected edges; degree = number of distinct neighbours
+    ends = np.sort(np.concatenate([T[:, [0, 1]], T[:, [1, 2]], T[:, [2, 0]]]), axis=1)
+    ends = np.unique(ends, axis=0)
+    deg = np.bincount(ends.ravel(), minlength=n).astype(np.float64)
+    free = np.flatnonzero(~on_loop)
+    slot = np.full(n, -1, np.int64)
+    slot[free] = np.arange(len(free))
+    a, b = ends[:, 0], ends[:, 1]
+    both = ~on_loop[a] & ~on_loop[b]
+    rows = np.concatenate([slot[a[both]], slot[b[both]], np.arange(len(free))])
+    cols = np.concatenate([slot[b[both]], slot[a[both]], np.arange(len(free))])
+    vals = np.concatenate([-np.ones(2 * int(both.sum())), deg[free]])
+    M = sp.csr_matrix((vals, (rows, cols)), shape=(len(free), len(free)))
+    rhs = np.zeros((len(free), 2))
+    for u, v in ((a, b), (b, a)):                  # u free, v on the loop
+        m = ~on_loop[u] & on_loop[v]
+        np.add.at(rhs, slot[u[m]], rhs_value[v[m]])
     U = np.zeros((n, 2))
-    U[loop] = Ub
-    free = ~fixed
-    U[free] = SPDSolver(Lu[free][:, free]).solve(-Lu[free][:, fixed] @ Ub)
+    U[loop] = circle
+    if len(free):
+        U[free] = SPDSolver(M).solve(rhs)
     return U
 
 
