@@ -134,6 +134,10 @@ typedef struct {
 
 int uva_version(void);
 const char *uva_last_error(void);
+/* sizeof of the ABI structs, for bindings that declare them by hand:
+ * which = 0 uva_arap_opts, 1 uva_slim_opts, 2 uva_island_stats,
+ * 3 uva_plan_info; -1 for another value.  No counterpart in the reference. */
+int uva_struct_size(int32_t which);
 
 /* Build the device plan of a batch of islands.
  *   vert_off, tri_off  n_islands+1 prefix offsets (host, int64)

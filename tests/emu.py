@@ -52,7 +52,17 @@ def lib():
                                                C.c_int32, C.c_void_p]
         _lib.uva_emu_csr_pcg.argtypes = [C.c_int64, _i64p, _i64p, _dp, _dp, _dp, C.c_int32, C.c_int32, C.c_int32,
                                          C.c_double, C.c_int32, C.POINTER(C.c_int32)]
+        _lib.uva_emu_team_guard.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int64, C.POINTER(C.c_int64),
+                                            C.POINTER(C.c_int32)]
     return _lib
+
+
+def team_guard(members, absent, barriers, spin):
+    """(largest ticks a member spent, fault flag) of `barriers` team barriers
+    where member `absent` (-1: none) never arrives (team_wait on the host)."""
+    t, f = C.c_int64(0), C.c_int32(0)
+    lib().uva_emu_team_guard(members, absent, barriers, spin, C.byref(t), C.byref(f))
+    return t.value, f.value
 
 
 def overlap_flags(U, F, skip, rel, strict, grid):

@@ -512,3 +512,17 @@ def test_emu_jacobi_pcg_iteration_and_stop_rule():
     assert np.abs(x1 - x[:, 1]).max() <= 1e-12 * np.abs(x1).max()      # columns do not mix
     x, k = emu.jacobi_pcg(A, b, x0, 0)
     assert k == 0 and np.array_equal(x, x0)
+
+
+def test_emu_team_barrier_guard_stops_waiting_after_a_fault():
+    """The team barrier's wait (team_wait, the loop of TeamWG::sync) with a
+    member that never arrives: the first barrier gives up after the spin
+    limit and sets the fault flag, and every later barrier then returns at
+    once, so the launch ends in about one spin limit instead of one per
+    barrier.  With every member present nothing is flagged."""
+    spin, barriers = 20000, 200
+    ticks, fault = emu.team_guard(4, 2, barriers, spin)
+    assert fault == 1
+    assert spin < ticks < spin + 20 * barriers
+    ticks, fault = emu.team_guard(4, -1, barriers, spin)
+    assert fault == 0 and ticks < spin

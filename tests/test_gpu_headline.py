@@ -1,0 +1,125 @@
+"""The two workloads the throughput metric is defined on, through the same
+calls the bench times.
+
+* 512 disk islands (triangle counts log-uniform in [200, 40000], 3.83M
+  triangles): LSCM, a first step, the autotuned schedule, then a
+  graph-replayed 10-iteration ARAP run to relative residual 1e-6 per island.
+  Every island converges; on 40 islands chosen at evenly spaced quantiles of
+  the triangle mass the layout is within 1e-4 of the float64 oracle (SuperLU
+  global steps, unit box), validate.py gives identical verdicts, and p95
+  anisotropy and area distortion agree within 1 %.  Replays of the captured
+  sequence are bitwise equal to one another and to the eager run.
+* the 7,938-triangle grid island at its stated CG tolerance (1e-8): LSCM then
+  20 ARAP iterations through the plan, against the oracle.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _unit_err(g, r):
+    lo = r.min(0)
+    s = float((r.max(0) - lo).max())
+    return float(np.abs((g - lo) / s - (r - lo) / s).max())
+
+
+def test_512_disk_islands_tuned_graph_replay():
+    import torch
+
+    from oracle import param_cpu as oracle
+    from oracle import validate_cpu as vc
+    from ultimate_uv_artist_amd import _lib, workloads
+    from ultimate_uv_artist_amd.plan import IslandPlan
+
+    charts = workloads.disk_islands(seed=2, n_islands=512)
+    tri = np.array([len(T) for _, T in charts])
+    print(f"headline: 512 islands, {tri.sum()} triangles", flush=True)
+    assert 3.5e6 < tri.sum() < 4.2e6
+    plan = IslandPlan(charts)
+    U0 = plan.device_uv(np.zeros((plan.n, 2)))
+    plan.lscm(U0)
+    st = plan.stats()
+    assert st["converged"].all(), "LSCM did not converge on every island"
+    # majority orientation per island (what the bench and charts.flatten do)
+    Uh0 = U0.cpu().numpy()
+    vo = plan.vert_off
+    for i, (_, T) in enumerate(charts):
+        u = Uh0[vo[i]:vo[i + 1]]
+        sa = oracle.signed_areas(u, T)
+        if (sa < 0).sum() > (sa > 0).sum():
+            u[:, 1] = -u[:, 1]
+    U0 = plan.device_uv(Uh0)
+
+    def run(use_graph):
+        U = U0.clone()
+        plan.arap(U, 10, line_search=True, mode=_lib.UVA_MODE_REFINE, tol=0.0, cg_tol=1e-6, use_graph=use_graph)
+        torch.cuda.synchronize()
+        return U
+
+    eager = run(False)
+    plan.tune()
+    tuned = run(False)
+    g1 = run(True)
+    g2 = run(True)
+    g3 = run(True)
+    st = plan.stats()
+    kpi = st["cg_iters"] / np.maximum(st["arap_iters"], 1)
+    print(f"headline: converged {int(st['converged'].sum())}/512, CG iters per global step mean {kpi.mean():.1f} "
+          f"max {st['cg_iters_max'].max()}, max rel resid {st['rel_resid'].max():.2e}", flush=True)
+    assert st["converged"].all() and (st["arap_iters"] == 10).all()
+    assert st["rel_resid"].max() <= 1e-6
+    assert torch.equal(g1, g2) and torch.equal(g2, g3) and torch.equal(g1, tuned)
+    # the tuned schedule changes the order of FP64 team reductions: the same
+    # layout to CG accuracy
+    Ue, Ut = eager.cpu().numpy(), tuned.cpu().numpy()
+    assert max(_unit_err(Ut[vo[i]:vo[i + 1]], Ue[vo[i]:vo[i + 1]]) for i in range(512)) <= 1e-5
+    order = np.argsort(tri)
+    cum = np.cumsum(tri[order]) / tri.sum()
+    pick = np.unique(order[np.minimum(np.searchsorted(cum, (np.arange(40) + 0.5) / 40), len(order) - 1)])
+    assert len(pick) >= 32
+    Ug = g1.cpu().numpy()
+    Uh0 = U0.cpu().numpy()
+    worst, same, dq = 0.0, 0, 0.0
+    for i in pick:
+        P, T = charts[i]
+        ur, _ = oracle.refine(P, T, Uh0[vo[i]:vo[i + 1]], iters=10, tol=0.0)
+        g = Ug[vo[i]:vo[i + 1]]
+        worst = max(worst, _unit_err(g, ur))
+        same += int(vc.island_verdict(P, T, g) == vc.island_verdict(P, T, ur))
+        q, qr = oracle.chart_quality(P, T, g), oracle.chart_quality(P, T, ur)
+        dq = max(dq, abs(q["aniso_p95"] / qr["aniso_p95"] - 1), abs(q["area_p95"] / qr["area_p95"] - 1))
+    print(f"headline: {len(pick)} islands vs oracle: max unit-box err {worst:.2e}, verdicts identical "
+          f"{same}/{len(pick)}, p95 anisotropy/area within {dq:.1e}", flush=True)
+    assert worst <= 1e-4
+    assert same == len(pick)
+    assert dq <= 0.01
+
+
+def test_grid_island_at_cg_tolerance_1e8():
+    import torch
+
+    from oracle import param_cpu as oracle
+    from ultimate_uv_artist_amd import _lib, workloads
+    from ultimate_uv_artist_amd.plan import IslandPlan
+
+    P, T = workloads.grid_island(seed=1)
+    assert len(T) == 7938
+    plan = IslandPlan([(P, T)])
+    U = plan.device_uv(np.zeros((len(P), 2)))
+    plan.lscm(U)
+    U0r = oracle.lscm(P, T)
+    assert _unit_err(U.cpu().numpy(), U0r) <= 1e-6
+    plan.arap(U, 20, line_search=True, mode=_lib.UVA_MODE_REFINE, tol=0.0, cg_tol=1e-8, use_graph=True)
+    torch.cuda.synchronize()
+    st = plan.stats()
+    Ur, _ = oracle.refine(P, T, U0r, iters=20, tol=0.0)
+    err = _unit_err(U.cpu().numpy(), Ur)
+    print(f"grid at CG 1e-8: unit-box err {err:.2e}, CG iters {int(st['cg_iters'][0])}, "
+          f"rel resid {st['rel_resid'][0]:.1e}", flush=True)
+    assert st["converged"].all() and st["rel_resid"].max() <= 1e-8
+    assert err <= 1e-5
+    q, qr = oracle.chart_quality(P, T, U.cpu().numpy()), oracle.chart_quality(P, T, Ur)
+    assert abs(q["aniso_p95"] / qr["aniso_p95"] - 1) <= 0.01 and abs(q["area_p95"] / qr["area_p95"] - 1) <= 0.01

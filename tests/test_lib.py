@@ -58,3 +58,22 @@ def test_repair_seq_matches_oracle(lib):
     Tc = np.ascontiguousarray(T, np.int64)
     assert lib.lib().uva_repair_seq(lib.f64p(out), len(out), lib.i64p(Tc), len(Tc), 25) == 0
     assert np.array_equal(out, oracle.repair_flips(U, T, 25))
+
+
+def test_ctypes_structs_match_the_library(lib):
+    """The ctypes structs have the C structs' sizes (uva_struct_size), and the
+    stand-alone binding in INTEGRATION.md declares uva_arap_opts with all of
+    its fields (a short struct makes the library read past the caller's)."""
+    import re
+    from pathlib import Path
+
+    L = lib.lib()
+    for which, st in enumerate((lib.ArapOpts, lib.SlimOpts, lib.IslandStats, lib.PlanInfo)):
+        assert C.sizeof(st) == L.uva_struct_size(which), st.__name__
+    assert L.uva_struct_size(99) == -1
+    doc = (Path(__file__).resolve().parent.parent / "INTEGRATION.md").read_text()
+    block = re.search(r"class ArapOpts\(C\.Structure\):.*?\]\n", doc, re.S).group(0)
+    ns = {"C": C}
+    exec(block, ns)
+    assert C.sizeof(ns["ArapOpts"]) == L.uva_struct_size(0)
+    assert [f for f, _ in ns["ArapOpts"]._fields_] == [f for f, _ in lib.ArapOpts._fields_]

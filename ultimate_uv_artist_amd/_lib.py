@@ -73,6 +73,7 @@ _dp = C.POINTER(C.c_double)
 SIGNATURES = {
     "uva_version": (C.c_int, []),
     "uva_last_error": (C.c_char_p, []),
+    "uva_struct_size": (C.c_int, [C.c_int32]),
     "uva_plan_create": (C.c_int, [C.POINTER(_vp), C.c_int64, _i64p, _i64p, _dp, _i64p, _i64p, _dp, _dp, _dp,
                                   C.c_uint32, _vp]),
     "uva_plan_create_matrix": (C.c_int, [C.POINTER(_vp), C.c_int64, _i64p, _i64p, _dp, _vp]),

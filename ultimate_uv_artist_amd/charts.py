@@ -283,7 +283,15 @@ class FlattenQueue:
         self.close()
 
     def _run(self):
-        torch.cuda.set_device(self.dev)
+        # Every request's future is resolved, with its result or with the
+        # exception its call raised (as charts.flatten would raise it in the
+        # caller's thread); nothing a request does ends this thread.
+        try:
+            if self.dev.type == "cuda":
+                torch.cuda.set_device(self.dev)
+        except Exception as e:
+            self._fail_all(e)
+            return
         while True:
             with self._cv:
                 while not self._pending and not self._closed:
@@ -297,20 +305,39 @@ class FlattenQueue:
                         break
                     self._cv.wait(left)
                 batch, self._pending = self._pending[:self.max_batch], self._pending[self.max_batch:]
-            groups = {}
-            for P, T, f, key in batch:
-                if not f.set_running_or_notify_cancel():
-                    continue
-                if _has_boundary(T):
-                    groups.setdefault(key, []).append((P, T, f))
-                else:
-                    f.set_exception(ValueError("chart has no boundary (closed surface): needs a cut"))
-            for (it, sl), ok in groups.items():
-                try:
-                    res = flatten_many([(P, T) for P, T, _ in ok], it, self.dev, slim_iters=sl)
-                    for (_, _, f), r in zip(ok, res):
-                        f.set_result(r)
-                except Exception as e:        # the group failed as a whole: each of its callers sees it
-                    for _, _, f in ok:
-                        if not f.done():
-                            f.set_exception(e)
+            try:
+                self._run_batch(batch)
+            except BaseException as e:       # unexpected: every unresolved caller of the batch sees it
+                for _, _, f, _ in batch:
+                    if not f.done():
+                        f.set_exception(e)
+
+    def _run_batch(self, batch):
+        groups = {}
+        for P, T, f, key in batch:
+            if not f.set_running_or_notify_cancel():
+                continue
+            try:
+                if not _has_boundary(T):
+                    raise ValueError("chart has no boundary (closed surface): needs a cut")
+            except Exception as e:           # e.g. an empty or malformed triangle list
+                f.set_exception(e)
+                continue
+            groups.setdefault(key, []).append((P, T, f))
+        for (it, sl), ok in groups.items():
+            try:
+                res = flatten_many([(P, T) for P, T, _ in ok], it, self.dev, slim_iters=sl)
+                for (_, _, f), r in zip(ok, res):
+                    f.set_result(r)
+            except Exception as e:        # the group failed as a whole: each of its callers sees it
+                for _, _, f in ok:
+                    if not f.done():
+                        f.set_exception(e)
+
+    def _fail_all(self, e):
+        with self._cv:
+            self._closed = True
+            pending, self._pending = self._pending, []
+        for _, _, f, _ in pending:
+            if f.set_running_or_notify_cancel():
+                f.set_exception(e)

@@ -111,10 +111,32 @@ int from_plan(uva_plan* pl, int k, double* ext, hipStream_t s, int rc) {
     return UVA_OK;
 }
 
-int reset_queue(uva_plan* pl, int slots, hipStream_t s) {
+// Task counters and team barrier counters of a call sequence start at zero
+// (fill kernels, not memsets, so the sequence can be captured in a graph);
+// the first reset of a call also clears the team fault flag.
+int reset_queue(uva_plan* pl, int slots, hipStream_t s, bool first = true) {
     HIP_TRY(uva::launch_fill_int(pl->dp.queue, uva::kQueueStride * slots, 0, s));
-    if (pl->dp.n_teams > 0) HIP_TRY(uva::launch_fill_int(pl->dp.team_cnt, slots * pl->dp.n_teams, 0, s));
+    if (pl->dp.n_teams > 0) {
+        HIP_TRY(uva::launch_fill_int(pl->dp.team_cnt, slots * pl->dp.n_teams, 0, s));
+        if (first) HIP_TRY(uva::launch_fill_int(pl->dp.fault, 1, 0, s));
+    }
     return UVA_OK;
+}
+
+// A team whose workgroups were not all resident together gives up waiting at
+// a barrier after the hang guard and sets the plan's fault flag: the call's
+// results are then invalid.  Every call that ran island tasks on a plan with
+// multi-workgroup teams reads the flag back (waiting for the stream) and
+// reports it; the flag is cleared again for the next call.
+int check_fault(uva_plan* pl, hipStream_t s, int rc) {
+    if (rc || pl->dp.n_teams <= 0) return rc;
+    int f = 0;
+    HIP_TRY(hipMemcpyAsync(&f, pl->dp.fault, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (!f) return UVA_OK;
+    HIP_TRY(uva::launch_fill_int(pl->dp.fault, 1, 0, s));
+    return fail(UVA_EHIP, "a multi-workgroup island team gave up waiting at a barrier (its workgroups were not "
+                          "resident together); the results of this call are invalid");
 }
 
 uva::TaskArgs base_args(double* U) {
@@ -146,7 +168,7 @@ int arap_sequence(uva_plan* pl, double* Uext, const uva_arap_opts* o, hipStream_
     int done = 0;
     while (done < total) {
         int chunk = std::min(total - done, kMaxSlots);
-        int rc = reset_queue(pl, chunk, s);
+        int rc = reset_queue(pl, chunk, s, done == 0);
         if (rc) return rc;
         for (int k = 0; k < chunk; ++k) {
             int step = done + k;
@@ -373,6 +395,16 @@ extern "C" {
 int uva_version(void) { return 10000; }
 
 const char* uva_last_error(void) { return g_err.c_str(); }
+
+int uva_struct_size(int32_t which) {
+    switch (which) {
+        case 0: return (int)sizeof(uva_arap_opts);
+        case 1: return (int)sizeof(uva_slim_opts);
+        case 2: return (int)sizeof(uva_island_stats);
+        case 3: return (int)sizeof(uva_plan_info);
+        default: return -1;
+    }
+}
 
 int uva_plan_create(uva_plan** out, int64_t n_islands, const int64_t* vert_off, const int64_t* tri_off,
                     const double* P, const int64_t* T, const int64_t* pins, const double* X, const double* area,
@@ -609,7 +641,7 @@ int uva_lscm(uva_plan* plan, double* U, const int64_t* pins, double tol, int32_t
     a.cg_max = max_iters > 0 ? max_iters : kDefaultMaxIters;
     int rc = reset_queue(plan, 1, s);
     if (!rc) rc = launch(plan, uva::TASK_LSCM, UVA_PREC_F64, a, 0, s);
-    return from_plan(plan, 0, U, s, rc);
+    return check_fault(plan, s, from_plan(plan, 0, U, s, rc));
 }
 
 int uva_arap(uva_plan* plan, double* U, const uva_arap_opts* o, void* stream) {
@@ -621,7 +653,7 @@ int uva_arap(uva_plan* plan, double* U, const uva_arap_opts* o, void* stream) {
     if (o->mode != UVA_MODE_REFINE && o->mode != UVA_MODE_BATCH && o->mode != UVA_MODE_HIER)
         return fail(UVA_EINVAL, "bad mode");
     hipStream_t s = (hipStream_t)stream;
-    if (!o->use_graph) return arap_sequence(plan, U, o, s);
+    if (!o->use_graph) return check_fault(plan, s, arap_sequence(plan, U, o, s));
     // graph of the whole sequence, cached per (U, options)
     std::vector<char> key(sizeof(uva_arap_opts) + sizeof(double*));
     std::memcpy(key.data(), o, sizeof(uva_arap_opts));
@@ -641,7 +673,7 @@ int uva_arap(uva_plan* plan, double* U, const uva_arap_opts* o, void* stream) {
         plan->gkey = key;
     }
     HIP_TRY(hipGraphLaunch(plan->gexec, s));
-    return UVA_OK;
+    return check_fault(plan, s, UVA_OK);
 }
 
 int uva_arap_profile(uva_plan* plan, double* U, const uva_arap_opts* o, double* launch_ms, int32_t n_launch,
@@ -661,7 +693,7 @@ int uva_arap_profile(uva_plan* plan, double* U, const uva_arap_opts* o, double* 
     for (auto& e : ev) {
         if (hipEventCreate(&e) != hipSuccess) { rc = fail(UVA_EHIP, "hipEventCreate"); break; }
     }
-    if (rc == UVA_OK) rc = arap_sequence(plan, U, o, s, &ev);
+    if (rc == UVA_OK) rc = check_fault(plan, s, arap_sequence(plan, U, o, s, &ev));
     if (rc == UVA_OK && hipStreamSynchronize(s) != hipSuccess) rc = fail(UVA_EHIP, "hipStreamSynchronize");
     for (int k = 0; rc == UVA_OK && k < n_launch; ++k) {
         float ms = 0.0f;
@@ -693,12 +725,12 @@ int uva_slim(uva_plan* plan, double* U, const uva_slim_opts* o, void* stream) {
     const int total = 1 + o->iters;
     for (int done = 0; done < total && !rc;) {
         const int chunk = std::min(total - done, kMaxSlots);
-        rc = reset_queue(plan, chunk, s);
+        rc = reset_queue(plan, chunk, s, done == 0);
         for (int k = 0; k < chunk && !rc; ++k)
             rc = launch(plan, done + k == 0 ? uva::TASK_SLIM_PROLOGUE : uva::TASK_SLIM_ITER, UVA_PREC_F64, a, k, s);
         done += chunk;
     }
-    return from_plan(plan, 0, U, s, rc);
+    return check_fault(plan, s, from_plan(plan, 0, U, s, rc));
 }
 
 int uva_local_rhs(uva_plan* plan, const double* U, double* b, int32_t mode, void* stream) {
@@ -710,7 +742,7 @@ int uva_local_rhs(uva_plan* plan, const double* U, double* b, int32_t mode, void
     a.batch = mode == UVA_MODE_BATCH;
     if (!rc) rc = reset_queue(plan, 1, s);
     if (!rc) rc = launch(plan, uva::TASK_LOCAL_RHS, UVA_PREC_MIXED, a, 0, s);
-    return from_plan(plan, 1, b, s, rc);
+    return check_fault(plan, s, from_plan(plan, 1, b, s, rc));
 }
 
 int uva_cg_solve(uva_plan* plan, const double* b, double* x, double tol, int32_t max_iters, int32_t precision,
@@ -727,7 +759,7 @@ int uva_cg_solve(uva_plan* plan, const double* b, double* x, double tol, int32_t
     a.cg_max = max_iters > 0 ? max_iters : kDefaultMaxIters;
     if (!rc) rc = reset_queue(plan, 1, s);
     if (!rc) rc = launch(plan, uva::TASK_CG, precision, a, 0, s);
-    return from_plan(plan, 0, x, s, rc);
+    return check_fault(plan, s, from_plan(plan, 0, x, s, rc));
 }
 
 int uva_max_step(uva_plan* plan, const double* U, const double* D, double* tmax, int32_t mode, void* stream) {
@@ -741,7 +773,7 @@ int uva_max_step(uva_plan* plan, const double* U, const double* D, double* tmax,
     a.hier = mode == UVA_MODE_HIER;
     if (!rc) rc = reset_queue(plan, 1, s);
     if (!rc) rc = launch(plan, uva::TASK_MAX_STEP, UVA_PREC_MIXED, a, 0, s);
-    return rc;
+    return check_fault(plan, s, rc);
 }
 
 int uva_energy(uva_plan* plan, const double* U, int32_t kind, double* e, void* stream) {
@@ -754,7 +786,7 @@ int uva_energy(uva_plan* plan, const double* U, int32_t kind, double* e, void* s
     a.kind = kind;
     if (!rc) rc = reset_queue(plan, 1, s);
     if (!rc) rc = launch(plan, uva::TASK_ENERGY, UVA_PREC_MIXED, a, 0, s);
-    return rc;
+    return check_fault(plan, s, rc);
 }
 
 int uva_energy_fwd(uva_plan* plan, const double* U, const double* w, int32_t kind, double* e, void* stream) {
@@ -769,7 +801,7 @@ int uva_energy_fwd(uva_plan* plan, const double* U, const double* w, int32_t kin
     a.w = w;
     if (!rc) rc = reset_queue(plan, 1, s);
     if (!rc) rc = launch(plan, uva::TASK_ENERGY, UVA_PREC_MIXED, a, 0, s);
-    return rc;
+    return check_fault(plan, s, rc);
 }
 
 int uva_energy_bwd(uva_plan* plan, const double* U, const double* w, int32_t kind, const double* ge, double* gU,
@@ -787,7 +819,7 @@ int uva_energy_bwd(uva_plan* plan, const double* U, const double* w, int32_t kin
     a.gw = gw;
     if (!rc) rc = reset_queue(plan, 1, s);
     if (!rc) rc = launch(plan, uva::TASK_ENERGY_BWD, UVA_PREC_F64, a, 0, s);
-    return from_plan(plan, 1, gU, s, rc);
+    return check_fault(plan, s, from_plan(plan, 1, gU, s, rc));
 }
 
 int uva_adjoint_cg(uva_plan* plan, const double* g, double* lam, double tol, int32_t max_iters, int32_t precision,
@@ -804,7 +836,7 @@ int uva_adjoint_cg(uva_plan* plan, const double* g, double* lam, double tol, int
     a.zero_x = 1;
     if (!rc) rc = reset_queue(plan, 1, s);
     if (!rc) rc = launch(plan, uva::TASK_CG, precision, a, 0, s);
-    return from_plan(plan, 0, lam, s, rc);
+    return check_fault(plan, s, from_plan(plan, 0, lam, s, rc));
 }
 
 int uva_arap_step_bwd(uva_plan* plan, const double* U, const double* lam, double* gU, void* stream) {
@@ -817,7 +849,7 @@ int uva_arap_step_bwd(uva_plan* plan, const double* U, const double* lam, double
     a.out = plan->ubuf[2];
     if (!rc) rc = reset_queue(plan, 1, s);
     if (!rc) rc = launch(plan, uva::TASK_STEP_BWD, UVA_PREC_F64, a, 0, s);
-    return from_plan(plan, 2, gU, s, rc);
+    return check_fault(plan, s, from_plan(plan, 2, gU, s, rc));
 }
 
 int uva_signed_areas(uva_plan* plan, const double* U, double* sa, void* stream) {
@@ -837,7 +869,7 @@ int uva_flips_per_chart(uva_plan* plan, const double* U, double* counts, void* s
     a.out_isl = counts;
     if (!rc) rc = reset_queue(plan, 1, s);
     if (!rc) rc = launch(plan, uva::TASK_FLIPS, UVA_PREC_MIXED, a, 0, s);
-    return rc;
+    return check_fault(plan, s, rc);
 }
 
 int uva_repair_jacobi(uva_plan* plan, double* U, int32_t rounds, void* stream) {
@@ -849,7 +881,7 @@ int uva_repair_jacobi(uva_plan* plan, double* U, int32_t rounds, void* stream) {
     a.rounds = rounds;
     if (!rc) rc = reset_queue(plan, 1, s);
     if (!rc) rc = launch(plan, uva::TASK_REPAIR, UVA_PREC_MIXED, a, 0, s);
-    return from_plan(plan, 0, U, s, rc);
+    return check_fault(plan, s, from_plan(plan, 0, U, s, rc));
 }
 
 int uva_orient_positive(uva_plan* plan, double* U, void* stream) {
@@ -859,7 +891,7 @@ int uva_orient_positive(uva_plan* plan, double* U, void* stream) {
     uva::TaskArgs a = base_args((double*)to_plan(plan, U, 0, s, rc));
     if (!rc) rc = reset_queue(plan, 1, s);
     if (!rc) rc = launch(plan, uva::TASK_ORIENT, UVA_PREC_MIXED, a, 0, s);
-    return from_plan(plan, 0, U, s, rc);
+    return check_fault(plan, s, from_plan(plan, 0, U, s, rc));
 }
 
 int uva_repair_seq(double* U, int64_t n, const int64_t* T, int64_t t, int32_t rounds) {

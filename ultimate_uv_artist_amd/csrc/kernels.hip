@@ -187,6 +187,37 @@ struct BlockTeam {
     }
 };
 
+// Wait of one thread at a team barrier: until the arrival count reaches
+// target (returns 0), the team's fault flag is set (returns 2: a member gave
+// up at an earlier barrier, so the team's results are already invalid and
+// waiting again would only stall the launch), or more than spin ticks pass
+// (returns 1: the caller sets the flag).  The count and the flag are read
+// together each round.  Env supplies count(), fault(), clock() and nap(); the
+// CPU emulator runs the same loop with host atomics and a counting clock.
+template <class Env>
+UVA_HD __forceinline__ int team_wait(Env& e, int target, long long spin) {
+    const long long t0 = e.clock();
+    for (;;) {
+        const int c = e.count();
+        const int f = e.fault();
+        if (c >= target) return 0;
+        if (f) return 2;
+        if (e.clock() - t0 > spin) return 1;
+        e.nap();
+    }
+}
+
+// The device side of team_wait: relaxed agent-scope loads of the counter and
+// the flag (the caller orders memory around the wait).
+struct DevWait {
+    const int* cnt;
+    const int* flt;
+    __device__ int count() const { return __hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+    __device__ int fault() const { return __hip_atomic_load(flt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+    __device__ long long clock() const { return wall_clock64(); }
+    __device__ void nap() const { __builtin_amdgcn_s_sleep(1); }
+};
+
 // Hardware id of the XCD this wavefront runs on (HW_REG_XCC_ID).
 __device__ __forceinline__ int xcc_id() { return __builtin_amdgcn_s_getreg((15 << 11) | 20) & 15; }
 
@@ -206,7 +237,9 @@ __device__ __forceinline__ int xcc_id() { return __builtin_amdgcn_s_getreg((15 <
 // is invalidated so loads see the other members' data.  Otherwise the counter
 // uses agent-scope release/acquire, which writes back and invalidates L2.
 // A wait longer than spin_ticks of the device clock gives up and sets *fault
-// (a hang guard: the island's result is then invalid and the host reports it).
+// (a hang guard: the island's result is then invalid and the host reports it);
+// once the flag is set, no barrier of the launch waits any more, so every
+// workgroup still runs to its end.
 template <int BS>
 struct TeamWG {
     static constexpr int kWaves = BS / 64;
@@ -251,23 +284,18 @@ struct TeamWG {
         ++rounds;
         if (threadIdx.x == 0) {
             const int target = rounds * m;
-            const long long t0 = wall_clock64();
-            bool late = false;
+            DevWait w{cnt, fault};
+            int r;
             if (local) {
                 __hip_atomic_fetch_add(cnt, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                while (__hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-                    __builtin_amdgcn_s_sleep(1);
-                    if (wall_clock64() - t0 > spin) { late = true; break; }
-                }
+                r = team_wait(w, target, spin);
                 asm volatile("buffer_inv sc0\n\ts_waitcnt vmcnt(0)" ::: "memory");
             } else {
                 __hip_atomic_fetch_add(cnt, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-                while (__hip_atomic_load(cnt, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
-                    __builtin_amdgcn_s_sleep(1);
-                    if (wall_clock64() - t0 > spin) { late = true; break; }
-                }
+                r = team_wait(w, target, spin);
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
             }
-            if (late) atomicOr(fault, 1);
+            if (r == 1) atomicOr(fault, 1);
         }
         __syncthreads();
     }
@@ -1066,13 +1094,14 @@ __global__ void signed_areas_kernel(DevPlan P, const double2* __restrict__ U, do
 template <int KIND, int PREC>
 static hipError_t launch_kind(const DevPlan& P, const TaskArgs& a, int slot, int cus, hipStream_t s) {
     constexpr int BS = block_size(KIND, PREC);
-    static int occ = 0;   // resident workgroups per CU of this instantiation (same on every call)
-    if (occ == 0) {
+    // resident workgroups per CU of this instantiation (the same on every
+    // call; a function-local static is initialised once, thread-safely)
+    static const int occ = [] {
         int nb = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, island_kernel<BS, KIND, PREC>, BS, 0) != hipSuccess)
             nb = 1;
-        occ = nb > 0 ? nb : 1;
-    }
+        return nb > 0 ? nb : 1;
+    }();
     // enough workgroups for every workgroup task and the wavefront tasks, at
     // most what the device keeps resident; blocks go to the XCDs round-robin,
     // so every XCD list gets at least as many workgroups as its largest team

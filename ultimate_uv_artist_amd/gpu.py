@@ -20,7 +20,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .plan import CsrSystem, IslandPlan, pinned_laplacian, require_gpu
+from .plan import CsrSystem, IslandPlan, batch_frames_dev, pinned_laplacian, require_gpu
 
 BATCH_CG_TOL = 1e-6   # gpu.py:184 Batch._cg(tol=1e-6)
 
@@ -102,15 +102,63 @@ class Batch:
         self.pins = torch.as_tensor(self.offsets[:-1], device=self.dev)
         self._scale = torch.ones(len(self.T), device=self.dev, dtype=torch.float64)
         self._frames = None
+        self._A = None
 
     # ---------------------------------------------------------------- layout
+    # The layout lives in FP64 on the device and is updated in place by the
+    # kernels; the tensors handed out and taken in are copies, so a caller's
+    # tensor never changes behind its back (the reference rebinds self.U to
+    # new tensors, gpu.py:230-249).
     @property
     def U(self):
-        return self._U.to(self.dt)
+        return self._U.clone() if self.dt == torch.float64 else self._U.to(self.dt)
 
     @U.setter
     def U(self, value):
-        self._U = value.to(self.dev, torch.float64).reshape(self.n, 2).contiguous()
+        self._U = value.to(self.dev, torch.float64).reshape(self.n, 2).clone().contiguous()
+
+    # ------------------------------------------------ per-triangle attributes
+    # area, ok, w, x1, x2, y2, X as the reference builds them in
+    # Batch.__init__ (gpu.py:88-121): float64 frames cast to the batch dtype,
+    # computed on the device when first read.
+    def _frame(self, key):
+        if self._frames is None:
+            V = torch.as_tensor(self._P, device=self.dev)
+            self._frames = batch_frames_dev(V, self.T)
+        v = self._frames[key]
+        return v if v.dtype == torch.bool else v.to(self.dt)
+
+    area = property(lambda self: self._frame("area"))
+    ok = property(lambda self: self._frame("ok"))
+    w = property(lambda self: self._frame("w"))
+    x1 = property(lambda self: self._frame("x1"))
+    x2 = property(lambda self: self._frame("x2"))
+    y2 = property(lambda self: self._frame("y2"))
+    X = property(lambda self: self._frame("X"))
+
+    # ------------------------------------------------ the pinned Laplacian
+    @property
+    def L(self):
+        """The batch Laplacian with every chart's soft pin (gpu.py:126-142) as
+        a torch CSR tensor with int32 indices in the batch dtype."""
+        self._system()
+        A = self._A
+        return torch.sparse_csr_tensor(torch.as_tensor(A.indptr, dtype=torch.int32, device=self.dev),
+                                       torch.as_tensor(A.indices, dtype=torch.int32, device=self.dev),
+                                       torch.as_tensor(A.data, device=self.dev).to(self.dt), A.shape)
+
+    @property
+    def dinv(self):
+        """Jacobi weights 1 / max(diag L, 1e-30) (gpu.py:142)."""
+        self._system()
+        d = torch.as_tensor(self._A.diagonal(), device=self.dev)
+        return (1.0 / d.clamp_min(1e-30)).to(self.dt)
+
+    @property
+    def wpin(self):
+        """The batch-global soft-pin weight max |W| (gpu.py:137-139), a 0-d tensor."""
+        self._system()
+        return torch.tensor(self._wpin, device=self.dev, dtype=self.dt)
 
     @property
     def scale(self):
@@ -122,6 +170,16 @@ class Batch:
         self._plan.set_scale(self._scale)
 
     # ---------------------------------------------------------------- pieces
+    def jacobians(self, U):
+        """(j00, j01, j10, j11) per triangle of the map from the frames to U
+        (gpu.py:145-151), in U's dtype."""
+        T = self.T
+        x1, x2, y2 = (self._frame(k).to(U.dtype) for k in ("x1", "x2", "y2"))
+        u0 = U[T[:, 0]]
+        d1, d2 = U[T[:, 1]] - u0, U[T[:, 2]] - u0
+        j00, j10 = d1[:, 0] / x1, d1[:, 1] / x1
+        return j00, (d2[:, 0] - j00 * x2) / y2, j10, (d2[:, 1] - j10 * x2) / y2
+
     def signed_area(self, U):
         """Signed UV area per triangle (gpu.py:153-156)."""
         T = self.T
@@ -143,6 +201,7 @@ class Batch:
             A, wpin = pinned_laplacian(self._P, self._Tg, self.offsets[:-1], dev=self.dev)
             self._sys = CsrSystem(A, self.dev)
             self._plan.set_pins(self.offsets[:-1], wpin)
+            self._A, self._wpin = A, wpin
         return self._sys
 
     def arap(self, iters=8, line_search=True, cg_iters=40, cg_tol=BATCH_CG_TOL):
@@ -152,7 +211,7 @@ class Batch:
         or after ``cg_iters`` iterations (gpu.py:184-202), revert of
         non-finite charts, per-chart flip-safe step when ``line_search``."""
         system = self._system()
-        U = self._U
+        U = self._U.clone()
         for _ in range(int(iters)):
             b = self._plan.local_rhs(U, _lib.UVA_MODE_BATCH)
             Un = U.clone()
@@ -170,16 +229,21 @@ class Batch:
                 U.add_(D * step[self.cv][:, None])
             else:
                 U.copy_(Un)
+        self._U = U
         return self.U
 
     def repair_flips(self, rounds=60):
         """Jacobi 1-ring relaxation of the vertices of folded triangles (gpu.py:251-273)."""
-        self._plan.repair_jacobi(self._U, rounds)
+        U = self._U.clone()
+        self._plan.repair_jacobi(U, rounds)
+        self._U = U
         return self.U
 
     def orient_positive(self):
         """Mirror charts whose area-weighted orientation is negative (gpu.py:275-281)."""
-        self._plan.orient_positive(self._U)
+        U = self._U.clone()
+        self._plan.orient_positive(U)
+        self._U = U
 
     def shrink(self, tri_idx, factor=0.6, floor=0.05, rings=3):
         """Lower the target size of the given triangles and their ``rings``

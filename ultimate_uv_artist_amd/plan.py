@@ -75,10 +75,15 @@ def half_cots(V, F):
     return w
 
 
-def _half_cots_dev(V, F):
-    """half_cots on device tensors V (n,3) float64, F (t,3) int64."""
-    p0, p1, p2 = V[F[:, 0]], V[F[:, 1]], V[F[:, 2]]
-    e1, e2 = p1 - p0, p2 - p0
+def batch_frames_dev(V, F):
+    """The per-triangle quantities of gpu.Batch (gpu.py:88-121) on device
+    tensors V (n,3) float64 and F (t,3) int64, in float64: frames X (t,3,2)
+    (zero where the triangle is too thin for a frame), area (t,), the usable
+    mask ok (t,) bool, half cotangent weights w (t,3) clipped to +-5e3 (zero
+    where not usable), and the clamped frame entries x1, x2, y2 (t,) that
+    Batch.jacobians divides by."""
+    p0 = V[F[:, 0]]
+    e1, e2 = V[F[:, 1]] - p0, V[F[:, 2]] - p0
     l1 = e1.norm(dim=1)
     nrm = torch.cross(e1, e2, dim=1)
     dbl = nrm.norm(dim=1)
@@ -96,7 +101,14 @@ def _half_cots_dev(V, F):
         cr = (a[:, 0] * b[:, 1] - a[:, 1] * b[:, 0]).abs().clamp_min(1e-30)
         w[:, k] = 0.5 * ((a * b).sum(1) / cr).clamp(-1e4, 1e4)
     w[~ok] = 0.0
-    return w
+    y2 = X[:, 2, 1]
+    return {"X": X, "area": 0.5 * dbl, "ok": ok, "w": w, "x1": X[:, 1, 0].clamp_min(1e-30),
+            "x2": X[:, 2, 0].clone(), "y2": torch.where(y2.abs() > 1e-30, y2, torch.full_like(y2, 1e-30))}
+
+
+def _half_cots_dev(V, F):
+    """half_cots on device tensors V (n,3) float64, F (t,3) int64."""
+    return batch_frames_dev(V, F)["w"]
 
 
 def pinned_laplacian(V, F, pins, dev=None):
