@@ -15,11 +15,12 @@
 // over aggregates of g consecutive 64-row slices.  The plan orders vertices
 // by Morton code, so slices are compact surface patches and the coarse level
 // removes the smooth error that Jacobi-PCG needs O(sqrt(n)) iterations for.
-// Z^T r: each lane parks r of its rows in their u slots in LDS (free until the
-// new u is written), and one thread per slice sums the slice's 64 rows; the
-// slices' sums go through global memory to the whole team, and one thread per
-// aggregate of the member applies the aggregate's row of Ac^-1 (kept in LDS
-// for the whole solve).
+// Z^T r: a slice is one wavefront's slot (lane = row), so its sum is a
+// butterfly reduction across the wavefront's lanes of the r each lane holds
+// in registers (FP64).  The slices' sums go to LDS (one workgroup) or through
+// global memory to the whole team; then kPart threads per aggregate of the
+// member each apply a strided share of the aggregate's row of Ac^-1 (kept in
+// LDS for the whole solve) and one thread adds the shares in order.
 //
 // Members of a multi-workgroup team keep u for a window of rows that extends
 // H = min(bandwidth, kHalo) rows beyond each end of their own range.  They
@@ -43,8 +44,11 @@ UVA_HD __forceinline__ int res_vec_bytes(int rows, int halo) {
 // rows of Ac^-1 are stored with an odd stride, so the threads of different
 // aggregates read different LDS banks
 UVA_HD __forceinline__ int coarse_stride(int nc) { return nc | 1; }
-UVA_HD __forceinline__ int res_coarse_bytes(int nc, int na) {
-    return nc > 0 ? align16(na * coarse_stride(nc) * 4) + (nc + na) * 16 : 0;
+constexpr int kPart = 8;   // threads sharing one row of the coarse mat-vec
+// rows of Ac^-1, the restricted residual of every aggregate, the member's
+// correction, the mat-vec shares and the member's slice sums (nsl slices)
+UVA_HD __forceinline__ int res_coarse_bytes(int nc, int na, int nsl) {
+    return nc > 0 ? align16(na * coarse_stride(nc) * 4) + (nc + na + na * kPart + nsl) * 16 : 0;
 }
 
 template <int PREC, int RPT, class Team>
@@ -72,6 +76,8 @@ struct ResCG {
     float* lci;     // LDS: rows alo .. alo + na - 1 of Ac^-1
     double2* lrc;   // LDS: Z^T r of every aggregate
     double2* ly;    // LDS: the member's aggregates' coarse correction
+    double2* lpt;   // LDS: kPart shares of each of the member's mat-vec rows
+    double2* lcs;   // LDS: Z^T r of the member's slices (one workgroup)
     V2* gu;     // global: u of the published rows (teams only)
     V2 r[RPT], p[RPT], s[RPT], w[RPT];
 
@@ -104,6 +110,8 @@ struct ResCG {
         lci = (float*)(base + res_vec_bytes<V2, V>(rows, multi ? kHalo : 0));
         lrc = (double2*)((char*)lci + align16(na * ncp * 4));
         ly = lrc + nc;
+        lpt = ly + na;
+        lcs = lpt + na * kPart;
         gu = (V2*)P.vu;
 #pragma unroll
         for (int j = 0; j < RPT; ++j) {
@@ -150,54 +158,56 @@ struct ResCG {
         }
         tm.bsync();
     }
-    // Z^T r of the member's slices (r parked in the u slots) to global memory.
-    UVA_HD __forceinline__ void restrict_slices() {
-        tm.bsync();
-        for (int t = tm.tid(); t < shi - slo; t += tm.tstride()) {
-            double2 acc = make_double2(0.0, 0.0);
-            const int r0 = rlo + t * kSlice, r1 = min(r0 + kSlice, rhi);
-            for (int row = r0; row < r1; ++row) {
-                const V2 v = lu[row - wlo];
-                acc.x += (double)v.x;
-                acc.y += (double)v.y;
-            }
-            P.crc[d.s0 + slo + t] = acc;
+    // Z^T r of slot j's slice: the sum over the wavefront's lanes of the r
+    // each holds (0 for rows past the island).  Called by every lane of a
+    // wavefront whose slot j exists; lane 0 stores the sum, to LDS for one
+    // workgroup or to global memory for a team.
+    UVA_HD __forceinline__ void restrict_slot(int j, V2 rv) {
+        const double2 sm = tm.lane_sum2(make_double2((double)rv.x, (double)rv.y));
+        if (lane == 0) {
+            const int t = j * W + wv;
+            if (multi) P.crc[d.s0 + slo + t] = sm;
+            else lcs[t] = sm;
         }
     }
-    // After every slice's Z^T r is in global memory: Z^T r per aggregate,
-    // then y = Ac^-1 Z^T r for the member's aggregates, into LDS.
+    // After every slice's Z^T r is stored: Z^T r per aggregate, then
+    // y = Ac^-1 Z^T r for the member's aggregates, into LDS.
     UVA_HD __forceinline__ void coarse_solve() {
-        tm.sync();
+        if (multi) tm.sync();
+        else tm.bsync();
         for (int b = tm.tid(); b < nc; b += tm.tstride()) {
             double2 acc = make_double2(0.0, 0.0);
             const int s1 = min((b + 1) * g, d.ns);
             for (int sl = b * g; sl < s1; ++sl) {
-                const double2 v = P.crc[d.s0 + sl];
+                const double2 v = multi ? P.crc[d.s0 + sl] : lcs[sl - slo];
                 acc.x += v.x;
                 acc.y += v.y;
             }
             lrc[b] = acc;
         }
         tm.bsync();
-        for (int a = tm.tid(); a < na; a += tm.tstride()) {
+        // share q of row a: columns q, q + kPart, ...
+        for (int i = tm.tid(); i < na * kPart; i += tm.tstride()) {
+            const int a = i / kPart, q = i - a * kPart;
             const float* ci = lci + a * ncp;
-            double yx0 = 0.0, yy0 = 0.0, yx1 = 0.0, yy1 = 0.0;
-            int b = 0;
-            for (; b + 1 < nc; b += 2) {
-                const double c0 = (double)ci[b], c1 = (double)ci[b + 1];
-                const double2 v0 = lrc[b], v1 = lrc[b + 1];
-                yx0 = fma(c0, v0.x, yx0);
-                yy0 = fma(c0, v0.y, yy0);
-                yx1 = fma(c1, v1.x, yx1);
-                yy1 = fma(c1, v1.y, yy1);
+            double yx = 0.0, yy = 0.0;
+            for (int b = q; b < nc; b += kPart) {
+                const double c = (double)ci[b];
+                const double2 v = lrc[b];
+                yx = fma(c, v.x, yx);
+                yy = fma(c, v.y, yy);
             }
-            if (b < nc) {
-                const double c0 = (double)ci[b];
-                const double2 v0 = lrc[b];
-                yx0 = fma(c0, v0.x, yx0);
-                yy0 = fma(c0, v0.y, yy0);
+            lpt[i] = make_double2(yx, yy);
+        }
+        tm.bsync();
+        for (int a = tm.tid(); a < na; a += tm.tstride()) {
+            double yx = 0.0, yy = 0.0;
+            for (int q = 0; q < kPart; ++q) {
+                const double2 v = lpt[a * kPart + q];
+                yx += v.x;
+                yy += v.y;
             }
-            ly[a] = make_double2(yx0 + yx1, yy0 + yy1);
+            ly[a] = make_double2(yx, yy);
         }
         tm.bsync();
     }
@@ -352,10 +362,9 @@ struct ResCG {
                     le[l] = zero;
                     a1.x += PT::kResid64 ? rx * rx : (double)((float)rx * (float)rx);
                     a1.y += PT::kResid64 ? ry * ry : (double)((float)ry * (float)ry);
-                    if (coarse) lu[row - wlo] = r[j];
                 }
+                if (coarse) restrict_slot(j, r[j]);
             }
-            if (coarse) restrict_slices();
             const double2 ru = precondition();
             a1.z += ru.x;
             a1.w += ru.y;
@@ -400,10 +409,9 @@ struct ResCG {
                         r[j].y = fma(-ay, s[j].y, r[j].y);
                         g1.x += acc_mul<PREC>(r[j].x, r[j].x);
                         g1.y += acc_mul<PREC>(r[j].y, r[j].y);
-                        if (coarse) lu[rlo + l - wlo] = r[j];
                     }
+                    if (coarse) restrict_slot(j, r[j]);
                 }
-                if (coarse) restrict_slices();
                 const double2 ru2 = precondition();
                 g1.z += ru2.x;
                 g1.w += ru2.y;
@@ -480,7 +488,7 @@ UVA_HD __forceinline__ CGOut cg_arap(Team& tm, const DevPlan& P, const Isl& d, d
     const int vec = res_vec_bytes<V2, V>(nsl * kSlice, tm.multi() ? kHalo : 0);
     if (tm.lvec() != nullptr && vec <= tm.lvec_bytes() && rpt <= kMaxRpt) {
         const int nc = d.cg > 0 ? (d.ns + d.cg - 1) / d.cg : 0;
-        const bool co = coarse && d.cofs >= 0 && vec + res_coarse_bytes(nc, na) <= tm.lvec_bytes();
+        const bool co = coarse && d.cofs >= 0 && vec + res_coarse_bytes(nc, na, nsl) <= tm.lvec_bytes();
         if (rpt <= 2) return ResCG<PREC, 2, Team>(tm, P, d, co).solve(b2, tol, maxit);
         if (rpt <= 4) return ResCG<PREC, 4, Team>(tm, P, d, co).solve(b2, tol, maxit);
         return ResCG<PREC, kMaxRpt, Team>(tm, P, d, co).solve(b2, tol, maxit);

@@ -60,16 +60,59 @@ UVA_HD __forceinline__ long long uva_clock() {
 }
 
 // ---------------------------------------------------------------- reductions
+// Cross-lane reductions of a whole wavefront (every lane active), built from
+// register-to-register lane moves rather than LDS permutes: DPP quad
+// permutations swap lanes 1 and 2 apart, the half-row and row mirrors pair
+// the two halves of each 8- and 16-lane row, a swizzle pairs lanes 16 apart,
+// and the two 32-lane halves meet through lanes 0 and 32 read as scalars.
+// After each stage a lane holds the combination of two equal-sized groups
+// whose values are uniform within each group, and a + b == b + a bit for bit,
+// so every lane ends with the same value in a fixed order:
+//   ((((q0 + q1) + ...) per 4, 8, 16, 32 lanes), then lanes 0..31 + 32..63.
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+__device__ __forceinline__ double swizzle_xor16(double v) {
+    // ds_swizzle bit mode: and 0x1f, or 0, xor 0x10 (lane l reads l ^ 16 of its 32)
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_ds_swizzle((int)(b & 0xffffffffLL), 0x401F);
+    const int hi = __builtin_amdgcn_ds_swizzle((int)(b >> 32), 0x401F);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+__device__ __forceinline__ double read_lane(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+constexpr int kDppQuad1032 = 0xB1;   // quad_perm [1,0,3,2]
+constexpr int kDppQuad2301 = 0x4E;   // quad_perm [2,3,0,1]
+constexpr int kDppRowHalfMirror = 0x141;
+constexpr int kDppRowMirror = 0x140;
+
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
-    return v;  // identical in every lane: each butterfly stage adds a commuted pair
+    v += dpp_move<kDppQuad1032>(v);
+    v += dpp_move<kDppQuad2301>(v);
+    v += dpp_move<kDppRowHalfMirror>(v);
+    v += dpp_move<kDppRowMirror>(v);
+    v += swizzle_xor16(v);
+    return read_lane(v, 0) + read_lane(v, 32);
 }
 
 __device__ __forceinline__ double wave_min(double v) {
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) v = fmin(v, __shfl_xor(v, m, 64));
-    return v;
+    v = fmin(v, dpp_move<kDppQuad1032>(v));
+    v = fmin(v, dpp_move<kDppQuad2301>(v));
+    v = fmin(v, dpp_move<kDppRowHalfMirror>(v));
+    v = fmin(v, dpp_move<kDppRowMirror>(v));
+    v = fmin(v, swizzle_xor16(v));
+    return fmin(read_lane(v, 0), read_lane(v, 32));
 }
 
 // A team solves one island.  Its interface: lane()/wave()/nwaves() map rows
@@ -114,6 +157,8 @@ struct WaveTeam {
         sync();
         return uniform_d(v);
     }
+    // sum over the wavefront's lanes (every lane gets it; no barrier)
+    __device__ double2 lane_sum2(double2 v) const { return make_double2(wave_sum(v.x), wave_sum(v.y)); }
     __device__ void sum8(double4 a, double4 b, double4& sa, double4& sb) const {
         sa = sum4(a);
         b.x = wave_sum(b.x); b.y = wave_sum(b.y); b.z = wave_sum(b.z); b.w = wave_sum(b.w);
@@ -174,6 +219,7 @@ struct BlockTeam {
         sa = uniform_d4(s);
         sb = uniform_d4(t);
     }
+    __device__ double2 lane_sum2(double2 v) const { return make_double2(wave_sum(v.x), wave_sum(v.y)); }
     __device__ double min1(double v) {
         v = wave_min(v);
         double* buf = lds + par * kWaves * kSlots;
@@ -334,6 +380,7 @@ struct TeamWG {
         sum8(v, make_double4(0.0, 0.0, 0.0, 0.0), s, unused);
         return s;
     }
+    __device__ double2 lane_sum2(double2 v) const { return make_double2(wave_sum(v.x), wave_sum(v.y)); }
     __device__ double min1(double v) {
         v = wave_min(v);
         if (lane() == 0) lds[kSlots * wave()] = v;
