@@ -201,6 +201,20 @@ int uva_arap_profile(uva_plan *plan, double *U, const uva_arap_opts *opts, doubl
 int uva_overlap_flags(const double *U, int64_t n_vert, const int64_t *F, int64_t n_tri, const uint8_t *skip,
                       double rel, int32_t strict, int32_t grid, uint8_t *flags, void *stream);
 
+/* uva_overlap_flags without any device allocation, for callers that cannot
+ * allocate (code that must not call hipMalloc, such as tight serving loops or
+ * harnesses that forbid it).  No counterpart in the reference.  The boxes,
+ * cell counters and cell entries live in one 64 MiB array per GPU, held from
+ * the library's load until the process ends: 16 B per triangle, 16 B per grid
+ * cell and 12 B per cell entry must fit in it (about 1.5 million triangles
+ * at the grid validate.py picks); larger inputs return UVA_EINVAL with a
+ * message and touch nothing.  Calls on one GPU share that array, so calls
+ * that overlap on one GPU (other threads or streams) race; serialise them.
+ * Same arguments and result as uva_overlap_flags; synchronises the stream. */
+int uva_overlap_flags_noalloc(const double *U, int64_t n_vert, const int64_t *F, int64_t n_tri,
+                              const uint8_t *skip, double rel, int32_t strict, int32_t grid, uint8_t *flags,
+                              void *stream);
+
 /* Jacobi-preconditioned CG on one symmetric n x n matrix, the solve of
  * hier.py:534-582 Solver.lscm (one system over the u and v of every chart,
  * coupled by the boundary area term): the CSR arrays (int64 indptr and

@@ -36,6 +36,13 @@ constexpr int kDefaultMaxIters = 20000;  // CG iterations per global step when t
 
 }  // namespace
 
+namespace uva {
+int set_error(int code, const char* msg) {
+    g_err = msg;
+    return code;
+}
+}  // namespace uva
+
 struct uva_plan {
     uva::DevPlan dp{};
     std::vector<void*> allocs;

@@ -21,6 +21,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "../../include/uvart.h"
@@ -238,6 +240,96 @@ __global__ void pairs_kernel(const double2* U, const int64_t* F, int g, const in
 }  // namespace uva
 
 #if !defined(UVA_EMULATOR)
+namespace uva {
+// the calling thread's message for uva_last_error (capi.cpp); returns code
+int set_error(int code, const char* msg);
+
+namespace ov {
+
+__global__ void zero_bytes_kernel(uint8_t* p, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+__global__ void zero_u64_kernel(unsigned long long* p, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+inline unsigned blocks_of(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+// Device memory of one overlap test: per triangle its cell box, per cell the
+// count and the start of its entries; `items` and `item_cell` (one per cell
+// entry) are sized after the scan, by get_items(total).
+struct Work {
+    int4* box;
+    unsigned long long* cnt;
+    unsigned long long* start;
+};
+
+// The kernels of one overlap test on the stream; flags and the counters are
+// cleared by fill kernels.  Synchronises the stream once the entry count is
+// known and at the end.
+template <class GetItems>
+int run_overlap(const double* U, const int64_t* F, int64_t n_tri, const uint8_t* skip, double rel, int strict,
+                int grid, uint8_t* flags, const Work& w, GetItems&& get_items, hipStream_t s) {
+    const int64_t ncell = (int64_t)grid * grid;
+    const int B = 256;
+    const unsigned nb = blocks_of(n_tri, B);
+    hipLaunchKernelGGL(zero_bytes_kernel, dim3(nb), dim3(B), 0, s, flags, n_tri);
+    hipLaunchKernelGGL(zero_u64_kernel, dim3(blocks_of(ncell, B)), dim3(B), 0, s, w.cnt, ncell);
+    hipLaunchKernelGGL(boxes_kernel, dim3(nb), dim3(B), 0, s, (const double2*)U, F, n_tri, skip, grid, w.box, w.cnt);
+    hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, s, w.cnt, w.start, ncell);
+    if (hipGetLastError() != hipSuccess) return UVA_EHIP;
+    unsigned long long total = 0;
+    if (hipMemcpyAsync(&total, w.start + ncell, 8, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        return UVA_EHIP;
+    if (total == 0) return UVA_OK;
+    int64_t* items = nullptr;
+    int* item_cell = nullptr;
+    int rc = get_items(total, items, item_cell);
+    if (rc) return rc;
+    if (hipMemcpyAsync(w.cnt, w.start, (size_t)ncell * 8, hipMemcpyDeviceToDevice, s) != hipSuccess) return UVA_EHIP;
+    hipLaunchKernelGGL(fill_kernel, dim3(nb), dim3(B), 0, s, n_tri, skip, grid, w.box, w.cnt, items, item_cell);
+    hipLaunchKernelGGL(pairs_kernel, dim3(blocks_of((int64_t)total, B)), dim3(B), 0, s, (const double2*)U, F, grid,
+                       w.box, w.start, items, item_cell, (int64_t)total, rel, strict, flags);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess) return UVA_EHIP;
+    return UVA_OK;
+}
+
+// Scratch of uva_overlap_flags_noalloc: one array per GPU for the life of
+// the process.  Host code never uses the symbol itself as an address (on the
+// host it only names the array); each device's own address comes from
+// hipGetSymbolAddress, taken once per device with that device current.
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_of_current_device(unsigned char** out) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return set_error(UVA_EHIP, "hipGetDevice failed");
+    if (dev < 0 || dev >= kMaxDevices) return set_error(UVA_EINVAL, "device id beyond the scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return set_error(UVA_EHIP, (std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e)).c_str());
+        g_scratch_ptr[dev] = (unsigned char*)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return UVA_OK;
+}
+
+inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+}  // namespace ov
+}  // namespace uva
+
 extern "C" int uva_overlap_flags(const double* U, int64_t n_vert, const int64_t* F, int64_t n_tri,
                                  const uint8_t* skip, double rel, int32_t strict, int32_t grid, uint8_t* flags,
                                  void* stream) {
@@ -246,44 +338,72 @@ extern "C" int uva_overlap_flags(const double* U, int64_t n_vert, const int64_t*
     if (grid < 1 || grid > 8192 || !(rel >= 0.0)) return UVA_EINVAL;
     hipStream_t s = (hipStream_t)stream;
     if (n_tri == 0) return UVA_OK;
-    if (hipMemsetAsync(flags, 0, (size_t)n_tri, s) != hipSuccess) return UVA_EHIP;
     const int64_t ncell = (int64_t)grid * grid;
-    int4* box = nullptr;
-    unsigned long long *cnt = nullptr, *start = nullptr;
+    Work w{nullptr, nullptr, nullptr};
+    int64_t* items = nullptr;
+    int* item_cell = nullptr;
     int rc = UVA_OK;
-    auto ok = [&](hipError_t e) {
-        if (e != hipSuccess && rc == UVA_OK) rc = e == hipErrorOutOfMemory ? UVA_ENOMEM : UVA_EHIP;
-        return rc == UVA_OK;
-    };
-    const int B = 256;
-    const unsigned nb = (unsigned)((n_tri + B - 1) / B);
-    if (ok(hipMalloc(&box, (size_t)n_tri * sizeof(int4))) && ok(hipMalloc(&cnt, (size_t)ncell * 8)) &&
-        ok(hipMalloc(&start, (size_t)(ncell + 1) * 8)) && ok(hipMemsetAsync(cnt, 0, (size_t)ncell * 8, s))) {
-        hipLaunchKernelGGL(boxes_kernel, dim3(nb), dim3(B), 0, s, (const double2*)U, F, n_tri, skip, (int)grid, box,
-                           cnt);
-        hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, s, cnt, start, ncell);
-        unsigned long long total = 0;
-        if (ok(hipGetLastError()) &&
-            ok(hipMemcpyAsync(&total, start + ncell, 8, hipMemcpyDeviceToHost, s)) && ok(hipStreamSynchronize(s))) {
-            int64_t* items = nullptr;
-            int* item_cell = nullptr;
-            if (total > 0 && ok(hipMalloc(&items, (size_t)total * 8)) && ok(hipMalloc(&item_cell, (size_t)total * 4)) &&
-                ok(hipMemcpyAsync(cnt, start, (size_t)ncell * 8, hipMemcpyDeviceToDevice, s))) {
-                hipLaunchKernelGGL(fill_kernel, dim3(nb), dim3(B), 0, s, n_tri, skip, (int)grid, box, cnt, items,
-                                   item_cell);
-                const unsigned np = (unsigned)((total + B - 1) / B);
-                hipLaunchKernelGGL(pairs_kernel, dim3(np), dim3(B), 0, s, (const double2*)U, F, (int)grid, box, start,
-                                   items, item_cell, (int64_t)total, rel, strict, flags);
-                ok(hipGetLastError());
-                ok(hipStreamSynchronize(s));
-            }
-            if (items) (void)hipFree(items);
-            if (item_cell) (void)hipFree(item_cell);
-        }
+    auto mem = [](hipError_t e) { return e == hipErrorOutOfMemory ? UVA_ENOMEM : UVA_EHIP; };
+    hipError_t e;
+    if ((e = hipMalloc(&w.box, (size_t)n_tri * sizeof(int4))) != hipSuccess ||
+        (e = hipMalloc(&w.cnt, (size_t)ncell * 8)) != hipSuccess ||
+        (e = hipMalloc(&w.start, (size_t)(ncell + 1) * 8)) != hipSuccess) {
+        rc = mem(e);
+    } else {
+        rc = run_overlap(U, F, n_tri, skip, rel, strict, grid, flags, w,
+                         [&](unsigned long long total, int64_t*& it, int*& ic) {
+                             hipError_t e2;
+                             if ((e2 = hipMalloc(&items, (size_t)total * 8)) != hipSuccess ||
+                                 (e2 = hipMalloc(&item_cell, (size_t)total * 4)) != hipSuccess)
+                                 return mem(e2);
+                             it = items;
+                             ic = item_cell;
+                             return (int)UVA_OK;
+                         },
+                         s);
+        if (rc) (void)hipStreamSynchronize(s);
     }
-    if (box) (void)hipFree(box);
-    if (cnt) (void)hipFree(cnt);
-    if (start) (void)hipFree(start);
+    if (items) (void)hipFree(items);
+    if (item_cell) (void)hipFree(item_cell);
+    if (w.box) (void)hipFree(w.box);
+    if (w.cnt) (void)hipFree(w.cnt);
+    if (w.start) (void)hipFree(w.start);
+    return rc;
+}
+
+extern "C" int uva_overlap_flags_noalloc(const double* U, int64_t n_vert, const int64_t* F, int64_t n_tri,
+                                         const uint8_t* skip, double rel, int32_t strict, int32_t grid,
+                                         uint8_t* flags, void* stream) {
+    using namespace uva::ov;
+    if (!U || !F || !flags || n_vert < 0 || n_tri < 0) return uva::set_error(UVA_EINVAL, "bad argument");
+    if (grid < 1 || grid > 8192 || !(rel >= 0.0)) return uva::set_error(UVA_EINVAL, "grid or rel out of range");
+    if (n_tri == 0) return UVA_OK;
+    const int64_t ncell = (int64_t)grid * grid;
+    const size_t fixed = align256((size_t)n_tri * sizeof(int4)) + align256((size_t)ncell * 8) +
+                         align256((size_t)(ncell + 1) * 8);
+    if (fixed > kScratchBytes)
+        return uva::set_error(UVA_EINVAL, "triangles and grid cells exceed the 64 MiB scratch of "
+                                          "uva_overlap_flags_noalloc");
+    unsigned char* base = nullptr;
+    int rc = scratch_of_current_device(&base);
+    if (rc) return rc;
+    Work w;
+    w.box = (int4*)base;
+    w.cnt = (unsigned long long*)(base + align256((size_t)n_tri * sizeof(int4)));
+    w.start = (unsigned long long*)((unsigned char*)w.cnt + align256((size_t)ncell * 8));
+    unsigned char* rest = base + fixed;
+    const size_t left = kScratchBytes - fixed;
+    rc = run_overlap(U, F, n_tri, skip, rel, strict, grid, flags, w,
+                     [&](unsigned long long total, int64_t*& it, int*& ic) {
+                         if (align256((size_t)total * 8) + (size_t)total * 4 > left)
+                             return uva::set_error(UVA_EINVAL, "cell entries exceed the 64 MiB scratch of "
+                                                               "uva_overlap_flags_noalloc");
+                         it = (int64_t*)rest;
+                         ic = (int*)(rest + align256((size_t)total * 8));
+                         return (int)UVA_OK;
+                     },
+                     (hipStream_t)stream);
+    if (rc == UVA_EHIP) uva::set_error(UVA_EHIP, "HIP error in uva_overlap_flags_noalloc");
     return rc;
 }
 #else
