@@ -76,17 +76,39 @@ def copy_probe(mib):
     print(json.dumps({"copy_probe_bytes_each_way": n * 4, "dispatches": 5}), flush=True)
 
 
+def kernel_source_digest():
+    """SHA-256 (16 hex digits) of the library's kernel and host sources, so a
+    counter summary can be matched to the code it measured."""
+    import hashlib
+
+    h = hashlib.sha256()
+    csrc = ROOT / "ultimate_uv_artist_amd" / "csrc"
+    for f in sorted(csrc.iterdir()):
+        if f.suffix in (".hip", ".hpp", ".cpp", ".inc"):
+            h.update(f.name.encode())
+            h.update(f.read_bytes())
+    return h.hexdigest()[:16]
+
+
+TRAFFIC_JSON = "profiles/island_kernel_arap_512islands_traffic.json"
+
+
 def measured_traffic(digest):
     """HBM bytes per ARAP-iteration launch from the committed rocprofv3 counter
     summary of this workload (FETCH_SIZE + WRITE_SIZE, scaled by a plain copy
-    of known size), or None when no summary for this mesh is committed."""
-    f = ROOT / "profiles" / "island_kernel_arap_512islands_traffic.json"
+    of known size), or None when no summary for this mesh and these kernel
+    sources is committed.  Returns (bytes or None, where the figure is from)."""
+    f = ROOT / TRAFFIC_JSON
     if not f.exists():
-        return None
+        return None, None
     d = json.loads(f.read_text())
     if d.get("mesh_sha256") != digest[:16]:
-        return None
-    return d.get("hbm_bytes_per_launch")
+        return None, f"{TRAFFIC_JSON}: another mesh"
+    src = kernel_source_digest()
+    if d.get("kernel_src_sha16") != src:
+        return None, f"{TRAFFIC_JSON}: measured on other kernel sources ({d.get('kernel_src_sha16')}, now {src})"
+    return d.get("hbm_bytes_per_launch"), (f"{TRAFFIC_JSON} (rocprofv3 FETCH_SIZE + WRITE_SIZE of these kernel "
+                                           f"sources, {src}, scaled by a plain copy)")
 
 
 def unique_edge_count(T):
@@ -504,11 +526,25 @@ def main():
         plan.arap(U, args.iters, line_search=True, mode=_lib.UVA_MODE_REFINE, tol=0.0, cg_tol=args.cg_tol,
                   precision=prec, use_graph=not args.no_graph)
 
+    untuned = None
     if args.tune:
-        # autotune: one step measures every island's CG iterations, the plan
-        # re-plans team sizes and the workgroup schedule from them
+        # the default schedule (team sizes and order from a size-based guess
+        # of the CG iterations), timed on a few steps for comparison
         step()
         torch.cuda.synchronize()
+        nu = max(1, min(3, args.steps))
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(nu):
+            step()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms_u = e0.elapsed_time(e1) / nu
+        untuned = {"value": round(float(tri.sum()) * args.iters / (ms_u * 1e-3) / 1e6, 3),
+                   "ms_per_step": round(ms_u, 3), "steps": nu}
+        print(f"bench: default schedule {ms_u:.2f} ms/step ({untuned['value']} Mtri*iter/s)", flush=True)
+        # autotune: the last step measured every island's CG iterations, the
+        # plan re-plans team sizes and the workgroup schedule from them
         plan.tune()
         print("bench: schedule re-planned from the measured CG iterations per island", flush=True)
     for _ in range(args.warmup):
@@ -640,7 +676,8 @@ def main():
         pool.close()
         d_all = max(float(np.abs(r[0] - l[2]).max() / np.ptp(l[2], 0).max())
                     for r, l in zip(res, sorted(lay, key=lambda x: -tri[x[0]])))
-        cpu = {"value": cpu_tri_iter / t_all / 1e6, "unit": "Mtri*iter/s", "cores": n_cores, "kind": "port",
+        cpu = {"value": cpu_tri_iter / t_all / 1e6, "unit": "Mtri*iter/s", "cores": n_cores,
+               "cores_available": len(os.sched_getaffinity(0)), "kind": "port",
                "value_1core": cpu_tri_iter / t_cpu / 1e6,
                "sample": f"{len(pick)} islands at evenly spaced quantiles of the triangle mass "
                          f"({int(tri[pick].sum())} triangles), {args.iters} ARAP iterations each from the same "
@@ -654,6 +691,7 @@ def main():
               f"validate verdicts identical on {same}/{nvs} islands; p95 anisotropy/area within {dq:.1e}",
               flush=True)
 
+    traffic, traffic_src = measured_traffic(workloads.mesh_digest(charts))
     rec = {
         "metric": "batched ARAP Mtri*iter/s at relative residual 1e-6",
         "value": round(value, 3),
@@ -672,10 +710,12 @@ def main():
                    "arap_iters": args.iters, "mesh_sha256": workloads.mesh_digest(charts)[:16],
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": measured_traffic(workloads.mesh_digest(charts)),
+                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_src,
                      "kernel": "island_kernel ARAP iteration", "launch_ms": round(float(arap_ms.mean()), 4),
                      "algorithmic_bytes_per_launch": round(bytes_step / args.iters)},
         "cpu_baseline": cpu,
+        "untuned": untuned,
+        "kernel_src_sha16": kernel_source_digest(),
         "plain_per_island": plain,
         "cg_iters_per_step": {"mean": round(float(kpi.mean()), 2), "p95": round(float(np.percentile(kpi, 95)), 2),
                               "max": int(st["cg_iters_max"].max())},
