@@ -49,8 +49,9 @@ def parse():
                     help="keep the size-based schedule (no re-planning from measured iterations)")
     ap.add_argument("--no-plain", dest="plain", action="store_false",
                     help="skip the plain per-island GPU baseline")
-    ap.add_argument("--workload", default="disks", choices=["disks", "heightfield", "needles", "validate", "batch", "hier"],
-                    help="disks: the throughput workload; heightfield: 3,988 islands of 50-1,946 triangles "
+    ap.add_argument("--workload", default="disks", choices=["disks", "grid", "heightfield", "needles", "validate", "batch", "hier"],
+                    help="disks: the throughput workload; grid: the single 7,938-triangle grid island "
+                         "(use --iters 20 --cg-tol 1e-8); heightfield: 3,988 islands of 50-1,946 triangles "
                          "cut from a 2M-triangle heightfield (use --iters 15); needles: the precision study on "
                          "256 islands of 5k-60k triangles with 10%% needle triangles (one JSON line with every "
                          "precision variant); validate: validate_uvs of the 512 solved islands packed "
@@ -489,7 +490,11 @@ def main():
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
     t0 = time.perf_counter()
-    if args.workload == "heightfield":
+    if args.workload == "grid":
+        charts = [workloads.grid_island(seed=1)]
+        wl = (f"one 7,938-triangle grid island (64x64 grid, paraboloid, jitter), LSCM + {args.iters} ARAP, "
+              f"CG rel resid {args.cg_tol:g}")
+    elif args.workload == "heightfield":
         charts = workloads.heightfield_islands()
         wl = (f"{len(charts)} heightfield islands (1001x1001 fBm grid, Voronoi cells), LSCM + {args.iters} ARAP, "
               f"CG rel resid {args.cg_tol:g}")

@@ -201,6 +201,19 @@ int uva_arap_profile(uva_plan *plan, double *U, const uva_arap_opts *opts, doubl
 int uva_overlap_flags(const double *U, int64_t n_vert, const int64_t *F, int64_t n_tri, const uint8_t *skip,
                       double rel, int32_t strict, int32_t grid, uint8_t *flags, void *stream);
 
+/* Conservative raster of the gutter measurement (metrics.py:315-345
+ * _raster_labels, used by validate.py:180-185 min_gutter): for each texel of
+ * a res x res image over [0,1]^2 (row py, column px), last[py * res + px]
+ * gets the largest triangle index whose raster covers it, -1 where none does
+ * (the reference writes island labels in triangle order, so the texel's
+ * label is that triangle's island).  A triangle covers the texels of its
+ * floored, clamped bounding box (texel units) whose centre lies within
+ * 0.7072 texel of every edge line; NaN or out-of-range corners cover
+ * nothing.  U (n_vert x 2 float64), F (n_tri x 3 int64) and last (res * res
+ * int32) are device arrays; 1 <= res <= 8192.  Synchronises the stream. */
+int uva_raster_last(const double *U, int64_t n_vert, const int64_t *F, int64_t n_tri, int32_t res, int32_t *last,
+                    void *stream);
+
 /* uva_overlap_flags without any device allocation, for callers that cannot
  * allocate (code that must not call hipMalloc, such as tight serving loops or
  * harnesses that forbid it).  No counterpart in the reference.  The boxes,

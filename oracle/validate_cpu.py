@@ -288,3 +288,119 @@ def island_verdict(P, T, U):
     rep, _ = validate_islands(P, T, Un, np.zeros(len(T), np.int64))
     rep["chart_overlaps"] = int(chart_overlaps(U, T).sum())
     return rep
+
+
+# ----------------------------------------------------------------- gutters
+_I64_MIN = np.iinfo(np.int64).min
+
+
+def _floor_int(x):
+    """int(np.floor(x)) as the reference's compiled kernel converts it: NaN and
+    values outside the int64 range become the int64 minimum."""
+    f = np.floor(x)
+    ok = np.isfinite(f) & (np.abs(f) < 2.0 ** 63)
+    return np.where(ok, np.where(ok, f, 0.0).astype(np.int64), _I64_MIN)
+
+
+def raster_last(U, F, res, chunk=1 << 22):
+    """metrics.py:315-345 _raster_labels, as the triangle index that writes
+    each texel last: per texel of a res x res image the largest triangle index
+    whose conservative raster covers the texel centre, -1 where none does
+    (the reference writes labels in triangle order, so the texel keeps the
+    label of that triangle).  Vectorised over (triangle, texel) pairs of the
+    clamped bounding boxes."""
+    U = np.asarray(U, np.float64)
+    F = np.asarray(F, np.int64)
+    x0, y0 = U[F[:, 0], 0] * res, U[F[:, 0], 1] * res
+    x1, y1 = U[F[:, 1], 0] * res, U[F[:, 1], 1] * res
+    x2, y2 = U[F[:, 2], 0] * res, U[F[:, 2], 1] * res
+    area = (x1 - x0) * (y2 - y0) - (x2 - x0) * (y1 - y0)
+    sw = area < 0                                  # counter-clockwise order
+    x1, x2 = np.where(sw, x2, x1), np.where(sw, x1, x2)
+    y1, y2 = np.where(sw, y2, y1), np.where(sw, y1, y2)
+    # Python's min/max of the reference: min(x0, min(x1, x2)) keeps x0 when it
+    # is NaN and skips a NaN x1/x2 (comparisons with NaN are false)
+    def pymin(a, b, c):
+        bc = np.where(c < b, c, b)
+        return np.where(bc < a, bc, a)
+
+    def pymax(a, b, c):
+        bc = np.where(c > b, c, b)
+        return np.where(bc > a, bc, a)
+
+    with np.errstate(invalid="ignore"):
+        minx = np.maximum(_floor_int(pymin(x0, x1, x2)), 0)
+        maxx = np.minimum(_floor_int(pymax(x0, x1, x2)), res - 1)
+        miny = np.maximum(_floor_int(pymin(y0, y1, y2)), 0)
+        maxy = np.minimum(_floor_int(pymax(y0, y1, y2)), res - 1)
+    ex = (x1 - x0, x2 - x1, x0 - x2)
+    ey = (y1 - y0, y2 - y1, y0 - y2)
+    ox, oy = (x0, x1, x2), (y0, y1, y2)
+    L = [np.sqrt(ex[k] ** 2 + ey[k] ** 2) for k in range(3)]
+    wx = np.where(maxx >= minx, maxx - minx + 1, 0)   # no wrap-around for the int64 minimum
+    wy = np.where(maxy >= miny, maxy - miny + 1, 0)
+    cnt = wx * wy
+    last = np.full(res * res, -1, np.int64)
+    tri = np.repeat(np.arange(len(F)), cnt)
+    start = np.cumsum(cnt) - cnt
+    for a in range(0, len(tri), chunk):
+        t = tri[a:a + chunk]
+        k = np.arange(a, a + len(t)) - start[t]
+        px = minx[t] + k % np.maximum(wx[t], 1)
+        py = miny[t] + k // np.maximum(wx[t], 1)
+        cx, cy = px + 0.5, py + 0.5
+        inside = np.ones(len(t), bool)
+        for e in range(3):
+            val = ex[e][t] * (cy - oy[e][t]) - ey[e][t] * (cx - ox[e][t])
+            inside &= ~(val < -0.7072 * L[e][t])
+        np.maximum.at(last, (py * res + px)[inside], t[inside])
+    return last.reshape(res, res)
+
+
+def min_gutter(U, F, isl, res=4096):
+    """metrics.py:348-373: the smallest gap in texels between two different
+    islands of the conservative raster (island id per triangle in isl): NaN
+    when the image is all empty or all full, 0 when two islands touch, else
+    the smallest sum of the distances from two adjacent empty texels to
+    their nearest filled texels of different islands, minus one.  Returns
+    (gap, label image)."""
+    last = raster_last(U, F, res)
+    isl = np.asarray(isl, np.int64)
+    img = np.where(last >= 0, isl[np.maximum(last, 0)], -1).astype(np.int32)
+    return gutter_of_labels(img), img
+
+
+def gutter_of_labels(img):
+    """The gap measurement of metrics.py:355-373 on a label image (-1 empty)."""
+    from scipy import ndimage
+
+    empty = img < 0
+    if empty.all() or not empty.any():
+        return float("nan")
+    d, (iy, ix) = ndimage.distance_transform_edt(empty, return_indices=True)
+    near = img[iy, ix]
+    touch = bool(((img[:, 1:] >= 0) & (img[:, :-1] >= 0) & (img[:, 1:] != img[:, :-1])).any() or
+                 ((img[1:, :] >= 0) & (img[:-1, :] >= 0) & (img[1:, :] != img[:-1, :])).any())
+    if touch:
+        return 0.0
+    g = np.inf
+    h = near[:, 1:] != near[:, :-1]
+    if h.any():
+        g = min(g, float((d[:, 1:] + d[:, :-1])[h].min()))
+    vv = near[1:, :] != near[:-1, :]
+    if vv.any():
+        g = min(g, float((d[1:, :] + d[:-1, :])[vv].min()))
+    return max(g - 1.0, 0.0)
+
+
+def validate_gutter(v, f, uv, atlas=4096, padding=None):
+    """The gutter entries of validate.py:180-185 (min_gutter_texels, and
+    gutter_ok with a padding) for UVs validated as stored (float32)."""
+    U = np.asarray(uv, np.float32).astype(np.float64)
+    isl, _ = uv_islands(f, len(v), U, v)
+    res = min(atlas, 8192)
+    g, _ = min_gutter(U, f, isl, res)
+    out = {"min_gutter_texels": float(g * atlas / res)}
+    if padding is not None:
+        out["gutter_ok"] = bool(out["min_gutter_texels"] >= padding - 1e-6)
+    return out

@@ -52,6 +52,7 @@ def lib():
                                                C.c_int32, C.c_void_p]
         _lib.uva_emu_csr_pcg.argtypes = [C.c_int64, _i64p, _i64p, _dp, _dp, _dp, C.c_int32, C.c_int32, C.c_int32,
                                          C.c_double, C.c_int32, C.POINTER(C.c_int32)]
+        _lib.uva_emu_raster_last.argtypes = [_dp, _i64p, C.c_int64, C.c_int32, C.c_void_p]
         _lib.uva_emu_team_guard.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int64, C.POINTER(C.c_int64),
                                             C.POINTER(C.c_int32)]
     return _lib
@@ -204,3 +205,13 @@ class EmuPlan:
         lib().uva_emu_stats2(self.h, en, ni, sp)
         return {"cg_iters": np.array(it[:]), "converged": np.array(cv[:]), "rel_resid": np.array(rr[:]),
                 "energy": np.array(en[:]), "iters": np.array(ni[:]), "step": np.array(sp[:])}
+
+
+def raster_last(U, F, res):
+    """uva_raster_last's per-texel code run on the host (triangles in order)."""
+    U = np.ascontiguousarray(U, np.float64)
+    F = np.ascontiguousarray(F, np.int64)
+    out = np.empty(res * res, np.int32)
+    lib().uva_emu_raster_last(U.ctypes.data_as(_dp), F.ctypes.data_as(_i64p), len(F), int(res),
+                              out.ctypes.data_as(C.c_void_p))
+    return out.reshape(res, res)

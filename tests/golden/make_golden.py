@@ -26,11 +26,15 @@ Only behaviour that the oracle's own checks cannot pin is recorded:
   * gpu.Batch.arap at its default CG budget (40 iterations: not converged),
   * charts.flatten on small charts (engine/charts.py with param.py and
     metrics.py), and charts.boundary_intersections / untangle_boundary
-    (with overlapfix.py and segment.py).
+    (with overlapfix.py and segment.py),
+  * tutte on three charts, and the gutter measurement of validate_uvs
+    (metrics.min_gutter's label image and gap, validate_uvs(gutter=True)
+    reports at small atlas sizes).
 Output: tests/golden/param_golden.npz, batch_golden.npz, slim_golden.npz,
 validate_golden.npz, hier_golden.npz and batch40_golden.npz
 flatten_golden.npz and boundary_golden.npz
-tutte_golden.npz (``--only slim|validate|hier|batch40|flatten|boundary|tutte`` writes one alone).
+tutte_golden.npz and gutter_golden.npz
+(``--only slim|validate|hier|batch40|flatten|boundary|tutte|gutter`` writes one alone).
 """
 from __future__ import annotations
 
@@ -461,6 +465,57 @@ def validate_fixtures(root, rp):
     print("validate_golden.npz", (HERE / "validate_golden.npz").stat().st_size, "bytes")
 
 
+def gutter_cases(rp):
+    """Layouts for the gutter measurement of validate_uvs: the validate_cases
+    layouts plus two squares a known gap apart, two squares touching, and a
+    ring of four islands around a fifth."""
+    v, _ = validate_cases(rp)
+    sq = np.array([[0, 0], [1, 0], [1, 1], [0, 1]], float)
+    sqF = np.array([[0, 1, 2], [0, 2, 3]])
+    out = [(n, P, F, U) for n, P, F, U in v]
+    for name, gap in (("squares_gap", 0.1), ("squares_touch", 0.0)):
+        U = np.r_[0.1 + 0.3 * sq, (0.4 + gap) + 0.3 * sq]
+        P = _flat(U)
+        P[4:, 2] = 1.0
+        out.append((name, P, np.r_[sqF, sqF + 4], U))
+    Us, Fs, off = [], [], 0
+    for cx, cy, s_ in ((0.5, 0.5, 0.2), (0.15, 0.15, 0.1), (0.85, 0.15, 0.1), (0.15, 0.85, 0.12), (0.8, 0.8, 0.15)):
+        Us.append(np.array([cx, cy]) + s_ * (sq - 0.5))
+        Fs.append(sqF + off)
+        off += 4
+    U = np.concatenate(Us)
+    P = _flat(U)
+    P[:, 2] = np.repeat(np.arange(5), 4)
+    out.append(("five", P, np.concatenate(Fs), U))
+    return out
+
+
+def gutter_fixtures(root, rp):
+    """validate_uvs(gutter=True) reports at small atlas sizes (the raster is
+    plain Python here), and metrics.min_gutter's label image with the island
+    labels it used.  Output: tests/golden/gutter_golden.npz."""
+    rv = load_validate(root)
+    rm = sys.modules["ref_engine.metrics"]
+    cases = gutter_cases(rp)
+    out = {"names": np.array([n for n, *_ in cases]), "atlases": np.array([64, 256])}
+    keys = ("min_gutter_texels", "gutter_ok", "valid")
+    for i, (name, P, F, U) in enumerate(cases):
+        out[f"g{i}_P"], out[f"g{i}_F"], out[f"g{i}_U"] = P, np.asarray(F, np.int32), U
+        for a in (64, 256):
+            rep, _ = rv.validate_uvs(P, F, U, atlas=a, padding=2, gutter=True)
+            out[f"g{i}_a{a}"] = np.array([float(rep.get(k, -1)) for k in keys])
+        U32 = np.asarray(U, np.float32).astype(np.float64)
+        isl, _ = rm.uv_islands(np.asarray(F, np.int64), len(P), U32, np.asarray(P, np.float64))
+        g, img = rm.min_gutter(U32, np.asarray(F, np.int64), isl, 128)
+        out[f"g{i}_isl"] = np.asarray(isl, np.int32)
+        out[f"g{i}_img128"] = img.astype(np.int32)
+        out[f"g{i}_gap128"] = np.array(g)
+        print(name, out[f"g{i}_a64"], out[f"g{i}_a256"], g, flush=True)
+    out["report_keys"] = np.array(keys)
+    np.savez_compressed(HERE / "gutter_golden.npz", **out)
+    print("gutter_golden.npz", (HERE / "gutter_golden.npz").stat().st_size, "bytes")
+
+
 def load_hier(root):
     """engine/hier.py with engine/gpu.py as its package sibling (the numba
     stand-in of load_reference; the Solver runs on CPU torch)."""
@@ -665,6 +720,8 @@ if __name__ == "__main__":
         boundary_fixtures(root)
     elif only == "tutte":
         tutte_fixtures(root, load_reference(root)[0])
+    elif only == "gutter":
+        gutter_fixtures(root, load_reference(root)[0])
     else:
         main(root)
         validate_fixtures(root, load_reference(root)[0])
@@ -673,3 +730,4 @@ if __name__ == "__main__":
         flatten_fixtures(root)
         boundary_fixtures(root)
         tutte_fixtures(root, load_reference(root)[0])
+        gutter_fixtures(root, load_reference(root)[0])

@@ -526,3 +526,26 @@ def test_emu_team_barrier_guard_stops_waiting_after_a_fault():
     assert spin < ticks < spin + 20 * barriers
     ticks, fault = emu.team_guard(4, -1, barriers, spin)
     assert fault == 0 and ticks < spin
+
+
+def test_emu_gutter_raster_equals_the_restatement():
+    """The per-texel code of uva_raster_last (host build) equals the
+    restatement of metrics.py's conservative raster on the gutter golden
+    layouts and on a random atlas with huge, tiny and non-finite corners."""
+    from pathlib import Path
+
+    from oracle import validate_cpu as vc
+
+    g = dict(np.load(Path(__file__).resolve().parent / "golden" / "gutter_golden.npz"))
+    for i in range(len(g["names"])):
+        U = np.asarray(g[f"g{i}_U"], np.float32).astype(np.float64)
+        F = g[f"g{i}_F"].astype(np.int64)
+        for res in (64, 128, 257):
+            assert np.array_equal(emu.raster_last(U, F, res), vc.raster_last(U, F, res)), (g["names"][i], res)
+    rng = np.random.default_rng(3)
+    U = rng.random((600, 2)) * 1.2 - 0.1
+    U[5] = [np.nan, 0.5]
+    U[6] = [1e30, 0.2]
+    U[7] = [-1e25, 0.9]
+    F = rng.integers(0, 600, (400, 3))
+    assert np.array_equal(emu.raster_last(U, F, 200), vc.raster_last(U, F, 200))

@@ -303,3 +303,27 @@ def test_tutte_matches_reference():
         P, T = g[f"{name}_P"], g[f"{name}_T"].astype(np.int64)
         U = pc.tutte(P, T)
         assert np.abs(U - g[f"{name}_U"]).max() <= 1e-10 * np.abs(g[f"{name}_U"]).max(), name
+
+
+def test_gutter_matches_reference():
+    """The gutter restatement (conservative raster as last-writer triangle,
+    distance-transform gap) against the reference's metrics.min_gutter label
+    images and gaps at 128 x 128 and its validate_uvs(gutter=True) entries at
+    atlas 64 and 256 (tests/golden/gutter_golden.npz)."""
+    from oracle import validate_cpu as vc
+
+    g = dict(np.load(GOLD / "gutter_golden.npz"))
+    keys = [str(k) for k in g["report_keys"]]
+    for i, name in enumerate(g["names"]):
+        P, F, U = g[f"g{i}_P"], g[f"g{i}_F"].astype(np.int64), g[f"g{i}_U"]
+        U32 = np.asarray(U, np.float32).astype(np.float64)
+        gap, img = vc.min_gutter(U32, F, g[f"g{i}_isl"], 128)
+        assert np.array_equal(img, g[f"g{i}_img128"]), name
+        ref = float(g[f"g{i}_gap128"])
+        assert (np.isnan(gap) and np.isnan(ref)) or gap == ref, name
+        for a in (64, 256):
+            rep = vc.validate_gutter(P, F, U, atlas=a, padding=2)
+            want = dict(zip(keys, g[f"g{i}_a{a}"]))
+            assert rep["min_gutter_texels"] == want["min_gutter_texels"] or \
+                (np.isnan(rep["min_gutter_texels"]) and np.isnan(want["min_gutter_texels"])), (name, a)
+            assert float(rep["gutter_ok"]) == want["gutter_ok"], (name, a)

@@ -774,3 +774,45 @@ def test_csr_pcg_matches_the_definition():
         assert its == kr, (k, its, kr)
         assert np.abs(x.cpu().numpy() - xr).max() <= 1e-10 * np.abs(xr).max(), k
     S.close()
+
+
+def test_gutter_on_gpu_matches_reference_and_oracle():
+    """validate_uvs with the gutter (the default) and min_gutter: the
+    reference's own entries at atlas 64 and 256 and its label images at 128
+    (tests/golden/gutter_golden.npz), and the GPU raster bit for bit equal to
+    the restatement's on a 60k-triangle atlas at 1024 and 4096 texels."""
+    from pathlib import Path
+
+    from oracle import validate_cpu as vc
+    from ultimate_uv_artist_amd import validate, workloads
+
+    g = dict(np.load(Path(__file__).resolve().parent / "golden" / "gutter_golden.npz"))
+    keys = [str(k) for k in g["report_keys"]]
+    for i, name in enumerate(g["names"]):
+        P, F, U = g[f"g{i}_P"], g[f"g{i}_F"].astype(np.int64), g[f"g{i}_U"]
+        U32 = np.asarray(U, np.float32).astype(np.float64)
+        gap, img = validate.min_gutter(U32, F, g[f"g{i}_isl"], 128)
+        assert np.array_equal(img, g[f"g{i}_img128"]), name
+        ref = float(g[f"g{i}_gap128"])
+        assert (np.isnan(gap) and np.isnan(ref)) or gap == ref, name
+        for a in (64, 256):
+            rep, _ = validate.validate_uvs(P, F, U, atlas=a, padding=2)
+            want = dict(zip(keys, g[f"g{i}_a{a}"]))
+            got = rep["min_gutter_texels"]
+            assert got == want["min_gutter_texels"] or (np.isnan(got) and np.isnan(want["min_gutter_texels"])), name
+            assert float(rep["gutter_ok"]) == want["gutter_ok"] and float(rep["valid"]) == want["valid"], name
+    rng = np.random.default_rng(73)
+    Ts, Us, off = [], [], 0
+    for k in range(4):
+        P, T = workloads.disk_island(rng, 15000)
+        U = (P[:, :2] - P[:, :2].min(0)) / np.ptp(P[:, :2]).max() * 0.45 + np.array([0.5 * (k % 2), 0.5 * (k // 2)])
+        Ts.append(T + off)
+        Us.append(U)
+        off += len(P)
+    U, T = np.concatenate(Us), np.concatenate(Ts)
+    isl = np.repeat(np.arange(4), [len(t) for t in Ts])
+    for res in (1024, 4096):
+        gap, img = validate.min_gutter(U, T, isl, res)
+        gap_r, img_r = vc.min_gutter(U, T, isl, res)
+        assert np.array_equal(img, img_r) and gap == gap_r, res
+        print(f"gutter: 60k-triangle atlas at {res}: {gap:.3f} texels", flush=True)

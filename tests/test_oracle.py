@@ -573,3 +573,31 @@ def test_tutte_rotated_loop_is_not_harmonic():
     interior = np.setdiff1d(np.arange(len(P)), loop)
     mean = np.array([U[sorted(nb[v])].mean(0) for v in interior])
     assert np.abs(U[interior] - mean).max() > 1e-3 * seg.sum()
+
+
+def test_gutter_of_two_aligned_squares():
+    """Two axis-aligned squares side by side on texel boundaries: the raster
+    visits the texels of the floored bounding box and keeps those whose
+    centre lies within 0.7072 texel of the triangle, so a square spanning
+    [10, 30] texels fills columns 10-30 (centre 30.5 is half a texel out).
+    Squares at [10, 30] and [40, 60] leave 9 empty columns between them; the
+    measure (the smallest sum of distances to different islands over
+    adjacent empty texels, minus one) is 8.  One square alone has no gap
+    (inf); touching squares give 0."""
+    from oracle import validate_cpu as vc
+
+    sq = np.array([[0, 0], [1, 0], [1, 1], [0, 1]], float)
+    F = np.array([[0, 1, 2], [0, 2, 3], [4, 5, 6], [4, 6, 7]])
+    res = 128
+    for x2, want in ((40.0, 8.0), (31.0, 0.0)):
+        U = np.r_[np.array([10.0, 40.0]) + 20.0 * sq, np.array([x2, 40.0]) + 20.0 * sq] / res
+        gap, img = vc.min_gutter(U, F, np.array([0, 0, 1, 1]), res)
+        assert gap == want
+    cols = np.nonzero((img >= 0).any(0))[0]
+    assert cols.min() == 10 and cols.max() == 51 and len(cols) == 42
+    gap, _ = vc.min_gutter(U[:4], F[:2], np.array([0, 0]), res)
+    assert gap == np.inf
+    # last writer: where the two squares overlap the later triangles' island wins
+    U = np.r_[np.array([10.0, 40.0]) + 20.0 * sq, np.array([20.0, 40.0]) + 20.0 * sq] / res
+    _, img = vc.min_gutter(U, F, np.array([0, 0, 1, 1]), res)
+    assert (img[45, 20:41] == 1).all() and (img[45, 10:20] == 0).all()

@@ -85,6 +85,7 @@ SIGNATURES = {
     "uva_overlap_flags": (C.c_int, [_vp, C.c_int64, _vp, C.c_int64, _vp, C.c_double, C.c_int32, C.c_int32, _vp, _vp]),
     "uva_overlap_flags_noalloc": (C.c_int, [_vp, C.c_int64, _vp, C.c_int64, _vp, C.c_double, C.c_int32, C.c_int32,
                                             _vp, _vp]),
+    "uva_raster_last": (C.c_int, [_vp, C.c_int64, _vp, C.c_int64, C.c_int32, _vp, _vp]),
     "uva_jacobi_pcg": (C.c_int, [C.c_int64, _i64p, _i64p, _dp, _vp, _vp, C.c_int32, C.c_int32, C.c_double,
                                  C.POINTER(C.c_int32), _vp]),
     "uva_csr_create": (C.c_int, [C.POINTER(_vp), C.c_int64, _i64p, _i64p, _dp, _vp]),

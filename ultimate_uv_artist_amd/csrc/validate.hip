@@ -159,6 +159,69 @@ __host__ __device__ inline void tri_box(const double2* U, const int64_t* F, int6
     y1 = cell_of(fmax(a.y, fmax(b.y, c.y)), g);
 }
 
+// ---------------------------------------------------------------- gutter raster
+// Conservative raster of metrics.py:315-345 (_raster_labels): a texel (px,
+// py) of the res x res image belongs to a triangle when it lies in the
+// floored bounding box of the triangle's corners (in texel units, clamped to
+// the image) and its centre is within 0.7072 texel of every edge line (on the
+// inside or up to that far outside).  The reference writes island labels in
+// triangle order, so a texel keeps the label of the largest triangle index
+// covering it: the kernels take the maximum index per texel with atomics.
+struct RasterTri {
+    double ox[3], oy[3], ex[3], ey[3], L[3];
+    long long minx, maxx, miny, maxy;
+};
+
+// int(np.floor(x)) of the reference's compiled kernel: NaN and values outside
+// the int64 range convert to the int64 minimum.
+__host__ __device__ inline long long floor_i64(double x) {
+    const double f = floor(x);
+    if (!(fabs(f) < 9223372036854775808.0)) return (long long)0x8000000000000000ULL;
+    return (long long)f;
+}
+
+// Python's min(a, min(b, c)) / max(...): a later argument replaces an
+// earlier one only when strictly smaller / larger.
+__host__ __device__ inline double pymin3(double a, double b, double c) {
+    const double bc = c < b ? c : b;
+    return bc < a ? bc : a;
+}
+__host__ __device__ inline double pymax3(double a, double b, double c) {
+    const double bc = c > b ? c : b;
+    return bc > a ? bc : a;
+}
+
+__host__ __device__ inline void raster_setup(const double2* U, const int64_t* F, int64_t f, int res, RasterTri& r) {
+    const double2 a = U[F[3 * f]], b = U[F[3 * f + 1]], c = U[F[3 * f + 2]];
+    double x0 = a.x * res, y0 = a.y * res, x1 = b.x * res, y1 = b.y * res, x2 = c.x * res, y2 = c.y * res;
+    const double area = (x1 - x0) * (y2 - y0) - (x2 - x0) * (y1 - y0);
+    if (area < 0) {
+        double t = x1; x1 = x2; x2 = t;
+        t = y1; y1 = y2; y2 = t;
+    }
+    long long v;
+    v = floor_i64(pymin3(x0, x1, x2)); r.minx = v > 0 ? v : 0;
+    v = floor_i64(pymax3(x0, x1, x2)); r.maxx = v < res - 1 ? v : res - 1;
+    v = floor_i64(pymin3(y0, y1, y2)); r.miny = v > 0 ? v : 0;
+    v = floor_i64(pymax3(y0, y1, y2)); r.maxy = v < res - 1 ? v : res - 1;
+    r.ex[0] = x1 - x0; r.ex[1] = x2 - x1; r.ex[2] = x0 - x2;
+    r.ey[0] = y1 - y0; r.ey[1] = y2 - y1; r.ey[2] = y0 - y2;
+    r.ox[0] = x0; r.ox[1] = x1; r.ox[2] = x2;
+    r.oy[0] = y0; r.oy[1] = y1; r.oy[2] = y2;
+    for (int k = 0; k < 3; ++k) r.L[k] = sqrt(r.ex[k] * r.ex[k] + r.ey[k] * r.ey[k]);
+}
+
+__host__ __device__ inline bool raster_covers(const RasterTri& r, long long px, long long py) {
+    const double cx = px + 0.5, cy = py + 0.5;
+    for (int k = 0; k < 3; ++k) {
+        const double e = r.ex[k] * (cy - r.oy[k]) - r.ey[k] * (cx - r.ox[k]);
+        if (e < -0.7072 * r.L[k]) return false;
+    }
+    return true;
+}
+
+constexpr long long kRasterSmall = 256;   // texels of a box one thread covers alone
+
 #if !defined(UVA_EMULATOR)
 __global__ void boxes_kernel(const double2* U, const int64_t* F, int64_t t, const uint8_t* skip, int g, int4* box,
                              unsigned long long* cnt) {
@@ -210,6 +273,46 @@ __global__ void fill_kernel(int64_t t, const uint8_t* skip, int g, const int4* b
             items[p] = f;
             item_cell[p] = c;
         }
+}
+
+__global__ void fill_i32_kernel(int* p, int64_t n, int v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// One thread per triangle: boxes of at most kRasterSmall texels are covered
+// here, larger ones are listed for raster_big_kernel.
+__global__ void raster_kernel(const double2* U, const int64_t* F, int64_t t, int res, int* last, int* big,
+                              int* nbig) {
+    const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= t) return;
+    RasterTri r;
+    raster_setup(U, F, f, res, r);
+    if (r.maxx < r.minx || r.maxy < r.miny) return;
+    const long long w = r.maxx - r.minx + 1, h = r.maxy - r.miny + 1;
+    if (w * h > kRasterSmall) {
+        big[atomicAdd(nbig, 1)] = (int)f;
+        return;
+    }
+    for (long long py = r.miny; py <= r.maxy; ++py)
+        for (long long px = r.minx; px <= r.maxx; ++px)
+            if (raster_covers(r, px, py)) atomicMax(last + py * res + px, (int)f);
+}
+
+// One workgroup per listed triangle at a time, its threads over the box.
+__global__ void raster_big_kernel(const double2* U, const int64_t* F, int res, int* last, const int* big,
+                                  const int* nbig) {
+    const int n = *nbig;
+    for (int i = blockIdx.x; i < n; i += gridDim.x) {
+        const int f = big[i];
+        RasterTri r;
+        raster_setup(U, F, f, res, r);
+        const long long w = r.maxx - r.minx + 1, h = r.maxy - r.miny + 1;
+        for (long long q = threadIdx.x; q < w * h; q += blockDim.x) {
+            const long long py = r.miny + q / w, px = r.minx + q % w;
+            if (raster_covers(r, px, py)) atomicMax(last + py * res + px, f);
+        }
+    }
 }
 
 __global__ void pairs_kernel(const double2* U, const int64_t* F, int g, const int4* box,
@@ -406,6 +509,31 @@ extern "C" int uva_overlap_flags_noalloc(const double* U, int64_t n_vert, const 
     if (rc == UVA_EHIP) uva::set_error(UVA_EHIP, "HIP error in uva_overlap_flags_noalloc");
     return rc;
 }
+extern "C" int uva_raster_last(const double* U, int64_t n_vert, const int64_t* F, int64_t n_tri, int32_t res,
+                               int32_t* last, void* stream) {
+    using namespace uva::ov;
+    if (!U || !F || !last || n_vert < 0 || n_tri < 0 || n_tri > 0x7fffffffLL)
+        return uva::set_error(UVA_EINVAL, "bad argument");
+    if (res < 1 || res > 8192) return uva::set_error(UVA_EINVAL, "res must be in [1, 8192]");
+    hipStream_t s = (hipStream_t)stream;
+    const int64_t npix = (int64_t)res * res;
+    const int B = 256;
+    hipLaunchKernelGGL(fill_i32_kernel, dim3(blocks_of(npix, B)), dim3(B), 0, s, last, npix, -1);
+    if (n_tri == 0) return hipGetLastError() == hipSuccess ? UVA_OK : uva::set_error(UVA_EHIP, "raster fill");
+    int* big = nullptr;
+    hipError_t e = hipMalloc(&big, ((size_t)n_tri + 1) * sizeof(int));
+    if (e != hipSuccess) return uva::set_error(e == hipErrorOutOfMemory ? UVA_ENOMEM : UVA_EHIP, "raster list");
+    int* nbig = big + n_tri;
+    hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(1), 0, s, nbig, (int64_t)1, 0);
+    hipLaunchKernelGGL(raster_kernel, dim3(blocks_of(n_tri, B)), dim3(B), 0, s, (const double2*)U, F, n_tri, (int)res,
+                       last, big, nbig);
+    hipLaunchKernelGGL(raster_big_kernel, dim3(1024), dim3(B), 0, s, (const double2*)U, F, (int)res, last, big, nbig);
+    int rc = UVA_OK;
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
+        rc = uva::set_error(UVA_EHIP, "raster kernels");
+    (void)hipFree(big);
+    return rc;
+}
 #else
 // The same algorithm on the host, one pair at a time (test-only emulation).
 extern "C" int uva_emu_overlap_flags(const double* U, int64_t n_vert, const int64_t* F, int64_t n_tri,
@@ -431,6 +559,21 @@ extern "C" int uva_emu_overlap_flags(const double* U, int64_t n_vert, const int6
                 if (x0[s] > x1[t] || x0[t] > x1[s] || y0[s] > y1[t] || y0[t] > y1[s]) continue;
                 if (pair_hits((const double2*)U, F, s, t, rel, strict)) flags[s] = flags[t] = 1;
             }
+    }
+    return 0;
+}
+// The raster of uva_raster_last on the host, triangles in order (test-only
+// emulation of the same per-texel code): last (res*res) gets the largest
+// covering triangle index, -1 elsewhere.
+extern "C" int uva_emu_raster_last(const double* U, const int64_t* F, int64_t n_tri, int32_t res, int32_t* last) {
+    using namespace uva::ov;
+    for (int64_t i = 0; i < (int64_t)res * res; ++i) last[i] = -1;
+    for (int64_t f = 0; f < n_tri; ++f) {
+        RasterTri r;
+        raster_setup((const double2*)U, F, f, res, r);
+        for (long long py = r.miny; py <= r.maxy; ++py)
+            for (long long px = r.minx; px <= r.maxx; ++px)
+                if (raster_covers(r, px, py)) last[py * res + px] = (int32_t)f;
     }
     return 0;
 }

@@ -3,15 +3,14 @@ triangle-overlap test on the GPU (``uva_overlap_flags``).
 
   overlap_flags        validate.py:94-136 _overlaps (and 257-299 _overlaps_strict)
   chart_overlaps       validate.py:212-226
-  validate_uvs         validate.py:139-210 with gutter=False: range, degenerate and
-                       sub-float32 triangles, flips against the island's dominant
+  validate_uvs         validate.py:139-210: range, degenerate and sub-float32
+                       triangles, flips against the island's dominant
                        orientation, mirrored islands, exact overlaps, float32
-                       contacts, preservation of positions / normals / colours
+                       contacts, the gutter between islands (gutter=True, the
+                       default), preservation of positions / normals / colours
   uv_islands           metrics.py:76-103
-
-The gutter measurement (metrics.min_gutter, ``gutter=True``) is not part of
-this build; the reference's pipeline validates with ``gutter=False``
-(pipeline.py:1254).
+  min_gutter           metrics.py:348-373: the conservative raster on the GPU
+                       (uva_raster_last), scipy's distance transform on the host
 """
 from __future__ import annotations
 
@@ -174,12 +173,66 @@ def uv_islands(faces, n_vertices, uv=None, vertices=None, dev=None):
     return isl.to(torch.int32).cpu().numpy(), n
 
 
+def _raster_last_dev(Ud, Fd, res, dev):
+    """uva_raster_last on device tensors: (res, res) int32 tensor of the
+    largest covering triangle index per texel, -1 where none covers it."""
+    last = torch.empty(int(res) * int(res), dtype=torch.int32, device=dev)
+    _lib.check(_lib.lib().uva_raster_last(Ud.data_ptr(), len(Ud), Fd.data_ptr(), len(Fd), int(res), last.data_ptr(),
+                                          stream_of(dev)), "uva_raster_last")
+    return last.view(int(res), int(res))
+
+
+def _gap_of_labels(img):
+    """The gap measurement of metrics.py:355-373 on a label image (-1 empty):
+    NaN for an image all empty or all full, 0 when texels of two islands are
+    side by side, else the smallest sum of the Euclidean distances of two
+    side-by-side empty texels to their nearest filled texels (scipy's exact
+    distance transform, as the reference) when those belong to different
+    islands, minus one."""
+    from scipy import ndimage
+
+    empty = img < 0
+    if empty.all() or not empty.any():
+        return float("nan")
+    filled = ~empty
+    if (filled[:, 1:] & filled[:, :-1] & (img[:, 1:] != img[:, :-1])).any() or \
+            (filled[1:, :] & filled[:-1, :] & (img[1:, :] != img[:-1, :])).any():
+        return 0.0
+    dist, (iy, ix) = ndimage.distance_transform_edt(empty, return_indices=True)
+    owner = img[iy, ix]
+    best = np.inf
+    for a, b, da, db in ((owner[:, 1:], owner[:, :-1], dist[:, 1:], dist[:, :-1]),
+                         (owner[1:, :], owner[:-1, :], dist[1:, :], dist[:-1, :])):
+        m = a != b
+        if m.any():
+            best = min(best, float((da + db)[m].min()))
+    return max(best - 1.0, 0.0)
+
+
+def min_gutter(uv, faces, islands, res=4096, dev=None):
+    """Smallest texel gap between two different islands (metrics.py:348-373),
+    with the conservative raster on the GPU (uva_raster_last) and the distance
+    transform on the host.  Returns (gap, label image int32)."""
+    dev = require_gpu(dev)
+    Ud = torch.as_tensor(np.ascontiguousarray(uv, np.float64).reshape(-1, 2), device=dev)
+    Fd = torch.as_tensor(np.ascontiguousarray(faces, np.int64).reshape(-1, 3), device=dev)
+    isl = torch.as_tensor(np.ascontiguousarray(islands, np.int64).reshape(-1), device=dev)
+    img = _labels_of_raster(_raster_last_dev(Ud, Fd, res, dev), isl).cpu().numpy()
+    return _gap_of_labels(img), img
+
+
+def _labels_of_raster(last, isl):
+    """Island label per texel (int32, -1 empty) from the last-triangle image."""
+    lab = isl[last.clamp_min(0).long()].to(torch.int32)
+    return torch.where(last >= 0, lab, torch.full_like(lab, -1))
+
+
 def validate_uvs(v, f, uv, atlas=4096, padding=None, in_v=None, in_f=None, in_normals=None, out_normals=None,
                  in_colors=None, out_colors=None, xref=None, gutter=True):
-    """Checks on final UVs (validate.py:139-210).  Returns (report dict,
-    per-triangle problem mask).  ``gutter=True`` is not supported."""
-    if gutter:
-        raise NotImplementedError("the gutter measurement is not part of this build; call with gutter=False")
+    """Checks on final UVs (validate.py:139-210), the gutter measurement
+    included (``gutter``: min_gutter_texels at min(atlas, 8192) texels per
+    side, and gutter_ok with a ``padding``).  Returns (report dict,
+    per-triangle problem mask)."""
     dev = require_gpu(None)
     v = np.ascontiguousarray(v, np.float64).reshape(-1, 3)
     f = np.ascontiguousarray(f, np.int64).reshape(-1, 3)
@@ -219,6 +272,12 @@ def validate_uvs(v, f, uv, atlas=4096, padding=None, in_v=None, in_f=None, in_no
     rep["float32_contacts"] = int((ov_strict & ~ov).sum())
     if ov.any():
         rep["islands_with_overlap"] = int(len(np.unique(isl[ov])))
+    if gutter:
+        res = min(int(atlas), 8192)
+        img = _labels_of_raster(_raster_last_dev(Ud, fd, res, dev), isld).cpu().numpy()
+        rep["min_gutter_texels"] = float(_gap_of_labels(img) * atlas / res)
+        if padding is not None:
+            rep["gutter_ok"] = bool(rep["min_gutter_texels"] >= padding - 1e-6)
     if in_v is not None and in_f is not None:
         in_v = np.asarray(in_v, np.float64)
         in_f = np.asarray(in_f, np.int64)
@@ -232,6 +291,8 @@ def validate_uvs(v, f, uv, atlas=4096, padding=None, in_v=None, in_f=None, in_no
     problems = ov | flipped
     bad = (rep["non_finite_uv"] + rep["uv_outside_unit_square"] + rep["flipped_triangles"] +
            rep["overlapping_triangles_exact"])
+    if gutter and padding is not None and not rep.get("gutter_ok", True):
+        bad += 1
     for k in ("triangle_count_preserved", "corner_positions_exact", "corner_normals_exact", "corner_colors_exact"):
         if rep.get(k) is False:
             bad += 1
