@@ -49,7 +49,7 @@ def parse():
                     help="keep the size-based schedule (no re-planning from measured iterations)")
     ap.add_argument("--no-plain", dest="plain", action="store_false",
                     help="skip the plain per-island GPU baseline")
-    ap.add_argument("--workload", default="disks", choices=["disks", "grid", "heightfield", "needles", "validate", "batch", "hier"],
+    ap.add_argument("--workload", default="disks", choices=["disks", "grid", "heightfield", "needles", "validate", "batch", "hier", "small"],
                     help="disks: the throughput workload; grid: the single 7,938-triangle grid island "
                          "(use --iters 20 --cg-tol 1e-8); heightfield: 3,988 islands of 50-1,946 triangles "
                          "cut from a 2M-triangle heightfield (use --iters 15); needles: the precision study on "
@@ -58,7 +58,10 @@ def parse():
                          "into one atlas, GPU overlap test against the CPU restatement on a sample; batch: "
                          "gpu.Batch.arap(8) of the 512 islands as one batch (one coupled 40-iteration CG "
                          "per global step), against the oracle's Batch on a sample; hier: hier.Solver.relax "
-                         "of the 512 islands as the charts of one subset, against the oracle on a sample")
+                         "of the 512 islands as the charts of one subset, against the oracle on a sample; "
+                         "small: charts.flatten of 40-90-face charts from 8 threads (per-call param, "
+                         "FlattenQueue, flatten_many) against the oracle on 1 and 8 cores")
+    ap.add_argument("--charts", type=int, default=2000, help="charts of the small workload")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB 5 times (HBM counter calibration)")
     return ap.parse_args()
@@ -383,6 +386,109 @@ def batch_study(args):
     print(json.dumps(line), flush=True)
 
 
+def _oracle_flatten(job):
+    from oracle import param_cpu as oracle
+
+    P, T, iters = job
+    return oracle.flatten(P, T, iters=iters)
+
+
+def small_study(args):
+    """The reference's most frequent call: charts.flatten of small charts
+    (40-90 faces) from a pool of 8 threads (pipeline.py:608-611,
+    charts.py:398-438).  Times, on the same charts:
+      * param.lscm + param.refine(8 iterations) per chart, called from 8
+        threads (one plan per call, as the reference's threads call it);
+      * FlattenQueue: the 8 threads call flatten through one queue that
+        gathers their calls into batched solves;
+      * flatten_many of all charts in one call;
+      * the float64 oracle's flatten on one core and over 8 worker processes.
+    Reports charts per second and the per-call latency of each, and the
+    largest layout difference of the batched path to the oracle."""
+    import multiprocessing as mp
+    import threading
+
+    n_threads = 8
+    pool = mp.get_context("fork").Pool(n_threads, initializer=_one_thread)
+    import torch
+
+    from oracle import param_cpu as oracle
+    from ultimate_uv_artist_amd import charts as gch
+    from ultimate_uv_artist_amd import param, workloads
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    rng = np.random.default_rng(9)
+    n = args.charts
+    cs = [workloads.disk_island(rng, int(t)) for t in rng.integers(40, 91, n)]
+    tri = int(sum(len(T) for _, T in cs))
+    iters = 8
+
+    def threaded(fn):
+        lat = [0.0] * n
+        start = threading.Barrier(n_threads)
+
+        def work(k):
+            start.wait()
+            for i in range(k, n, n_threads):
+                t0 = time.perf_counter()
+                fn(*cs[i])
+                lat[i] = time.perf_counter() - t0
+
+        th = [threading.Thread(target=work, args=(k,)) for k in range(n_threads)]
+        t0 = time.perf_counter()
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        return time.perf_counter() - t0, float(np.median(lat)) * 1e3
+
+    def plain_call(P, T):
+        U = param.lscm(P, T)
+        param.refine(P, T, U, iters=iters, tol=1e-4)
+
+    plain_call(*cs[0])                                     # warm-up
+    t_plain, lat_plain = threaded(plain_call)
+    with gch.FlattenQueue(iters=iters, dev=dev) as q:
+        q.flatten(*cs[0])                                  # warm-up
+        t_queue, lat_queue = threaded(lambda P, T: q.flatten(P, T, iters))
+    gch.flatten_many(cs[:8], iters, dev)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = gch.flatten_many(cs, iters, dev)
+    t_many = time.perf_counter() - t0
+    k = min(args.cpu_sample if args.cpu_sample > 0 else n, n)
+    t0 = time.perf_counter()
+    ref = [oracle.flatten(P, T, iters=iters) for P, T in cs[:k]]
+    t_cpu1 = (time.perf_counter() - t0) / k * n
+    t0 = time.perf_counter()
+    pool.map(_oracle_flatten, [(P, T, iters) for P, T in cs], chunksize=16)
+    t_cpu8 = time.perf_counter() - t0
+    pool.close()
+    err = 0.0
+    for (U, _), Ur in zip(res[:k], ref):          # the oracle leaves out the boundary untangle
+        lo = Ur.min(0)
+        err = max(err, float(np.abs(U - Ur).max() / (Ur.max(0) - lo).max()))
+    line = {"workload": f"charts.flatten of {n} small disk charts (40-90 faces, {tri} triangles), {iters} ARAP "
+                        f"iterations, callers on {n_threads} threads",
+            "charts": n, "triangles": tri,
+            "param_calls_8_threads": {"charts_per_s": round(n / t_plain, 1), "median_latency_ms": round(lat_plain, 3)},
+            "flatten_queue_8_threads": {"charts_per_s": round(n / t_queue, 1),
+                                        "median_latency_ms": round(lat_queue, 3)},
+            "flatten_many": {"charts_per_s": round(n / t_many, 1), "ms": round(t_many * 1e3, 1)},
+            "cpu_baseline": {"one_core_charts_per_s": round(n / t_cpu1, 1), "cores": n_threads,
+                             "cores_available": len(os.sched_getaffinity(0)),
+                             "charts_per_s": round(n / t_cpu8, 1), "kind": "port",
+                             "sample": f"oracle flatten (SuperLU) of {k} charts on one core, all {n} over "
+                                       f"{n_threads} worker processes"},
+            "parity": {"charts": k, "max_unit_box_err": err}}
+    print(f"bench: small charts: param calls {line['param_calls_8_threads']}, FlattenQueue "
+          f"{line['flatten_queue_8_threads']}, flatten_many {line['flatten_many']}, CPU "
+          f"{line['cpu_baseline']['one_core_charts_per_s']} charts/s on one core, "
+          f"{line['cpu_baseline']['charts_per_s']} on {n_threads}; max err {err:.1e}", flush=True)
+    print(json.dumps(line), flush=True)
+
+
 def hier_study(args):
     """hier.Solver.relax (defaults: 6 iterations, 20 CG iterations, 24 repair
     rounds) of the 512 disk islands as the charts of one subset of one mesh
@@ -474,6 +580,9 @@ def main():
     if args.workload == "hier":
         hier_study(args)
         return
+    if args.workload == "small":
+        small_study(args)
+        return
     import torch
 
     from ultimate_uv_artist_amd import _lib, workloads
@@ -520,8 +629,19 @@ def main():
     t_lscm = time.perf_counter() - t0
     st_l = plan.stats()
     U0 = orient_majority(U0, plan, plan.vert_off, plan.tri_off, plan.T_local)
+    # LSCM byte model (FP64 Jacobi CG on the coupled 2n operator, vectors
+    # streamed): per island and CG iteration 16 B per stored off-diagonal
+    # entry (value hi + lo, column, boundary coupling) and 216 B per row (SpMV:
+    # diagonal 8, u 16, w 16; update: r, u, w, p, s, e read, r, u, p, s, e
+    # written, 16 each)
+    kl = st_l["cg_iters"].astype(np.float64)
+    lscm_bytes = float(np.sum(kl * (16.0 * (nnz - nv) + 216.0 * nv)))
+    lscm = {"ms": round(t_lscm * 1e3, 2), "cg_iters_mean": round(float(kl.mean()), 1), "cg_iters_max": int(kl.max()),
+            "algorithmic_bytes": round(lscm_bytes), "achieved_GBps": round(lscm_bytes / t_lscm / 1e9, 1),
+            "frac": round(lscm_bytes / t_lscm / 1e9 / HBM_PEAK_GBS, 4)}
     print(f"bench: setup {t_setup:.2f}s, LSCM {t_lscm * 1e3:.1f} ms (CG iters mean {st_l['cg_iters'].mean():.0f}, "
-          f"max {st_l['cg_iters'].max()}, converged {int(st_l['converged'].sum())}/{len(charts)})", flush=True)
+          f"max {st_l['cg_iters'].max()}, converged {int(st_l['converged'].sum())}/{len(charts)}; "
+          f"{lscm_bytes / 1e9:.1f} GB algorithmic = {lscm['achieved_GBps']:.0f} GB/s)", flush=True)
 
     U = torch.empty_like(U0)
     stream = torch.cuda.current_stream(dev)
@@ -532,6 +652,7 @@ def main():
                   precision=prec, use_graph=not args.no_graph)
 
     untuned = None
+    tuning = None
     if args.tune:
         # the default schedule (team sizes and order from a size-based guess
         # of the CG iterations), timed on a few steps for comparison
@@ -550,8 +671,13 @@ def main():
         print(f"bench: default schedule {ms_u:.2f} ms/step ({untuned['value']} Mtri*iter/s)", flush=True)
         # autotune: the last step measured every island's CG iterations, the
         # plan re-plans team sizes and the workgroup schedule from them
+        cost0 = plan.tuning()
         plan.tune()
-        print("bench: schedule re-planned from the measured CG iterations per island", flush=True)
+        tuning = {"default_cost_us": [round(float(c), 4) for c in cost0],
+                  "fitted_cost_us": [round(float(c), 4) for c in plan.tuning()]}
+        print(f"bench: schedule re-planned from the measured CG iterations per island; cost model (us per CG "
+              f"iteration: workgroup, team member, per slice) {tuning['default_cost_us']} -> "
+              f"{tuning['fitted_cost_us']}", flush=True)
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
@@ -720,6 +846,7 @@ def main():
                      "algorithmic_bytes_per_launch": round(bytes_step / args.iters)},
         "cpu_baseline": cpu,
         "untuned": untuned,
+        "tuning": tuning,
         "kernel_src_sha16": kernel_source_digest(),
         "plain_per_island": plain,
         "cg_iters_per_step": {"mean": round(float(kpi.mean()), 2), "p95": round(float(np.percentile(kpi, 95)), 2),
@@ -727,6 +854,7 @@ def main():
         "converged_islands": int(st["converged"].sum()),
         "setup_s": round(t_setup, 3),
         "lscm_ms": round(t_lscm * 1e3, 2),
+        "lscm": lscm,
         "parity": parity,
     }
     print(json.dumps(rec), flush=True)

@@ -155,16 +155,29 @@ int uva_plan_create(uva_plan **out, int64_t n_islands, const int64_t *vert_off, 
  * param.py:103-132 SPDSolver(A). */
 int uva_plan_create_matrix(uva_plan **out, int64_t n, const int64_t *indptr, const int64_t *indices,
                            const double *data, void *stream);
+/* Release a plan: its device memory (taken from the device's stream-ordered
+ * pool) returns to the pool in the order of the stream of the plan's latest
+ * call, without waiting for the device; that stream must still exist. */
 int uva_plan_destroy(uva_plan *plan);
 int uva_plan_get_info(const uva_plan *plan, uva_plan_info *info);
 /* Per-island statistics of the last solve call (host array of n_islands). */
 int uva_plan_get_stats(uva_plan *plan, uva_island_stats *stats, void *stream);
-/* Autotune the schedule: re-plan team sizes and the workgroup schedule from
- * the CG iterations per global step each island took in the last solve call
- * on the plan (the iteration counts of a repeated solve are reproducible).
+/* Autotune the schedule: fit the schedule's cost model to the island times
+ * of the last solve call and re-plan team sizes and the workgroup schedule
+ * from the CG iterations per global step each island took in it (the
+ * iteration counts of a repeated solve are reproducible).
  * Later calls run the new schedule; results are unchanged up to the order of
  * FP64 reductions across team members.  No counterpart in the reference. */
 int uva_plan_tune(uva_plan *plan, void *stream);
+/* The schedule's cost model, microseconds per CG iteration: cost[0] fixed
+ * part of an island on one workgroup, cost[1] fixed part of a team member,
+ * cost[2] per 64-row slice of a member.  uva_plan_tune fits it to the
+ * measured island times of the last solve; uva_plan_set_tuning sets it (a
+ * value measured on this GPU model earlier, say) and re-plans with the
+ * iterations uva_plan_tune last measured, or the size-based guess.  No
+ * counterpart in the reference. */
+int uva_plan_get_tuning(const uva_plan *plan, double *cost);
+int uva_plan_set_tuning(uva_plan *plan, const double *cost, void *stream);
 /* Soft pins of weight w (w = 0: the island's own pin weight, max |W| of the
  * island) at the given vertices (caller ids) in place of each island's own
  * pin, for a plan with Batch conventions whose single island holds several

@@ -816,3 +816,45 @@ def test_gutter_on_gpu_matches_reference_and_oracle():
         gap_r, img_r = vc.min_gutter(U, T, isl, res)
         assert np.array_equal(img, img_r) and gap == gap_r, res
         print(f"gutter: 60k-triangle atlas at {res}: {gap:.3f} texels", flush=True)
+
+
+def test_tuning_fit_cache_and_override(tmp_path, monkeypatch):
+    """uva_plan_tune fits the schedule's cost model to measured island times
+    (positive constants), stores it in UVART_TUNE_CACHE under this GPU model,
+    and a new plan starts from the cached model (or UVART_TUNE_COST); the
+    layouts agree with the default schedule's to CG accuracy."""
+    import json
+
+    from oracle import param_cpu as oracle
+    from ultimate_uv_artist_amd import _lib, workloads
+    from ultimate_uv_artist_amd.plan import IslandPlan
+
+    torch = _torch()
+    rng = np.random.default_rng(58)
+    charts = [workloads.disk_island(rng, t) for t in (26000, 9000, 3000, 600, 300, 14000)]
+    U0 = np.concatenate([oracle.lscm(P, T) for P, T in charts])
+    cache = tmp_path / "tune.json"
+    monkeypatch.setenv("UVART_TUNE_CACHE", str(cache))
+    plan = IslandPlan(charts)
+    assert np.allclose(plan.tuning(), [15.0, 30.0, 0.1])
+    ref = plan.device_uv(U0)
+    plan.arap(ref, 3, mode=_lib.UVA_MODE_REFINE, tol=0.0)
+    plan.tune()
+    fitted = plan.tuning()
+    print(f"tuning: fitted cost model {fitted.tolist()}", flush=True)
+    assert (fitted > 0).all() and np.isfinite(fitted).all()
+    data = json.loads(cache.read_text())
+    assert len(data) == 1 and np.allclose(next(iter(data.values()))["cost_us"], fitted)
+    plan2 = IslandPlan(charts)
+    assert np.allclose(plan2.tuning(), fitted)
+    U = plan2.device_uv(U0)
+    plan2.arap(U, 3, mode=_lib.UVA_MODE_REFINE, tol=0.0)
+    torch.cuda.synchronize()
+    assert plan2.stats()["converged"].all()
+    Ur, Uh = ref.cpu().numpy(), U.cpu().numpy()
+    vo = plan.vert_off
+    for i in range(len(charts)):
+        s = slice(vo[i], vo[i + 1])
+        assert unit_box_err(Uh[s], Ur[s]) <= 1e-5
+    monkeypatch.setenv("UVART_TUNE_COST", "11,22,0.2")
+    assert np.allclose(IslandPlan(charts[:2]).tuning(), [11, 22, 0.2])

@@ -93,6 +93,8 @@ SIGNATURES = {
     "uva_csr_pcg": (C.c_int, [_vp, _vp, _vp, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_int32,
                               C.POINTER(C.c_int32), _vp]),
     "uva_plan_tune": (C.c_int, [_vp, _vp]),
+    "uva_plan_get_tuning": (C.c_int, [_vp, _dp]),
+    "uva_plan_set_tuning": (C.c_int, [_vp, _dp, _vp]),
     "uva_lscm": (C.c_int, [_vp, _vp, _i64p, C.c_double, C.c_int32, _vp]),
     "uva_arap": (C.c_int, [_vp, _vp, C.POINTER(ArapOpts), _vp]),
     "uva_arap_profile": (C.c_int, [_vp, _vp, C.POINTER(ArapOpts), _dp, C.c_int32, _vp]),

@@ -56,6 +56,7 @@ struct uva_plan {
     uva::HostPlan sched;                   // island sizes and schedule inputs (for uva_plan_tune)
     int team_alloc = 0;                    // teams the barrier arrays hold
     std::vector<int> members;              // per island: workgroups of its team (0 = one wavefront)
+    std::vector<double> est;               // CG iterations per global step measured by uva_plan_tune
     double2* ubuf[3] = {nullptr, nullptr, nullptr};  // plan-order copies of the caller's vertex arrays
     int* d_lpin0 = nullptr;
     int* d_lpin1 = nullptr;
@@ -65,19 +66,29 @@ struct uva_plan {
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
     std::vector<char> gkey;
+    // Device memory comes from the device's stream-ordered pool, allocated
+    // and released in the order of the stream of the plan's latest call:
+    // creating and destroying plans (one per chart for the reference's
+    // per-chart calls) then never waits for the whole device.
+    hipStream_t mem = nullptr;
 
+    // (mem is always a caller's stream, never the private capture stream:
+    // the public entry points set it, the helpers they call do not.)
     ~uva_plan() {
-        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (gexec) {
+            (void)hipStreamSynchronize(mem);      // a replay may still run
+            (void)hipGraphExecDestroy(gexec);
+        }
         if (graph) (void)hipGraphDestroy(graph);
+        for (void* p : allocs) (void)hipFreeAsync(p, mem);
         if (cap) (void)hipStreamDestroy(cap);
-        for (void* p : allocs) (void)hipFree(p);
     }
 
     template <class T>
     hipError_t alloc(T** out, size_t n) {
         void* p = nullptr;
         size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
-        hipError_t e = hipMalloc(&p, bytes);
+        hipError_t e = hipMallocAsync(&p, bytes, mem);
         if (e != hipSuccess) return e;
         allocs.push_back(p);
         device_bytes += (int64_t)bytes;
@@ -237,6 +248,23 @@ XccInfo xcc_info() {
     return xi;
 }
 
+// The current device's default memory pool keeps what plans release for
+// reuse instead of returning it at every synchronisation (once per device).
+void keep_pool_memory() {
+    static std::mutex mu;
+    static bool done[64] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return;
+    std::lock_guard<std::mutex> lk(mu);
+    if (done[dev]) return;
+    hipMemPool_t pool = nullptr;
+    if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess && pool) {
+        uint64_t keep = UINT64_MAX;
+        (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+    }
+    done[dev] = true;
+}
+
 // Upload a host plan and allocate the scratch of a new device plan.
 // Per island: workgroups of its team in the schedule (0 = a wavefront task).
 std::vector<int> island_members(const uva::HostPlan& hp) {
@@ -249,6 +277,8 @@ int upload_plan(uva::HostPlan& hp, const double* P, hipStream_t s, uva_plan** ou
     std::unique_ptr<uva_plan> hold(new (std::nothrow) uva_plan());
     uva_plan* pl = hold.get();
     if (!pl) return fail(UVA_ENOMEM, "plan allocation failed");
+    pl->mem = s;
+    keep_pool_memory();
     auto bail = [&](int code, const std::string& m) { return fail(code, m); };
 #define UP(field, vec)                                                                           \
     do {                                                                                         \
@@ -462,8 +492,7 @@ int uva_plan_create_matrix(uva_plan** out, int64_t n, const int64_t* indptr, con
 
 int uva_plan_destroy(uva_plan* plan) {
     if (!plan) return UVA_OK;
-    (void)hipDeviceSynchronize();
-    delete plan;
+    delete plan;   // memory released in the order of the plan's latest stream
     return UVA_OK;
 }
 
@@ -482,6 +511,7 @@ int uva_plan_get_info(const uva_plan* plan, uva_plan_info* info) {
 
 int uva_plan_get_stats(uva_plan* plan, uva_island_stats* stats, void* stream) {
     if (!valid_plan(plan) || !stats) return fail(UVA_EINVAL, "null argument");
+    plan->mem = (hipStream_t)stream;   // allocations and the final release follow this stream
     hipStream_t s = (hipStream_t)stream;
     std::vector<uva::IslandState> st(plan->dp.n_isl);
     int fault = 0;
@@ -507,23 +537,19 @@ int uva_plan_get_stats(uva_plan* plan, uva_island_stats* stats, void* stream) {
     return UVA_OK;
 }
 
-int uva_plan_tune(uva_plan* plan, void* stream) {
-    if (!valid_plan(plan)) return fail(UVA_EINVAL, "null argument");
-    hipStream_t s = (hipStream_t)stream;
+}  // extern "C"
+
+namespace {
+
+// Re-plan team sizes and the workgroup schedule from per-island CG
+// iterations per global step (est, or NULL for the size-based guess) with the
+// plan's cost model, and upload the new task arrays.
+int replan(uva_plan* plan, const double* est, hipStream_t s) {
     uva::DevPlan& d = plan->dp;
-    std::vector<uva::IslandState> st(d.n_isl);
-    HIP_TRY(hipMemcpyAsync(st.data(), d.st, st.size() * sizeof(uva::IslandState), hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
-    std::vector<double> est(d.n_isl);
-    bool any = false;
-    for (int i = 0; i < d.n_isl; ++i) {
-        est[i] = (double)st[i].cg_iters / std::max(1, st[i].arap_iters);
-        any = any || st[i].cg_iters > 0;
-    }
-    if (!any) return fail(UVA_EINVAL, "uva_plan_tune needs a finished solve on the plan (no CG iterations recorded)");
+    plan->mem = s;
     uva::HostPlan& sc = plan->sched;
     try {
-        uva::plan_teams(sc, sc.group_isl, sc.sched_slots, sc.n_xcd, sc.sched_teams, est.data());
+        uva::plan_teams(sc, sc.group_isl, sc.sched_slots, sc.n_xcd, sc.sched_teams, est);
     } catch (const std::bad_alloc&) {
         return fail(UVA_ENOMEM, "host allocation failed while re-planning");
     }
@@ -556,8 +582,102 @@ int uva_plan_tune(uva_plan* plan, void* stream) {
     return UVA_OK;
 }
 
+// Weighted least squares fit of the schedule's cost model to the measured
+// island times: microseconds per CG iteration = c0 (one workgroup) or c1
+// (team member) + c2 x slices per member, weighted by each island's CG
+// iterations.  A class without islands keeps its constant; a fit that is not
+// positive keeps the old model.
+void fit_costs(const std::vector<uva::IslandState>& st, const std::vector<int>& members,
+               const std::vector<int>& ns, int clock_khz, double cost[3]) {
+    if (clock_khz <= 0) return;
+    double A[3][3] = {{0}}, r[3] = {0};
+    bool have[2] = {false, false};
+    for (size_t i = 0; i < st.size(); ++i) {
+        const int m = i < members.size() ? members[i] : 0;
+        if (m < 1 || st[i].cg_iters <= 0 || st[i].ticks <= 0) continue;
+        const double w = st[i].cg_iters;
+        const double y = (double)st[i].ticks / clock_khz * 1e3 / st[i].cg_iters;
+        const double x[3] = {m == 1 ? 1.0 : 0.0, m > 1 ? 1.0 : 0.0, (double)((ns[i] + m - 1) / m)};
+        have[m > 1 ? 1 : 0] = true;
+        for (int a = 0; a < 3; ++a) {
+            r[a] += w * x[a] * y;
+            for (int b = 0; b < 3; ++b) A[a][b] += w * x[a] * x[b];
+        }
+    }
+    // classes without data: their constant moves to the right-hand side
+    double c[3] = {cost[0], cost[1], cost[2]};
+    int act[3], na = 0;
+    for (int k = 0; k < 2; ++k)
+        if (have[k]) act[na++] = k;
+    if (na == 0) return;
+    act[na++] = 2;
+    double M[3][4];
+    for (int a = 0; a < na; ++a) {
+        double rhs = r[act[a]];
+        for (int k = 0; k < 2; ++k)
+            if (!have[k]) rhs -= A[act[a]][k] * c[k];
+        for (int b = 0; b < na; ++b) M[a][b] = A[act[a]][act[b]];
+        M[a][na] = rhs;
+    }
+    for (int k = 0; k < na; ++k) {             // Gaussian elimination with partial pivoting
+        int piv = k;
+        for (int a = k + 1; a < na; ++a)
+            if (std::fabs(M[a][k]) > std::fabs(M[piv][k])) piv = a;
+        if (std::fabs(M[piv][k]) < 1e-12) return;
+        for (int b = 0; b <= na; ++b) std::swap(M[k][b], M[piv][b]);
+        for (int a = 0; a < na; ++a) {
+            if (a == k) continue;
+            const double f = M[a][k] / M[k][k];
+            for (int b = k; b <= na; ++b) M[a][b] -= f * M[k][b];
+        }
+    }
+    double out[3] = {c[0], c[1], c[2]};
+    for (int a = 0; a < na; ++a) out[act[a]] = M[a][na] / M[a][a];
+    for (int k = 0; k < 3; ++k)
+        if (!(out[k] > 0.0) || !std::isfinite(out[k])) return;
+    for (int k = 0; k < 3; ++k) cost[k] = out[k];
+}
+
+}  // namespace
+
+extern "C" {
+
+int uva_plan_tune(uva_plan* plan, void* stream) {
+    if (!valid_plan(plan)) return fail(UVA_EINVAL, "null argument");
+    hipStream_t s = (hipStream_t)stream;
+    uva::DevPlan& d = plan->dp;
+    std::vector<uva::IslandState> st(d.n_isl);
+    HIP_TRY(hipMemcpyAsync(st.data(), d.st, st.size() * sizeof(uva::IslandState), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    std::vector<double> est(d.n_isl);
+    bool any = false;
+    for (int i = 0; i < d.n_isl; ++i) {
+        est[i] = (double)st[i].cg_iters / std::max(1, st[i].arap_iters);
+        any = any || st[i].cg_iters > 0;
+    }
+    if (!any) return fail(UVA_EINVAL, "uva_plan_tune needs a finished solve on the plan (no CG iterations recorded)");
+    fit_costs(st, plan->members, plan->sched.isl_ns, plan->clock_khz, plan->sched.cost);
+    plan->est.assign(est.begin(), est.end());
+    return replan(plan, est.data(), s);
+}
+
+int uva_plan_get_tuning(const uva_plan* plan, double* cost) {
+    if (!valid_plan(plan) || !cost) return fail(UVA_EINVAL, "null argument");
+    for (int k = 0; k < 3; ++k) cost[k] = plan->sched.cost[k];
+    return UVA_OK;
+}
+
+int uva_plan_set_tuning(uva_plan* plan, const double* cost, void* stream) {
+    if (!valid_plan(plan) || !cost) return fail(UVA_EINVAL, "null argument");
+    for (int k = 0; k < 3; ++k)
+        if (!(cost[k] > 0.0) || !std::isfinite(cost[k])) return fail(UVA_EINVAL, "costs must be positive and finite");
+    for (int k = 0; k < 3; ++k) plan->sched.cost[k] = cost[k];
+    return replan(plan, plan->est.empty() ? nullptr : plan->est.data(), (hipStream_t)stream);
+}
+
 int uva_plan_set_scale(uva_plan* plan, const double* scale_dev, void* stream) {
     if (!valid_plan(plan) || !scale_dev) return fail(UVA_EINVAL, "null argument");
+    plan->mem = (hipStream_t)stream;   // allocations and the final release follow this stream
     if (plan->dp.n_tri)
         HIP_TRY(hipMemcpyAsync(plan->dp.fscale, scale_dev, plan->dp.n_tri * sizeof(double), hipMemcpyDeviceToDevice,
                                (hipStream_t)stream));
@@ -571,6 +691,7 @@ int uva_plan_set_pins(uva_plan* plan, const int64_t* pins, int64_t n_pins, doubl
     if (!(w >= 0.0) || !std::isfinite(w)) return fail(UVA_EINVAL, "pin weight must be >= 0 and finite");
     if (n_pins < 0) return fail(UVA_EINVAL, "n_pins must be >= 0");
     hipStream_t s = (hipStream_t)stream;
+    plan->mem = s;
     const int n = d.n_vert;
     // w = 0: each pin takes the soft-pin weight of its island
     std::vector<double> wisl((size_t)std::max(d.n_isl, 1), w);
@@ -615,6 +736,7 @@ int uva_plan_set_pins(uva_plan* plan, const int64_t* pins, int64_t n_pins, doubl
 
 int uva_lscm(uva_plan* plan, double* U, const int64_t* pins, double tol, int32_t max_iters, void* stream) {
     if (!valid_plan(plan) || !U) return fail(UVA_EINVAL, "null argument");
+    plan->mem = (hipStream_t)stream;   // allocations and the final release follow this stream
     if (!(tol > 0.0)) return fail(UVA_EINVAL, "tol must be positive");
     hipStream_t s = (hipStream_t)stream;
     uva::DevPlan& d = plan->dp;
@@ -653,6 +775,7 @@ int uva_lscm(uva_plan* plan, double* U, const int64_t* pins, double tol, int32_t
 
 int uva_arap(uva_plan* plan, double* U, const uva_arap_opts* o, void* stream) {
     if (!valid_plan(plan) || !U || !o) return fail(UVA_EINVAL, "null argument");
+    plan->mem = (hipStream_t)stream;   // allocations and the final release follow this stream
     if (o->iters < 0 || o->iters_folded < 0) return fail(UVA_EINVAL, "iters must be >= 0");
     if (!(o->cg_tol >= 0.0) || (o->cg_tol == 0.0 && o->cg_max_iters <= 0))
         return fail(UVA_EINVAL, "cg_tol must be positive, or 0 with cg_max_iters > 0");
@@ -686,6 +809,7 @@ int uva_arap(uva_plan* plan, double* U, const uva_arap_opts* o, void* stream) {
 int uva_arap_profile(uva_plan* plan, double* U, const uva_arap_opts* o, double* launch_ms, int32_t n_launch,
                      void* stream) {
     if (!valid_plan(plan) || !U || !o || !launch_ms) return fail(UVA_EINVAL, "null argument");
+    plan->mem = (hipStream_t)stream;   // allocations and the final release follow this stream
     if (o->iters < 0 || o->iters_folded < 0) return fail(UVA_EINVAL, "iters must be >= 0");
     if (n_launch != 1 + std::max(o->iters, o->iters_folded))
         return fail(UVA_EINVAL, "n_launch must be 1 + max(iters, iters_folded)");
@@ -714,12 +838,14 @@ int uva_arap_profile(uva_plan* plan, double* U, const uva_arap_opts* o, double* 
 
 int uva_slim(uva_plan* plan, double* U, const uva_slim_opts* o, void* stream) {
     if (!valid_plan(plan) || !U || !o) return fail(UVA_EINVAL, "null argument");
+    plan->mem = (hipStream_t)stream;   // allocations and the final release follow this stream
     if (o->iters < 0) return fail(UVA_EINVAL, "iters must be >= 0");
     if (!(o->cg_tol > 0.0)) return fail(UVA_EINVAL, "cg_tol must be positive");
     if (plan->dp.mode_batch) return fail(UVA_EINVAL, "SLIM needs a plan built with refine conventions");
     if (plan->dp.n_tri == 0) return fail(UVA_EINVAL, "SLIM needs triangles");
     hipStream_t s = (hipStream_t)stream;
     uva::DevPlan& d = plan->dp;
+    plan->mem = s;
     if (!d.sw) {
         if (plan->alloc(&d.sw, d.n_tri) != hipSuccess || plan->alloc(&d.sdi, d.n_vert) != hipSuccess)
             return fail(UVA_ENOMEM, "SLIM scratch");
@@ -742,6 +868,7 @@ int uva_slim(uva_plan* plan, double* U, const uva_slim_opts* o, void* stream) {
 
 int uva_local_rhs(uva_plan* plan, const double* U, double* b, int32_t mode, void* stream) {
     if (!valid_plan(plan) || !U || !b) return fail(UVA_EINVAL, "null argument");
+    plan->mem = (hipStream_t)stream;   // allocations and the final release follow this stream
     hipStream_t s = (hipStream_t)stream;
     int rc = UVA_OK;
     uva::TaskArgs a = base_args((double*)to_plan(plan, U, 0, s, rc));
@@ -755,6 +882,7 @@ int uva_local_rhs(uva_plan* plan, const double* U, double* b, int32_t mode, void
 int uva_cg_solve(uva_plan* plan, const double* b, double* x, double tol, int32_t max_iters, int32_t precision,
                  void* stream) {
     if (!valid_plan(plan) || !b || !x) return fail(UVA_EINVAL, "null argument");
+    plan->mem = (hipStream_t)stream;   // allocations and the final release follow this stream
     if (!(tol >= 0.0) || (tol == 0.0 && max_iters <= 0))
         return fail(UVA_EINVAL, "tol must be positive, or 0 with max_iters > 0");
     if (precision < UVA_PREC_MIXED || precision > UVA_PREC_F64) return fail(UVA_EINVAL, "bad precision");
@@ -771,6 +899,7 @@ int uva_cg_solve(uva_plan* plan, const double* b, double* x, double tol, int32_t
 
 int uva_max_step(uva_plan* plan, const double* U, const double* D, double* tmax, int32_t mode, void* stream) {
     if (!valid_plan(plan) || !U || !D || !tmax) return fail(UVA_EINVAL, "null argument");
+    plan->mem = (hipStream_t)stream;   // allocations and the final release follow this stream
     hipStream_t s = (hipStream_t)stream;
     int rc = UVA_OK;
     uva::TaskArgs a = base_args((double*)to_plan(plan, U, 0, s, rc));
@@ -785,6 +914,7 @@ int uva_max_step(uva_plan* plan, const double* U, const double* D, double* tmax,
 
 int uva_energy(uva_plan* plan, const double* U, int32_t kind, double* e, void* stream) {
     if (!valid_plan(plan) || !U || !e) return fail(UVA_EINVAL, "null argument");
+    plan->mem = (hipStream_t)stream;   // allocations and the final release follow this stream
     if (kind != 0 && kind != 1) return fail(UVA_EINVAL, "kind must be 0 (ARAP) or 1 (symmetric Dirichlet)");
     hipStream_t s = (hipStream_t)stream;
     int rc = UVA_OK;
@@ -798,6 +928,7 @@ int uva_energy(uva_plan* plan, const double* U, int32_t kind, double* e, void* s
 
 int uva_energy_fwd(uva_plan* plan, const double* U, const double* w, int32_t kind, double* e, void* stream) {
     if (!valid_plan(plan) || !U || !e) return fail(UVA_EINVAL, "null argument");
+    plan->mem = (hipStream_t)stream;   // allocations and the final release follow this stream
     if (kind != 0 && kind != 1) return fail(UVA_EINVAL, "kind must be 0 (ARAP) or 1 (symmetric Dirichlet)");
     if (plan->dp.mode_batch) return fail(UVA_EINVAL, "the energy gradients need a plan built with refine conventions");
     hipStream_t s = (hipStream_t)stream;
@@ -814,6 +945,7 @@ int uva_energy_fwd(uva_plan* plan, const double* U, const double* w, int32_t kin
 int uva_energy_bwd(uva_plan* plan, const double* U, const double* w, int32_t kind, const double* ge, double* gU,
                    double* gw, void* stream) {
     if (!valid_plan(plan) || !U || !ge || !gU) return fail(UVA_EINVAL, "null argument");
+    plan->mem = (hipStream_t)stream;   // allocations and the final release follow this stream
     if (kind != 0 && kind != 1) return fail(UVA_EINVAL, "kind must be 0 (ARAP) or 1 (symmetric Dirichlet)");
     if (plan->dp.mode_batch) return fail(UVA_EINVAL, "the energy gradients need a plan built with refine conventions");
     hipStream_t s = (hipStream_t)stream;
@@ -832,6 +964,7 @@ int uva_energy_bwd(uva_plan* plan, const double* U, const double* w, int32_t kin
 int uva_adjoint_cg(uva_plan* plan, const double* g, double* lam, double tol, int32_t max_iters, int32_t precision,
                    void* stream) {
     if (!valid_plan(plan) || !g || !lam) return fail(UVA_EINVAL, "null argument");
+    plan->mem = (hipStream_t)stream;   // allocations and the final release follow this stream
     if (!(tol > 0.0)) return fail(UVA_EINVAL, "tol must be positive");
     if (precision < UVA_PREC_MIXED || precision > UVA_PREC_F64) return fail(UVA_EINVAL, "bad precision");
     hipStream_t s = (hipStream_t)stream;
@@ -848,6 +981,7 @@ int uva_adjoint_cg(uva_plan* plan, const double* g, double* lam, double tol, int
 
 int uva_arap_step_bwd(uva_plan* plan, const double* U, const double* lam, double* gU, void* stream) {
     if (!valid_plan(plan) || !U || !lam || !gU) return fail(UVA_EINVAL, "null argument");
+    plan->mem = (hipStream_t)stream;   // allocations and the final release follow this stream
     if (plan->dp.mode_batch) return fail(UVA_EINVAL, "the step backward needs a plan built with refine conventions");
     hipStream_t s = (hipStream_t)stream;
     int rc = UVA_OK;
@@ -861,6 +995,7 @@ int uva_arap_step_bwd(uva_plan* plan, const double* U, const double* lam, double
 
 int uva_signed_areas(uva_plan* plan, const double* U, double* sa, void* stream) {
     if (!valid_plan(plan) || !U || !sa) return fail(UVA_EINVAL, "null argument");
+    plan->mem = (hipStream_t)stream;   // allocations and the final release follow this stream
     int rc = UVA_OK;
     const double2* Up = to_plan(plan, U, 0, (hipStream_t)stream, rc);
     if (rc) return rc;
@@ -870,6 +1005,7 @@ int uva_signed_areas(uva_plan* plan, const double* U, double* sa, void* stream) 
 
 int uva_flips_per_chart(uva_plan* plan, const double* U, double* counts, void* stream) {
     if (!valid_plan(plan) || !U || !counts) return fail(UVA_EINVAL, "null argument");
+    plan->mem = (hipStream_t)stream;   // allocations and the final release follow this stream
     hipStream_t s = (hipStream_t)stream;
     int rc = UVA_OK;
     uva::TaskArgs a = base_args((double*)to_plan(plan, U, 0, s, rc));
@@ -881,6 +1017,7 @@ int uva_flips_per_chart(uva_plan* plan, const double* U, double* counts, void* s
 
 int uva_repair_jacobi(uva_plan* plan, double* U, int32_t rounds, void* stream) {
     if (!valid_plan(plan) || !U) return fail(UVA_EINVAL, "null argument");
+    plan->mem = (hipStream_t)stream;   // allocations and the final release follow this stream
     if (rounds < 0) return fail(UVA_EINVAL, "rounds must be >= 0");
     hipStream_t s = (hipStream_t)stream;
     int rc = UVA_OK;
@@ -893,6 +1030,7 @@ int uva_repair_jacobi(uva_plan* plan, double* U, int32_t rounds, void* stream) {
 
 int uva_orient_positive(uva_plan* plan, double* U, void* stream) {
     if (!valid_plan(plan) || !U) return fail(UVA_EINVAL, "null argument");
+    plan->mem = (hipStream_t)stream;   // allocations and the final release follow this stream
     hipStream_t s = (hipStream_t)stream;
     int rc = UVA_OK;
     uva::TaskArgs a = base_args((double*)to_plan(plan, U, 0, s, rc));

@@ -581,7 +581,7 @@ void plan_teams(HostPlan& hp, const std::vector<int>& group_islands, int slots, 
     };
     auto t_of = [&](int i, int m) {
         const int per = (hp.isl_ns[i] + m - 1) / m;
-        return iters(i) * ((m > 1 ? 30.0 : 15.0) + 0.1 * per);
+        return iters(i) * ((m > 1 ? hp.cost[1] : hp.cost[0]) + hp.cost[2] * per);
     };
     const int n = (int)group_islands.size();
     std::vector<int> m(n, 1);

@@ -44,6 +44,11 @@ struct HostPlan {
     std::vector<int> group_isl;
     int sched_slots = 1;
     bool sched_teams = false;
+    // cost model of the schedule, microseconds per CG iteration: fixed part
+    // of a one-workgroup island, fixed part of a team member, and per 64-row
+    // slice of a member (the defaults until uva_plan_tune fits them to
+    // measured island times, or a caller sets them)
+    double cost[3] = {15.0, 30.0, 0.1};
     int64_t nnz_true = 0;
     // plan vertex -> caller's global vertex id (islands keep their offsets;
     // inside an island the plan orders vertices by Morton code)

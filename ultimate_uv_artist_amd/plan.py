@@ -31,6 +31,63 @@ def require_gpu(dev=None):
     return d
 
 
+def _tuning_key(dev):
+    """Cache key of a GPU model: its architecture name and compute units."""
+    pr = torch.cuda.get_device_properties(dev)
+    return f"{getattr(pr, 'gcnArchName', pr.name)}/{pr.multi_processor_count}cu"
+
+
+def tuning_for(dev):
+    """Schedule cost model to start a new plan with (microseconds per CG
+    iteration: one-workgroup fixed part, team-member fixed part, per slice):
+    UVART_TUNE_COST="a,b,c" when set, else the entry for this GPU model in the
+    JSON file UVART_TUNE_CACHE names (written by IslandPlan.tune), else None
+    (the library's defaults)."""
+    import json
+    import os
+
+    env = os.environ.get("UVART_TUNE_COST")
+    if env:
+        vals = [float(x) for x in env.split(",")]
+        if len(vals) != 3 or not all(v > 0 for v in vals):
+            raise ValueError("UVART_TUNE_COST must be three positive numbers a,b,c")
+        return vals
+    path = os.environ.get("UVART_TUNE_CACHE")
+    if not path:
+        return None
+    try:
+        with open(path) as f:
+            entry = json.load(f).get(_tuning_key(dev))
+    except (OSError, ValueError):
+        return None
+    return list(entry["cost_us"]) if entry else None
+
+
+def save_tuning(dev, cost, info=None):
+    """Store a fitted cost model under this GPU model in UVART_TUNE_CACHE (no
+    file named: nothing is stored).  Returns the path written or None."""
+    import json
+    import os
+    import tempfile
+
+    path = os.environ.get("UVART_TUNE_CACHE")
+    if not path:
+        return None
+    try:
+        with open(path) as f:
+            data = json.load(f)
+    except (OSError, ValueError):
+        data = {}
+    data[_tuning_key(dev)] = {"cost_us": [float(c) for c in cost], **(info or {})}
+    d = os.path.dirname(os.path.abspath(path))
+    os.makedirs(d, exist_ok=True)
+    fd, tmp = tempfile.mkstemp(dir=d, suffix=".tmp")
+    with os.fdopen(fd, "w") as f:
+        json.dump(data, f, indent=1, sort_keys=True)
+    os.replace(tmp, path)
+    return path
+
+
 def stream_of(dev):
     return C.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
 
@@ -241,6 +298,9 @@ class IslandPlan:
                                    stream_of(self.dev))
         check(rc, "uva_plan_create")
         self.T_local = T
+        cost = tuning_for(self.dev)
+        if cost is not None and self.info()["n_group_tasks"] > 0:
+            self.set_tuning(cost)
 
     # ------------------------------------------------------------- lifetime
     def close(self):
@@ -267,9 +327,23 @@ class IslandPlan:
         return {k: np.array([getattr(s, k) for s in arr]) for k in names}
 
     def tune(self):
-        """Re-plan team sizes and the workgroup schedule from the CG iterations
-        each island took in the last solve call (see uva_plan_tune)."""
+        """Fit the schedule's cost model to the island times of the last solve
+        call and re-plan team sizes and the workgroup schedule from the CG
+        iterations each island took in it (see uva_plan_tune); the fitted
+        model is stored in UVART_TUNE_CACHE when that names a file."""
         check(_lib.lib().uva_plan_tune(self._h, stream_of(self.dev)), "uva_plan_tune")
+        save_tuning(self.dev, self.tuning(), {"islands": self.n_islands, "triangles": self.t})
+
+    def tuning(self):
+        """The schedule's cost model (see uva_plan_get_tuning)."""
+        out = np.zeros(3, np.float64)
+        check(_lib.lib().uva_plan_get_tuning(self._h, f64p(out)), "uva_plan_get_tuning")
+        return out
+
+    def set_tuning(self, cost):
+        """Set the schedule's cost model and re-plan (see uva_plan_set_tuning)."""
+        c = np.ascontiguousarray(np.asarray(cost, np.float64).reshape(3))
+        check(_lib.lib().uva_plan_set_tuning(self._h, f64p(c), stream_of(self.dev)), "uva_plan_set_tuning")
 
     # ------------------------------------------------------------- helpers
     def device_uv(self, U):
