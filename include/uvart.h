@@ -96,6 +96,10 @@ typedef struct {
     int64_t device_bytes;    /* device memory held by the plan */
     int32_t n_group_tasks;   /* islands solved by a whole workgroup */
     int32_t n_wave_tasks;    /* islands solved by a single wavefront */
+    int32_t n_teams;         /* teams of two or more workgroups (their members
+                                wait on each other: such a plan's solves need
+                                its workgroups resident together) */
+    int32_t reserved;
 } uva_plan_info;
 
 typedef struct {

@@ -134,12 +134,22 @@ def test_batch_shrink_and_scale():
 
 def test_param_from_eight_threads():
     """param.lscm + param.refine on independent charts from eight threads at
-    once, each against the oracle (unit box 1e-4 for refine, 1e-6 for LSCM)."""
+    once, each against the oracle (unit box 1e-4 for refine; LSCM 1e-6, and
+    1e-5 on the two charts of 45k-60k triangles: the FP64 CG stops at relative
+    residual 1e-8, and the error that leaves grows with the system's condition
+    number, i.e. with its size; the oracle solves directly).  Each thread runs on a
+    stream of its own; the two large charts get teams of several workgroups,
+    whose solves take turns."""
     from oracle import param_cpu as oracle
     from ultimate_uv_artist_amd import param, workloads
+    from ultimate_uv_artist_amd.plan import IslandPlan
 
     rng = np.random.default_rng(77)
-    charts = [workloads.disk_island(rng, int(t)) for t in rng.integers(60, 4000, 16)]
+    sizes = list(rng.integers(60, 4000, 14)) + [60000, 45000]
+    charts = [workloads.disk_island(rng, int(t)) for t in sizes]
+    teams = [IslandPlan([c], pins=[0]).info()["n_teams"] for c in charts[-2:]]
+    print("teams of the two large charts:", teams, flush=True)
+    assert max(teams) > 0
     refs = []
     for P, T in charts:
         U0 = oracle.lscm(P, T)
@@ -165,8 +175,8 @@ def test_param_from_eight_threads():
     for t in th:
         t.join()
     assert not errors, errors
-    for (U0, U1), (R0, R1) in zip(out, refs):
-        for got, ref, tol in ((U0, R0, 1e-6), (U1, R1, 1e-4)):
+    for (U0, U1), (R0, R1), (_, T) in zip(out, refs, charts):
+        for got, ref, tol in ((U0, R0, 1e-6 if len(T) < 20000 else 1e-5), (U1, R1, 1e-4)):
             lo = ref.min(0)
             sc = float((ref.max(0) - lo).max())
             assert np.abs((got - lo) / sc - (ref - lo) / sc).max() <= tol

@@ -46,7 +46,8 @@ class UvaError(RuntimeError):
 class PlanInfo(C.Structure):
     _fields_ = [("n_islands", C.c_int64), ("n_vertices", C.c_int64), ("n_triangles", C.c_int64),
                 ("nnz", C.c_int64), ("nnz_true", C.c_int64), ("device_bytes", C.c_int64),
-                ("n_group_tasks", C.c_int32), ("n_wave_tasks", C.c_int32)]
+                ("n_group_tasks", C.c_int32), ("n_wave_tasks", C.c_int32), ("n_teams", C.c_int32),
+                ("reserved", C.c_int32)]
 
 
 class IslandStats(C.Structure):

@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <mutex>
 #include <memory>
@@ -273,6 +274,69 @@ std::vector<int> island_members(const uva::HostPlan& hp) {
     return m;
 }
 
+// Every array a plan build creates is a slice of one device block: one pool
+// allocation and one or a few host-to-device copies per plan instead of one
+// of each per array (the reference's per-chart calls build and drop a plan
+// for every chart, so these calls are the cost of a small chart).
+// Layout: uploaded arrays, then zero-filled arrays, then scratch whose
+// contents the kernels define; slices start at 256-byte boundaries.
+class Arena {
+  public:
+    template <class T, class P>
+    void put(P** out, const T* src, size_t n) {          // upload n values of src
+        static_assert(sizeof(T) % sizeof(P) == 0 || sizeof(P) % sizeof(T) == 0, "slice type");
+        up_items_.push_back({up_, slice(n * sizeof(T)), src, n * sizeof(T), [out](char* b) { *out = (P*)b; }});
+        up_ += up_items_.back().bytes;
+    }
+    template <class T>
+    void put(const T** out, const std::vector<T>& v) { put<T, const T>(out, v.data(), v.size()); }
+    template <class T>
+    void zero(T** out, size_t n) { zero_items_.push_back({0, slice(n * sizeof(T)), nullptr, 0, setter(out)}); }
+    template <class T>
+    void raw(T** out, size_t n) { raw_items_.push_back({0, slice(n * sizeof(T)), nullptr, 0, setter(out)}); }
+
+    // One allocation from the plan's pool, the copies and the zero fill, all
+    // on stream s; the host sources must live until s has passed the copies.
+    hipError_t commit(uva_plan* pl, hipStream_t s) {
+        size_t off = up_;
+        for (Item& it : zero_items_) { it.off = off; off += it.bytes; }
+        const size_t zero_end = off;
+        for (Item& it : raw_items_) { it.off = off; off += it.bytes; }
+        char* base = nullptr;
+        hipError_t e = pl->alloc(&base, off);
+        if (e != hipSuccess) return e;
+        if (up_ > 0 && up_ <= kImageBytes) {              // small plans: one copy of a packed image
+            image_.assign(up_, 0);
+            for (const Item& it : up_items_)
+                if (it.len) std::memcpy(image_.data() + it.off, it.src, it.len);
+            e = hipMemcpyAsync(base, image_.data(), up_, hipMemcpyHostToDevice, s);
+        } else {                                          // large plans: straight from the host arrays
+            for (const Item& it : up_items_)
+                if (e == hipSuccess && it.len)
+                    e = hipMemcpyAsync(base + it.off, it.src, it.len, hipMemcpyHostToDevice, s);
+        }
+        if (e == hipSuccess && zero_end > up_) e = hipMemsetAsync(base + up_, 0, zero_end - up_, s);
+        for (const std::vector<Item>* v : {&up_items_, &zero_items_, &raw_items_})
+            for (const Item& it : *v) it.set(base + it.off);
+        return e;
+    }
+
+  private:
+    static constexpr size_t kImageBytes = size_t(8) << 20;
+    struct Item {
+        size_t off, bytes;                                // slice
+        const void* src;                                  // uploads: host source of len bytes
+        size_t len;
+        std::function<void(char*)> set;                   // stores the slice's address
+    };
+    static size_t slice(size_t b) { return (std::max<size_t>(b, 1) + 255) & ~size_t(255); }
+    template <class T>
+    static std::function<void(char*)> setter(T** out) { return [out](char* b) { *out = (T*)b; }; }
+    size_t up_ = 0;
+    std::vector<Item> up_items_, zero_items_, raw_items_;
+    std::vector<char> image_;
+};
+
 int upload_plan(uva::HostPlan& hp, const double* P, hipStream_t s, uva_plan** out) {
     std::unique_ptr<uva_plan> hold(new (std::nothrow) uva_plan());
     uva_plan* pl = hold.get();
@@ -280,17 +344,13 @@ int upload_plan(uva::HostPlan& hp, const double* P, hipStream_t s, uva_plan** ou
     pl->mem = s;
     keep_pool_memory();
     auto bail = [&](int code, const std::string& m) { return fail(code, m); };
-#define UP(field, vec)                                                                           \
-    do {                                                                                         \
-        hipError_t _e = pl->upload(&pl->dp.field, hp.vec, s);                                    \
-        if (_e != hipSuccess) return bail(_e == hipErrorOutOfMemory ? UVA_ENOMEM : UVA_EHIP,     \
-                                          std::string("upload " #vec ": ") + hipGetErrorString(_e)); \
-    } while (0)
     uva::DevPlan& d = pl->dp;
     d.n_isl = hp.n_isl;
     d.n_vert = hp.n_vert;
     d.n_tri = hp.n_tri;
     d.mode_batch = hp.batch;
+    Arena ar;
+#define UP(field, vec) ar.put(&d.field, hp.vec)
     UP(isl_v0, isl_v0); UP(isl_nv, isl_nv); UP(isl_t0, isl_t0); UP(isl_nt, isl_nt);
     UP(isl_s0, isl_s0); UP(isl_ns, isl_ns); UP(isl_pin, isl_pin); UP(isl_wpin, isl_wpin); UP(isl_bw, isl_bw);
     UP(tv0, tv0); UP(tv1, tv1); UP(tv2, tv2);
@@ -303,51 +363,51 @@ int upload_plan(uva::HostPlan& hp, const double* P, hipStream_t s, uva_plan** ou
     UP(grp_mem, grp_mem); UP(grp_m, grp_m); UP(grp_team, grp_team); UP(slot_ptr, slot_ptr);
     UP(vperm, vperm); UP(isl_cofs, isl_cofs); UP(isl_cg, isl_cg); UP(cinv, cinv);
 #undef UP
+    ar.put<double, const double2>(&d.inc_g, hp.inc_g.data(), hp.inc_g.size());
+    ar.put<double, const double2>(&d.tri_g, hp.tri_g.data(), hp.tri_g.size());
+    // LSCM pins (mutable through uva_lscm)
+    ar.put<int, int>(&pl->d_lpin0, hp.isl_lpin0.data(), hp.isl_lpin0.size());
+    ar.put<int, int>(&pl->d_lpin1, hp.isl_lpin1.data(), hp.isl_lpin1.size());
+    ar.put<double, double>(&pl->d_ldist, hp.isl_ldist.data(), hp.isl_ldist.size());
+    const size_t nv = hp.n_vert, nt = hp.n_tri;
+    const std::vector<double> ones(nt, 1.0);
+    ar.put<double, double>(&d.fscale, ones.data(), nt);
+    // zero-filled: the CG vectors (defined contents from the start), the
+    // island states and the fault flag
+    void** vecs[6] = {&d.vr, &d.vp, &d.vAp, &d.ve, &d.vu, &d.vw};
+    for (void** v : vecs) ar.zero((double2**)v, nv);
+    ar.zero(&d.st, (size_t)hp.n_isl);
+    ar.zero(&d.fault, 1);
+    // scratch the kernels write before reading
+    ar.raw(&d.crc, hp.sl_ptr.empty() ? 1 : hp.sl_ptr.size());
+    for (int k = 0; k < 3; ++k) ar.raw(&pl->ubuf[k], nv);
+    ar.raw(&d.team_cnt, (size_t)kMaxSlots * std::max(1, hp.n_teams));
+    ar.raw(&d.team_part, (size_t)std::max(1, hp.n_teams) * 2 * uva::kMaxTeam * 8);
+    ar.raw(&d.R, nt);
+    ar.raw(&d.b64, nv);
+    ar.raw(&d.X64, nv);
+    ar.raw(&d.sy, 3 * nt);               // per-triangle corner values of the matrix-free passes (SLIM, backward)
+    ar.raw(&d.red, 4 * (size_t)hp.n_isl);
+    ar.raw(&d.queue, uva::kQueueStride * (size_t)kMaxSlots);
+    {
+        hipError_t e = ar.commit(pl, s);
+        if (e != hipSuccess)
+            return bail(e == hipErrorOutOfMemory ? UVA_ENOMEM : UVA_EHIP,
+                        std::string("plan upload: ") + hipGetErrorString(e));
+    }
+    d.isl_lpin0 = pl->d_lpin0;
+    d.isl_lpin1 = pl->d_lpin1;
+    d.isl_ldist = pl->d_ldist;
+    pl->team_alloc = std::max(1, hp.n_teams);
     pl->vinv.assign(hp.n_vert, 0);
     for (int v = 0; v < hp.n_vert; ++v) pl->vinv[hp.vperm[v]] = v;
-    {
-        hipError_t e;
-        const size_t nsl = hp.sl_ptr.empty() ? 1 : hp.sl_ptr.size();
-        if ((e = pl->alloc(&d.crc, nsl)) != hipSuccess) return bail(UVA_ENOMEM, "coarse vector");
-        for (int k = 0; k < 3; ++k)
-            if ((e = pl->alloc(&pl->ubuf[k], (size_t)hp.n_vert)) != hipSuccess) return bail(UVA_ENOMEM, "vertex buffers");
-    }
-    {
-        const double* g = nullptr;
-        hipError_t e = pl->upload(&g, hp.inc_g, s);
-        if (e != hipSuccess) return bail(UVA_EHIP, "upload inc_g");
-        d.inc_g = (const double2*)g;
-        if ((e = pl->upload(&g, hp.tri_g, s)) != hipSuccess) return bail(UVA_EHIP, "upload tri_g");
-        d.tri_g = (const double2*)g;
-    }
-    // LSCM pins (mutable through uva_lscm)
     pl->lpin0 = hp.isl_lpin0;
     pl->lpin1 = hp.isl_lpin1;
     pl->v0 = hp.isl_v0;
     if (P) pl->P.assign(P, P + 3 * (size_t)hp.n_vert);
-    {
-        hipError_t e;
-        if ((e = pl->alloc(&pl->d_lpin0, hp.n_isl)) != hipSuccess ||
-            (e = pl->alloc(&pl->d_lpin1, hp.n_isl)) != hipSuccess ||
-            (e = pl->alloc(&pl->d_ldist, hp.n_isl)) != hipSuccess)
-            return bail(UVA_ENOMEM, "pin arrays");
-        HIP_TRY(hipMemcpyAsync(pl->d_lpin0, hp.isl_lpin0.data(), hp.n_isl * sizeof(int), hipMemcpyHostToDevice, s));
-        HIP_TRY(hipMemcpyAsync(pl->d_lpin1, hp.isl_lpin1.data(), hp.n_isl * sizeof(int), hipMemcpyHostToDevice, s));
-        HIP_TRY(hipMemcpyAsync(pl->d_ldist, hp.isl_ldist.data(), hp.n_isl * sizeof(double), hipMemcpyHostToDevice, s));
-        d.isl_lpin0 = pl->d_lpin0;
-        d.isl_lpin1 = pl->d_lpin1;
-        d.isl_ldist = pl->d_ldist;
-    }
     d.n_group = (int)hp.order_group.size();
     d.n_wave = (int)hp.order_wave.size();
     d.n_teams = hp.n_teams;
-    {
-        hipError_t e;
-        if ((e = pl->alloc(&d.team_cnt, (size_t)kMaxSlots * std::max(1, hp.n_teams))) != hipSuccess ||
-            (e = pl->alloc(&d.team_part, (size_t)std::max(1, hp.n_teams) * 2 * uva::kMaxTeam * 8)) != hipSuccess)
-            return bail(UVA_ENOMEM, "team arrays");
-        pl->team_alloc = std::max(1, hp.n_teams);
-    }
     pl->members = island_members(hp);
     {
         uva::HostPlan& sc = pl->sched;
@@ -358,46 +418,6 @@ int upload_plan(uva::HostPlan& hp, const double* P, hipStream_t s, uva_plan** ou
         sc.sched_slots = hp.sched_slots;
         sc.sched_teams = hp.sched_teams;
         sc.n_xcd = hp.n_xcd;
-    }
-    // scratch
-    {
-        hipError_t e = hipSuccess;
-        size_t nv = hp.n_vert, nt = hp.n_tri;
-        double* sc = nullptr;
-        if ((e = pl->alloc(&sc, nt)) != hipSuccess) return bail(UVA_ENOMEM, "scale");
-        d.fscale = sc;
-        std::vector<double> ones(nt, 1.0);
-        if (nt) HIP_TRY(hipMemcpyAsync(sc, ones.data(), nt * sizeof(double), hipMemcpyHostToDevice, s));
-        double2* p2 = nullptr;
-        if ((e = pl->alloc(&p2, nt)) != hipSuccess) return bail(UVA_ENOMEM, "R");
-        d.R = p2;
-        if ((e = pl->alloc(&p2, nv)) != hipSuccess) return bail(UVA_ENOMEM, "b64");
-        d.b64 = p2;
-        if ((e = pl->alloc(&p2, nv)) != hipSuccess) return bail(UVA_ENOMEM, "X64");
-        d.X64 = p2;
-        // per-triangle corner values of the matrix-free passes (SLIM, backward)
-        if ((e = pl->alloc(&p2, 3 * nt)) != hipSuccess) return bail(UVA_ENOMEM, "corner scratch");
-        d.sy = p2;
-        void** vecs[6] = {&d.vr, &d.vp, &d.vAp, &d.ve, &d.vu, &d.vw};
-        for (void** v : vecs) {
-            double2* q = nullptr;
-            if ((e = pl->alloc(&q, nv)) != hipSuccess) return bail(UVA_ENOMEM, "CG vectors");
-            // defined contents from the start (plan creation, never inside a graph)
-            HIP_TRY(hipMemsetAsync(q, 0, std::max<size_t>(nv, 1) * sizeof(double2), s));
-            *v = q;
-        }
-        double* red = nullptr;
-        if ((e = pl->alloc(&red, 4 * (size_t)hp.n_isl)) != hipSuccess) return bail(UVA_ENOMEM, "red");
-        d.red = red;
-        uva::IslandState* st = nullptr;
-        if ((e = pl->alloc(&st, hp.n_isl)) != hipSuccess) return bail(UVA_ENOMEM, "state");
-        HIP_TRY(hipMemsetAsync(st, 0, hp.n_isl * sizeof(uva::IslandState), s));
-        d.st = st;
-        int* q = nullptr;
-        if ((e = pl->alloc(&q, uva::kQueueStride * (size_t)kMaxSlots)) != hipSuccess) return bail(UVA_ENOMEM, "queue");
-        d.queue = q;
-        if ((e = pl->alloc(&d.fault, 1)) != hipSuccess) return bail(UVA_ENOMEM, "fault flag");
-        HIP_TRY(hipMemsetAsync(d.fault, 0, sizeof(int), s));
     }
     // XCD lists: the probed hardware ids map to lists 0..n_xcd-1 when the plan
     // was built for them, else everything maps to the single list
@@ -506,6 +526,8 @@ int uva_plan_get_info(const uva_plan* plan, uva_plan_info* info) {
     info->device_bytes = plan->device_bytes;
     info->n_group_tasks = plan->dp.n_group;
     info->n_wave_tasks = plan->dp.n_wave;
+    info->n_teams = plan->dp.n_teams;
+    info->reserved = 0;
     return UVA_OK;
 }
 

@@ -21,7 +21,7 @@ import numpy as np
 import scipy.sparse as sp
 
 from . import _lib
-from .plan import CG_TOL, LSCM_TOL, IslandPlan, require_gpu, stream_of
+from .plan import CG_TOL, LSCM_TOL, IslandPlan, own_stream, require_gpu, solve_guard, stream_of
 
 DIRECT_LIMIT = 220_000        # vertices: above this the reference uses warm-started CG (param.py:15)
 PARALLEL_MIN_TRIS = 20000     # param.py:328
@@ -190,10 +190,13 @@ def lscm(P, T, X=None, cots=None, pins=None):
     bi, _ = boundary_edges_oriented(T)
     if len(bi) == 0:
         raise ValueError("chart has no boundary (closed surface): needs a cut")
-    plan = IslandPlan([(P, T)], frames=frames)
-    U = torch.zeros(len(P), 2, dtype=torch.float64, device=plan.dev)
-    plan.lscm(U, pins=None if pins is None else [int(pins[0]), int(pins[1])], tol=LSCM_TOL)
-    return U.cpu().numpy()
+    dev = require_gpu()
+    with torch.cuda.stream(own_stream(dev)):
+        plan = IslandPlan([(P, T)], frames=frames, dev=dev)
+        with solve_guard(plan):
+            U = torch.zeros(len(P), 2, dtype=torch.float64, device=dev)
+            plan.lscm(U, pins=None if pins is None else [int(pins[0]), int(pins[1])], tol=LSCM_TOL)
+            return U.cpu().numpy()
 
 
 def _far_pair(P, cand):
@@ -396,23 +399,30 @@ def refine(P, T, U, iters=10, method="arap", X=None, area=None, cots=None, solve
     frames = None
     if X is not None:   # caller's frames (the reference uses them instead of P's)
         frames = (X, area if area is not None else local_frames(P, T)[1], cots if cots is not None else cot_weights(X))
-    if method == "slim":
-        plan = IslandPlan([(P, T)], frames=frames)
-        Ud = plan.device_uv(U)
-        plan.slim(Ud, iters, tol=tol)
-        return Ud.cpu().numpy(), None
-    if method != "arap":
+    if method not in ("arap", "slim"):
         raise ValueError(f"unknown refine method {method!r}")
-    pin = 0 if pin is None else int(pin)
-    plan = None
-    if isinstance(solver, ArapSolver) and solver.n == len(P) and solver.t == len(T) and solver.pin == pin:
-        plan = solver.plan
-    if plan is None:
-        plan = IslandPlan([(P, T)], pins=[pin], frames=frames)
-        solver = ArapSolver(plan, len(P), len(T), pin)
-    Ud = plan.device_uv(U)
-    plan.arap(Ud, iters, line_search=line_search, mode=_lib.UVA_MODE_REFINE, tol=tol, cg_tol=CG_TOL)
-    return Ud.cpu().numpy(), solver
+    import torch
+
+    dev = require_gpu()
+    with torch.cuda.stream(own_stream(dev)):
+        if method == "slim":
+            plan = IslandPlan([(P, T)], frames=frames, dev=dev)
+            with solve_guard(plan):
+                Ud = plan.device_uv(U)
+                plan.slim(Ud, iters, tol=tol)
+                return Ud.cpu().numpy(), None
+        pin = 0 if pin is None else int(pin)
+        plan = None
+        if (isinstance(solver, ArapSolver) and solver.n == len(P) and solver.t == len(T) and solver.pin == pin
+                and solver.plan.dev == dev):
+            plan = solver.plan
+        if plan is None:
+            plan = IslandPlan([(P, T)], pins=[pin], frames=frames, dev=dev)
+            solver = ArapSolver(plan, len(P), len(T), pin)
+        with solve_guard(plan):
+            Ud = plan.device_uv(U)
+            plan.arap(Ud, iters, line_search=line_search, mode=_lib.UVA_MODE_REFINE, tol=tol, cg_tol=CG_TOL)
+            return Ud.cpu().numpy(), solver
 
 
 # ------------------------------------------------------------- flip repair

@@ -6,7 +6,9 @@ tensors (n x 2 float64, torch, on the plan's GPU).
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes as C
+import threading
 
 import numpy as np
 import torch
@@ -29,6 +31,39 @@ def require_gpu(dev=None):
     if d.type != "cuda":
         raise RuntimeError(f"ultimate_uv_artist_amd runs on a GPU device, got {d}")
     return d
+
+
+_local = threading.local()
+_team_solves = threading.Lock()
+
+
+def own_stream(dev):
+    """A stream of the calling thread's own on ``dev`` (created on first use).
+
+    The per-chart calls (``param.lscm``, ``param.refine``) run on it: the
+    reference calls them one chart per call from a pool of threads, and on one
+    shared stream those calls would queue behind each other and every call's
+    final synchronisation would wait for all threads' work.  Each call copies
+    its result to the host before it returns, so the caller never sees the
+    stream."""
+    streams = getattr(_local, "streams", None)
+    if streams is None:
+        streams = _local.streams = {}
+    s = streams.get(dev)
+    if s is None:
+        s = streams[dev] = torch.cuda.Stream(dev)
+    return s
+
+
+def solve_guard(plan):
+    """Context for one call's solves on ``plan`` from a thread's own stream.
+
+    A plan with teams of several workgroups needs each team's workgroups
+    resident together; two such solves side by side could each hold CUs that
+    the other's members wait for, so those take one process-wide lock until
+    their results are on the host.  Plans with wavefront and one-workgroup
+    tasks only never wait across workgroups and run side by side."""
+    return _team_solves if plan.info()["n_teams"] > 0 else contextlib.nullcontext()
 
 
 def _tuning_key(dev):
