@@ -36,7 +36,7 @@ def lib():
         _lib.uva_emu_create.restype = C.c_void_p
         _lib.uva_emu_create.argtypes = [C.c_int64, _i64p, _i64p, _dp, _i64p, _i64p, C.c_uint32, C.c_int32, C.c_int32]
         _lib.uva_emu_destroy.argtypes = [C.c_void_p]
-        _lib.uva_emu_lscm.argtypes = [C.c_void_p, _dp, C.c_double, C.c_int32]
+        _lib.uva_emu_lscm.argtypes = [C.c_void_p, _dp, C.c_double, C.c_int32, C.c_int32]
         _lib.uva_emu_arap.argtypes = [C.c_void_p, _dp, C.POINTER(L.ArapOpts)]
         _lib.uva_emu_cg.argtypes = [C.c_void_p, _dp, _dp, C.c_double, C.c_int32, C.c_int32]
         _lib.uva_emu_stats.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), _dp]
@@ -47,6 +47,9 @@ def lib():
         _lib.uva_emu_energy_bwd.argtypes = [C.c_void_p, _dp, _dp, C.c_int32, _dp, _dp, _dp]
         _lib.uva_emu_step_bwd.argtypes = [C.c_void_p, _dp, _dp, _dp]
         _lib.uva_emu_adjoint_cg.argtypes = [C.c_void_p, _dp, _dp, C.c_double, C.c_int32]
+        _lib.uva_emu_vperm.argtypes = [C.c_void_p, C.POINTER(C.c_int32)]
+        _lib.uva_emu_lscm_coarse.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_float), C.c_int64,
+                                             C.POINTER(C.c_int32)]
         _lib.uva_emu_set_pins.argtypes = [C.c_void_p, _i64p, C.c_int64, C.c_double]
         _lib.uva_emu_overlap_flags.argtypes = [_dp, C.c_int64, _i64p, C.c_int64, C.c_void_p, C.c_double, C.c_int32,
                                                C.c_int32, C.c_void_p]
@@ -120,9 +123,9 @@ class EmuPlan:
         if getattr(self, "h", None):
             lib().uva_emu_destroy(self.h)
 
-    def lscm(self, tol=1e-8, max_iters=0):
+    def lscm(self, tol=1e-8, max_iters=0, precision=0):
         U = np.zeros((self.n, 2))
-        rc = lib().uva_emu_lscm(self.h, U.ctypes.data_as(_dp), tol, max_iters)
+        rc = lib().uva_emu_lscm(self.h, U.ctypes.data_as(_dp), tol, max_iters, precision)
         if rc != 0:
             raise ValueError("island without boundary")
         return U
@@ -144,6 +147,19 @@ class EmuPlan:
         o = L.SlimOpts(iters, cg_max_iters, tol, cg_tol)
         lib().uva_emu_slim(self.h, U.ctypes.data_as(_dp), C.byref(o))
         return U
+
+    def vperm(self):
+        out = np.zeros(self.n, np.int32)
+        lib().uva_emu_vperm(self.h, out.ctypes.data_as(C.POINTER(C.c_int32)))
+        return out
+
+    def lscm_coarse(self, isl):
+        """(rows of [Pc | Qc] as an nc x 2nc array or None, slices per aggregate)."""
+        cap = 2 * 128 * 128
+        buf = np.zeros(cap, np.float32)
+        g = C.c_int32(0)
+        nc = lib().uva_emu_lscm_coarse(self.h, int(isl), buf.ctypes.data_as(C.POINTER(C.c_float)), cap, C.byref(g))
+        return (buf[:2 * nc * nc].reshape(nc, 2 * nc) if nc > 0 else None), g.value
 
     def set_pins(self, pins, w):
         pins = np.ascontiguousarray(np.asarray(pins, np.int64))

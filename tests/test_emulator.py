@@ -72,15 +72,78 @@ def test_emu_cg_vs_direct(disks, shape, precision):
 
 
 @pytest.mark.parametrize("shape", SHAPES)
-def test_emu_lscm_vs_oracle(disks, shape):
+@pytest.mark.parametrize("no_coarse", [False, True])
+def test_emu_lscm_vs_oracle(disks, shape, no_coarse):
+    """The FP64 LSCM solve (resident where the member's rows fit, with the
+    coarse level of the coupled operator unless left out) against SuperLU."""
     from oracle import param_cpu as oracle
 
-    e = EmuPlan(disks, *shape)
-    U = e.lscm(tol=1e-10)
+    e = EmuPlan(disks, *shape, no_coarse=no_coarse)
+    U = e.lscm(tol=1e-10, precision=2)
     assert e.stats()["converged"].all()
     for i, (P, T) in enumerate(disks):
         s = slice(e.vo[i], e.vo[i + 1])
         assert unit_box_err(U[s], oracle.lscm(P, T)) <= 1e-7
+
+
+@pytest.mark.parametrize("shape", [(1, 4), (3, 2), (2, 8)])
+def test_emu_lscm_resident_equals_streaming(disks, shape):
+    """The resident LSCM solve (both columns coupled through the boundary
+    terms, hard pins kept at zero) performs the streaming solve's arithmetic
+    in the same order: the same layout bit for bit and the same iterations."""
+    out, its = [], []
+    for resident in (True, False):
+        e = EmuPlan(disks, *shape, resident=resident, no_coarse=True)
+        out.append(e.lscm(tol=1e-9, precision=2))
+        st = e.stats()
+        assert st["converged"].all()
+        its.append(st["cg_iters"].copy())
+    assert np.array_equal(its[0], its[1])
+    assert np.array_equal(out[0], out[1])
+
+
+def test_emu_lscm_coarse_level_is_the_inverse_of_the_restricted_operator():
+    """The plan's coarse level of the LSCM operator against a construction
+    from the oracle's conformal system (param.py:136-165 with the two pins
+    eliminated, assembled by scipy from the reference's triplets): with Z the
+    indicator of the aggregates (g consecutive 64-row slices of the plan's
+    vertex order) for u and for v over the free rows, the rows of [Pc | Qc]
+    are the top half of (Z^T Hff Z)^-1 = [[Pc, Qc], [-Qc, Pc]].  The coarse
+    level leaves the converged layout as it is and cuts the iterations."""
+    import scipy.sparse as sp
+
+    from oracle import param_cpu as oracle
+    from ultimate_uv_artist_amd import workloads
+
+    P, T = workloads.disk_island(np.random.default_rng(21), 3000)
+    n = len(P)
+    e = EmuPlan([(P, T)], members=2, waves=8)
+    C_, g = e.lscm_coarse(0)
+    assert C_ is not None
+    nc = C_.shape[0]
+    rank = np.empty(n, np.int64)
+    rank[e.vperm()] = np.arange(n)       # plan position of each caller vertex
+    agg = rank // (64 * g)
+    assert agg.max() + 1 == nc
+    Hff, _, keep, _, _ = oracle.lscm_system(P, T)
+    free = np.flatnonzero(keep)           # rows of Hff: free u, then free v
+    col = np.where(free < n, agg[free % n], nc + agg[free % n])
+    Z = sp.csr_matrix((np.ones(len(free)), (np.arange(len(free)), col)), shape=(len(free), 2 * nc))
+    S = np.linalg.inv((Z.T @ Hff @ Z).toarray())
+    assert np.isfinite(S).all()
+    ref = np.hstack([0.5 * (S[:nc, :nc] + S[nc:, nc:]), 0.5 * (S[:nc, nc:] - S[nc:, :nc])])
+    assert np.abs(S[:nc, :nc] - S[nc:, nc:]).max() <= 1e-9 * np.abs(S).max()   # the Hermitian form
+    assert np.abs(C_ - ref).max() <= 1e-6 * np.abs(ref).max()
+    it, U = {}, {}
+    for no_coarse in (False, True):
+        e = EmuPlan([(P, T)], members=2, waves=8, no_coarse=no_coarse)
+        U[no_coarse] = e.lscm(tol=1e-10, precision=2)
+        st = e.stats()
+        assert st["converged"].all()
+        it[no_coarse] = int(st["cg_iters"][0])
+    Ur = oracle.lscm(P, T)
+    assert unit_box_err(U[False], Ur) <= 1e-7 and unit_box_err(U[True], Ur) <= 1e-7
+    assert it[False] < it[True]
 
 
 @pytest.mark.parametrize("shape", SHAPES)

@@ -5,6 +5,8 @@ import os
 import shutil
 import subprocess
 import sys
+import tempfile
+from concurrent.futures import ThreadPoolExecutor
 from pathlib import Path
 
 HERE = Path(__file__).resolve().parent
@@ -21,21 +23,38 @@ def hipcc():
     raise RuntimeError("hipcc not found")
 
 
+FLAGS = ["-O3", "-std=c++17", "-fPIC",
+         # packed FP32 instructions stay off (hipcc forms them from float code)
+         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+         "-Wno-unused-command-line-argument", "-pthread"]
+
+
 def build(verbose=False):
-    """Compile every source for gfx950 and link libuvart.so next to this file."""
+    """Compile every source for gfx950 (one hipcc per source, in parallel) and
+    link libuvart.so next to this file."""
     tmp = LIB.with_suffix(".so.tmp")
-    cmd = [hipcc(), f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
-           # packed FP32 instructions stay off (hipcc forms them from float code)
-           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-           "-Wno-unused-command-line-argument", "-pthread",
-           "-I", str(HERE.parent / "include"),
-           *[str(CSRC / s) for s in SOURCES], "-o", str(tmp)]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("hipcc failed building libuvart.so")
+    with tempfile.TemporaryDirectory(prefix="uvart_build_") as td:
+        def compile_one(src):
+            obj = str(Path(td) / (src + ".o"))
+            cmd = [hipcc(), f"--offload-arch={ARCH}", *FLAGS, "-I", str(HERE.parent / "include"), "-c",
+                   str(CSRC / src), "-o", obj]
+            if verbose:
+                print(" ".join(cmd), flush=True)
+            return obj, subprocess.run(cmd, capture_output=True, text=True)
+
+        with ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
+            results = list(ex.map(compile_one, SOURCES))
+        for (obj, r), src in zip(results, SOURCES):
+            if r.returncode != 0:
+                sys.stderr.write(r.stdout + r.stderr)
+                raise RuntimeError(f"hipcc failed compiling {src}")
+        cmd = [hipcc(), f"--offload-arch={ARCH}", *FLAGS, "-shared", *[o for o, _ in results], "-o", str(tmp)]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError("hipcc failed linking libuvart.so")
     os.replace(tmp, LIB)
     return LIB
 

@@ -53,6 +53,24 @@ struct uva_plan {
     int clock_khz = 0;                    // device wall-clock rate (wall_clock64 ticks per ms)
     std::vector<double> P;                 // host positions (LSCM pins given later)
     std::vector<int> lpin0, lpin1, v0;     // host copies
+    std::vector<double> ldist;             // host copy of the default pins' distances
+    std::vector<int64_t> lcofs;            // host copy of the LSCM coarse offsets (the plan's own pins)
+    bool lpins_custom = false;             // the device holds caller-given LSCM pins
+    // Task schedule of the LSCM solve.  Its FP64 vectors keep fewer rows per
+    // workgroup resident (kResSlicesF64) than the FP32 ARAP CG, and its CG
+    // iterations grow faster with island size, so its teams and slot lists
+    // differ from the ARAP schedule's; built at the first uva_lscm call.
+    struct TaskList {
+        const int* order_group = nullptr;
+        const int* grp_mem = nullptr;
+        const int* grp_m = nullptr;
+        const int* grp_team = nullptr;
+        const int* slot_ptr = nullptr;
+        int n_group = 0, n_teams = 0, max_m = 1, slots_per_xcd = 1;
+    } lsched;
+    bool have_lsched = false;
+    std::vector<int> lmembers;             // per island: workgroups of its LSCM team
+    bool last_lscm = false;                // the latest solve call was uva_lscm
     std::vector<int> vinv;                 // caller's vertex id -> plan vertex
     uva::HostPlan sched;                   // island sizes and schedule inputs (for uva_plan_tune)
     int team_alloc = 0;                    // teams the barrier arrays hold
@@ -109,8 +127,26 @@ struct uva_plan {
 
 namespace {
 
+// The plan's device view with the LSCM task schedule in place of the ARAP one.
+uva::DevPlan lscm_view(const uva_plan* pl) {
+    uva::DevPlan d = pl->dp;
+    const auto& l = pl->lsched;
+    d.order_group = l.order_group;
+    d.grp_mem = l.grp_mem;
+    d.grp_m = l.grp_m;
+    d.grp_team = l.grp_team;
+    d.slot_ptr = l.slot_ptr;
+    d.n_group = l.n_group;
+    d.n_teams = l.n_teams;
+    d.max_m = l.max_m;
+    d.slots_per_xcd = l.slots_per_xcd;
+    return d;
+}
+
 int launch(uva_plan* pl, int kind, int prec, const uva::TaskArgs& a, int slot, hipStream_t s) {
-    HIP_TRY(uva::launch_island_tasks(kind, prec, pl->dp, a, slot, pl->cus, s));
+    pl->last_lscm = kind == uva::TASK_LSCM;
+    if (kind == uva::TASK_LSCM) HIP_TRY(uva::launch_island_tasks(kind, prec, lscm_view(pl), a, slot, pl->cus, s));
+    else HIP_TRY(uva::launch_island_tasks(kind, prec, pl->dp, a, slot, pl->cus, s));
     return UVA_OK;
 }
 
@@ -133,10 +169,12 @@ int from_plan(uva_plan* pl, int k, double* ext, hipStream_t s, int rc) {
 // Task counters and team barrier counters of a call sequence start at zero
 // (fill kernels, not memsets, so the sequence can be captured in a graph);
 // the first reset of a call also clears the team fault flag.
-int reset_queue(uva_plan* pl, int slots, hipStream_t s, bool first = true) {
+// (n_teams: teams of the schedule the sequence runs, -1 for the ARAP one)
+int reset_queue(uva_plan* pl, int slots, hipStream_t s, bool first = true, int n_teams = -1) {
+    if (n_teams < 0) n_teams = pl->dp.n_teams;
     HIP_TRY(uva::launch_fill_int(pl->dp.queue, uva::kQueueStride * slots, 0, s));
-    if (pl->dp.n_teams > 0) {
-        HIP_TRY(uva::launch_fill_int(pl->dp.team_cnt, slots * pl->dp.n_teams, 0, s));
+    if (n_teams > 0) {
+        HIP_TRY(uva::launch_fill_int(pl->dp.team_cnt, slots * n_teams, 0, s));
         if (first) HIP_TRY(uva::launch_fill_int(pl->dp.fault, 1, 0, s));
     }
     return UVA_OK;
@@ -147,8 +185,9 @@ int reset_queue(uva_plan* pl, int slots, hipStream_t s, bool first = true) {
 // results are then invalid.  Every call that ran island tasks on a plan with
 // multi-workgroup teams reads the flag back (waiting for the stream) and
 // reports it; the flag is cleared again for the next call.
-int check_fault(uva_plan* pl, hipStream_t s, int rc) {
-    if (rc || pl->dp.n_teams <= 0) return rc;
+int check_fault(uva_plan* pl, hipStream_t s, int rc, int n_teams = -1) {
+    if (n_teams < 0) n_teams = pl->dp.n_teams;
+    if (rc || n_teams <= 0) return rc;
     int f = 0;
     HIP_TRY(hipMemcpyAsync(&f, pl->dp.fault, sizeof(int), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
@@ -362,6 +401,7 @@ int upload_plan(uva::HostPlan& hp, const double* P, hipStream_t s, uva_plan** ou
     UP(order_group, order_group); UP(order_wave, order_wave);
     UP(grp_mem, grp_mem); UP(grp_m, grp_m); UP(grp_team, grp_team); UP(slot_ptr, slot_ptr);
     UP(vperm, vperm); UP(isl_cofs, isl_cofs); UP(isl_cg, isl_cg); UP(cinv, cinv);
+    UP(isl_lcofs, isl_lcofs); UP(lcinv, lcinv);
 #undef UP
     ar.put<double, const double2>(&d.inc_g, hp.inc_g.data(), hp.inc_g.size());
     ar.put<double, const double2>(&d.tri_g, hp.tri_g.data(), hp.tri_g.size());
@@ -403,6 +443,8 @@ int upload_plan(uva::HostPlan& hp, const double* P, hipStream_t s, uva_plan** ou
     for (int v = 0; v < hp.n_vert; ++v) pl->vinv[hp.vperm[v]] = v;
     pl->lpin0 = hp.isl_lpin0;
     pl->lpin1 = hp.isl_lpin1;
+    pl->ldist = hp.isl_ldist;
+    pl->lcofs = hp.isl_lcofs;
     pl->v0 = hp.isl_v0;
     if (P) pl->P.assign(P, P + 3 * (size_t)hp.n_vert);
     d.n_group = (int)hp.order_group.size();
@@ -550,7 +592,8 @@ int uva_plan_get_stats(uva_plan* plan, uva_island_stats* stats, void* stream) {
         o.flips0 = st[i].flips0;
         o.use_ls = st[i].use_ls;
         o.status = st[i].status;
-        o.members = i < plan->members.size() ? plan->members[i] : 0;
+        const std::vector<int>& mem = plan->last_lscm ? plan->lmembers : plan->members;
+        o.members = i < mem.size() ? mem[i] : 0;
         o.rel_resid = st[i].rel_resid;
         o.energy = st[i].e_prev;
         o.step = st[i].step;
@@ -600,6 +643,53 @@ int replan(uva_plan* plan, const double* est, hipStream_t s) {
     if (plan->gexec) { (void)hipGraphExecDestroy(plan->gexec); plan->gexec = nullptr; }
     if (plan->graph) { (void)hipGraphDestroy(plan->graph); plan->graph = nullptr; }
     plan->gkey.clear();
+    HIP_TRY(hipStreamSynchronize(s));
+    return UVA_OK;
+}
+
+// The LSCM task schedule from per-island CG iterations (est, or NULL for a
+// size-based guess: about 1.3 n^0.8 Jacobi-PCG iterations to 1e-8 on the
+// conformal operator of an n-vertex disk) with the plan's cost model.
+int plan_lscm(uva_plan* plan, const double* est, hipStream_t s) {
+    uva::DevPlan& d = plan->dp;
+    plan->mem = s;
+    const uva::HostPlan& ms = plan->sched;
+    uva::HostPlan sc;
+    sc.n_isl = ms.n_isl;
+    sc.isl_nv = ms.isl_nv;
+    sc.isl_ns = ms.isl_ns;
+    for (int k = 0; k < 3; ++k) sc.cost[k] = ms.cost[k];
+    std::vector<double> guess;
+    if (!est) {
+        guess.resize((size_t)std::max(ms.n_isl, 1));
+        for (int i = 0; i < ms.n_isl; ++i) guess[i] = 1.3 * std::pow((double)ms.isl_nv[i], 0.8);
+        est = guess.data();
+    }
+    try {
+        uva::plan_teams(sc, ms.group_isl, ms.sched_slots, ms.n_xcd, ms.sched_teams, est, uva::kResSlicesF64);
+    } catch (const std::bad_alloc&) {
+        return fail(UVA_ENOMEM, "host allocation failed while planning the LSCM schedule");
+    }
+    auto& l = plan->lsched;
+#define UPL(field)                                                                               \
+    do {                                                                                         \
+        hipError_t _e = plan->upload(&l.field, sc.field, s);                                      \
+        if (_e != hipSuccess) return fail(UVA_EHIP, std::string("upload " #field ": ") + hipGetErrorString(_e)); \
+    } while (0)
+    UPL(order_group); UPL(grp_mem); UPL(grp_m); UPL(grp_team); UPL(slot_ptr);
+#undef UPL
+    if (sc.n_teams > plan->team_alloc) {
+        if (plan->alloc(&d.team_cnt, (size_t)kMaxSlots * sc.n_teams) != hipSuccess ||
+            plan->alloc(&d.team_part, (size_t)sc.n_teams * 2 * uva::kMaxTeam * 8) != hipSuccess)
+            return fail(UVA_ENOMEM, "team arrays");
+        plan->team_alloc = sc.n_teams;
+    }
+    l.n_group = (int)sc.order_group.size();
+    l.n_teams = sc.n_teams;
+    l.max_m = sc.max_m;
+    l.slots_per_xcd = sc.slots_per_xcd;
+    plan->lmembers = island_members(sc);
+    plan->have_lsched = true;
     HIP_TRY(hipStreamSynchronize(s));
     return UVA_OK;
 }
@@ -678,6 +768,8 @@ int uva_plan_tune(uva_plan* plan, void* stream) {
         any = any || st[i].cg_iters > 0;
     }
     if (!any) return fail(UVA_EINVAL, "uva_plan_tune needs a finished solve on the plan (no CG iterations recorded)");
+    // after an LSCM solve: its own schedule from the measured iterations
+    if (plan->last_lscm) return plan_lscm(plan, est.data(), s);
     fit_costs(st, plan->members, plan->sched.isl_ns, plan->clock_khz, plan->sched.cost);
     plan->est.assign(est.begin(), est.end());
     return replan(plan, est.data(), s);
@@ -762,9 +854,23 @@ int uva_lscm(uva_plan* plan, double* U, const int64_t* pins, double tol, int32_t
     if (!(tol > 0.0)) return fail(UVA_EINVAL, "tol must be positive");
     hipStream_t s = (hipStream_t)stream;
     uva::DevPlan& d = plan->dp;
+    // The coarse level of an island's LSCM operator leaves out the plan's own
+    // pins; an island given other pins solves with the Jacobi preconditioner.
+    auto put_pins = [&](const std::vector<int>& a, const std::vector<int>& b, const std::vector<double>& dist,
+                        const std::vector<int64_t>& lco) -> int {
+        if (d.n_isl <= 0) return UVA_OK;
+        HIP_TRY(hipMemcpyAsync(plan->d_lpin0, a.data(), a.size() * sizeof(int), hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(plan->d_lpin1, b.data(), b.size() * sizeof(int), hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(plan->d_ldist, dist.data(), dist.size() * sizeof(double), hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync((void*)d.isl_lcofs, lco.data(), lco.size() * sizeof(int64_t), hipMemcpyHostToDevice,
+                               s));
+        HIP_TRY(hipStreamSynchronize(s));
+        return UVA_OK;
+    };
     if (pins) {
         std::vector<int> a(d.n_isl), b(d.n_isl);
         std::vector<double> dist(d.n_isl);
+        std::vector<int64_t> lco(plan->lcofs);
         for (int i = 0; i < d.n_isl; ++i) {
             int nv = (i + 1 < d.n_isl ? plan->v0[i + 1] : d.n_vert) - plan->v0[i];
             if (pins[2 * i] < 0 || pins[2 * i] >= nv || pins[2 * i + 1] < 0 || pins[2 * i + 1] >= nv)
@@ -776,23 +882,34 @@ int uva_lscm(uva_plan* plan, double* U, const int64_t* pins, double tol, int32_t
             const double* p1 = &plan->P[3 * (size_t)cb];
             dist[i] = std::sqrt((p1[0] - p0[0]) * (p1[0] - p0[0]) + (p1[1] - p0[1]) * (p1[1] - p0[1]) +
                                 (p1[2] - p0[2]) * (p1[2] - p0[2]));
+            if (a[i] != plan->lpin0[i] || b[i] != plan->lpin1[i]) lco[i] = -1;
         }
-        HIP_TRY(hipMemcpyAsync(plan->d_lpin0, a.data(), a.size() * sizeof(int), hipMemcpyHostToDevice, s));
-        HIP_TRY(hipMemcpyAsync(plan->d_lpin1, b.data(), b.size() * sizeof(int), hipMemcpyHostToDevice, s));
-        HIP_TRY(hipMemcpyAsync(plan->d_ldist, dist.data(), dist.size() * sizeof(double), hipMemcpyHostToDevice, s));
-        HIP_TRY(hipStreamSynchronize(s));
+        const int rc = put_pins(a, b, dist, lco);
+        if (rc) return rc;
+        plan->lpins_custom = true;
     } else {
         for (int i = 0; i < d.n_isl; ++i)
             if (plan->lpin0[i] < 0)
                 return fail(UVA_ENOBOUNDARY, "island " + std::to_string(i) +
                                                  " has no boundary (closed surface): needs a cut");
+        if (plan->lpins_custom) {
+            // back to the plan's own pins
+            const int rc = put_pins(plan->lpin0, plan->lpin1, plan->ldist, plan->lcofs);
+            if (rc) return rc;
+            plan->lpins_custom = false;
+        }
+    }
+    if (!plan->have_lsched) {
+        const int rc = plan_lscm(plan, nullptr, s);
+        if (rc) return rc;
     }
     uva::TaskArgs a = base_args((double*)plan->ubuf[0]);
     a.cg_tol = tol;
     a.cg_max = max_iters > 0 ? max_iters : kDefaultMaxIters;
-    int rc = reset_queue(plan, 1, s);
+    const int nt = plan->lsched.n_teams;
+    int rc = reset_queue(plan, 1, s, true, nt);
     if (!rc) rc = launch(plan, uva::TASK_LSCM, UVA_PREC_F64, a, 0, s);
-    return check_fault(plan, s, from_plan(plan, 0, U, s, rc));
+    return check_fault(plan, s, from_plan(plan, 0, U, s, rc), nt);
 }
 
 int uva_arap(uva_plan* plan, double* U, const uva_arap_opts* o, void* stream) {

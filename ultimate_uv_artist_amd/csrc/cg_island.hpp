@@ -18,6 +18,12 @@
 // two rows' loads are in flight together.
 #pragma once
 
+// Residual reduction of one inner FP32 pass of the mixed-precision LSCM
+// solve: an FP32 recurrence cannot reach the FP64 target (1e-8) in one pass,
+// so each pass stops here and the next outer pass restarts from the FP64
+// true residual.
+constexpr double kInnerRel = 1e-4;
+
 struct CGOut {
     int iters;
     int conv;
@@ -113,6 +119,7 @@ UVA_HD __forceinline__ CGOut cg_island(Team& tm, const DevPlan& P, const Isl& d,
     V2* __restrict__ u = (V2*)P.vu;
     V2* __restrict__ w = (V2*)P.vw;
     const bool coupled = (OP == OP_LSCM);
+    constexpr bool kRelInner = OP == OP_LSCM && sizeof(V) == 4 && PT::kResid64;
     const double bx2 = coupled ? fmax(b2.x + b2.y, 1e-300) : fmax(b2.x, 1e-300);
     const double by2 = coupled ? bx2 : fmax(b2.y, 1e-300);
     const double t2 = tol * tol;
@@ -198,6 +205,7 @@ UVA_HD __forceinline__ CGOut cg_island(Team& tm, const DevPlan& P, const Isl& d,
         }
         if (relx <= t2 && rely <= t2) { out.conv = 1; break; }
         if (outer >= PT::kMaxOuter || it >= maxit || !__builtin_isfinite(relx) || !__builtin_isfinite(rely)) break;
+        const double rr0 = S1.x + S1.y;   // squared true residual at the start of this pass
         double gx = S1.z, gy = S1.w, dx = S2.x, dy = S2.y;
         if (coupled) { gx = gy = S1.z + S1.w; dx = dy = S2.x + S2.y; }
         double alx = cg_alpha<PREC>(gx, dx), aly = cg_alpha<PREC>(gy, dy);
@@ -277,6 +285,7 @@ UVA_HD __forceinline__ CGOut cg_island(Team& tm, const DevPlan& P, const Isl& d,
             double rrx = T1.x, rry = T1.y;
             if (coupled) rrx = rry = T1.x + T1.y;
             if ((rrx <= PT::kInnerFrac2 * t2 * bx2 && rry <= PT::kInnerFrac2 * t2 * by2) || it >= maxit) break;
+            if (kRelInner && rrx <= kInnerRel * kInnerRel * rr0) break;
             if (!__builtin_isfinite(rrx) || !__builtin_isfinite(rry)) break;   // a non-finite island stops (it is not converged)
             double ngx = T1.z, ngy = T1.w, ndx = T2.x, ndy = T2.y;
             if (coupled) { ngx = ngy = T1.z + T1.w; ndx = ndy = T2.x + T2.y; }

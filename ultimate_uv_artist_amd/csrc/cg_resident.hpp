@@ -22,6 +22,16 @@
 // member each apply a strided share of the aggregate's row of Ac^-1 (kept in
 // LDS for the whole solve) and one thread adds the shares in order.
 //
+// The same code solves the LSCM system (OP_LSCM): the operator couples the
+// two columns through the boundary terms, [[K, B], [-B, K]], so both columns
+// share one step length and one stop test; the two hard-pinned rows keep
+// r = u = w = 0; and the coarse operator is [[Kc, Bc], [-Bc, Kc]] over the
+// same aggregates (pin rows left out), whose inverse [[Pc, Qc], [-Qc, Pc]]
+// is applied from the rows of [Pc | Qc].  In mixed precision each inner
+// FP32 pass of the LSCM solve stops once it has reduced its residual by
+// kInnerRel (an FP32 recurrence cannot reach the FP64 target of 1e-8 in one
+// pass); the FP64 true residual of the next outer pass decides convergence.
+//
 // Members of a multi-workgroup team keep u for a window of rows that extends
 // H = min(bandwidth, kHalo) rows beyond each end of their own range.  They
 // publish the rows other members' windows cover to global memory, and after
@@ -45,20 +55,22 @@ UVA_HD __forceinline__ int res_vec_bytes(int rows, int halo) {
 // aggregates read different LDS banks
 UVA_HD __forceinline__ int coarse_stride(int nc) { return nc | 1; }
 constexpr int kPart = 8;   // threads sharing one row of the coarse mat-vec
-// rows of Ac^-1, the restricted residual of every aggregate, the member's
-// correction, the mat-vec shares and the member's slice sums (nsl slices)
-UVA_HD __forceinline__ int res_coarse_bytes(int nc, int na, int nsl) {
-    return nc > 0 ? align16(na * coarse_stride(nc) * 4) + (nc + na + na * kPart + nsl) * 16 : 0;
+// rows of Ac^-1 (twice as long for the coupled LSCM operator), the
+// restricted residual of every aggregate, the member's correction, the
+// mat-vec shares and the member's slice sums (nsl slices)
+UVA_HD __forceinline__ int res_coarse_bytes(int nc, int na, int nsl, bool coupled = false) {
+    return nc > 0 ? align16(na * coarse_stride(nc) * 4 * (coupled ? 2 : 1)) + (nc + na + na * kPart + nsl) * 16 : 0;
 }
 
-template <int PREC, int RPT, class Team>
+template <int PREC, int RPT, class Team, int OP = OP_ARAP>
 struct ResCG {
     using PT = Prec<PREC>;
     using V = typename PT::V;
     using V2 = typename Vec2<V>::type;
+    static constexpr bool kCoupled = OP == OP_LSCM;
     // matrix entries per slot whose loads are issued together (as many as the
     // registers hold beside the slots' vectors)
-    static constexpr int kG = RPT * (int)sizeof(V) <= 16 ? 2 : 1;
+    static constexpr int kG = RPT * (int)sizeof(V) * (kCoupled ? 2 : 1) <= 16 ? 2 : 1;
 
     Team& tm;
     const DevPlan& P;
@@ -69,6 +81,8 @@ struct ResCG {
     bool gfull;        // the bandwidth exceeds the window: publish every row
     bool coarse;       // additive coarse correction
     int g, nc, ncp, alo, na;   // g slices per aggregate, nc aggregates (row stride ncp); the member's are [alo, alo + na)
+    int crow;          // LDS stride of a row of the coarse inverse (ncp, or 2 ncp for [Pc | Qc])
+    int lp0, lp1;      // hard-pinned rows of the LSCM system (-1 for ARAP)
     V2* lu;     // LDS: u of the window, row - wlo
     V2* le;     // LDS: e of the member's rows
     V* ldv;     // LDS: D^-1
@@ -81,8 +95,8 @@ struct ResCG {
     V2* gu;     // global: u of the published rows (teams only)
     V2 r[RPT], p[RPT], s[RPT], w[RPT];
 
-    UVA_HD __forceinline__ ResCG(Team& t, const DevPlan& P_, const Isl& d_, bool use_coarse)
-        : tm(t), P(P_), d(d_) {
+    UVA_HD __forceinline__ ResCG(Team& t, const DevPlan& P_, const Isl& d_, bool use_coarse, int p0 = -1, int p1 = -1)
+        : tm(t), P(P_), d(d_), lp0(p0), lp1(p1) {
         W = tm.nwaves();
         wv = tm.wave();
         lane = tm.lane();
@@ -99,6 +113,7 @@ struct ResCG {
         g = d.cg;
         nc = (d.ns + g - 1) / g;
         ncp = coarse_stride(nc);
+        crow = kCoupled ? 2 * ncp : ncp;
         alo = slo / g;
         na = shi > slo ? (shi - 1) / g - alo + 1 : 0;
         const int rows = (shi - slo) * kSlice;
@@ -108,7 +123,7 @@ struct ResCG {
         ldv = (V*)(le + rows);
         ldg = ldv + rows;
         lci = (float*)(base + res_vec_bytes<V2, V>(rows, multi ? kHalo : 0));
-        lrc = (double2*)((char*)lci + align16(na * ncp * 4));
+        lrc = (double2*)((char*)lci + align16(na * crow * 4));
         ly = lrc + nc;
         lpt = ly + na;
         lcs = lpt + na * kPart;
@@ -118,19 +133,25 @@ struct ResCG {
             const int l = (j * W + wv) * kSlice + lane;
             const int row = rlo + l;
             if (slot(j) && row < rhi) {
-                ldv[l] = (V)dinv_of<OP_ARAP>(P, d, row);
-                ldg[l] = (V)diag_of<OP_ARAP>(P, d, row);
+                ldv[l] = (V)dinv_of<OP>(P, d, row);
+                ldg[l] = (V)diag_of<OP>(P, d, row);
             }
             r[j].x = 0; r[j].y = 0;
             p[j] = r[j]; s[j] = r[j]; w[j] = r[j];
         }
         if (coarse) {
-            const float* gci = P.cinv + d.cofs + (long long)alo * nc;
-            for (int i = tm.tid(); i < na * nc; i += tm.tstride()) lci[(i / nc) * ncp + i % nc] = gci[i];
+            // global rows: nc values (ARAP), or nc of Pc then nc of Qc (LSCM)
+            const int gl = kCoupled ? 2 * nc : nc;
+            const float* gci = kCoupled ? P.lcinv + d.lcofs + (long long)alo * gl : P.cinv + d.cofs + (long long)alo * nc;
+            for (int i = tm.tid(); i < na * gl; i += tm.tstride()) {
+                const int a = i / gl, b = i - a * gl;
+                lci[a * crow + (b < nc ? b : ncp + (b - nc))] = gci[i];
+            }
         }
         // each pass reads D^-1 and D of its own rows only, and the coarse
         // rows after a team barrier
     }
+    UVA_HD __forceinline__ bool is_pin(int row) const { return kCoupled && (row == lp0 || row == lp1); }
     // slot j exists for this wavefront (uniform)
     UVA_HD __forceinline__ bool slot(int j) const { return slo + j * W + wv < shi; }
     UVA_HD __forceinline__ int lrow(int j) const { return (j * W + wv) * kSlice + lane; }
@@ -189,13 +210,20 @@ struct ResCG {
         // share q of row a: columns q, q + kPart, ...
         for (int i = tm.tid(); i < na * kPart; i += tm.tstride()) {
             const int a = i / kPart, q = i - a * kPart;
-            const float* ci = lci + a * ncp;
+            const float* ci = lci + a * crow;
             double yx = 0.0, yy = 0.0;
             for (int b = q; b < nc; b += kPart) {
                 const double c = (double)ci[b];
                 const double2 v = lrc[b];
-                yx = fma(c, v.x, yx);
-                yy = fma(c, v.y, yy);
+                if (kCoupled) {
+                    // (Pc r_u + Qc r_v, -Qc r_u + Pc r_v)
+                    const double e = (double)ci[ncp + b];
+                    yx = fma(c, v.x, fma(e, v.y, yx));
+                    yy = fma(c, v.y, fma(-e, v.x, yy));
+                } else {
+                    yx = fma(c, v.x, yx);
+                    yy = fma(c, v.y, yy);
+                }
             }
             lpt[i] = make_double2(yx, yy);
         }
@@ -231,6 +259,7 @@ struct ResCG {
                 u.x += (V)y.x;
                 u.y += (V)y.y;
             }
+            if (is_pin(row)) { u.x = 0; u.y = 0; }
             publish(row, u);
             ru.x += acc_mul<PREC>(r[j].x, u.x);
             ru.y += acc_mul<PREC>(r[j].y, u.y);
@@ -279,6 +308,7 @@ struct ResCG {
             // form one straight run the compiler issues back to back
             int c[RPT][kG];
             V v[RPT][kG];
+            V bv[RPT][kG];
 #pragma unroll
             for (int j = 0; j < RPT; ++j) {
 #pragma unroll
@@ -287,6 +317,7 @@ struct ResCG {
                     const int o = in ? base[j] + (k0 + q) * kSlice + lane : 0;
                     c[j][q] = P.m_col[o];
                     v[j][q] = mval<V>(P, o);
+                    if (kCoupled) bv[j][q] = (V)P.m_b[o];
                     c[j][q] = in ? c[j][q] : wlo;
                 }
             }
@@ -296,8 +327,13 @@ struct ResCG {
                 for (int q = 0; q < kG; ++q) {
                     const bool in = ok[j] && k0 + q < wid[j];
                     const V2 x = gather(c[j][q]);
-                    w[j].x = in ? fma(v[j][q], x.x, w[j].x) : w[j].x;
-                    w[j].y = in ? fma(v[j][q], x.y, w[j].y) : w[j].y;
+                    if (kCoupled) {
+                        w[j].x = in ? fma(v[j][q], x.x, fma(bv[j][q], x.y, w[j].x)) : w[j].x;
+                        w[j].y = in ? fma(v[j][q], x.y, -fma(bv[j][q], x.x, -w[j].y)) : w[j].y;
+                    } else {
+                        w[j].x = in ? fma(v[j][q], x.x, w[j].x) : w[j].x;
+                        w[j].y = in ? fma(v[j][q], x.y, w[j].y) : w[j].y;
+                    }
                 }
             }
         }
@@ -305,6 +341,7 @@ struct ResCG {
 #pragma unroll
         for (int j = 0; j < RPT; ++j) {
             if (!ok[j]) continue;
+            if (is_pin(rlo + lrow(j))) { w[j].x = 0; w[j].y = 0; }
             const V2 uo = lu[rlo + lrow(j) - wlo];
             acc.x += acc_mul<PREC>(w[j].x, uo.x);
             acc.y += acc_mul<PREC>(w[j].y, uo.y);
@@ -313,7 +350,10 @@ struct ResCG {
     }
 
     UVA_HD __forceinline__ CGOut solve(double2 b2, double tol, int maxit) {
-        const double bx2 = fmax(b2.x, 1e-300), by2 = fmax(b2.y, 1e-300);
+        const double bx2 = kCoupled ? fmax(b2.x + b2.y, 1e-300) : fmax(b2.x, 1e-300);
+        const double by2 = kCoupled ? bx2 : fmax(b2.y, 1e-300);
+        // an inner FP32 pass of the coupled solve also stops at kInnerRel of its start
+        constexpr bool kRelInner = kCoupled && sizeof(V) == 4 && PT::kResid64;
         const double t2 = tol * tol;
         int it = 0;
         CGOut out{0, 0, 0.0};
@@ -333,8 +373,7 @@ struct ResCG {
                     const int sg = d.s0 + slo + j * W + wv;
                     double rx, ry;
                     if (PT::kResid64) {
-                        const double2 y =
-                            spmv_row<double, OP_ARAP>(P, row, sg, lane, P.X64, diag_of<OP_ARAP>(P, d, row));
+                        const double2 y = spmv_row<double, OP>(P, row, sg, lane, P.X64, diag_of<OP>(P, d, row));
                         const double2 bb = P.b64[row];
                         rx = bb.x - y.x;
                         ry = bb.y - y.y;
@@ -342,18 +381,23 @@ struct ResCG {
                         const int base = P.sl_ptr[sg];
                         const int width = (P.sl_ptr[sg + 1] - base) >> 6;
                         const double2 xr = P.X64[row];
-                        const float dgf = (float)diag_of<OP_ARAP>(P, d, row);
+                        const float dgf = (float)diag_of<OP>(P, d, row);
                         float yx = dgf * (float)xr.x, yy = dgf * (float)xr.y;
                         int o = base + lane;
                         for (int k = 0; k < width; ++k, o += kSlice) {
                             const double2 xc = P.X64[P.m_col[o]];
                             yx = fmaf(P.m_hi[o], (float)xc.x, yx);
                             yy = fmaf(P.m_hi[o], (float)xc.y, yy);
+                            if (kCoupled) {
+                                yx = fmaf(P.m_b[o], (float)xc.y, yx);
+                                yy = fmaf(-P.m_b[o], (float)xc.x, yy);
+                            }
                         }
                         const double2 bb = P.b64[row];
                         rx = (double)((float)bb.x - yx);
                         ry = (double)((float)bb.y - yy);
                     }
+                    if (is_pin(row)) { rx = 0.0; ry = 0.0; }
                     r[j].x = (V)rx;
                     r[j].y = (V)ry;
                     V2 zero;
@@ -373,11 +417,17 @@ struct ResCG {
             const double2 wu = spmv();
             double4 S1, S2;
             tm.sum8(a1, make_double4(wu.x, wu.y, 0.0, 0.0), S1, S2);
-            relx = S1.x / bx2;
-            rely = S1.y / by2;
+            if (kCoupled) {
+                relx = rely = (S1.x + S1.y) / bx2;
+            } else {
+                relx = S1.x / bx2;
+                rely = S1.y / by2;
+            }
             if (relx <= t2 && rely <= t2) { out.conv = 1; break; }
             if (outer >= PT::kMaxOuter || it >= maxit || !__builtin_isfinite(relx) || !__builtin_isfinite(rely)) break;
+            const double rr0 = S1.x + S1.y;   // squared true residual at the start of this pass
             double gx = S1.z, gy = S1.w, dx = S2.x, dy = S2.y;
+            if (kCoupled) { gx = gy = S1.z + S1.w; dx = dy = S2.x + S2.y; }
             double alx = cg_alpha<PREC>(gx, dx), aly = cg_alpha<PREC>(gy, dy);
             double bex = 0.0, bey = 0.0;
             bool first = true;
@@ -422,10 +472,13 @@ struct ResCG {
                 tm.sum8(g1, make_double4(wu2.x, wu2.y, 0.0, 0.0), T1, T2);
                 ++it;
                 first = false;
-                const double rrx = T1.x, rry = T1.y;
+                double rrx = T1.x, rry = T1.y;
+                if (kCoupled) rrx = rry = T1.x + T1.y;
                 if ((rrx <= PT::kInnerFrac2 * t2 * bx2 && rry <= PT::kInnerFrac2 * t2 * by2) || it >= maxit) break;
+                if (kRelInner && rrx <= kInnerRel * kInnerRel * rr0) break;
                 if (!__builtin_isfinite(rrx) || !__builtin_isfinite(rry)) break;
-                const double ngx = T1.z, ngy = T1.w, ndx = T2.x, ndy = T2.y;
+                double ngx = T1.z, ngy = T1.w, ndx = T2.x, ndy = T2.y;
+                if (kCoupled) { ngx = ngy = T1.z + T1.w; ndx = ndy = T2.x + T2.y; }
                 bex = gx > 0.0 ? ngx / gx : 0.0;
                 bey = gy > 0.0 ? ngy / gy : 0.0;
                 const double denx = ndx - (alx != 0.0 ? bex * ngx / alx : 0.0);
@@ -458,13 +511,14 @@ struct ResCG {
     }
 };
 
-// ARAP global-step CG of one island: the resident form when the member's
-// rows fit in the team's registers and LDS, the streaming form otherwise.
-// The coarse level is used when the plan has one for the island, `coarse` is
-// set and its rows fit in LDS beside the vectors.
-template <int PREC, class Team>
-UVA_HD __forceinline__ CGOut cg_arap(Team& tm, const DevPlan& P, const Isl& d, double2 b2, double tol, int maxit,
-                                     bool coarse) {
+// CG of one island (the ARAP global step, or the LSCM system with its hard
+// pins p0, p1): the resident form when the member's rows fit in the team's
+// registers and LDS, the streaming form otherwise.  The coarse level is used
+// when the plan has one for the island and operator, `coarse` is set and its
+// rows fit in LDS beside the vectors.
+template <int PREC, int OP, class Team>
+UVA_HD __forceinline__ CGOut cg_solve(Team& tm, const DevPlan& P, const Isl& d, double2 b2, double tol, int maxit,
+                                      bool coarse, int p0, int p1) {
     using V = typename Prec<PREC>::V;
     using V2 = typename Vec2<V>::type;
     // rows per lane the registers hold without spilling at 512 threads
@@ -487,11 +541,19 @@ UVA_HD __forceinline__ CGOut cg_arap(Team& tm, const DevPlan& P, const Isl& d, d
     const int rpt = (nsl + W - 1) / W;
     const int vec = res_vec_bytes<V2, V>(nsl * kSlice, tm.multi() ? kHalo : 0);
     if (tm.lvec() != nullptr && vec <= tm.lvec_bytes() && rpt <= kMaxRpt) {
+        constexpr bool cp = OP == OP_LSCM;
         const int nc = d.cg > 0 ? (d.ns + d.cg - 1) / d.cg : 0;
-        const bool co = coarse && d.cofs >= 0 && vec + res_coarse_bytes(nc, na, nsl) <= tm.lvec_bytes();
-        if (rpt <= 2) return ResCG<PREC, 2, Team>(tm, P, d, co).solve(b2, tol, maxit);
-        if (rpt <= 4) return ResCG<PREC, 4, Team>(tm, P, d, co).solve(b2, tol, maxit);
-        return ResCG<PREC, kMaxRpt, Team>(tm, P, d, co).solve(b2, tol, maxit);
+        const bool co = coarse && (cp ? d.lcofs : d.cofs) >= 0 &&
+                        vec + res_coarse_bytes(nc, na, nsl, cp) <= tm.lvec_bytes();
+        if (rpt <= 2) return ResCG<PREC, 2, Team, OP>(tm, P, d, co, p0, p1).solve(b2, tol, maxit);
+        if (rpt <= 4) return ResCG<PREC, 4, Team, OP>(tm, P, d, co, p0, p1).solve(b2, tol, maxit);
+        return ResCG<PREC, kMaxRpt, Team, OP>(tm, P, d, co, p0, p1).solve(b2, tol, maxit);
     }
-    return cg_island<PREC, OP_ARAP>(tm, P, d, b2, tol, maxit, -1, -1);
+    return cg_island<PREC, OP>(tm, P, d, b2, tol, maxit, p0, p1);
+}
+
+template <int PREC, class Team>
+UVA_HD __forceinline__ CGOut cg_arap(Team& tm, const DevPlan& P, const Isl& d, double2 b2, double tol, int maxit,
+                                     bool coarse) {
+    return cg_solve<PREC, OP_ARAP>(tm, P, d, b2, tol, maxit, coarse, -1, -1);
 }

@@ -412,8 +412,13 @@ struct Isl {
     int v0, nv, t0, nt, s0, ns, pin, bw;
     int cg;               // slices per coarse aggregate
     long long cofs;       // offset of the inverse coarse operator, -1 without one
+    long long lcofs;      // the same for the LSCM operator
     double wpin;
 };
+
+UVA_HD __forceinline__ long long uniform_ll(long long v) {
+    return ((long long)uniform_i((int)(v >> 32)) << 32) | (long long)(unsigned)uniform_i((int)(v & 0xffffffffLL));
+}
 
 UVA_HD __forceinline__ Isl load_isl(const DevPlan& P, int i) {
     Isl d;
@@ -423,8 +428,8 @@ UVA_HD __forceinline__ Isl load_isl(const DevPlan& P, int i) {
     d.pin = uniform_i(P.isl_pin[i]); d.wpin = uniform_d(P.isl_wpin[i]);
     d.bw = uniform_i(P.isl_bw[i]);
     d.cg = uniform_i(P.isl_cg[i]);
-    const long long co = P.isl_cofs[i];
-    d.cofs = ((long long)uniform_i((int)(co >> 32)) << 32) | (long long)(unsigned)uniform_i((int)(co & 0xffffffffLL));
+    d.cofs = uniform_ll(P.isl_cofs[i]);
+    d.lcofs = P.isl_lcofs ? uniform_ll(P.isl_lcofs[i]) : -1;
     return d;
 }
 
@@ -858,7 +863,7 @@ UVA_HD __forceinline__ void run_task(Team& tm, const DevPlan& P, int isl, const 
             P.X64[row] = make_double2(0.0, 0.0);
         }
         tm.sync();
-        const CGOut cg = cg_island<UVA_PREC_F64, OP_LSCM>(tm, P, d, make_double2(s.x, s.y), a.cg_tol, a.cg_max, p0, p1);
+        const CGOut cg = cg_solve<PREC, OP_LSCM>(tm, P, d, make_double2(s.x, s.y), a.cg_tol, a.cg_max, true, p0, p1);
         FOR_ROWS(tm, d, row, sg) {
             (void)sg;
             double2 x = P.X64[row];
@@ -1067,7 +1072,7 @@ __global__ __launch_bounds__(BS) void island_kernel(DevPlan P, TaskArgs a, int s
                              ? 2 * (BS / 64) * BlockTeam<BS>::kSlots : TeamWG<BS>::kLdsDoubles;
     __shared__ double red[kRed];
     __shared__ int s_task;
-    // vectors of the resident CG (kinds that solve the ARAP system only)
+    // vectors of the resident CG (kinds that solve the ARAP or LSCM system only)
     constexpr int kLv = lds_vec_bytes(KIND);
     __shared__ double lvec[kLv > 0 ? kLv / 8 : 1];
     char* const lv = kLv > 0 ? (char*)lvec : nullptr;
@@ -1174,6 +1179,8 @@ hipError_t launch_island_tasks(int kind, int prec, const DevPlan& P, const TaskA
         UVA_K(TASK_ARAP_ITER)
         UVA_K(TASK_CG)
         case TASK_PROLOGUE: return launch_kind<TASK_PROLOGUE, UVA_PREC_MIXED>(P, a, slot, cus, s);
+        // FP64 vectors: FP32 inner passes restarted from the FP64 residual need
+        // 1.6-3x the iterations on the conformal operator
         case TASK_LSCM: return launch_kind<TASK_LSCM, UVA_PREC_F64>(P, a, slot, cus, s);
         case TASK_LOCAL_RHS: return launch_kind<TASK_LOCAL_RHS, UVA_PREC_MIXED>(P, a, slot, cus, s);
         case TASK_MAX_STEP: return launch_kind<TASK_MAX_STEP, UVA_PREC_MIXED>(P, a, slot, cus, s);

@@ -33,11 +33,12 @@ enum TaskKind {
     TASK_STEP_BWD = 13,       // backward of one ARAP global step given the adjoint solution
 };
 
-// Kinds whose tasks solve the ARAP system (resident CG when the island fits).
-constexpr bool solves_arap(int kind) { return kind == TASK_ARAP_ITER || kind == TASK_CG; }
-constexpr int lds_vec_bytes(int kind) { return solves_arap(kind) ? kLdsVec : 0; }
+// Kinds whose tasks solve the ARAP or LSCM system (resident CG when the
+// island fits).
+constexpr bool solves_resident(int kind) { return kind == TASK_ARAP_ITER || kind == TASK_CG || kind == TASK_LSCM; }
+constexpr int lds_vec_bytes(int kind) { return solves_resident(kind) ? kLdsVec : 0; }
 constexpr int block_size(int kind, int prec) {
-    return solves_arap(kind) ? kBlockRes : (prec == UVA_PREC_F64 ? kBlockF64 : kBlock);
+    return solves_resident(kind) ? kBlockRes : (prec == UVA_PREC_F64 ? kBlockF64 : kBlock);
 }
 
 struct TaskArgs {

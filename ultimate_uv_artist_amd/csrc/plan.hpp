@@ -20,6 +20,7 @@ namespace uva {
 constexpr int kSlice = 64;          // rows per slice (= wavefront width)
 constexpr int kWaveMaxRows = 384;   // islands up to this many vertices run on one wavefront
 constexpr int kResSlices = 48;      // slices one workgroup keeps resident in the FP32 ARAP CG (8 waves x 6)
+constexpr int kResSlicesF64 = 32;   // the same for FP64 vectors (8 waves x 4): the LSCM solve
 constexpr int kMaxTeam = 16;        // most workgroups on one island
 constexpr int kCoarseN = 128;       // most aggregates of an island's coarse level (64 KB inverse)
 constexpr int kTeamMinRows = 2048;  // fewest rows per team member
@@ -118,6 +119,11 @@ struct DevPlan {
     const int64_t* isl_cofs;
     const int* isl_cg;           // slices per aggregate
     const float* cinv;
+    // the LSCM operator's coarse level over the same aggregates: island i's
+    // inverse [[Pc, Qc], [-Qc, Pc]] as nc rows of [Pc | Qc] at isl_lcofs[i]
+    // in lcinv (-1: none)
+    const int64_t* isl_lcofs;
+    const float* lcinv;
     double2* crc;
     int* queue;                  // task counters
     const int* order_group;      // group tasks: island ids, longest first

@@ -336,6 +336,96 @@ bool coarse_inverse(int nv, int g, const std::vector<int>& row_ptr, const std::v
     return true;
 }
 
+// Inverse of a dense SPD matrix A (n x n, row-major, overwritten by its
+// Cholesky factor) into S (full, exactly symmetric).  Returns false if A is
+// not numerically positive definite or the inverse is not finite.
+bool spd_inverse(std::vector<double>& A, int n, std::vector<double>& S) {
+    for (int j = 0; j < n; ++j) {
+        double* aj = &A[(size_t)j * n];
+        double dj = aj[j];
+        for (int k = 0; k < j; ++k) dj -= aj[k] * aj[k];
+        if (!(dj > 0.0) || !std::isfinite(dj)) return false;
+        const double ljj = std::sqrt(dj);
+        aj[j] = ljj;
+        for (int i = j + 1; i < n; ++i) {
+            double* ai = &A[(size_t)i * n];
+            double v = ai[j];
+            for (int k = 0; k < j; ++k) v -= ai[k] * aj[k];
+            ai[j] = v / ljj;
+        }
+    }
+    // M = L^-1 (lower, stored transposed: Mt[c][i] = M[i][c]), then S = M^T M
+    std::vector<double> Mt((size_t)n * n, 0.0);
+    for (int c = 0; c < n; ++c) {
+        double* mc = &Mt[(size_t)c * n];
+        mc[c] = 1.0 / A[(size_t)c * n + c];
+        for (int i = c + 1; i < n; ++i) {
+            const double* ai = &A[(size_t)i * n];
+            double v = 0.0;
+            for (int k = c; k < i; ++k) v -= ai[k] * mc[k];
+            mc[i] = v / ai[i];
+        }
+    }
+    S.assign((size_t)n * n, 0.0);
+    for (int a = 0; a < n; ++a)
+        for (int b = 0; b <= a; ++b) {
+            const double* ma = &Mt[(size_t)a * n];
+            const double* mb = &Mt[(size_t)b * n];
+            double v = 0.0;
+            for (int k = a; k < n; ++k) v += ma[k] * mb[k];
+            if (!std::isfinite(v)) return false;
+            S[(size_t)a * n + b] = v;
+            S[(size_t)b * n + a] = v;
+        }
+    return true;
+}
+
+// Coarse level of the LSCM operator of one island.  The operator acts on
+// (u, v) as [[K, B], [-B, K]] (K = Laplacian with its diagonal, B the
+// antisymmetric boundary coupling), restricted to the rows and columns that
+// are not hard pins: it is the real form of the Hermitian matrix K - iB.
+// With Z the indicator of the aggregates (g consecutive 64-row slices, pin
+// rows left out) for u and v alike, Ac = Z^T A Z = [[Kc, Bc], [-Bc, Kc]] has
+// the same form, and so has its inverse [[Pc, Qc], [-Qc, Pc]].  Writes row a
+// of [Pc | Qc] (2 nc FP32 values, Pc symmetric, Qc antisymmetric) per
+// aggregate.  Returns false if Ac is not numerically positive definite.
+bool lscm_coarse_inverse(int nv, int g, const std::vector<int>& row_ptr, const std::vector<int>& row_col,
+                         const std::vector<double>& row_val, const std::vector<double>& row_b,
+                         const std::vector<double>& diag, int lp0, int lp1, float* out) {
+    const int rows = kSlice * g;
+    const int nc = (nv + rows - 1) / rows;
+    const int n2 = 2 * nc;
+    std::vector<double> A((size_t)n2 * n2, 0.0);
+    auto pinned = [&](int i) { return i == lp0 || i == lp1; };
+    for (int i = 0; i < nv; ++i) {
+        if (pinned(i)) continue;
+        const int a = i / rows;
+        A[(size_t)a * n2 + a] += diag[i];
+        for (int q = row_ptr[i]; q < row_ptr[i + 1]; ++q) {
+            const int j = row_col[q];
+            if (pinned(j)) continue;
+            const int b = j / rows;
+            A[(size_t)a * n2 + b] += row_val[q];
+            A[(size_t)a * n2 + nc + b] += row_b[q];
+        }
+    }
+    for (int a = 0; a < nc; ++a)
+        for (int b = 0; b < nc; ++b) {
+            A[(size_t)(nc + a) * n2 + nc + b] = A[(size_t)a * n2 + b];
+            A[(size_t)(nc + a) * n2 + b] = -A[(size_t)a * n2 + nc + b];
+        }
+    std::vector<double> S;
+    if (!spd_inverse(A, n2, S)) return false;
+    for (int a = 0; a < nc; ++a)
+        for (int b = 0; b < nc; ++b) {
+            const double p = 0.5 * (S[(size_t)a * n2 + b] + S[(size_t)(nc + a) * n2 + nc + b]);
+            const double q = 0.5 * (S[(size_t)a * n2 + nc + b] - S[(size_t)(nc + a) * n2 + b]);
+            out[(size_t)a * n2 + b] = (float)p;
+            out[(size_t)a * n2 + nc + b] = (float)q;
+        }
+    return true;
+}
+
 template <class F>
 void parallel_for(int n, F fn) {
     int nth = (int)std::min<unsigned>(std::max(1u, std::thread::hardware_concurrency()), 16u);
@@ -444,6 +534,15 @@ std::string build_host_plan(HostPlan& hp, int64_t n_islands, const int64_t* vert
             if (coarse && nc >= 2) { hp.isl_cofs[i] = c; c += nc * nc; }
         }
         hp.cinv.assign(std::max<int64_t>(c, 1), 0.0f);
+        // the LSCM operator's coarse level (islands with a boundary): rows of [Pc | Qc]
+        hp.isl_lcofs.assign(n, -1);
+        int64_t c2 = 0;
+        for (int i = 0; i < n; ++i) {
+            const int ns = (int)widths[i].size();
+            const int64_t nc = (ns + hp.isl_cg[i] - 1) / hp.isl_cg[i];
+            if (coarse && nc >= 2 && isl[i].lp0 >= 0) { hp.isl_lcofs[i] = c2; c2 += 2 * nc * nc; }
+        }
+        hp.lcinv.assign(std::max<int64_t>(c2, 1), 0.0f);
     }
     hp.tv0.resize(nt_all); hp.tv1.resize(nt_all); hp.tv2.resize(nt_all);
     hp.fx1.resize(nt_all); hp.fx2.resize(nt_all); hp.fy2.resize(nt_all); hp.farea.resize(nt_all); hp.fok.resize(nt_all);
@@ -522,6 +621,10 @@ std::string build_host_plan(HostPlan& hp, int64_t n_islands, const int64_t* vert
         if (hp.isl_cofs[i] >= 0 && !coarse_inverse(I.nv, hp.isl_cg[i], I.row_ptr, I.row_col, I.row_val, I.diag, lp,
                                                    wp, hp.cinv.data() + hp.isl_cofs[i]))
             hp.isl_cofs[i] = -1;
+        if (hp.isl_lcofs[i] >= 0 &&
+            !lscm_coarse_inverse(I.nv, hp.isl_cg[i], I.row_ptr, I.row_col, I.row_val, I.row_b, I.diag, I.lp0, I.lp1,
+                                 hp.lcinv.data() + hp.isl_lcofs[i]))
+            hp.isl_lcofs[i] = -1;
         // sliced ELL
         for (int s = 0; s < (int)widths[i].size(); ++s) {
             int base = hp.sl_ptr[slice_start[i] + s];
@@ -564,7 +667,7 @@ std::string build_host_plan(HostPlan& hp, int64_t n_islands, const int64_t* vert
 }
 
 void plan_teams(HostPlan& hp, const std::vector<int>& group_islands, int slots, int n_xcd, bool teams,
-                const double* est_iters) {
+                const double* est_iters, int res_slices) {
     hp.order_group.clear(); hp.grp_mem.clear(); hp.grp_m.clear(); hp.grp_team.clear(); hp.slot_ptr.clear();
     hp.n_teams = 0;
     hp.max_m = 1;
@@ -589,7 +692,7 @@ void plan_teams(HostPlan& hp, const std::vector<int>& group_islands, int slots, 
         double work = 0.0;
         for (int k = 0; k < n; ++k) {
             const int i = group_islands[k];
-            m[k] = std::max(1, std::min(mcap, (hp.isl_ns[i] + kResSlices - 1) / kResSlices));
+            m[k] = std::max(1, std::min(mcap, (hp.isl_ns[i] + res_slices - 1) / res_slices));
             work += m[k] * t_of(i, m[k]);
         }
         // islands longer than the ideal makespan get members while that pays

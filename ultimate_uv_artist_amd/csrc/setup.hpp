@@ -59,6 +59,11 @@ struct HostPlan {
     std::vector<int64_t> isl_cofs;
     std::vector<int> isl_cg;      // slices per aggregate
     std::vector<float> cinv;
+    // coarse level of the LSCM operator over the same aggregates (for the
+    // plan's LSCM pins): isl_lcofs[i] is the offset of island i's nc rows of
+    // [Pc | Qc] (2 nc FP32 values each) in lcinv, or -1
+    std::vector<int64_t> isl_lcofs;
+    std::vector<float> lcinv;
 };
 
 // Returns an empty string on success, else an error message.
@@ -77,8 +82,10 @@ std::string build_host_plan(HostPlan& hp, int64_t n_islands, const int64_t* vert
 // least-loaded XCD; within an XCD, longest first, an island's m members take
 // the m slots that free up first and start together.  Every slot lists its
 // tasks in one global order, so no two teams wait on each other.
+// res_slices: slices one member keeps resident (kResSlices for the FP32 ARAP
+// CG, kResSlicesF64 for the FP64 LSCM solve).
 void plan_teams(HostPlan& hp, const std::vector<int>& group_islands, int slots, int n_xcd, bool teams,
-                const double* est_iters);
+                const double* est_iters, int res_slices = kResSlices);
 
 // One island holding a general symmetric matrix (no triangles, no pin).
 std::string build_matrix_plan(HostPlan& hp, int64_t n, const int64_t* indptr, const int64_t* indices,
