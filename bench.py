@@ -316,7 +316,6 @@ def batch_study(args):
     b = gpu.Batch(items, dev=dev, dtype=torch.float64)
     b.orient_positive()
     b.repair_flips(80)
-    sysm = b._system()
     torch.cuda.synchronize()
     t_setup = time.perf_counter() - t0
     U0 = b._U.clone()
@@ -341,17 +340,21 @@ def batch_study(args):
     for _ in range(5):
         x.copy_(U0)
         e0.record(s)
-        its = sysm.pcg(rhs, x, 40, check_every=1, tol2=0.0, strict=True)
+        its = b._plan.pcg(rhs, x, 40, check_every=1, tol2=0.0, strict=True)
         e1.record(s)
         torch.cuda.synchronize()
         cg_ms.append(e0.elapsed_time(e1))
     cg_t = float(np.median(cg_ms)) / 1e3
-    n, nnz = sysm.n, sysm.nnz
-    # algorithmic bytes of one iteration (two float64 columns): SpMV 8 B value
-    # + 4 B column id per stored entry, p gathered once, Ap written (12 nnz +
-    # 32 n); update reads D, p, Ap, x, r and writes x, r, z (120 n); direction
-    # reads z, p and writes p (48 n).  The sliced-ELL padding is not counted.
-    per_it = 12 * nnz + 200 * n
+    info = b._plan.info()
+    n, nnz_off = int(info["n_vertices"]), int(info["nnz_true"])
+    nnz = nnz_off + n
+    # algorithmic bytes of one iteration of uva_plan_pcg (two float64
+    # columns): the SpMV launch reads 12 B per stored off-diagonal entry
+    # (value hi + lo, column id), diagonal and pin weight (16 n), z gathered
+    # once (16 n), p and A p read and written (64 n); the update launch reads
+    # p, A p, x, r, diagonal and pin weight and writes x, r, z (128 n).  The
+    # sliced-ELL padding is not counted.
+    per_it = 12 * nnz_off + 224 * n
     achieved = per_it * its / cg_t / 1e9
     # CPU: the oracle's Batch with the same coupled CG on 16 islands, one core
     k = min(16, len(items))

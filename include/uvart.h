@@ -40,6 +40,8 @@
  *   uva_overlap_flags   validate.py:94-136 _overlaps, 257-299 _overlaps_strict
  *   uva_jacobi_pcg      hier.py:565-577 (the CG of Solver.lscm)
  *   uva_csr_pcg         gpu.py:184-202 Batch._cg; hier.py:465-474, 565-577
+ *   uva_plan_pcg        gpu.py:184-202 Batch._cg; hier.py:465-474 (the
+ *                       CG of Solver.relax), on a plan's own matrix
  */
 #ifndef UVART_H
 #define UVART_H
@@ -189,6 +191,17 @@ int uva_plan_set_tuning(uva_plan *plan, const double *cost, void *stream);
  * every chart of a batch that one CG solves together.  Turns the coarse level
  * off (Jacobi-preconditioned CG). */
 int uva_plan_set_pins(uva_plan *plan, const int64_t *pins, int64_t n_pins, double w, void *stream);
+/* The Jacobi-PCG of uva_csr_pcg (same step lengths, stop test and
+ * iteration count; two columns) over every row of the plan at once: its
+ * islands' Laplacians side by side with the per-row soft pins of
+ * uva_plan_set_pins (required), so the islands share the step lengths as the
+ * reference's batched CG over a whole Batch or hier subset does (gpu.py:
+ * 184-202 Batch._cg, hier.py:465-474).  b and x: device arrays n x 2 float64
+ * in the caller's vertex order; x is the start and the result.  *iters_out
+ * (may be NULL, then the call does not wait for the stream) gets the
+ * iterations run.  Two launches per iteration. */
+int uva_plan_pcg(uva_plan *plan, const double *b, double *x, int32_t max_iters, int32_t check_every, double tol2,
+                 int32_t strict, int32_t *iters_out, void *stream);
 /* Set the per-triangle target scale (gpu.py:123 Batch.scale); device array. */
 int uva_plan_set_scale(uva_plan *plan, const double *scale_dev, void *stream);
 

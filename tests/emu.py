@@ -47,6 +47,7 @@ def lib():
         _lib.uva_emu_energy_bwd.argtypes = [C.c_void_p, _dp, _dp, C.c_int32, _dp, _dp, _dp]
         _lib.uva_emu_step_bwd.argtypes = [C.c_void_p, _dp, _dp, _dp]
         _lib.uva_emu_adjoint_cg.argtypes = [C.c_void_p, _dp, _dp, C.c_double, C.c_int32]
+        _lib.uva_emu_plan_pcg.argtypes = [C.c_void_p, _dp, _dp, C.c_int32, C.c_int32, C.c_double, C.c_int32]
         _lib.uva_emu_vperm.argtypes = [C.c_void_p, C.POINTER(C.c_int32)]
         _lib.uva_emu_lscm_coarse.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_float), C.c_int64,
                                              C.POINTER(C.c_int32)]
@@ -147,6 +148,14 @@ class EmuPlan:
         o = L.SlimOpts(iters, cg_max_iters, tol, cg_tol)
         lib().uva_emu_slim(self.h, U.ctypes.data_as(_dp), C.byref(o))
         return U
+
+    def plan_pcg(self, b, x0, max_iters, check_every=1, tol2=0.0, strict=False):
+        """uva_plan_pcg's iteration on the host: (x, iterations run)."""
+        b = np.ascontiguousarray(np.array(b, np.float64))
+        x = np.ascontiguousarray(np.array(x0, np.float64))
+        its = lib().uva_emu_plan_pcg(self.h, b.ctypes.data_as(_dp), x.ctypes.data_as(_dp), int(max_iters),
+                                     int(check_every), float(tol2), int(bool(strict)))
+        return x, its
 
     def vperm(self):
         out = np.zeros(self.n, np.int32)

@@ -439,6 +439,68 @@ def test_emu_chart_pins_coupled_cg():
     assert np.abs(x2 - xr).max() > 1e-6 * np.abs(xr).max()
 
 
+def _batch_pcg_reference(L, rhs, x0, max_iters, check_every, tol2, strict):
+    """Plain Jacobi-PCG with the reference's step lengths and stop test
+    (gpu.py:184-202): (x, iterations run)."""
+    dinv = 1.0 / np.maximum(L.diagonal(), 1e-30)[:, None]
+    x = x0.copy()
+    r = rhs - L @ x
+    z = r * dinv
+    p = z.copy()
+    rz = (r * z).sum(0)
+    b2 = np.maximum((rhs * rhs).sum(0), 1e-30)
+    for it in range(max_iters):
+        Ap = L @ p
+        al = rz / np.maximum((p * Ap).sum(0), 1e-30)
+        x = x + p * al
+        r = r - Ap * al
+        z = r * dinv
+        rr = (r * r).sum(0)
+        if it % check_every == 0 and (np.all(rr < tol2 * b2) if strict else np.all(rr <= tol2 * b2)):
+            return x, it + 1
+        rz2 = (r * z).sum(0)
+        p = z + p * (rz2 / np.maximum(rz, 1e-30))
+        rz = rz2
+    return x, max_iters
+
+
+@pytest.mark.parametrize("cg_iters,tol,strict", [(20, 0.0, True), (40, 1e-6, False), (400, 1e-9, False)])
+def test_emu_plan_pcg_matches_the_batch_cg(cg_iters, tol, strict):
+    """uva_plan_pcg (the global step of gpu.Batch.arap and hier.Solver.relax
+    on the chart plan's own matrix: SpMV on z, A p by recurrence) against
+    plain Jacobi-PCG on the batch's pinned Laplacian assembled by scipy from
+    the reference's triplets (gpu.py:126-142): three charts in one plan, a
+    soft pin at every chart's first vertex, a fixed count (hier), the Batch
+    stop rule at 40 iterations, and a run to convergence."""
+    import scipy.sparse as sp
+
+    from oracle import hier_cpu
+    from oracle import param_cpu as oracle
+    from ultimate_uv_artist_amd import workloads
+
+    rng = np.random.default_rng(64)
+    charts = [workloads.disk_island(rng, t) for t in (300, 800, 1400)]
+    U0 = np.concatenate([oracle.lscm(P, T) for P, T in charts])
+    e = EmuPlan(charts, batch=True)
+    starts = e.vo[:-1]
+    e.set_pins(starts, 0.0)
+    rhs = e.local_rhs(U0)
+    P = np.concatenate([c[0] for c in charts])
+    T = np.concatenate([c[1] + e.vo[i] for i, c in enumerate(charts)])
+    _, w, *_ = hier_cpu.frames(P, T)
+    W = np.concatenate([w[:, 0], w[:, 1], w[:, 2]])
+    I = np.concatenate([T[:, 1], T[:, 2], T[:, 0]])
+    J = np.concatenate([T[:, 2], T[:, 0], T[:, 1]])
+    wpin = max(np.abs(W).max(), 1e-12)
+    n = len(P)
+    L = sp.csr_matrix((np.concatenate([-W, -W, W, W, np.full(len(starts), wpin)]),
+                       (np.concatenate([I, J, I, J, starts]), np.concatenate([J, I, I, J, starts]))), shape=(n, n))
+    xr, itr = _batch_pcg_reference(L, rhs, U0, cg_iters, 1, tol * tol, strict)
+    x, its = e.plan_pcg(rhs, U0, cg_iters, check_every=1, tol2=tol * tol, strict=strict)
+    assert its == itr
+    assert np.abs(x - xr).max() <= 1e-10 * np.abs(xr).max()
+
+
 def test_emu_folded_starts_run_more_iterations():
     """charts.flatten's ARAP (charts.py:411-415) for several charts in one
     call: a chart whose start is folded runs iters_folded iterations without

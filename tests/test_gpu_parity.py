@@ -776,6 +776,86 @@ def test_csr_pcg_matches_the_definition():
     S.close()
 
 
+def test_plan_pcg_matches_the_definition():
+    """uva_plan_pcg (IslandPlan.pcg, the global step of gpu.Batch.arap and
+    hier.Solver.relax on the chart plan's own matrix) on 40 disk charts with
+    a soft pin at every chart's first vertex: solves in a row on one plan --
+    the Batch rule at 40 iterations, a fixed count (hier), a run to 1e-9 --
+    with eager torch work between them, each against Jacobi-PCG on the
+    pinned Laplacian that scipy assembles from the reference's triplets
+    (iterations equal, x to 1e-10 relative).  The run to convergence takes
+    about 2,000 iterations, over which the recurrence for A p and the sum
+    order move the iteration that crosses the threshold by one or two: there
+    the iterations agree to 0.5 % and x meets the stop rule on the true
+    residual."""
+    import scipy.sparse as sp
+    import torch
+
+    from oracle import hier_cpu
+    from ultimate_uv_artist_amd import workloads
+    from ultimate_uv_artist_amd.plan import IslandPlan
+
+    rng = np.random.default_rng(93)
+    charts = [workloads.disk_island(rng, int(t)) for t in rng.integers(200, 6000, 40)]
+    dev = torch.device("cuda", 0)
+    plan = IslandPlan(charts, batch=True, dev=dev, no_coarse=True)
+    voff = plan.vert_off
+    plan.set_pins(voff[:-1], 0.0)
+    P = np.concatenate([c[0] for c in charts])
+    T = np.concatenate([c[1] + voff[i] for i, c in enumerate(charts)])
+    _, w, *_ = hier_cpu.frames(P, T)
+    W = np.concatenate([w[:, 0], w[:, 1], w[:, 2]])
+    I = np.concatenate([T[:, 1], T[:, 2], T[:, 0]])
+    J = np.concatenate([T[:, 2], T[:, 0], T[:, 1]])
+    n = len(P)
+    L = sp.csr_matrix((np.concatenate([-W, -W, W, W, np.full(len(charts), max(np.abs(W).max(), 1e-12))]),
+                       (np.concatenate([I, J, I, J, voff[:-1]]), np.concatenate([J, I, I, J, voff[:-1]]))),
+                      shape=(n, n))
+    dinv = 1.0 / np.maximum(L.diagonal(), 1e-30)[:, None]
+
+    def pcg(B, x, iters, tol2, strict):
+        x = x.copy()
+        r = B - L @ x
+        z = r * dinv
+        p = z.copy()
+        rz = (r * z).sum(0)
+        b2 = np.maximum((B * B).sum(0), 1e-30)
+        for k in range(iters):
+            Ap = L @ p
+            al = rz / np.maximum((p * Ap).sum(0), 1e-30)
+            x, r = x + al * p, r - al * Ap
+            rr = (r * r).sum(0)
+            if ((rr < tol2 * b2) if strict else (rr <= tol2 * b2)).all():
+                return x, k + 1
+            z = r * dinv
+            rz2 = (r * z).sum(0)
+            p = z + rz2 / np.maximum(rz, 1e-30) * p
+            rz = rz2
+        return x, iters
+
+    cases = [(40, 1e-12, False)] * 2 + [(20, 0.0, True), (2000, 1e-18, False)]
+    for k, (iters, tol2, strict) in enumerate(cases):
+        print(f"plan pcg case {k}: iters {iters}, tol2 {tol2}", flush=True)
+        B = rng.normal(size=(n, 2)) * 10.0 ** rng.integers(-2, 3)
+        X0 = rng.normal(size=(n, 2))
+        b = torch.as_tensor(B, device=dev)
+        x = torch.as_tensor(X0, device=dev).clone()
+        its = plan.pcg(b, x, iters, 1, tol2, strict)
+        junk = (x * 2.0).sum()                           # eager work between solves
+        torch.cuda.synchronize()
+        assert torch.isfinite(junk)
+        xr, kr = pcg(B, X0, iters, tol2, strict)
+        xg = x.cpu().numpy()
+        if kr < iters and kr > 200:
+            assert abs(its - kr) <= max(2, 0.005 * kr), (k, its, kr)
+            rel = np.linalg.norm(B - L @ xg, axis=0) / np.linalg.norm(B, axis=0)
+            assert rel.max() <= 1.01 * np.sqrt(tol2), (k, rel)
+        else:
+            assert its == kr, (k, its, kr)
+            assert np.abs(xg - xr).max() <= 1e-10 * np.abs(xr).max(), k
+    plan.close()
+
+
 def test_gutter_on_gpu_matches_reference_and_oracle():
     """validate_uvs with the gutter (the default) and min_gutter: the
     reference's own entries at atlas 64 and 256 and its label images at 128

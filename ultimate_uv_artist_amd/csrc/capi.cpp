@@ -71,6 +71,8 @@ struct uva_plan {
     bool have_lsched = false;
     std::vector<int> lmembers;             // per island: workgroups of its LSCM team
     bool last_lscm = false;                // the latest solve call was uva_lscm
+    double* pcg_scratch = nullptr;         // block partials and state of uva_plan_pcg
+    int pcg_grid = 0;
     std::vector<int> vinv;                 // caller's vertex id -> plan vertex
     uva::HostPlan sched;                   // island sizes and schedule inputs (for uva_plan_tune)
     int team_alloc = 0;                    // teams the barrier arrays hold
@@ -388,6 +390,7 @@ int upload_plan(uva::HostPlan& hp, const double* P, hipStream_t s, uva_plan** ou
     d.n_vert = hp.n_vert;
     d.n_tri = hp.n_tri;
     d.mode_batch = hp.batch;
+    d.n_slices = hp.sl_ptr.empty() ? 0 : (int)hp.sl_ptr.size() - 1;
     Arena ar;
 #define UP(field, vec) ar.put(&d.field, hp.vec)
     UP(isl_v0, isl_v0); UP(isl_nv, isl_nv); UP(isl_t0, isl_t0); UP(isl_nt, isl_nt);
@@ -395,7 +398,7 @@ int upload_plan(uva::HostPlan& hp, const double* P, hipStream_t s, uva_plan** ou
     UP(tv0, tv0); UP(tv1, tv1); UP(tv2, tv2);
     UP(fx1, fx1); UP(fx2, fx2); UP(fy2, fy2); UP(farea, farea); UP(fok, fok);
     UP(inc_ptr, inc_ptr); UP(inc_tri, inc_tri);
-    UP(sl_ptr, sl_ptr); UP(m_col, m_col); UP(m_hi, m_hi); UP(m_lo, m_lo); UP(m_b, m_b);
+    UP(sl_ptr, sl_ptr); UP(sl_row, sl_row); UP(m_col, m_col); UP(m_hi, m_hi); UP(m_lo, m_lo); UP(m_b, m_b);
     UP(diag, diag); UP(dinv32, dinv32);
     UP(adj_ptr, adj_ptr); UP(adj_idx, adj_idx); UP(vbnd, vbnd);
     UP(order_group, order_group); UP(order_wave, order_wave);
@@ -787,6 +790,38 @@ int uva_plan_set_tuning(uva_plan* plan, const double* cost, void* stream) {
         if (!(cost[k] > 0.0) || !std::isfinite(cost[k])) return fail(UVA_EINVAL, "costs must be positive and finite");
     for (int k = 0; k < 3; ++k) plan->sched.cost[k] = cost[k];
     return replan(plan, plan->est.empty() ? nullptr : plan->est.data(), (hipStream_t)stream);
+}
+
+int uva_plan_pcg(uva_plan* plan, const double* b, double* x, int32_t max_iters, int32_t check_every, double tol2,
+                 int32_t strict, int32_t* iters_out, void* stream) {
+    if (!valid_plan(plan) || !b || !x) return fail(UVA_EINVAL, "null argument");
+    if (max_iters < 0 || check_every < 1 || !(tol2 >= 0.0))
+        return fail(UVA_EINVAL, "max_iters must be >= 0, check_every >= 1 and tol2 >= 0");
+    uva::DevPlan& d = plan->dp;
+    if (!d.rpin) return fail(UVA_EINVAL, "uva_plan_pcg needs per-row pins (uva_plan_set_pins)");
+    if (iters_out) *iters_out = 0;
+    if (max_iters == 0 || d.n_vert == 0) return UVA_OK;
+    hipStream_t s = (hipStream_t)stream;
+    plan->mem = s;
+    if (!plan->pcg_scratch) {
+        plan->pcg_grid = uva::plan_pcg_grid(d, plan->cus);
+        if (plan->alloc(&plan->pcg_scratch, uva::plan_pcg_scratch_doubles(plan->pcg_grid)) != hipSuccess)
+            return fail(UVA_ENOMEM, "plan PCG scratch");
+    }
+    int rc = UVA_OK;
+    const double2* bp = to_plan(plan, b, 1, s, rc);
+    double2* xp = to_plan(plan, x, 0, s, rc);
+    if (rc) return rc;
+    HIP_TRY(uva::launch_plan_pcg(d, bp, xp, (double2*)d.vr, (double2*)d.vu, (double2*)d.vp, (double2*)d.vAp,
+                                 plan->pcg_scratch, plan->pcg_grid, max_iters, check_every, tol2, strict, s));
+    rc = from_plan(plan, 0, x, s, UVA_OK);
+    if (rc || !iters_out) return rc;
+    int its = 0;
+    HIP_TRY(hipMemcpyAsync(&its, uva::plan_pcg_iters_ptr(plan->pcg_scratch, plan->pcg_grid), sizeof(int),
+                           hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    *iters_out = its;
+    return UVA_OK;
 }
 
 int uva_plan_set_scale(uva_plan* plan, const double* scale_dev, void* stream) {

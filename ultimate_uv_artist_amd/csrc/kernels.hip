@@ -837,6 +837,7 @@ UVA_HD __forceinline__ void run_task(Team& tm, const DevPlan& P, int isl, const 
             }
         }
     } else if (KIND == TASK_LSCM) {
+        const long long t_start = uva_clock();
         const int p0 = uniform_i(P.isl_lpin0[isl]), p1 = uniform_i(P.isl_lpin1[isl]);
         if (p0 < 0) {
             if (tm.leader()) st->status = -5;
@@ -877,6 +878,7 @@ UVA_HD __forceinline__ void run_task(Team& tm, const DevPlan& P, int isl, const 
             st->converged = cg.conv;
             st->rel_resid = cg.rel;
             st->status = 0;
+            st->ticks = uva_clock() - t_start;
         }
         tm.sync();
     } else if (KIND == TASK_LOCAL_RHS) {

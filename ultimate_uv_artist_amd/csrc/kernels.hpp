@@ -68,6 +68,16 @@ hipError_t launch_fill_int(int* p, int n, int v, hipStream_t s);
 // out[b] = hardware XCD id of block b (blocks of 64 threads)
 hipError_t launch_xcc_probe(int* out, int blocks, hipStream_t s);
 hipError_t launch_signed_areas(const DevPlan& P, const double2* U, double* sa, hipStream_t s);
+
+// Jacobi-PCG over every row of a plan (pcg.hip): workgroups for a device of
+// `cus` CUs, scratch doubles they need, the launches of one solve (vectors in
+// plan order; r, z, p, Ap work vectors), and where the iteration count lands.
+int plan_pcg_grid(const DevPlan& P, int cus);
+size_t plan_pcg_scratch_doubles(int grid);
+hipError_t launch_plan_pcg(const DevPlan& P, const double2* b, double2* x, double2* r, double2* z, double2* p,
+                           double2* Ap, double* scratch, int grid, int max_iters, int check_every, double tol2,
+                           int strict, hipStream_t s);
+int* plan_pcg_iters_ptr(double* scratch, int grid);
 // caller-order vertex array -> plan order (gather), or back (scatter)
 hipError_t launch_permute(const DevPlan& P, const double2* src, double2* dst, bool to_plan, hipStream_t s);
 

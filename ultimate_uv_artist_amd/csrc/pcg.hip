@@ -37,8 +37,10 @@
 #include <vector>
 
 #include "../../include/uvart.h"
+#include "plan_pcg.hpp"
 
 namespace uva {
+
 namespace pcg {
 
 constexpr int kB = 256;        // rows per workgroup
@@ -292,6 +294,200 @@ int solve(uva_csr* m, const double* b, double* x, int max_iters, int check_every
 }
 
 }  // namespace pcg
+
+namespace ppcg {
+
+// fixed-order sum of the G block partials (K values each) into every thread
+template <int K>
+__device__ inline void sum_parts(const double* part, int G, double* sh, double (&out)[K]) {
+    double u[K];
+    for (int k = 0; k < K; ++k) u[k] = 0.0;
+    for (int q = threadIdx.x; q < G; q += kT)
+        for (int k = 0; k < K; ++k) u[k] += part[K * q + k];
+    for (int k = 0; k < K; ++k) sh[k * kT + threadIdx.x] = u[k];
+    __syncthreads();
+    for (int h = kT / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h)
+            for (int k = 0; k < K; ++k) sh[k * kT + threadIdx.x] += sh[k * kT + threadIdx.x + h];
+        __syncthreads();
+    }
+    for (int k = 0; k < K; ++k) out[k] = sh[k * kT];
+    __syncthreads();
+}
+
+// block sum of K values per thread into part[K * blockIdx.x + k]
+template <int K>
+__device__ inline void block_part(const double (&v)[K], double* sh, double* part) {
+    for (int k = 0; k < K; ++k) sh[k * kT + threadIdx.x] = v[k];
+    __syncthreads();
+    for (int h = kT / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h)
+            for (int k = 0; k < K; ++k) sh[k * kT + threadIdx.x] += sh[k * kT + threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < K; ++k) part[K * blockIdx.x + k] = sh[k * kT];
+}
+
+// Rows of this workgroup: slices blockIdx.x * 8 + wave, then every 8 gridDim.x.
+#define PPCG_ROWS(P, s, lane, row)                                                                     \
+    for (int s = blockIdx.x * kSlicesPerWG + (int)(threadIdx.x >> 6); s < (P).n_slices;                \
+         s += gridDim.x * kSlicesPerWG)                                                                \
+        for (int lane = (int)(threadIdx.x & 63), row = (P).sl_row[s] + lane, _once = 1;                \
+             _once && row < (P).sl_row[s + 1]; _once = 0)
+
+// r = b - (A + pins) x, z = r / max(diag, 1e-30); block r.z and b.b
+__global__ __launch_bounds__(kT) void init_kernel(DevPlan P, const double2* __restrict__ b,
+                                                   const double2* __restrict__ x, double2* __restrict__ r,
+                                                   double2* __restrict__ z, double* partB, State* st, int max_iters) {
+    __shared__ double sh[4 * kT];
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    PPCG_ROWS(P, s, lane, row) {
+        const double2 ax = row_mv(P, s, lane, row, x);
+        const double2 bi = b[row];
+        const double di = 1.0 / fmax(pinned_diag(P, row), 1e-30);
+        const double rx = bi.x - ax.x, ry = bi.y - ax.y;
+        const double zx = rx * di, zy = ry * di;
+        r[row] = make_double2(rx, ry);
+        z[row] = make_double2(zx, zy);
+        v[0] += rx * zx;
+        v[1] += ry * zy;
+        v[2] += bi.x * bi.x;
+        v[3] += bi.y * bi.y;
+    }
+    block_part<4>(v, sh, partB);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st->done = 0;
+        st->its = max_iters;
+    }
+}
+
+// K1(it): see the namespace comment
+__global__ __launch_bounds__(kT) void spmv_kernel(DevPlan P, const double2* __restrict__ z, double2* __restrict__ p,
+                                                   double2* __restrict__ Ap, const double* partB, double* partA,
+                                                   State* st, int it, int check_every, double tol2, int strict) {
+    __shared__ double sh[4 * kT];
+    if (st->done) return;
+    double red[4];
+    sum_parts<4>(partB, gridDim.x, sh, red);
+    double rz[2], beta[2] = {0.0, 0.0};
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    if (it == 0) {
+        rz[0] = red[0];
+        rz[1] = red[1];
+        if (lead) {
+            st->b2[0] = red[2] > 1e-30 ? red[2] : 1e-30;
+            st->b2[1] = red[3] > 1e-30 ? red[3] : 1e-30;
+        }
+    } else {
+        const double rr[2] = {red[0], red[1]};
+        const double b2[2] = {st->b2[0], st->b2[1]};
+        if ((it - 1) % check_every == 0 && stop_now(rr, b2, tol2, strict)) {
+            if (lead) {
+                st->done = 1;
+                st->its = it;
+            }
+            return;
+        }
+        rz[0] = red[2];
+        rz[1] = red[3];
+        for (int c = 0; c < 2; ++c) {
+            const double prev = st->rz[(it - 1) & 1][c];
+            beta[c] = rz[c] / (prev > 1e-30 ? prev : 1e-30);
+        }
+    }
+    if (lead) {
+        st->rz[it & 1][0] = rz[0];
+        st->rz[it & 1][1] = rz[1];
+    }
+    double v[2] = {0.0, 0.0};
+    PPCG_ROWS(P, s, lane, row) {
+        const double2 q = row_mv(P, s, lane, row, z);
+        const double2 zi = z[row];
+        double2 pi, api;
+        if (it == 0) {
+            pi = zi;
+            api = q;
+        } else {
+            const double2 po = p[row], ao = Ap[row];
+            pi = make_double2(zi.x + beta[0] * po.x, zi.y + beta[1] * po.y);
+            api = make_double2(q.x + beta[0] * ao.x, q.y + beta[1] * ao.y);
+        }
+        p[row] = pi;
+        Ap[row] = api;
+        v[0] += pi.x * api.x;
+        v[1] += pi.y * api.y;
+    }
+    block_part<2>(v, sh, partA);
+}
+
+// K2(it): see the namespace comment
+__global__ __launch_bounds__(kT) void update_kernel(DevPlan P, const double2* __restrict__ p,
+                                                     const double2* __restrict__ Ap, double2* __restrict__ x,
+                                                     double2* __restrict__ r, double2* __restrict__ z,
+                                                     const double* partA, double* partB, const State* st, int it) {
+    __shared__ double sh[4 * kT];
+    if (st->done) return;
+    double red[2];
+    sum_parts<2>(partA, gridDim.x, sh, red);
+    double al[2];
+    for (int c = 0; c < 2; ++c) al[c] = st->rz[it & 1][c] / (red[c] > 1e-30 ? red[c] : 1e-30);
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    PPCG_ROWS(P, s, lane, row) {
+        const double2 pi = p[row], api = Ap[row];
+        double2 xi = x[row];
+        xi.x += al[0] * pi.x;
+        xi.y += al[1] * pi.y;
+        x[row] = xi;
+        double2 ri = r[row];
+        ri.x -= al[0] * api.x;
+        ri.y -= al[1] * api.y;
+        r[row] = ri;
+        const double di = 1.0 / fmax(pinned_diag(P, row), 1e-30);
+        const double zx = ri.x * di, zy = ri.y * di;
+        z[row] = make_double2(zx, zy);
+        v[0] += ri.x * ri.x;
+        v[1] += ri.y * ri.y;
+        v[2] += ri.x * zx;
+        v[3] += ri.y * zy;
+    }
+    block_part<4>(v, sh, partB);
+}
+#undef PPCG_ROWS
+
+}  // namespace ppcg
+
+// Workgroups of the plan PCG on a device of `cus` compute units: two per CU
+// at most, one per eight slices at least.
+int plan_pcg_grid(const DevPlan& P, int cus) {
+    const int need = (P.n_slices + ppcg::kSlicesPerWG - 1) / ppcg::kSlicesPerWG;
+    const int cap = 2 * (cus > 0 ? cus : 1);
+    return need < cap ? (need > 0 ? need : 1) : cap;
+}
+
+size_t plan_pcg_scratch_doubles(int grid) { return (size_t)grid * 6 + sizeof(ppcg::State) / sizeof(double) + 1; }
+
+// The launches of one plan PCG (x, b, the four work vectors in plan order;
+// scratch: plan_pcg_scratch_doubles(grid) doubles).  Returns the launch error.
+hipError_t launch_plan_pcg(const DevPlan& P, const double2* b, double2* x, double2* r, double2* z, double2* p,
+                           double2* Ap, double* scratch, int grid, int max_iters, int check_every, double tol2,
+                           int strict, hipStream_t s) {
+    using namespace ppcg;
+    double* partA = scratch;
+    double* partB = scratch + 2 * (size_t)grid;
+    State* st = (State*)(scratch + 6 * (size_t)grid);
+    hipLaunchKernelGGL(init_kernel, dim3(grid), dim3(kT), 0, s, P, b, x, r, z, partB, st, max_iters);
+    for (int it = 0; it < max_iters; ++it) {
+        hipLaunchKernelGGL(spmv_kernel, dim3(grid), dim3(kT), 0, s, P, z, p, Ap, partB, partA, st, it, check_every,
+                           tol2, strict);
+        hipLaunchKernelGGL(update_kernel, dim3(grid), dim3(kT), 0, s, P, p, Ap, x, r, z, partA, partB, st, it);
+    }
+    return hipGetLastError();
+}
+
+// iterations the last plan PCG ran (the State at the end of the scratch)
+int* plan_pcg_iters_ptr(double* scratch, int grid) { return &((ppcg::State*)(scratch + 6 * (size_t)grid))->its; }
+
 }  // namespace uva
 
 static void csr_free(uva_csr* m) {

@@ -80,6 +80,8 @@ struct DevPlan {
     const double2* inc_g;        // b[v] += R(tri) g
     // matrix
     const int* sl_ptr;           // n_slices + 1
+    const int* sl_row;           // first row of each slice; sl_row[s + 1] - sl_row[s] rows (n_slices + 1)
+    int n_slices;
     const int* m_col;
     const float* m_hi;
     const float* m_lo;

@@ -559,6 +559,12 @@ std::string build_host_plan(HostPlan& hp, int64_t n_islands, const int64_t* vert
         for (int i = 0; i < n; ++i)
             for (int w : widths[i]) { hp.sl_ptr[s] = (int)off; off += (int64_t)w * kSlice; ++s; }
         hp.sl_ptr[nslices] = (int)off;
+        // first row of every slice (islands' rows are contiguous, so the next
+        // slice's first row ends this one)
+        hp.sl_row.assign(nslices + 1, (int)nv_all);
+        for (int i = 0; i < n; ++i)
+            for (int q = 0; q < (int)widths[i].size(); ++q)
+                hp.sl_row[slice_start[i] + q] = (int)vert_off[i] + q * kSlice;
     }
     // prefix of adjacency
     {
@@ -842,6 +848,10 @@ std::string build_matrix_plan(HostPlan& hp, int64_t n, const int64_t* indptr, co
     hp.inc_ptr.assign(nv + 1, 0);
     hp.vperm.resize(nv);
     for (int i = 0; i < nv; ++i) hp.vperm[i] = i;
+    hp.sl_row.assign(ns + 1, nv);
+    for (int q = 0; q < ns; ++q) hp.sl_row[q] = q * kSlice;
+    hp.isl_lcofs.assign(1, -1);   // no LSCM system on a general matrix
+    hp.lcinv.assign(1, 0.0f);
     // coarse level over the 64-row slices of the given order
     hp.isl_cofs.assign(1, -1);
     const int g = (ns + kCoarseN - 1) / kCoarseN;

@@ -23,6 +23,7 @@ struct HostPlan {
     std::vector<double> inc_g;          // 2 per incidence
     std::vector<double> tri_g;          // the same corner vectors triangle-major: 6 per triangle
     std::vector<int> sl_ptr, m_col;
+    std::vector<int> sl_row;            // first row of each slice (n_slices + 1, the last = n_vert)
     std::vector<float> m_hi, m_lo, m_b;
     std::vector<double> diag;
     std::vector<float> dinv32;

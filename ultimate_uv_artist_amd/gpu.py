@@ -20,7 +20,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .plan import CsrSystem, IslandPlan, batch_frames_dev, pinned_laplacian, require_gpu
+from .plan import IslandPlan, batch_frames_dev, pinned_laplacian, require_gpu
 
 BATCH_CG_TOL = 1e-6   # gpu.py:184 Batch._cg(tol=1e-6)
 
@@ -86,11 +86,13 @@ class Batch:
         self.nc = len(charts)
         self.offsets = np.cumsum([0] + [len(p) for p, _, _ in charts])
         self.n = int(self.offsets[-1])
-        # the per-chart work; the global step runs on the resident Laplacian, so
-        # the plan's own CG (and its coarse level) is not used
+        # the per-chart work, and the global step: one Jacobi-PCG over every
+        # row of the plan with a soft pin at every chart's first vertex of the
+        # batch-global weight max |W| (the islands' own pin weight under Batch
+        # conventions; gpu.py:133-141); the per-island CG and its coarse level
+        # are not used
         self._plan = IslandPlan([(p, t) for p, t, _ in charts], batch=True, dev=self.dev, no_coarse=True)
         self._P = np.concatenate([np.asarray(p, np.float64).reshape(-1, 3) for p, _, _ in charts])
-        self._sys = None
         self.last_cg_iters = 0
         U = np.concatenate([np.asarray(u, np.float64).reshape(-1, 2) for _, _, u in charts])
         self._U = torch.as_tensor(U, device=self.dev).contiguous()
@@ -103,6 +105,7 @@ class Batch:
         self._scale = torch.ones(len(self.T), device=self.dev, dtype=torch.float64)
         self._frames = None
         self._A = None
+        self._plan.set_pins(self.offsets[:-1], 0.0)
 
     # ---------------------------------------------------------------- layout
     # The layout lives in FP64 on the device and is updated in place by the
@@ -195,14 +198,11 @@ class Batch:
     def _system(self):
         """The batch Laplacian (block-diagonal over the charts) with a soft pin
         at every chart's first vertex of the batch-global weight max |W|
-        (gpu.py:126-142), resident on the GPU for the coupled CG; the chart
-        plan's rhs takes the same pin weight."""
-        if self._sys is None:
-            A, wpin = pinned_laplacian(self._P, self._Tg, self.offsets[:-1], dev=self.dev)
-            self._sys = CsrSystem(A, self.dev)
-            self._plan.set_pins(self.offsets[:-1], wpin)
-            self._A, self._wpin = A, wpin
-        return self._sys
+        (gpu.py:126-142), assembled on the GPU for the attributes L, dinv and
+        wpin only: the solves run on the chart plan's own matrix."""
+        if self._A is None:
+            self._A, self._wpin = pinned_laplacian(self._P, self._Tg, self.offsets[:-1], dev=self.dev)
+        return self._A
 
     def arap(self, iters=8, line_search=True, cg_iters=40, cg_tol=BATCH_CG_TOL):
         """ARAP iterations of every chart (gpu.py:230-249): rotations x scale
@@ -210,14 +210,13 @@ class Batch:
         layout, stopped when both columns reach relative residual ``cg_tol``
         or after ``cg_iters`` iterations (gpu.py:184-202), revert of
         non-finite charts, per-chart flip-safe step when ``line_search``."""
-        system = self._system()
         U = self._U.clone()
         for _ in range(int(iters)):
             b = self._plan.local_rhs(U, _lib.UVA_MODE_BATCH)
             Un = U.clone()
             if cg_iters > 0:                                   # no CG iteration: the layout stays
-                self.last_cg_iters = system.pcg(b, Un, int(cg_iters), check_every=1, tol2=float(cg_tol) ** 2,
-                                                strict=False)
+                self.last_cg_iters = self._plan.pcg(b, Un, int(cg_iters), check_every=1,
+                                                    tol2=float(cg_tol) ** 2, strict=False)
             bad = ~torch.isfinite(Un).all(1)
             if bool(bad.any()):
                 badc = torch.zeros(self.nc, dtype=torch.bool, device=self.dev)

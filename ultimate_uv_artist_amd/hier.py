@@ -44,7 +44,7 @@ import scipy.sparse as sp
 import torch
 
 from . import _lib
-from .plan import CsrSystem, IslandPlan, half_cots, pinned_laplacian, require_gpu, stream_of
+from .plan import IslandPlan, half_cots, require_gpu, stream_of
 
 
 class Solver:
@@ -104,16 +104,13 @@ class Solver:
             self._q = None
             return U0.astype(np.float32).astype(np.float64), flips, scale
         # per-chart plan: orientation, repair, flips, rotations and rhs, step
-        # limits; the subset's Laplacian on the GPU: the global step, one CG
-        # over every chart as the reference solves them (shared step lengths),
-        # one soft pin per chart with the subset's weight max |W|
+        # limits, and the global step: one Jacobi-PCG over every row of the
+        # plan as the reference solves the subset (shared step lengths), one
+        # soft pin per chart with the subset's weight max |W| (the islands'
+        # own pin weight under Batch conventions)
         plan = IslandPlan(charts, batch=True, dev=self.dev, no_coarse=True)    # its own CG is not used
         voff = plan.vert_off
-        Pall = np.concatenate([c[0] for c in charts])
-        Tall = plan.T_local + np.repeat(voff[:-1], np.diff(plan.tri_off))[:, None]
-        A, wpin = pinned_laplacian(Pall, Tall, voff[:-1], dev=self.dev)
-        system = CsrSystem(A, self.dev)
-        plan.set_pins(voff[:-1], wpin)
+        plan.set_pins(voff[:-1], 0.0)
         cv = torch.as_tensor(np.repeat(np.arange(len(charts)), np.diff(voff)), device=self.dev)
         # the reference holds the layout in float32 (hier.py:400)
         U = plan.device_uv(U0[vorder].astype(np.float32).astype(np.float64))
@@ -123,7 +120,7 @@ class Solver:
             b = plan.local_rhs(U, _lib.UVA_MODE_BATCH)
             Un = U.clone()
             if cg_iters > 0:                                   # exactly cg_iters iterations; none: the layout stays
-                system.pcg(b, Un, int(cg_iters), check_every=1, tol2=0.0, strict=True)
+                plan.pcg(b, Un, int(cg_iters), check_every=1, tol2=0.0, strict=True, want_iters=False)
             bad = ~torch.isfinite(Un).all(1)
             if bool(bad.any()):                              # hier.py:500-503
                 badc = torch.zeros(len(charts), dtype=torch.bool, device=self.dev)
@@ -138,7 +135,7 @@ class Solver:
                 plan.repair_jacobi(U, repair_rounds)
         plan.repair_jacobi(U, repair_rounds)
         U.copy_(U.float().double())                          # the reference returns float32 values
-        q = self._quality(plan, Pall, U, len(charts))
+        q = self._quality(plan, np.concatenate([c[0] for c in charts]), U, len(charts))
         out = np.empty((nv, 2))
         out[vorder] = U.cpu().numpy()
         flips[keep] = q["flips"]

@@ -533,6 +533,20 @@ class IslandPlan:
         check(_lib.lib().uva_plan_set_pins(self._h, i64p(p), len(p), float(w), stream_of(self.dev)),
               "uva_plan_set_pins")
 
+    def pcg(self, b, x, max_iters, check_every=1, tol2=0.0, strict=False, want_iters=True):
+        """The reference's Jacobi-PCG over every row of the plan at once (its
+        islands' Laplacians with the pins of ``set_pins``, two columns sharing
+        nothing but the launches; see uva_plan_pcg).  x (n,2) float64 device
+        tensor: start in, result out, for rhs b of the same shape.  Returns the
+        iterations run (None with ``want_iters`` False: no wait for the stream)."""
+        self._check_uv(x)
+        self._check_uv(b)
+        its = C.c_int32(0)
+        check(_lib.lib().uva_plan_pcg(self._h, ptr(b), ptr(x), int(max_iters), int(check_every), float(tol2),
+                                      int(bool(strict)), C.byref(its) if want_iters else None,
+                                      stream_of(self.dev)), "uva_plan_pcg")
+        return its.value if want_iters else None
+
     def set_scale(self, scale):
         s = scale.to(self.dev, torch.float64).contiguous()
         if s.numel() != self.t:
