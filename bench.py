@@ -350,11 +350,11 @@ def batch_study(args):
     nnz = nnz_off + n
     # algorithmic bytes of one iteration of uva_plan_pcg (two float64
     # columns): the SpMV launch reads 12 B per stored off-diagonal entry
-    # (value hi + lo, column id), diagonal and pin weight (16 n), z gathered
-    # once (16 n), p and A p read and written (64 n); the update launch reads
-    # p, A p, x, r, diagonal and pin weight and writes x, r, z (128 n).  The
+    # (value hi + lo, column id), the pinned diagonal (8 n), z gathered once
+    # (16 n), p and A p read and written (64 n); the update launch reads p,
+    # A p, x, r and the pinned diagonal and writes x, r, z (120 n).  The
     # sliced-ELL padding is not counted.
-    per_it = 12 * nnz_off + 224 * n
+    per_it = 12 * nnz_off + 208 * n
     achieved = per_it * its / cg_t / 1e9
     # CPU: the oracle's Batch with the same coupled CG on 16 islands, one core
     k = min(16, len(items))

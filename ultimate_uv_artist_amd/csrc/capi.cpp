@@ -859,17 +859,22 @@ int uva_plan_set_pins(uva_plan* plan, const int64_t* pins, int64_t n_pins, doubl
     std::vector<float> dinv((size_t)std::max(n, 1), 1.0f);
     if (n) HIP_TRY(hipMemcpyAsync(diag.data(), d.diag, n * sizeof(double), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
+    std::vector<double> pd((size_t)std::max(n, 1), 0.0);
     for (int r = 0; r < n; ++r) {
         const double g = diag[r] + rp[r];
+        pd[r] = g;
         dinv[r] = (float)(g > 1e-30 ? 1.0 / g : 1.0);
     }
     if (!d.rpin) {
-        double* p = nullptr;
-        if (plan->alloc(&p, (size_t)std::max(n, 1)) != hipSuccess) return fail(UVA_ENOMEM, "pin weights");
+        double *p = nullptr, *q = nullptr;
+        if (plan->alloc(&p, (size_t)std::max(n, 1)) != hipSuccess || plan->alloc(&q, (size_t)std::max(n, 1)) != hipSuccess)
+            return fail(UVA_ENOMEM, "pin weights");
         d.rpin = p;
+        d.pdiag = q;
     }
     if (n) {
         HIP_TRY(hipMemcpyAsync((void*)d.rpin, rp.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync((void*)d.pdiag, pd.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
         HIP_TRY(hipMemcpyAsync((void*)d.dinv32, dinv.data(), n * sizeof(float), hipMemcpyHostToDevice, s));
     }
     // the coarse inverses hold the islands' own pins: Jacobi only from here on

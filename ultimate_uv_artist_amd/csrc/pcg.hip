@@ -457,11 +457,13 @@ __global__ __launch_bounds__(kT) void update_kernel(DevPlan P, const double2* __
 
 }  // namespace ppcg
 
-// Workgroups of the plan PCG on a device of `cus` compute units: two per CU
-// at most, one per eight slices at least.
+// Workgroups of the plan PCG on a device of `cus` compute units: four per CU
+// at most (32 wavefronts per CU hide the gathers' latency; every workgroup
+// reads all block partials, so more would cost L2 traffic), one per eight
+// slices at least.
 int plan_pcg_grid(const DevPlan& P, int cus) {
     const int need = (P.n_slices + ppcg::kSlicesPerWG - 1) / ppcg::kSlicesPerWG;
-    const int cap = 2 * (cus > 0 ? cus : 1);
+    const int cap = 4 * (cus > 0 ? cus : 1);
     return need < cap ? (need > 0 ? need : 1) : cap;
 }
 

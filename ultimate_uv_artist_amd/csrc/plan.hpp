@@ -88,6 +88,7 @@ struct DevPlan {
     const float* m_b;            // LSCM boundary coupling B_ij (exact in fp32)
     const double* diag;          // Laplacian diagonal without pin
     const double* rpin;          // per-row soft-pin weights replacing the islands' pins (NULL = unused)
+    const double* pdiag;         // diag + rpin (set with rpin)
     const float* dinv32;         // 1 / (diag + pin) for the ARAP operator
     // adjacency (unique neighbours) for Jacobi flip repair; boundary flags
     const int* adj_ptr;

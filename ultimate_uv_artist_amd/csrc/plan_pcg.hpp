@@ -36,7 +36,7 @@ struct State {
     int done, its;
 };
 
-__host__ __device__ inline double pinned_diag(const DevPlan& P, int row) { return P.diag[row] + P.rpin[row]; }
+__host__ __device__ inline double pinned_diag(const DevPlan& P, int row) { return P.pdiag[row]; }
 
 // y = (A + pins) v of `row` (lane `lane` of slice s): the diagonal, then the
 // row's entries in column order; padding entries past the slice's width of
