@@ -391,6 +391,7 @@ int upload_plan(uva::HostPlan& hp, const double* P, hipStream_t s, uva_plan** ou
     d.n_tri = hp.n_tri;
     d.mode_batch = hp.batch;
     d.n_slices = hp.sl_ptr.empty() ? 0 : (int)hp.sl_ptr.size() - 1;
+    d.m_zero = hp.sl_ptr.empty() ? 0 : hp.sl_ptr.back();
     Arena ar;
 #define UP(field, vec) ar.put(&d.field, hp.vec)
     UP(isl_v0, isl_v0); UP(isl_nv, isl_nv); UP(isl_t0, isl_t0); UP(isl_nt, isl_nt);

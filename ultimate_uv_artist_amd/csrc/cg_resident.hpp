@@ -83,15 +83,18 @@ struct ResCG {
     int g, nc, ncp, alo, na;   // g slices per aggregate, nc aggregates (row stride ncp); the member's are [alo, alo + na)
     int crow;          // LDS stride of a row of the coarse inverse (ncp, or 2 ncp for [Pc | Qc])
     int lp0, lp1;      // hard-pinned rows of the LSCM system (-1 for ARAP)
-    V2* lu;     // LDS: u of the window, row - wlo
-    V2* le;     // LDS: e of the member's rows
-    V* ldv;     // LDS: D^-1
-    V* ldg;     // LDS: D (with the pin weight)
-    float* lci;     // LDS: rows alo .. alo + na - 1 of Ac^-1
-    double2* lrc;   // LDS: Z^T r of every aggregate
-    double2* ly;    // LDS: the member's aggregates' coarse correction
-    double2* lpt;   // LDS: kPart shares of each of the member's mat-vec rows
-    double2* lcs;   // LDS: Z^T r of the member's slices (one workgroup)
+    // LDS pointers are typed as such (UVA_LDS): the accesses are then LDS
+    // instructions with 32-bit addresses, not generic (flat) ones, whose
+    // waits also drain every global load in flight
+    V2 UVA_LDS* lu;     // LDS: u of the window, row - wlo
+    V2 UVA_LDS* le;     // LDS: e of the member's rows
+    V UVA_LDS* ldv;     // LDS: D^-1
+    V UVA_LDS* ldg;     // LDS: D (with the pin weight)
+    float UVA_LDS* lci;     // LDS: rows alo .. alo + na - 1 of Ac^-1
+    double2 UVA_LDS* lrc;   // LDS: Z^T r of every aggregate
+    double2 UVA_LDS* ly;    // LDS: the member's aggregates' coarse correction
+    double2 UVA_LDS* lpt;   // LDS: kPart shares of each of the member's mat-vec rows
+    double2 UVA_LDS* lcs;   // LDS: Z^T r of the member's slices (one workgroup)
     V2* gu;     // global: u of the published rows (teams only)
     V2 r[RPT], p[RPT], s[RPT], w[RPT];
 
@@ -117,13 +120,13 @@ struct ResCG {
         alo = slo / g;
         na = shi > slo ? (shi - 1) / g - alo + 1 : 0;
         const int rows = (shi - slo) * kSlice;
-        char* base = (char*)tm.lvec();
-        lu = (V2*)base;
+        char UVA_LDS* base = (char UVA_LDS*)tm.lvec();
+        lu = (V2 UVA_LDS*)base;
         le = lu + rows + 2 * (multi ? kHalo : 0);
-        ldv = (V*)(le + rows);
+        ldv = (V UVA_LDS*)(le + rows);
         ldg = ldv + rows;
-        lci = (float*)(base + res_vec_bytes<V2, V>(rows, multi ? kHalo : 0));
-        lrc = (double2*)((char*)lci + align16(na * crow * 4));
+        lci = (float UVA_LDS*)(base + res_vec_bytes<V2, V>(rows, multi ? kHalo : 0));
+        lrc = (double2 UVA_LDS*)((char UVA_LDS*)lci + align16(na * crow * 4));
         ly = lrc + nc;
         lpt = ly + na;
         lcs = lpt + na * kPart;
@@ -210,7 +213,7 @@ struct ResCG {
         // share q of row a: columns q, q + kPart, ...
         for (int i = tm.tid(); i < na * kPart; i += tm.tstride()) {
             const int a = i / kPart, q = i - a * kPart;
-            const float* ci = lci + a * crow;
+            const float UVA_LDS* ci = lci + a * crow;
             double yx = 0.0, yy = 0.0;
             for (int b = q; b < nc; b += kPart) {
                 const double c = (double)ci[b];
@@ -267,11 +270,6 @@ struct ResCG {
         return ru;
     }
 
-    UVA_HD __forceinline__ V2 gather(int c) const {
-        if (gfull && (c < wlo || c >= whi)) return gu[c];
-        return lu[c - wlo];
-    }
-
     // w = A u over the member's rows (own u from the window, columns from the
     // window or, beyond it, global memory) and the dot w.u.  The matrix loads
     // of every slot are issued together, kG entries per slot at a time, so a
@@ -302,10 +300,19 @@ struct ResCG {
             w[j].x = dgj * uo.x;
             w[j].y = dgj * uo.y;
         }
+        const unsigned ulane = (unsigned)lane;
+        const unsigned zero = (unsigned)P.m_zero + ulane;
         for (int k0 = 0; k0 < wmax; k0 += kG) {
-            // every lane loads (entries past a row's width read entry 0 of the
-            // matrix and are left out by select), so the loads of all slots
-            // form one straight run the compiler issues back to back
+            // every lane loads, so the loads of all slots form one straight
+            // run the compiler issues back to back; offsets are unsigned.  A
+            // step past a slot's width reads the zero entries after the last
+            // slice and weighs the window's first row by 0: the FMAs run
+            // unconditionally (no branch around each gather, no select on a
+            // loaded value that would make the scheduler wait for it early).
+            // A non-finite u there would make that 0 weight non-finite, but u
+            // is the island's own preconditioned residual, and a non-finite
+            // value in it already makes the next step length, and so every
+            // row of the island, non-finite.
             int c[RPT][kG];
             V v[RPT][kG];
             V bv[RPT][kG];
@@ -314,25 +321,41 @@ struct ResCG {
 #pragma unroll
                 for (int q = 0; q < kG; ++q) {
                     const bool in = ok[j] && k0 + q < wid[j];
-                    const int o = in ? base[j] + (k0 + q) * kSlice + lane : 0;
+                    const unsigned o = in ? (unsigned)(base[j] + (k0 + q) * kSlice) + ulane : zero;
                     c[j][q] = P.m_col[o];
                     v[j][q] = mval<V>(P, o);
                     if (kCoupled) bv[j][q] = (V)P.m_b[o];
                     c[j][q] = in ? c[j][q] : wlo;
                 }
             }
+            // columns beyond the window (teams whose bandwidth exceeds the
+            // halo): their global loads are issued first, all together, and
+            // only by the lanes that need them; every other gather is an LDS
+            // read
+            V2 xf[RPT][kG];
+            bool far[RPT][kG];
 #pragma unroll
             for (int j = 0; j < RPT; ++j) {
 #pragma unroll
                 for (int q = 0; q < kG; ++q) {
-                    const bool in = ok[j] && k0 + q < wid[j];
-                    const V2 x = gather(c[j][q]);
+                    far[j][q] = gfull && (c[j][q] < wlo || c[j][q] >= whi);
+                    xf[j][q].x = 0;
+                    xf[j][q].y = 0;
+                    if (far[j][q]) xf[j][q] = gu[c[j][q]];
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < RPT; ++j) {
+#pragma unroll
+                for (int q = 0; q < kG; ++q) {
+                    const V2 xn = lu[far[j][q] ? 0 : c[j][q] - wlo];
+                    const V2 x = far[j][q] ? xf[j][q] : xn;
                     if (kCoupled) {
-                        w[j].x = in ? fma(v[j][q], x.x, fma(bv[j][q], x.y, w[j].x)) : w[j].x;
-                        w[j].y = in ? fma(v[j][q], x.y, -fma(bv[j][q], x.x, -w[j].y)) : w[j].y;
+                        w[j].x = fma(v[j][q], x.x, fma(bv[j][q], x.y, w[j].x));
+                        w[j].y = fma(v[j][q], x.y, -fma(bv[j][q], x.x, -w[j].y));
                     } else {
-                        w[j].x = in ? fma(v[j][q], x.x, w[j].x) : w[j].x;
-                        w[j].y = in ? fma(v[j][q], x.y, w[j].y) : w[j].y;
+                        w[j].x = fma(v[j][q], x.x, w[j].x);
+                        w[j].y = fma(v[j][q], x.y, w[j].y);
                     }
                 }
             }

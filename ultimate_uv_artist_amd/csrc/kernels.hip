@@ -28,6 +28,14 @@ namespace uva {
 // is already uniform and the device clock reads 0.
 #define UVA_HD __host__ __device__
 
+// Pointers into LDS carry the LDS address space on the device, so their
+// accesses compile to LDS instructions (the CPU emulator has no such space).
+#if defined(__HIP_DEVICE_COMPILE__)
+#define UVA_LDS __attribute__((address_space(3)))
+#else
+#define UVA_LDS
+#endif
+
 UVA_HD __forceinline__ int uniform_i(int v) {
 #if defined(__HIP_DEVICE_COMPILE__)
     return __builtin_amdgcn_readfirstlane(v);
@@ -170,11 +178,11 @@ template <int BS>
 struct BlockTeam {
     static constexpr int kWaves = BS / 64;
     static constexpr int kSlots = 8;   // doubles per wavefront slot
-    double* lds;  // 2 * kWaves * kSlots doubles
+    double UVA_LDS* lds;  // 2 * kWaves * kSlots doubles
     int par;
     char* lv;     // LDS vector buffer of the workgroup (or NULL)
     int lvb;
-    __device__ BlockTeam(double* l, char* v, int vb) : lds(l), par(0), lv(v), lvb(vb) {}
+    __device__ BlockTeam(double UVA_LDS* l, char* v, int vb) : lds(l), par(0), lv(v), lvb(vb) {}
     __device__ void* lvec() const { return lv; }
     __device__ int lvec_bytes() const { return lvb; }
     __device__ bool multi() const { return false; }
@@ -201,10 +209,10 @@ struct BlockTeam {
     __device__ void sum8(double4 a, double4 b, double4& sa, double4& sb) {
         a.x = wave_sum(a.x); a.y = wave_sum(a.y); a.z = wave_sum(a.z); a.w = wave_sum(a.w);
         b.x = wave_sum(b.x); b.y = wave_sum(b.y); b.z = wave_sum(b.z); b.w = wave_sum(b.w);
-        double* buf = lds + par * kWaves * kSlots;
+        double UVA_LDS* buf = lds + par * kWaves * kSlots;
         par ^= 1;
         if (lane() == 0) {
-            double* q = buf + kSlots * wave();
+            double UVA_LDS* q = buf + kSlots * wave();
             q[0] = a.x; q[1] = a.y; q[2] = a.z; q[3] = a.w;
             q[4] = b.x; q[5] = b.y; q[6] = b.z; q[7] = b.w;
         }
@@ -212,7 +220,7 @@ struct BlockTeam {
         double4 s = make_double4(0.0, 0.0, 0.0, 0.0), t = make_double4(0.0, 0.0, 0.0, 0.0);
 #pragma unroll
         for (int w = 0; w < kWaves; ++w) {
-            const double* q = buf + kSlots * w;
+            const double UVA_LDS* q = buf + kSlots * w;
             s.x += q[0]; s.y += q[1]; s.z += q[2]; s.w += q[3];
             t.x += q[4]; t.y += q[5]; t.z += q[6]; t.w += q[7];
         }
@@ -222,7 +230,7 @@ struct BlockTeam {
     __device__ double2 lane_sum2(double2 v) const { return make_double2(wave_sum(v.x), wave_sum(v.y)); }
     __device__ double min1(double v) {
         v = wave_min(v);
-        double* buf = lds + par * kWaves * kSlots;
+        double UVA_LDS* buf = lds + par * kWaves * kSlots;
         par ^= 1;
         if (lane() == 0) buf[kSlots * wave()] = v;
         __syncthreads();
@@ -293,7 +301,7 @@ struct TeamWG {
     // LDS doubles a team reduction needs: wavefront slots, the members'
     // partials and the eight results
     static constexpr int kLdsDoubles = kWaves * kSlots + kMaxTeam * kSlots + kSlots;
-    double* lds;        // kLdsDoubles doubles
+    double UVA_LDS* lds;        // kLdsDoubles doubles
     int* cnt;           // arrival counter of this team (zero at launch)
     double* part;       // 2 * kMaxTeam * kSlots partial sums
     int m, mem;
@@ -304,7 +312,7 @@ struct TeamWG {
     int* fault;
     char* lv;           // LDS vector buffer of this member (or NULL)
     int lvb;
-    __device__ TeamWG(double* l, int* c, double* pt, int m_, int k_, int local_, long long spin_, int* fault_,
+    __device__ TeamWG(double UVA_LDS* l, int* c, double* pt, int m_, int k_, int local_, long long spin_, int* fault_,
                       char* v, int vb)
         : lds(l), cnt(c), part(pt), m(m_), mem(k_), rounds(0), nred(0), local(local_), spin(spin_),
           fault(fault_), lv(v), lvb(vb) {}
@@ -349,7 +357,7 @@ struct TeamWG {
         a.x = wave_sum(a.x); a.y = wave_sum(a.y); a.z = wave_sum(a.z); a.w = wave_sum(a.w);
         b.x = wave_sum(b.x); b.y = wave_sum(b.y); b.z = wave_sum(b.z); b.w = wave_sum(b.w);
         if (lane() == 0) {
-            double* q = lds + kSlots * wave();
+            double UVA_LDS* q = lds + kSlots * wave();
             q[0] = a.x; q[1] = a.y; q[2] = a.z; q[3] = a.w;
             q[4] = b.x; q[5] = b.y; q[6] = b.z; q[7] = b.w;
         }
@@ -362,8 +370,8 @@ struct TeamWG {
             pb[kSlots * mem + threadIdx.x] = v;
         }
         sync();
-        double* mp = lds + kWaves * kSlots;          // members' partials
-        double* res = mp + kMaxTeam * kSlots;        // the eight sums
+        double UVA_LDS* mp = lds + kWaves * kSlots;          // members' partials
+        double UVA_LDS* res = mp + kMaxTeam * kSlots;        // the eight sums
         if (threadIdx.x < m * kSlots) mp[threadIdx.x] = pb[threadIdx.x];
         __syncthreads();
         if (threadIdx.x < kSlots) {
@@ -393,8 +401,8 @@ struct TeamWG {
             pb[kSlots * mem] = mn;
         }
         sync();
-        double* mp = lds + kWaves * kSlots;
-        double* res = mp + kMaxTeam * kSlots;
+        double UVA_LDS* mp = lds + kWaves * kSlots;
+        double UVA_LDS* res = mp + kMaxTeam * kSlots;
         if (threadIdx.x < m) mp[threadIdx.x] = pb[kSlots * threadIdx.x];
         __syncthreads();
         if (threadIdx.x == 0) {
@@ -459,12 +467,14 @@ enum { OP_ARAP = 0, OP_LSCM = 1 };
 constexpr int kGather = 8;   // matrix entries whose gathers are issued together
 
 // y = A x for one row: FP32 values (hi) for float vectors, hi + lo for double.
+// (o is an entry offset, always within the matrix: unsigned, so the load
+// takes a scalar base and one 32-bit offset register)
 template <typename V>
-UVA_HD __forceinline__ V mval(const DevPlan& P, int o);
+UVA_HD __forceinline__ V mval(const DevPlan& P, unsigned o);
 template <>
-UVA_HD __forceinline__ float mval<float>(const DevPlan& P, int o) { return P.m_hi[o]; }
+UVA_HD __forceinline__ float mval<float>(const DevPlan& P, unsigned o) { return P.m_hi[o]; }
 template <>
-UVA_HD __forceinline__ double mval<double>(const DevPlan& P, int o) {
+UVA_HD __forceinline__ double mval<double>(const DevPlan& P, unsigned o) {
     return (double)P.m_hi[o] + (double)P.m_lo[o];
 }
 
@@ -1078,7 +1088,7 @@ __global__ __launch_bounds__(BS) void island_kernel(DevPlan P, TaskArgs a, int s
     constexpr int kLv = lds_vec_bytes(KIND);
     __shared__ double lvec[kLv > 0 ? kLv / 8 : 1];
     char* const lv = kLv > 0 ? (char*)lvec : nullptr;
-    BlockTeam<BS> bt(red, lv, kLv);
+    BlockTeam<BS> bt((double UVA_LDS*)red, lv, kLv);
     // the workgroup slots of this XCD (each slot's tasks in order), then
     // wavefront tasks of any island
     const int list = uniform_i(P.xcc_map[xcc_id()]);
@@ -1102,7 +1112,7 @@ __global__ __launch_bounds__(BS) void island_kernel(DevPlan P, TaskArgs a, int s
             run_task<KIND, PREC>(bt, P, isl, a);
         } else {
             const int team = uniform_i(P.grp_team[task]);
-            TeamWG<BS> tw(red, P.team_cnt + slot * P.n_teams + team,
+            TeamWG<BS> tw((double UVA_LDS*)red, P.team_cnt + slot * P.n_teams + team,
                           P.team_part + (size_t)team * 2 * kMaxTeam * TeamWG<BS>::kSlots, m,
                           uniform_i(P.grp_mem[task]), P.xcd_local, P.spin_ticks, P.fault, lv, kLv);
             run_task<KIND, PREC>(tw, P, isl, a);

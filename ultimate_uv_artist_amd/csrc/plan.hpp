@@ -86,6 +86,7 @@ struct DevPlan {
     const float* m_hi;
     const float* m_lo;
     const float* m_b;            // LSCM boundary coupling B_ij (exact in fp32)
+    int m_zero;                  // offset of kSlice entries of value 0 after the last slice
     const double* diag;          // Laplacian diagonal without pin
     const double* rpin;          // per-row soft-pin weights replacing the islands' pins (NULL = unused)
     const double* pdiag;         // diag + rpin (set with rpin)

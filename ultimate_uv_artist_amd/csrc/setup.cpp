@@ -519,7 +519,10 @@ std::string build_host_plan(HostPlan& hp, int64_t n_islands, const int64_t* vert
     }
     if (nnz_pad >= (int64_t(1) << 31)) return "batch too large for 32-bit matrix offsets";
     hp.sl_ptr.assign(nslices + 1, 0);
-    hp.m_col.resize(nnz_pad); hp.m_hi.resize(nnz_pad); hp.m_lo.resize(nnz_pad); hp.m_b.resize(nnz_pad);
+    // kSlice zero entries after the last slice: the resident SpMV reads them
+    // for the steps past a row's width (weight 0, no select after the load)
+    hp.m_col.assign(nnz_pad + kSlice, 0); hp.m_hi.assign(nnz_pad + kSlice, 0.0f);
+    hp.m_lo.assign(nnz_pad + kSlice, 0.0f); hp.m_b.assign(nnz_pad + kSlice, 0.0f);
     hp.diag.resize(nv_all); hp.dinv32.resize(nv_all); hp.vbnd.resize(nv_all); hp.vperm.resize(nv_all);
     // coarse level: aggregates of g consecutive slices, at most kCoarseN of them
     hp.isl_cofs.assign(n, -1);
@@ -815,7 +818,8 @@ std::string build_matrix_plan(HostPlan& hp, int64_t n, const int64_t* indptr, co
     }
     if (off >= (int64_t(1) << 31)) return "matrix too large for 32-bit offsets";
     hp.sl_ptr[ns] = (int)off;
-    hp.m_col.resize(off); hp.m_hi.resize(off); hp.m_lo.resize(off); hp.m_b.assign(off, 0.0f);
+    hp.m_col.assign(off + kSlice, 0); hp.m_hi.assign(off + kSlice, 0.0f);
+    hp.m_lo.assign(off + kSlice, 0.0f); hp.m_b.assign(off + kSlice, 0.0f);
     for (int s = 0; s < ns; ++s)
         for (int l = 0; l < kSlice; ++l) {
             int r = s * kSlice + l;
