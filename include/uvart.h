@@ -27,6 +27,8 @@
  *                       param.py:457-474 _arap_rhs; gpu.py:165-182 Batch._rhs
  *   uva_cg_solve        param.py:103-132 SPDSolver.solve; gpu.py:184-202 Batch._cg
  *   uva_max_step        param.py:259-291 _max_step; gpu.py:204-227 Batch._max_step
+ *   uva_step_apply      gpu.py:235-247 (the rest of a Batch.arap iteration after
+ *                       its CG); hier.py:498-509 (the same in Solver.relax)
  *   uva_energy          param.py:294-338 _energy
  *   uva_energy_fwd      param.py:294-338 _energy with per-triangle weights
  *   uva_energy_bwd,     no counterpart in the reference (it has no
@@ -311,6 +313,13 @@ int uva_cg_solve(uva_plan *plan, const double *b, double *x, double tol, int32_t
                  void *stream);
 /* flip-preventing step of U along D per island -> tmax (device, n_islands) */
 int uva_max_step(uva_plan *plan, const double *U, const double *D, double *tmax, int32_t mode, void *stream);
+/* The rest of one Batch (mode UVA_MODE_BATCH) or hier (UVA_MODE_HIER) ARAP
+ * iteration after its global step, per island: Un (device, n x 2, the step's
+ * result) is replaced by U on an island with a non-finite row; with
+ * line_search (hier: unless a usable triangle of U is folded) the step is
+ * min(1, 0.9 x the flip-safe limit along Un - U), else 1; U += step (Un - U)
+ * in place.  No host synchronisation. */
+int uva_step_apply(uva_plan *plan, double *U, const double *Un, int32_t mode, int32_t line_search, void *stream);
 /* per-island energy (kind 0 ARAP, 1 symmetric Dirichlet) -> e (device) */
 int uva_energy(uva_plan *plan, const double *U, int32_t kind, double *e, void *stream);
 /* Differentiation (refine plans).  Per-island weighted energy

@@ -47,6 +47,7 @@ def lib():
         _lib.uva_emu_energy_bwd.argtypes = [C.c_void_p, _dp, _dp, C.c_int32, _dp, _dp, _dp]
         _lib.uva_emu_step_bwd.argtypes = [C.c_void_p, _dp, _dp, _dp]
         _lib.uva_emu_adjoint_cg.argtypes = [C.c_void_p, _dp, _dp, C.c_double, C.c_int32]
+        _lib.uva_emu_step_apply.argtypes = [C.c_void_p, _dp, _dp, C.c_int32, C.c_int32]
         _lib.uva_emu_plan_pcg.argtypes = [C.c_void_p, _dp, _dp, C.c_int32, C.c_int32, C.c_double, C.c_int32]
         _lib.uva_emu_vperm.argtypes = [C.c_void_p, C.POINTER(C.c_int32)]
         _lib.uva_emu_lscm_coarse.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_float), C.c_int64,
@@ -156,6 +157,14 @@ class EmuPlan:
         its = lib().uva_emu_plan_pcg(self.h, b.ctypes.data_as(_dp), x.ctypes.data_as(_dp), int(max_iters),
                                      int(check_every), float(tol2), int(bool(strict)))
         return x, its
+
+    def step_apply(self, U, Un, mode=1, line_search=True):
+        """uva_step_apply on the host: the updated layout."""
+        U = np.ascontiguousarray(np.array(U, np.float64))
+        Un = np.ascontiguousarray(np.array(Un, np.float64))
+        lib().uva_emu_step_apply(self.h, U.ctypes.data_as(_dp), Un.ctypes.data_as(_dp), int(mode),
+                                 int(bool(line_search)))
+        return U
 
     def vperm(self):
         out = np.zeros(self.n, np.int32)

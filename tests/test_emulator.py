@@ -501,6 +501,68 @@ def test_emu_plan_pcg_matches_the_batch_cg(cg_iters, tol, strict):
     assert np.abs(x - xr).max() <= 1e-10 * np.abs(xr).max()
 
 
+def test_emu_batch_iteration_sequence_vs_batch_oracle():
+    """One gpu.Batch.arap iteration as the product runs it -- rotations and
+    rhs (uva_local_rhs), the coupled CG of 40 iterations on the plan's own
+    matrix (uva_plan_pcg), then non-finite revert, flip-safe step and update
+    (uva_step_apply) -- three times, against the Batch oracle's arap with the
+    reference's coupled CG (gpu.py:184-202, 230-249)."""
+    from oracle import param_cpu as oracle
+    from oracle.batch_cpu import BatchOracle
+    from ultimate_uv_artist_amd import workloads
+
+    rng = np.random.default_rng(66)
+    charts = [workloads.disk_island(rng, t) for t in (400, 900, 1600)]
+    items = [(P, T, oracle.lscm(P, T)) for P, T in charts]
+    r = BatchOracle(items)
+    r.orient_positive()
+    U = r.U.copy()
+    e = EmuPlan(charts, batch=True)
+    e.set_pins(e.vo[:-1], 0.0)
+    for _ in range(3):
+        b = e.local_rhs(U)
+        Un, _ = e.plan_pcg(b, U, 40, check_every=1, tol2=1e-12, strict=False)
+        U = e.step_apply(U, Un, mode=1, line_search=True)
+    r.arap(3, line_search=True, cg_iters=40)
+    assert np.abs(U - r.U).max() <= 1e-9 * np.abs(r.U).max()
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_emu_step_apply_reverts_and_steps_per_chart(mode):
+    """uva_step_apply per chart: a chart with a non-finite row keeps its
+    layout; the others move min(1, 0.9 x the flip-safe limit) of the way to
+    the step's result (gpu.py:235-247); in hier mode a chart with a folded
+    usable triangle takes the whole step (hier.py:500-505)."""
+    from oracle import param_cpu as oracle
+    from oracle.batch_cpu import BatchOracle
+    from ultimate_uv_artist_amd import workloads
+
+    rng = np.random.default_rng(67)
+    charts = [workloads.disk_island(rng, t) for t in (300, 500, 700)]
+    items = [(P, T, oracle.lscm(P, T)) for P, T in charts]
+    r = BatchOracle(items)
+    r.orient_positive()
+    U = r.U.copy()
+    vo = np.cumsum([0] + [len(P) for P, _ in charts])
+    # chart 2 folded: one of its triangles turned over
+    T2 = charts[2][1]
+    f = T2[len(T2) // 2]
+    U[vo[2] + f[0]], U[vo[2] + f[1]] = U[vo[2] + f[1]].copy(), U[vo[2] + f[0]].copy()
+    Un = U + rng.normal(0, 0.05, U.shape) * np.ptp(U, 0)
+    Un[vo[1] + 3] = np.nan                                  # chart 1 non-finite
+    e = EmuPlan(charts, batch=True)
+    got = e.step_apply(U, Un, mode=mode, line_search=True)
+    D = Un - U
+    D[vo[1]:vo[2]] = 0.0
+    step = np.minimum(0.9 * r.max_step(U, np.nan_to_num(D)), 1.0)
+    if mode == 2:
+        assert r.flips_per_chart(U)[2] > 0
+        step[2] = 1.0
+    want = U + D * step[r.cv][:, None]
+    assert np.array_equal(got[vo[1]:vo[2]], U[vo[1]:vo[2]])
+    assert np.abs(got - want).max() <= 1e-12 * np.abs(want).max()
+
+
 def test_emu_folded_starts_run_more_iterations():
     """charts.flatten's ARAP (charts.py:411-415) for several charts in one
     call: a chart whose start is folded runs iters_folded iterations without

@@ -83,6 +83,7 @@ SIGNATURES = {
     "uva_plan_get_stats": (C.c_int, [_vp, C.POINTER(IslandStats), _vp]),
     "uva_plan_set_scale": (C.c_int, [_vp, _vp, _vp]),
     "uva_plan_set_pins": (C.c_int, [_vp, C.POINTER(C.c_int64), C.c_int64, C.c_double, _vp]),
+    "uva_step_apply": (C.c_int, [_vp, _vp, _vp, C.c_int32, C.c_int32, _vp]),
     "uva_plan_pcg": (C.c_int, [_vp, _vp, _vp, C.c_int32, C.c_int32, C.c_double, C.c_int32, C.POINTER(C.c_int32),
                                _vp]),
     "uva_overlap_flags": (C.c_int, [_vp, C.c_int64, _vp, C.c_int64, _vp, C.c_double, C.c_int32, C.c_int32, _vp, _vp]),

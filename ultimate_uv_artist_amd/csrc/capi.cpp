@@ -1092,6 +1092,22 @@ int uva_max_step(uva_plan* plan, const double* U, const double* D, double* tmax,
     return check_fault(plan, s, rc);
 }
 
+int uva_step_apply(uva_plan* plan, double* U, const double* Un, int32_t mode, int32_t line_search, void* stream) {
+    if (!valid_plan(plan) || !U || !Un) return fail(UVA_EINVAL, "null argument");
+    if (mode != UVA_MODE_BATCH && mode != UVA_MODE_HIER) return fail(UVA_EINVAL, "mode must be BATCH or HIER");
+    plan->mem = (hipStream_t)stream;   // allocations and the final release follow this stream
+    hipStream_t s = (hipStream_t)stream;
+    int rc = UVA_OK;
+    uva::TaskArgs a = base_args((double*)to_plan(plan, U, 0, s, rc));
+    a.in = to_plan(plan, Un, 1, s, rc);
+    a.batch = 1;
+    a.hier = mode == UVA_MODE_HIER;
+    a.line_search = line_search != 0;
+    if (!rc) rc = reset_queue(plan, 1, s);
+    if (!rc) rc = launch(plan, uva::TASK_STEP_APPLY, UVA_PREC_MIXED, a, 0, s);
+    return check_fault(plan, s, from_plan(plan, 0, U, s, rc));
+}
+
 int uva_energy(uva_plan* plan, const double* U, int32_t kind, double* e, void* stream) {
     if (!valid_plan(plan) || !U || !e) return fail(UVA_EINVAL, "null argument");
     plan->mem = (hipStream_t)stream;   // allocations and the final release follow this stream

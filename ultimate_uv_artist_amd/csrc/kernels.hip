@@ -933,6 +933,48 @@ UVA_HD __forceinline__ void run_task(Team& tm, const DevPlan& P, int isl, const 
         tm.sync();
         const double t = max_step_island(tm, P, d, U, P.X64, a.batch, a.hier != 0);
         if (tm.leader()) a.out_isl[isl] = t;
+    } else if (KIND == TASK_STEP_APPLY) {
+        // The rest of one Batch / hier ARAP iteration after its coupled
+        // global step (gpu.py:235-247, hier.py:498-509): a.in holds the
+        // step's result Un.  An island with a non-finite row keeps its
+        // layout; with line search (hier: unless a usable triangle of U is
+        // folded) the step is min(1, 0.9 x the flip-safe limit along Un - U),
+        // else 1; then U += step (Un - U).
+        double bad = 0.0;
+        FOR_ROWS(tm, d, row, sg) {
+            (void)sg;
+            const double2 x = a.in[row];
+            bad += (__builtin_isfinite(x.x) && __builtin_isfinite(x.y)) ? 0.0 : 1.0;
+            P.X64[row] = x;
+        }
+        if (tm.sum4(make_double4(bad, 0.0, 0.0, 0.0)).x > 0.0) {
+            FOR_ROWS(tm, d, row, sg) {
+                (void)sg;
+                P.X64[row] = U[row];
+            }
+            tm.sync();
+        }
+        int use_ls = a.line_search;
+        if (a.hier && use_ls) {
+            double nf = 0.0;
+            FOR_TRIS(tm, d, f) nf += (P.fok[f] && signed_area(P, f, U) < 0.0) ? 1.0 : 0.0;
+            if (tm.sum4(make_double4(nf, 0.0, 0.0, 0.0)).x > 0.0) use_ls = 0;
+        }
+        double step = 1.0, tmax = 1e30;
+        if (use_ls) {
+            tmax = max_step_island(tm, P, d, U, P.X64, true, a.hier != 0);
+            step = fmin(1.0, 0.9 * tmax);
+        }
+        FOR_ROWS(tm, d, row, sg) {
+            (void)sg;
+            const double2 u = U[row], x = P.X64[row];
+            U[row] = make_double2(u.x + (x.x - u.x) * step, u.y + (x.y - u.y) * step);
+        }
+        if (tm.leader()) {
+            st->step = step;
+            st->tmax = tmax;
+        }
+        tm.sync();
     } else if (KIND == TASK_ENERGY) {
         const double s = energy_fwd_island(tm, P, d, U, a.w, a.kind);
         if (tm.leader()) a.out_isl[isl] = s;
@@ -1196,6 +1238,7 @@ hipError_t launch_island_tasks(int kind, int prec, const DevPlan& P, const TaskA
         case TASK_LSCM: return launch_kind<TASK_LSCM, UVA_PREC_F64>(P, a, slot, cus, s);
         case TASK_LOCAL_RHS: return launch_kind<TASK_LOCAL_RHS, UVA_PREC_MIXED>(P, a, slot, cus, s);
         case TASK_MAX_STEP: return launch_kind<TASK_MAX_STEP, UVA_PREC_MIXED>(P, a, slot, cus, s);
+        case TASK_STEP_APPLY: return launch_kind<TASK_STEP_APPLY, UVA_PREC_MIXED>(P, a, slot, cus, s);
         case TASK_ENERGY: return launch_kind<TASK_ENERGY, UVA_PREC_MIXED>(P, a, slot, cus, s);
         case TASK_REPAIR: return launch_kind<TASK_REPAIR, UVA_PREC_MIXED>(P, a, slot, cus, s);
         case TASK_ORIENT: return launch_kind<TASK_ORIENT, UVA_PREC_MIXED>(P, a, slot, cus, s);

@@ -31,6 +31,7 @@ enum TaskKind {
     TASK_SLIM_ITER = 11,      // SLIM local step -> normal equations by PCG -> step with backtracking
     TASK_ENERGY_BWD = 12,     // dE/dU and dE/dw of the weighted energy
     TASK_STEP_BWD = 13,       // backward of one ARAP global step given the adjoint solution
+    TASK_STEP_APPLY = 14,     // Batch / hier: non-finite revert, step rule and update from a global step's result
 };
 
 // Kinds whose tasks solve the ARAP or LSCM system (resident CG when the

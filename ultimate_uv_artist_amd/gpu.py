@@ -211,23 +211,21 @@ class Batch:
         or after ``cg_iters`` iterations (gpu.py:184-202), revert of
         non-finite charts, per-chart flip-safe step when ``line_search``."""
         U = self._U.clone()
-        for _ in range(int(iters)):
+        Un = torch.empty_like(U)
+        n = int(iters)
+        for k in range(n):
+            if cg_iters <= 0:                                  # no CG iteration: the layout stays
+                break
             b = self._plan.local_rhs(U, _lib.UVA_MODE_BATCH)
-            Un = U.clone()
-            if cg_iters > 0:                                   # no CG iteration: the layout stays
-                self.last_cg_iters = self._plan.pcg(b, Un, int(cg_iters), check_every=1,
-                                                    tol2=float(cg_tol) ** 2, strict=False)
-            bad = ~torch.isfinite(Un).all(1)
-            if bool(bad.any()):
-                badc = torch.zeros(self.nc, dtype=torch.bool, device=self.dev)
-                badc[self.cv[bad]] = True
-                Un = torch.where(badc[self.cv][:, None], U, Un)
-            if line_search:
-                D = (Un - U).contiguous()
-                step = torch.clamp(0.9 * self._plan.max_step(U, D, mode=_lib.UVA_MODE_BATCH), max=1.0)
-                U.add_(D * step[self.cv][:, None])
-            else:
-                U.copy_(Un)
+            Un.copy_(U)
+            # the CG's iteration count is read back once, after the last call
+            its = self._plan.pcg(b, Un, int(cg_iters), check_every=1, tol2=float(cg_tol) ** 2, strict=False,
+                                 want_iters=k == n - 1)
+            if its is not None:
+                self.last_cg_iters = its
+            # non-finite charts keep their layout, then the per-chart
+            # flip-safe step (or the whole step without line search)
+            self._plan.step_apply(U, Un, mode=_lib.UVA_MODE_BATCH, line_search=line_search)
         self._U = U
         return self.U
 

@@ -111,26 +111,20 @@ class Solver:
         plan = IslandPlan(charts, batch=True, dev=self.dev, no_coarse=True)    # its own CG is not used
         voff = plan.vert_off
         plan.set_pins(voff[:-1], 0.0)
-        cv = torch.as_tensor(np.repeat(np.arange(len(charts)), np.diff(voff)), device=self.dev)
         # the reference holds the layout in float32 (hier.py:400)
         U = plan.device_uv(U0[vorder].astype(np.float32).astype(np.float64))
         plan.orient_positive(U)
         plan.repair_jacobi(U, repair_rounds)
+        Un = torch.empty_like(U)
         for it in range(int(iters)):
-            b = plan.local_rhs(U, _lib.UVA_MODE_BATCH)
-            Un = U.clone()
             if cg_iters > 0:                                   # exactly cg_iters iterations; none: the layout stays
+                b = plan.local_rhs(U, _lib.UVA_MODE_BATCH)
+                Un.copy_(U)
                 plan.pcg(b, Un, int(cg_iters), check_every=1, tol2=0.0, strict=True, want_iters=False)
-            bad = ~torch.isfinite(Un).all(1)
-            if bool(bad.any()):                              # hier.py:500-503
-                badc = torch.zeros(len(charts), dtype=torch.bool, device=self.dev)
-                badc[cv[bad]] = True
-                Un = torch.where(badc[cv][:, None], U, Un)
-            folded = plan.flips_per_chart(U) > 0
-            D = (Un - U).contiguous()
-            step = torch.clamp(0.9 * plan.max_step(U, D, mode=_lib.UVA_MODE_HIER), max=1.0)
-            step = torch.where(folded, torch.ones_like(step), step)
-            U.add_(D * step[cv][:, None])
+                # non-finite charts keep their layout (hier.py:500-503); folded
+                # charts take the whole step, the others 0.9 of the largest
+                # fold-free one over usable triangles (hier.py:504-509)
+                plan.step_apply(U, Un, mode=_lib.UVA_MODE_HIER, line_search=True)
             if it == iters // 2:
                 plan.repair_jacobi(U, repair_rounds)
         plan.repair_jacobi(U, repair_rounds)

@@ -533,6 +533,15 @@ class IslandPlan:
         check(_lib.lib().uva_plan_set_pins(self._h, i64p(p), len(p), float(w), stream_of(self.dev)),
               "uva_plan_set_pins")
 
+    def step_apply(self, U, Un, mode=_lib.UVA_MODE_BATCH, line_search=True):
+        """The rest of a Batch / hier ARAP iteration after its global step
+        (non-finite revert, step rule, update of U in place; uva_step_apply)."""
+        self._check_uv(U)
+        self._check_uv(Un)
+        check(_lib.lib().uva_step_apply(self._h, ptr(U), ptr(Un), int(mode), int(bool(line_search)),
+                                        stream_of(self.dev)), "uva_step_apply")
+        return U
+
     def pcg(self, b, x, max_iters, check_every=1, tol2=0.0, strict=False, want_iters=True):
         """The reference's Jacobi-PCG over every row of the plan at once (its
         islands' Laplacians with the pins of ``set_pins``, two columns sharing
