@@ -329,8 +329,8 @@ class Arena {
         up_items_.push_back({up_, slice(n * sizeof(T)), src, n * sizeof(T), [out](char* b) { *out = (P*)b; }});
         up_ += up_items_.back().bytes;
     }
-    template <class T>
-    void put(const T** out, const std::vector<T>& v) { put<T, const T>(out, v.data(), v.size()); }
+    template <class T, class A>
+    void put(const T** out, const std::vector<T, A>& v) { put<T, const T>(out, v.data(), v.size()); }
     template <class T>
     void zero(T** out, size_t n) { zero_items_.push_back({0, slice(n * sizeof(T)), nullptr, 0, setter(out)}); }
     template <class T>

@@ -291,9 +291,9 @@ std::vector<int> morton_order(int nv, const double* P) {
 // + pin weight; Z = indicator of the aggregates, g consecutive 64-row slices
 // each), symmetric, in FP32.  Returns false if Ac is not numerically positive
 // definite.
-bool coarse_inverse(int nv, int g, const std::vector<int>& row_ptr, const std::vector<int>& row_col,
-                    const std::vector<double>& row_val, const std::vector<double>& diag, int pin, double wpin,
-                    float* out) {
+template <class VP, class VC, class VV, class VD>
+bool coarse_inverse(int nv, int g, const VP& row_ptr, const VC& row_col, const VV& row_val, const VD& diag, int pin,
+                    double wpin, float* out) {
     const int rows = kSlice * g;
     const int ns = (nv + rows - 1) / rows;
     std::vector<double> A((size_t)ns * ns, 0.0);
@@ -427,12 +427,14 @@ bool lscm_coarse_inverse(int nv, int g, const std::vector<int>& row_ptr, const s
 }
 
 template <class F>
-void parallel_for(int n, F fn) {
+void parallel_for(int n, F fn, const std::vector<int>* order = nullptr) {
+    // (order: the items in the order threads take them, e.g. largest first,
+    // so one large item is not left for last)
     int nth = (int)std::min<unsigned>(std::max(1u, std::thread::hardware_concurrency()), 16u);
     nth = std::min(nth, std::max(1, n));
     std::atomic<int> next(0);
     auto work = [&]() {
-        for (int i = next++; i < n; i = next++) fn(i);
+        for (int i = next++; i < n; i = next++) fn(order ? (*order)[i] : i);
     };
     if (nth <= 1) { work(); return; }
     std::vector<std::thread> th;
@@ -468,6 +470,12 @@ std::string build_host_plan(HostPlan& hp, int64_t n_islands, const int64_t* vert
     }
     int n = (int)n_islands;
     std::vector<Island> isl(n);
+    // threads take the islands largest first
+    std::vector<int> order(n);
+    for (int i = 0; i < n; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        return tri_off[a + 1] - tri_off[a] > tri_off[b + 1] - tri_off[b];
+    });
     // per island: perm[v] = caller's local id of plan vertex v, inv its inverse
     std::vector<std::vector<int>> perm(n), inv(n);
     parallel_for(n, [&](int i) {
@@ -486,7 +494,7 @@ std::string build_host_plan(HostPlan& hp, int64_t n_islands, const int64_t* vert
         if (lscm_pins) { lp[0] = inv[i][lscm_pins[2 * i]]; lp[1] = inv[i][lscm_pins[2 * i + 1]]; }
         build_island(isl[i], nv, nt, Pp.data(), Tp.data(), X ? X + 6 * t0 : nullptr, area ? area + t0 : nullptr,
                      cots ? cots + 3 * t0 : nullptr, batch, lscm_pins ? lp : nullptr, perm[i].data());
-    });
+    }, &order);
     hp.n_isl = n;
     hp.n_vert = (int)nv_all;
     hp.n_tri = (int)nt_all;
@@ -501,28 +509,38 @@ std::string build_host_plan(HostPlan& hp, int64_t n_islands, const int64_t* vert
     int64_t nslices = 0;
     std::vector<int64_t> slice_start(n);
     std::vector<std::vector<int>> widths(n);
-    int64_t nnz_pad = 0;
-    for (int i = 0; i < n; ++i) {
+    std::vector<int64_t> isl_pad(n);
+    parallel_for(n, [&](int i) {
         const Island& I = isl[i];
-        int ns = (I.nv + kSlice - 1) / kSlice;
+        const int ns = (I.nv + kSlice - 1) / kSlice;
         widths[i].resize(ns);
+        int64_t pad = 0;
         for (int s = 0; s < ns; ++s) {
             int wmx = 0;
             for (int r = s * kSlice; r < std::min(I.nv, (s + 1) * kSlice); ++r)
                 wmx = std::max(wmx, I.row_ptr[r + 1] - I.row_ptr[r]);
             widths[i][s] = wmx;
-            nnz_pad += (int64_t)wmx * kSlice;
+            pad += (int64_t)wmx * kSlice;
         }
+        isl_pad[i] = pad;
+    }, &order);
+    int64_t nnz_pad = 0;
+    for (int i = 0; i < n; ++i) {
         slice_start[i] = nslices;
-        nslices += ns;
-        hp.nnz_true += I.nnz;
+        nslices += (int64_t)widths[i].size();
+        nnz_pad += isl_pad[i];
+        hp.nnz_true += isl[i].nnz;
     }
     if (nnz_pad >= (int64_t(1) << 31)) return "batch too large for 32-bit matrix offsets";
     hp.sl_ptr.assign(nslices + 1, 0);
     // kSlice zero entries after the last slice: the resident SpMV reads them
-    // for the steps past a row's width (weight 0, no select after the load)
-    hp.m_col.assign(nnz_pad + kSlice, 0); hp.m_hi.assign(nnz_pad + kSlice, 0.0f);
-    hp.m_lo.assign(nnz_pad + kSlice, 0.0f); hp.m_b.assign(nnz_pad + kSlice, 0.0f);
+    // for the steps past a row's width (weight 0, no select after the load);
+    // every other entry is written by the threaded fill below
+    hp.m_col.resize(nnz_pad + kSlice); hp.m_hi.resize(nnz_pad + kSlice);
+    hp.m_lo.resize(nnz_pad + kSlice); hp.m_b.resize(nnz_pad + kSlice);
+    for (int64_t o = nnz_pad; o < nnz_pad + kSlice; ++o) {
+        hp.m_col[o] = 0; hp.m_hi[o] = 0.0f; hp.m_lo[o] = 0.0f; hp.m_b[o] = 0.0f;
+    }
     hp.diag.resize(nv_all); hp.dinv32.resize(nv_all); hp.vbnd.resize(nv_all); hp.vperm.resize(nv_all);
     // coarse level: aggregates of g consecutive slices, at most kCoarseN of them
     hp.isl_cofs.assign(n, -1);
@@ -549,11 +567,12 @@ std::string build_host_plan(HostPlan& hp, int64_t n_islands, const int64_t* vert
     }
     hp.tv0.resize(nt_all); hp.tv1.resize(nt_all); hp.tv2.resize(nt_all);
     hp.fx1.resize(nt_all); hp.fx2.resize(nt_all); hp.fy2.resize(nt_all); hp.farea.resize(nt_all); hp.fok.resize(nt_all);
-    hp.inc_ptr.assign(nv_all + 1, 0);
+    hp.inc_ptr.resize(nv_all + 1);
+    hp.inc_ptr[nv_all] = 3 * (int)nt_all;
     hp.inc_tri.resize(3 * nt_all);
     hp.inc_g.resize(6 * nt_all);
     hp.tri_g.resize(6 * nt_all);
-    hp.adj_ptr.assign(nv_all + 1, 0);
+    hp.adj_ptr.resize(nv_all + 1);
     int64_t adj_total = 0;
     for (auto& I : isl) adj_total += I.nnz;
     hp.adj_idx.resize(adj_total);
@@ -569,17 +588,15 @@ std::string build_host_plan(HostPlan& hp, int64_t n_islands, const int64_t* vert
             for (int q = 0; q < (int)widths[i].size(); ++q)
                 hp.sl_row[slice_start[i] + q] = (int)vert_off[i] + q * kSlice;
     }
-    // prefix of adjacency
+    // prefix of adjacency: island bases, then each island's rows in parallel
     {
-        int64_t a = 0;
-        for (int i = 0; i < n; ++i) {
+        std::vector<int64_t> abase(n + 1, 0);
+        for (int i = 0; i < n; ++i) abase[i + 1] = abase[i] + isl[i].nnz;
+        parallel_for(n, [&](int i) {
             const Island& I = isl[i];
-            for (int v = 0; v < I.nv; ++v) {
-                hp.adj_ptr[vert_off[i] + v] = (int)a;
-                a += I.row_ptr[v + 1] - I.row_ptr[v];
-            }
-        }
-        hp.adj_ptr[nv_all] = (int)a;
+            for (int v = 0; v < I.nv; ++v) hp.adj_ptr[vert_off[i] + v] = (int)(abase[i] + I.row_ptr[v]);
+        }, &order);
+        hp.adj_ptr[nv_all] = (int)abase[n];
     }
     parallel_for(n, [&](int i) {
         const Island& I = isl[i];
@@ -658,7 +675,7 @@ std::string build_host_plan(HostPlan& hp, int64_t n_islands, const int64_t* vert
                 }
             }
         }
-    });
+    }, &order);
     // task order: most expensive first (cost ~ rows^1.5 for Jacobi-PCG on a 2-D Laplacian)
     std::vector<std::pair<double, int>> wv;
     std::vector<int> g;

@@ -4,11 +4,34 @@
 #include <stdint.h>
 
 #include <string>
+#include <memory>
+#include <utility>
 #include <vector>
 
 #include "plan.hpp"
 
 namespace uva {
+
+// Vectors whose resize leaves new elements uninitialised (default-init, no
+// zero fill): the plan build writes every element of its large arrays in
+// threaded passes, so the pages are first touched there, in parallel, and
+// not by a serial fill.
+template <class T>
+struct NoInitAlloc : std::allocator<T> {
+    template <class U>
+    struct rebind {
+        using other = NoInitAlloc<U>;
+    };
+    NoInitAlloc() = default;
+    template <class U>
+    NoInitAlloc(const NoInitAlloc<U>&) noexcept {}
+    template <class U>
+    void construct(U* p) noexcept { ::new ((void*)p) U; }
+    template <class U, class... A>
+    void construct(U* p, A&&... a) { ::new ((void*)p) U(std::forward<A>(a)...); }
+};
+template <class T>
+using hvec = std::vector<T, NoInitAlloc<T>>;
 
 struct HostPlan {
     int n_isl = 0, n_vert = 0, n_tri = 0;
@@ -16,19 +39,20 @@ struct HostPlan {
     std::vector<int> isl_v0, isl_nv, isl_t0, isl_nt, isl_s0, isl_ns, isl_pin, isl_lpin0, isl_lpin1;
     std::vector<int> isl_bw;     // matrix bandwidth: largest |column - row| of the island
     std::vector<double> isl_wpin, isl_ldist;
-    std::vector<int> tv0, tv1, tv2;
-    std::vector<double> fx1, fx2, fy2, farea;
-    std::vector<unsigned char> fok;
-    std::vector<int> inc_ptr, inc_tri;
-    std::vector<double> inc_g;          // 2 per incidence
-    std::vector<double> tri_g;          // the same corner vectors triangle-major: 6 per triangle
-    std::vector<int> sl_ptr, m_col;
+    hvec<int> tv0, tv1, tv2;
+    hvec<double> fx1, fx2, fy2, farea;
+    hvec<unsigned char> fok;
+    hvec<int> inc_ptr, inc_tri;
+    hvec<double> inc_g;          // 2 per incidence
+    hvec<double> tri_g;          // the same corner vectors triangle-major: 6 per triangle
+    std::vector<int> sl_ptr;
+    hvec<int> m_col;
     std::vector<int> sl_row;            // first row of each slice (n_slices + 1, the last = n_vert)
-    std::vector<float> m_hi, m_lo, m_b;
-    std::vector<double> diag;
-    std::vector<float> dinv32;
-    std::vector<int> adj_ptr, adj_idx;
-    std::vector<unsigned char> vbnd;
+    hvec<float> m_hi, m_lo, m_b;
+    hvec<double> diag;
+    hvec<float> dinv32;
+    hvec<int> adj_ptr, adj_idx;
+    hvec<unsigned char> vbnd;
     std::vector<int> order_group, order_wave;
     // group tasks: island order_group[k] is solved by member grp_mem[k] of a
     // team of grp_m[k] workgroups; teams with more than one member have a
@@ -53,7 +77,7 @@ struct HostPlan {
     int64_t nnz_true = 0;
     // plan vertex -> caller's global vertex id (islands keep their offsets;
     // inside an island the plan orders vertices by Morton code)
-    std::vector<int> vperm;
+    hvec<int> vperm;
     // coarse level: isl_cofs[i] is the offset of island i's inverse coarse
     // operator (nc x nc, FP32, nc = ceil(ns / isl_cg[i]) aggregates of
     // isl_cg[i] slices) in cinv, or -1 without a coarse level

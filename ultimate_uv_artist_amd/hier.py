@@ -58,8 +58,11 @@ class Solver:
         self._q = None
 
     def _charts(self, faces, CV, nv, chart, nchart):
-        """Islands of the subset in chart order: per chart its vertices in
-        ascending local id and its faces in subset order."""
+        """The subset's islands in chart order, concatenated: per chart its
+        vertices in ascending local id and its faces in subset order.  Returns
+        the island offsets of vertices and faces, positions (n,3), faces in
+        island-local ids (t,3) (the form IslandPlan takes), the charts that
+        have faces, and the vertex and face orders."""
         if CV.min(initial=0) < 0 or CV.max(initial=-1) >= nv:
             raise ValueError("CV holds vertex ids outside [0, nv)")
         if len(chart) and (chart.min() < 0 or chart.max() >= nchart):
@@ -70,23 +73,22 @@ class Solver:
             raise ValueError("every vertex of the subset must belong to one of its faces")
         corner = np.empty(nv, np.int64)                   # one mesh vertex per subset vertex
         corner[CV.reshape(-1)] = self.F[faces].reshape(-1)
-        pos = self.V[corner]
-        vorder = np.argsort(cv, kind="stable")            # chart-major, ascending id inside a chart
-        local = np.empty(nv, np.int64)
+        # stable sorts by chart id; with at most 65,535 charts the keys fit 16
+        # bits, for which numpy's stable sort is a radix sort (linear time)
+        small = nchart <= 65535
+        vorder = np.argsort(cv.astype(np.uint16) if small else cv, kind="stable")   # chart-major, ascending id inside a chart
         vcount = np.bincount(cv, minlength=nchart)
         voff = np.concatenate([[0], np.cumsum(vcount)])
+        local = np.empty(nv, np.int64)
         local[vorder] = np.arange(nv) - np.repeat(voff[:-1], vcount)
-        forder = np.argsort(chart, kind="stable")
+        forder = np.argsort(chart.astype(np.uint16) if small else chart, kind="stable")
         fcount = np.bincount(chart, minlength=nchart)
-        foff = np.concatenate([[0], np.cumsum(fcount)])
-        charts, keep = [], []
-        for c in range(nchart):
-            if fcount[c] == 0:
-                continue
-            fs = forder[foff[c]:foff[c + 1]]
-            charts.append((pos[vorder[voff[c]:voff[c + 1]]], local[CV[fs]]))
-            keep.append(c)
-        return charts, np.array(keep, np.int64), vorder, forder
+        keep = np.flatnonzero(fcount > 0)                 # a chart without faces has no vertices either
+        vo = np.concatenate([[0], np.cumsum(vcount[keep])]).astype(np.int64)
+        to = np.concatenate([[0], np.cumsum(fcount[keep])]).astype(np.int64)
+        P = np.ascontiguousarray(self.V[corner[vorder]])
+        T = np.ascontiguousarray(local[CV[forder]])
+        return (vo, to, P, T), keep, vorder, forder
 
     def relax(self, faces, CV, nv, chart, nchart, U0, iters=6, cg_iters=20, repair_rounds=24):
         """faces (n,) global face ids, CV (n,3) local vertex ids, chart (n,)
@@ -97,10 +99,10 @@ class Solver:
         chart = np.asarray(chart, np.int64).reshape(-1)
         nv, nchart = int(nv), int(nchart)
         U0 = np.asarray(U0, np.float64).reshape(nv, 2)
-        charts, keep, vorder, forder = self._charts(faces, CV, nv, chart, nchart)
+        cat, keep, vorder, forder = self._charts(faces, CV, nv, chart, nchart)
         flips = np.zeros(nchart)
         scale = np.zeros(nchart)
-        if not charts:
+        if len(keep) == 0:
             self._q = None
             return U0.astype(np.float32).astype(np.float64), flips, scale
         # per-chart plan: orientation, repair, flips, rotations and rhs, step
@@ -108,7 +110,7 @@ class Solver:
         # plan as the reference solves the subset (shared step lengths), one
         # soft pin per chart with the subset's weight max |W| (the islands'
         # own pin weight under Batch conventions)
-        plan = IslandPlan(charts, batch=True, dev=self.dev, no_coarse=True)    # its own CG is not used
+        plan = IslandPlan(None, batch=True, dev=self.dev, no_coarse=True, concat=cat)   # its own CG is not used
         voff = plan.vert_off
         plan.set_pins(voff[:-1], 0.0)
         # the reference holds the layout in float32 (hier.py:400)
@@ -129,7 +131,7 @@ class Solver:
                 plan.repair_jacobi(U, repair_rounds)
         plan.repair_jacobi(U, repair_rounds)
         U.copy_(U.float().double())                          # the reference returns float32 values
-        q = self._quality(plan, np.concatenate([c[0] for c in charts]), U, len(charts))
+        q = self._quality(plan, cat[2], U, len(keep))
         out = np.empty((nv, 2))
         out[vorder] = U.cpu().numpy()
         flips[keep] = q["flips"]

@@ -309,11 +309,20 @@ class IslandPlan:
     """
 
     def __init__(self, charts, pins=None, frames=None, batch=False, dev=None, no_wave=False, no_teams=False,
-                 keep_order=False, no_coarse=False):
+                 keep_order=False, no_coarse=False, concat=None):
+        # (concat: the charts already concatenated as concat_charts returns
+        # them, in place of ``charts``)
         self.dev = require_gpu(dev)
         L = _lib.lib()
-        self.vert_off, self.tri_off, P, T = concat_charts(charts)
-        self.n_islands = len(charts)
+        if concat is not None:
+            vo, to, P, T = concat
+            self.vert_off = np.ascontiguousarray(vo, np.int64)
+            self.tri_off = np.ascontiguousarray(to, np.int64)
+            P = np.ascontiguousarray(P, np.float64)
+            T = np.ascontiguousarray(T, np.int64)
+        else:
+            self.vert_off, self.tri_off, P, T = concat_charts(charts)
+        self.n_islands = len(self.vert_off) - 1
         self.n = int(self.vert_off[-1])
         self.t = int(self.tri_off[-1])
         pins_a = None if pins is None else np.ascontiguousarray(np.asarray(pins, np.int64).reshape(-1))
