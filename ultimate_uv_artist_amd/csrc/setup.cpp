@@ -135,9 +135,11 @@ void build_island(Island& I, int nv, int nt, const double* P, const int64_t* T, 
     }
     I.wmax = wmax;
 
-    // unique undirected edges with summed weights, use counts and boundary coupling
+    // unique undirected edges with summed weights, use counts and boundary
+    // coupling.  The edge opposite corner k, taken from corner k+1 to k+2, is
+    // one of the triangle's directed half-edges (0->1, 1->2, 2->0), so the
+    // directions within a group of equal keys count the half-edges each way.
     std::vector<Edge> E(3 * (size_t)nt);
-    std::vector<int64_t> dkeys(3 * (size_t)nt);
     for (int f = 0; f < nt; ++f) {
         for (int k = 0; k < 3; ++k) {
             int64_t a = T[3 * (size_t)f + (k + 1) % 3], b = T[3 * (size_t)f + (k + 2) % 3];
@@ -145,29 +147,27 @@ void build_island(Island& I, int nv, int nt, const double* P, const int64_t* T, 
             e.key = std::min(a, b) * nv + std::max(a, b);
             e.w = w[3 * (size_t)f + k];
             e.dir = a < b ? 1 : -1;
-            // directed half-edges of the triangle in corner order 0->1, 1->2, 2->0
-            dkeys[3 * (size_t)f + k] = T[3 * (size_t)f + k] * nv + T[3 * (size_t)f + (k + 1) % 3];
         }
     }
-    std::sort(dkeys.begin(), dkeys.end());
     std::sort(E.begin(), E.end(), [](const Edge& x, const Edge& y) { return x.key < y.key; });
     std::vector<int64_t> uk;
     std::vector<double> uw, ub;
     std::vector<int> ucnt;
     std::vector<int> bsrc;       // sources of boundary half-edges
-    auto n_half = [&](int64_t a, int64_t b) {
-        auto rg = std::equal_range(dkeys.begin(), dkeys.end(), a * nv + b);
-        return (int)(rg.second - rg.first);
-    };
     for (size_t q = 0; q < E.size();) {
         size_t r = q;
         double ws = 0.0;
-        while (r < E.size() && E[r].key == E[q].key) { ws += E[r].w; ++r; }
+        int nab = 0, nba = 0;    // half-edges a->b and b->a (a < b)
+        while (r < E.size() && E[r].key == E[q].key) {
+            ws += E[r].w;
+            if (E[r].dir > 0) ++nab;
+            else ++nba;
+            ++r;
+        }
         int64_t a = E[q].key / nv, b = E[q].key % nv;
         // every boundary half-edge (its reverse absent) a->b adds B_ab -= 1/2,
         // B_ba += 1/2 (param.py:142-150)
         double beta = 0.0;
-        int nab = n_half(a, b), nba = n_half(b, a);
         if (nab && !nba) { beta -= 0.5 * nab; bsrc.push_back((int)a); }
         if (nba && !nab) { beta += 0.5 * nba; bsrc.push_back((int)b); }
         uk.push_back(E[q].key);
