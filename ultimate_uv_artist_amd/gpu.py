@@ -6,9 +6,11 @@ charts on one MI355X.
 work (orientation, repair, rotations and rhs, step limits) is one persistent
 HIP kernel over all charts, islands pulled from a device work queue.  The
 global step is the reference's one CG over the whole batch (one step length
-per column, shared by every chart): grid-wide Jacobi-PCG kernels on the
-batch's pinned Laplacian, resident on the GPU (``uva_csr_pcg``), with the
-stop test on the device instead of a host sync per CG iteration.  The layout
+per column, shared by every chart): grid-wide Jacobi-PCG kernels on the chart
+plan's own sliced-ELL Laplacian with the batch's per-chart soft pins
+(``uva_plan_pcg``), with the stop test on the device instead of a host sync
+per CG iteration; the rest of each ARAP iteration (non-finite revert,
+flip-safe step, update) is one more kernel (``uva_step_apply``).  The layout
 is kept in FP64 on the device; ``Batch.U`` returns it in the requested dtype.
 
 The small torch helpers (``csr32``, ``spmm``, ``device``, ``unique_edges``,

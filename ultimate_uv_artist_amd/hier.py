@@ -11,15 +11,15 @@ plan and runs the reference's sequence on the GPU kernels:
   ``iters`` ARAP iterations:
     rotations and rhs of every chart                uva_local_rhs (pin weight of the
                                                       subset: uva_plan_set_pins)
-    one Jacobi-PCG of exactly ``cg_iters``          uva_csr_pcg on the subset's pinned
-    iterations over all charts together               Laplacian
-    (the reference's batched CG shares its
+    one Jacobi-PCG of exactly ``cg_iters``          uva_plan_pcg on the plan's own
+    iterations over all charts together               Laplacian with the subset's
+    (the reference's batched CG shares its            per-chart soft pins
     step lengths between charts), from the
     current layout
-    non-finite charts keep their layout             torch
-    folded charts take the plain step, the others  uva_flips_per_chart,
-    0.9 of the largest fold-free step over          uva_max_step(UVA_MODE_HIER)
-    usable triangles
+    non-finite charts keep their layout;            uva_step_apply(UVA_MODE_HIER)
+    folded charts take the plain step, the
+    others 0.9 of the largest fold-free step
+    over usable triangles
     a repair after iteration ``iters // 2``         uva_repair_jacobi
   final repair, per-chart quality                   torch on the plan's GPU
 
