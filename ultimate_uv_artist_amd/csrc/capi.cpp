@@ -404,7 +404,7 @@ int upload_plan(uva::HostPlan& hp, const double* P, hipStream_t s, uva_plan** ou
     UP(adj_ptr, adj_ptr); UP(adj_idx, adj_idx); UP(vbnd, vbnd);
     UP(order_group, order_group); UP(order_wave, order_wave);
     UP(grp_mem, grp_mem); UP(grp_m, grp_m); UP(grp_team, grp_team); UP(slot_ptr, slot_ptr);
-    UP(vperm, vperm); UP(isl_cofs, isl_cofs); UP(isl_cg, isl_cg); UP(cinv, cinv);
+    UP(vperm, vperm); UP(isl_cofs, isl_cofs); UP(isl_cg, isl_cg); UP(isl_lcg, isl_lcg); UP(cinv, cinv);
     UP(isl_lcofs, isl_lcofs); UP(lcinv, lcinv);
 #undef UP
     ar.put<double, const double2>(&d.inc_g, hp.inc_g.data(), hp.inc_g.size());

@@ -41,25 +41,12 @@
 // the window from global memory.)
 #pragma once
 
-UVA_HD __forceinline__ int align16(int b) { return (b + 15) & ~15; }
-
 // LDS bytes of a member holding `rows` rows (its slice range): the u window
-// (rows + 2 * halo), e, D^-1 and D; with a coarse level of nc aggregates, of
-// which the member's slices touch na: na rows of Ac^-1 (FP32), the
-// restricted residual of all aggregates and the correction of the member's.
+// (rows + 2 * halo), e, D^-1 and D (the coarse level's share:
+// res_coarse_bytes).
 template <class V2, class V>
 UVA_HD __forceinline__ int res_vec_bytes(int rows, int halo) {
-    return align16((rows + 2 * halo) * (int)sizeof(V2) + rows * ((int)sizeof(V2) + 2 * (int)sizeof(V)));
-}
-// rows of Ac^-1 are stored with an odd stride, so the threads of different
-// aggregates read different LDS banks
-UVA_HD __forceinline__ int coarse_stride(int nc) { return nc | 1; }
-constexpr int kPart = 8;   // threads sharing one row of the coarse mat-vec
-// rows of Ac^-1 (twice as long for the coupled LSCM operator), the
-// restricted residual of every aggregate, the member's correction, the
-// mat-vec shares and the member's slice sums (nsl slices)
-UVA_HD __forceinline__ int res_coarse_bytes(int nc, int na, int nsl, bool coupled = false) {
-    return nc > 0 ? align16(na * coarse_stride(nc) * 4 * (coupled ? 2 : 1)) + (nc + na + na * kPart + nsl) * 16 : 0;
+    return res_vec_bytes_of(rows, halo, (int)sizeof(V2), (int)sizeof(V));
 }
 
 template <int PREC, int RPT, class Team, int OP = OP_ARAP>
@@ -113,7 +100,7 @@ struct ResCG {
         wlo = max(rlo - H, d.v0);
         whi = min(rhi + H, d.v0 + d.nv);
         coarse = use_coarse;
-        g = d.cg;
+        g = kCoupled ? d.lcg : d.cg;
         nc = (d.ns + g - 1) / g;
         ncp = coarse_stride(nc);
         crow = kCoupled ? 2 * ncp : ncp;
@@ -551,12 +538,13 @@ UVA_HD __forceinline__ CGOut cg_solve(Team& tm, const DevPlan& P, const Isl& d, 
     // of team barriers.  The choice is made for the largest member share
     // (members' shares differ by one slice), which every member computes alike.
     const int m = tm.members();
+    const int g = OP == OP_LSCM ? d.lcg : d.cg;     // slices per coarse aggregate
     int nsl = 0, na = 0;
     for (int k = 0; k < m; ++k) {
         const int lo = (int)((long long)d.ns * k / m), hi = (int)((long long)d.ns * (k + 1) / m);
         nsl = hi - lo > nsl ? hi - lo : nsl;
-        if (d.cg > 0 && hi > lo) {
-            const int a = (hi - 1) / d.cg - lo / d.cg + 1;
+        if (g > 0 && hi > lo) {
+            const int a = (hi - 1) / g - lo / g + 1;
             na = a > na ? a : na;
         }
     }
@@ -565,7 +553,7 @@ UVA_HD __forceinline__ CGOut cg_solve(Team& tm, const DevPlan& P, const Isl& d, 
     const int vec = res_vec_bytes<V2, V>(nsl * kSlice, tm.multi() ? kHalo : 0);
     if (tm.lvec() != nullptr && vec <= tm.lvec_bytes() && rpt <= kMaxRpt) {
         constexpr bool cp = OP == OP_LSCM;
-        const int nc = d.cg > 0 ? (d.ns + d.cg - 1) / d.cg : 0;
+        const int nc = g > 0 ? (d.ns + g - 1) / g : 0;
         const bool co = coarse && (cp ? d.lcofs : d.cofs) >= 0 &&
                         vec + res_coarse_bytes(nc, na, nsl, cp) <= tm.lvec_bytes();
         if (rpt <= 2) return ResCG<PREC, 2, Team, OP>(tm, P, d, co, p0, p1).solve(b2, tol, maxit);

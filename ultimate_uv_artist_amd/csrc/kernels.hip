@@ -419,6 +419,7 @@ struct TeamWG {
 struct Isl {
     int v0, nv, t0, nt, s0, ns, pin, bw;
     int cg;               // slices per coarse aggregate
+    int lcg;              // the same for the LSCM operator's coarse level
     long long cofs;       // offset of the inverse coarse operator, -1 without one
     long long lcofs;      // the same for the LSCM operator
     double wpin;
@@ -436,6 +437,7 @@ UVA_HD __forceinline__ Isl load_isl(const DevPlan& P, int i) {
     d.pin = uniform_i(P.isl_pin[i]); d.wpin = uniform_d(P.isl_wpin[i]);
     d.bw = uniform_i(P.isl_bw[i]);
     d.cg = uniform_i(P.isl_cg[i]);
+    d.lcg = P.isl_lcg ? uniform_i(P.isl_lcg[i]) : d.cg;
     d.cofs = uniform_ll(P.isl_cofs[i]);
     d.lcofs = P.isl_lcofs ? uniform_ll(P.isl_lcofs[i]) : -1;
     return d;

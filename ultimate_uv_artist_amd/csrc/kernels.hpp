@@ -10,11 +10,6 @@ namespace uva {
 constexpr int kBlock = 1024;    // workgroup size of the island kernel (16 wavefronts)
 constexpr int kBlockF64 = 512;  // FP64 instantiations (more registers per lane)
 constexpr int kBlockRes = 512;  // kinds with the resident CG (up to 8 rows per lane in registers)
-// LDS bytes of the resident CG per workgroup: u with halo, e, D^-1, D (80 KB
-// for 3072 FP32 rows) and the coarse level (inverse rows and two vectors of
-// up to kCoarseN aggregates, 68 KB)
-constexpr int kLdsVec = 151552;
-constexpr int kHalo = 512;      // rows of u a team member keeps beyond each end of its range
 
 enum TaskKind {
     TASK_PROLOGUE = 0,   // flips of the start, line-search decision, rotations + energy

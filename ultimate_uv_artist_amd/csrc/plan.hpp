@@ -22,12 +22,35 @@ constexpr int kWaveMaxRows = 384;   // islands up to this many vertices run on o
 constexpr int kResSlices = 48;      // slices one workgroup keeps resident in the FP32 ARAP CG (8 waves x 6)
 constexpr int kResSlicesF64 = 32;   // the same for FP64 vectors (8 waves x 4): the LSCM solve
 constexpr int kMaxTeam = 16;        // most workgroups on one island
-constexpr int kCoarseN = 128;       // most aggregates of an island's coarse level (64 KB inverse)
+constexpr int kCoarseN = 128;       // most aggregates of a general matrix's coarse level (64 KB inverse)
 constexpr int kTeamMinRows = 2048;  // fewest rows per team member
 constexpr int kMaxXcd = 8;          // XCDs (own L2 each) a device may have
 // Task counters of one launch: one per XCD list of workgroup tasks, then one
 // for the wavefront tasks.
 constexpr int kQueueStride = kMaxXcd + 1;
+
+// LDS of the resident CG per workgroup: u with its halo, e, D^-1 and D (80 KB
+// for 3072 FP32 rows) and the coarse level (rows of the inverse coarse
+// operator for the member's aggregates, the restricted residual of every
+// aggregate, the member's correction, mat-vec shares and slice sums)
+constexpr int kLdsVec = 151552;
+constexpr int kHalo = 512;          // rows of u a team member keeps beyond each end of its range
+constexpr int kPart = 8;            // threads sharing one row of the coarse mat-vec
+
+__host__ __device__ constexpr int align16(int b) { return (b + 15) & ~15; }
+// rows of Ac^-1 are stored with an odd stride, so the threads of different
+// aggregates read different LDS banks
+__host__ __device__ constexpr int coarse_stride(int nc) { return nc | 1; }
+// vectors of the resident CG: u (rows + both halos) and e of v2 bytes per
+// row, D^-1 and D of v bytes
+__host__ __device__ constexpr int res_vec_bytes_of(int rows, int halo, int v2, int v) {
+    return align16((rows + 2 * halo) * v2 + rows * (v2 + 2 * v));
+}
+// the coarse level of a member with na of the nc aggregates and nsl slices
+// (rows of Ac^-1 twice as long for the coupled LSCM operator)
+__host__ __device__ constexpr int res_coarse_bytes(int nc, int na, int nsl, bool coupled = false) {
+    return nc > 0 ? align16(na * coarse_stride(nc) * 4 * (coupled ? 2 : 1)) + (nc + na + na * kPart + nsl) * 16 : 0;
+}
 
 struct IslandState {
     double e_prev;       // ARAP energy of the current layout
@@ -122,6 +145,7 @@ struct DevPlan {
     // residual (FP64, two columns) of every slice during a solve
     const int64_t* isl_cofs;
     const int* isl_cg;           // slices per aggregate
+    const int* isl_lcg;          // slices per aggregate of the LSCM coarse level
     const float* cinv;
     // the LSCM operator's coarse level over the same aggregates: island i's
     // inverse [[Pc, Qc], [-Qc, Pc]] as nc rows of [Pc | Qc] at isl_lcofs[i]

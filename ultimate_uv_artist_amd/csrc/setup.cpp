@@ -287,6 +287,8 @@ std::vector<int> morton_order(int nv, const double* P) {
     return perm;
 }
 
+bool spd_inverse(std::vector<double>& A, int n, std::vector<double>& S);
+
 // Inverse of the coarse operator Ac = Z^T A Z of one island (A = Laplacian
 // + pin weight; Z = indicator of the aggregates, g consecutive 64-row slices
 // each), symmetric, in FP32.  Returns false if Ac is not numerically positive
@@ -302,38 +304,38 @@ bool coarse_inverse(int nv, int g, const VP& row_ptr, const VC& row_col, const V
         A[(size_t)a * ns + a] += diag[i] + (i == pin ? wpin : 0.0);
         for (int q = row_ptr[i]; q < row_ptr[i + 1]; ++q) A[(size_t)a * ns + row_col[q] / rows] += row_val[q];
     }
-    // Cholesky A = L L^T (lower, in place)
-    for (int j = 0; j < ns; ++j) {
-        double dj = A[(size_t)j * ns + j];
-        for (int k = 0; k < j; ++k) dj -= A[(size_t)j * ns + k] * A[(size_t)j * ns + k];
-        if (!(dj > 0.0) || !std::isfinite(dj)) return false;
-        const double ljj = std::sqrt(dj);
-        A[(size_t)j * ns + j] = ljj;
-        for (int i = j + 1; i < ns; ++i) {
-            double v = A[(size_t)i * ns + j];
-            for (int k = 0; k < j; ++k) v -= A[(size_t)i * ns + k] * A[(size_t)j * ns + k];
-            A[(size_t)i * ns + j] = v / ljj;
-        }
-    }
-    // M = L^-1 (lower), then Ac^-1 = M^T M
-    std::vector<double> M((size_t)ns * ns, 0.0);
-    for (int c = 0; c < ns; ++c) {
-        M[(size_t)c * ns + c] = 1.0 / A[(size_t)c * ns + c];
-        for (int i = c + 1; i < ns; ++i) {
-            double v = 0.0;
-            for (int k = c; k < i; ++k) v -= A[(size_t)i * ns + k] * M[(size_t)k * ns + c];
-            M[(size_t)i * ns + c] = v / A[(size_t)i * ns + i];
-        }
-    }
-    for (int a = 0; a < ns; ++a)
-        for (int b = 0; b <= a; ++b) {
-            double v = 0.0;
-            for (int k = a; k < ns; ++k) v += M[(size_t)k * ns + a] * M[(size_t)k * ns + b];
-            if (!std::isfinite(v)) return false;
-            out[(size_t)a * ns + b] = (float)v;
-            out[(size_t)b * ns + a] = (float)v;
-        }
+    std::vector<double> S;
+    if (!spd_inverse(A, ns, S)) return false;
+    for (size_t k = 0; k < S.size(); ++k) out[k] = (float)S[k];
     return true;
+}
+
+// Slices per coarse aggregate of an island of ns slices: the smallest g whose
+// coarse level fits beside the resident CG's vectors (v2 / v bytes per row and
+// component) in the LDS of every member of every team the schedule may give
+// the island (at least ceil(ns / res_slices) members, each holding at most
+// res_slices slices; kMaxTeam at most), with the members' slice shares and
+// aggregate counts as the kernel computes them.
+int aggregate_slices(int ns, int res_slices, int v2, int v, bool coupled) {
+    if (ns <= 1) return 1;
+    const int m_min = std::max(1, (ns + res_slices - 1) / res_slices);
+    const int m_max = std::max(m_min, std::min(kMaxTeam, ns));
+    for (int g = 1; g < ns; ++g) {
+        const int nc = (ns + g - 1) / g;
+        bool ok = true;
+        for (int m = m_min; m <= m_max && ok; ++m) {
+            int nsl = 0, na = 0;
+            for (int k = 0; k < m; ++k) {
+                const int lo = (int)((long long)ns * k / m), hi = (int)((long long)ns * (k + 1) / m);
+                nsl = std::max(nsl, hi - lo);
+                if (hi > lo) na = std::max(na, (hi - 1) / g - lo / g + 1);
+            }
+            ok = res_vec_bytes_of(nsl * kSlice, m > 1 ? kHalo : 0, v2, v) + res_coarse_bytes(nc, na, nsl, coupled) <=
+                 kLdsVec;
+        }
+        if (ok) return g;
+    }
+    return ns;
 }
 
 // Inverse of a dense SPD matrix A (n x n, row-major, overwritten by its
@@ -542,27 +544,26 @@ std::string build_host_plan(HostPlan& hp, int64_t n_islands, const int64_t* vert
         hp.m_col[o] = 0; hp.m_hi[o] = 0.0f; hp.m_lo[o] = 0.0f; hp.m_b[o] = 0.0f;
     }
     hp.diag.resize(nv_all); hp.dinv32.resize(nv_all); hp.vbnd.resize(nv_all); hp.vperm.resize(nv_all);
-    // coarse level: aggregates of g consecutive slices, at most kCoarseN of them
+    // coarse level: aggregates of g consecutive slices, as small as the
+    // resident CG's LDS holds (FP32 ARAP vectors; FP64 LSCM vectors with the
+    // coupled rows)
     hp.isl_cofs.assign(n, -1);
     hp.isl_cg.assign(n, 1);
+    hp.isl_lcg.assign(n, 1);
     {
-        int64_t c = 0;
+        int64_t c = 0, c2 = 0;
+        hp.isl_lcofs.assign(n, -1);
         for (int i = 0; i < n; ++i) {
             const int ns = (int)widths[i].size();
-            const int g = (ns + kCoarseN - 1) / kCoarseN;
-            const int64_t nc = (ns + g - 1) / g;
-            hp.isl_cg[i] = g;
+            hp.isl_cg[i] = aggregate_slices(ns, kResSlices, 8, 4, false);
+            hp.isl_lcg[i] = aggregate_slices(ns, kResSlicesF64, 16, 8, true);
+            const int64_t nc = (ns + hp.isl_cg[i] - 1) / hp.isl_cg[i];
+            const int64_t ncl = (ns + hp.isl_lcg[i] - 1) / hp.isl_lcg[i];
             if (coarse && nc >= 2) { hp.isl_cofs[i] = c; c += nc * nc; }
+            // the LSCM operator's coarse level (islands with a boundary): rows of [Pc | Qc]
+            if (coarse && ncl >= 2 && isl[i].lp0 >= 0) { hp.isl_lcofs[i] = c2; c2 += 2 * ncl * ncl; }
         }
         hp.cinv.assign(std::max<int64_t>(c, 1), 0.0f);
-        // the LSCM operator's coarse level (islands with a boundary): rows of [Pc | Qc]
-        hp.isl_lcofs.assign(n, -1);
-        int64_t c2 = 0;
-        for (int i = 0; i < n; ++i) {
-            const int ns = (int)widths[i].size();
-            const int64_t nc = (ns + hp.isl_cg[i] - 1) / hp.isl_cg[i];
-            if (coarse && nc >= 2 && isl[i].lp0 >= 0) { hp.isl_lcofs[i] = c2; c2 += 2 * nc * nc; }
-        }
         hp.lcinv.assign(std::max<int64_t>(c2, 1), 0.0f);
     }
     hp.tv0.resize(nt_all); hp.tv1.resize(nt_all); hp.tv2.resize(nt_all);
@@ -648,7 +649,7 @@ std::string build_host_plan(HostPlan& hp, int64_t n_islands, const int64_t* vert
                                                    wp, hp.cinv.data() + hp.isl_cofs[i]))
             hp.isl_cofs[i] = -1;
         if (hp.isl_lcofs[i] >= 0 &&
-            !lscm_coarse_inverse(I.nv, hp.isl_cg[i], I.row_ptr, I.row_col, I.row_val, I.row_b, I.diag, I.lp0, I.lp1,
+            !lscm_coarse_inverse(I.nv, hp.isl_lcg[i], I.row_ptr, I.row_col, I.row_val, I.row_b, I.diag, I.lp0, I.lp1,
                                  hp.lcinv.data() + hp.isl_lcofs[i]))
             hp.isl_lcofs[i] = -1;
         // sliced ELL
@@ -878,6 +879,7 @@ std::string build_matrix_plan(HostPlan& hp, int64_t n, const int64_t* indptr, co
     const int g = (ns + kCoarseN - 1) / kCoarseN;
     const int nc = (ns + g - 1) / g;
     hp.isl_cg.assign(1, g);
+    hp.isl_lcg.assign(1, g);
     hp.cinv.assign(1, 0.0f);
     if (nc >= 2) {
         std::vector<double> val(hp.adj_idx.size());

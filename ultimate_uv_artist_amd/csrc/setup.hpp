@@ -83,10 +83,11 @@ struct HostPlan {
     // isl_cg[i] slices) in cinv, or -1 without a coarse level
     std::vector<int64_t> isl_cofs;
     std::vector<int> isl_cg;      // slices per aggregate
+    std::vector<int> isl_lcg;     // slices per aggregate of the LSCM coarse level
     std::vector<float> cinv;
-    // coarse level of the LSCM operator over the same aggregates (for the
-    // plan's LSCM pins): isl_lcofs[i] is the offset of island i's nc rows of
-    // [Pc | Qc] (2 nc FP32 values each) in lcinv, or -1
+    // coarse level of the LSCM operator over aggregates of isl_lcg[i] slices
+    // (for the plan's LSCM pins): isl_lcofs[i] is the offset of island i's nc
+    // rows of [Pc | Qc] (2 nc FP32 values each) in lcinv, or -1
     std::vector<int64_t> isl_lcofs;
     std::vector<float> lcinv;
 };
