@@ -62,6 +62,10 @@ def parse():
                          "small: charts.flatten of 40-90-face charts from 8 threads (per-call param, "
                          "FlattenQueue, flatten_many) against the oracle on 1 and 8 cores")
     ap.add_argument("--charts", type=int, default=2000, help="charts of the small workload")
+    ap.add_argument("--gen-workers", type=int, default=None, metavar="N",
+                    help="worker processes that generate the islands (default: up to 8; use 1 under a "
+                         "profiler that initialises the GPU before the program starts, since the workers "
+                         "are forked)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB 5 times (HBM counter calibration)")
     return ap.parse_args()
@@ -177,7 +181,7 @@ def precision_study(args):
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
     t0 = time.perf_counter()
-    charts = workloads.sliver_islands(seed=3, n_islands=256)
+    charts = workloads.sliver_islands(seed=3, n_islands=256, workers=args.gen_workers)
     tri = np.array([len(T) for _, T in charts])
     print(f"bench: {len(charts)} needle islands, {tri.sum()} triangles (generated in "
           f"{time.perf_counter() - t0:.1f}s)", flush=True)
@@ -242,7 +246,7 @@ def validate_study(args):
 
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
-    charts = workloads.disk_islands(seed=2, n_islands=args.islands)
+    charts = workloads.disk_islands(seed=2, n_islands=args.islands, workers=args.gen_workers)
     plan = IslandPlan(charts, dev=dev)
     U = torch.zeros(plan.n, 2, dtype=torch.float64, device=dev)
     plan.lscm(U)
@@ -303,7 +307,7 @@ def batch_study(args):
 
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
-    charts = workloads.disk_islands(seed=2, n_islands=args.islands)
+    charts = workloads.disk_islands(seed=2, n_islands=args.islands, workers=args.gen_workers)
     plan = IslandPlan(charts, dev=dev)
     U = torch.zeros(plan.n, 2, dtype=torch.float64, device=dev)
     plan.lscm(U)
@@ -510,7 +514,7 @@ def hier_study(args):
 
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
-    charts = workloads.disk_islands(seed=2, n_islands=args.islands)
+    charts = workloads.disk_islands(seed=2, n_islands=args.islands, workers=args.gen_workers)
     plan = IslandPlan(charts, dev=dev)
     U = torch.zeros(plan.n, 2, dtype=torch.float64, device=dev)
     plan.lscm(U)
@@ -611,7 +615,7 @@ def main():
         wl = (f"{len(charts)} heightfield islands (1001x1001 fBm grid, Voronoi cells), LSCM + {args.iters} ARAP, "
               f"CG rel resid {args.cg_tol:g}")
     else:
-        charts = workloads.disk_islands(seed=2, n_islands=args.islands)
+        charts = workloads.disk_islands(seed=2, n_islands=args.islands, workers=args.gen_workers)
         wl = (f"{len(charts)} disk islands, t log-uniform [200,40000], LSCM + {args.iters} ARAP, "
               f"CG rel resid {args.cg_tol:g}")
     t_gen = time.perf_counter() - t0

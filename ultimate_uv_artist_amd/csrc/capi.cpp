@@ -66,7 +66,7 @@ struct uva_plan {
         const int* grp_m = nullptr;
         const int* grp_team = nullptr;
         const int* slot_ptr = nullptr;
-        int n_group = 0, n_teams = 0, max_m = 1, slots_per_xcd = 1;
+        int n_group = 0, n_teams = 0, max_m = 1, slots_per_xcd = 1, claim_tasks = 0;
     } lsched;
     bool have_lsched = false;
     std::vector<int> lmembers;             // per island: workgroups of its LSCM team
@@ -142,6 +142,7 @@ uva::DevPlan lscm_view(const uva_plan* pl) {
     d.n_teams = l.n_teams;
     d.max_m = l.max_m;
     d.slots_per_xcd = l.slots_per_xcd;
+    d.claim_tasks = l.claim_tasks;
     return d;
 }
 
@@ -470,6 +471,7 @@ int upload_plan(uva::HostPlan& hp, const double* P, hipStream_t s, uva_plan** ou
     d.n_xcd = hp.n_xcd;
     d.max_m = hp.max_m;
     d.slots_per_xcd = hp.slots_per_xcd;
+    d.claim_tasks = hp.claim_tasks ? 1 : 0;
     const XccInfo xi = xcc_info();
     d.xcd_local = (hp.n_xcd > 1 && hp.n_xcd == xi.n) ? 1 : 0;
     // an XCD the probe never saw has no list (its workgroups only take wavefront tasks)
@@ -643,6 +645,7 @@ int replan(uva_plan* plan, const double* est, hipStream_t s) {
     d.n_teams = sc.n_teams;
     d.max_m = sc.max_m;
     d.slots_per_xcd = sc.slots_per_xcd;
+    d.claim_tasks = sc.claim_tasks ? 1 : 0;
     // a cached graph holds the old launch parameters
     if (plan->gexec) { (void)hipGraphExecDestroy(plan->gexec); plan->gexec = nullptr; }
     if (plan->graph) { (void)hipGraphDestroy(plan->graph); plan->graph = nullptr; }
@@ -692,6 +695,7 @@ int plan_lscm(uva_plan* plan, const double* est, hipStream_t s) {
     l.n_teams = sc.n_teams;
     l.max_m = sc.max_m;
     l.slots_per_xcd = sc.slots_per_xcd;
+    l.claim_tasks = sc.claim_tasks ? 1 : 0;
     plan->lmembers = island_members(sc);
     plan->have_lsched = true;
     HIP_TRY(hipStreamSynchronize(s));

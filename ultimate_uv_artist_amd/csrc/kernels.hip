@@ -1139,14 +1139,22 @@ __global__ __launch_bounds__(BS) void island_kernel(DevPlan P, TaskArgs a, int s
     int* const q = P.queue + kQueueStride * slot;
     for (int t_cur = 0, t_end = 0; list >= 0;) {
         if (t_cur == t_end) {
-            // claim the next unclaimed slot of the XCD
+            // claim the next unclaimed slot (or task) of the XCD
             if (threadIdx.x == 0) s_task = atomicAdd(q + list, 1);
             __syncthreads();
             const int k = uniform_i(s_task);
             __syncthreads();
-            if (k >= P.slots_per_xcd) break;
-            t_cur = uniform_i(P.slot_ptr[list * P.slots_per_xcd + k]);
-            t_end = uniform_i(P.slot_ptr[list * P.slots_per_xcd + k + 1]);
+            if (P.claim_tasks) {
+                // the next task of the XCD's list
+                const int b = uniform_i(P.slot_ptr[list]), e = uniform_i(P.slot_ptr[list + 1]);
+                if (k >= e - b) break;
+                t_cur = b + k;
+                t_end = t_cur + 1;
+            } else {
+                if (k >= P.slots_per_xcd) break;
+                t_cur = uniform_i(P.slot_ptr[list * P.slots_per_xcd + k]);
+                t_end = uniform_i(P.slot_ptr[list * P.slots_per_xcd + k + 1]);
+            }
             continue;
         }
         const int task = t_cur++;

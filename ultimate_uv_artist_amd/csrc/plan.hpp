@@ -163,17 +163,24 @@ struct DevPlan {
     int n_teams;
     int* team_cnt;               // [launch slot][team] barrier counters, zeroed per call sequence
     double* team_part;           // [team][2][kMaxTeam][8] partial sums
-    // Static schedule of the workgroup tasks: slots_per_xcd slots per XCD;
-    // slot q = x * slots_per_xcd + k runs tasks [slot_ptr[q], slot_ptr[q+1])
-    // of order_group in order.  A workgroup claims the slots of the XCD it
-    // runs on (hardware id -> list x through xcc_map), so all members of a
-    // team share one L2 and their barrier needs no L2 write-back or
-    // invalidation (xcd_local).  Without a probed XCD map there is one list
-    // and teams use agent-scope release/acquire.
+    // Schedule of the workgroup tasks, slots_per_xcd workgroups per XCD.  A
+    // workgroup takes tasks of the XCD it runs on (hardware id -> list x
+    // through xcc_map), so all members of a team share one L2 and their
+    // barrier needs no L2 write-back or invalidation (xcd_local).  Without a
+    // probed XCD map there is one list and teams use agent-scope
+    // release/acquire.
+    //   claim_tasks = 0: static slot lists; slot q = x * slots_per_xcd + k runs
+    //     tasks [slot_ptr[q], slot_ptr[q+1]) of order_group in order, and a
+    //     workgroup claims the next slot of its XCD.
+    //   claim_tasks = 1: one list per XCD, tasks [slot_ptr[x], slot_ptr[x+1]),
+    //     longest first, a team's members consecutive; a workgroup claims the
+    //     next task, so a team is formed by the next workgroups of the XCD that
+    //     come free, whatever the earlier tasks took.
     int n_xcd;
     int xcd_local;
     int max_m;                   // largest team
     int slots_per_xcd;
+    int claim_tasks;
     const int* slot_ptr;
     int xcc_map[16];
     long long spin_ticks;        // hang guard of a team barrier (device clock ticks)

@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <thread>
 
@@ -693,6 +694,13 @@ std::string build_host_plan(HostPlan& hp, int64_t n_islands, const int64_t* vert
     return "";
 }
 
+// Workgroups claim single tasks from one list per XCD unless UVART_SCHED is
+// "static" (slot lists planned on the host).
+bool claim_tasks_by_default() {
+    const char* e = std::getenv("UVART_SCHED");
+    return !(e && std::strcmp(e, "static") == 0);
+}
+
 void plan_teams(HostPlan& hp, const std::vector<int>& group_islands, int slots, int n_xcd, bool teams,
                 const double* est_iters, int res_slices) {
     hp.order_group.clear(); hp.grp_mem.clear(); hp.grp_m.clear(); hp.grp_team.clear(); hp.slot_ptr.clear();
@@ -745,6 +753,26 @@ void plan_teams(HostPlan& hp, const std::vector<int>& group_islands, int slots, 
         const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
         load[x] += it.t * it.m;
         lists[x].push_back(it);
+    }
+    hp.claim_tasks = claim_tasks_by_default();
+    if (hp.claim_tasks) {
+        // one list per XCD, longest first, a team's members consecutive: the
+        // workgroups claim its tasks one at a time
+        hp.slot_ptr.assign(n_xcd + 1, 0);
+        for (int x = 0; x < n_xcd; ++x) {
+            for (const Item& it : lists[x]) {
+                const int team = it.m > 1 ? hp.n_teams++ : -1;
+                hp.max_m = std::max(hp.max_m, it.m);
+                for (int k = 0; k < it.m; ++k) {
+                    hp.order_group.push_back(it.isl);
+                    hp.grp_mem.push_back(k);
+                    hp.grp_m.push_back(it.m);
+                    hp.grp_team.push_back(team);
+                }
+            }
+            hp.slot_ptr[x + 1] = (int)hp.order_group.size();
+        }
+        return;
     }
     // per XCD: longest first, each island's members on the slots that free
     // up first, starting together; every slot keeps the global order

@@ -60,10 +60,13 @@ struct HostPlan {
     std::vector<int> grp_mem, grp_m, grp_team;
     int n_teams = 0;
     int max_m = 1;
-    // static schedule: n_xcd x slots_per_xcd workgroup slots; slot q runs the
-    // tasks [slot_ptr[q], slot_ptr[q+1]) of order_group in order
+    // schedule: n_xcd x slots_per_xcd workgroup slots; slot q runs the tasks
+    // [slot_ptr[q], slot_ptr[q+1]) of order_group in order, or (claim_tasks)
+    // XCD x's workgroups claim the tasks [slot_ptr[x], slot_ptr[x+1]) one at a
+    // time
     int n_xcd = 1;
     int slots_per_xcd = 1;
+    bool claim_tasks = false;
     std::vector<int> slot_ptr{0, 0};
     // inputs of the schedule, kept to re-plan it from measured iterations
     std::vector<int> group_isl;
