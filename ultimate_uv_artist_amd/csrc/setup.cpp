@@ -20,7 +20,7 @@ namespace uva {
 namespace {
 
 struct Edge {
-    int64_t key;     // a * nv + b with a < b
+    int a, b;        // a < b
     double w;        // half-cotangent weight of one triangle edge
     int dir;         // +1 for a->b as the triangle's half-edge, -1 for b->a
 };
@@ -140,38 +140,61 @@ void build_island(Island& I, int nv, int nt, const double* P, const int64_t* T, 
     // coupling.  The edge opposite corner k, taken from corner k+1 to k+2, is
     // one of the triangle's directed half-edges (0->1, 1->2, 2->0), so the
     // directions within a group of equal keys count the half-edges each way.
-    std::vector<Edge> E(3 * (size_t)nt);
+    // (the edge arrays are per-thread buffers kept across islands, so only
+    // their growth touches new memory)
+    thread_local std::vector<Edge> E, S;
+    thread_local std::vector<int> pos;
+    E.resize(3 * (size_t)nt);
     for (int f = 0; f < nt; ++f) {
         for (int k = 0; k < 3; ++k) {
-            int64_t a = T[3 * (size_t)f + (k + 1) % 3], b = T[3 * (size_t)f + (k + 2) % 3];
+            const int a = (int)T[3 * (size_t)f + (k + 1) % 3], b = (int)T[3 * (size_t)f + (k + 2) % 3];
             Edge& e = E[3 * (size_t)f + k];
-            e.key = std::min(a, b) * nv + std::max(a, b);
+            e.a = std::min(a, b);
+            e.b = std::max(a, b);
             e.w = w[3 * (size_t)f + k];
             e.dir = a < b ? 1 : -1;
         }
     }
-    std::sort(E.begin(), E.end(), [](const Edge& x, const Edge& y) { return x.key < y.key; });
-    std::vector<int64_t> uk;
+    // sort by (a, b): a counting sort by a (stable), then an insertion sort of
+    // each vertex's few edges to larger vertices
+    pos.assign(nv + 1, 0);
+    for (const Edge& e : E) pos[e.a + 1]++;
+    for (int v = 0; v < nv; ++v) pos[v + 1] += pos[v];
+    S.resize(E.size());
+    for (const Edge& e : E) S[pos[e.a]++] = e;     // pos[v] ends at the start of v + 1
+    for (int v = nv; v > 0; --v) pos[v] = pos[v - 1];
+    pos[0] = 0;
+    for (int v = 0; v < nv; ++v)
+        for (int q = pos[v] + 1; q < pos[v + 1]; ++q) {
+            const Edge x = S[q];
+            int r = q - 1;
+            for (; r >= pos[v] && S[r].b > x.b; --r) S[r + 1] = S[r];
+            S[r + 1] = x;
+        }
+    std::vector<int> ua, ubv;
     std::vector<double> uw, ub;
     std::vector<int> ucnt;
     std::vector<int> bsrc;       // sources of boundary half-edges
-    for (size_t q = 0; q < E.size();) {
+    ua.reserve(S.size() / 2 + 16); ubv.reserve(S.size() / 2 + 16);
+    uw.reserve(S.size() / 2 + 16); ub.reserve(S.size() / 2 + 16); ucnt.reserve(S.size() / 2 + 16);
+    for (size_t q = 0; q < S.size();) {
         size_t r = q;
         double ws = 0.0;
         int nab = 0, nba = 0;    // half-edges a->b and b->a (a < b)
-        while (r < E.size() && E[r].key == E[q].key) {
-            ws += E[r].w;
-            if (E[r].dir > 0) ++nab;
+        while (r < S.size() && S[r].a == S[q].a && S[r].b == S[q].b) {
+            ws += S[r].w;
+            if (S[r].dir > 0) ++nab;
             else ++nba;
             ++r;
         }
-        int64_t a = E[q].key / nv, b = E[q].key % nv;
+        const int a = S[q].a, b = S[q].b;
         // every boundary half-edge (its reverse absent) a->b adds B_ab -= 1/2,
         // B_ba += 1/2 (param.py:142-150)
         double beta = 0.0;
-        if (nab && !nba) { beta -= 0.5 * nab; bsrc.push_back((int)a); }
-        if (nba && !nab) { beta += 0.5 * nba; bsrc.push_back((int)b); }
-        uk.push_back(E[q].key);
+        if (nab && !nba) { beta -= 0.5 * nab; bsrc.push_back(a); }
+        if (nba && !nab) { beta += 0.5 * nba; bsrc.push_back(b); }
+        ua.push_back(a);
+        ubv.push_back(b);
         uw.push_back(ws);
         ub.push_back(beta);
         ucnt.push_back((int)(r - q));
@@ -179,7 +202,7 @@ void build_island(Island& I, int nv, int nt, const double* P, const int64_t* T, 
     }
     // rows
     std::vector<int> deg(nv, 0);
-    for (size_t u = 0; u < uk.size(); ++u) { deg[uk[u] / nv]++; deg[uk[u] % nv]++; }
+    for (size_t u = 0; u < ua.size(); ++u) { deg[ua[u]]++; deg[ubv[u]]++; }
     I.row_ptr.assign(nv + 1, 0);
     for (int v = 0; v < nv; ++v) I.row_ptr[v + 1] = I.row_ptr[v] + deg[v];
     I.row_col.resize(I.row_ptr[nv]);
@@ -188,8 +211,8 @@ void build_island(Island& I, int nv, int nt, const double* P, const int64_t* T, 
     I.diag.assign(nv, 0.0);
     I.bnd.assign(nv, 0);
     std::vector<int> fill(I.row_ptr.begin(), I.row_ptr.end() - 1);
-    for (size_t u = 0; u < uk.size(); ++u) {
-        int a = (int)(uk[u] / nv), b = (int)(uk[u] % nv);
+    for (size_t u = 0; u < ua.size(); ++u) {
+        const int a = ua[u], b = ubv[u];
         int pa = fill[a]++, pb = fill[b]++;
         I.row_col[pa] = b; I.row_val[pa] = -uw[u]; I.row_b[pa] = ub[u];
         I.row_col[pb] = a; I.row_val[pb] = -uw[u]; I.row_b[pb] = -ub[u];
@@ -429,11 +452,22 @@ bool lscm_coarse_inverse(int nv, int g, const std::vector<int>& row_ptr, const s
     return true;
 }
 
+// Threads of the host plan build: the hardware threads, at most 16, or
+// UVART_HOST_THREADS.
+int host_threads() {
+    static const int n = [] {
+        const char* e = std::getenv("UVART_HOST_THREADS");
+        if (e && std::atoi(e) > 0) return std::atoi(e);
+        return (int)std::min<unsigned>(std::max(1u, std::thread::hardware_concurrency()), 16u);
+    }();
+    return n;
+}
+
 template <class F>
 void parallel_for(int n, F fn, const std::vector<int>* order = nullptr) {
     // (order: the items in the order threads take them, e.g. largest first,
     // so one large item is not left for last)
-    int nth = (int)std::min<unsigned>(std::max(1u, std::thread::hardware_concurrency()), 16u);
+    int nth = host_threads();
     nth = std::min(nth, std::max(1, n));
     std::atomic<int> next(0);
     auto work = [&]() {
@@ -642,9 +676,14 @@ std::string build_host_plan(HostPlan& hp, int64_t n_islands, const int64_t* vert
                 hp.adj_idx[a++] = v0 + I.row_col[q];
                 hp.isl_bw[i] = std::max(hp.isl_bw[i], std::abs(I.row_col[q] - v));
             }
-            std::sort(hp.adj_idx.begin() + a0, hp.adj_idx.begin() + a, [&](int x, int y) {
-                return perm[i][x - v0] < perm[i][y - v0];
-            });
+            // (a vertex has a few neighbours: insertion sort by caller id)
+            const int* pi = perm[i].data();
+            for (int q = a0 + 1; q < a; ++q) {
+                const int x = hp.adj_idx[q], kx = pi[x - v0];
+                int r = q - 1;
+                for (; r >= a0 && pi[hp.adj_idx[r] - v0] > kx; --r) hp.adj_idx[r + 1] = hp.adj_idx[r];
+                hp.adj_idx[r + 1] = x;
+            }
         }
         if (hp.isl_cofs[i] >= 0 && !coarse_inverse(I.nv, hp.isl_cg[i], I.row_ptr, I.row_col, I.row_val, I.diag, lp,
                                                    wp, hp.cinv.data() + hp.isl_cofs[i]))

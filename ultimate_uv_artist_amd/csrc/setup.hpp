@@ -3,6 +3,10 @@
 #pragma once
 #include <stdint.h>
 
+#include <sys/mman.h>
+
+#include <cstdlib>
+#include <new>
 #include <string>
 #include <memory>
 #include <utility>
@@ -25,6 +29,21 @@ struct NoInitAlloc : std::allocator<T> {
     NoInitAlloc() = default;
     template <class U>
     NoInitAlloc(const NoInitAlloc<U>&) noexcept {}
+    // arrays of 4 MiB or more on 2 MiB boundaries, marked for huge pages: the
+    // first touch then faults once per 2 MiB instead of once per 4 KiB
+    T* allocate(std::size_t n) {
+        const std::size_t bytes = n * sizeof(T);
+        if (bytes < (std::size_t(4) << 20)) return std::allocator<T>::allocate(n);
+        const std::size_t huge = std::size_t(2) << 20;
+        void* p = std::aligned_alloc(huge, (bytes + huge - 1) / huge * huge);
+        if (!p) throw std::bad_alloc();
+        (void)madvise(p, bytes, MADV_HUGEPAGE);
+        return static_cast<T*>(p);
+    }
+    void deallocate(T* p, std::size_t n) {
+        if (n * sizeof(T) < (std::size_t(4) << 20)) std::allocator<T>::deallocate(p, n);
+        else std::free(p);
+    }
     template <class U>
     void construct(U* p) noexcept { ::new ((void*)p) U; }
     template <class U, class... A>
