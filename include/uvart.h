@@ -39,6 +39,8 @@
  *   uva_repair_jacobi   gpu.py:251-273 Batch.repair_flips
  *   uva_orient_positive gpu.py:275-281 Batch.orient_positive
  *   uva_repair_seq      param.py:478-516 repair_flips (host, exact order)
+ *   uva_chart_regroup   hier.py:398-408 (the subset's faces, local vertex ids
+ *                       and chart ids that Solver.relax starts from; host)
  *   uva_overlap_flags   validate.py:94-136 _overlaps, 257-299 _overlaps_strict
  *   uva_jacobi_pcg      hier.py:565-577 (the CG of Solver.lscm)
  *   uva_csr_pcg         gpu.py:184-202 Batch._cg; hier.py:465-474, 565-577
@@ -349,6 +351,21 @@ int uva_orient_positive(uva_plan *plan, double *U, void *stream);
  * host (the reference's exact update order).  U (n x 2 float64) is updated
  * in place; T is t x 3 int64. */
 int uva_repair_seq(double *U, int64_t n, const int64_t *T, int64_t t, int32_t rounds);
+
+/* hier.Solver.relax's subset as islands, on the host with threads: n faces
+ * (faces: their ids in the mesh mesh_F, n_mesh_f x 3 global vertex ids into
+ * mesh_V, n_mesh_v x 3), their local vertex ids CV (n x 3, in [0, nv)) and
+ * chart ids (n, in [0, nchart)).  Island k is the k-th chart with faces
+ * (keep[k], *nk of them): its vertices in ascending local id, its faces in
+ * subset order.  Outputs (caller-allocated): keep (nchart), vert_off and
+ * tri_off (nk + 1), P (nv x 3, each vertex's mesh position), T (n x 3,
+ * island-local ids), vorder (nv: local id of each island-order vertex), forder
+ * (n).  Every local vertex must lie on one of the faces: UVA_EINVAL for ids out
+ * of range or an uncovered vertex. */
+int uva_chart_regroup(int64_t n, const int64_t *faces, const int64_t *CV, int64_t nv, const int64_t *chart,
+                      int64_t nchart, const int64_t *mesh_F, int64_t n_mesh_f, const double *mesh_V, int64_t n_mesh_v,
+                      int64_t *keep, int64_t *nk, int64_t *vert_off, int64_t *tri_off, double *P, int64_t *T,
+                      int64_t *vorder, int64_t *forder);
 
 #ifdef __cplusplus
 }

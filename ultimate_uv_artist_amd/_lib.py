@@ -116,6 +116,8 @@ SIGNATURES = {
     "uva_repair_jacobi": (C.c_int, [_vp, _vp, C.c_int32, _vp]),
     "uva_orient_positive": (C.c_int, [_vp, _vp, _vp]),
     "uva_repair_seq": (C.c_int, [_dp, C.c_int64, _i64p, C.c_int64, C.c_int32]),
+    "uva_chart_regroup": (C.c_int, [C.c_int64, _i64p, _i64p, C.c_int64, _i64p, C.c_int64, _i64p, C.c_int64, _dp,
+                                    C.c_int64, _i64p, _i64p, _i64p, _i64p, _dp, _i64p, _i64p, _i64p]),
 }
 
 _lock = threading.Lock()

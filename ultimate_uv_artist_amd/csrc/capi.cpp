@@ -1239,6 +1239,23 @@ int uva_orient_positive(uva_plan* plan, double* U, void* stream) {
     return check_fault(plan, s, from_plan(plan, 0, U, s, rc));
 }
 
+int uva_chart_regroup(int64_t n, const int64_t* faces, const int64_t* CV, int64_t nv, const int64_t* chart,
+                      int64_t nchart, const int64_t* mesh_F, int64_t n_mesh_f, const double* mesh_V, int64_t n_mesh_v,
+                      int64_t* keep, int64_t* nk, int64_t* vert_off, int64_t* tri_off, double* P, int64_t* T,
+                      int64_t* vorder, int64_t* forder) {
+    if (!nk || !keep || !vert_off || !tri_off || (n > 0 && (!faces || !CV || !chart || !mesh_F || !T || !forder)) ||
+        (nv > 0 && (!P || !vorder || !mesh_V)))
+        return fail(UVA_EINVAL, "null argument");
+    std::string err;
+    try {
+        err = uva::chart_regroup(n, faces, CV, nv, chart, nchart, mesh_F, n_mesh_f, mesh_V, n_mesh_v, keep, nk,
+                                 vert_off, tri_off, P, T, vorder, forder);
+    } catch (const std::bad_alloc&) {
+        return fail(UVA_ENOMEM, "host allocation failed");
+    }
+    return err.empty() ? UVA_OK : fail(UVA_EINVAL, err);
+}
+
 int uva_repair_seq(double* U, int64_t n, const int64_t* T, int64_t t, int32_t rounds) {
     if (!U || (!T && t > 0) || n < 0 || t < 0) return fail(UVA_EINVAL, "bad argument");
     for (int64_t i = 0; i < 3 * t; ++i)

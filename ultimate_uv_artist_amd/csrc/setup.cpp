@@ -850,6 +850,91 @@ void plan_teams(HostPlan& hp, const std::vector<int>& group_islands, int slots, 
     }
 }
 
+std::string chart_regroup(int64_t n, const int64_t* faces, const int64_t* CV, int64_t nv, const int64_t* chart,
+                          int64_t nchart, const int64_t* mesh_F, int64_t n_mesh_f, const double* mesh_V,
+                          int64_t n_mesh_v, int64_t* keep, int64_t* nk, int64_t* vert_off, int64_t* tri_off, double* P,
+                          int64_t* T, int64_t* vorder, int64_t* forder) {
+    if (n < 0 || nv < 0 || nchart < 0) return "negative size";
+    // range checks in parallel blocks of faces
+    constexpr int64_t kBlk = 1 << 16;
+    const int nb = (int)((n + kBlk - 1) / kBlk);
+    std::vector<char> bad(std::max(nb, 1), 0);
+    parallel_for(nb, [&](int b) {
+        const int64_t f0 = b * kBlk, f1 = std::min(n, f0 + kBlk);
+        char e = 0;
+        for (int64_t f = f0; f < f1; ++f) {
+            for (int k = 0; k < 3; ++k) e |= (CV[3 * f + k] < 0 || CV[3 * f + k] >= nv) ? 1 : 0;
+            e |= (chart[f] < 0 || chart[f] >= nchart) ? 2 : 0;
+            if (faces[f] < 0 || faces[f] >= n_mesh_f) e |= 4;
+            else
+                for (int k = 0; k < 3; ++k)
+                    e |= (mesh_F[3 * faces[f] + k] < 0 || mesh_F[3 * faces[f] + k] >= n_mesh_v) ? 8 : 0;
+        }
+        bad[b] = e;
+    });
+    char e = 0;
+    for (char x : bad) e |= x;
+    if (e & 1) return "CV holds vertex ids outside [0, nv)";
+    if (e & 2) return "chart ids outside [0, nchart)";
+    if (e & 4) return "face ids outside the mesh";
+    if (e & 8) return "a mesh face holds a vertex id outside the mesh";
+    // each local vertex's chart and mesh vertex, from its faces in order (a
+    // later face overrides an earlier one, as an in-order scatter does)
+    std::vector<int64_t> cv(nv, -1), corner(nv, 0);
+    for (int64_t f = 0; f < n; ++f)
+        for (int k = 0; k < 3; ++k) {
+            const int64_t v = CV[3 * f + k];
+            cv[v] = chart[f];
+            corner[v] = mesh_F[3 * faces[f] + k];
+        }
+    for (int64_t v = 0; v < nv; ++v)
+        if (cv[v] < 0) return "every vertex of the subset must belong to one of its faces";
+    // stable counting sorts by chart: vertices (ascending local id inside a
+    // chart) and faces (subset order inside a chart)
+    std::vector<int64_t> vpos(nchart + 1, 0), fpos(nchart + 1, 0);
+    for (int64_t v = 0; v < nv; ++v) vpos[cv[v] + 1]++;
+    for (int64_t f = 0; f < n; ++f) fpos[chart[f] + 1]++;
+    for (int64_t c = 0; c < nchart; ++c) {
+        vpos[c + 1] += vpos[c];
+        fpos[c + 1] += fpos[c];
+    }
+    int64_t k = 0;
+    vert_off[0] = 0;
+    tri_off[0] = 0;
+    for (int64_t c = 0; c < nchart; ++c)
+        if (fpos[c + 1] > fpos[c]) {      // a chart without faces has no vertices either
+            keep[k] = c;
+            vert_off[k + 1] = vpos[c + 1];
+            tri_off[k + 1] = fpos[c + 1];
+            ++k;
+        }
+    *nk = k;
+    std::vector<int64_t> local(nv);
+    {
+        std::vector<int64_t> at(vpos.begin(), vpos.end() - 1);
+        for (int64_t v = 0; v < nv; ++v) {
+            const int64_t q = at[cv[v]]++;
+            vorder[q] = v;
+            local[v] = q - vpos[cv[v]];
+        }
+        std::vector<int64_t> ft(fpos.begin(), fpos.end() - 1);
+        for (int64_t f = 0; f < n; ++f) forder[ft[chart[f]]++] = f;
+    }
+    // gathers: positions in island order, faces in island-local ids
+    const int nbv = (int)((nv + kBlk - 1) / kBlk);
+    parallel_for(nbv, [&](int b) {
+        const int64_t i1 = std::min(nv, (int64_t)(b + 1) * kBlk);
+        for (int64_t i = (int64_t)b * kBlk; i < i1; ++i)
+            for (int q = 0; q < 3; ++q) P[3 * i + q] = mesh_V[3 * corner[vorder[i]] + q];
+    });
+    parallel_for(nb, [&](int b) {
+        const int64_t i1 = std::min(n, (int64_t)(b + 1) * kBlk);
+        for (int64_t i = (int64_t)b * kBlk; i < i1; ++i)
+            for (int q = 0; q < 3; ++q) T[3 * i + q] = local[CV[3 * forder[i] + q]];
+    });
+    return "";
+}
+
 std::string build_matrix_plan(HostPlan& hp, int64_t n, const int64_t* indptr, const int64_t* indices,
                               const double* data) {
     if (n <= 0) return "matrix must have at least one row";

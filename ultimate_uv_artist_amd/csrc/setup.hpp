@@ -139,6 +139,14 @@ void plan_teams(HostPlan& hp, const std::vector<int>& group_islands, int slots, 
 std::string build_matrix_plan(HostPlan& hp, int64_t n, const int64_t* indptr, const int64_t* indices,
                               const double* data);
 
+// hier.Solver.relax's regrouping of a subset's charts into islands
+// (uva_chart_regroup).  Returns an empty string on success, else an error
+// message; *nk receives the number of charts with faces.
+std::string chart_regroup(int64_t n, const int64_t* faces, const int64_t* CV, int64_t nv, const int64_t* chart,
+                          int64_t nchart, const int64_t* mesh_F, int64_t n_mesh_f, const double* mesh_V,
+                          int64_t n_mesh_v, int64_t* keep, int64_t* nk, int64_t* vert_off, int64_t* tri_off, double* P,
+                          int64_t* T, int64_t* vorder, int64_t* forder);
+
 // param.repair_flips (sequential Gauss-Seidel order) on one chart.
 void repair_sequential(double* U, int64_t n, const int64_t* T, int64_t t, int rounds);
 

@@ -53,42 +53,36 @@ class Solver:
 
     def __init__(self, topo, dev=None):
         self.dev = require_gpu(dev)
-        self.V = np.asarray(topo.V, np.float64)
-        self.F = np.asarray(topo.F, np.int64)
+        self.V = np.ascontiguousarray(topo.V, np.float64)
+        self.F = np.ascontiguousarray(topo.F, np.int64)
         self._q = None
 
     def _charts(self, faces, CV, nv, chart, nchart):
         """The subset's islands in chart order, concatenated: per chart its
-        vertices in ascending local id and its faces in subset order.  Returns
-        the island offsets of vertices and faces, positions (n,3), faces in
-        island-local ids (t,3) (the form IslandPlan takes), the charts that
-        have faces, and the vertex and face orders."""
-        if CV.min(initial=0) < 0 or CV.max(initial=-1) >= nv:
-            raise ValueError("CV holds vertex ids outside [0, nv)")
-        if len(chart) and (chart.min() < 0 or chart.max() >= nchart):
-            raise ValueError("chart ids outside [0, nchart)")
-        cv = np.full(nv, -1, np.int64)
-        cv[CV.reshape(-1)] = np.repeat(chart, 3)
-        if (cv < 0).any():
-            raise ValueError("every vertex of the subset must belong to one of its faces")
-        corner = np.empty(nv, np.int64)                   # one mesh vertex per subset vertex
-        corner[CV.reshape(-1)] = self.F[faces].reshape(-1)
-        # stable sorts by chart id; with at most 65,535 charts the keys fit 16
-        # bits, for which numpy's stable sort is a radix sort (linear time)
-        small = nchart <= 65535
-        vorder = np.argsort(cv.astype(np.uint16) if small else cv, kind="stable")   # chart-major, ascending id inside a chart
-        vcount = np.bincount(cv, minlength=nchart)
-        voff = np.concatenate([[0], np.cumsum(vcount)])
-        local = np.empty(nv, np.int64)
-        local[vorder] = np.arange(nv) - np.repeat(voff[:-1], vcount)
-        forder = np.argsort(chart.astype(np.uint16) if small else chart, kind="stable")
-        fcount = np.bincount(chart, minlength=nchart)
-        keep = np.flatnonzero(fcount > 0)                 # a chart without faces has no vertices either
-        vo = np.concatenate([[0], np.cumsum(vcount[keep])]).astype(np.int64)
-        to = np.concatenate([[0], np.cumsum(fcount[keep])]).astype(np.int64)
-        P = np.ascontiguousarray(self.V[corner[vorder]])
-        T = np.ascontiguousarray(local[CV[forder]])
-        return (vo, to, P, T), keep, vorder, forder
+        vertices in ascending local id and its faces in subset order (threaded
+        host code, uva_chart_regroup).  Returns the island offsets of vertices
+        and faces, positions (n,3), faces in island-local ids (t,3) (the form
+        IslandPlan takes), the charts that have faces, and the vertex and face
+        orders."""
+        n = len(faces)
+        faces = np.ascontiguousarray(faces, np.int64)
+        CV = np.ascontiguousarray(CV, np.int64)
+        chart = np.ascontiguousarray(chart, np.int64)
+        keep = np.empty(max(nchart, 1), np.int64)
+        nk = np.zeros(1, np.int64)
+        vo = np.empty(max(nchart, 1) + 1, np.int64)
+        to = np.empty(max(nchart, 1) + 1, np.int64)
+        P = np.empty((nv, 3), np.float64)
+        T = np.empty((n, 3), np.int64)
+        vorder = np.empty(nv, np.int64)
+        forder = np.empty(n, np.int64)
+        i64, f64 = _lib.i64p, _lib.f64p
+        rc = _lib.lib().uva_chart_regroup(n, i64(faces), i64(CV), nv, i64(chart), nchart, i64(self.F), len(self.F),
+                                          f64(self.V), len(self.V), i64(keep), i64(nk), i64(vo), i64(to), f64(P),
+                                          i64(T), i64(vorder), i64(forder))
+        _lib.check(rc)
+        k = int(nk[0])
+        return (vo[:k + 1].copy(), to[:k + 1].copy(), P, T), keep[:k].copy(), vorder, forder
 
     def relax(self, faces, CV, nv, chart, nchart, U0, iters=6, cg_iters=20, repair_rounds=24):
         """faces (n,) global face ids, CV (n,3) local vertex ids, chart (n,)
@@ -113,8 +107,10 @@ class Solver:
         plan = IslandPlan(None, batch=True, dev=self.dev, no_coarse=True, concat=cat)   # its own CG is not used
         voff = plan.vert_off
         plan.set_pins(voff[:-1], 0.0)
-        # the reference holds the layout in float32 (hier.py:400)
-        U = plan.device_uv(U0[vorder].astype(np.float32).astype(np.float64))
+        # the reference holds the layout in float32 (hier.py:400); the island
+        # order is gathered on the device
+        vord = torch.as_tensor(vorder, device=self.dev)
+        U = torch.as_tensor(U0, device=self.dev).index_select(0, vord).float().double()
         plan.orient_positive(U)
         plan.repair_jacobi(U, repair_rounds)
         Un = torch.empty_like(U)
@@ -132,8 +128,7 @@ class Solver:
         plan.repair_jacobi(U, repair_rounds)
         U.copy_(U.float().double())                          # the reference returns float32 values
         q = self._quality(plan, cat[2], U, len(keep))
-        out = np.empty((nv, 2))
-        out[vorder] = U.cpu().numpy()
+        out = torch.empty_like(U).index_copy_(0, vord, U).cpu().numpy()
         flips[keep] = q["flips"]
         scale[keep] = q["scale"]
         self._q = (q, keep, nchart)
