@@ -853,22 +853,28 @@ int uva_plan_set_pins(uva_plan* plan, const int64_t* pins, int64_t n_pins, doubl
         HIP_TRY(hipMemcpyAsync(wisl.data(), d.isl_wpin, d.n_isl * sizeof(double), hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
     }
-    std::vector<double> rp((size_t)std::max(n, 1), 0.0);
+    // the pinned rows (weights of a row pinned more than once add up, in
+    // the pins' order), then every row's weight, pinned diagonal and its
+    // FP32 inverse on the device
+    std::vector<std::pair<int, double>> pr;
+    pr.reserve((size_t)n_pins);
     for (int64_t i = 0; i < n_pins; ++i) {
         if (pins[i] < 0 || pins[i] >= n) return fail(UVA_EINVAL, "pin outside the plan's vertices");
         const int r = plan->vinv[pins[i]];
         const int isl = (int)(std::upper_bound(plan->v0.begin(), plan->v0.end(), r) - plan->v0.begin()) - 1;
-        rp[r] += wisl[std::max(isl, 0)];
+        pr.push_back({r, wisl[std::max(isl, 0)]});
     }
-    std::vector<double> diag((size_t)std::max(n, 1), 0.0);
-    std::vector<float> dinv((size_t)std::max(n, 1), 1.0f);
-    if (n) HIP_TRY(hipMemcpyAsync(diag.data(), d.diag, n * sizeof(double), hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
-    std::vector<double> pd((size_t)std::max(n, 1), 0.0);
-    for (int r = 0; r < n; ++r) {
-        const double g = diag[r] + rp[r];
-        pd[r] = g;
-        dinv[r] = (float)(g > 1e-30 ? 1.0 / g : 1.0);
+    std::stable_sort(pr.begin(), pr.end(),
+                     [](const std::pair<int, double>& a, const std::pair<int, double>& b) { return a.first < b.first; });
+    std::vector<int> rows;
+    std::vector<double> rw;
+    for (const auto& e : pr) {
+        if (!rows.empty() && rows.back() == e.first) {
+            rw.back() += e.second;
+        } else {
+            rows.push_back(e.first);
+            rw.push_back(0.0 + e.second);
+        }
     }
     if (!d.rpin) {
         double *p = nullptr, *q = nullptr;
@@ -877,11 +883,16 @@ int uva_plan_set_pins(uva_plan* plan, const int64_t* pins, int64_t n_pins, doubl
         d.rpin = p;
         d.pdiag = q;
     }
-    if (n) {
-        HIP_TRY(hipMemcpyAsync((void*)d.rpin, rp.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
-        HIP_TRY(hipMemcpyAsync((void*)d.pdiag, pd.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
-        HIP_TRY(hipMemcpyAsync((void*)d.dinv32, dinv.data(), n * sizeof(float), hipMemcpyHostToDevice, s));
+    int* drows = nullptr;
+    double* dw = nullptr;
+    const int m = (int)rows.size();
+    if (plan->alloc(&drows, (size_t)std::max(m, 1)) != hipSuccess || plan->alloc(&dw, (size_t)std::max(m, 1)) != hipSuccess)
+        return fail(UVA_ENOMEM, "pin list");
+    if (m) {
+        HIP_TRY(hipMemcpyAsync(drows, rows.data(), m * sizeof(int), hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(dw, rw.data(), m * sizeof(double), hipMemcpyHostToDevice, s));
     }
+    HIP_TRY(uva::launch_set_pins(d.diag, drows, dw, m, (double*)d.rpin, (double*)d.pdiag, (float*)d.dinv32, n, s));
     // the coarse inverses hold the islands' own pins: Jacobi only from here on
     std::vector<int64_t> none((size_t)std::max(d.n_isl, 1), -1);
     if (d.n_isl)

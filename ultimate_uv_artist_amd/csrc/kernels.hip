@@ -1206,6 +1206,35 @@ __global__ void signed_areas_kernel(DevPlan P, const double2* __restrict__ U, do
     if (f < P.n_tri) sa[f] = signed_area(P, f, U);
 }
 
+// Per-row soft pins: rpin[r] = the weight of row r in the m pinned rows
+// (ascending, distinct; 0 elsewhere), pdiag = diag + rpin and its FP32
+// inverse (1 where the pinned diagonal is not above 1e-30).
+__global__ void set_pins_kernel(const double* __restrict__ diag, const int* __restrict__ rows,
+                                const double* __restrict__ w, int m, double* __restrict__ rpin,
+                                double* __restrict__ pdiag, float* __restrict__ dinv32, int n) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    int lo = 0, hi = m;               // first pinned row >= r
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (rows[mid] < r) lo = mid + 1;
+        else hi = mid;
+    }
+    const double rp = (lo < m && rows[lo] == r) ? w[lo] : 0.0;
+    const double g = diag[r] + rp;
+    rpin[r] = rp;
+    pdiag[r] = g;
+    dinv32[r] = (float)(g > 1e-30 ? 1.0 / g : 1.0);
+}
+
+hipError_t launch_set_pins(const double* diag, const int* rows, const double* w, int m, double* rpin, double* pdiag,
+                           float* dinv32, int n, hipStream_t s) {
+    if (n > 0)
+        hipLaunchKernelGGL(set_pins_kernel, dim3((n + 255) / 256), dim3(256), 0, s, diag, rows, w, m, rpin, pdiag,
+                           dinv32, n);
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------------- launchers
 template <int KIND, int PREC>
 static hipError_t launch_kind(const DevPlan& P, const TaskArgs& a, int slot, int cus, hipStream_t s) {

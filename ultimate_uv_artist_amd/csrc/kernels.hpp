@@ -64,6 +64,10 @@ hipError_t launch_fill_int(int* p, int n, int v, hipStream_t s);
 // out[b] = hardware XCD id of block b (blocks of 64 threads)
 hipError_t launch_xcc_probe(int* out, int blocks, hipStream_t s);
 hipError_t launch_signed_areas(const DevPlan& P, const double2* U, double* sa, hipStream_t s);
+// per-row pin weights of m pinned rows (ascending, distinct) -> rpin, pdiag,
+// dinv32 over n rows
+hipError_t launch_set_pins(const double* diag, const int* rows, const double* w, int m, double* rpin, double* pdiag,
+                           float* dinv32, int n, hipStream_t s);
 
 // Jacobi-PCG over every row of a plan (pcg.hip): workgroups for a device of
 // `cus` CUs, scratch doubles they need, the launches of one solve (vectors in

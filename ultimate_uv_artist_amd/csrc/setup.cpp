@@ -607,7 +607,9 @@ std::string build_host_plan(HostPlan& hp, int64_t n_islands, const int64_t* vert
     hp.inc_ptr[nv_all] = 3 * (int)nt_all;
     hp.inc_tri.resize(3 * nt_all);
     hp.inc_g.resize(6 * nt_all);
-    hp.tri_g.resize(6 * nt_all);
+    // the triangle-major copy serves the step backward, which Batch-convention
+    // plans do not run
+    if (!batch) hp.tri_g.resize(6 * nt_all);
     hp.adj_ptr.resize(nv_all + 1);
     int64_t adj_total = 0;
     for (auto& I : isl) adj_total += I.nnz;
@@ -653,7 +655,8 @@ std::string build_host_plan(HostPlan& hp, int64_t n_islands, const int64_t* vert
             hp.tv2[t0 + f] = v0 + inv[i][T[3 * ((size_t)t0 + f) + 2]];
             hp.fx1[t0 + f] = I.x1[f]; hp.fx2[t0 + f] = I.x2[f]; hp.fy2[t0 + f] = I.y2[f];
             hp.farea[t0 + f] = I.area[f]; hp.fok[t0 + f] = I.ok[f];
-            for (int j = 0; j < 6; ++j) hp.tri_g[6 * ((size_t)t0 + f) + j] = I.g[6 * (size_t)f + j];
+            if (!batch)
+                for (int j = 0; j < 6; ++j) hp.tri_g[6 * ((size_t)t0 + f) + j] = I.g[6 * (size_t)f + j];
         }
         // incidences: global offsets are 3 * t0 + local offsets
         for (int v = 0; v <= I.nv; ++v) hp.inc_ptr[v0 + v] = 3 * t0 + I.inc_ptr[v];

@@ -63,7 +63,7 @@ struct HostPlan {
     hvec<unsigned char> fok;
     hvec<int> inc_ptr, inc_tri;
     hvec<double> inc_g;          // 2 per incidence
-    hvec<double> tri_g;          // the same corner vectors triangle-major: 6 per triangle
+    hvec<double> tri_g;          // the same corner vectors triangle-major: 6 per triangle (not in Batch plans)
     std::vector<int> sl_ptr;
     hvec<int> m_col;
     std::vector<int> sl_row;            // first row of each slice (n_slices + 1, the last = n_vert)

@@ -138,7 +138,9 @@ class Solver:
         """hier.py:511-530 per face and per chart, float64 on the plan's GPU."""
         d = self.dev
         P = torch.as_tensor(Pall, device=d)
-        T = torch.as_tensor(plan.T_local + np.repeat(plan.vert_off[:-1], np.diff(plan.tri_off))[:, None], device=d)
+        nf = torch.as_tensor(np.diff(plan.tri_off), device=d)
+        T = torch.as_tensor(plan.T_local, device=d) + torch.repeat_interleave(
+            torch.as_tensor(plan.vert_off[:-1], device=d), nf)[:, None]
         p0, p1, p2 = P[T[:, 0]], P[T[:, 1]], P[T[:, 2]]
         e1, e2 = p1 - p0, p2 - p0
         l1 = e1.norm(dim=1)
@@ -161,7 +163,7 @@ class Solver:
         s1, s2 = Q + R, (Q - R).abs()
         ar3 = 0.5 * dbl * ok
         sgn = 0.5 * ((a1[:, 0] - a0[:, 0]) * (a2[:, 1] - a0[:, 1]) - (a2[:, 0] - a0[:, 0]) * (a1[:, 1] - a0[:, 1]))
-        ct = torch.as_tensor(np.repeat(np.arange(nisl), np.diff(plan.tri_off)), device=d)
+        ct = torch.repeat_interleave(torch.arange(nisl, device=d), nf)
         uva = torch.zeros(nisl, dtype=torch.float64, device=d).index_add_(0, ct, sgn.abs() * ok)
         a3 = torch.zeros(nisl, dtype=torch.float64, device=d).index_add_(0, ct, ar3)
         scale = uva / a3.clamp_min(1e-30)
