@@ -211,7 +211,8 @@ int uva_plan_set_scale(uva_plan *plan, const double *scale_dev, void *stream);
 
 /* LSCM of every island into U (device, n x 2 float64).  pins: 2 local ids per
  * island (host) or NULL for the reference's farthest boundary pair.  FP64 CG
- * to relative residual tol. */
+ * to relative residual tol.  Plans built with refine conventions only
+ * (UVA_PLAN_BATCH plans hold no LSCM boundary coupling). */
 int uva_lscm(uva_plan *plan, double *U, const int64_t *pins, double tol, int32_t max_iters, void *stream);
 
 /* ARAP iterations on U in place (device, n x 2 float64). */

@@ -908,6 +908,7 @@ int uva_lscm(uva_plan* plan, double* U, const int64_t* pins, double tol, int32_t
     if (!valid_plan(plan) || !U) return fail(UVA_EINVAL, "null argument");
     plan->mem = (hipStream_t)stream;   // allocations and the final release follow this stream
     if (!(tol > 0.0)) return fail(UVA_EINVAL, "tol must be positive");
+    if (plan->dp.mode_batch) return fail(UVA_EINVAL, "LSCM needs a plan built with refine conventions");
     hipStream_t s = (hipStream_t)stream;
     uva::DevPlan& d = plan->dp;
     // The coarse level of an island's LSCM operator leaves out the plan's own

@@ -574,9 +574,12 @@ std::string build_host_plan(HostPlan& hp, int64_t n_islands, const int64_t* vert
     // for the steps past a row's width (weight 0, no select after the load);
     // every other entry is written by the threaded fill below
     hp.m_col.resize(nnz_pad + kSlice); hp.m_hi.resize(nnz_pad + kSlice);
-    hp.m_lo.resize(nnz_pad + kSlice); hp.m_b.resize(nnz_pad + kSlice);
+    hp.m_lo.resize(nnz_pad + kSlice);
+    // the LSCM boundary coupling (Batch-convention plans run no LSCM)
+    if (!batch) hp.m_b.resize(nnz_pad + kSlice);
     for (int64_t o = nnz_pad; o < nnz_pad + kSlice; ++o) {
-        hp.m_col[o] = 0; hp.m_hi[o] = 0.0f; hp.m_lo[o] = 0.0f; hp.m_b[o] = 0.0f;
+        hp.m_col[o] = 0; hp.m_hi[o] = 0.0f; hp.m_lo[o] = 0.0f;
+        if (!batch) hp.m_b[o] = 0.0f;
     }
     hp.diag.resize(nv_all); hp.dinv32.resize(nv_all); hp.vbnd.resize(nv_all); hp.vperm.resize(nv_all);
     // coarse level: aggregates of g consecutive slices, as small as the
@@ -711,10 +714,11 @@ std::string build_host_plan(HostPlan& hp, int64_t n_islands, const int64_t* vert
                         hp.m_col[o] = v0 + I.row_col[q];
                         hp.m_hi[o] = hi;
                         hp.m_lo[o] = (float)(v - (double)hi);
-                        hp.m_b[o] = (float)I.row_b[q];
+                        if (!batch) hp.m_b[o] = (float)I.row_b[q];
                     } else {
                         hp.m_col[o] = v0 + std::min(r, I.nv - 1);
-                        hp.m_hi[o] = 0.0f; hp.m_lo[o] = 0.0f; hp.m_b[o] = 0.0f;
+                        hp.m_hi[o] = 0.0f; hp.m_lo[o] = 0.0f;
+                        if (!batch) hp.m_b[o] = 0.0f;
                     }
                 }
             }

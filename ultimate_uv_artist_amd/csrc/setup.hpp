@@ -67,7 +67,7 @@ struct HostPlan {
     std::vector<int> sl_ptr;
     hvec<int> m_col;
     std::vector<int> sl_row;            // first row of each slice (n_slices + 1, the last = n_vert)
-    hvec<float> m_hi, m_lo, m_b;
+    hvec<float> m_hi, m_lo, m_b;        // m_b: LSCM coupling (empty in Batch plans)
     hvec<double> diag;
     hvec<float> dinv32;
     hvec<int> adj_ptr, adj_idx;
