@@ -885,15 +885,37 @@ std::string chart_regroup(int64_t n, const int64_t* faces, const int64_t* CV, in
     if (e & 2) return "chart ids outside [0, nchart)";
     if (e & 4) return "face ids outside the mesh";
     if (e & 8) return "a mesh face holds a vertex id outside the mesh";
-    // each local vertex's chart and mesh vertex, from its faces in order (a
-    // later face overrides an earlier one, as an in-order scatter does)
+    // each local vertex's chart and mesh vertex from the last of its faces
+    // (as an in-order scatter leaves them): the last face of every vertex by
+    // an atomic maximum, then that face writes (one writer per vertex)
+    std::unique_ptr<std::atomic<int64_t>[]> last(new std::atomic<int64_t>[std::max<int64_t>(nv, 1)]);
+    const int nbv0 = (int)((nv + kBlk - 1) / kBlk);
+    parallel_for(nbv0, [&](int b) {
+        const int64_t v1 = std::min(nv, (int64_t)(b + 1) * kBlk);
+        for (int64_t v = (int64_t)b * kBlk; v < v1; ++v) last[v].store(-1, std::memory_order_relaxed);
+    });
+    parallel_for(nb, [&](int b) {
+        const int64_t f1 = std::min(n, (int64_t)(b + 1) * kBlk);
+        for (int64_t f = (int64_t)b * kBlk; f < f1; ++f)
+            for (int k = 0; k < 3; ++k) {
+                std::atomic<int64_t>& l = last[CV[3 * f + k]];
+                int64_t cur = l.load(std::memory_order_relaxed);
+                while (cur < f && !l.compare_exchange_weak(cur, f, std::memory_order_relaxed)) {
+                }
+            }
+    });
     std::vector<int64_t> cv(nv, -1), corner(nv, 0);
-    for (int64_t f = 0; f < n; ++f)
-        for (int k = 0; k < 3; ++k) {
-            const int64_t v = CV[3 * f + k];
-            cv[v] = chart[f];
-            corner[v] = mesh_F[3 * faces[f] + k];
-        }
+    parallel_for(nb, [&](int b) {
+        const int64_t f1 = std::min(n, (int64_t)(b + 1) * kBlk);
+        for (int64_t f = (int64_t)b * kBlk; f < f1; ++f)
+            for (int k = 0; k < 3; ++k) {
+                const int64_t v = CV[3 * f + k];
+                if (last[v].load(std::memory_order_relaxed) == f) {
+                    cv[v] = chart[f];
+                    corner[v] = mesh_F[3 * faces[f] + k];
+                }
+            }
+    });
     for (int64_t v = 0; v < nv; ++v)
         if (cv[v] < 0) return "every vertex of the subset must belong to one of its faces";
     // stable counting sorts by chart: vertices (ascending local id inside a
