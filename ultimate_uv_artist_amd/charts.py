@@ -106,6 +106,148 @@ def boundary_intersections(U, T, max_report=1000):
     return int(len(_crossing_pairs(*_grid(U, bi, bj), max_report)))
 
 
+def _concat(Us, Ts):
+    """Charts' layouts and faces concatenated (faces in global ids), with the
+    vertex and face offsets."""
+    vo = np.concatenate([[0], np.cumsum([len(U) for U in Us])]).astype(np.int64)
+    to = np.concatenate([[0], np.cumsum([len(T) for T in Ts])]).astype(np.int64)
+    U = np.concatenate([np.asarray(u, np.float64).reshape(-1, 2) for u in Us]) if Us else np.zeros((0, 2))
+    T = (np.concatenate([np.asarray(t, np.int64).reshape(-1, 3) + vo[i] for i, t in enumerate(Ts)])
+         if Ts else np.zeros((0, 3), np.int64))
+    return vo, to, U, T
+
+
+def boundary_intersections_many(Us, Ts, max_report=1000):
+    """boundary_intersections of every chart (Us[i], Ts[i]), equal to the
+    per-chart calls: the boundary half-edges, cell grids and crossing tests
+    of all charts at once, each chart's cells apart from the others'."""
+    k = len(Ts)
+    out = np.zeros(k, np.int64)
+    if k == 0:
+        return out.tolist()
+    vo, to, U, T = _concat(Us, Ts)
+    n = len(U)
+    src = np.concatenate([T[:, 0], T[:, 1], T[:, 2]])
+    dst = np.concatenate([T[:, 1], T[:, 2], T[:, 0]])
+    has_twin = np.isin(dst * n + src, src * n + dst)
+    bi, bj = src[~has_twin], dst[~has_twin]
+    vch = np.repeat(np.arange(k), np.diff(vo))
+    ech = vch[bi]
+    nb = np.bincount(ech, minlength=k)
+    live = nb >= 3                                    # fewer than 3 boundary edges: no crossing
+    # a layout with a non-finite coordinate takes the per-chart path
+    finite = np.minimum.reduceat(np.isfinite(U).all(1), vo[:-1]) if n else np.ones(k, bool)
+    for c in np.flatnonzero(live & ~finite):
+        out[c] = boundary_intersections(Us[c], Ts[c], max_report)
+    live &= finite
+    keep = live[ech]
+    bi, bj, ech = bi[keep], bj[keep], ech[keep]
+    if len(bi) == 0:
+        return out.tolist()
+    A, B = U[bi], U[bj]
+    # per chart: the layout's box over all its vertices, the median boundary edge
+    lo = np.minimum.reduceat(U, vo[:-1], axis=0)
+    ext = np.maximum.reduceat(U, vo[:-1], axis=0) - lo
+    L = np.linalg.norm(B - A, axis=1)
+    o = np.lexsort((L, ech))
+    Ls, cs = L[o], ech[o]
+    cnt = np.bincount(cs, minlength=k)
+    st = np.concatenate([[0], np.cumsum(cnt)])[:-1]
+    med = np.zeros(k)
+    lv = np.flatnonzero(cnt > 0)
+    h = cnt[lv] // 2
+    odd = cnt[lv] % 2 == 1
+    a_ = Ls[st[lv] + np.where(odd, h, h - 1)]
+    b_ = Ls[st[lv] + h]
+    med[lv] = np.where(odd, b_, (a_ + b_) / 2)
+    cell = np.maximum(np.maximum(med * 4, ext.max(1) / 512), 1e-30)
+    nx = (ext[:, 0] / cell).astype(np.int64) + 1
+    ny = (ext[:, 1] / cell).astype(np.int64) + 1
+    # cells each segment's box covers (as _cells, with its chart's grid)
+    c_e, lo_e, nx_e, ny_e = cell[ech], lo[ech], nx[ech], ny[ech]
+    mn, mx = np.minimum(A, B), np.maximum(A, B)
+    x0 = np.maximum(((mn[:, 0] - lo_e[:, 0]) / c_e).astype(np.int64), 0)
+    x1 = np.minimum(((mx[:, 0] - lo_e[:, 0]) / c_e).astype(np.int64), nx_e - 1)
+    y0 = np.maximum(((mn[:, 1] - lo_e[:, 1]) / c_e).astype(np.int64), 0)
+    y1 = np.minimum(((mx[:, 1] - lo_e[:, 1]) / c_e).astype(np.int64), ny_e - 1)
+    wx, wy = np.maximum(x1 - x0 + 1, 0), np.maximum(y1 - y0 + 1, 0)
+    ncell = wx * wy
+    seg = np.repeat(np.arange(len(A)), ncell)
+    q = np.arange(len(seg)) - np.repeat(np.cumsum(ncell) - ncell, ncell)
+    gx = x0[seg] + q // np.maximum(wy[seg], 1)
+    gy = y0[seg] + q % np.maximum(wy[seg], 1)
+    cbase = np.concatenate([[0], np.cumsum(nx * ny)])[:-1]
+    c = cbase[ech[seg]] + gy * nx_e[seg] + gx
+    # pairs of segments in one cell, tested as _crossing_pairs does
+    o = np.lexsort((seg, c))
+    seg, c = seg[o], c[o]
+    _, start, cc = np.unique(c, return_index=True, return_counts=True)
+    r = np.arange(len(seg)) - np.repeat(start, cc)
+    npair = np.repeat(cc, cc) - 1 - r
+    first = np.repeat(np.arange(len(seg)), npair)
+    second = first + 1 + np.arange(len(first)) - np.repeat(np.cumsum(npair) - npair, npair)
+    i, j = seg[first], seg[second]
+    Ai, Bi, Aj, Bj = A[i], B[i], A[j], B[j]
+    touch = ((Ai == Aj).all(1) | (Ai == Bj).all(1) | (Bi == Aj).all(1) | (Bi == Bj).all(1))
+    d1, d2 = Bi - Ai, Bj - Aj
+    o1 = d1[:, 0] * (Aj[:, 1] - Ai[:, 1]) - d1[:, 1] * (Aj[:, 0] - Ai[:, 0])
+    o2 = d1[:, 0] * (Bj[:, 1] - Ai[:, 1]) - d1[:, 1] * (Bj[:, 0] - Ai[:, 0])
+    o3 = d2[:, 0] * (Ai[:, 1] - Aj[:, 1]) - d2[:, 1] * (Ai[:, 0] - Aj[:, 0])
+    o4 = d2[:, 0] * (Bi[:, 1] - Aj[:, 1]) - d2[:, 1] * (Bi[:, 0] - Aj[:, 0])
+    t1, t2 = 1e-9 * (d1 * d1).sum(1), 1e-9 * (d2 * d2).sum(1)
+    hit = ~touch & (((o1 > t1) & (o2 < -t1)) | ((o1 < -t1) & (o2 > t1))) & \
+        (((o3 > t2) & (o4 < -t2)) | ((o3 < -t2) & (o4 > t2)))
+    out[live] = np.minimum(np.bincount(ech[i[hit]], minlength=k), max_report)[live]
+    return out.tolist()
+
+
+def chart_quality_many(charts, Us):
+    """param.chart_quality of every chart (charts[i] = (P, T), layout Us[i]),
+    equal to the per-chart calls: the per-triangle terms of all charts at
+    once, the per-chart sums and weighted quantiles on each chart's slice."""
+    k = len(charts)
+    if k == 0:
+        return []
+    vo, to, U, T = _concat(Us, [t for _, t in charts])
+    P = np.concatenate([np.asarray(p, np.float64).reshape(-1, 3) for p, _ in charts])
+    X, area = param.local_frames(P, T)
+    sa = param.signed_areas(U, T)
+    x1, x2, y2 = X[:, 1, 0], X[:, 2, 0], X[:, 2, 1]
+    good = np.abs(x1 * y2) > 1e-300
+    x1 = np.where(good, x1, 1)
+    y2 = np.where(good, y2, 1)
+    d1 = U[T[:, 1]] - U[T[:, 0]]
+    d2 = U[T[:, 2]] - U[T[:, 0]]
+    j00, j10 = d1[:, 0] / x1, d1[:, 1] / x1
+    j01, j11 = (d2[:, 0] - j00 * x2) / y2, (d2[:, 1] - j10 * x2) / y2
+    conf = np.sqrt(((j00 + j11) / 2) ** 2 + ((j10 - j01) / 2) ** 2)
+    anti = np.sqrt(((j00 - j11) / 2) ** 2 + ((j10 + j01) / 2) ** 2)
+    s1, s2 = conf + anti, np.abs(conf - anti)
+    res = []
+    for i in range(k):
+        sl = slice(to[i], to[i + 1])
+        ar_i, sa_i, s1_i, s2_i = area[sl], sa[sl], s1[sl], s2[sl]
+        tot = ar_i.sum()
+        flips = int((sa_i <= 0).sum()) if (sa_i > 0).sum() >= (sa_i < 0).sum() else int((sa_i >= 0).sum())
+        sc = np.abs(sa_i).sum() / max(tot, 1e-300)
+        w = ar_i / max(tot, 1e-300)
+        an = s1_i / np.maximum(s2_i, 1e-12)
+        ar = (s1_i * s2_i) / max(sc, 1e-300)
+        lar = np.abs(np.log2(np.maximum(ar, 1e-12)))
+
+        def wq(x, q):
+            o = np.argsort(x)
+            cw = np.cumsum(w[o])
+            return float(x[o][min(np.searchsorted(cw, q * cw[-1]), len(o) - 1)])
+
+        st = np.sqrt((s1_i ** 2 + s2_i ** 2) / 2 / max(sc, 1e-300))
+        res.append({"flips": flips, "aniso_p95": wq(an, 0.95), "aniso_mean": float((an * w).sum()),
+                    "area_p95": float(2 ** wq(lar, 0.95)), "area_logdev": float((lar * w).sum()),
+                    "stretch": float(np.sqrt((st ** 2 * w).sum())), "scale": float(np.sqrt(sc)),
+                    "per_tri_aniso": an, "per_tri_area": ar})
+    return res
+
+
 def untangle_boundary(U, T, rounds=40):
     """Remove boundary self-crossings by moving the vertices around each
     crossing toward their 1-ring centroid, never folding a triangle and never
@@ -222,18 +364,19 @@ def flatten_many(charts, iters=8, dev=None, slim_iters=0):
                 Us = U3[sub.vert_off[k]:sub.vert_off[k + 1]]
                 if np.isfinite(Us).all():
                     out[i] = Us.copy()
-    res = []
-    for (P, T), Ui in zip(charts, out):
-        # boundary crossings and their local untangle (charts.py:428-437), on the host
-        hits = boundary_intersections(Ui, T)
-        if hits and not (param.signed_areas(Ui, T) <= 0).any():
-            U2, ok = untangle_boundary(Ui, T)
+    # boundary crossings and their local untangle (charts.py:428-437), then
+    # the quality, on the host for all charts at once
+    hits = boundary_intersections_many(out, [T for _, T in charts])
+    for i, (P, T) in enumerate(charts):
+        if hits[i] and not (param.signed_areas(out[i], T) <= 0).any():
+            U2, ok = untangle_boundary(out[i], T)
             if ok:
-                Ui, hits = U2, 0
-        q = param.chart_quality(P, T, Ui)
-        q["boundary_hits"] = hits
-        q["overlap"] = 1.0 if hits else 0.0
-        res.append((Ui, q))
+                out[i], hits[i] = U2, 0
+    res = []
+    for i, q in enumerate(chart_quality_many(charts, out)):
+        q["boundary_hits"] = hits[i]
+        q["overlap"] = 1.0 if hits[i] else 0.0
+        res.append((out[i], q))
     return res
 
 
