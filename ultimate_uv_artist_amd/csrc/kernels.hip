@@ -426,7 +426,11 @@ struct Isl {
 };
 
 UVA_HD __forceinline__ long long uniform_ll(long long v) {
-    return ((long long)uniform_i((int)(v >> 32)) << 32) | (long long)(unsigned)uniform_i((int)(v & 0xffffffffLL));
+    // the halves are joined as unsigned bits (a negative v's high half shifted
+    // as a signed value would be undefined)
+    const unsigned long long hi = (unsigned)uniform_i((int)(v >> 32));
+    const unsigned long long lo = (unsigned)uniform_i((int)(v & 0xffffffffLL));
+    return (long long)(hi << 32 | lo);
 }
 
 UVA_HD __forceinline__ Isl load_isl(const DevPlan& P, int i) {
