@@ -33,11 +33,11 @@
 // pass); the FP64 true residual of the next outer pass decides convergence.
 //
 // Members of a multi-workgroup team keep u for a window of rows that extends
-// H = min(bandwidth, kHalo) rows beyond each end of their own range.  They
+// H = min(bandwidth, halo_rows) rows beyond each end of their own range.  They
 // publish the rows other members' windows cover to global memory, and after
 // each team barrier copy their window's foreign rows from there into LDS in
 // one coalesced pass, so every gather of the SpMV reads LDS.  (An island whose
-// bandwidth exceeds kHalo publishes all rows and gathers the columns outside
+// bandwidth exceeds the halo publishes all rows and gathers the columns outside
 // the window from global memory.)
 #pragma once
 
@@ -58,6 +58,8 @@ struct ResCG {
     // matrix entries per slot whose loads are issued together (as many as the
     // registers hold beside the slots' vectors)
     static constexpr int kG = RPT * (int)sizeof(V) * (kCoupled ? 2 : 1) <= 16 ? 2 : 1;
+    // halo rows of a team member's u window
+    static constexpr int kH = halo_rows((int)sizeof(V2));
 
     Team& tm;
     const DevPlan& P;
@@ -95,8 +97,8 @@ struct ResCG {
         rlo = d.v0 + slo * kSlice;
         rhi = min(d.v0 + shi * kSlice, d.v0 + d.nv);
         multi = tm.multi();
-        H = multi ? min(d.bw, kHalo) : 0;
-        gfull = multi && d.bw > kHalo;
+        H = multi ? min(d.bw, kH) : 0;
+        gfull = multi && d.bw > kH;
         wlo = max(rlo - H, d.v0);
         whi = min(rhi + H, d.v0 + d.nv);
         coarse = use_coarse;
@@ -109,10 +111,10 @@ struct ResCG {
         const int rows = (shi - slo) * kSlice;
         char UVA_LDS* base = (char UVA_LDS*)tm.lvec();
         lu = (V2 UVA_LDS*)base;
-        le = lu + rows + 2 * (multi ? kHalo : 0);
+        le = lu + rows + 2 * (multi ? kH : 0);
         ldv = (V UVA_LDS*)(le + rows);
         ldg = ldv + rows;
-        lci = (float UVA_LDS*)(base + res_vec_bytes<V2, V>(rows, multi ? kHalo : 0));
+        lci = (float UVA_LDS*)(base + res_vec_bytes<V2, V>(rows, multi ? kH : 0));
         lrc = (double2 UVA_LDS*)((char UVA_LDS*)lci + align16(na * crow * 4));
         ly = lrc + nc;
         lpt = ly + na;
@@ -163,9 +165,21 @@ struct ResCG {
     UVA_HD __forceinline__ void fill_halo() {
         if (!multi) return;
         const int nlo = rlo - wlo, n = nlo + (whi - rhi);
-        for (int i = tm.tid(); i < n; i += tm.tstride()) {
-            const int row = i < nlo ? wlo + i : rhi + (i - nlo);
-            lu[row - wlo] = gu[row];
+        // kFill loads of each thread in flight together
+        constexpr int kFill = 4;
+        const int st = tm.tstride();
+        for (int i0 = tm.tid(); i0 < n; i0 += kFill * st) {
+            V2 t[kFill];
+#pragma unroll
+            for (int q = 0; q < kFill; ++q) {
+                const int i = i0 + q * st;
+                if (i < n) t[q] = gu[i < nlo ? wlo + i : rhi + (i - nlo)];
+            }
+#pragma unroll
+            for (int q = 0; q < kFill; ++q) {
+                const int i = i0 + q * st;
+                if (i < n) lu[(i < nlo ? wlo + i : rhi + (i - nlo)) - wlo] = t[q];
+            }
         }
         tm.bsync();
     }
@@ -550,7 +564,7 @@ UVA_HD __forceinline__ CGOut cg_solve(Team& tm, const DevPlan& P, const Isl& d, 
     }
     const int W = tm.nwaves();
     const int rpt = (nsl + W - 1) / W;
-    const int vec = res_vec_bytes<V2, V>(nsl * kSlice, tm.multi() ? kHalo : 0);
+    const int vec = res_vec_bytes<V2, V>(nsl * kSlice, tm.multi() ? halo_rows((int)sizeof(V2)) : 0);
     if (tm.lvec() != nullptr && vec <= tm.lvec_bytes() && rpt <= kMaxRpt) {
         constexpr bool cp = OP == OP_LSCM;
         const int nc = g > 0 ? (d.ns + g - 1) / g : 0;

@@ -34,7 +34,11 @@ constexpr int kQueueStride = kMaxXcd + 1;
 // operator for the member's aggregates, the restricted residual of every
 // aggregate, the member's correction, mat-vec shares and slice sums)
 constexpr int kLdsVec = 151552;
-constexpr int kHalo = 512;          // rows of u a team member keeps beyond each end of its range
+// rows of u a team member keeps beyond each end of its range, for FP32 and
+// for FP64 u (the FP64 rows of the LSCM solve leave less LDS for a halo)
+constexpr int kHalo = 512;
+constexpr int kHaloF64 = 512;
+__host__ __device__ constexpr int halo_rows(int v2) { return v2 <= 8 ? kHalo : kHaloF64; }
 constexpr int kPart = 8;            // threads sharing one row of the coarse mat-vec
 
 __host__ __device__ constexpr int align16(int b) { return (b + 15) & ~15; }

@@ -354,7 +354,7 @@ int aggregate_slices(int ns, int res_slices, int v2, int v, bool coupled) {
                 nsl = std::max(nsl, hi - lo);
                 if (hi > lo) na = std::max(na, (hi - 1) / g - lo / g + 1);
             }
-            ok = res_vec_bytes_of(nsl * kSlice, m > 1 ? kHalo : 0, v2, v) + res_coarse_bytes(nc, na, nsl, coupled) <=
+            ok = res_vec_bytes_of(nsl * kSlice, m > 1 ? halo_rows(v2) : 0, v2, v) + res_coarse_bytes(nc, na, nsl, coupled) <=
                  kLdsVec;
         }
         if (ok) return g;
