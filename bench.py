@@ -721,8 +721,9 @@ def main():
     slow = np.argsort(-busy)[:5]
     print(f"bench: island work {busy.sum():.0f} team-ms per step = {busy.sum() / ms:.1f} teams busy on average "
           f"({n_cu} CUs); workgroup slots busy {100.0 * slot_ms / (n_cu * ms):.0f}% of the step; "
-          f"slowest islands (n, CG iters, ms, us per CG iter): " +
-          ", ".join(f"({nv[i]}, {int(k[i])}, {busy[i]:.1f}, {1e3 * busy[i] / max(k[i], 1):.1f})" for i in slow),
+          f"slowest islands (n, workgroups, CG iters, ms, us per CG iter): " +
+          ", ".join(f"({nv[i]}, {int(st['members'][i])}, {int(k[i])}, {busy[i]:.1f}, "
+                    f"{1e3 * busy[i] / max(k[i], 1):.1f})" for i in slow),
           flush=True)
     print(f"bench: {ms:.2f} ms/step, {value:.1f} Mtri*iter/s, {gbs:.0f} GB/s algorithmic; CG iters per global "
           f"step mean {kpi.mean():.1f} p95 {np.percentile(kpi, 95):.1f}; converged {int(st['converged'].sum())}/"
